@@ -1,0 +1,174 @@
+/*
+ * protomol_amd.h -- C ABI of the MI355X (gfx950) nonbonded force path.
+ *
+ * Plain pointers and sizes only; every entry point returns PM_OK (0) or a
+ * negative PM_E* status, and pm_last_error() gives the message.  No C++
+ * exception crosses this boundary: the host adapter in
+ * protomol_amd_systemforce.hpp turns a status into ProtoMol's THROW.
+ *
+ * Reference interfaces each entry point replaces (paths relative to the
+ * ProtoMol source tree):
+ *   pm_ctx_create    Topology<PBC,CubicCellManager> + ForceFactory::make of
+ *                    NonbondedCutoffSystemForce<CCM, OneAtomPair[Two]<...>>
+ *                    (factory/ForceFactory.cpp:21, module/NonbondedCutoffForceModule.cpp:56-88),
+ *                    NonbondedFullEwaldSystemForce<PBC,CCM,true,false,corr,Cutoff>
+ *                    (module/NonbondedFullElectrostaticForceModule.cpp:31-36) and the GB
+ *                    NonbondedSimpleFullSystemForce<OneAtomPairNoExclusion<VBC,Universal,GB*>>
+ *                    (module/NonbondedSimpleFullForceModule.cpp:203-206)
+ *   pm_evaluate      SystemForce::evaluate(topo, positions, forces, energies)
+ *                    (force/system/SystemForce.h:32-35) -- ADDS into forces and energies
+ *   pm_md_*          LeapfrogIntegrator::run (integrator/leapfrog/LeapfrogIntegrator.cpp:89-104)
+ *                    with the force group evaluated on the device every step
+ *   pm_pair_census   no reference counterpart: order-independent digest of the
+ *                    evaluated pair set, for bit-exact pair-set checks
+ */
+#ifndef PROTOMOL_AMD_H
+#define PROTOMOL_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PM_ABI_VERSION 1
+
+/* status codes */
+#define PM_OK 0
+#define PM_EINVAL -1    /* bad argument or unsupported configuration */
+#define PM_ENOMEM -2    /* device or host allocation failed */
+#define PM_EDEVICE -3   /* HIP runtime error */
+#define PM_ECAPACITY -4 /* a device-side capacity was exceeded (column, list) */
+#define PM_ENOTREADY -5 /* state not set yet */
+
+/* switching functions (switch/*SwitchingFunction.h) */
+#define PM_SWITCH_NONE 0    /* UniversalSwitchingFunction */
+#define PM_SWITCH_CUTOFF 1  /* CutoffSwitchingFunction */
+#define PM_SWITCH_C1 2      /* C1SwitchingFunction */
+#define PM_SWITCH_C2 3      /* C2SwitchingFunction */
+#define PM_SWITCH_CN 4      /* CnSwitchingFunction */
+#define PM_SWITCH_CMPCNCN 5 /* CmpCnCnSwitchingFunction */
+
+/* pair terms */
+#define PM_TERM_NONE 0
+#define PM_TERM_LJ 1         /* LennardJonesForce (force/LennardJonesForce.h:28-60) */
+#define PM_TERM_COULOMB 2    /* CoulombForce (force/CoulombForce.h:29-40) */
+#define PM_TERM_EWALD_REAL 3 /* NonbondedFullEwaldSystemForce::realTerm (:758-858) */
+
+/* LJ parameter modes */
+#define PM_LJ_TABLE 0   /* LennardJonesParameterTable: T x T of (A, B, A14, B14) */
+#define PM_LJ_PERATOM 1 /* per-atom sigma/epsilon, GROMACS mixing (BuildTopology.cpp:650-659) */
+
+/* exclusion classes (topology/ExclusionType.h) */
+#define PM_EXCL_FULL 1
+#define PM_EXCL_MODIFIED 2
+
+/* flags */
+#define PM_FLAG_VIRIAL 1u     /* ScalarStructure::virial() on */
+#define PM_FLAG_GB 2u         /* GBBornRadii -> GBPartialSum -> GBForce, all pairs, vacuum */
+#define PM_FLAG_BONDED 4u     /* harmonic Bond + Angle (force/bonded/*SystemForce.h) */
+
+/* ScalarStructure slots (type/ScalarStructure.h:26-67); energies[] has 36 */
+#define PM_E_COULOMB 0
+#define PM_E_LENNARDJONES 1
+#define PM_E_BOND 2
+#define PM_E_ANGLE 3
+#define PM_E_VIRIALXX 7
+#define PM_E_NSLOTS 36
+
+typedef struct pm_switch_desc {
+  int32_t kind;     /* PM_SWITCH_* */
+  double switchon;  /* -switchon */
+  double cutoff;    /* -cutoff */
+  double switchoff; /* -switchoff (Cn, CmpCnCn) */
+  double order;     /* -n (Cn, CmpCnCn) */
+} pm_switch_desc;
+
+typedef struct pm_system_desc {
+  int32_t n_atoms;
+  const double *scaled_charge; /* n; charge * 18.2226123264 (BuildTopology.cpp:92) */
+  const double *mass;          /* n; amu */
+  const int32_t *type;         /* n; LJ type (PM_LJ_TABLE) */
+  const int32_t *molecule;     /* n; exclusions are only looked up within a molecule */
+  int32_t lj_mode;
+  int32_t n_types;
+  const double *lj_table;      /* n_types^2 * 4: A, B, A14, B14 */
+  const double *lj_sigma;      /* n (PM_LJ_PERATOM) */
+  const double *lj_epsilon;    /* n (PM_LJ_PERATOM) */
+  double lj_scaling;           /* LJScalingFactor for MODIFIED pairs (PM_LJ_PERATOM) */
+  int32_t n_exclusions;
+  const int32_t *exclusions;   /* n_exclusions * 3: atom a, atom b, PM_EXCL_* (ExclusionTable::add order) */
+  double coulomb_scaling;      /* GenericTopology::coulombScalingFactor */
+  int32_t periodic;            /* 1: orthogonal PeriodicBoundaryConditions, 0: vacuum */
+  double box[3];               /* cellBasisVector1.x, 2.y, 3.z */
+  double origin[3];            /* cellorigin */
+  int32_t n_bonds;
+  const int32_t *bonds;        /* 2 per bond */
+  const double *bond_k, *bond_r0;
+  int32_t n_angles;
+  const int32_t *angles;       /* 3 per angle, atom2 = vertex */
+  const double *angle_k, *angle_theta0; /* theta0 in radians */
+  const double *gb_radius;     /* n; vanDerWaalRadius (GB) */
+  const double *gb_scaling;    /* n; scalingFactor (GB) */
+} pm_system_desc;
+
+typedef struct pm_force_desc {
+  int32_t term_a;              /* ForceA of OneAtomPair[Two]: PM_TERM_LJ or PM_TERM_COULOMB or NONE */
+  pm_switch_desc switch_a;
+  int32_t term_b;              /* ForceB: PM_TERM_COULOMB, PM_TERM_EWALD_REAL or NONE */
+  pm_switch_desc switch_b;     /* ignored for PM_TERM_EWALD_REAL (hard cutoff Rc) */
+  double cutoff;               /* NonbondedCutoffForce -cutoff (cell search radius) */
+  double ewald_alpha;          /* Ewald: alpha */
+  double ewald_accuracy;       /* Ewald: Rc = sqrt(-ln(accuracy)) / alpha */
+  int32_t ewald_correction;    /* add excluded-pair erf, point-self and charged-system terms */
+  double gb_offset;            /* dielecOffset (0.09) */
+  double gb_alpha, gb_beta, gb_gamma; /* OBC constants (ProtoMolApp.cpp:285-301) */
+  double gb_solute_dielec, gb_solvent_dielec;
+  double skin;                 /* pair-list buffer (Angstrom); 0 = choose */
+  int32_t rebuild_interval;    /* MD steps between pair-list rebuilds (>= 1) */
+  uint32_t flags;              /* PM_FLAG_* */
+} pm_force_desc;
+
+typedef struct pm_ctx pm_ctx;
+
+/* Library / device identity. */
+int pm_abi_version(void);
+int pm_device_info(char *name, int32_t name_len, int32_t *n_cu, double *hbm_gib);
+
+/* Create a context holding n_systems independent systems (a batch of 1 for
+ * the plain drop-in).  All systems share the force description. */
+int pm_ctx_create(const pm_force_desc *force, int32_t n_systems, const pm_system_desc *systems,
+                  pm_ctx **out);
+int pm_ctx_destroy(pm_ctx *ctx);
+/* Last error message for ctx (or the calling thread when ctx is NULL). */
+const char *pm_last_error(const pm_ctx *ctx);
+
+/* SystemForce::evaluate: positions x (3n AoS FP64, host) of one system;
+ * ADDS forces into f (3n AoS FP64, host) and energies into e[36]. */
+int pm_evaluate(pm_ctx *ctx, int32_t system, const double *x, double *f, double *e);
+
+/* The same over all systems at once (one fused launch sequence); x, f are the
+ * systems' arrays concatenated in order, e holds 36 slots per system. */
+int pm_evaluate_batch(pm_ctx *ctx, const double *x, double *f, double *e);
+
+/* Digest of the pair set the last evaluation of `system` used: count and
+ * sum of splitmix64(lo | hi << 32) over pairs (lo < hi, original indices). */
+int pm_pair_census(pm_ctx *ctx, int32_t system, const double *x, uint64_t *count, uint64_t *hash);
+
+/* Device-resident leapfrog.  Set x, v (concatenated over systems), run
+ * n_steps of LeapfrogIntegrator::run at dt_fs, read back state.  energies_every
+ * > 0 evaluates energies on every k-th step and on the last step. */
+int pm_md_set_state(pm_ctx *ctx, const double *x, const double *v);
+int pm_md_run(pm_ctx *ctx, int32_t n_steps, double dt_fs, int32_t energies_every);
+int pm_md_get_state(pm_ctx *ctx, double *x, double *v, double *f, double *e);
+
+/* Statistics of the last call: stats[0] pair-list rebuilds, [1] segments
+ * re-run because an atom moved beyond skin/2, [2] pairs evaluated (last
+ * evaluation, all systems), [3] device ms of the last call,
+ * [4] pair-kernel ms (last evaluation), [5] pair-kernel launches. */
+int pm_stats(pm_ctx *ctx, double *stats, int32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

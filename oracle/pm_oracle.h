@@ -1,0 +1,177 @@
+/*
+ * pm_oracle.h -- FP64 CPU restatement of ProtoMol's nonbonded force path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * implementation in protomol_amd/ and the CPU baseline timed beside it.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.  The product path never links or calls it.
+ *
+ * Every routine restates the reference algorithm it names (paths relative to
+ * the ProtoMol source tree).  Units are ProtoMol's: Angstrom, kcal/mol, amu,
+ * charges pre-scaled by sqrt(Coulomb constant) = 18.2226123264
+ * (protomol/base/PMConstants.cpp:67, protomol/topology/BuildTopology.cpp:92).
+ */
+#ifndef PM_ORACLE_H
+#define PM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ScalarStructure slot indices (protomol/type/ScalarStructure.h:26-67). */
+enum {
+  ORC_COULOMB = 0, ORC_LENNARDJONES = 1, ORC_BOND = 2, ORC_ANGLE = 3,
+  ORC_DIHEDRAL = 4, ORC_IMPROPER = 5, ORC_OTHER = 6,
+  ORC_VIRIALXX = 7, ORC_VIRIALXY = 8, ORC_VIRIALXZ = 9,
+  ORC_VIRIALYX = 10, ORC_VIRIALYY = 11, ORC_VIRIALYZ = 12,
+  ORC_VIRIALZX = 13, ORC_VIRIALZY = 14, ORC_VIRIALZZ = 15,
+  ORC_MOLVIRIALXX = 16,
+  ORC_NSLOTS = 36
+};
+
+/* Switching function kinds (protomol/switch/). */
+enum {
+  ORC_SW_UNIVERSAL = 0, /* UniversalSwitchingFunction.h */
+  ORC_SW_CUTOFF = 1,    /* CutoffSwitchingFunction.h */
+  ORC_SW_C1 = 2,        /* C1SwitchingFunction.{h,cpp} */
+  ORC_SW_C2 = 3,        /* C2SwitchingFunction.{h,cpp} */
+  ORC_SW_CN = 4,        /* CnSwitchingFunction.{h,cpp} */
+  ORC_SW_CMPCNCN = 5    /* CmpCnCnSwitchingFunction.{h,cpp} */
+};
+
+/* Pair term kinds. */
+enum {
+  ORC_TERM_NONE = 0,
+  ORC_TERM_LJ = 1,         /* LennardJonesForce.h:28-60 */
+  ORC_TERM_COULOMB = 2,    /* CoulombForce.h:29-40 */
+  ORC_TERM_EWALD_REAL = 3  /* NonbondedFullEwaldSystemForce.h:758-858 */
+};
+
+/* Exclusion classes (protomol/topology/ExclusionType.h). */
+enum { ORC_EXCL_NONE = 0, ORC_EXCL_FULL = 1, ORC_EXCL_MODIFIED = 2 };
+
+typedef struct orc_switch {
+  int kind;
+  int use, modify;           /* enum USE / MODIFY of the switch class */
+  double cutoff, cutoff2;    /* cutoffSquared() of the switch */
+  double switchon, switchon2, switchoff, switchoff2;
+  double s1, s2, s3;         /* C2 constants (C2SwitchingFunction.cpp:12-18) */
+  double c1_15rc1, c1_05rc3, c1_15rc3;  /* C1 constants (C1SwitchingFunction.cpp:12-18) */
+  int ord_int, ord_idx;      /* Cn / CmpCnCn order bookkeeping */
+  double irange[8], itrange[8];
+} orc_switch;
+
+typedef struct orc_system {
+  int n;
+  const double *charge;   /* scaled charges, n */
+  const int *type;        /* LJ type index, n */
+  const int *molecule;    /* molecule id, n */
+  const double *mass;     /* n (only used by integrator routines) */
+  int ntypes;
+  const double *lj;       /* ntypes*ntypes*4: A, B, A14, B14 */
+  int nexcl;
+  const int *excl;        /* nexcl*3: a, b, class (FULL/MODIFIED) in add() order */
+  double box[3];          /* orthogonal periodic box edges (e1.x, e2.y, e3.z) */
+  double origin[3];
+  double coulomb_scaling; /* GenericTopology::coulombScalingFactor */
+  int periodic;           /* 1 = PeriodicBoundaryConditions, 0 = Vacuum */
+} orc_system;
+
+typedef struct orc_nonbonded {
+  int term_a, term_b;        /* ForceA / ForceB (ORC_TERM_*) */
+  orc_switch sw_a, sw_b;     /* SwitchA / SwitchB */
+  double algorithm_cutoff;   /* NonbondedCutoffForce -cutoff (enumerator) */
+  double cellsize;           /* CubicCellManager cellsize */
+  int virial, molvirial;
+} orc_nonbonded;
+
+/* Order-independent census of an evaluated pair set. */
+typedef struct orc_census {
+  uint64_t count;
+  uint64_t hash;   /* sum over pairs of orc_pair_hash(min, max) mod 2^64 */
+} orc_census;
+
+/* Ewald parameters (NonbondedFullEwaldSystemForce.h:552-744). */
+typedef struct orc_ewald {
+  double alpha, accuracy;
+  double rc, rc2;            /* derived: sqrt(-log(accuracy))/alpha */
+  double two_alpha_pi;       /* 2 alpha / sqrt(pi) */
+  double cellsize;
+  int correction;            /* add correction + point self + charged system */
+  int virial;
+} orc_ewald;
+
+/* GB OBC parameters (ProtoMolApp.cpp:285-301, GBForce.cpp:11-12). */
+typedef struct orc_gb {
+  const double *vdw_radius;  /* rho_i, n */
+  const double *scaling;     /* S_i, n */
+  double offset;             /* dielecOffset (0.09) */
+  double alpha_obc, beta_obc, gamma_obc;
+  double solute_dielec, solvent_dielec;
+  int block_size;            /* NonbondedSimpleFullSystemForce default 64 */
+} orc_gb;
+
+uint64_t orc_pair_hash(uint32_t lo, uint32_t hi);
+
+void orc_switch_init(orc_switch *s, int kind, double switchon, double cutoff,
+                     double switchoff, double order);
+void orc_switch_eval(const orc_switch *s, double r2, double *value, double *deriv);
+
+/* Minimal image difference, PeriodicBoundaryConditions.h:72-110 (orthogonal). */
+double orc_min_diff(const orc_system *sys, const double *c1, const double *c2, double *d);
+
+/* NonbondedCutoffSystemForce<Cubic, OneAtomPair[Two]<PBC,...>>::evaluate.
+ * Adds into f (3n, AoS) and e (36 slots).  nthreads > 1 splits cell-pair
+ * packages the way parallelEvaluate splits them across MPI ranks.
+ * Returns 0 on success, negative on error. */
+int orc_nonbonded_cutoff(const orc_system *sys, const orc_nonbonded *nb,
+                         const double *x, double *f, double *e,
+                         orc_census *census, int nthreads);
+
+/* O(N^2) census of {i<j : min-image r2 <= cut2, not FULL-excluded}. */
+void orc_census_bruteforce(const orc_system *sys, const double *x, double cut2,
+                           orc_census *census);
+
+/* NonbondedFullEwaldSystemForce<PBC,Cubic,real,false,correction,Cutoff>. */
+int orc_ewald_init(orc_ewald *ew, double alpha, double accuracy, double cellsize,
+                   int correction, int virial);
+int orc_ewald_real(const orc_system *sys, const orc_ewald *ew, const double *x,
+                   double *f, double *e, orc_census *census, int nthreads);
+
+/* MathUtilities.cpp:12-49 */
+double orc_poly5(double ar);
+double orc_myerf(double x);
+
+/* GB: GBBornRadii -> GBPartialSum -> GBForce, all pairs, vacuum.
+ * born_out (n, may be NULL) receives the Born radii. */
+int orc_gb_evaluate(const orc_system *sys, const orc_gb *gb, const double *x,
+                    double *f, double *e, double *born_out);
+
+/* Harmonic bonds/angles (force/bonded/BondSystemForce.h, AngleSystemForce.h).
+ * bonds: nb*2 atoms, bond_k, bond_r0; angles: na*3 atoms, angle_k, angle_t0 (rad). */
+void orc_bonded(const orc_system *sys, int nbonds, const int *bonds, const double *bond_k,
+                const double *bond_r0, int nangles, const int *angles,
+                const double *angle_k, const double *angle_t0, const double *x,
+                double *f, double *e);
+
+/* NR ran1 (Random.cpp:33-68) and Box-Muller Gaussian (MathUtilities.cpp:206-236). */
+typedef struct orc_rng {
+  int32_t iv[32];
+  int32_t iy, idum;
+  int have_spare;
+  double spare;
+} orc_rng;
+void orc_rng_seed(orc_rng *r, uint32_t seed);
+double orc_rng_uniform(orc_rng *r);
+double orc_rng_gaussian(orc_rng *r);
+/* TopologyUtilities.cpp:21-48 */
+void orc_random_velocity(const orc_system *sys, double temperature, uint32_t seed, double *v);
+/* TopologyUtilities.cpp:163-171 */
+double orc_kinetic_energy(const orc_system *sys, const double *v);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

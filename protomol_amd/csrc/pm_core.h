@@ -1,0 +1,120 @@
+// pm_core.h -- device data layout shared by the gfx950 kernels and the host
+// side of the C ABI.
+//
+// Layout in HBM (per system; a context holds one or more systems and the
+// kernels reach each through a device-resident table of SysDev records):
+//   original atom order  x, v, xref (FP64 SoA), charge, inverse mass, slot
+//   cluster slot order   xq (float4: cluster-local x, y, z and scaled charge),
+//                        LJ type, original index, fixed-point force (3 x int64)
+//   clusters of 8 slots  FP64 origin, FP32 bounding box, pair list of
+//                        16-byte entries (j cluster, image shift, FP32 offset)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pm {
+
+constexpr int kCluster = 8;        // atoms per cluster (one 8x8 tile = one wave64)
+constexpr int kWave = 64;
+constexpr int kSortCap = 4096;     // max atoms per x-y column (LDS bitonic sort)
+constexpr double kFScale = 4294967296.0;      // 2^32 fixed point for forces
+constexpr double kFScaleInv = 1.0 / 4294967296.0;
+constexpr double kEScale = 1048576.0;         // 2^20 fixed point for energy/virial totals
+constexpr double kEScaleInv = 1.0 / 1048576.0;
+constexpr int kShiftCenter = 13;   // (0,0,0) image in the 27-shift encoding
+
+// energy accumulator slots (int64 fixed point, per system)
+enum { kAccLJ = 0, kAccCoul = 1, kAccVxx, kAccVxy, kAccVxz, kAccVyy, kAccVyz, kAccVzz,
+       kAccBond, kAccAngle, kAccN = 16 };
+
+// device-side status bits (per system)
+enum { kErrColumnCap = 1, kErrListCap = 2, kErrNonFinite = 4, kErrClusterCap = 8 };
+
+struct SwitchDev {
+  int kind;                // PM_SWITCH_*
+  float cut2, on2;         // C2 / cutoff: cutoff^2, switchon^2 (FP32 path)
+  float s1, s2, s3;        // C2 constants (C2SwitchingFunction.cpp:12-18)
+  float c15rc1, c05rc3, c15rc3;  // C1 constants (C1SwitchingFunction.cpp:12-18)
+  // Cn / CmpCnCn are evaluated in FP64: their alternating-sign polynomials lose
+  // too many digits in FP32
+  double dcut, dcut2, don, don2, doff, doff2;
+  int ord_int, ord_idx;
+  double irange[8], itrange[8];
+};
+
+struct ForceDev {
+  int term_a, term_b;      // PM_TERM_*
+  SwitchDev sw_a, sw_b;
+  float cutA2, cutB2;      // inclusion cutoff^2 per term (FP32 decision)
+  double cutA2d, cutB2d;   // the same in FP64 (guard-band recheck)
+  float guard;             // |r2 - cut2| <= guard  ->  FP64 recheck
+  float alpha, alpha2_log2e, two_alpha_pi;  // Ewald real space
+  double dalpha, dtwo_alpha_pi;             // Ewald correction (FP64)
+  float rlist2;
+  unsigned flags;
+};
+
+// One system as the kernels see it.  All pointers are device pointers owned
+// by the host-side context; the struct itself lives in device memory.
+struct SysDev {
+  // geometry: PeriodicBoundaryConditions constants, reference-exact FP64
+  double L[3], Lr[3], H[3], H2[3], D;
+  double omin[3];          // origin - L/2: lower corner of the wrapped frame
+  int periodic;
+  int ncx, ncy, ncol;      // x-y column grid
+  int rx, ry;              // column search radius for the pair list
+  double ax, ay;           // column widths
+  int n_atoms, cl_cap, slot_cap, list_cap;
+  int lj_mode, n_types;
+  float csf;               // coulombScalingFactor
+  float lj_scaling;
+  // original-order arrays
+  double *x, *v, *xref;    // 3 * n each, SoA (x[0..n), y[n..2n), z[2n..3n))
+  double *f_out;           // 3 * n AoS export buffer
+  const double *charge, *inv_mass;
+  const int *molecule, *type;
+  const float *sig_half, *eps_sqrt; // PM_LJ_PERATOM: sigma/2, sqrt(epsilon)
+  const int *excl_start, *excl_partner, *excl_class;  // CSR of exclusions (both directions)
+  int *slot;               // atom -> slot
+  float4 *lbuild;          // local coordinates at the last rebuild (displacement check)
+  // binning scratch
+  int *col_count, *col_cursor, *col_atom_start, *col_cl_start, *col_ncl;
+  int *bin_col;            // atom -> column
+  float *bin_z;            // atom -> wrapped z
+  unsigned long long *colbuf; // column-ordered (zkey << 32 | atom)
+  // slot order
+  float4 *xq;              // local coordinates + scaled charge
+  int *ljt;                // LJ type (table mode) or -1 padding
+  float2 *ljp;             // per-atom mode (sigma/2, sqrt(eps))
+  int *orig;               // slot -> atom (-1 padding)
+  int *smol;               // slot -> molecule (-1 padding)
+  long long *fx, *fy, *fz; // fixed-point forces
+  // clusters
+  double *cl_origin;       // 3 * cl_cap SoA
+  float4 *cl_lo, *cl_hi;   // bounding boxes (w = unused)
+  int *cl_col;             // cluster -> column
+  int *cl_nvalid;
+  int *list_len;
+  uint4 *list;             // cl_cap * list_cap entries
+  unsigned long long *list_mask; // 2 per entry: FULL, MODIFIED
+  // LJ table (table mode): n_types^2 float4 (A, B, A14, B14)
+  const float4 *lj_table;
+  // bonded
+  int n_bonds, n_angles;
+  const int *bonds, *angles;
+  const double *bond_k, *bond_r0, *angle_k, *angle_t0;
+  // Ewald correction pairs (unique exclusion table pairs)
+  int n_corr;
+  const int *corr;         // 3 per pair: a, b, class
+  // GB
+  const double *gb_rho, *gb_S;
+  double *gb_burial, *gb_born, *gb_psi, *gb_partial;
+  // per-system scalars
+  int *n_clusters;
+  long long *acc;          // kAccN fixed-point energy/virial accumulators
+  unsigned long long *census; // count, hash
+  unsigned *err;
+  unsigned *maxdisp2;      // float bits of the largest squared displacement since rebuild
+};
+
+}  // namespace pm

@@ -1,0 +1,1068 @@
+// pm_kernels.hip -- gfx950 kernels of the nonbonded force path.
+//
+//   rebuild:  k_bin -> k_col_scan -> k_col_fill -> k_col_sort -> k_cluster_geom -> k_pairlist
+//   force:    k_pair<TermA, TermB, Energies, Census>, k_bonded, k_ewald_corr
+//   step:     k_kick_drift, k_half_kick, k_export_f, k_kinetic
+//
+// All kernels take the device table of SysDev records plus a prefix array
+// over the systems, so one launch covers every system of a batch.
+#include "pm_kernels.h"
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "../../include/protomol_amd.h"
+
+namespace pm {
+
+// ------------------------------------------------------------- helpers
+
+__device__ __forceinline__ int find_sys(const int *__restrict__ prefix, int nsys, int w) {
+  int lo = 0, hi = nsys;  // prefix[lo] <= w < prefix[hi]
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= w) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ long long to_fixed(double v, double scale) {
+  return __double2ll_rn(v * scale);
+}
+
+__device__ __forceinline__ void atomic_add_ll(long long *p, long long v) {
+  atomicAdd(reinterpret_cast<unsigned long long *>(p), static_cast<unsigned long long>(v));
+}
+
+__device__ __forceinline__ unsigned long long pair_hash(unsigned lo, unsigned hi) {
+  unsigned long long z = ((unsigned long long)hi << 32) | lo;
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ unsigned long long wave_ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ float wave_max_f(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ double wave_sum_d(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// wrapped coordinate in [0, L) relative to the box's lower corner
+__device__ __forceinline__ double wrap_rel(double x, double omin, double L) {
+  double r = x - omin;
+  return r - L * floor(r / L);
+}
+
+// PeriodicBoundaryConditions::minimalDifference(c1, c2, r2), orthogonal branch
+// (PeriodicBoundaryConditions.h:72-110), evaluated without contraction so the
+// squared distance rounds exactly as the reference's FP64 arithmetic.
+__device__ __noinline__ double min_image_r2(const SysDev &S, int a, int b) {
+#pragma clang fp contract(off)
+  const int n = S.n_atoms;
+  double d0 = S.x[b] - S.x[a];
+  double d1 = S.x[n + b] - S.x[n + a];
+  double d2 = S.x[2 * n + b] - S.x[2 * n + a];
+  double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+  if (S.periodic && r2 > S.D) {
+    if (d0 < -S.H2[0]) d0 += S.H[0];
+    else if (d0 > S.H2[0]) d0 -= S.H[0];
+    if (d1 < -S.H2[1]) d1 += S.H[1];
+    else if (d1 > S.H2[1]) d1 -= S.H[1];
+    if (d2 < -S.H2[2]) d2 += S.H[2];
+    else if (d2 > S.H2[2]) d2 -= S.H[2];
+    r2 = d0 * d0 + d1 * d1 + d2 * d2;
+    if (r2 > S.D) {
+      d0 -= S.L[0] * rint(S.Lr[0] * d0);
+      d1 -= S.L[1] * rint(S.Lr[1] * d1);
+      d2 -= S.L[2] * rint(S.Lr[2] * d2);
+      r2 = d0 * d0 + d1 * d1 + d2 * d2;
+    }
+  }
+  return r2;
+}
+
+// Cn coefficient tables (CnSwitchingFunction.cpp:9-31), rows = order 2,3,4,(5),6
+__constant__ double kSwCoef[5][7] = {{10., -15., 6., 0, 0, 0, 0},
+                                     {35., -84., 70., -20., 0, 0, 0},
+                                     {126., -420., 540., -315., 70., 0, 0},
+                                     {0, 0, 0, 0, 0, 0, 0},
+                                     {1716., -9009., 20020., -24024., 16380., -6006., 924.0}};
+__constant__ double kDSwCoef[5][7] = {{30., -60., 30., 0, 0, 0, 0},
+                                      {140., -420., 420., -140., 0, 0, 0},
+                                      {630., -2520., 3780., -2520., 630., 0, 0},
+                                      {0, 0, 0, 0, 0, 0, 0},
+                                      {12012., -72072., 180180., -240240., 180180., -72072., 12012.0}};
+
+__device__ __noinline__ void switch_cn_d(const SwitchDev &sw, double r2, double &val, double &der) {
+  // CnSwitchingFunction::operator() (CnSwitchingFunction.cpp:149-175) and the
+  // two branches of CmpCnCnSwitchingFunction::operator() (CmpCnCnSwitchingFunction.h:52-95)
+  val = 0.0;
+  der = 0.0;
+  if (r2 > sw.dcut2) return;
+  double sd = sqrt(r2);
+  double c[9];
+  if (sw.kind == PM_SWITCH_CN) {
+    if (r2 < sw.don2) { val = 1.0; return; }
+    double swd = sd - sw.dcut;
+    double p = 1.0;
+    for (int k = 0; k < sw.ord_int; ++k) p *= swd;
+    c[0] = p;
+    for (int i = 1; i < sw.ord_int + 2; ++i) c[i] = c[i - 1] * swd;
+    for (int i = 0; i < sw.ord_int + 1; ++i) {
+      val += kSwCoef[sw.ord_idx][i] * c[i + 1] * sw.irange[i];
+      der += kDSwCoef[sw.ord_idx][i] * c[i] * sw.irange[i];
+    }
+    der /= sd;
+    return;
+  }
+  // CmpCnCn
+  if (r2 > sw.doff2) {
+    double swd = sd - sw.dcut;
+    double p = 1.0;
+    for (int k = 0; k < sw.ord_int; ++k) p *= swd;
+    c[0] = p;
+    for (int i = 1; i < sw.ord_int + 2; ++i) c[i] = c[i - 1] * swd;
+    for (int i = 0; i < sw.ord_int + 1; ++i) {
+      val += kSwCoef[sw.ord_idx][i] * c[i + 1] * sw.itrange[i];
+      der += kDSwCoef[sw.ord_idx][i] * c[i] * sw.itrange[i];
+    }
+    der /= sd;
+  } else if (r2 >= sw.don2) {
+    double swd = sd - sw.doff;
+    double p = 1.0;
+    for (int k = 0; k < sw.ord_int; ++k) p *= swd;
+    c[0] = p;
+    for (int i = 1; i < sw.ord_int + 2; ++i) c[i] = c[i - 1] * swd;
+    double v = 0.0;
+    for (int i = 0; i < sw.ord_int + 1; ++i) {
+      v += kSwCoef[sw.ord_idx][i] * c[i + 1] * sw.irange[i];
+      der -= kDSwCoef[sw.ord_idx][i] * c[i] * sw.irange[i];
+    }
+    val = 1.0 - v;
+    der /= sd;
+  }
+}
+
+// switched force/energy: F = F*S - E*dS, E = E*S (OneAtomPair.h:76-84)
+__device__ __forceinline__ void apply_switch(const SwitchDev &sw, float r2, float &E, float &F) {
+  switch (sw.kind) {
+  case PM_SWITCH_C2: {  // C2SwitchingFunction.h:42-53
+    float c2 = sw.cut2 - r2;
+    float c4 = c2 * (sw.s2 + 2.0f * r2);
+    float S = sw.s1 * (c2 * c4);
+    float dS = sw.s3 * (c2 * c2 - c4);
+    bool inner = r2 < sw.on2;
+    bool outer = r2 > sw.cut2;
+    S = inner ? 1.0f : (outer ? 0.0f : S);
+    dS = (inner || outer) ? 0.0f : dS;
+    F = F * S - E * dS;
+    E = E * S;
+    break;
+  }
+  case PM_SWITCH_C1: {  // C1SwitchingFunction.h:41-50
+    float d = r2 * rsqrtf(r2);
+    float S = 1.0f - d * (sw.c15rc1 - r2 * sw.c05rc3);
+    float dS = -sw.c15rc1 / d + d * sw.c15rc3;
+    bool outer = r2 > sw.cut2;
+    S = outer ? 0.0f : S;
+    dS = outer ? 0.0f : dS;
+    F = F * S - E * dS;
+    E = E * S;
+    break;
+  }
+  case PM_SWITCH_CUTOFF: {  // CutoffSwitchingFunction.h:41-44
+    float S = r2 > sw.cut2 ? 0.0f : 1.0f;
+    F = F * S;
+    E = E * S;
+    break;
+  }
+  case PM_SWITCH_CN:
+  case PM_SWITCH_CMPCNCN: {
+    double S, dS;
+    switch_cn_d(sw, (double)r2, S, dS);
+    F = (float)((double)F * S - (double)E * dS);
+    E = (float)((double)E * S);
+    break;
+  }
+  default:
+    break;
+  }
+}
+
+// ------------------------------------------------------------- rebuild
+
+__global__ void k_import_x(SysDev *sys, const int *ap, int nsys, const double *xaos) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ap[nsys]) return;
+  int s = find_sys(ap, nsys, g);
+  SysDev &S = sys[s];
+  int a = g - ap[s], n = S.n_atoms;
+  S.x[a] = xaos[3 * (size_t)g];
+  S.x[n + a] = xaos[3 * (size_t)g + 1];
+  S.x[2 * n + a] = xaos[3 * (size_t)g + 2];
+}
+
+__global__ void k_import_v(SysDev *sys, const int *ap, int nsys, const double *vaos) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ap[nsys]) return;
+  int s = find_sys(ap, nsys, g);
+  SysDev &S = sys[s];
+  int a = g - ap[s], n = S.n_atoms;
+  S.v[a] = vaos[3 * (size_t)g];
+  S.v[n + a] = vaos[3 * (size_t)g + 1];
+  S.v[2 * n + a] = vaos[3 * (size_t)g + 2];
+}
+
+__global__ void k_bin(SysDev *sys, const int *ap, int nsys) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ap[nsys]) return;
+  int s = find_sys(ap, nsys, g);
+  SysDev &S = sys[s];
+  int a = g - ap[s], n = S.n_atoms;
+  double x = S.x[a], y = S.x[n + a], z = S.x[2 * n + a];
+  if (!(isfinite(x) && isfinite(y) && isfinite(z))) {
+    atomicOr(S.err, (unsigned)kErrNonFinite);
+    x = y = z = 0.0;
+  }
+  double rx = wrap_rel(x, S.omin[0], S.L[0]);
+  double ry = wrap_rel(y, S.omin[1], S.L[1]);
+  double rz = wrap_rel(z, S.omin[2], S.L[2]);
+  int cx = (int)(rx / S.ax), cy = (int)(ry / S.ay);
+  cx = cx < 0 ? 0 : (cx >= S.ncx ? S.ncx - 1 : cx);
+  cy = cy < 0 ? 0 : (cy >= S.ncy ? S.ncy - 1 : cy);
+  int col = cy * S.ncx + cx;
+  S.bin_col[a] = col;
+  S.bin_z[a] = (float)rz;
+  atomicAdd(&S.col_count[col], 1);
+}
+
+// one block per system: exclusive scans of atoms and clusters over columns
+__global__ __launch_bounds__(1024) void k_col_scan(SysDev *sys) {
+  SysDev &S = sys[blockIdx.x];
+  __shared__ int sa[1024], sc[1024];
+  const int t = threadIdx.x, nt = blockDim.x;
+  const int ncol = S.ncol;
+  const int chunk = (ncol + nt - 1) / nt;
+  const int c0 = t * chunk, c1 = min(ncol, c0 + chunk);
+  int la = 0, lc = 0;
+  for (int c = c0; c < c1; ++c) {
+    int n = S.col_count[c];
+    la += n;
+    lc += (n + kCluster - 1) / kCluster;
+  }
+  sa[t] = la;
+  sc[t] = lc;
+  __syncthreads();
+  for (int o = 1; o < nt; o <<= 1) {
+    int va = t >= o ? sa[t - o] : 0, vc = t >= o ? sc[t - o] : 0;
+    __syncthreads();
+    sa[t] += va;
+    sc[t] += vc;
+    __syncthreads();
+  }
+  int ba = sa[t] - la, bc = sc[t] - lc;
+  for (int c = c0; c < c1; ++c) {
+    int n = S.col_count[c];
+    int k = (n + kCluster - 1) / kCluster;
+    S.col_atom_start[c] = ba;
+    S.col_cl_start[c] = bc;
+    S.col_ncl[c] = k;
+    S.col_cursor[c] = 0;
+    ba += n;
+    bc += k;
+  }
+  if (t == nt - 1) {
+    int tot = sc[t];
+    if (tot > S.cl_cap) {
+      atomicOr(S.err, (unsigned)kErrClusterCap);
+      tot = S.cl_cap;
+    }
+    *S.n_clusters = tot;
+  }
+}
+
+__global__ void k_col_fill(SysDev *sys, const int *ap, int nsys) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ap[nsys]) return;
+  int s = find_sys(ap, nsys, g);
+  SysDev &S = sys[s];
+  int a = g - ap[s];
+  int col = S.bin_col[a];
+  int pos = atomicAdd(&S.col_cursor[col], 1);
+  unsigned zb = __float_as_uint(S.bin_z[a]);  // z >= 0: bit order == value order
+  S.colbuf[S.col_atom_start[col] + pos] = ((unsigned long long)zb << 32) | (unsigned)a;
+}
+
+// one block per column: bitonic sort by (z, atom) and slot assignment
+__global__ __launch_bounds__(1024) void k_col_sort(SysDev *sys, const int *cp, int nsys) {
+  __shared__ unsigned long long keys[kSortCap];
+  int gcol = blockIdx.x;
+  if (gcol >= cp[nsys]) return;
+  int s = find_sys(cp, nsys, gcol);
+  SysDev &S = sys[s];
+  int col = gcol - cp[s];
+  const int n = S.col_count[col];
+  const int t = threadIdx.x, nt = blockDim.x;
+  if (n > kSortCap) {
+    if (t == 0) atomicOr(S.err, (unsigned)kErrColumnCap);
+    return;
+  }
+  const int start = S.col_atom_start[col];
+  int P = 1;
+  while (P < n) P <<= 1;
+  for (int i = t; i < P; i += nt) keys[i] = i < n ? S.colbuf[start + i] : ~0ull;
+  __syncthreads();
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = t; i < P; i += nt) {
+        int l = i ^ j;
+        if (l > i) {
+          unsigned long long a = keys[i], b = keys[l];
+          bool up = (i & k) == 0;
+          if ((a > b) == up) {
+            keys[i] = b;
+            keys[l] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  const int cl0 = S.col_cl_start[col], ncl = S.col_ncl[col];
+  for (int k = t; k < ncl * kCluster; k += nt) {
+    int slot = cl0 * kCluster + k;
+    if (k < n) {
+      int a = (int)(keys[k] & 0xffffffffull);
+      S.slot[a] = slot;
+      S.orig[slot] = a;
+      S.smol[slot] = S.molecule[a];
+      if (S.lj_mode == PM_LJ_PERATOM) S.ljp[slot] = make_float2(S.sig_half[a], S.eps_sqrt[a]);
+      else S.ljt[slot] = S.type[a];
+    } else {
+      S.orig[slot] = -1;
+      S.smol[slot] = -1;
+      if (S.lj_mode == PM_LJ_PERATOM) S.ljp[slot] = make_float2(0.f, 0.f);
+      else S.ljt[slot] = 0;
+      S.xq[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  }
+  for (int c = t; c < ncl; c += nt) {
+    S.cl_col[cl0 + c] = col;
+    S.cl_nvalid[cl0 + c] = min(kCluster, n - c * kCluster);
+  }
+  __syncthreads();
+  if (t == 0) S.col_count[col] = 0;  // ready for the next rebuild
+}
+
+// one thread per cluster: bounding box, origin, local coordinates
+__global__ void k_cluster_geom(SysDev *sys, const int *clp, int nsys) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= clp[nsys]) return;
+  int s = find_sys(clp, nsys, g);
+  SysDev &S = sys[s];
+  int c = g - clp[s];
+  if (c >= *S.n_clusters) return;
+  const int n = S.n_atoms, nv = S.cl_nvalid[c];
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  double rel[kCluster][3];
+  int at[kCluster];
+  for (int k = 0; k < nv; ++k) {
+    int a = S.orig[c * kCluster + k];
+    at[k] = a;
+    for (int d = 0; d < 3; ++d) {
+      double r = wrap_rel(S.x[d * n + a], S.omin[d], S.L[d]);
+      rel[k][d] = r;
+      lo[d] = fmin(lo[d], r);
+      hi[d] = fmax(hi[d], r);
+    }
+  }
+  double org[3];
+  for (int d = 0; d < 3; ++d) {
+    org[d] = 0.5 * (lo[d] + hi[d]);
+    S.cl_origin[d * S.cl_cap + c] = org[d];
+  }
+  S.cl_lo[c] = make_float4((float)lo[0] - 1e-3f, (float)lo[1] - 1e-3f, (float)lo[2] - 1e-3f, 0.f);
+  S.cl_hi[c] = make_float4((float)hi[0] + 1e-3f, (float)hi[1] + 1e-3f, (float)hi[2] + 1e-3f, 0.f);
+  for (int k = 0; k < nv; ++k) {
+    int a = at[k];
+    float l[3];
+    for (int d = 0; d < 3; ++d) {
+      double xa = S.x[d * n + a];
+      double ref = S.omin[d] + org[d] + (xa - S.omin[d] - rel[k][d]);
+      S.xref[d * n + a] = ref;
+      l[d] = (float)(xa - ref);
+    }
+    S.xq[c * kCluster + k] = make_float4(l[0], l[1], l[2], (float)S.charge[a]);
+    S.lbuild[a] = make_float4(l[0], l[1], l[2], 0.f);
+  }
+}
+
+__device__ __forceinline__ int floordiv(int a, int b) {
+  int q = a / b;
+  return (a % b != 0 && ((a < 0) != (b < 0))) ? q - 1 : q;
+}
+
+__device__ __forceinline__ int excl_lookup(const SysDev &S, int a, int b) {
+  int e0 = S.excl_start[a], e1 = S.excl_start[a + 1];
+  for (int k = e0; k < e1; ++k)
+    if (S.excl_partner[k] == b) return S.excl_class[k];
+  return 0;
+}
+
+// one wave per i-cluster: geometric half-shell cluster-pair list within rlist
+__global__ __launch_bounds__(256) void k_pairlist(SysDev *sys, const int *clp, int nsys, float rlist2) {
+  const int lane = threadIdx.x & 63;
+  const int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (w >= clp[nsys]) return;
+  const int s = find_sys(clp, nsys, w);
+  SysDev &S = sys[s];
+  const int ci = w - clp[s];
+  if (ci >= *S.n_clusters) return;
+  const int coli = S.cl_col[ci];
+  const int cxi = coli % S.ncx, cyi = coli / S.ncx;
+  const float4 loi = S.cl_lo[ci], hii = S.cl_hi[ci];
+  const double oix = S.cl_origin[ci], oiy = S.cl_origin[S.cl_cap + ci], oiz = S.cl_origin[2 * S.cl_cap + ci];
+  const float Lx = (float)S.L[0], Ly = (float)S.L[1], Lz = (float)S.L[2];
+  const float ax = (float)S.ax, ay = (float)S.ay;
+  const int cap = S.list_cap;
+  uint4 *out = S.list + (size_t)ci * cap;
+  int count = 0;
+  for (int dy = -S.ry; dy <= S.ry; ++dy) {
+    const int cyu = cyi + dy;
+    const int sy = floordiv(cyu, S.ncy);
+    const int cyj = cyu - sy * S.ncy;
+    const float ylo = cyu * ay, yhi = (cyu + 1) * ay;
+    const float gy = fmaxf(0.f, fmaxf(ylo - hii.y, loi.y - yhi) - 0.01f);
+    if (gy * gy >= rlist2) continue;
+    for (int dx = -S.rx; dx <= S.rx; ++dx) {
+      if (dy < 0 || (dy == 0 && dx < 0)) continue;  // half shell over columns
+      const int cxu = cxi + dx;
+      const int sx = floordiv(cxu, S.ncx);
+      const int cxj = cxu - sx * S.ncx;
+      const float xlo = cxu * ax, xhi = (cxu + 1) * ax;
+      const float gx = fmaxf(0.f, fmaxf(xlo - hii.x, loi.x - xhi) - 0.01f);
+      if (gx * gx + gy * gy >= rlist2) continue;
+      const int colj = cyj * S.ncx + cxj;
+      const int c0 = S.col_cl_start[colj], nc = S.col_ncl[colj];
+      const bool samecol = dx == 0 && dy == 0;
+      const float sxL = sx * Lx, syL = sy * Ly;
+      for (int sz = -1; sz <= 1; ++sz) {
+        const float szL = sz * Lz;
+        for (int base = 0; base < nc; base += kWave) {
+          const int k = base + lane;
+          const int cj = c0 + k;
+          bool hit = false;
+          if (k < nc && !(samecol && !(sz > 0 || (sz == 0 && cj >= ci)))) {
+            const float4 loj = S.cl_lo[cj], hij = S.cl_hi[cj];
+            float ex = fmaxf(0.f, fmaxf(loj.x + sxL - hii.x, loi.x - (hij.x + sxL)));
+            float ey = fmaxf(0.f, fmaxf(loj.y + syL - hii.y, loi.y - (hij.y + syL)));
+            float ez = fmaxf(0.f, fmaxf(loj.z + szL - hii.z, loi.z - (hij.z + szL)));
+            hit = ex * ex + ey * ey + ez * ez < rlist2;
+          }
+          const unsigned long long b = wave_ballot(hit);
+          if (b == 0) continue;
+          const int pos = count + __popcll(b & ((1ull << lane) - 1ull));
+          if (hit && pos < cap) {
+            const int sh = (sx + 1) * 9 + (sy + 1) * 3 + (sz + 1);
+            const int nvj = S.cl_nvalid[cj];
+            const double ojx = S.cl_origin[cj] + (double)sx * S.L[0];
+            const double ojy = S.cl_origin[S.cl_cap + cj] + (double)sy * S.L[1];
+            const double ojz = S.cl_origin[2 * S.cl_cap + cj] + (double)sz * S.L[2];
+            uint4 e;
+            e.x = (unsigned)cj | ((unsigned)sh << 23) | ((unsigned)(nvj - 1) << 28);
+            e.y = __float_as_uint((float)(ojx - oix));
+            e.z = __float_as_uint((float)(ojy - oiy));
+            e.w = __float_as_uint((float)(ojz - oiz));
+            out[pos] = e;
+          }
+          count += __popcll(b);
+        }
+      }
+    }
+  }
+  if (count > cap) {
+    if (lane == 0) atomicOr(S.err, (unsigned)kErrListCap);
+    count = cap;
+  }
+  // exclusion masks for entries whose clusters share a molecule
+  const int ii = lane >> 3, jj = lane & 7;
+  const int si = ci * kCluster + ii;
+  const int mi = S.smol[si];
+  unsigned long long *mask = S.list_mask + 2 * (size_t)ci * cap;
+  for (int e = 0; e < count; ++e) {
+    const int cj = (int)(out[e].x & 0x7FFFFFu);
+    const int sj = cj * kCluster + jj;
+    const int mj = S.smol[sj];
+    const bool same = mi >= 0 && mi == mj;
+    if (wave_ballot(same) == 0) continue;
+    int cls = 0;
+    if (same) {
+      const int a = S.orig[si], b = S.orig[sj];
+      if (a != b) cls = excl_lookup(S, a, b);
+    }
+    const unsigned long long mf = wave_ballot(cls == PM_EXCL_FULL);
+    const unsigned long long mm = wave_ballot(cls == PM_EXCL_MODIFIED);
+    if ((mf | mm) != 0 && lane == 0) {
+      mask[2 * e] = mf;
+      mask[2 * e + 1] = mm;
+      out[e].x |= 0x80000000u;
+    }
+  }
+  if (lane == 0) S.list_len[ci] = count;
+}
+
+// ------------------------------------------------------------ pair force
+
+// One wave per i-cluster.  Lane = (ii, jj): i atom ii of the i-cluster against
+// j atom jj of the current j-cluster, so one 8x8 cluster pair per step.
+// FP32 arithmetic on cluster-local coordinates; pairs within `guard` of a
+// cutoff are decided on the reference's FP64 minimal image instead, so the
+// within-cutoff pair set is exactly the FP64 one.  i-forces accumulate in
+// registers, j-forces reduce across the wave (xor shuffles over the 8 i lanes)
+// and both land in 64-bit fixed point, so the sum is order independent.
+template <int TA, int TB, bool EN, bool CEN>
+__global__ __launch_bounds__(256) void k_pair(SysDev *sys, const int *clp, int nsys, ForceDev P) {
+  const int lane = threadIdx.x & 63;
+  const int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (w >= clp[nsys]) return;
+  const int s = find_sys(clp, nsys, w);
+  const SysDev &S = sys[s];
+  const int ci = w - clp[s];
+  if (ci >= *S.n_clusters) return;
+  const int ii = lane >> 3, jj = lane & 7;
+  const int si = ci * kCluster + ii;
+  const bool vi = ii < S.cl_nvalid[ci];
+  const float4 xi = S.xq[si];
+  const bool peratom = S.lj_mode == PM_LJ_PERATOM;
+  const int T = S.n_types;
+  int ti = 0;
+  float2 pi = make_float2(0.f, 0.f);
+  if (TA == PM_TERM_LJ) {
+    if (peratom) pi = S.ljp[si];
+    else ti = S.ljt[si];
+  }
+  int ai = 0;
+  if (CEN) ai = S.orig[si];
+  float fix = 0.f, fiy = 0.f, fiz = 0.f;
+  double eA = 0.0, eB = 0.0;
+  double vxx = 0.0, vxy = 0.0, vxz = 0.0, vyy = 0.0, vyz = 0.0, vzz = 0.0;
+  unsigned long long ccount = 0, chash = 0;
+  const int len = S.list_len[ci];
+  const uint4 *L = S.list + (size_t)ci * S.list_cap;
+  const unsigned long long *M = S.list_mask + 2 * (size_t)ci * S.list_cap;
+  for (int e = 0; e < len; ++e) {
+    const uint4 ent = L[e];
+    const unsigned w0 = ent.x;
+    const int cj = (int)(w0 & 0x7FFFFFu);
+    const int sh = (int)((w0 >> 23) & 31u);
+    const int nvj = (int)((w0 >> 28) & 7u) + 1;
+    const int sj = cj * kCluster + jj;
+    const float4 xj = S.xq[sj];
+    const float dx = (xj.x - xi.x) + __uint_as_float(ent.y);
+    const float dy = (xj.y - xi.y) + __uint_as_float(ent.z);
+    const float dz = (xj.z - xi.z) + __uint_as_float(ent.w);
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    bool act = vi && (jj < nvj);
+    if (cj == ci && sh == kShiftCenter) act = act && (ii < jj);
+    bool mod = false;
+    if (w0 & 0x80000000u) {
+      const unsigned long long mf = M[2 * e], mm = M[2 * e + 1];
+      act = act && !((mf >> lane) & 1ull);
+      mod = (mm >> lane) & 1ull;
+    }
+    bool inA = (TA != PM_TERM_NONE) && r2 <= P.cutA2;
+    bool inB = (TB != PM_TERM_NONE) && r2 <= P.cutB2;
+    const bool nearA = (TA != PM_TERM_NONE) && fabsf(r2 - P.cutA2) <= P.guard;
+    const bool nearB = (TB != PM_TERM_NONE) && fabsf(r2 - P.cutB2) <= P.guard;
+    if (act && (nearA || nearB)) {
+      const double r2d = min_image_r2(S, S.orig[si], S.orig[sj]);
+      inA = (TA != PM_TERM_NONE) && r2d <= P.cutA2d;
+      inB = (TB != PM_TERM_NONE) && r2d <= P.cutB2d;
+    }
+    inA = inA && act;
+    inB = inB && act;
+    if (wave_ballot(inA || inB) == 0) continue;
+
+    const float rinv = rsqrtf(r2);
+    const float r2inv = rinv * rinv;
+    float fs = 0.f, ea = 0.f, eb = 0.f;
+    if (TA == PM_TERM_LJ) {
+      float A, B;
+      if (peratom) {
+        // GROMACS mixing (BuildTopology.cpp:650-659): sigma = (si+sj)/2, eps = sqrt(ei ej)
+        const float2 pj = S.ljp[sj];
+        const float sg = pi.x + pj.x, ep = pi.y * pj.y;
+        const float sg2 = sg * sg, sg6 = sg2 * sg2 * sg2;
+        B = 4.0f * ep * sg6;
+        A = B * sg6;
+        if (mod) {
+          A *= S.lj_scaling;
+          B *= S.lj_scaling;
+        }
+      } else {
+        const float4 p = S.lj_table[ti * T + S.ljt[sj]];
+        A = mod ? p.z : p.x;
+        B = mod ? p.w : p.y;
+      }
+      // LennardJonesForce::operator() (LennardJonesForce.h:45-58)
+      const float r6 = r2inv * r2inv * r2inv;
+      const float r6B = B * r6;
+      const float r12A = A * r6 * r6;
+      float E = r12A - r6B;
+      float F = (12.0f * r12A - 6.0f * r6B) * r2inv;
+      apply_switch(P.sw_a, r2, E, F);
+      fs += inA ? F : 0.f;
+      ea = inA ? E : 0.f;
+    } else if (TA == PM_TERM_COULOMB) {
+      float E = xi.w * xj.w * rinv;
+      if (mod) E *= S.csf;
+      float F = E * r2inv;
+      apply_switch(P.sw_a, r2, E, F);
+      fs += inA ? F : 0.f;
+      ea = inA ? E : 0.f;
+    }
+    if (TB == PM_TERM_COULOMB) {
+      // CoulombForce::operator() (CoulombForce.h:29-40)
+      float E = xi.w * xj.w * rinv;
+      if (mod) E *= S.csf;
+      float F = E * r2inv;
+      apply_switch(P.sw_b, r2, E, F);
+      fs += inB ? F : 0.f;
+      eb = inB ? E : 0.f;
+    } else if (TB == PM_TERM_EWALD_REAL) {
+      // NonbondedFullEwaldSystemForce::realTerm (NonbondedFullEwaldSystemForce.h:818-834):
+      // erfc(ar) by Abramowitz & Stegun 7.1.26, the approximation the reference uses
+      float qq = xi.w * xj.w;
+      if (mod) qq *= S.csf;
+      const float r = r2 * rinv;
+      const float ar = P.alpha * r;
+      const float t = __frcp_rn(1.0f + 0.3275911f * ar);
+      const float poly =
+          t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
+      const float ee = qq * exp2f(-(ar * ar) * 1.4426950408889634f);
+      const float E = poly * ee * rinv;
+      const float F = (E + P.two_alpha_pi * ee) * r2inv;
+      fs += inB ? F : 0.f;
+      eb = inB ? E : 0.f;
+    }
+    const float fx = fs * dx, fy = fs * dy, fz = fs * dz;
+    fix -= fx;
+    fiy -= fy;
+    fiz -= fz;
+    if (EN) {
+      eA += (double)ea;
+      eB += (double)eb;
+      vxx += (double)(fx * dx);
+      vxy += (double)(fx * dy);
+      vxz += (double)(fx * dz);
+      vyy += (double)(fy * dy);
+      vyz += (double)(fy * dz);
+      vzz += (double)(fz * dz);
+    }
+    if (CEN) {
+      if (inA || inB) {
+        const int aj = S.orig[sj];
+        const unsigned lo = (unsigned)min(ai, aj), hi = (unsigned)max(ai, aj);
+        ccount += 1;
+        chash += pair_hash(lo, hi);
+      }
+    }
+    // Newton's third law: j force = sum over the 8 i lanes that share jj
+    float gx = fx, gy = fy, gz = fz;
+    gx += __shfl_xor(gx, 8);
+    gy += __shfl_xor(gy, 8);
+    gz += __shfl_xor(gz, 8);
+    gx += __shfl_xor(gx, 16);
+    gy += __shfl_xor(gy, 16);
+    gz += __shfl_xor(gz, 16);
+    gx += __shfl_xor(gx, 32);
+    gy += __shfl_xor(gy, 32);
+    gz += __shfl_xor(gz, 32);
+    if (ii == 0 && jj < nvj) {
+      atomic_add_ll(&S.fx[sj], to_fixed((double)gx, kFScale));
+      atomic_add_ll(&S.fy[sj], to_fixed((double)gy, kFScale));
+      atomic_add_ll(&S.fz[sj], to_fixed((double)gz, kFScale));
+    }
+  }
+  // i force = sum over the 8 j lanes that share ii
+  fix += __shfl_xor(fix, 1);
+  fiy += __shfl_xor(fiy, 1);
+  fiz += __shfl_xor(fiz, 1);
+  fix += __shfl_xor(fix, 2);
+  fiy += __shfl_xor(fiy, 2);
+  fiz += __shfl_xor(fiz, 2);
+  fix += __shfl_xor(fix, 4);
+  fiy += __shfl_xor(fiy, 4);
+  fiz += __shfl_xor(fiz, 4);
+  if (jj == 0 && vi) {
+    atomic_add_ll(&S.fx[si], to_fixed((double)fix, kFScale));
+    atomic_add_ll(&S.fy[si], to_fixed((double)fiy, kFScale));
+    atomic_add_ll(&S.fz[si], to_fixed((double)fiz, kFScale));
+  }
+  if (EN) {
+    eA = wave_sum_d(eA);
+    eB = wave_sum_d(eB);
+    vxx = wave_sum_d(vxx);
+    vxy = wave_sum_d(vxy);
+    vxz = wave_sum_d(vxz);
+    vyy = wave_sum_d(vyy);
+    vyz = wave_sum_d(vyz);
+    vzz = wave_sum_d(vzz);
+    if (lane == 0) {
+      const int slotA = TA == PM_TERM_LJ ? kAccLJ : kAccCoul;
+      atomic_add_ll(&S.acc[slotA], to_fixed(eA, kEScale));
+      atomic_add_ll(&S.acc[kAccCoul], to_fixed(eB, kEScale));
+      atomic_add_ll(&S.acc[kAccVxx], to_fixed(vxx, kEScale));
+      atomic_add_ll(&S.acc[kAccVxy], to_fixed(vxy, kEScale));
+      atomic_add_ll(&S.acc[kAccVxz], to_fixed(vxz, kEScale));
+      atomic_add_ll(&S.acc[kAccVyy], to_fixed(vyy, kEScale));
+      atomic_add_ll(&S.acc[kAccVyz], to_fixed(vyz, kEScale));
+      atomic_add_ll(&S.acc[kAccVzz], to_fixed(vzz, kEScale));
+    }
+  }
+  if (CEN) {
+    ccount = wave_sum_u64(ccount);
+    chash = wave_sum_u64(chash);
+    if (lane == 0) {
+      atomicAdd(&S.census[0], ccount);
+      atomicAdd(&S.census[1], chash);
+    }
+  }
+}
+
+// ------------------------------------------------------- FP64 side terms
+
+// Numerical Recipes erfc (Press et al., "erfcc", fractional error < 1.2e-7):
+// the approximation ProtoMol's myerf uses (MathUtilities.cpp:12-36)
+__device__ __forceinline__ double nr_erf(double x) {
+  double erfcc = 0.0;
+  if (x < 9.0) {
+    double z = fabs(x);
+    double t = 1.0 / (1.0 + 0.5 * z);
+    erfcc = t * exp(-z * z - 1.26551223 +
+                    t * (1.00002368 +
+                         t * (.37409196 +
+                              t * (.09678418 +
+                                   t * (-.18628806 +
+                                        t * (.27886807 + t * (-1.13520398 + t * (1.48851587 + t * (-.82215223 + t * .17087277)))))))));
+    if (x < 0.0) erfcc = 2.0 - erfcc;
+  }
+  return 1.0 - erfcc;
+}
+
+__device__ __forceinline__ void min_image_vec(const SysDev &S, double *d) {
+  double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  if (!S.periodic || r2 <= S.D) return;
+  for (int k = 0; k < 3; ++k) {
+    if (d[k] < -S.H2[k]) d[k] += S.H[k];
+    else if (d[k] > S.H2[k]) d[k] -= S.H[k];
+  }
+  r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  if (r2 > S.D)
+    for (int k = 0; k < 3; ++k) d[k] -= S.L[k] * rint(S.Lr[k] * d[k]);
+}
+
+__device__ __forceinline__ void add_force_fixed(const SysDev &S, int a, double fx, double fy, double fz) {
+  const int sl = S.slot[a];
+  atomic_add_ll(&S.fx[sl], to_fixed(fx, kFScale));
+  atomic_add_ll(&S.fy[sl], to_fixed(fy, kFScale));
+  atomic_add_ll(&S.fz[sl], to_fixed(fz, kFScale));
+}
+
+// NonbondedFullEwaldSystemForce::correctionTerm (NonbondedFullEwaldSystemForce.h:1097-1131)
+// in FP64, one thread per excluded pair
+__global__ void k_ewald_corr(SysDev *sys, const int *xp, int nsys, ForceDev P, int energies) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  double eacc = 0.0, v[6] = {0, 0, 0, 0, 0, 0};
+  int s = -1;
+  if (g < xp[nsys]) {
+    s = find_sys(xp, nsys, g);
+    const SysDev &S = sys[s];
+    const int k = g - xp[s], n = S.n_atoms;
+    const int a1 = S.corr[3 * k], a2 = S.corr[3 * k + 1], cls = S.corr[3 * k + 2];
+    double d[3] = {S.x[a2] - S.x[a1], S.x[n + a2] - S.x[n + a1], S.x[2 * n + a2] - S.x[2 * n + a1]};
+    min_image_vec(S, d);
+    const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    double qq = S.charge[a1] * S.charge[a2];
+    if (cls == PM_EXCL_MODIFIED) qq *= 1.0 - (double)S.csf;
+    const double r = sqrt(r2), rr = 1.0 / r;
+    const double e = nr_erf(P.dalpha * r) * rr;
+    eacc = -qq * e;
+    const double fs = qq * (P.dtwo_alpha_pi * exp(-(P.dalpha * P.dalpha) * r2) - e) * rr * rr;
+    const double f[3] = {d[0] * fs, d[1] * fs, d[2] * fs};
+    add_force_fixed(S, a1, -f[0], -f[1], -f[2]);
+    add_force_fixed(S, a2, f[0], f[1], f[2]);
+    v[0] = f[0] * d[0]; v[1] = f[0] * d[1]; v[2] = f[0] * d[2];
+    v[3] = f[1] * d[1]; v[4] = f[1] * d[2]; v[5] = f[2] * d[2];
+  }
+  if (!energies || s < 0) return;
+  // per-thread atomics: exclusion pairs are few (3 per water)
+  const SysDev &S = sys[s];
+  atomic_add_ll(&S.acc[kAccCoul], to_fixed(eacc, kEScale));
+  atomic_add_ll(&S.acc[kAccVxx], to_fixed(v[0], kEScale));
+  atomic_add_ll(&S.acc[kAccVxy], to_fixed(v[1], kEScale));
+  atomic_add_ll(&S.acc[kAccVxz], to_fixed(v[2], kEScale));
+  atomic_add_ll(&S.acc[kAccVyy], to_fixed(v[3], kEScale));
+  atomic_add_ll(&S.acc[kAccVyz], to_fixed(v[4], kEScale));
+  atomic_add_ll(&S.acc[kAccVzz], to_fixed(v[5], kEScale));
+  (void)lane;
+}
+
+// BondSystemForce::calcBond (force/bonded/BondSystemForce.h:77-108) and
+// AngleSystemForce::calcAngle (AngleSystemForce.h:66-140), FP64, one thread per term
+__global__ void k_bonded(SysDev *sys, const int *tp, int nsys, int energies) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= tp[nsys]) return;
+  int s = find_sys(tp, nsys, g);
+  const SysDev &S = sys[s];
+  const int k = g - tp[s], n = S.n_atoms;
+  if (k < S.n_bonds) {
+    const int a1 = S.bonds[2 * k], a2 = S.bonds[2 * k + 1];
+    double r12[3] = {S.x[a1] - S.x[a2], S.x[n + a1] - S.x[n + a2], S.x[2 * n + a1] - S.x[2 * n + a2]};
+    min_image_vec(S, r12);
+    const double r = sqrt(r12[0] * r12[0] + r12[1] * r12[1] + r12[2] * r12[2]);
+    const double dp = 2.0 * S.bond_k[k] * (r - S.bond_r0[k]);
+    const double c = -dp / r;
+    add_force_fixed(S, a1, r12[0] * c, r12[1] * c, r12[2] * c);
+    add_force_fixed(S, a2, -r12[0] * c, -r12[1] * c, -r12[2] * c);
+    if (energies) {
+      const double e = S.bond_k[k] * (r - S.bond_r0[k]) * (r - S.bond_r0[k]);
+      atomic_add_ll(&S.acc[kAccBond], to_fixed(e, kEScale));
+      atomic_add_ll(&S.acc[kAccVxx], to_fixed(r12[0] * c * r12[0], kEScale));
+      atomic_add_ll(&S.acc[kAccVxy], to_fixed(r12[0] * c * r12[1], kEScale));
+      atomic_add_ll(&S.acc[kAccVxz], to_fixed(r12[0] * c * r12[2], kEScale));
+      atomic_add_ll(&S.acc[kAccVyy], to_fixed(r12[1] * c * r12[1], kEScale));
+      atomic_add_ll(&S.acc[kAccVyz], to_fixed(r12[1] * c * r12[2], kEScale));
+      atomic_add_ll(&S.acc[kAccVzz], to_fixed(r12[2] * c * r12[2], kEScale));
+    }
+    return;
+  }
+  const int q = k - S.n_bonds;
+  if (q >= S.n_angles) return;
+  const int a1 = S.angles[3 * q], a2 = S.angles[3 * q + 1], a3 = S.angles[3 * q + 2];
+  double r12[3] = {S.x[a1] - S.x[a2], S.x[n + a1] - S.x[n + a2], S.x[2 * n + a1] - S.x[2 * n + a2]};
+  double r32[3] = {S.x[a3] - S.x[a2], S.x[n + a3] - S.x[n + a2], S.x[2 * n + a3] - S.x[2 * n + a2]};
+  min_image_vec(S, r12);
+  min_image_vec(S, r32);
+  const double d12 = sqrt(r12[0] * r12[0] + r12[1] * r12[1] + r12[2] * r12[2]);
+  const double d32 = sqrt(r32[0] * r32[0] + r32[1] * r32[1] + r32[2] * r32[2]);
+  const double cr0 = r12[1] * r32[2] - r12[2] * r32[1], cr1 = r12[2] * r32[0] - r12[0] * r32[2],
+               cr2 = r12[0] * r32[1] - r12[1] * r32[0];
+  const double theta = atan2(sqrt(cr0 * cr0 + cr1 * cr1 + cr2 * cr2), r12[0] * r32[0] + r12[1] * r32[1] + r12[2] * r32[2]);
+  const double st = sin(theta), ct = cos(theta);
+  const double dp = 2.0 * S.angle_k[q] * (theta - S.angle_t0[q]);
+  double f1[3], f3[3];
+  for (int d = 0; d < 3; ++d) {
+    const double u12 = r12[d] / d12, u32 = r32[d] / d32;
+    f1[d] = -((u12 * ct - u32) / (st * d12)) * dp;
+    f3[d] = -((u32 * ct - u12) / (st * d32)) * dp;
+  }
+  add_force_fixed(S, a1, f1[0], f1[1], f1[2]);
+  add_force_fixed(S, a2, -f1[0] - f3[0], -f1[1] - f3[1], -f1[2] - f3[2]);
+  add_force_fixed(S, a3, f3[0], f3[1], f3[2]);
+  if (energies) {
+    const double e = S.angle_k[q] * (theta - S.angle_t0[q]) * (theta - S.angle_t0[q]);
+    atomic_add_ll(&S.acc[kAccAngle], to_fixed(e, kEScale));
+    atomic_add_ll(&S.acc[kAccVxx], to_fixed(f1[0] * r12[0] + f3[0] * r32[0], kEScale));
+    atomic_add_ll(&S.acc[kAccVxy], to_fixed(f1[0] * r12[1] + f3[0] * r32[1], kEScale));
+    atomic_add_ll(&S.acc[kAccVxz], to_fixed(f1[0] * r12[2] + f3[0] * r32[2], kEScale));
+    atomic_add_ll(&S.acc[kAccVyy], to_fixed(f1[1] * r12[1] + f3[1] * r32[1], kEScale));
+    atomic_add_ll(&S.acc[kAccVyz], to_fixed(f1[1] * r12[2] + f3[1] * r32[2], kEScale));
+    atomic_add_ll(&S.acc[kAccVzz], to_fixed(f1[2] * r12[2] + f3[2] * r32[2], kEScale));
+  }
+}
+
+// ------------------------------------------------------------ integrator
+
+// mode 0: doHalfKickdoDrift (LeapfrogIntegrator.cpp:33-56)  v += F*h*0.5/m; x += v*h
+// mode 1: doKickdoDrift     (LeapfrogIntegrator.cpp:58-87)  v += F*h/m;     x += v*h
+// The force buffer is consumed (zeroed) and the FP32 cluster-local
+// coordinates for the next force evaluation are written.
+__global__ void k_kick_drift(SysDev *sys, const int *ap, int nsys, double h, int mode) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  float d2 = 0.f;
+  int s = 0;
+  bool ok = g < ap[nsys];
+  if (ok) {
+    s = find_sys(ap, nsys, g);
+    SysDev &S = sys[s];
+    const int a = g - ap[s], n = S.n_atoms;
+    const int sl = S.slot[a];
+    const double m = 1.0 / S.inv_mass[a];
+    const double F[3] = {(double)S.fx[sl] * kFScaleInv, (double)S.fy[sl] * kFScaleInv,
+                         (double)S.fz[sl] * kFScaleInv};
+    S.fx[sl] = 0;
+    S.fy[sl] = 0;
+    S.fz[sl] = 0;
+    float l[3];
+    for (int d = 0; d < 3; ++d) {
+      double vv = S.v[d * n + a];
+      if (mode == 0) vv += F[d] * h * 0.5 / m;
+      else vv += F[d] * h / m;
+      S.v[d * n + a] = vv;
+      const double xx = S.x[d * n + a] + vv * h;
+      S.x[d * n + a] = xx;
+      l[d] = (float)(xx - S.xref[d * n + a]);
+    }
+    const float4 q = S.xq[sl];
+    S.xq[sl] = make_float4(l[0], l[1], l[2], q.w);
+    const float4 b = S.lbuild[a];
+    const float ex = l[0] - b.x, ey = l[1] - b.y, ez = l[2] - b.z;
+    d2 = ex * ex + ey * ey + ez * ez;
+    if (!isfinite(d2)) d2 = 3.0e38f;
+  }
+  // wave max per system: lanes of one wave may straddle two systems, so
+  // reduce across the wave only when every active lane has the same system
+  const int s0 = __shfl(s, 0);
+  const unsigned long long mixed = wave_ballot(ok && s != s0);
+  const float mx = wave_max_f(d2);
+  if (mixed == 0) {
+    if ((threadIdx.x & 63) == 0 && ok) atomicMax(sys[s0].maxdisp2, __float_as_uint(mx));
+  } else if (ok) {
+    atomicMax(sys[s].maxdisp2, __float_as_uint(d2));
+  }
+}
+
+// StandardIntegrator::doHalfKick (StandardIntegrator.cpp:80-92): h = 0.5*dt*ITF, v += F*h/m
+__global__ void k_half_kick(SysDev *sys, const int *ap, int nsys, double hh) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ap[nsys]) return;
+  const int s = find_sys(ap, nsys, g);
+  SysDev &S = sys[s];
+  const int a = g - ap[s], n = S.n_atoms;
+  const int sl = S.slot[a];
+  const double m = 1.0 / S.inv_mass[a];
+  const double F[3] = {(double)S.fx[sl] * kFScaleInv, (double)S.fy[sl] * kFScaleInv,
+                       (double)S.fz[sl] * kFScaleInv};
+  for (int d = 0; d < 3; ++d) S.v[d * n + a] += F[d] * hh / m;
+}
+
+// zero the fixed-point forces of every slot (a fill kernel, not a memset)
+__global__ void k_zero_forces(SysDev *sys, const int *slp, int nsys) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= slp[nsys]) return;
+  const int s = find_sys(slp, nsys, g);
+  SysDev &S = sys[s];
+  const int k = g - slp[s];
+  S.fx[k] = 0;
+  S.fy[k] = 0;
+  S.fz[k] = 0;
+}
+
+// per-system scalars: accumulators, census, displacement
+__global__ void k_clear_scalars(SysDev *sys, int nsys, int clear_disp) {
+  const int s = blockIdx.x;
+  if (s >= nsys) return;
+  SysDev &S = sys[s];
+  for (int k = threadIdx.x; k < kAccN; k += blockDim.x) S.acc[k] = 0;
+  if (threadIdx.x == 0) {
+    S.census[0] = 0;
+    S.census[1] = 0;
+    if (clear_disp) *S.maxdisp2 = 0u;
+  }
+}
+
+__global__ void k_export_f(SysDev *sys, const int *ap, int nsys, double *out) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ap[nsys]) return;
+  const int s = find_sys(ap, nsys, g);
+  const SysDev &S = sys[s];
+  const int a = g - ap[s];
+  const int sl = S.slot[a];
+  out[3 * (size_t)g] = (double)S.fx[sl] * kFScaleInv;
+  out[3 * (size_t)g + 1] = (double)S.fy[sl] * kFScaleInv;
+  out[3 * (size_t)g + 2] = (double)S.fz[sl] * kFScaleInv;
+}
+
+__global__ void k_export_xv(SysDev *sys, const int *ap, int nsys, double *xo, double *vo) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ap[nsys]) return;
+  const int s = find_sys(ap, nsys, g);
+  const SysDev &S = sys[s];
+  const int a = g - ap[s], n = S.n_atoms;
+  for (int d = 0; d < 3; ++d) {
+    if (xo) xo[3 * (size_t)g + d] = S.x[d * n + a];
+    if (vo) vo[3 * (size_t)g + d] = S.v[d * n + a];
+  }
+}
+
+// ------------------------------------------------------------- launchers
+
+static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
+
+void launch_import_x(const Launch &L, const double *xaos) {
+  if (L.n_atoms_total) hipLaunchKernelGGL(k_import_x, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys, xaos);
+}
+void launch_import_v(const Launch &L, const double *vaos) {
+  if (L.n_atoms_total) hipLaunchKernelGGL(k_import_v, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys, vaos);
+}
+
+void launch_rebuild(const Launch &L, float rlist2) {
+  if (!L.n_atoms_total) return;
+  hipLaunchKernelGGL(k_bin, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys);
+  hipLaunchKernelGGL(k_col_scan, dim3(L.nsys), dim3(1024), 0, L.stream, L.sys);
+  hipLaunchKernelGGL(k_col_fill, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys);
+  hipLaunchKernelGGL(k_col_sort, dim3(L.n_cols_total), dim3(1024), 0, L.stream, L.sys, L.col_prefix, L.nsys);
+  hipLaunchKernelGGL(k_cluster_geom, dim3(nblk(L.n_cl_total, 128)), dim3(128), 0, L.stream, L.sys, L.cl_prefix, L.nsys);
+  hipLaunchKernelGGL(k_pairlist, dim3(nblk((long long)L.n_cl_total * kWave, 256)), dim3(256), 0, L.stream, L.sys, L.cl_prefix, L.nsys, rlist2);
+}
+
+template <int TA, int TB>
+static void launch_pair_t(const Launch &L, const ForceDev &P, bool en, bool cen) {
+  dim3 g(nblk((long long)L.n_cl_total * kWave, 256)), b(256);
+  if (en && cen) hipLaunchKernelGGL((k_pair<TA, TB, true, true>), g, b, 0, L.stream, L.sys, L.cl_prefix, L.nsys, P);
+  else if (en) hipLaunchKernelGGL((k_pair<TA, TB, true, false>), g, b, 0, L.stream, L.sys, L.cl_prefix, L.nsys, P);
+  else if (cen) hipLaunchKernelGGL((k_pair<TA, TB, false, true>), g, b, 0, L.stream, L.sys, L.cl_prefix, L.nsys, P);
+  else hipLaunchKernelGGL((k_pair<TA, TB, false, false>), g, b, 0, L.stream, L.sys, L.cl_prefix, L.nsys, P);
+}
+
+bool launch_pair(const Launch &L, const ForceDev &P, bool en, bool cen) {
+  if (!L.n_cl_total) return true;
+  const int a = P.term_a, b = P.term_b;
+  if (a == PM_TERM_LJ && b == PM_TERM_NONE) launch_pair_t<PM_TERM_LJ, PM_TERM_NONE>(L, P, en, cen);
+  else if (a == PM_TERM_COULOMB && b == PM_TERM_NONE) launch_pair_t<PM_TERM_COULOMB, PM_TERM_NONE>(L, P, en, cen);
+  else if (a == PM_TERM_LJ && b == PM_TERM_COULOMB) launch_pair_t<PM_TERM_LJ, PM_TERM_COULOMB>(L, P, en, cen);
+  else if (a == PM_TERM_LJ && b == PM_TERM_EWALD_REAL) launch_pair_t<PM_TERM_LJ, PM_TERM_EWALD_REAL>(L, P, en, cen);
+  else if (a == PM_TERM_NONE && b == PM_TERM_EWALD_REAL) launch_pair_t<PM_TERM_NONE, PM_TERM_EWALD_REAL>(L, P, en, cen);
+  else return false;
+  return true;
+}
+
+void launch_ewald_corr(const Launch &L, const ForceDev &P, bool en) {
+  if (L.n_corr_total) hipLaunchKernelGGL(k_ewald_corr, dim3(nblk(L.n_corr_total, 256)), dim3(256), 0, L.stream, L.sys, L.corr_prefix, L.nsys, P, en ? 1 : 0);
+}
+void launch_bonded(const Launch &L, bool en) {
+  if (L.n_bonded_total) hipLaunchKernelGGL(k_bonded, dim3(nblk(L.n_bonded_total, 256)), dim3(256), 0, L.stream, L.sys, L.bonded_prefix, L.nsys, en ? 1 : 0);
+}
+void launch_kick_drift(const Launch &L, double h, int mode) {
+  if (L.n_atoms_total) hipLaunchKernelGGL(k_kick_drift, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys, h, mode);
+}
+void launch_half_kick(const Launch &L, double hh) {
+  if (L.n_atoms_total) hipLaunchKernelGGL(k_half_kick, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys, hh);
+}
+void launch_zero_forces(const Launch &L) {
+  if (L.n_slots_total) hipLaunchKernelGGL(k_zero_forces, dim3(nblk(L.n_slots_total, 256)), dim3(256), 0, L.stream, L.sys, L.slot_prefix, L.nsys);
+}
+void launch_clear_scalars(const Launch &L, bool clear_disp) {
+  hipLaunchKernelGGL(k_clear_scalars, dim3(L.nsys), dim3(64), 0, L.stream, L.sys, L.nsys, clear_disp ? 1 : 0);
+}
+void launch_export_f(const Launch &L, double *out) {
+  if (L.n_atoms_total) hipLaunchKernelGGL(k_export_f, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys, out);
+}
+void launch_export_xv(const Launch &L, double *xo, double *vo) {
+  if (L.n_atoms_total) hipLaunchKernelGGL(k_export_xv, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys, xo, vo);
+}
+
+}  // namespace pm
