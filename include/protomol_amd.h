@@ -41,7 +41,7 @@ extern "C" {
 #define PM_ECAPACITY -4 /* a device-side capacity was exceeded (column, list) */
 #define PM_ENOTREADY -5 /* state not set yet */
 
-/* switching functions (switch/*SwitchingFunction.h) */
+/* switching functions (protomol/switch/) */
 #define PM_SWITCH_NONE 0    /* UniversalSwitchingFunction */
 #define PM_SWITCH_CUTOFF 1  /* CutoffSwitchingFunction */
 #define PM_SWITCH_C1 2      /* C1SwitchingFunction */
@@ -66,7 +66,7 @@ extern "C" {
 /* flags */
 #define PM_FLAG_VIRIAL 1u     /* ScalarStructure::virial() on */
 #define PM_FLAG_GB 2u         /* GBBornRadii -> GBPartialSum -> GBForce, all pairs, vacuum */
-#define PM_FLAG_BONDED 4u     /* harmonic Bond + Angle (force/bonded/*SystemForce.h) */
+#define PM_FLAG_BONDED 4u     /* harmonic Bond + Angle (force/bonded/{Bond,Angle}SystemForce.h) */
 
 /* ScalarStructure slots (type/ScalarStructure.h:26-67); energies[] has 36 */
 #define PM_E_COULOMB 0

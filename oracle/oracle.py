@@ -1,0 +1,226 @@
+"""ctypes wrapper of liboracle.so, the FP64 C restatement of the reference path.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, always as the checker / CPU baseline, never as
+the product path.  Build with ``make -C oracle lib``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+SW_UNIVERSAL, SW_CUTOFF, SW_C1, SW_C2, SW_CN, SW_CMPCNCN = range(6)
+TERM_NONE, TERM_LJ, TERM_COULOMB, TERM_EWALD_REAL = range(4)
+COULOMB, LENNARDJONES, BOND, ANGLE = 0, 1, 2, 3
+VIRIALXX = 7
+NSLOTS = 36
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class OrcSwitch(C.Structure):
+    _fields_ = [("kind", C.c_int), ("use", C.c_int), ("modify", C.c_int),
+                ("cutoff", C.c_double), ("cutoff2", C.c_double),
+                ("switchon", C.c_double), ("switchon2", C.c_double),
+                ("switchoff", C.c_double), ("switchoff2", C.c_double),
+                ("s1", C.c_double), ("s2", C.c_double), ("s3", C.c_double),
+                ("c1_15rc1", C.c_double), ("c1_05rc3", C.c_double), ("c1_15rc3", C.c_double),
+                ("ord_int", C.c_int), ("ord_idx", C.c_int),
+                ("irange", C.c_double * 8), ("itrange", C.c_double * 8)]
+
+
+class OrcSystem(C.Structure):
+    _fields_ = [("n", C.c_int), ("charge", _dp), ("type", _ip), ("molecule", _ip), ("mass", _dp),
+                ("ntypes", C.c_int), ("lj", _dp), ("nexcl", C.c_int), ("excl", _ip),
+                ("box", C.c_double * 3), ("origin", C.c_double * 3),
+                ("coulomb_scaling", C.c_double), ("periodic", C.c_int)]
+
+
+class OrcNonbonded(C.Structure):
+    _fields_ = [("term_a", C.c_int), ("term_b", C.c_int), ("sw_a", OrcSwitch), ("sw_b", OrcSwitch),
+                ("algorithm_cutoff", C.c_double), ("cellsize", C.c_double),
+                ("virial", C.c_int), ("molvirial", C.c_int)]
+
+
+class OrcCensus(C.Structure):
+    _fields_ = [("count", C.c_uint64), ("hash", C.c_uint64)]
+
+
+class OrcEwald(C.Structure):
+    _fields_ = [("alpha", C.c_double), ("accuracy", C.c_double), ("rc", C.c_double), ("rc2", C.c_double),
+                ("two_alpha_pi", C.c_double), ("cellsize", C.c_double), ("correction", C.c_int),
+                ("virial", C.c_int)]
+
+
+class OrcGB(C.Structure):
+    _fields_ = [("vdw_radius", _dp), ("scaling", _dp), ("offset", C.c_double),
+                ("alpha_obc", C.c_double), ("beta_obc", C.c_double), ("gamma_obc", C.c_double),
+                ("solute_dielec", C.c_double), ("solvent_dielec", C.c_double), ("block_size", C.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle lib`")
+        L = C.CDLL(LIB_PATH)
+        L.orc_switch_init.argtypes = [C.POINTER(OrcSwitch), C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.orc_switch_eval.argtypes = [C.POINTER(OrcSwitch), C.c_double, _dp, _dp]
+        L.orc_min_diff.argtypes = [C.POINTER(OrcSystem), _dp, _dp, _dp]
+        L.orc_min_diff.restype = C.c_double
+        L.orc_nonbonded_cutoff.argtypes = [C.POINTER(OrcSystem), C.POINTER(OrcNonbonded), _dp, _dp, _dp,
+                                           C.POINTER(OrcCensus), C.c_int]
+        L.orc_census_bruteforce.argtypes = [C.POINTER(OrcSystem), _dp, C.c_double, C.POINTER(OrcCensus)]
+        L.orc_ewald_init.argtypes = [C.POINTER(OrcEwald), C.c_double, C.c_double, C.c_double, C.c_int, C.c_int]
+        L.orc_ewald_real.argtypes = [C.POINTER(OrcSystem), C.POINTER(OrcEwald), _dp, _dp, _dp,
+                                     C.POINTER(OrcCensus), C.c_int]
+        L.orc_poly5.argtypes = [C.c_double]
+        L.orc_poly5.restype = C.c_double
+        L.orc_myerf.argtypes = [C.c_double]
+        L.orc_myerf.restype = C.c_double
+        L.orc_gb_evaluate.argtypes = [C.POINTER(OrcSystem), C.POINTER(OrcGB), _dp, _dp, _dp, _dp]
+        L.orc_bonded.argtypes = [C.POINTER(OrcSystem), C.c_int, _ip, _dp, _dp, C.c_int, _ip, _dp, _dp, _dp, _dp, _dp]
+        L.orc_pair_hash.argtypes = [C.c_uint32, C.c_uint32]
+        L.orc_pair_hash.restype = C.c_uint64
+        _lib = L
+    return _lib
+
+
+def _p(a, t=_dp):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+class OracleSystem:
+    """Keeps the numpy buffers alive behind an OrcSystem."""
+
+    def __init__(self, sys):
+        self.sys = sys
+        self.charge = np.ascontiguousarray(sys.charge, np.float64)
+        self.mass = np.ascontiguousarray(sys.mass, np.float64)
+        self.molecule = np.ascontiguousarray(sys.molecule, np.int32)
+        self.type = np.ascontiguousarray(sys.lj_types(), np.int32)
+        self.lj = np.ascontiguousarray(sys.lj_table_full(), np.float64)
+        self.excl = np.ascontiguousarray(sys.exclusions, np.int32)
+        s = OrcSystem()
+        s.n = sys.n
+        s.charge = _p(self.charge)
+        s.type = _p(self.type, _ip)
+        s.molecule = _p(self.molecule, _ip)
+        s.mass = _p(self.mass)
+        s.ntypes = self.lj.shape[0]
+        s.lj = _p(self.lj)
+        s.nexcl = len(self.excl)
+        s.excl = _p(self.excl, _ip)
+        for k in range(3):
+            s.box[k] = float(sys.box[k])
+            s.origin[k] = float(sys.origin[k])
+        s.coulomb_scaling = float(sys.coulomb_scaling)
+        s.periodic = 1 if sys.periodic else 0
+        self.c = s
+
+
+def make_switch(kind, switchon=0.0, cutoff=0.0, switchoff=0.0, order=0.0):
+    sw = OrcSwitch()
+    lib().orc_switch_init(C.byref(sw), kind, switchon, cutoff, switchoff, order)
+    return sw
+
+
+def switch_eval(sw, r2):
+    v = C.c_double()
+    d = C.c_double()
+    lib().orc_switch_eval(C.byref(sw), r2, C.byref(v), C.byref(d))
+    return v.value, d.value
+
+
+def nonbonded_cutoff(sys, term_a, sw_a, term_b=TERM_NONE, sw_b=None, algorithm_cutoff=None, cellsize=5.0,
+                     virial=False, nthreads=1, x=None, census=True):
+    """NonbondedCutoffSystemForce::evaluate on fresh force/energy buffers.
+    Returns (forces (n,3), energies[36], (count, hash))."""
+    os_ = OracleSystem(sys)
+    nb = OrcNonbonded()
+    nb.term_a, nb.term_b = term_a, term_b
+    nb.sw_a = sw_a
+    if sw_b is not None:
+        nb.sw_b = sw_b
+    cut = max(sw_a.cutoff, sw_b.cutoff if sw_b is not None and term_b else 0.0)
+    nb.algorithm_cutoff = algorithm_cutoff if algorithm_cutoff else cut
+    nb.cellsize = cellsize
+    nb.virial = 1 if virial else 0
+    xx = np.ascontiguousarray(sys.x if x is None else x, np.float64).reshape(-1)
+    f = np.zeros(3 * sys.n)
+    e = np.zeros(NSLOTS)
+    cen = OrcCensus()
+    rc = lib().orc_nonbonded_cutoff(C.byref(os_.c), C.byref(nb), _p(xx), _p(f), _p(e),
+                                    C.byref(cen) if census else None, nthreads)
+    if rc != 0:
+        raise RuntimeError(f"orc_nonbonded_cutoff failed: {rc}")
+    return f.reshape(-1, 3), e, (cen.count, cen.hash)
+
+
+def ewald_real(sys, alpha, accuracy, cellsize=5.0, correction=True, virial=False, nthreads=1, x=None):
+    os_ = OracleSystem(sys)
+    ew = OrcEwald()
+    lib().orc_ewald_init(C.byref(ew), alpha, accuracy, cellsize, 1 if correction else 0, 1 if virial else 0)
+    xx = np.ascontiguousarray(sys.x if x is None else x, np.float64).reshape(-1)
+    f = np.zeros(3 * sys.n)
+    e = np.zeros(NSLOTS)
+    cen = OrcCensus()
+    rc = lib().orc_ewald_real(C.byref(os_.c), C.byref(ew), _p(xx), _p(f), _p(e), C.byref(cen), nthreads)
+    if rc != 0:
+        raise RuntimeError(f"orc_ewald_real failed: {rc}")
+    return f.reshape(-1, 3), e, (cen.count, cen.hash), ew.rc
+
+
+def census_bruteforce(sys, cut2, x=None):
+    os_ = OracleSystem(sys)
+    xx = np.ascontiguousarray(sys.x if x is None else x, np.float64).reshape(-1)
+    cen = OrcCensus()
+    lib().orc_census_bruteforce(C.byref(os_.c), _p(xx), cut2, C.byref(cen))
+    return cen.count, cen.hash
+
+
+def gb_evaluate(sys, offset=0.09, a=1.0, b=0.8, g=4.85, eps_in=1.0, eps_out=78.3, x=None):
+    os_ = OracleSystem(sys)
+    rho = np.ascontiguousarray(sys.gb_radius, np.float64)
+    S = np.ascontiguousarray(sys.gb_scaling, np.float64)
+    gb = OrcGB()
+    gb.vdw_radius = _p(rho)
+    gb.scaling = _p(S)
+    gb.offset, gb.alpha_obc, gb.beta_obc, gb.gamma_obc = offset, a, b, g
+    gb.solute_dielec, gb.solvent_dielec = eps_in, eps_out
+    gb.block_size = 64
+    xx = np.ascontiguousarray(sys.x if x is None else x, np.float64).reshape(-1)
+    f = np.zeros(3 * sys.n)
+    e = np.zeros(NSLOTS)
+    born = np.zeros(sys.n)
+    rc = lib().orc_gb_evaluate(C.byref(os_.c), C.byref(gb), _p(xx), _p(f), _p(e), _p(born))
+    if rc != 0:
+        raise RuntimeError("orc_gb_evaluate failed")
+    return f.reshape(-1, 3), e, born
+
+
+def bonded(sys, x=None):
+    os_ = OracleSystem(sys)
+    bonds = np.ascontiguousarray(sys.bonds, np.int32)
+    angles = np.ascontiguousarray(sys.angles, np.int32)
+    bk, br = np.ascontiguousarray(sys.bond_k, np.float64), np.ascontiguousarray(sys.bond_r0, np.float64)
+    ak, at = np.ascontiguousarray(sys.angle_k, np.float64), np.ascontiguousarray(sys.angle_t0, np.float64)
+    xx = np.ascontiguousarray(sys.x if x is None else x, np.float64).reshape(-1)
+    f = np.zeros(3 * sys.n)
+    e = np.zeros(NSLOTS)
+    lib().orc_bonded(C.byref(os_.c), len(bonds), _p(bonds, _ip), _p(bk), _p(br), len(angles), _p(angles, _ip),
+                     _p(ak), _p(at), _p(xx), _p(f), _p(e))
+    return f.reshape(-1, 3), e
+
+
+def pair_hash(lo, hi):
+    return lib().orc_pair_hash(lo, hi)

@@ -819,7 +819,9 @@ int orc_ewald_real(const orc_system *sys, const orc_ewald *ew, const double *x, 
   double intra = 0.0, point = 0.0, charged = 0.0;
   if (ew->correction) {
     /* correctionTerm (NonbondedFullEwaldSystemForce.h:1097-1131) over the
-     * ExclusionTable's pair table in add() order, duplicates dropped */
+     * ExclusionTable's pair table in add() order, duplicates dropped.  Off
+     * Windows, MathUtilities.h:154-161 declares no ProtoMol::erf, so the
+     * unqualified erf() there is the C library's. */
     for (int k = 0; k < sys->nexcl; k++) {
       int a1 = sys->excl[3 * k], a2 = sys->excl[3 * k + 1], cls = sys->excl[3 * k + 2];
       if (cls == ORC_EXCL_NONE || a1 == a2) continue;
@@ -837,7 +839,7 @@ int orc_ewald_real(const orc_system *sys, const orc_ewald *ew, const double *x, 
       if (cls == ORC_EXCL_MODIFIED) qq *= 1 - sys->coulomb_scaling;
       double r = sqrt(r2);
       double rr = 1 / r;
-      double ee = orc_myerf(ew->alpha * r) * rr;
+      double ee = erf(ew->alpha * r) * rr;
       intra -= qq * ee;
       double fs = qq * (ew->two_alpha_pi * exp(-(ew->alpha * ew->alpha) * r2) - ee) * rr * rr;
       double fij[3] = {rij[0] * fs, rij[1] * fs, rij[2] * fs};

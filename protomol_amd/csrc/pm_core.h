@@ -52,6 +52,8 @@ struct ForceDev {
   double dalpha, dtwo_alpha_pi;             // Ewald correction (FP64)
   float rlist2;
   unsigned flags;
+  // GB / OBC (ProtoMolApp.cpp:285-301, GBForce.cpp:11-12)
+  double gb_offset, gb_a, gb_b, gb_g, gb_fac;  // fac = 1/eps_solute - 1/eps_solvent
 };
 
 // One system as the kernels see it.  All pointers are device pointers owned
@@ -108,13 +110,14 @@ struct SysDev {
   const int *corr;         // 3 per pair: a, b, class
   // GB
   const double *gb_rho, *gb_S;
-  double *gb_burial, *gb_born, *gb_psi, *gb_partial;
+  double *gb_born, *gb_drfac, *gb_partial, *gb_spare;
   // per-system scalars
   int *n_clusters;
   long long *acc;          // kAccN fixed-point energy/virial accumulators
   unsigned long long *census; // count, hash
   unsigned *err;
   unsigned *maxdisp2;      // float bits of the largest squared displacement since rebuild
+  unsigned *maxdisp_run;   // the same, folded over every rebuild of the current run
 };
 
 }  // namespace pm

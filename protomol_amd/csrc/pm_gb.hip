@@ -1,0 +1,368 @@
+// pm_gb.hip -- Generalized Born (OBC) implicit solvent on gfx950.
+//
+// ProtoMol evaluates GB as three dependent all-pairs passes over
+// NonbondedSimpleFullSystemForce (GBBornRadii -> GBPartialSum -> GBForce) and
+// stores ten N x N intermediate arrays per system (topology/Atom.h:62-83).
+// Here each pass recomputes what it needs instead: one workgroup owns 256
+// i atoms (one per lane) and streams every j atom through LDS, so an atom's
+// Born radius, partial sum and force are complete in its own lane.  No N x N
+// storage, no atomics inside a pass, a fixed summation order per atom
+// (bitwise reproducible), FP32 pair arithmetic on FP64 coordinate differences,
+// FP64 accumulation and FP64 per-atom Born-radius algebra.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "pm_gb.h"
+
+namespace pm {
+
+void gb_setup_force(ForceDev &P, const pm_force_desc &d) {
+  P.gb_offset = d.gb_offset > 0 ? d.gb_offset : 0.09;
+  P.gb_a = d.gb_alpha;
+  P.gb_b = d.gb_beta;
+  P.gb_g = d.gb_gamma;
+  const double ein = d.gb_solute_dielec > 0 ? d.gb_solute_dielec : 1.0;
+  const double eout = d.gb_solvent_dielec > 0 ? d.gb_solvent_dielec : 80.0;
+  P.gb_fac = (1.0 / ein) - (1.0 / eout);
+  P.term_a = PM_TERM_LJ;
+  P.term_b = PM_TERM_COULOMB;
+}
+
+void gb_setup_system(SysDev &S, const double *rho, const double *scaling, double *born, double *drfac,
+                     double *partial, double *spare) {
+  S.gb_rho = rho;
+  S.gb_S = scaling;
+  S.gb_born = born;
+  S.gb_drfac = drfac;
+  S.gb_partial = partial;
+  S.gb_spare = spare;
+}
+
+namespace {
+
+__device__ __forceinline__ int find_sys_gb(const int *__restrict__ prefix, int nsys, int w) {
+  int lo = 0, hi = nsys;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= w) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// contribution of atom j to the burial term of atom i (GBBornRadii.cpp:59-91)
+__device__ __forceinline__ float burial_term(float dist, float ori, float orj, float sj) {
+  const float sorj = sj * orj;
+  float L, U;
+  if (ori >= dist + sorj) {
+    L = 1.0f;
+    U = 1.0f;
+  } else {
+    const float t = fabsf(dist - sorj);
+    L = ori > t ? ori : t;
+    U = dist + sorj;
+  }
+  const float C = ori < sorj - dist ? 2.0f * (1.0f / ori - 1.0f / L) : 0.0f;
+  const float iL = 1.0f / L, iU = 1.0f / U, iL2 = iL * iL, iU2 = iU * iU;
+  const float idist = 1.0f / dist;
+  return (iL - iU) + (0.25f * dist) * (iU2 - iL2) + (0.5f * idist) * __logf(L * iU) +
+         (0.25f * sorj * sorj * idist) * (iL2 - iU2) + C;
+}
+
+// d(burial_i)/d(r_ij) (GBForce.cpp:55-126); the term (U/(4 d L))(dL/U - L dU/U^2)
+// of the reference is written in its equal form (dL/L - dU/U)/(4 d)
+__device__ __forceinline__ float dburial(float dist, float ori, float orj, float sj) {
+  const float sorj = sj * orj;
+  float L, U;
+  if (ori >= dist + sorj) {
+    L = 1.0f;
+    U = 1.0f;
+  } else {
+    const float t = fabsf(dist - sorj);
+    L = ori > t ? ori : t;
+    U = dist + sorj;
+  }
+  const float iL = 1.0f / L, iU = 1.0f / U, iL2 = iL * iL, iU2 = iU * iU;
+  const float dL = ori <= dist - sorj ? 1.0f : 0.0f;
+  const float dU = ori < dist + sorj ? 1.0f : 0.0f;
+  const float dC = ori <= sorj - dist ? 2.0f * iL2 * dL : 0.0f;
+  const float idist = 1.0f / dist;
+  const float st = sorj * idist;
+  const float s2 = sorj * sorj;
+  return -0.5f * dL * iL2 + 0.5f * dU * iU2 + 0.125f * (iU2 - iL2) +
+         0.125f * dist * (2.0f * iL2 * iL * dL - 2.0f * iU2 * iU * dU) -
+         0.25f * idist * idist * __logf(L * iU) + 0.25f * idist * (dL * iL - dU * iU) - 0.125f * st * st * (iL2 - iU2) +
+         0.25f * s2 * idist * iU2 * iU * dU - 0.25f * s2 * idist * iL2 * iL * dL + dC;
+}
+
+struct GBShared {
+  double x[kGBBlock], y[kGBBlock], z[kGBBlock];
+  float q[kGBBlock], ori[kGBBlock], S[kGBBlock], B[kGBBlock], dr[kGBBlock], part[kGBBlock], sg[kGBBlock],
+      ep[kGBBlock];
+};
+
+__device__ void load_chunk(GBShared &sh, const SysDev &S, const ForceDev &P, int j0, int pass) {
+  const int t = threadIdx.x, n = S.n_atoms;
+  const int j = j0 + t;
+  if (j < n) {
+    sh.x[t] = S.x[j];
+    sh.y[t] = S.x[n + j];
+    sh.z[t] = S.x[2 * n + j];
+    sh.ori[t] = (float)(S.gb_rho[j] - P.gb_offset);
+    sh.S[t] = (float)S.gb_S[j];
+    if (pass >= 2) {
+      sh.q[t] = (float)S.charge[j];
+      sh.B[t] = (float)S.gb_born[j];
+    }
+    if (pass >= 3) {
+      sh.dr[t] = (float)S.gb_drfac[j];
+      sh.part[t] = (float)S.gb_partial[j];
+      if (S.lj_mode == PM_LJ_PERATOM) {
+        sh.sg[t] = S.sig_half[j];
+        sh.ep[t] = S.eps_sqrt[j];
+      } else {
+        sh.sg[t] = __int_as_float(S.type[j]);
+        sh.ep[t] = 0.f;
+      }
+    }
+  }
+}
+
+}  // namespace
+
+// pass 1: burial sums, then the OBC Born radius (GBBornRadii::postProcess,
+// GBBornRadii.cpp:145-173) and the per-atom factor of dR_i/dr_ij
+// (GBForce.cpp:126: R_i^2 rho~_i (1 - tanh^2) dtanh/dpsi / rho_i)
+__global__ __launch_bounds__(kGBBlock) void k_gb_born(SysDev *sys, const int *gp, int nsys, ForceDev P) {
+  __shared__ GBShared sh;
+  const int tile = blockIdx.x;
+  if (tile >= gp[nsys]) return;
+  const int s = find_sys_gb(gp, nsys, tile);
+  const SysDev &S = sys[s];
+  const int n = S.n_atoms;
+  const int i = (tile - gp[s]) * kGBBlock + threadIdx.x;
+  const bool vi = i < n;
+  double xi = 0, yi = 0, zi = 0;
+  float ori = 1.f;
+  if (vi) {
+    xi = S.x[i];
+    yi = S.x[n + i];
+    zi = S.x[2 * n + i];
+    ori = (float)(S.gb_rho[i] - P.gb_offset);
+  }
+  double burial = 0.0;
+  for (int j0 = 0; j0 < n; j0 += kGBBlock) {
+    __syncthreads();
+    load_chunk(sh, S, P, j0, 1);
+    __syncthreads();
+    const int cnt = min(kGBBlock, n - j0);
+    if (!vi) continue;
+    for (int k = 0; k < cnt; ++k) {
+      if (j0 + k == i) continue;
+      const float dx = (float)(sh.x[k] - xi), dy = (float)(sh.y[k] - yi), dz = (float)(sh.z[k] - zi);
+      const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+      burial += (double)burial_term(dist, ori, sh.ori[k], sh.S[k]);
+    }
+  }
+  if (!vi) return;
+  const double rho = S.gb_rho[i], orid = rho - P.gb_offset;
+  const double psi = 0.5 * burial * orid;
+  const double tp = P.gb_a * psi - P.gb_b * psi * psi + P.gb_g * psi * psi * psi;
+  const double th = tanh(tp);
+  const double B = 1.0 / ((1.0 / orid) - (1.0 / rho) * th);
+  const double tderv = P.gb_a - 2.0 * P.gb_b * psi + 3.0 * P.gb_g * psi * psi;
+  S.gb_born[i] = B;
+  S.gb_drfac[i] = B * B * orid * (1.0 - th * th) * tderv * (1.0 / rho);
+}
+
+// pass 2: partial sums (GBPartialSum.cpp:11-42): sum over l != i of
+// q_i q_l f^-3 / 2 e^{-r^2/4RiRl} (R_l + r^2 / 4 R_i)
+__global__ __launch_bounds__(kGBBlock) void k_gb_partial(SysDev *sys, const int *gp, int nsys, ForceDev P) {
+  __shared__ GBShared sh;
+  const int tile = blockIdx.x;
+  if (tile >= gp[nsys]) return;
+  const int s = find_sys_gb(gp, nsys, tile);
+  const SysDev &S = sys[s];
+  const int n = S.n_atoms;
+  const int i = (tile - gp[s]) * kGBBlock + threadIdx.x;
+  const bool vi = i < n;
+  double xi = 0, yi = 0, zi = 0;
+  float qi = 0.f, Bi = 1.f;
+  if (vi) {
+    xi = S.x[i];
+    yi = S.x[n + i];
+    zi = S.x[2 * n + i];
+    qi = (float)S.charge[i];
+    Bi = (float)S.gb_born[i];
+  }
+  const float i4Bi = 0.25f / Bi;
+  double acc = 0.0;
+  for (int j0 = 0; j0 < n; j0 += kGBBlock) {
+    __syncthreads();
+    load_chunk(sh, S, P, j0, 2);
+    __syncthreads();
+    const int cnt = min(kGBBlock, n - j0);
+    if (!vi) continue;
+    for (int k = 0; k < cnt; ++k) {
+      if (j0 + k == i) continue;
+      const float dx = (float)(sh.x[k] - xi), dy = (float)(sh.y[k] - yi), dz = (float)(sh.z[k] - zi);
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      const float Bj = sh.B[k];
+      const float BB = Bi * Bj;
+      const float ex = __expf(-r2 / (4.0f * BB));
+      const float f = sqrtf(r2 + BB * ex);
+      const float inv = 1.0f / f;
+      const float part = qi * sh.q[k] * inv * inv * 0.5f * inv * ex;
+      acc += (double)(part * (Bj + r2 * i4Bi));
+    }
+  }
+  if (vi) S.gb_partial[i] = acc;
+}
+
+// pass 3: GB pair force and energy (GBForce.cpp:14-175), self energy
+// (GBForce.cpp:50-60, 230-242), plus the all-pairs Lennard-Jones and
+// Coulomb terms (OneAtomPair<Vacuum, Universal, ...> with the exclusion table)
+__global__ __launch_bounds__(kGBBlock) void k_gb_force(SysDev *sys, const int *gp, int nsys, ForceDev P, int energies) {
+  __shared__ GBShared sh;
+  __shared__ double red[4][kGBBlock / 64];
+  const int tile = blockIdx.x;
+  if (tile >= gp[nsys]) return;
+  const int s = find_sys_gb(gp, nsys, tile);
+  const SysDev &S = sys[s];
+  const int n = S.n_atoms;
+  const int i = (tile - gp[s]) * kGBBlock + threadIdx.x;
+  const bool vi = i < n;
+  const bool peratom = S.lj_mode == PM_LJ_PERATOM;
+  double xi = 0, yi = 0, zi = 0;
+  float qi = 0.f, Bi = 1.f, ori = 1.f, Si = 1.f, dri = 0.f, Pi = 0.f, sgi = 0.f, epi = 0.f;
+  int ti = 0;
+  int ep = 0, ee = 0, nextx = 0x7fffffff;
+  if (vi) {
+    xi = S.x[i];
+    yi = S.x[n + i];
+    zi = S.x[2 * n + i];
+    qi = (float)S.charge[i];
+    Bi = (float)S.gb_born[i];
+    ori = (float)(S.gb_rho[i] - P.gb_offset);
+    Si = (float)S.gb_S[i];
+    dri = (float)S.gb_drfac[i];
+    Pi = (float)S.gb_partial[i];
+    if (peratom) {
+      sgi = S.sig_half[i];
+      epi = S.eps_sqrt[i];
+    } else {
+      ti = S.type[i];
+    }
+    ep = S.excl_start[i];
+    ee = S.excl_start[i + 1];
+    if (ep < ee) nextx = S.excl_partner[ep];
+  }
+  const float fac = (float)P.gb_fac;
+  const float i4Bi = 0.25f / Bi;
+  const float csf = S.csf;
+  double fx = 0.0, fy = 0.0, fz = 0.0, egb = 0.0, elj = 0.0, eco = 0.0;
+  for (int j0 = 0; j0 < n; j0 += kGBBlock) {
+    __syncthreads();
+    load_chunk(sh, S, P, j0, 3);
+    __syncthreads();
+    const int cnt = min(kGBBlock, n - j0);
+    if (!vi) continue;
+    for (int k = 0; k < cnt; ++k) {
+      const int j = j0 + k;
+      if (j == i) continue;
+      int cls = 0;
+      while (j > nextx) {  // partners are sorted; skip ones passed already
+        ++ep;
+        nextx = ep < ee ? S.excl_partner[ep] : 0x7fffffff;
+      }
+      if (j == nextx) cls = S.excl_class[ep];
+      const float dx = (float)(sh.x[k] - xi), dy = (float)(sh.y[k] - yi), dz = (float)(sh.z[k] - zi);
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      const float rinv = rsqrtf(r2);
+      const float dist = r2 * rinv;
+      const float Bj = sh.B[k];
+      const float BB = Bi * Bj;
+      const float ex = __expf(-r2 / (4.0f * BB));
+      const float fgb = sqrtf(r2 + BB * ex);
+      const float inv = 1.0f / fgb;
+      const float qq = qi * sh.q[k];
+      const float dRi = dri * dburial(dist, ori, sh.ori[k], sh.S[k]);
+      const float dRj = sh.dr[k] * dburial(dist, sh.ori[k], ori, Si);
+      const float c0 = qq * inv * inv * 0.5f * inv;
+      const float ai = Bj + r2 * i4Bi, bj = Bi + r2 * (0.25f / Bj);
+      float force = -c0 * ((2.0f * dist - 0.5f * ex * dist) + ex * dRi * ai + ex * dRj * bj) * rinv;
+      force -= 0.5f * qi * qi * dRi / (Bi * Bi) * rinv;
+      force -= 0.5f * sh.q[k] * sh.q[k] * dRj / (Bj * Bj) * rinv;
+      const float part = c0 * ex;
+      force -= (Pi - part * ai) * (dRi * rinv);
+      force -= (sh.part[k] - part * bj) * (dRj * rinv);
+      force *= fac;
+      float fs = force;
+      const bool upper = j > i;
+      if (energies && upper) egb += (double)(-qq * inv * fac);
+      if (cls != PM_EXCL_FULL) {
+        float A, Bc;
+        if (peratom) {
+          const float sg = sgi + sh.sg[k], e2 = epi * sh.ep[k];
+          const float sg2 = sg * sg, sg6 = sg2 * sg2 * sg2;
+          Bc = 4.0f * e2 * sg6;
+          A = Bc * sg6;
+          if (cls == PM_EXCL_MODIFIED) {
+            A *= S.lj_scaling;
+            Bc *= S.lj_scaling;
+          }
+        } else {
+          const float4 p = S.lj_table[ti * S.n_types + __float_as_int(sh.sg[k])];
+          A = cls == PM_EXCL_MODIFIED ? p.z : p.x;
+          Bc = cls == PM_EXCL_MODIFIED ? p.w : p.y;
+        }
+        const float r2inv = rinv * rinv;
+        const float r6 = r2inv * r2inv * r2inv;
+        const float r6B = Bc * r6, r12A = A * r6 * r6;
+        float ec = qq * rinv;
+        if (cls == PM_EXCL_MODIFIED) ec *= csf;
+        fs += (12.0f * r12A - 6.0f * r6B) * r2inv + ec * r2inv;
+        if (energies && upper) {
+          elj += (double)(r12A - r6B);
+          eco += (double)ec;
+        }
+      }
+      fx -= (double)(fs * dx);
+      fy -= (double)(fs * dy);
+      fz -= (double)(fs * dz);
+    }
+  }
+  if (vi) {
+    atomicAdd(reinterpret_cast<unsigned long long *>(&S.fx[i]), (unsigned long long)__double2ll_rn(fx * kFScale));
+    atomicAdd(reinterpret_cast<unsigned long long *>(&S.fy[i]), (unsigned long long)__double2ll_rn(fy * kFScale));
+    atomicAdd(reinterpret_cast<unsigned long long *>(&S.fz[i]), (unsigned long long)__double2ll_rn(fz * kFScale));
+    if (energies) egb += -0.5 * (double)S.charge[i] * (double)S.charge[i] * (1.0 / S.gb_born[i]) * P.gb_fac;
+  }
+  if (!energies) return;
+  // block reduction in a fixed order, then fixed-point totals
+  double v[3] = {egb, elj, eco};
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int c = 0; c < 3; ++c) {
+    double t = v[c];
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+    if (lane == 0) red[c][wv] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t[3] = {0, 0, 0};
+    for (int c = 0; c < 3; ++c)
+      for (int w = 0; w < kGBBlock / 64; ++w) t[c] += red[c][w];
+    atomicAdd(reinterpret_cast<unsigned long long *>(&S.acc[kAccCoul]), (unsigned long long)__double2ll_rn((t[0] + t[2]) * kEScale));
+    atomicAdd(reinterpret_cast<unsigned long long *>(&S.acc[kAccLJ]), (unsigned long long)__double2ll_rn(t[1] * kEScale));
+  }
+}
+
+void launch_gb(const Launch &L, const ForceDev &P, bool energies) {
+  if (!L.n_gb_tiles_total) return;
+  dim3 g(L.n_gb_tiles_total), b(kGBBlock);
+  hipLaunchKernelGGL(k_gb_born, g, b, 0, L.stream, L.sys, L.gb_prefix, L.nsys, P);
+  hipLaunchKernelGGL(k_gb_partial, g, b, 0, L.stream, L.sys, L.gb_prefix, L.nsys, P);
+  hipLaunchKernelGGL(k_gb_force, g, b, 0, L.stream, L.sys, L.gb_prefix, L.nsys, P, energies ? 1 : 0);
+}
+
+}  // namespace pm

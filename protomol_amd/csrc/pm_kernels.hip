@@ -104,53 +104,43 @@ __constant__ double kDSwCoef[5][7] = {{30., -60., 30., 0, 0, 0, 0},
                                       {0, 0, 0, 0, 0, 0, 0},
                                       {12012., -72072., 180180., -240240., 180180., -72072., 12012.0}};
 
-__device__ __noinline__ void switch_cn_d(const SwitchDev &sw, double r2, double &val, double &der) {
+// sum_i coef[i] t^(ord+1+i) r[i] and sum_i dcoef[i] t^(ord+i) r[i], running powers
+__device__ __forceinline__ void cn_poly(const SwitchDev &sw, double t, const double *r, double &val, double &der) {
+  double p = 1.0;
+  for (int k = 0; k < sw.ord_int; ++k) p *= t;
+  val = 0.0;
+  der = 0.0;
+  for (int i = 0; i < sw.ord_int + 1; ++i) {
+    der += kDSwCoef[sw.ord_idx][i] * p * r[i];
+    p *= t;
+    val += kSwCoef[sw.ord_idx][i] * p * r[i];
+  }
+}
+
+__device__ __forceinline__ void switch_cn_d(const SwitchDev &sw, double r2, double &val, double &der) {
   // CnSwitchingFunction::operator() (CnSwitchingFunction.cpp:149-175) and the
   // two branches of CmpCnCnSwitchingFunction::operator() (CmpCnCnSwitchingFunction.h:52-95)
   val = 0.0;
   der = 0.0;
   if (r2 > sw.dcut2) return;
-  double sd = sqrt(r2);
-  double c[9];
+  const double sd = sqrt(r2);
   if (sw.kind == PM_SWITCH_CN) {
-    if (r2 < sw.don2) { val = 1.0; return; }
-    double swd = sd - sw.dcut;
-    double p = 1.0;
-    for (int k = 0; k < sw.ord_int; ++k) p *= swd;
-    c[0] = p;
-    for (int i = 1; i < sw.ord_int + 2; ++i) c[i] = c[i - 1] * swd;
-    for (int i = 0; i < sw.ord_int + 1; ++i) {
-      val += kSwCoef[sw.ord_idx][i] * c[i + 1] * sw.irange[i];
-      der += kDSwCoef[sw.ord_idx][i] * c[i] * sw.irange[i];
+    if (r2 < sw.don2) {
+      val = 1.0;
+      return;
     }
+    cn_poly(sw, sd - sw.dcut, sw.irange, val, der);
     der /= sd;
     return;
   }
-  // CmpCnCn
   if (r2 > sw.doff2) {
-    double swd = sd - sw.dcut;
-    double p = 1.0;
-    for (int k = 0; k < sw.ord_int; ++k) p *= swd;
-    c[0] = p;
-    for (int i = 1; i < sw.ord_int + 2; ++i) c[i] = c[i - 1] * swd;
-    for (int i = 0; i < sw.ord_int + 1; ++i) {
-      val += kSwCoef[sw.ord_idx][i] * c[i + 1] * sw.itrange[i];
-      der += kDSwCoef[sw.ord_idx][i] * c[i] * sw.itrange[i];
-    }
+    cn_poly(sw, sd - sw.dcut, sw.itrange, val, der);
     der /= sd;
   } else if (r2 >= sw.don2) {
-    double swd = sd - sw.doff;
-    double p = 1.0;
-    for (int k = 0; k < sw.ord_int; ++k) p *= swd;
-    c[0] = p;
-    for (int i = 1; i < sw.ord_int + 2; ++i) c[i] = c[i - 1] * swd;
-    double v = 0.0;
-    for (int i = 0; i < sw.ord_int + 1; ++i) {
-      v += kSwCoef[sw.ord_idx][i] * c[i + 1] * sw.irange[i];
-      der -= kDSwCoef[sw.ord_idx][i] * c[i] * sw.irange[i];
-    }
+    double v, d;
+    cn_poly(sw, sd - sw.doff, sw.irange, v, d);
     val = 1.0 - v;
-    der /= sd;
+    der = -d / sd;
   }
 }
 
@@ -743,24 +733,6 @@ __global__ __launch_bounds__(256) void k_pair(SysDev *sys, const int *clp, int n
 
 // ------------------------------------------------------- FP64 side terms
 
-// Numerical Recipes erfc (Press et al., "erfcc", fractional error < 1.2e-7):
-// the approximation ProtoMol's myerf uses (MathUtilities.cpp:12-36)
-__device__ __forceinline__ double nr_erf(double x) {
-  double erfcc = 0.0;
-  if (x < 9.0) {
-    double z = fabs(x);
-    double t = 1.0 / (1.0 + 0.5 * z);
-    erfcc = t * exp(-z * z - 1.26551223 +
-                    t * (1.00002368 +
-                         t * (.37409196 +
-                              t * (.09678418 +
-                                   t * (-.18628806 +
-                                        t * (.27886807 + t * (-1.13520398 + t * (1.48851587 + t * (-.82215223 + t * .17087277)))))))));
-    if (x < 0.0) erfcc = 2.0 - erfcc;
-  }
-  return 1.0 - erfcc;
-}
-
 __device__ __forceinline__ void min_image_vec(const SysDev &S, double *d) {
   double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
   if (!S.periodic || r2 <= S.D) return;
@@ -798,7 +770,7 @@ __global__ void k_ewald_corr(SysDev *sys, const int *xp, int nsys, ForceDev P, i
     double qq = S.charge[a1] * S.charge[a2];
     if (cls == PM_EXCL_MODIFIED) qq *= 1.0 - (double)S.csf;
     const double r = sqrt(r2), rr = 1.0 / r;
-    const double e = nr_erf(P.dalpha * r) * rr;
+    const double e = erf(P.dalpha * r) * rr;  // C-library erf, as the reference on Linux
     eacc = -qq * e;
     const double fs = qq * (P.dtwo_alpha_pi * exp(-(P.dalpha * P.dalpha) * r2) - e) * rr * rr;
     const double f[3] = {d[0] * fs, d[1] * fs, d[2] * fs};
@@ -961,8 +933,10 @@ __global__ void k_zero_forces(SysDev *sys, const int *slp, int nsys) {
   S.fz[k] = 0;
 }
 
-// per-system scalars: accumulators, census, displacement
-__global__ void k_clear_scalars(SysDev *sys, int nsys, int clear_disp) {
+// per-system scalars: accumulators, census, displacement.  disp: 0 keep,
+// 1 fold the segment's displacement into the run maximum and restart it,
+// 2 reset both (start of a run)
+__global__ void k_clear_scalars(SysDev *sys, int nsys, int disp) {
   const int s = blockIdx.x;
   if (s >= nsys) return;
   SysDev &S = sys[s];
@@ -970,7 +944,13 @@ __global__ void k_clear_scalars(SysDev *sys, int nsys, int clear_disp) {
   if (threadIdx.x == 0) {
     S.census[0] = 0;
     S.census[1] = 0;
-    if (clear_disp) *S.maxdisp2 = 0u;
+    if (disp == 1) {
+      *S.maxdisp_run = max(*S.maxdisp_run, *S.maxdisp2);
+      *S.maxdisp2 = 0u;
+    } else if (disp == 2) {
+      *S.maxdisp_run = 0u;
+      *S.maxdisp2 = 0u;
+    }
   }
 }
 
@@ -1055,8 +1035,8 @@ void launch_half_kick(const Launch &L, double hh) {
 void launch_zero_forces(const Launch &L) {
   if (L.n_slots_total) hipLaunchKernelGGL(k_zero_forces, dim3(nblk(L.n_slots_total, 256)), dim3(256), 0, L.stream, L.sys, L.slot_prefix, L.nsys);
 }
-void launch_clear_scalars(const Launch &L, bool clear_disp) {
-  hipLaunchKernelGGL(k_clear_scalars, dim3(L.nsys), dim3(64), 0, L.stream, L.sys, L.nsys, clear_disp ? 1 : 0);
+void launch_clear_scalars(const Launch &L, int disp) {
+  hipLaunchKernelGGL(k_clear_scalars, dim3(L.nsys), dim3(64), 0, L.stream, L.sys, L.nsys, disp);
 }
 void launch_export_f(const Launch &L, double *out) {
   if (L.n_atoms_total) hipLaunchKernelGGL(k_export_f, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys, out);

@@ -1,0 +1,251 @@
+"""Synthetic systems of the benchmark configurations, generated from seeds.
+
+A ``System`` holds what ProtoMol's ``GenericTopology`` holds for the nonbonded
+path (protomol/topology/GenericTopology.h:65-210): per-atom scaled charge,
+mass, LJ type or per-atom sigma/epsilon, molecule id, the exclusion table in
+``ExclusionTable::add`` order, harmonic bonds/angles, GB parameters and an
+orthogonal periodic box.  Units are ProtoMol's: Angstrom, kcal/mol, amu,
+charges scaled by sqrt(Coulomb constant) (protomol/base/PMConstants.cpp:67,
+protomol/topology/BuildTopology.cpp:92).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+SQRT_COULOMB = 18.2226123264      # PMConstants.cpp:67
+BOLTZMANN = 0.001987191           # PMConstants.cpp:70
+INV_TIMEFACTOR = 0.02045482828087295  # PMConstants.cpp:54
+
+EXCL_FULL = 1
+EXCL_MODIFIED = 2
+
+# three-site water: O-H 0.9572 A, H-O-H 104.52 deg, charges -0.834/+0.417,
+# LJ on O only (sigma 3.1507 A, epsilon 0.1521 kcal/mol)
+WATER_ROH = 0.9572
+WATER_THETA = math.radians(104.52)
+WATER_Q = (-0.834, 0.417, 0.417)
+WATER_MASS = (15.9994, 1.008, 1.008)
+WATER_SIGMA = 3.1507
+WATER_EPS = 0.1521
+# intramolecular model: the CHARMM TIP3 bond/angle constants of the
+# reference's water test (par_all27_prot_lipid.inp: OT-HT 450/0.9572, HT-OT-HT 55/104.52)
+WATER_KB = 450.0
+WATER_KA = 55.0
+
+
+@dataclasses.dataclass
+class System:
+    x: np.ndarray                  # (n, 3) float64
+    charge: np.ndarray             # (n,) scaled charges
+    mass: np.ndarray               # (n,)
+    molecule: np.ndarray           # (n,) int32
+    box: np.ndarray                # (3,) orthogonal edges (0 for vacuum)
+    origin: np.ndarray             # (3,)
+    periodic: bool = True
+    type: np.ndarray | None = None           # (n,) int32, table mode
+    lj_table: np.ndarray | None = None       # (T, T, 4) A, B, A14, B14
+    lj_sigma: np.ndarray | None = None       # (n,) per-atom mode
+    lj_epsilon: np.ndarray | None = None     # (n,)
+    lj_scaling: float = 1.0
+    exclusions: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros((0, 3), np.int32))
+    coulomb_scaling: float = 1.0
+    bonds: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros((0, 2), np.int32))
+    bond_k: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0))
+    bond_r0: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0))
+    angles: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros((0, 3), np.int32))
+    angle_k: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0))
+    angle_t0: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0))
+    gb_radius: np.ndarray | None = None
+    gb_scaling: np.ndarray | None = None
+    name: str = ""
+
+    @property
+    def n(self) -> int:
+        return int(self.x.shape[0])
+
+    def lj_table_full(self) -> np.ndarray:
+        """A T x T x 4 table; per-atom parameters become one type per atom
+        with GROMACS mixing (BuildTopology.cpp:650-659)."""
+        if self.lj_table is not None:
+            return self.lj_table
+        s, e = self.lj_sigma, self.lj_epsilon
+        sij = 0.5 * (s[:, None] + s[None, :])
+        eij = np.sqrt(e[:, None] * e[None, :])
+        A = sij ** 12 * eij * 4.0
+        B = sij ** 6 * eij * 4.0
+        return np.stack([A, B, self.lj_scaling * A, self.lj_scaling * B], axis=-1)
+
+    def lj_types(self) -> np.ndarray:
+        if self.type is not None:
+            return self.type
+        return np.arange(self.n, dtype=np.int32)
+
+
+def lj_gromacs(sigma: float, eps: float) -> tuple[float, float]:
+    """A = 4 eps sigma^12, B = 4 eps sigma^6 (BuildTopology.cpp:650-659)."""
+    return 4.0 * eps * sigma ** 12, 4.0 * eps * sigma ** 6
+
+
+def _random_rotations(rng: np.random.Generator, m: int) -> np.ndarray:
+    # uniform rotations from unit quaternions
+    q = rng.normal(size=(m, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    return np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], -1),
+        np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], -1),
+        np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], -1),
+    ], 1)
+
+
+def water_box(n_mol: int, box: float, seed: int, origin=(0.0, 0.0, 0.0), jitter: float = 0.0) -> System:
+    """``n_mol`` rigid-geometry 3-site waters on a cubic lattice in a cubic box.
+
+    The lattice has m = ceil(n_mol^(1/3)) sites per edge (spacing box/m); when
+    n_mol < m^3 a seeded subset of sites is used.  Each molecule gets a seeded
+    random orientation.  Atoms are ordered O, H, H per molecule.
+    """
+    rng = np.random.default_rng(seed)
+    m = int(math.ceil(round(n_mol ** (1.0 / 3.0), 9)))
+    while m ** 3 < n_mol:
+        m += 1
+    a = box / m
+    sites = np.stack(np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij"), -1).reshape(-1, 3)
+    if n_mol < m ** 3:
+        keep = np.sort(rng.choice(m ** 3, size=n_mol, replace=False))
+        sites = sites[keep]
+    org = np.asarray(origin, float)
+    centers = (sites + 0.5) * a - 0.5 * box + org
+    if jitter:
+        centers = centers + rng.uniform(-jitter, jitter, size=centers.shape)
+    half = 0.5 * WATER_THETA
+    local = np.array([[0.0, 0.0, 0.0],
+                      [WATER_ROH * math.sin(half), 0.0, WATER_ROH * math.cos(half)],
+                      [-WATER_ROH * math.sin(half), 0.0, WATER_ROH * math.cos(half)]])
+    R = _random_rotations(rng, n_mol)
+    xyz = centers[:, None, :] + np.einsum("mij,aj->mai", R, local)
+    n = 3 * n_mol
+    x = xyz.reshape(n, 3).astype(np.float64)
+    charge = np.tile(np.array(WATER_Q) * SQRT_COULOMB, n_mol)
+    mass = np.tile(np.array(WATER_MASS), n_mol)
+    typ = np.tile(np.array([0, 1, 1], np.int32), n_mol)
+    mol = np.repeat(np.arange(n_mol, dtype=np.int32), 3)
+    A, B = lj_gromacs(WATER_SIGMA, WATER_EPS)
+    table = np.zeros((2, 2, 4))
+    table[0, 0] = (A, B, A, B)
+    base = 3 * np.arange(n_mol, dtype=np.int32)
+    # 1-2 (O-H1, O-H2) and 1-3 (H1-H2) exclusions in buildExclusionTable order:
+    # bonds first, then angle ends (BuildTopology.cpp:1119-1131)
+    ex = np.concatenate([
+        np.stack([base, base + 1, np.full(n_mol, EXCL_FULL)], 1),
+        np.stack([base, base + 2, np.full(n_mol, EXCL_FULL)], 1),
+        np.stack([base + 1, base + 2, np.full(n_mol, EXCL_FULL)], 1),
+    ]).astype(np.int32)
+    order = np.argsort(np.concatenate([2 * np.arange(n_mol), 2 * np.arange(n_mol) + 1, 2 * n_mol + np.arange(n_mol)]),
+                       kind="stable")
+    ex = ex[order]
+    bonds = np.stack([np.stack([base, base + 1], 1), np.stack([base, base + 2], 1)], 1).reshape(-1, 2).astype(np.int32)
+    angles = np.stack([base + 1, base, base + 2], 1).astype(np.int32)
+    return System(x=x, charge=charge, mass=mass, molecule=mol, box=np.full(3, float(box)), origin=org.copy(),
+                  periodic=True, type=typ, lj_table=table, exclusions=ex,
+                  bonds=bonds, bond_k=np.full(len(bonds), WATER_KB), bond_r0=np.full(len(bonds), WATER_ROH),
+                  angles=angles, angle_k=np.full(n_mol, WATER_KA), angle_t0=np.full(n_mol, WATER_THETA),
+                  name=f"water{n_mol}")
+
+
+def water_box_density(n_mol: int, seed: int, density: float = 0.0334) -> System:
+    """Water box of ``n_mol`` molecules at ``density`` molecules/A^3."""
+    box = (n_mol / density) ** (1.0 / 3.0)
+    return water_box(n_mol, box, seed)
+
+
+def ensemble_sizes(count: int, lo: int, hi: int, seed: int) -> list[int]:
+    """Molecule counts drawn log-uniform in [lo, hi] by seed."""
+    rng = np.random.default_rng(seed)
+    return [int(math.floor(math.exp(v))) for v in rng.uniform(math.log(lo), math.log(hi), size=count)]
+
+
+_ELEMENTS = ("C", "N", "O", "H", "S", "P")
+_ELEMENT_MASS = {"C": 12.011, "N": 14.007, "O": 15.999, "H": 1.008, "S": 32.06, "P": 30.974}
+# GB scaling factor by the first letter of the atom type (BuildTopology.cpp:843-858)
+_ELEMENT_GBS = {"H": 0.85, "C": 0.72, "N": 0.79, "O": 0.85, "P": 0.86, "S": 0.96}
+
+
+def gb_chain(n_atoms: int, seed: int, bond: float = 1.53, min_dist: float = 2.5) -> System:
+    """Bead polymer in vacuum: seeded self-avoiding random walk of ``bond``-A
+    steps (non-bonded beads kept at least ``min_dist`` apart), charges
+    N(0, 0.3) e shifted to net zero, LJ sigma U[2.5, 4.0] A, epsilon
+    U[0.05, 0.2] kcal/mol (per-atom GROMACS mixing), intrinsic Born radii
+    U[1.2, 2.0] A, 1-2 and 1-3 exclusions along the chain."""
+    rng = np.random.default_rng(seed)
+    pos = np.zeros((n_atoms, 3))
+    cell = min_dist
+    grid: dict[tuple[int, int, int], list[int]] = {}
+
+    def key(p):
+        return (int(math.floor(p[0] / cell)), int(math.floor(p[1] / cell)), int(math.floor(p[2] / cell)))
+
+    def ok(p, upto):
+        k = key(p)
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    for j in grid.get((k[0] + dx, k[1] + dy, k[2] + dz), ()):
+                        if j < upto - 1 and np.sum((pos[j] - p) ** 2) < min_dist * min_dist:
+                            return False
+        return True
+
+    grid.setdefault(key(pos[0]), []).append(0)
+    i = 1
+    while i < n_atoms:
+        for _ in range(200):
+            v = rng.normal(size=3)
+            p = pos[i - 1] + bond * v / np.linalg.norm(v)
+            if ok(p, i):
+                break
+        else:
+            raise RuntimeError("self-avoiding walk got stuck")
+        pos[i] = p
+        grid.setdefault(key(p), []).append(i)
+        i += 1
+    pos -= pos.mean(axis=0)
+    q = rng.normal(0.0, 0.3, size=n_atoms)
+    q -= q.mean()
+    elem = rng.choice(len(_ELEMENTS), size=n_atoms, p=[0.35, 0.15, 0.15, 0.25, 0.05, 0.05])
+    mass = np.array([_ELEMENT_MASS[_ELEMENTS[e]] for e in elem])
+    gbs = np.array([_ELEMENT_GBS[_ELEMENTS[e]] for e in elem])
+    sigma = rng.uniform(2.5, 4.0, size=n_atoms)
+    eps = rng.uniform(0.05, 0.2, size=n_atoms)
+    rho = rng.uniform(1.2, 2.0, size=n_atoms)
+    idx = np.arange(n_atoms, dtype=np.int32)
+    bonds = np.stack([idx[:-1], idx[1:]], 1)
+    angles = np.stack([idx[:-2], idx[1:-1], idx[2:]], 1)
+    ex = np.concatenate([np.concatenate([bonds, np.full((len(bonds), 1), EXCL_FULL)], 1),
+                         np.concatenate([angles[:, [0, 2]], np.full((len(angles), 1), EXCL_FULL)], 1)]).astype(np.int32)
+    return System(x=pos, charge=q * SQRT_COULOMB, mass=mass, molecule=np.zeros(n_atoms, np.int32),
+                  box=np.zeros(3), origin=np.zeros(3), periodic=False, lj_sigma=sigma, lj_epsilon=eps,
+                  lj_scaling=1.0, exclusions=ex, bonds=bonds.astype(np.int32),
+                  gb_radius=rho, gb_scaling=gbs, name=f"gbchain{n_atoms}")
+
+
+def maxwell_velocities(sys: System, temperature: float, seed: int) -> np.ndarray:
+    """Seeded Maxwell velocities, sigma = sqrt(kB T / m) per component, in
+    ProtoMol velocity units (TopologyUtilities.cpp:21-48)."""
+    rng = np.random.default_rng(seed)
+    s = np.sqrt(temperature * BOLTZMANN / sys.mass)
+    return rng.normal(size=(sys.n, 3)) * s[:, None]
+
+
+def kinetic_energy(sys: System, v: np.ndarray) -> float:
+    """TopologyUtilities.cpp:163-171"""
+    return 0.5 * float(np.sum(sys.mass * np.sum(v * v, axis=1)))
+
+
+def temperature(sys: System, v: np.ndarray, dof: int | None = None) -> float:
+    """T = 2 KE / (kB * dof), dof = 3N - 3 (BuildTopology.cpp:121, TopologyUtilities.cpp:109-111)."""
+    if dof is None:
+        dof = 3 * sys.n - 3
+    return 2.0 * kinetic_energy(sys, v) / (BOLTZMANN * dof)

@@ -1,0 +1,200 @@
+"""Write the golden fixtures under tests/golden/.
+
+Run in a container that holds the ProtoMol sources (never on the GPU box):
+
+    make -C oracle lib ref && python tests/golden/make_golden.py
+
+Two kinds of fixtures:
+  ref_*.npz        inputs + outputs of the reference's own force classes
+                   (oracle/_ref/ref_driver, compiled from the reference
+                   sources) on seeded synthetic systems: forces, the 36
+                   ScalarStructure slots, the evaluated pair-set census and,
+                   for GB, the Born radii.
+  water423.npz     the data files of the reference's regression test
+                   test/tests/water_CHARMM_PERIODIC_L.conf: PSF/PAR/PDB
+                   inputs and the committed expected outputs (forces at step 0
+                   and 200, final positions/velocities, energies), converted
+                   to arrays.
+"""
+from __future__ import annotations
+
+import math
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import refio  # noqa: E402
+
+from protomol_amd import systems as S  # noqa: E402
+
+REFTESTS = "/root/reference/test/tests"
+
+# switch vectors for the driver: kind (1 C1, 2 C2, 3 Cn, 4 CmpCnCn), switchon, cutoff, switchoff, order
+CUTOFF_CASES = {
+    "c2c2": (1, 2, (2, 8.0, 10.0, 0.0, 0.0), (2, 8.0, 10.0, 0.0, 0.0)),
+    "c2c1": (1, 2, (2, 0.1, 10.0, 0.0, 0.0), (1, 0.0, 10.0, 0.0, 0.0)),
+    "cncn": (1, 2, (3, 8.0, 10.0, 0.0, 6.0), (3, 8.0, 10.0, 0.0, 6.0)),
+    "cmpcncn": (1, 2, (4, 7.0, 10.0, 9.0, 2.0), (4, 7.0, 10.0, 9.0, 2.0)),
+    "lj_c2": (1, 0, (2, 8.0, 10.0, 0.0, 0.0), None),
+    "coul_c1": (2, 0, (1, 0.0, 10.0, 0.0, 0.0), None),
+}
+
+
+def water_fixture_system():
+    return S.water_box(343, 22.5, seed=5)
+
+
+def make_cutoff(name, sysobj, case):
+    ta, tb, sa, sb = case
+    r = refio.system_records(sysobj)
+    r.update(mode=np.int32(1), term_a=np.int32(ta), term_b=np.int32(tb), sw_a=np.array(sa, float),
+             alg_cutoff=10.0, cellsize=5.0, virial=np.int32(1))
+    if sb is not None:
+        r["sw_b"] = np.array(sb, float)
+    out = refio.run(r)
+    np.savez_compressed(os.path.join(HERE, f"ref_cutoff_{name}.npz"), x=sysobj.x, f=out["f"].reshape(-1, 3),
+                        e=out["e"], census=np.array(out["census"], dtype=np.uint64),
+                        term_a=ta, term_b=tb, sw_a=np.array(sa, float),
+                        sw_b=np.array(sb if sb is not None else sa, float))
+
+
+def make_ewald(sysobj):
+    acc = math.exp(-10.24)
+    r = refio.system_records(sysobj)
+    r.update(mode=np.int32(2), ew_alpha=0.32, ew_accuracy=acc, ew_correction=np.int32(1), cellsize=5.0,
+             virial=np.int32(1))
+    out = refio.run(r)
+    np.savez_compressed(os.path.join(HERE, "ref_ewald.npz"), x=sysobj.x, f=out["f"].reshape(-1, 3), e=out["e"],
+                        census=np.array(out["census"], dtype=np.uint64), alpha=0.32, accuracy=acc)
+
+
+def make_bonded(sysobj):
+    r = refio.system_records(sysobj)
+    r.update(mode=np.int32(4), bonds=sysobj.bonds.reshape(-1), bond_k=sysobj.bond_k, bond_r0=sysobj.bond_r0,
+             angles=sysobj.angles.reshape(-1), angle_k=sysobj.angle_k, angle_t0=sysobj.angle_t0, cellsize=5.0)
+    out = refio.run(r)
+    np.savez_compressed(os.path.join(HERE, "ref_bonded.npz"), x=sysobj.x, f=out["f"].reshape(-1, 3), e=out["e"])
+
+
+def make_gb(n, seed):
+    g = S.gb_chain(n, seed=seed)
+    params = np.array([0.09, 1.0, 0.8, 4.85, 1.0, 78.3])
+    r = refio.system_records(g)
+    r.update(mode=np.int32(3), gb_rho=g.gb_radius, gb_S=g.gb_scaling, gb_params=params, cellsize=np.int32(5),
+             gb_ljcoul=np.int32(1))
+    out = refio.run(r)
+    np.savez_compressed(os.path.join(HERE, f"ref_gb{n}.npz"), x=g.x, f=out["f"].reshape(-1, 3), e=out["e"],
+                        born=out["born"], params=params, seed=seed)
+
+
+# ------------------------------------------------------------- water_423
+
+def _pdb_xyz(path):
+    out = []
+    for line in open(path):
+        if line.startswith("ATOM") or line.startswith("HETATM"):
+            out.append([float(line[30:38]), float(line[38:46]), float(line[46:54])])
+    return np.array(out)
+
+
+def _xyz_frames(path):
+    # ProtoMol XYZ output: an optional frame-count line, then per frame an
+    # atom-count line and "name x y z" rows (io/XYZWriter.cpp:41-57)
+    lines = [l for l in open(path).read().split("\n") if l.strip()]
+    frames, k = [], 0
+    while k < len(lines):
+        if len(lines[k].split()) == 1 and k + 1 < len(lines) and len(lines[k + 1].split()) == 1:
+            k += 1
+            continue
+        n = int(lines[k].split()[0])
+        rows = [[float(v) for v in lines[j].split()[1:4]] for j in range(k + 1, k + 1 + n)]
+        frames.append(np.array(rows))
+        k += 1 + n
+    return frames
+
+
+def make_water423():
+    psf = open(os.path.join(REFTESTS, "data/water_423.psf")).read().split("\n")
+    i = next(k for k, l in enumerate(psf) if "!NATOM" in l)
+    nat = int(psf[i].split()[0])
+    types, q, m, resid = [], [], [], []
+    for l in psf[i + 1:i + 1 + nat]:
+        p = l.split()
+        resid.append(int(p[2]))
+        types.append(p[5])
+        q.append(float(p[6]))
+        m.append(float(p[7]))
+
+    def section(tag, width):
+        j = next(k for k, l in enumerate(psf) if tag in l)
+        cnt = int(psf[j].split()[0])
+        vals = []
+        k = j + 1
+        while len(vals) < cnt * width:
+            vals += [int(v) - 1 for v in psf[k].split()]
+            k += 1
+        return np.array(vals, np.int32).reshape(cnt, width)
+
+    bonds = section("!NBOND", 2)
+    angles = section("!NTHETA", 3)
+    # CHARMM parameters for TIP3 (par_all27_prot_lipid.inp): nonbonded eps, Rmin/2;
+    # bond OT-HT 450 / 0.9572, angle HT-OT-HT 55 / 104.52 deg
+    par = {}
+    for l in open(os.path.join(REFTESTS, "data/par_all27_prot_lipid.inp"), errors="replace"):
+        p = l.split("!")[0].split()
+        if len(p) >= 4 and p[0] in ("OT", "HT") and len(p) in (4, 7):
+            try:
+                par[p[0]] = (abs(float(p[2])), float(p[3]))
+            except ValueError:
+                pass
+    tnames = ["OT", "HT"]
+    tidx = np.array([tnames.index(t) for t in types], np.int32)
+    table = np.zeros((2, 2, 4))
+    for a in range(2):
+        for b in range(2):
+            ea, ra = par[tnames[a]]
+            eb, rb = par[tnames[b]]
+            r = ra + rb
+            e = math.sqrt(ea * eb)
+            # CHARMM mixing (BuildTopology.cpp:664-672)
+            A, B = r ** 12 * e, 2 * r ** 6 * e
+            table[a, b] = (A, B, A, B)
+    x0 = _pdb_xyz(os.path.join(REFTESTS, "data/water_423.pos.pdb"))
+    v0 = _pdb_xyz(os.path.join(REFTESTS, "data/water_423.vel.pdb")) / 20.45482706  # PMConstants.cpp:71
+    fr = _xyz_frames(os.path.join(REFTESTS, "expected/water_CHARMM_PERIODIC_L.forces"))
+    pos = _xyz_frames(os.path.join(REFTESTS, "expected/water_CHARMM_PERIODIC_L.pos"))[0]
+    vel = _xyz_frames(os.path.join(REFTESTS, "expected/water_CHARMM_PERIODIC_L.vel"))[0]
+    en = np.array([[float(v) for v in l.split()] for l in
+                   open(os.path.join(REFTESTS, "expected/water_CHARMM_PERIODIC_L.energy")) if l.strip()])
+    # molecules = residues (BuildTopology assigns molecules from bonded connectivity)
+    _, mol = np.unique(np.array(resid), return_inverse=True)
+    np.savez_compressed(os.path.join(HERE, "water423.npz"), x0=x0, v0=v0, charge=np.array(q), mass=np.array(m),
+                        type=tidx, lj_table=table, molecule=mol.astype(np.int32), bonds=bonds, angles=angles,
+                        bond_k=np.full(len(bonds), 450.0), bond_r0=np.full(len(bonds), 0.9572),
+                        angle_k=np.full(len(angles), 55.0), angle_t0=np.full(len(angles), math.radians(104.52)),
+                        forces0=fr[0], forces200=fr[1], pos200=pos, vel200=vel, energy=en,
+                        box=np.array([28.0, 28.0, 28.0]), origin=np.zeros(3), dt=0.5, steps=200, cutoff=6.5,
+                        switchon=0.1, cellsize=6.5)
+
+
+def main():
+    w = water_fixture_system()
+    for name, case in CUTOFF_CASES.items():
+        make_cutoff(name, w, case)
+    make_ewald(w)
+    make_bonded(w)
+    make_gb(300, seed=7)
+    make_water423()
+    for f in sorted(os.listdir(HERE)):
+        if f.endswith(".npz"):
+            print(f, os.path.getsize(os.path.join(HERE, f)))
+
+
+if __name__ == "__main__":
+    main()
