@@ -67,6 +67,7 @@ extern "C" {
 #define PM_FLAG_VIRIAL 1u     /* ScalarStructure::virial() on */
 #define PM_FLAG_GB 2u         /* GBBornRadii -> GBPartialSum -> GBForce, all pairs, vacuum */
 #define PM_FLAG_BONDED 4u     /* harmonic Bond + Angle (force/bonded/{Bond,Angle}SystemForce.h) */
+#define PM_FLAG_TIMING 8u     /* time every pair-kernel launch with HIP events (pm_stats [4], [5]) */
 
 /* ScalarStructure slots (type/ScalarStructure.h:26-67); energies[] has 36 */
 #define PM_E_COULOMB 0
@@ -165,7 +166,8 @@ int pm_md_get_state(pm_ctx *ctx, double *x, double *v, double *f, double *e);
 /* Statistics of the last call: stats[0] pair-list rebuilds, [1] segments
  * re-run because an atom moved beyond skin/2, [2] pairs evaluated (last
  * evaluation, all systems), [3] device ms of the last call,
- * [4] pair-kernel ms (last evaluation), [5] pair-kernel launches. */
+ * [4] pair-kernel ms summed over the call's launches (PM_FLAG_TIMING, or the
+ * single launch of an evaluation), [5] pair-kernel launches timed. */
 int pm_stats(pm_ctx *ctx, double *stats, int32_t n);
 
 #ifdef __cplusplus

@@ -17,7 +17,7 @@ PM_OK, PM_EINVAL, PM_ENOMEM, PM_EDEVICE, PM_ECAPACITY, PM_ENOTREADY = 0, -1, -2,
 SWITCH_NONE, SWITCH_CUTOFF, SWITCH_C1, SWITCH_C2, SWITCH_CN, SWITCH_CMPCNCN = range(6)
 TERM_NONE, TERM_LJ, TERM_COULOMB, TERM_EWALD_REAL = range(4)
 LJ_TABLE, LJ_PERATOM = 0, 1
-FLAG_VIRIAL, FLAG_GB, FLAG_BONDED = 1, 2, 4
+FLAG_VIRIAL, FLAG_GB, FLAG_BONDED, FLAG_TIMING = 1, 2, 4, 8
 NSLOTS = 36
 
 EXPORTED = ("pm_abi_version", "pm_device_info", "pm_ctx_create", "pm_ctx_destroy", "pm_last_error",
@@ -111,6 +111,8 @@ class _Keep:
     """Holds the numpy arrays a SystemDesc points into."""
 
     def __init__(self, sys, bonded: bool):
+        if bonded and (len(sys.bond_k) != len(sys.bonds) or len(sys.angle_k) != len(sys.angles)):
+            raise ValueError(f"{sys.name}: bonded terms need a constant per bond and per angle")
         f64 = lambda a: None if a is None else np.ascontiguousarray(a, np.float64).reshape(-1)
         i32 = lambda a: None if a is None else np.ascontiguousarray(a, np.int32).reshape(-1)
         self.charge = f64(sys.charge)

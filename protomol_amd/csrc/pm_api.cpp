@@ -176,6 +176,8 @@ struct pm_ctx {
   bool md_ready = false;
   double stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evp0 = nullptr, evp1 = nullptr;
+  std::vector<hipEvent_t> evs;  // PM_FLAG_TIMING: start/stop per pair launch of an MD call
+  int evs_used = 0;
   Allocs common;
 };
 
@@ -497,12 +499,24 @@ void build_launches(pm_ctx *c) {
 }
 
 // one force evaluation of the systems in `L` at the positions already in S.x
+hipEvent_t next_event(pm_ctx *c) {
+  if (c->evs_used == (int)c->evs.size()) {
+    hipEvent_t e;
+    HIPCHECK(hipEventCreate(&e));
+    c->evs.push_back(e);
+  }
+  return c->evs[c->evs_used++];
+}
+
 void force_eval(pm_ctx *c, const Launch &L, bool rebuild, float rlist, bool energies, bool census, bool timed) {
   pm::launch_clear_scalars(L, rebuild ? 1 : 0);
+  const bool pool = !timed && (c->desc.flags & PM_FLAG_TIMING);
   if (c->gb) {
     pm::launch_zero_forces(L);
     if (timed) HIPCHECK(hipEventRecord(c->evp0, c->stream));
+    if (pool) HIPCHECK(hipEventRecord(next_event(c), c->stream));
     pm::launch_gb(L, c->P, energies);
+    if (pool) HIPCHECK(hipEventRecord(next_event(c), c->stream));
     if (timed) HIPCHECK(hipEventRecord(c->evp1, c->stream));
     pm::launch_bonded(L, energies);
     return;
@@ -512,7 +526,9 @@ void force_eval(pm_ctx *c, const Launch &L, bool rebuild, float rlist, bool ener
     pm::launch_rebuild(L, rlist * rlist);
   }
   if (timed) HIPCHECK(hipEventRecord(c->evp0, c->stream));
+  if (pool) HIPCHECK(hipEventRecord(next_event(c), c->stream));
   if (!pm::launch_pair(L, c->P, energies, census)) throw Fail{PM_EINVAL, "unsupported term combination"};
+  if (pool) HIPCHECK(hipEventRecord(next_event(c), c->stream));
   if (timed) HIPCHECK(hipEventRecord(c->evp1, c->stream));
   pm::launch_ewald_corr(L, c->P, energies);
   pm::launch_bonded(L, energies);
@@ -600,6 +616,8 @@ int run_guarded(pm_ctx *c, const std::function<void()> &fn) {
 // evaluate systems [s0, s1) through launch L at host positions x; add into f, e
 void evaluate_range(pm_ctx *c, const Launch &L, int s0, int s1, const double *x, double *f, double *e, bool census,
                     uint64_t *count, uint64_t *hash) {
+  // an evaluation overwrites the device positions: any MD state is gone
+  c->md_ready = false;
   int off = 0;
   for (int s = 0; s < s0; ++s) off += c->sys[s].n;
   int n = 0;
@@ -746,6 +764,7 @@ int pm_ctx_destroy(pm_ctx *c) {
   c->common.release();
   for (hipEvent_t e : {c->ev0, c->ev1, c->evp0, c->evp1})
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : c->evs) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return PM_OK;
@@ -815,6 +834,7 @@ int pm_md_run(pm_ctx *c, int32_t n_steps, double dt_fs, int32_t energies_every) 
     // snapshot of x, v for the (rare) re-run with a rebuild every step
     pm::launch_export_xv(c->L, c->d_xaos, c->d_vaos);
     HIPCHECK(hipEventRecord(c->ev0, c->stream));
+    c->evs_used = 0;
     int rebuilds = 0, reruns = 0;
     for (int pass = 0; pass < 2; ++pass) {
       const int every = pass == 0 ? K : 1;
@@ -850,9 +870,17 @@ int pm_md_run(pm_ctx *c, int32_t n_steps, double dt_fs, int32_t energies_every) 
     HIPCHECK(hipStreamSynchronize(c->stream));
     float ms = 0.f;
     HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    double pair_ms = 0.0;
+    for (int k = 0; k + 1 < c->evs_used; k += 2) {
+      float t = 0.f;
+      HIPCHECK(hipEventElapsedTime(&t, c->evs[k], c->evs[k + 1]));
+      pair_ms += t;
+    }
     c->stats[0] = rebuilds;
     c->stats[1] = reruns;
     c->stats[3] = ms;
+    c->stats[4] = pair_ms;
+    c->stats[5] = c->evs_used / 2;
   });
 }
 

@@ -1,0 +1,278 @@
+"""Parity of the HIP path (through the C ABI) with the oracle and the
+reference's golden vectors.
+
+Tolerances: the within-cutoff pair set is compared bit-exactly (census);
+forces as RMS relative error sqrt(sum|dF|^2 / sum|F|^2) <= 1e-5 against FP64;
+energies relative <= 1e-5; the virial relative to its largest component
+<= 1e-5.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+import pmtest as T
+from protomol_amd import _lib as L
+from protomol_amd import systems as S
+
+pytestmark = pytest.mark.gpu
+
+RTOL_F = 1e-5
+RTOL_E = 1e-5
+
+MODELS = {
+    # name: (term_a, switch_a, term_b, switch_b)
+    "c2c2": (L.TERM_LJ, (3, 8.0, 10.0, 0.0, 0.0), L.TERM_COULOMB, (3, 8.0, 10.0, 0.0, 0.0)),
+    "c2c1": (L.TERM_LJ, (3, 0.1, 10.0, 0.0, 0.0), L.TERM_COULOMB, (2, 0.0, 10.0, 0.0, 0.0)),
+    "cncn": (L.TERM_LJ, (4, 8.0, 10.0, 0.0, 6.0), L.TERM_COULOMB, (4, 8.0, 10.0, 0.0, 6.0)),
+    "cmpcncn": (L.TERM_LJ, (5, 7.0, 10.0, 9.0, 2.0), L.TERM_COULOMB, (5, 7.0, 10.0, 9.0, 2.0)),
+    "lj_c2": (L.TERM_LJ, (3, 8.0, 10.0, 0.0, 0.0), L.TERM_NONE, (0, 0, 0, 0, 0)),
+    "coul_c1": (L.TERM_COULOMB, (2, 0.0, 10.0, 0.0, 0.0), L.TERM_NONE, (0, 0, 0, 0, 0)),
+}
+
+
+def force_desc(term_a, sw_a, term_b, sw_b, flags=L.FLAG_VIRIAL, skin=2.0, interval=10, ewald=False,
+               cutoff=10.0):
+    d = L.ForceDesc()
+    d.term_a = term_a
+    d.switch_a = L.switch(*sw_a)
+    d.term_b = L.TERM_EWALD_REAL if ewald else term_b
+    d.switch_b = L.switch(*sw_b)
+    d.cutoff = cutoff
+    d.ewald_alpha = T.EWALD_ALPHA
+    d.ewald_accuracy = T.EWALD_ACCURACY
+    d.ewald_correction = 1
+    d.skin = skin
+    d.rebuild_interval = interval
+    d.flags = flags
+    return d
+
+
+def gb_desc(flags=0):
+    d = L.ForceDesc()
+    d.gb_offset, d.gb_alpha, d.gb_beta, d.gb_gamma = 0.09, 1.0, 0.8, 4.85
+    d.gb_solute_dielec, d.gb_solvent_dielec = 1.0, 78.3
+    d.flags = L.FLAG_GB | flags
+    return d
+
+
+def evaluate(ctx, sys_index, x):
+    n = x.shape[0]
+    f = np.zeros((n, 3))
+    e = np.zeros(L.NSLOTS)
+    ctx.evaluate(sys_index, x, f, e)
+    return f, e
+
+
+def check_energies(e, ref, slots=(0, 1)):
+    for s in slots:
+        assert abs(e[s] - ref[s]) <= RTOL_E * max(1.0, abs(ref[s])), (s, e[s], ref[s])
+
+
+def check_virial(e, ref):
+    v, r = e[7:16], ref[7:16]
+    assert np.max(np.abs(v - r)) <= RTOL_E * max(1.0, np.max(np.abs(r))), (v, r)
+
+
+@pytest.mark.parametrize("model", list(MODELS))
+def test_cutoff_vs_reference_golden(model):
+    print(f"case cutoff/{model}", flush=True)
+    g = T.golden(f"ref_cutoff_{model}.npz")
+    w = T.fixture_water()
+    ta, sa, tb, sb = MODELS[model]
+    with L.Context(force_desc(ta, sa, tb, sb), [w]) as ctx:
+        f, e = evaluate(ctx, 0, w.x)
+        cen = ctx.census(0, w.x)
+    assert cen == (int(g["census"][0]), int(g["census"][1]))
+    assert T.rms_rel(f, g["f"]) <= RTOL_F
+    check_energies(e, g["e"])
+    check_virial(e, g["e"])
+
+
+def test_ewald_vs_reference_golden():
+    print("case ewald", flush=True)
+    g = T.golden("ref_ewald.npz")
+    w = T.fixture_water()
+    with L.Context(force_desc(L.TERM_NONE, (0, 0, 0, 0, 0), 0, (0, 0, 0, 0, 0), ewald=True), [w]) as ctx:
+        f, e = evaluate(ctx, 0, w.x)
+        cen = ctx.census(0, w.x)
+    assert cen == (int(g["census"][0]), int(g["census"][1]))
+    assert T.rms_rel(f, g["f"]) <= RTOL_F
+    check_energies(e, g["e"], slots=(0,))
+    check_virial(e, g["e"])
+
+
+def test_lj_plus_ewald_fused_vs_oracle():
+    print("case lj+ewald fused", flush=True)
+    w = S.water_box(512, 25.0, seed=11)
+    fl, el, cl = O.nonbonded_cutoff(w, O.TERM_LJ, O.make_switch(O.SW_C2, 8.0, 10.0), cellsize=5.0, virial=True)
+    fe, ee, ce, _ = O.ewald_real(w, T.EWALD_ALPHA, T.EWALD_ACCURACY, cellsize=5.0, virial=True)
+    with L.Context(force_desc(L.TERM_LJ, T.C2_8_10, 0, (0, 0, 0, 0, 0), ewald=True), [w]) as ctx:
+        f, e = evaluate(ctx, 0, w.x)
+    assert T.rms_rel(f, fl + fe) <= RTOL_F
+    check_energies(e, el + ee)
+    check_virial(e, el + ee)
+
+
+def _probe_system():
+    """Water filler plus atom pairs placed at distances straddling the 10 A
+    cutoff by less than FP32 resolution."""
+    w = S.water_box(216, 30.0, seed=21)
+    offs = [10.0 - 1e-6, 10.0 - 1e-9, 10.0, 10.0 + 1e-12, 10.0 + 1e-9, 10.0 + 3e-7, 9.9999999, 10.0000001]
+    xs, n0 = [], w.n
+    for k, dd in enumerate(offs):
+        base = np.array([-13.0 + 1.7 * k, 13.7, 13.9])
+        u = np.array([0.0, 0.6, 0.8]) if k % 2 else np.array([1.0, 0.0, 0.0])
+        xs += [base, base - u * dd]
+    probes = np.array(xs)
+    x = np.concatenate([w.x, probes])
+    n = x.shape[0]
+    npb = len(probes)
+    sysp = S.System(x=x, charge=np.concatenate([w.charge, np.tile([0.3, -0.3], npb // 2) * S.SQRT_COULOMB]),
+                    mass=np.concatenate([w.mass, np.full(npb, 12.0)]),
+                    molecule=np.concatenate([w.molecule, w.molecule.max() + 1 + np.arange(npb)]).astype(np.int32),
+                    box=w.box, origin=w.origin, type=np.concatenate([w.type, np.zeros(npb, np.int32)]),
+                    lj_table=w.lj_table, exclusions=w.exclusions)
+    assert sysp.n == n
+    return sysp, n0
+
+
+@pytest.mark.parametrize("shift", [0.0, 30.0, -61.3])
+def test_census_bitexact_near_cutoff(shift):
+    """Pairs within FP32 rounding of the cutoff are decided on the FP64 image,
+    and atoms far outside the box still map to their minimal image."""
+    print(f"case census shift={shift}", flush=True)
+    sysp, n0 = _probe_system()
+    rng = np.random.default_rng(5)
+    x = sysp.x.copy()
+    x[:n0] += rng.integers(-2, 3, size=(n0, 3)) * 30.0  # whole-box images
+    x += shift
+    ref = O.census_bruteforce(sysp, 100.0, x=x)
+    ta, sa, tb, sb = MODELS["c2c2"]
+    with L.Context(force_desc(ta, sa, tb, sb), [sysp]) as ctx:
+        cen = ctx.census(0, x)
+    assert cen == ref
+
+
+def test_evaluate_is_deterministic_and_adds():
+    print("case determinism", flush=True)
+    w = T.fixture_water()
+    ta, sa, tb, sb = MODELS["c2c2"]
+    with L.Context(force_desc(ta, sa, tb, sb), [w]) as ctx:
+        f1, e1 = evaluate(ctx, 0, w.x)
+        f2, e2 = evaluate(ctx, 0, w.x)
+        f3 = f1.copy()
+        e3 = e1.copy()
+        ctx.evaluate(0, w.x, f3, e3)   # SystemForce::evaluate adds into its outputs
+    assert np.array_equal(f1, f2) and np.array_equal(e1, e2)
+    assert np.array_equal(f3, 2 * f1) and np.allclose(e3, 2 * e1, rtol=1e-15)
+
+
+def test_batch_alone_vs_mix_bitwise():
+    print("case batch alone-vs-mix", flush=True)
+    systems = [S.water_box(n, (n / 0.0334) ** (1 / 3), seed=30 + k) for k, n in enumerate((500, 777, 1300))]
+    ta, sa, tb, sb = MODELS["c2c2"]
+    desc = force_desc(ta, sa, tb, sb, ewald=True)
+    alone = []
+    for s in systems:
+        with L.Context(desc, [s]) as ctx:
+            alone.append(evaluate(ctx, 0, s.x))
+    with L.Context(desc, systems) as ctx:
+        x = np.concatenate([s.x for s in systems])
+        f = np.zeros_like(x)
+        e = np.zeros((len(systems), L.NSLOTS))
+        ctx.evaluate_batch(x, f, e)
+    off = 0
+    for k, s in enumerate(systems):
+        assert np.array_equal(f[off:off + s.n], alone[k][0])
+        assert np.array_equal(e[k], alone[k][1])
+        off += s.n
+
+
+def test_bonded_water_vs_oracle():
+    print("case bonded", flush=True)
+    w = T.fixture_water()
+    fb, eb = O.bonded(w)
+    fl, el, _ = O.nonbonded_cutoff(w, O.TERM_LJ, O.make_switch(O.SW_C2, 8.0, 10.0), cellsize=5.0)
+    with L.Context(force_desc(L.TERM_LJ, T.C2_8_10, 0, (0, 0, 0, 0, 0), flags=L.FLAG_BONDED), [w]) as ctx:
+        f, e = evaluate(ctx, 0, w.x)
+    assert T.rms_rel(f, fb + fl) <= RTOL_F
+    check_energies(e, eb + el, slots=(1, 2, 3))
+
+
+def test_gb_vs_reference_golden():
+    print("case gb", flush=True)
+    g = T.golden("ref_gb300.npz")
+    c = S.gb_chain(300, seed=int(g["seed"]))
+    with L.Context(gb_desc(), [c]) as ctx:
+        f, e = evaluate(ctx, 0, c.x)
+    assert T.rms_rel(f, g["f"]) <= RTOL_F
+    check_energies(e, g["e"])
+
+
+def test_gb_batch_vs_oracle():
+    print("case gb batch", flush=True)
+    chains = [S.gb_chain(n, seed=40 + n) for n in (257, 600, 1031)]
+    with L.Context(gb_desc(), chains) as ctx:
+        x = np.concatenate([c.x for c in chains])
+        f = np.zeros_like(x)
+        e = np.zeros((len(chains), L.NSLOTS))
+        ctx.evaluate_batch(x, f, e)
+    off = 0
+    for k, c in enumerate(chains):
+        fo, eo, _ = O.gb_evaluate(c)
+        from test_oracle import _vacuum_lj_coulomb
+        fl, el = _vacuum_lj_coulomb(c)
+        assert T.rms_rel(f[off:off + c.n], fo + fl) <= RTOL_F
+        assert abs(e[k][0] - (eo[0] + el[0])) <= RTOL_E * abs(eo[0] + el[0])
+        assert abs(e[k][1] - el[1]) <= RTOL_E * max(1.0, abs(el[1]))
+        off += c.n
+
+
+def test_md_water_vs_oracle_leapfrog():
+    """Device-resident leapfrog vs the oracle's FP64 leapfrog on the same start."""
+    print("case md", flush=True)
+    w = S.water_box(343, 22.5, seed=5)
+    v0 = S.maxwell_velocities(w, 300.0, seed=1)
+    sw = O.make_switch(O.SW_C2, 8.0, 10.0)
+
+    def forces(x):
+        f, e, _ = O.nonbonded_cutoff(w, O.TERM_LJ, sw, O.TERM_COULOMB, sw, x=x, cellsize=5.0)
+        fb, eb = O.bonded(w, x=x)
+        return f + fb, e + eb
+
+    steps, dt = 20, 0.5
+    xo, vo, Fo, Eo = T.leapfrog_oracle(forces, w.x.copy(), v0.copy(), w.mass, dt, steps)
+    ta, sa, tb, sb = MODELS["c2c2"]
+    with L.Context(force_desc(ta, sa, tb, sb, flags=L.FLAG_BONDED, skin=1.5, interval=5), [w]) as ctx:
+        ctx.md_set_state(w.x, v0)
+        ctx.md_run(steps, dt, energies=True)
+        x, v, F, E = ctx.md_get_state()
+        st = ctx.stats()
+    assert np.max(np.abs(x - xo)) < 1e-4
+    assert T.rms_rel(v, vo) < 1e-4
+    assert T.rms_rel(F, Fo) < 1e-4
+    for s in (0, 1, 2, 3):
+        assert abs(E[0][s] - Eo[s]) <= 1e-4 * max(1.0, abs(Eo[s]))
+    assert st["rebuilds"] >= steps // 5
+
+
+def test_water423_reference_regression_on_device():
+    """The reference's regression test on the HIP path: forces at step 0 and
+    the 200-step trajectory (Coulomb C1 + LJ C2 switchon 0.1, cutoff 6.5,
+    Bond + Angle, 0.5 fs)."""
+    print("case water423", flush=True)
+    sysw, d = T.water423_system()
+    desc = force_desc(L.TERM_LJ, (3, 0.1, 6.5, 0.0, 0.0), L.TERM_COULOMB, (2, 0.0, 6.5, 0.0, 0.0),
+                      flags=L.FLAG_BONDED, skin=1.0, interval=5, cutoff=6.5)
+    with L.Context(desc, [sysw]) as ctx:
+        f, e = evaluate(ctx, 0, d["x0"])
+        assert T.rms_rel(f, d["forces0"]) <= RTOL_F
+        en = d["energy"][0]
+        assert abs(e[0] - en[10]) < 1e-3 and abs(e[1] - en[9]) < 1e-3
+        ctx.md_set_state(d["x0"], d["v0"])
+        ctx.md_run(int(d["steps"]), float(d["dt"]), energies=True)
+        x, v, F, E = ctx.md_get_state()
+    assert np.max(np.abs(x - d["pos200"])) < 2e-3
+    assert T.rms_rel(F, d["forces200"]) < 1e-3
