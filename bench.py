@@ -89,7 +89,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--skin", type=float, default=2.0)
+    ap.add_argument("--skin", type=float, default=2.4)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -136,7 +136,8 @@ def main():
         "config": {"workload": "water 32,000 mol / 96,000 atoms, box 98.6 A, LJ C2 8-10 A + Ewald real "
                                "alpha 0.32 Rc 10 A + bond/angle, leapfrog dt 1 fs, pair list every 10 steps",
                    "atoms": system.n, "pairs_per_step": pairs, "skin_A": args.skin, "rebuild_interval": REBUILD,
-                   "rebuilds": st["rebuilds"], "reruns": st["reruns"]},
+                   "rebuilds": st["rebuilds"], "reruns": st["reruns"],
+                   "max_displacement_A": st["max_displacement"]},
         "ns_per_day": steps_s * DT_FS * 0.0864,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,

@@ -167,7 +167,9 @@ int pm_md_get_state(pm_ctx *ctx, double *x, double *v, double *f, double *e);
  * re-run because an atom moved beyond skin/2, [2] pairs evaluated (last
  * evaluation, all systems), [3] device ms of the last call,
  * [4] pair-kernel ms summed over the call's launches (PM_FLAG_TIMING, or the
- * single launch of an evaluation), [5] pair-kernel launches timed. */
+ * single launch of an evaluation), [5] pair-kernel launches timed,
+ * [6] largest displacement (A) of an atom from its position at the pair-list
+ * build during the last MD call, [7] the pair-list skin (A) in use. */
 int pm_stats(pm_ctx *ctx, double *stats, int32_t n);
 
 #ifdef __cplusplus

@@ -306,6 +306,8 @@ void setup_system(pm_ctx *c, SysHost &h, const pm_system_desc &d) {
     S.type = A.get<int>(n);
   } else {
     if (d.n_types < 1 || !d.lj_table || !d.type) throw Fail{PM_EINVAL, "LJ table mode needs n_types, type and lj_table"};
+    if (d.n_types > 16 && !c->gb)
+      throw Fail{PM_EINVAL, "the cell-list pair kernel keeps the LJ table in LDS: at most 16 types (use PM_LJ_PERATOM)"};
     for (int i = 0; i < n; ++i)
       if (d.type[i] < 0 || d.type[i] >= d.n_types) throw Fail{PM_EINVAL, "atom type out of range"};
     std::vector<float4> t((size_t)d.n_types * d.n_types);
@@ -734,6 +736,19 @@ int pm_ctx_create(const pm_force_desc *force, int32_t n_systems, const pm_system
       c->skin_md = force->skin > 0 ? force->skin : 2.0;
       c->skin_eval = 0.1;
     }
+    if (!c->gb) {
+      // the pair list radius rc + skin must stay below half the smallest box
+      // edge (one image per pair); a larger requested skin is reduced to fit
+      double minL = 1e300;
+      for (int s = 0; s < n_systems; ++s)
+        for (int k = 0; k < 3; ++k) minL = std::min(minL, systems[s].box[k]);
+      const double room = 0.5 * minL - c->rc - 0.01;
+      if (room < 0.05)
+        throw Fail{PM_EINVAL, "box edge " + std::to_string(minL) + " leaves no room for a pair list beyond rc " +
+                                  std::to_string(c->rc) + " (needs box > 2 rc)"};
+      c->skin_md = std::min(c->skin_md, room);
+      c->skin_eval = std::min(c->skin_eval, room);
+    }
     c->sys.resize(n_systems);
     for (int s = 0; s < n_systems; ++s) setup_system(c, c->sys[s], systems[s]);
     c->d_sys = c->common.get<SysDev>(n_systems);
@@ -836,6 +851,7 @@ int pm_md_run(pm_ctx *c, int32_t n_steps, double dt_fs, int32_t energies_every) 
     HIPCHECK(hipEventRecord(c->ev0, c->stream));
     c->evs_used = 0;
     int rebuilds = 0, reruns = 0;
+    double maxdisp = 0.0;
     for (int pass = 0; pass < 2; ++pass) {
       const int every = pass == 0 ? K : 1;
       pm::launch_clear_scalars(c->L, 2);
@@ -858,6 +874,7 @@ int pm_md_run(pm_ctx *c, int32_t n_steps, double dt_fs, int32_t energies_every) 
       for (unsigned u : md) worst = std::max(worst, u);
       float w;
       std::memcpy(&w, &worst, sizeof w);
+      if (pass == 0) maxdisp = std::sqrt((double)w);
       if (c->gb || every == 1 || std::sqrt((double)w) <= lim) break;
       // an atom moved further than skin/2 between rebuilds: restore the state
       // of the call's start and redo it with a rebuild before every evaluation
@@ -881,6 +898,8 @@ int pm_md_run(pm_ctx *c, int32_t n_steps, double dt_fs, int32_t energies_every) 
     c->stats[3] = ms;
     c->stats[4] = pair_ms;
     c->stats[5] = c->evs_used / 2;
+    c->stats[6] = maxdisp;
+    c->stats[7] = c->skin_md;
   });
 }
 

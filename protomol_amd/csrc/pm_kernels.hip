@@ -17,6 +17,26 @@ namespace pm {
 
 // ------------------------------------------------------------- helpers
 
+// Pointers read out of the SysDev table are generic; casting them to the
+// global (1) or constant (4) address space lets the compiler use global_* and,
+// for wave-uniform reads of data no kernel writes meanwhile, s_load_*.
+// (the host pass of hipcc sees kernel bodies too, without address spaces)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define PM_GLOBAL __attribute__((address_space(1)))
+#define PM_CONST __attribute__((address_space(4)))
+#else
+#define PM_GLOBAL
+#define PM_CONST
+#endif
+template <class T>
+__device__ __forceinline__ PM_GLOBAL T *gp(T *p) {
+  return (PM_GLOBAL T *)p;
+}
+template <class T>
+__device__ __forceinline__ const PM_CONST T *cp(const T *p) {
+  return (const PM_CONST T *)p;
+}
+
 __device__ __forceinline__ int find_sys(const int *__restrict__ prefix, int nsys, int w) {
   int lo = 0, hi = nsys;  // prefix[lo] <= w < prefix[hi]
   while (hi - lo > 1) {
@@ -34,6 +54,12 @@ __device__ __forceinline__ long long to_fixed(double v, double scale) {
 __device__ __forceinline__ void atomic_add_ll(long long *p, long long v) {
   atomicAdd(reinterpret_cast<unsigned long long *>(p), static_cast<unsigned long long>(v));
 }
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ void atomic_add_ll(PM_GLOBAL long long *p, long long v) {
+  __hip_atomic_fetch_add((PM_GLOBAL unsigned long long *)p, static_cast<unsigned long long>(v), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+}
+#endif
 
 __device__ __forceinline__ unsigned long long pair_hash(unsigned lo, unsigned hi) {
   unsigned long long z = ((unsigned long long)hi << 32) | lo;
@@ -67,7 +93,7 @@ __device__ __forceinline__ double wrap_rel(double x, double omin, double L) {
 // PeriodicBoundaryConditions::minimalDifference(c1, c2, r2), orthogonal branch
 // (PeriodicBoundaryConditions.h:72-110), evaluated without contraction so the
 // squared distance rounds exactly as the reference's FP64 arithmetic.
-__device__ __noinline__ double min_image_r2(const SysDev &S, int a, int b) {
+__device__ __forceinline__ double min_image_r2(const SysDev &S, int a, int b) {
 #pragma clang fp contract(off)
   const int n = S.n_atoms;
   double d0 = S.x[b] - S.x[a];
@@ -214,27 +240,45 @@ __global__ void k_import_v(SysDev *sys, const int *ap, int nsys, const double *v
   S.v[2 * n + a] = vaos[3 * (size_t)g + 2];
 }
 
+// Lanes of a wave that share a (system, column) key are served by one atomic:
+// the wave peels off the group of the first remaining lane until every lane is
+// done.  Atoms in original order are spatially coherent, so a wave usually
+// holds only a few keys.
 __global__ void k_bin(SysDev *sys, const int *ap, int nsys) {
-  int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= ap[nsys]) return;
-  int s = find_sys(ap, nsys, g);
-  SysDev &S = sys[s];
-  int a = g - ap[s], n = S.n_atoms;
-  double x = S.x[a], y = S.x[n + a], z = S.x[2 * n + a];
-  if (!(isfinite(x) && isfinite(y) && isfinite(z))) {
-    atomicOr(S.err, (unsigned)kErrNonFinite);
-    x = y = z = 0.0;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  bool todo = g < ap[nsys];
+  int s = 0, col = 0;
+  if (todo) {
+    s = find_sys(ap, nsys, g);
+    SysDev &S = sys[s];
+    const int a = g - ap[s], n = S.n_atoms;
+    double x = S.x[a], y = S.x[n + a], z = S.x[2 * n + a];
+    if (!(isfinite(x) && isfinite(y) && isfinite(z))) {
+      atomicOr(S.err, (unsigned)kErrNonFinite);
+      x = y = z = 0.0;
+    }
+    const double rx = wrap_rel(x, S.omin[0], S.L[0]);
+    const double ry = wrap_rel(y, S.omin[1], S.L[1]);
+    const double rz = wrap_rel(z, S.omin[2], S.L[2]);
+    int cx = (int)(rx / S.ax), cy = (int)(ry / S.ay);
+    cx = cx < 0 ? 0 : (cx >= S.ncx ? S.ncx - 1 : cx);
+    cy = cy < 0 ? 0 : (cy >= S.ncy ? S.ncy - 1 : cy);
+    col = cy * S.ncx + cx;
+    S.bin_col[a] = col;
+    S.bin_z[a] = (float)rz;
   }
-  double rx = wrap_rel(x, S.omin[0], S.L[0]);
-  double ry = wrap_rel(y, S.omin[1], S.L[1]);
-  double rz = wrap_rel(z, S.omin[2], S.L[2]);
-  int cx = (int)(rx / S.ax), cy = (int)(ry / S.ay);
-  cx = cx < 0 ? 0 : (cx >= S.ncx ? S.ncx - 1 : cx);
-  cy = cy < 0 ? 0 : (cy >= S.ncy ? S.ncy - 1 : cy);
-  int col = cy * S.ncx + cx;
-  S.bin_col[a] = col;
-  S.bin_z[a] = (float)rz;
-  atomicAdd(&S.col_count[col], 1);
+  // one atomic per (system, column) group of the wave
+  while (true) {
+    const unsigned long long rem = wave_ballot(todo);
+    if (rem == 0) break;
+    const int leader = __ffsll((long long)rem) - 1;
+    const int ls = __shfl(s, leader), lc = __shfl(col, leader);
+    const bool mine = todo && s == ls && col == lc;
+    const unsigned long long grp = wave_ballot(mine);
+    if (lane == leader) atomicAdd(&sys[ls].col_count[lc], __popcll(grp));
+    todo = todo && !mine;
+  }
 }
 
 // one block per system: exclusive scans of atoms and clusters over columns
@@ -283,15 +327,33 @@ __global__ __launch_bounds__(1024) void k_col_scan(SysDev *sys) {
 }
 
 __global__ void k_col_fill(SysDev *sys, const int *ap, int nsys) {
-  int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= ap[nsys]) return;
-  int s = find_sys(ap, nsys, g);
-  SysDev &S = sys[s];
-  int a = g - ap[s];
-  int col = S.bin_col[a];
-  int pos = atomicAdd(&S.col_cursor[col], 1);
-  unsigned zb = __float_as_uint(S.bin_z[a]);  // z >= 0: bit order == value order
-  S.colbuf[S.col_atom_start[col] + pos] = ((unsigned long long)zb << 32) | (unsigned)a;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  bool todo = g < ap[nsys];
+  int s = 0, col = 0, a = 0;
+  if (todo) {
+    s = find_sys(ap, nsys, g);
+    a = g - ap[s];
+    col = sys[s].bin_col[a];
+  }
+  while (true) {
+    const unsigned long long rem = wave_ballot(todo);
+    if (rem == 0) break;
+    const int leader = __ffsll((long long)rem) - 1;
+    const int ls = __shfl(s, leader), lc = __shfl(col, leader);
+    const bool mine = todo && s == ls && col == lc;
+    const unsigned long long grp = wave_ballot(mine);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&sys[ls].col_cursor[lc], __popcll(grp));
+    base = __shfl(base, leader);
+    if (mine) {
+      SysDev &S = sys[s];
+      const int pos = base + __popcll(grp & ((1ull << lane) - 1ull));
+      const unsigned zb = __float_as_uint(S.bin_z[a]);  // z >= 0: bit order == value order
+      S.colbuf[S.col_atom_start[col] + pos] = ((unsigned long long)zb << 32) | (unsigned)a;
+    }
+    todo = todo && !mine;
+  }
 }
 
 // one block per column: bitonic sort by (z, atom) and slot assignment
@@ -514,51 +576,111 @@ __global__ __launch_bounds__(256) void k_pairlist(SysDev *sys, const int *clp, i
 
 // ------------------------------------------------------------ pair force
 
-// One wave per i-cluster.  Lane = (ii, jj): i atom ii of the i-cluster against
-// j atom jj of the current j-cluster, so one 8x8 cluster pair per step.
-// FP32 arithmetic on cluster-local coordinates; pairs within `guard` of a
-// cutoff are decided on the reference's FP64 minimal image instead, so the
-// within-cutoff pair set is exactly the FP64 one.  i-forces accumulate in
-// registers, j-forces reduce across the wave (xor shuffles over the 8 i lanes)
-// and both land in 64-bit fixed point, so the sum is order independent.
-template <int TA, int TB, bool EN, bool CEN>
-__global__ __launch_bounds__(256) void k_pair(SysDev *sys, const int *clp, int nsys, ForceDev P) {
+// compile-time switch kinds for the hot path; kSwGeneric dispatches at run time
+constexpr int kSwGeneric = -1;
+
+template <int SW>
+__device__ __forceinline__ void apply_switch_t(const SwitchDev &sw, float r2, float &E, float &F) {
+  if (SW == PM_SWITCH_C2) {  // C2SwitchingFunction.h:42-53
+    const float c2 = sw.cut2 - r2;
+    const float c4 = c2 * (sw.s2 + 2.0f * r2);
+    float S = sw.s1 * (c2 * c4);
+    float dS = sw.s3 * (c2 * c2 - c4);
+    const bool inner = r2 < sw.on2;
+    const bool outer = r2 > sw.cut2;  // a shorter switch inside OneAtomPairTwo's larger cutoff
+    S = inner ? 1.0f : (outer ? 0.0f : S);
+    dS = (inner || outer) ? 0.0f : dS;
+    F = F * S - E * dS;
+    E = E * S;
+  } else if (SW == PM_SWITCH_CUTOFF || SW == PM_SWITCH_NONE) {
+    // inside the inclusion cutoff the value is 1 (CutoffSwitchingFunction.h:41-44)
+  } else {
+    apply_switch(sw, r2, E, F);
+  }
+}
+
+// sum over 8 consecutive lanes with DPP (quad swaps, then half-row mirror);
+// every lane of the group ends with the same bits
+__device__ __forceinline__ float dpp_sum8(float v) {
+  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));
+  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141, 0xF, 0xF, false));
+  return v;
+}
+
+constexpr int kLJLds = 256;  // LJ table entries kept in LDS per wave (T <= 16)
+
+// One wave per i-cluster.  Lane = (jj, ii) = (lane >> 3, lane & 7): i atom ii
+// of the i-cluster against j atom jj of the current j-cluster, so one 8x8
+// cluster pair per step.  FP32 arithmetic on cluster-local coordinates; pairs
+// within `guard` of a cutoff are decided on the reference's FP64 minimal image
+// instead, so the within-cutoff pair set is exactly the FP64 one.  i-forces
+// accumulate in registers; j-forces are summed over the 8 i lanes with DPP and
+// land, like the i-forces, in 64-bit fixed point, so the total is independent
+// of the order in which waves finish.
+template <int TA, int TB, int SWA, int SWB, bool EN, bool CEN>
+__global__ __launch_bounds__(256) void k_pair(const SysDev *__restrict__ sys, const int *__restrict__ clp, int nsys,
+                                              ForceDev P) {
+  __shared__ float4 ljs[4][kLJLds];
   const int lane = threadIdx.x & 63;
-  const int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int wv = threadIdx.x >> 6;
+  const int w = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   if (w >= clp[nsys]) return;
-  const int s = find_sys(clp, nsys, w);
-  const SysDev &S = sys[s];
+  const int s = __builtin_amdgcn_readfirstlane(find_sys(clp, nsys, w));
+  const SysDev *__restrict__ S = sys + s;
   const int ci = w - clp[s];
-  if (ci >= *S.n_clusters) return;
-  const int ii = lane >> 3, jj = lane & 7;
+  if (ci >= *cp(S->n_clusters)) return;
+  const PM_GLOBAL float4 *xq = gp(S->xq);
+  const PM_GLOBAL int *ljt = gp(S->ljt);
+  const PM_GLOBAL float2 *ljp = gp(S->ljp);
+  const PM_GLOBAL int *orig = gp(S->orig);
+  PM_GLOBAL long long *gfx = gp(S->fx);
+  PM_GLOBAL long long *gfy = gp(S->fy);
+  PM_GLOBAL long long *gfz = gp(S->fz);
+  const int ii = lane & 7, jj = lane >> 3;
   const int si = ci * kCluster + ii;
-  const bool vi = ii < S.cl_nvalid[ci];
-  const float4 xi = S.xq[si];
-  const bool peratom = S.lj_mode == PM_LJ_PERATOM;
-  const int T = S.n_types;
-  int ti = 0;
+  const bool vi = ii < *cp(S->cl_nvalid + ci);
+  const float4 xi = xq[si];
+  const bool peratom = S->lj_mode == PM_LJ_PERATOM;
+  const int T = S->n_types;
+  const bool tab_lds = TA == PM_TERM_LJ && !peratom;  // T <= 16 (checked at context creation)
+  const PM_GLOBAL float4 *gtab = gp(S->lj_table);
+  if (tab_lds) {
+    for (int k = lane; k < T * T; k += kWave) ljs[wv][k] = gtab[k];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  int tiT = 0;
   float2 pi = make_float2(0.f, 0.f);
   if (TA == PM_TERM_LJ) {
-    if (peratom) pi = S.ljp[si];
-    else ti = S.ljt[si];
+    if (peratom) pi = ljp[si];
+    else tiT = ljt[si] * T;
   }
+  const float csf = S->csf, ljsc = S->lj_scaling;
   int ai = 0;
-  if (CEN) ai = S.orig[si];
+  if (CEN) ai = orig[si];
   float fix = 0.f, fiy = 0.f, fiz = 0.f;
   double eA = 0.0, eB = 0.0;
   double vxx = 0.0, vxy = 0.0, vxz = 0.0, vyy = 0.0, vyz = 0.0, vzz = 0.0;
   unsigned long long ccount = 0, chash = 0;
-  const int len = S.list_len[ci];
-  const uint4 *L = S.list + (size_t)ci * S.list_cap;
-  const unsigned long long *M = S.list_mask + 2 * (size_t)ci * S.list_cap;
+  const int len = *cp(S->list_len + ci);
+  const PM_CONST uint4 *Lst = cp(S->list + (size_t)ci * S->list_cap);
+  const PM_CONST unsigned long long *M = cp(S->list_mask + 2 * (size_t)ci * S->list_cap);
   for (int e = 0; e < len; ++e) {
-    const uint4 ent = L[e];
+    const uint4 ent = Lst[e];
     const unsigned w0 = ent.x;
     const int cj = (int)(w0 & 0x7FFFFFu);
     const int sh = (int)((w0 >> 23) & 31u);
     const int nvj = (int)((w0 >> 28) & 7u) + 1;
     const int sj = cj * kCluster + jj;
-    const float4 xj = S.xq[sj];
+    const float4 xj = xq[sj];
+    int tj = 0;
+    float2 pj = make_float2(0.f, 0.f);
+    if (TA == PM_TERM_LJ) {
+      if (peratom) pj = ljp[sj];
+      else tj = ljt[sj];
+    }
     const float dx = (xj.x - xi.x) + __uint_as_float(ent.y);
     const float dy = (xj.y - xi.y) + __uint_as_float(ent.z);
     const float dz = (xj.z - xi.z) + __uint_as_float(ent.w);
@@ -568,15 +690,16 @@ __global__ __launch_bounds__(256) void k_pair(SysDev *sys, const int *clp, int n
     bool mod = false;
     if (w0 & 0x80000000u) {
       const unsigned long long mf = M[2 * e], mm = M[2 * e + 1];
-      act = act && !((mf >> lane) & 1ull);
-      mod = (mm >> lane) & 1ull;
+      // masks are indexed by (i slot, j slot) as bit ii * 8 + jj
+      act = act && !((mf >> (ii * 8 + jj)) & 1ull);
+      mod = (mm >> (ii * 8 + jj)) & 1ull;
     }
     bool inA = (TA != PM_TERM_NONE) && r2 <= P.cutA2;
     bool inB = (TB != PM_TERM_NONE) && r2 <= P.cutB2;
     const bool nearA = (TA != PM_TERM_NONE) && fabsf(r2 - P.cutA2) <= P.guard;
     const bool nearB = (TB != PM_TERM_NONE) && fabsf(r2 - P.cutB2) <= P.guard;
-    if (act && (nearA || nearB)) {
-      const double r2d = min_image_r2(S, S.orig[si], S.orig[sj]);
+    if (__builtin_expect(act && (nearA || nearB), 0)) {
+      const double r2d = min_image_r2(*S, orig[si], orig[sj]);
       inA = (TA != PM_TERM_NONE) && r2d <= P.cutA2d;
       inB = (TB != PM_TERM_NONE) && r2d <= P.cutB2d;
     }
@@ -584,24 +707,23 @@ __global__ __launch_bounds__(256) void k_pair(SysDev *sys, const int *clp, int n
     inB = inB && act;
     if (wave_ballot(inA || inB) == 0) continue;
 
-    const float rinv = rsqrtf(r2);
+    const float rinv = __builtin_amdgcn_rsqf(r2);
     const float r2inv = rinv * rinv;
     float fs = 0.f, ea = 0.f, eb = 0.f;
     if (TA == PM_TERM_LJ) {
       float A, B;
       if (peratom) {
         // GROMACS mixing (BuildTopology.cpp:650-659): sigma = (si+sj)/2, eps = sqrt(ei ej)
-        const float2 pj = S.ljp[sj];
         const float sg = pi.x + pj.x, ep = pi.y * pj.y;
         const float sg2 = sg * sg, sg6 = sg2 * sg2 * sg2;
         B = 4.0f * ep * sg6;
         A = B * sg6;
         if (mod) {
-          A *= S.lj_scaling;
-          B *= S.lj_scaling;
+          A *= ljsc;
+          B *= ljsc;
         }
       } else {
-        const float4 p = S.lj_table[ti * T + S.ljt[sj]];
+        const float4 p = ljs[wv][tiT + tj];
         A = mod ? p.z : p.x;
         B = mod ? p.w : p.y;
       }
@@ -611,36 +733,36 @@ __global__ __launch_bounds__(256) void k_pair(SysDev *sys, const int *clp, int n
       const float r12A = A * r6 * r6;
       float E = r12A - r6B;
       float F = (12.0f * r12A - 6.0f * r6B) * r2inv;
-      apply_switch(P.sw_a, r2, E, F);
+      apply_switch_t<SWA>(P.sw_a, r2, E, F);
       fs += inA ? F : 0.f;
       ea = inA ? E : 0.f;
     } else if (TA == PM_TERM_COULOMB) {
       float E = xi.w * xj.w * rinv;
-      if (mod) E *= S.csf;
+      if (mod) E *= csf;
       float F = E * r2inv;
-      apply_switch(P.sw_a, r2, E, F);
+      apply_switch_t<SWA>(P.sw_a, r2, E, F);
       fs += inA ? F : 0.f;
       ea = inA ? E : 0.f;
     }
     if (TB == PM_TERM_COULOMB) {
       // CoulombForce::operator() (CoulombForce.h:29-40)
       float E = xi.w * xj.w * rinv;
-      if (mod) E *= S.csf;
+      if (mod) E *= csf;
       float F = E * r2inv;
-      apply_switch(P.sw_b, r2, E, F);
+      apply_switch_t<SWB>(P.sw_b, r2, E, F);
       fs += inB ? F : 0.f;
       eb = inB ? E : 0.f;
     } else if (TB == PM_TERM_EWALD_REAL) {
       // NonbondedFullEwaldSystemForce::realTerm (NonbondedFullEwaldSystemForce.h:818-834):
       // erfc(ar) by Abramowitz & Stegun 7.1.26, the approximation the reference uses
       float qq = xi.w * xj.w;
-      if (mod) qq *= S.csf;
+      if (mod) qq *= csf;
       const float r = r2 * rinv;
       const float ar = P.alpha * r;
-      const float t = __frcp_rn(1.0f + 0.3275911f * ar);
+      const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * ar);
       const float poly =
           t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
-      const float ee = qq * exp2f(-(ar * ar) * 1.4426950408889634f);
+      const float ee = qq * __builtin_amdgcn_exp2f(-(ar * ar) * 1.4426950408889634f);
       const float E = poly * ee * rinv;
       const float F = (E + P.two_alpha_pi * ee) * r2inv;
       fs += inB ? F : 0.f;
@@ -662,43 +784,34 @@ __global__ __launch_bounds__(256) void k_pair(SysDev *sys, const int *clp, int n
     }
     if (CEN) {
       if (inA || inB) {
-        const int aj = S.orig[sj];
+        const int aj = orig[sj];
         const unsigned lo = (unsigned)min(ai, aj), hi = (unsigned)max(ai, aj);
         ccount += 1;
         chash += pair_hash(lo, hi);
       }
     }
-    // Newton's third law: j force = sum over the 8 i lanes that share jj
-    float gx = fx, gy = fy, gz = fz;
-    gx += __shfl_xor(gx, 8);
-    gy += __shfl_xor(gy, 8);
-    gz += __shfl_xor(gz, 8);
-    gx += __shfl_xor(gx, 16);
-    gy += __shfl_xor(gy, 16);
-    gz += __shfl_xor(gz, 16);
-    gx += __shfl_xor(gx, 32);
-    gy += __shfl_xor(gy, 32);
-    gz += __shfl_xor(gz, 32);
+    // Newton's third law: j force = sum over the 8 i lanes of this j atom
+    const float gx = dpp_sum8(fx), gy = dpp_sum8(fy), gz = dpp_sum8(fz);
     if (ii == 0 && jj < nvj) {
-      atomic_add_ll(&S.fx[sj], to_fixed((double)gx, kFScale));
-      atomic_add_ll(&S.fy[sj], to_fixed((double)gy, kFScale));
-      atomic_add_ll(&S.fz[sj], to_fixed((double)gz, kFScale));
+      atomic_add_ll(&gfx[sj], to_fixed((double)gx, kFScale));
+      atomic_add_ll(&gfy[sj], to_fixed((double)gy, kFScale));
+      atomic_add_ll(&gfz[sj], to_fixed((double)gz, kFScale));
     }
   }
   // i force = sum over the 8 j lanes that share ii
-  fix += __shfl_xor(fix, 1);
-  fiy += __shfl_xor(fiy, 1);
-  fiz += __shfl_xor(fiz, 1);
-  fix += __shfl_xor(fix, 2);
-  fiy += __shfl_xor(fiy, 2);
-  fiz += __shfl_xor(fiz, 2);
-  fix += __shfl_xor(fix, 4);
-  fiy += __shfl_xor(fiy, 4);
-  fiz += __shfl_xor(fiz, 4);
+  fix += __shfl_xor(fix, 8);
+  fiy += __shfl_xor(fiy, 8);
+  fiz += __shfl_xor(fiz, 8);
+  fix += __shfl_xor(fix, 16);
+  fiy += __shfl_xor(fiy, 16);
+  fiz += __shfl_xor(fiz, 16);
+  fix += __shfl_xor(fix, 32);
+  fiy += __shfl_xor(fiy, 32);
+  fiz += __shfl_xor(fiz, 32);
   if (jj == 0 && vi) {
-    atomic_add_ll(&S.fx[si], to_fixed((double)fix, kFScale));
-    atomic_add_ll(&S.fy[si], to_fixed((double)fiy, kFScale));
-    atomic_add_ll(&S.fz[si], to_fixed((double)fiz, kFScale));
+    atomic_add_ll(&gfx[si], to_fixed((double)fix, kFScale));
+    atomic_add_ll(&gfy[si], to_fixed((double)fiy, kFScale));
+    atomic_add_ll(&gfz[si], to_fixed((double)fiz, kFScale));
   }
   if (EN) {
     eA = wave_sum_d(eA);
@@ -710,23 +823,24 @@ __global__ __launch_bounds__(256) void k_pair(SysDev *sys, const int *clp, int n
     vyz = wave_sum_d(vyz);
     vzz = wave_sum_d(vzz);
     if (lane == 0) {
+      PM_GLOBAL long long *acc = gp(S->acc);
       const int slotA = TA == PM_TERM_LJ ? kAccLJ : kAccCoul;
-      atomic_add_ll(&S.acc[slotA], to_fixed(eA, kEScale));
-      atomic_add_ll(&S.acc[kAccCoul], to_fixed(eB, kEScale));
-      atomic_add_ll(&S.acc[kAccVxx], to_fixed(vxx, kEScale));
-      atomic_add_ll(&S.acc[kAccVxy], to_fixed(vxy, kEScale));
-      atomic_add_ll(&S.acc[kAccVxz], to_fixed(vxz, kEScale));
-      atomic_add_ll(&S.acc[kAccVyy], to_fixed(vyy, kEScale));
-      atomic_add_ll(&S.acc[kAccVyz], to_fixed(vyz, kEScale));
-      atomic_add_ll(&S.acc[kAccVzz], to_fixed(vzz, kEScale));
+      atomic_add_ll(&acc[slotA], to_fixed(eA, kEScale));
+      atomic_add_ll(&acc[kAccCoul], to_fixed(eB, kEScale));
+      atomic_add_ll(&acc[kAccVxx], to_fixed(vxx, kEScale));
+      atomic_add_ll(&acc[kAccVxy], to_fixed(vxy, kEScale));
+      atomic_add_ll(&acc[kAccVxz], to_fixed(vxz, kEScale));
+      atomic_add_ll(&acc[kAccVyy], to_fixed(vyy, kEScale));
+      atomic_add_ll(&acc[kAccVyz], to_fixed(vyz, kEScale));
+      atomic_add_ll(&acc[kAccVzz], to_fixed(vzz, kEScale));
     }
   }
   if (CEN) {
     ccount = wave_sum_u64(ccount);
     chash = wave_sum_u64(chash);
     if (lane == 0) {
-      atomicAdd(&S.census[0], ccount);
-      atomicAdd(&S.census[1], chash);
+      atomicAdd((unsigned long long *)&S->census[0], ccount);
+      atomicAdd((unsigned long long *)&S->census[1], chash);
     }
   }
 }
@@ -999,23 +1113,33 @@ void launch_rebuild(const Launch &L, float rlist2) {
   hipLaunchKernelGGL(k_pairlist, dim3(nblk((long long)L.n_cl_total * kWave, 256)), dim3(256), 0, L.stream, L.sys, L.cl_prefix, L.nsys, rlist2);
 }
 
-template <int TA, int TB>
+template <int TA, int TB, int SWA, int SWB>
 static void launch_pair_t(const Launch &L, const ForceDev &P, bool en, bool cen) {
   dim3 g(nblk((long long)L.n_cl_total * kWave, 256)), b(256);
-  if (en && cen) hipLaunchKernelGGL((k_pair<TA, TB, true, true>), g, b, 0, L.stream, L.sys, L.cl_prefix, L.nsys, P);
-  else if (en) hipLaunchKernelGGL((k_pair<TA, TB, true, false>), g, b, 0, L.stream, L.sys, L.cl_prefix, L.nsys, P);
-  else if (cen) hipLaunchKernelGGL((k_pair<TA, TB, false, true>), g, b, 0, L.stream, L.sys, L.cl_prefix, L.nsys, P);
-  else hipLaunchKernelGGL((k_pair<TA, TB, false, false>), g, b, 0, L.stream, L.sys, L.cl_prefix, L.nsys, P);
+  if (cen) hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, false, true>), g, b, 0, L.stream, L.sys, L.cl_prefix, L.nsys, P);
+  else if (en) hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, true, false>), g, b, 0, L.stream, L.sys, L.cl_prefix, L.nsys, P);
+  else hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, false, false>), g, b, 0, L.stream, L.sys, L.cl_prefix, L.nsys, P);
+}
+
+// C2 switches get a compile-time fast path; every other kind runs the generic one
+template <int TA, int TB>
+static void launch_pair_sw(const Launch &L, const ForceDev &P, bool en, bool cen) {
+  const bool a2 = TA == PM_TERM_NONE || P.sw_a.kind == PM_SWITCH_C2;
+  const bool b2 = TB != PM_TERM_COULOMB || P.sw_b.kind == PM_SWITCH_C2;
+  constexpr int SB = TB == PM_TERM_COULOMB ? PM_SWITCH_C2 : PM_SWITCH_CUTOFF;
+  constexpr int SBG = TB == PM_TERM_COULOMB ? kSwGeneric : PM_SWITCH_CUTOFF;
+  if (a2 && b2) launch_pair_t<TA, TB, PM_SWITCH_C2, SB>(L, P, en, cen);
+  else launch_pair_t<TA, TB, kSwGeneric, SBG>(L, P, en, cen);
 }
 
 bool launch_pair(const Launch &L, const ForceDev &P, bool en, bool cen) {
   if (!L.n_cl_total) return true;
   const int a = P.term_a, b = P.term_b;
-  if (a == PM_TERM_LJ && b == PM_TERM_NONE) launch_pair_t<PM_TERM_LJ, PM_TERM_NONE>(L, P, en, cen);
-  else if (a == PM_TERM_COULOMB && b == PM_TERM_NONE) launch_pair_t<PM_TERM_COULOMB, PM_TERM_NONE>(L, P, en, cen);
-  else if (a == PM_TERM_LJ && b == PM_TERM_COULOMB) launch_pair_t<PM_TERM_LJ, PM_TERM_COULOMB>(L, P, en, cen);
-  else if (a == PM_TERM_LJ && b == PM_TERM_EWALD_REAL) launch_pair_t<PM_TERM_LJ, PM_TERM_EWALD_REAL>(L, P, en, cen);
-  else if (a == PM_TERM_NONE && b == PM_TERM_EWALD_REAL) launch_pair_t<PM_TERM_NONE, PM_TERM_EWALD_REAL>(L, P, en, cen);
+  if (a == PM_TERM_LJ && b == PM_TERM_NONE) launch_pair_sw<PM_TERM_LJ, PM_TERM_NONE>(L, P, en, cen);
+  else if (a == PM_TERM_COULOMB && b == PM_TERM_NONE) launch_pair_sw<PM_TERM_COULOMB, PM_TERM_NONE>(L, P, en, cen);
+  else if (a == PM_TERM_LJ && b == PM_TERM_COULOMB) launch_pair_sw<PM_TERM_LJ, PM_TERM_COULOMB>(L, P, en, cen);
+  else if (a == PM_TERM_LJ && b == PM_TERM_EWALD_REAL) launch_pair_sw<PM_TERM_LJ, PM_TERM_EWALD_REAL>(L, P, en, cen);
+  else if (a == PM_TERM_NONE && b == PM_TERM_EWALD_REAL) launch_pair_t<PM_TERM_NONE, PM_TERM_EWALD_REAL, PM_SWITCH_NONE, PM_SWITCH_CUTOFF>(L, P, en, cen);
   else return false;
   return true;
 }

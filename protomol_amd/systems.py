@@ -101,12 +101,15 @@ def _random_rotations(rng: np.random.Generator, m: int) -> np.ndarray:
     ], 1)
 
 
-def water_box(n_mol: int, box: float, seed: int, origin=(0.0, 0.0, 0.0), jitter: float = 0.0) -> System:
+def water_box(n_mol: int, box: float, seed: int, origin=(0.0, 0.0, 0.0), jitter: float = 0.0,
+              atom_noise: float = 0.0) -> System:
     """``n_mol`` rigid-geometry 3-site waters on a cubic lattice in a cubic box.
 
     The lattice has m = ceil(n_mol^(1/3)) sites per edge (spacing box/m); when
     n_mol < m^3 a seeded subset of sites is used.  Each molecule gets a seeded
-    random orientation.  Atoms are ordered O, H, H per molecule.
+    random orientation.  Atoms are ordered O, H, H per molecule.  ``jitter``
+    moves molecule centres uniformly by up to that many A, ``atom_noise`` adds
+    Gaussian noise (A) to every atom, away from the ideal geometry.
     """
     rng = np.random.default_rng(seed)
     m = int(math.ceil(round(n_mol ** (1.0 / 3.0), 9)))
@@ -129,6 +132,8 @@ def water_box(n_mol: int, box: float, seed: int, origin=(0.0, 0.0, 0.0), jitter:
     xyz = centers[:, None, :] + np.einsum("mij,aj->mai", R, local)
     n = 3 * n_mol
     x = xyz.reshape(n, 3).astype(np.float64)
+    if atom_noise:
+        x = x + rng.normal(0.0, atom_noise, size=x.shape)
     charge = np.tile(np.array(WATER_Q) * SQRT_COULOMB, n_mol)
     mass = np.tile(np.array(WATER_MASS), n_mol)
     typ = np.tile(np.array([0, 1, 1], np.int32), n_mol)
@@ -200,14 +205,23 @@ def gb_chain(n_atoms: int, seed: int, bond: float = 1.53, min_dist: float = 2.5)
 
     grid.setdefault(key(pos[0]), []).append(0)
     i = 1
+    stuck = 0
     while i < n_atoms:
-        for _ in range(200):
+        for _ in range(100):
             v = rng.normal(size=3)
             p = pos[i - 1] + bond * v / np.linalg.norm(v)
             if ok(p, i):
                 break
         else:
-            raise RuntimeError("self-avoiding walk got stuck")
+            # dead end: drop the last few beads and grow again
+            stuck += 1
+            if stuck > 100 * n_atoms:
+                raise RuntimeError("self-avoiding walk got stuck")
+            back = min(i - 1, 5)
+            for j in range(i - back, i):
+                grid[key(pos[j])].remove(j)
+            i -= back
+            continue
         pos[i] = p
         grid.setdefault(key(p), []).append(i)
         i += 1

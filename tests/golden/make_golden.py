@@ -47,7 +47,7 @@ CUTOFF_CASES = {
 
 
 def water_fixture_system():
-    return S.water_box(343, 22.5, seed=5)
+    return S.water_box(343, 22.5, seed=5, jitter=0.3, atom_noise=0.03)
 
 
 def make_cutoff(name, sysobj, case):

@@ -27,7 +27,7 @@ def golden(name):
 
 def fixture_water():
     """The seeded system the ref_cutoff_* / ref_ewald / ref_bonded fixtures were written for."""
-    return S.water_box(343, 22.5, seed=5)
+    return S.water_box(343, 22.5, seed=5, jitter=0.3, atom_noise=0.03)
 
 
 def rms_rel(f, ref):
