@@ -126,7 +126,9 @@ typedef struct pm_force_desc {
   double gb_alpha, gb_beta, gb_gamma; /* OBC constants (ProtoMolApp.cpp:285-301) */
   double gb_solute_dielec, gb_solvent_dielec;
   double skin;                 /* pair-list buffer (Angstrom); 0 = choose */
-  int32_t rebuild_interval;    /* MD steps between pair-list rebuilds (>= 1) */
+  int32_t rebuild_interval;    /* MD: rebuild the pair list at least every this many steps
+                                  (0 = only by the displacement test); it is always rebuilt
+                                  when an atom has moved more than skin/2 since the build */
   uint32_t flags;              /* PM_FLAG_* */
 } pm_force_desc;
 
@@ -158,13 +160,15 @@ int pm_pair_census(pm_ctx *ctx, int32_t system, const double *x, uint64_t *count
 
 /* Device-resident leapfrog.  Set x, v (concatenated over systems), run
  * n_steps of LeapfrogIntegrator::run at dt_fs, read back state.  energies_every
- * > 0 evaluates energies on every k-th step and on the last step. */
+ * > 0 evaluates energies and virial at the last step (pm_md_get_state reads
+ * them).  The steps between the first and the last replay a captured HIP
+ * graph unless PM_FLAG_TIMING is set. */
 int pm_md_set_state(pm_ctx *ctx, const double *x, const double *v);
 int pm_md_run(pm_ctx *ctx, int32_t n_steps, double dt_fs, int32_t energies_every);
 int pm_md_get_state(pm_ctx *ctx, double *x, double *v, double *f, double *e);
 
-/* Statistics of the last call: stats[0] pair-list rebuilds, [1] segments
- * re-run because an atom moved beyond skin/2, [2] pairs evaluated (last
+/* Statistics of the last call: stats[0] pair-list rebuilds (summed over
+ * systems), [1] MD calls re-run after a pair-list capacity grew, [2] pairs evaluated (last
  * evaluation, all systems), [3] device ms of the last call,
  * [4] pair-kernel ms summed over the call's launches (PM_FLAG_TIMING, or the
  * single launch of an evaluation), [5] pair-kernel launches timed,

@@ -178,6 +178,10 @@ struct pm_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evp0 = nullptr, evp1 = nullptr;
   std::vector<hipEvent_t> evs;  // PM_FLAG_TIMING: start/stop per pair launch of an MD call
   int evs_used = 0;
+  // captured MD steps (kick + drift, rebuild decision, conditional rebuild,
+  // forces) for a given time step; replayed between the first and last step
+  hipGraphExec_t gexec = nullptr;
+  double g_h = 0.0;
   Allocs common;
 };
 
@@ -348,6 +352,7 @@ void setup_system(pm_ctx *c, SysHost &h, const pm_system_desc &d) {
   S.err = A.get<unsigned>(1);
   S.maxdisp2 = A.get<unsigned>(1);
   S.maxdisp_run = A.get<unsigned>(1);
+  S.ctl = A.get<int>(pm::kCtlN);
 
   if (c->gb) {
     if (!d.gb_radius || !d.gb_scaling) throw Fail{PM_EINVAL, "GB needs gb_radius and gb_scaling"};
@@ -510,11 +515,32 @@ hipEvent_t next_event(pm_ctx *c) {
   return c->evs[c->evs_used++];
 }
 
-void force_eval(pm_ctx *c, const Launch &L, bool rebuild, float rlist, bool energies, bool census, bool timed) {
-  pm::launch_clear_scalars(L, rebuild ? 1 : 0);
+// Rebuild policy of an MD run: the list (radius rc + skin) is rebuilt when
+// an atom has moved more than skin/2 from where it was built, or after
+// rebuild_interval steps when that is > 0.
+struct StepPolicy {
+  float rlist = 0.f, lim2 = 0.f;
+  int kmax = 0;
+};
+
+StepPolicy md_policy(const pm_ctx *c) {
+  StepPolicy p;
+  p.rlist = (float)(c->rc + c->skin_md);
+  const double lim = std::max(0.0, 0.5 * c->skin_md - 1e-3);
+  p.lim2 = (float)(lim * lim);
+  p.kmax = std::max(0, c->desc.rebuild_interval);
+  return p;
+}
+
+// one force evaluation of the systems in `L` at the positions already in S.x.
+// mode kStepForce rebuilds the list and zeroes the forces first; kStepAuto
+// (MD steps, whose kick has consumed and zeroed the forces) leaves the
+// rebuild decision to the device.
+void force_eval(pm_ctx *c, const Launch &L, int mode, const StepPolicy &pol, bool energies, bool census, bool timed) {
+  pm::launch_step_begin(L, mode, pol.lim2, pol.kmax);
   const bool pool = !timed && (c->desc.flags & PM_FLAG_TIMING);
+  if (mode & pm::kStepForce) pm::launch_zero_forces(L);
   if (c->gb) {
-    pm::launch_zero_forces(L);
     if (timed) HIPCHECK(hipEventRecord(c->evp0, c->stream));
     if (pool) HIPCHECK(hipEventRecord(next_event(c), c->stream));
     pm::launch_gb(L, c->P, energies);
@@ -523,10 +549,7 @@ void force_eval(pm_ctx *c, const Launch &L, bool rebuild, float rlist, bool ener
     pm::launch_bonded(L, energies);
     return;
   }
-  if (rebuild) {
-    pm::launch_zero_forces(L);
-    pm::launch_rebuild(L, rlist * rlist);
-  }
+  pm::launch_rebuild(L, pol.rlist * pol.rlist);
   if (timed) HIPCHECK(hipEventRecord(c->evp0, c->stream));
   if (pool) HIPCHECK(hipEventRecord(next_event(c), c->stream));
   if (!pm::launch_pair(L, c->P, energies, census)) throw Fail{PM_EINVAL, "unsupported term combination"};
@@ -629,7 +652,9 @@ void evaluate_range(pm_ctx *c, const Launch &L, int s0, int s1, const double *x,
   // the kernels index the staging buffer from the launch's first atom
   pm::launch_import_x(L, c->d_xaos + 3 * (size_t)off);
   for (int attempt = 0;; ++attempt) {
-    force_eval(c, L, true, (float)(c->rc + c->skin_eval), true, census, true);
+    StepPolicy pol;
+    pol.rlist = (float)(c->rc + c->skin_eval);
+    force_eval(c, L, pm::kStepForce, pol, true, census, true);
     HIPCHECK(hipEventRecord(c->ev1, c->stream));
     Readback r = read_scalars(c, s0, s1);
     if (check_errors(c, s0, r.err)) {
@@ -657,6 +682,39 @@ void evaluate_range(pm_ctx *c, const Launch &L, int s0, int s1, const double *x,
     c->stats[0] = 1 + attempt;
     return;
   }
+}
+
+constexpr int kGraphSteps = 10;
+
+// The graph of kGraphSteps MD steps (kick + drift, then forces with the
+// device's own rebuild decision) for time step h, captured on first use.
+// Kernel arguments are the context's fixed device tables, so a graph stays
+// valid when pair lists are regrown (only the tables' contents change).
+template <class Step>
+hipGraphExec_t md_graph(pm_ctx *c, double h, const StepPolicy &, Step &step) {
+  if (c->gexec && c->g_h == h) return c->gexec;
+  if (c->gexec) {
+    HIPCHECK(hipGraphExecDestroy(c->gexec));
+    c->gexec = nullptr;
+  }
+  hipGraph_t g = nullptr;
+  HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  try {
+    for (int k = 0; k < kGraphSteps; ++k) step(1, false);
+  } catch (...) {
+    (void)hipStreamEndCapture(c->stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  HIPCHECK(hipStreamEndCapture(c->stream, &g));
+  hipError_t e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    c->gexec = nullptr;
+    throw Fail{PM_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e)};
+  }
+  c->g_h = h;
+  return c->gexec;
 }
 
 }  // namespace
@@ -780,6 +838,7 @@ int pm_ctx_destroy(pm_ctx *c) {
   for (hipEvent_t e : {c->ev0, c->ev1, c->evp0, c->evp1})
     if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : c->evs) (void)hipEventDestroy(e);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return PM_OK;
@@ -822,7 +881,7 @@ int pm_md_set_state(pm_ctx *c, const double *x, const double *v) {
     pm::launch_import_v(c->L, c->d_vaos);
     // StandardIntegrator::initializeForces: F(x0) before the first step
     for (int attempt = 0;; ++attempt) {
-      force_eval(c, c->L, true, (float)(c->rc + c->skin_md), false, false, false);
+      force_eval(c, c->L, pm::kStepForce, md_policy(c), false, false, false);
       Readback r = read_scalars(c, 0, (int)c->sys.size());
       if (check_errors(c, 0, r.err)) {
         if (attempt > 4) throw Fail{PM_ECAPACITY, "pair list capacity kept overflowing"};
@@ -839,67 +898,76 @@ int pm_md_run(pm_ctx *c, int32_t n_steps, double dt_fs, int32_t energies_every) 
   return run_guarded(c, [&] {
     if (!c->md_ready) throw Fail{PM_ENOTREADY, "pm_md_set_state first"};
     if (n_steps < 1) return;
+    if (!(dt_fs > 0.0)) throw Fail{PM_EINVAL, "dt must be > 0"};
     // LeapfrogIntegrator::run (LeapfrogIntegrator.cpp:89-104): half kick + drift,
     // forces, then (kick + drift, forces) per step, closing half kick
     const double h = dt_fs * kInvTimeFactor;
-    const int K = std::max(1, c->desc.rebuild_interval);
-    const float rl = (float)(c->rc + c->skin_md);
-    const double lim = std::max(0.0, 0.5 * c->skin_md - 1e-3);
+    const StepPolicy pol = md_policy(c);
+    const bool timing = (c->desc.flags & PM_FLAG_TIMING) != 0;
+    const bool en = energies_every > 0;
     const int ns = (int)c->sys.size();
-    // snapshot of x, v for the (rare) re-run with a rebuild every step
+    // snapshot of x, v: a pair-list overflow (capacity grown) re-runs the call
     pm::launch_export_xv(c->L, c->d_xaos, c->d_vaos);
-    HIPCHECK(hipEventRecord(c->ev0, c->stream));
-    c->evs_used = 0;
-    int rebuilds = 0, reruns = 0;
-    double maxdisp = 0.0;
-    for (int pass = 0; pass < 2; ++pass) {
-      const int every = pass == 0 ? K : 1;
-      pm::launch_clear_scalars(c->L, 2);
-      for (int i = 0; i < n_steps; ++i) {
+    for (int attempt = 0;; ++attempt) {
+      HIPCHECK(hipEventRecord(c->ev0, c->stream));
+      c->evs_used = 0;
+      auto step = [&](int i, bool energies) {
         pm::launch_kick_drift(c->L, h, i == 0 ? 0 : 1);
-        const bool rb = !c->gb && (i % every == 0);
-        const bool last = i == n_steps - 1;
-        force_eval(c, c->L, rb, rl, last && energies_every > 0, false, false);
-        rebuilds += rb;
+        force_eval(c, c->L, pm::kStepAuto | (i == 0 ? pm::kStepResetRun : 0), pol, energies, false, false);
+      };
+      int i = 0;
+      step(i++, en && n_steps == 1);
+      if (!timing) {
+        // the steps between the first and the last replay a captured graph
+        while (n_steps - 1 - i >= kGraphSteps) {
+          HIPCHECK(hipGraphLaunch(md_graph(c, h, pol, step), c->stream));
+          i += kGraphSteps;
+        }
       }
+      for (; i < n_steps; ++i) step(i, en && i == n_steps - 1);
       pm::launch_half_kick(c->L, 0.5 * dt_fs * kInvTimeFactor);
-      std::vector<unsigned> md(2 * (size_t)ns);
+      HIPCHECK(hipEventRecord(c->ev1, c->stream));
+      std::vector<unsigned> md(ns);
+      std::vector<int> ctl(pm::kCtlN * (size_t)ns);
       for (int s = 0; s < ns; ++s) {
-        HIPCHECK(hipMemcpyAsync(&md[2 * s], c->sys[s].dev.maxdisp2, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
-        HIPCHECK(hipMemcpyAsync(&md[2 * s + 1], c->sys[s].dev.maxdisp_run, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipMemcpyAsync(&md[s], c->sys[s].dev.maxdisp_run, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipMemcpyAsync(&ctl[pm::kCtlN * (size_t)s], c->sys[s].dev.ctl, pm::kCtlN * sizeof(int),
+                                hipMemcpyDeviceToHost, c->stream));
       }
       Readback r = read_scalars(c, 0, ns);
-      if (check_errors(c, 0, r.err)) throw Fail{PM_ECAPACITY, "pair list overflow during MD (capacity grown; rerun)"};
+      if (check_errors(c, 0, r.err)) {
+        if (attempt > 4) throw Fail{PM_ECAPACITY, "pair list capacity kept overflowing"};
+        // lists regrown: restore the state of the call's start and run it again
+        pm::launch_import_x(c->L, c->d_xaos);
+        pm::launch_import_v(c->L, c->d_vaos);
+        force_eval(c, c->L, pm::kStepForce, pol, false, false, false);
+        continue;
+      }
+      float ms = 0.f;
+      HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+      double pair_ms = 0.0;
+      for (int k = 0; k + 1 < c->evs_used; k += 2) {
+        float t = 0.f;
+        HIPCHECK(hipEventElapsedTime(&t, c->evs[k], c->evs[k + 1]));
+        pair_ms += t;
+      }
       unsigned worst = 0;
-      for (unsigned u : md) worst = std::max(worst, u);
+      double rebuilds = 0;
+      for (int s = 0; s < ns; ++s) {
+        worst = std::max(worst, md[s]);
+        rebuilds += ctl[pm::kCtlN * (size_t)s + pm::kCtlCount];
+      }
       float w;
       std::memcpy(&w, &worst, sizeof w);
-      if (pass == 0) maxdisp = std::sqrt((double)w);
-      if (c->gb || every == 1 || std::sqrt((double)w) <= lim) break;
-      // an atom moved further than skin/2 between rebuilds: restore the state
-      // of the call's start and redo it with a rebuild before every evaluation
-      ++reruns;
-      pm::launch_import_x(c->L, c->d_xaos);
-      pm::launch_import_v(c->L, c->d_vaos);
-      force_eval(c, c->L, true, rl, false, false, false);
+      c->stats[0] = rebuilds;
+      c->stats[1] = attempt;
+      c->stats[3] = ms;
+      c->stats[4] = pair_ms;
+      c->stats[5] = c->evs_used / 2;
+      c->stats[6] = std::sqrt((double)w);
+      c->stats[7] = c->skin_md;
+      break;
     }
-    HIPCHECK(hipEventRecord(c->ev1, c->stream));
-    HIPCHECK(hipStreamSynchronize(c->stream));
-    float ms = 0.f;
-    HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    double pair_ms = 0.0;
-    for (int k = 0; k + 1 < c->evs_used; k += 2) {
-      float t = 0.f;
-      HIPCHECK(hipEventElapsedTime(&t, c->evs[k], c->evs[k + 1]));
-      pair_ms += t;
-    }
-    c->stats[0] = rebuilds;
-    c->stats[1] = reruns;
-    c->stats[3] = ms;
-    c->stats[4] = pair_ms;
-    c->stats[5] = c->evs_used / 2;
-    c->stats[6] = maxdisp;
-    c->stats[7] = c->skin_md;
   });
 }
 

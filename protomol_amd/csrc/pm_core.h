@@ -30,6 +30,12 @@ enum { kAccLJ = 0, kAccCoul = 1, kAccVxx, kAccVxy, kAccVxz, kAccVyy, kAccVyz, kA
 // device-side status bits (per system)
 enum { kErrColumnCap = 1, kErrListCap = 2, kErrNonFinite = 4, kErrClusterCap = 8 };
 
+// per-system control words: [0] rebuild the pair list in this step (0/1),
+// [1] steps since the last rebuild, [2] rebuilds since the start of the run
+enum { kCtlRebuild = 0, kCtlSince = 1, kCtlCount = 2, kCtlN = 4 };
+// k_step_begin modes (bit set)
+enum { kStepForce = 1, kStepAuto = 2, kStepResetRun = 4 };
+
 struct SwitchDev {
   int kind;                // PM_SWITCH_*
   float cut2, on2;         // C2 / cutoff: cutoff^2, switchon^2 (FP32 path)
@@ -116,8 +122,9 @@ struct SysDev {
   long long *acc;          // kAccN fixed-point energy/virial accumulators
   unsigned long long *census; // count, hash
   unsigned *err;
-  unsigned *maxdisp2;      // float bits of the largest squared displacement since rebuild
-  unsigned *maxdisp_run;   // the same, folded over every rebuild of the current run
+  unsigned *maxdisp2;      // float bits of the largest squared displacement from the last rebuild (this step)
+  unsigned *maxdisp_run;   // the same, folded over every step of the current run
+  int *ctl;                // kCtlN control words
 };
 
 }  // namespace pm

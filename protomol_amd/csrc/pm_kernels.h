@@ -26,7 +26,7 @@ void launch_bonded(const Launch &L, bool energies);
 void launch_kick_drift(const Launch &L, double h, int mode);
 void launch_half_kick(const Launch &L, double hh);
 void launch_zero_forces(const Launch &L);
-void launch_clear_scalars(const Launch &L, int disp);
+void launch_step_begin(const Launch &L, int mode, float lim2, int kmax);
 void launch_export_f(const Launch &L, double *out);
 void launch_export_xv(const Launch &L, double *xo, double *vo);
 

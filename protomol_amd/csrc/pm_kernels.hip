@@ -251,6 +251,9 @@ __global__ void k_bin(SysDev *sys, const int *ap, int nsys) {
   int s = 0, col = 0;
   if (todo) {
     s = find_sys(ap, nsys, g);
+    todo = sys[s].ctl[kCtlRebuild] != 0;
+  }
+  if (todo) {
     SysDev &S = sys[s];
     const int a = g - ap[s], n = S.n_atoms;
     double x = S.x[a], y = S.x[n + a], z = S.x[2 * n + a];
@@ -284,6 +287,7 @@ __global__ void k_bin(SysDev *sys, const int *ap, int nsys) {
 // one block per system: exclusive scans of atoms and clusters over columns
 __global__ __launch_bounds__(1024) void k_col_scan(SysDev *sys) {
   SysDev &S = sys[blockIdx.x];
+  if (!S.ctl[kCtlRebuild]) return;
   __shared__ int sa[1024], sc[1024];
   const int t = threadIdx.x, nt = blockDim.x;
   const int ncol = S.ncol;
@@ -333,6 +337,9 @@ __global__ void k_col_fill(SysDev *sys, const int *ap, int nsys) {
   int s = 0, col = 0, a = 0;
   if (todo) {
     s = find_sys(ap, nsys, g);
+    todo = sys[s].ctl[kCtlRebuild] != 0;
+  }
+  if (todo) {
     a = g - ap[s];
     col = sys[s].bin_col[a];
   }
@@ -363,6 +370,7 @@ __global__ __launch_bounds__(1024) void k_col_sort(SysDev *sys, const int *cp, i
   if (gcol >= cp[nsys]) return;
   int s = find_sys(cp, nsys, gcol);
   SysDev &S = sys[s];
+  if (!S.ctl[kCtlRebuild]) return;
   int col = gcol - cp[s];
   const int n = S.col_count[col];
   const int t = threadIdx.x, nt = blockDim.x;
@@ -424,7 +432,7 @@ __global__ void k_cluster_geom(SysDev *sys, const int *clp, int nsys) {
   int s = find_sys(clp, nsys, g);
   SysDev &S = sys[s];
   int c = g - clp[s];
-  if (c >= *S.n_clusters) return;
+  if (!S.ctl[kCtlRebuild] || c >= *S.n_clusters) return;
   const int n = S.n_atoms, nv = S.cl_nvalid[c];
   double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
   double rel[kCluster][3];
@@ -480,7 +488,7 @@ __global__ __launch_bounds__(256) void k_pairlist(SysDev *sys, const int *clp, i
   const int s = find_sys(clp, nsys, w);
   SysDev &S = sys[s];
   const int ci = w - clp[s];
-  if (ci >= *S.n_clusters) return;
+  if (!S.ctl[kCtlRebuild] || ci >= *S.n_clusters) return;
   const int coli = S.cl_col[ci];
   const int cxi = coli % S.ncx, cyi = coli / S.ncx;
   const float4 loi = S.cl_lo[ci], hii = S.cl_hi[ci];
@@ -976,7 +984,7 @@ __global__ void k_bonded(SysDev *sys, const int *tp, int nsys, int energies) {
 // mode 1: doKickdoDrift     (LeapfrogIntegrator.cpp:58-87)  v += F*h/m;     x += v*h
 // The force buffer is consumed (zeroed) and the FP32 cluster-local
 // coordinates for the next force evaluation are written.
-__global__ void k_kick_drift(SysDev *sys, const int *ap, int nsys, double h, int mode) {
+__global__ __launch_bounds__(256) void k_kick_drift(SysDev *sys, const int *ap, int nsys, double h, int mode) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   float d2 = 0.f;
   int s = 0;
@@ -1009,13 +1017,24 @@ __global__ void k_kick_drift(SysDev *sys, const int *ap, int nsys, double h, int
     d2 = ex * ex + ey * ey + ez * ez;
     if (!isfinite(d2)) d2 = 3.0e38f;
   }
-  // wave max per system: lanes of one wave may straddle two systems, so
-  // reduce across the wave only when every active lane has the same system
-  const int s0 = __shfl(s, 0);
-  const unsigned long long mixed = wave_ballot(ok && s != s0);
+  // max per system: a block whose atoms all belong to one system reduces
+  // through LDS to one atomic; a block straddling systems uses one per lane
+  __shared__ float red[4];
+  __shared__ int bs[2];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int glast = min((int)(blockIdx.x * blockDim.x + blockDim.x), ap[nsys]) - 1;
+  if (threadIdx.x == 0) bs[0] = s;
+  if (g == glast) bs[1] = s;
   const float mx = wave_max_f(d2);
-  if (mixed == 0) {
-    if ((threadIdx.x & 63) == 0 && ok) atomicMax(sys[s0].maxdisp2, __float_as_uint(mx));
+  if (lane == 0) red[wv] = mx;
+  __syncthreads();
+  const int nw = blockDim.x >> 6;
+  if (bs[0] == bs[1]) {
+    if (threadIdx.x == 0) {
+      float m = red[0];
+      for (int k = 1; k < nw; ++k) m = fmaxf(m, red[k]);
+      atomicMax(sys[s].maxdisp2, __float_as_uint(m));
+    }
   } else if (ok) {
     atomicMax(sys[s].maxdisp2, __float_as_uint(d2));
   }
@@ -1047,10 +1066,12 @@ __global__ void k_zero_forces(SysDev *sys, const int *slp, int nsys) {
   S.fz[k] = 0;
 }
 
-// per-system scalars: accumulators, census, displacement.  disp: 0 keep,
-// 1 fold the segment's displacement into the run maximum and restart it,
-// 2 reset both (start of a run)
-__global__ void k_clear_scalars(SysDev *sys, int nsys, int disp) {
+// Start of a force evaluation, one block per system: clear the energy/virial
+// accumulators and the census, and decide whether the pair list is rebuilt
+// (kStepForce: always; kStepAuto: when an atom moved more than lim from where
+// the list was built -- the Verlet criterion, with lim = skin/2 -- or kmax
+// steps have passed).  The decision is read by every rebuild kernel.
+__global__ void k_step_begin(SysDev *sys, int nsys, int mode, float lim2, int kmax) {
   const int s = blockIdx.x;
   if (s >= nsys) return;
   SysDev &S = sys[s];
@@ -1058,12 +1079,23 @@ __global__ void k_clear_scalars(SysDev *sys, int nsys, int disp) {
   if (threadIdx.x == 0) {
     S.census[0] = 0;
     S.census[1] = 0;
-    if (disp == 1) {
-      *S.maxdisp_run = max(*S.maxdisp_run, *S.maxdisp2);
-      *S.maxdisp2 = 0u;
-    } else if (disp == 2) {
+    int *ctl = S.ctl;
+    if (mode & kStepResetRun) {
       *S.maxdisp_run = 0u;
-      *S.maxdisp2 = 0u;
+      ctl[kCtlCount] = 0;
+    }
+    const unsigned d2 = *S.maxdisp2;
+    bool rb = (mode & kStepForce) != 0;
+    if (!rb && (mode & kStepAuto))
+      rb = __uint_as_float(d2) > lim2 || (kmax > 0 && ctl[kCtlSince] + 1 >= kmax);
+    *S.maxdisp_run = max(*S.maxdisp_run, d2);
+    *S.maxdisp2 = 0u;
+    ctl[kCtlRebuild] = rb ? 1 : 0;
+    if (rb) {
+      ctl[kCtlSince] = 0;
+      ctl[kCtlCount] += 1;
+    } else {
+      ctl[kCtlSince] += 1;
     }
   }
 }
@@ -1159,8 +1191,8 @@ void launch_half_kick(const Launch &L, double hh) {
 void launch_zero_forces(const Launch &L) {
   if (L.n_slots_total) hipLaunchKernelGGL(k_zero_forces, dim3(nblk(L.n_slots_total, 256)), dim3(256), 0, L.stream, L.sys, L.slot_prefix, L.nsys);
 }
-void launch_clear_scalars(const Launch &L, int disp) {
-  hipLaunchKernelGGL(k_clear_scalars, dim3(L.nsys), dim3(64), 0, L.stream, L.sys, L.nsys, disp);
+void launch_step_begin(const Launch &L, int mode, float lim2, int kmax) {
+  hipLaunchKernelGGL(k_step_begin, dim3(L.nsys), dim3(64), 0, L.stream, L.sys, L.nsys, mode, lim2, kmax);
 }
 void launch_export_f(const Launch &L, double *out) {
   if (L.n_atoms_total) hipLaunchKernelGGL(k_export_f, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys, out);

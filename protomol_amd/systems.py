@@ -101,8 +101,48 @@ def _random_rotations(rng: np.random.Generator, m: int) -> np.ndarray:
     ], 1)
 
 
+def _relieve_contacts(rng, centers, R, local, lo, box, min_contact, n_try=32, max_iter=200):
+    """Re-orient molecules whose hydrogens come closer than ``min_contact`` to
+    a hydrogen of another molecule (periodic box of edge ``box`` whose lower
+    corner is ``lo``).  Each offender takes the best of ``n_try`` random
+    orientations, judged by the distance from its hydrogens to the other
+    molecules' hydrogens.  (O-H contacts are hydrogen bonds and stay.)"""
+    from scipy.spatial import cKDTree
+    xyz = centers[:, None, :] + np.einsum("mij,aj->mai", R, local)
+    n_mol = len(centers)
+
+    def wrap(p):
+        q = np.mod(p - lo, box)
+        q[q >= box] = 0.0
+        return q
+
+    for _ in range(max_iter):
+        hyd = xyz[:, 1:3, :].reshape(-1, 3)          # H atoms, molecule = index // 2
+        tree = cKDTree(wrap(hyd), boxsize=box)
+        pairs = tree.query_pairs(min_contact, output_type="ndarray")
+        mi, mj = pairs[:, 0] // 2, pairs[:, 1] // 2
+        inter = mi != mj
+        if not inter.any():
+            return xyz
+        # one molecule of each contact, chosen at random: a molecule hemmed in
+        # by its neighbours' hydrogens may need a neighbour to turn instead
+        pick = rng.random(int(inter.sum())) < 0.5
+        redo = np.unique(np.where(pick, mi[inter], mj[inter]))
+        cand = _random_rotations(rng, len(redo) * n_try).reshape(len(redo), n_try, 3, 3)
+        pos = centers[redo, None, None, :] + np.einsum("mtij,aj->mtai", cand, local[1:3])
+        # the two nearest hydrogens: one may be the molecule's own other H
+        d, idx = tree.query(wrap(pos.reshape(-1, 3)), k=3)
+        own = (idx // 2) == np.repeat(redo, n_try * 2)[:, None]
+        d = np.where(own | (idx >= 2 * n_mol), np.inf, d)
+        score = d.min(axis=1).reshape(len(redo), n_try, 2).min(axis=2)
+        best = np.argmax(score, axis=1)
+        R[redo] = cand[np.arange(len(redo)), best]
+        xyz[redo] = centers[redo, None, :] + np.einsum("mij,aj->mai", R[redo], local)
+    raise RuntimeError("water_box: could not remove close contacts")
+
+
 def water_box(n_mol: int, box: float, seed: int, origin=(0.0, 0.0, 0.0), jitter: float = 0.0,
-              atom_noise: float = 0.0) -> System:
+              atom_noise: float = 0.0, min_contact: float = 0.0) -> System:
     """``n_mol`` rigid-geometry 3-site waters on a cubic lattice in a cubic box.
 
     The lattice has m = ceil(n_mol^(1/3)) sites per edge (spacing box/m); when
@@ -110,6 +150,10 @@ def water_box(n_mol: int, box: float, seed: int, origin=(0.0, 0.0, 0.0), jitter:
     random orientation.  Atoms are ordered O, H, H per molecule.  ``jitter``
     moves molecule centres uniformly by up to that many A, ``atom_noise`` adds
     Gaussian noise (A) to every atom, away from the ideal geometry.
+    ``min_contact`` > 0 re-draws the orientation of molecules that have a
+    hydrogen closer than that to a hydrogen of another molecule (periodic),
+    until no such contact is left: random orientations on a ~3 A lattice put
+    some H-H pairs at 1.2 A, whose repulsion would heat the start of a run.
     """
     rng = np.random.default_rng(seed)
     m = int(math.ceil(round(n_mol ** (1.0 / 3.0), 9)))
@@ -130,6 +174,8 @@ def water_box(n_mol: int, box: float, seed: int, origin=(0.0, 0.0, 0.0), jitter:
                       [-WATER_ROH * math.sin(half), 0.0, WATER_ROH * math.cos(half)]])
     R = _random_rotations(rng, n_mol)
     xyz = centers[:, None, :] + np.einsum("mij,aj->mai", R, local)
+    if min_contact > 0.0:
+        xyz = _relieve_contacts(rng, centers, R, local, org - 0.5 * box, box, min_contact)
     n = 3 * n_mol
     x = xyz.reshape(n, 3).astype(np.float64)
     if atom_noise:

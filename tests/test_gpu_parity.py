@@ -281,3 +281,66 @@ def test_water423_reference_regression_on_device():
         x, v, F, E = ctx.md_get_state()
     assert np.max(np.abs(x - d["pos200"])) < 2e-3
     assert T.rms_rel(F, d["forces200"]) < 1e-3
+
+
+def _md(desc, systems, v0s, steps, dt):
+    with L.Context(desc, systems) as ctx:
+        ctx.md_set_state(np.concatenate([s.x for s in systems]), np.concatenate(v0s))
+        ctx.md_run(steps, dt, energies=True)
+        x, v, F, E = ctx.md_get_state()
+        return x, v, F, E, ctx.stats()
+
+
+def test_md_graph_replay_equals_direct_launches():
+    """Steps replayed from the captured graph give the same bits as the same
+    steps launched one by one (PM_FLAG_TIMING disables the graph)."""
+    print("case md graph vs direct", flush=True)
+    w = S.water_box(343, 22.5, seed=8, min_contact=2.0)
+    v0 = S.maxwell_velocities(w, 300.0, seed=3)
+    ta, sa, tb, sb = MODELS["c2c2"]
+    runs = []
+    for flags in (L.FLAG_BONDED, L.FLAG_BONDED | L.FLAG_TIMING):
+        runs.append(_md(force_desc(ta, sa, tb, sb, flags=flags, skin=1.0, interval=7, ewald=True), [w], [v0], 37, 1.0))
+    (x1, v1, F1, E1, s1), (x2, v2, F2, E2, s2) = runs
+    assert np.array_equal(x1, x2) and np.array_equal(v1, v2) and np.array_equal(F1, F2)
+    assert np.array_equal(E1, E2)
+    assert s1["rebuilds"] == s2["rebuilds"] >= 37 // 7
+
+
+def test_md_displacement_triggers_rebuild():
+    """With no fixed interval the list is rebuilt only when an atom has moved
+    more than skin/2; the trajectory still follows the FP64 leapfrog."""
+    print("case md displacement rebuild", flush=True)
+    w = S.water_box(343, 22.5, seed=5)
+    v0 = S.maxwell_velocities(w, 300.0, seed=1)
+    sw = O.make_switch(O.SW_C2, 8.0, 10.0)
+
+    def forces(x):
+        f, e, _ = O.nonbonded_cutoff(w, O.TERM_LJ, sw, O.TERM_COULOMB, sw, x=x, cellsize=5.0)
+        fb, eb = O.bonded(w, x=x)
+        return f + fb, e + eb
+
+    steps, dt, skin = 30, 0.5, 0.3
+    xo, vo, Fo, Eo = T.leapfrog_oracle(forces, w.x.copy(), v0.copy(), w.mass, dt, steps)
+    ta, sa, tb, sb = MODELS["c2c2"]
+    x, v, F, E, st = _md(force_desc(ta, sa, tb, sb, flags=L.FLAG_BONDED, skin=skin, interval=0), [w], [v0], steps, dt)
+    assert np.max(np.abs(x - xo)) < 1e-4
+    assert T.rms_rel(F, Fo) < 1e-4
+    assert 1 <= st["rebuilds"] < steps
+
+
+def test_md_batch_alone_vs_mix_bitwise():
+    """Each system of a batched MD run (one launch per kernel for all, graph
+    replay, per-system rebuild decisions) ends bitwise where it ends alone."""
+    print("case md batch alone-vs-mix", flush=True)
+    systems = [S.water_box(n, (n / 0.0334) ** (1 / 3), seed=60 + k, min_contact=2.0) for k, n in enumerate((520, 900, 1400))]
+    v0s = [S.maxwell_velocities(s, 300.0, seed=70 + k) for k, s in enumerate(systems)]
+    ta, sa, tb, sb = MODELS["c2c2"]
+    desc = force_desc(ta, sa, tb, sb, flags=L.FLAG_BONDED, skin=0.8, interval=0, ewald=True)
+    xb, vb, Fb, Eb, _ = _md(desc, systems, v0s, 33, 1.0)
+    off = 0
+    for k, s in enumerate(systems):
+        x, v, F, E, _ = _md(desc, [s], [v0s[k]], 33, 1.0)
+        assert np.array_equal(xb[off:off + s.n], x) and np.array_equal(vb[off:off + s.n], v)
+        assert np.array_equal(Fb[off:off + s.n], F) and np.array_equal(Eb[k], E[0])
+        off += s.n
