@@ -21,6 +21,11 @@
  *                    with the force group evaluated on the device every step
  *   pm_pair_census   no reference counterpart: order-independent digest of the
  *                    evaluated pair set, for bit-exact pair-set checks
+ *   pm_ctx_reserve, pm_system_*, pm_*_systems, pm_md_begin/advance/end
+ *                    no reference counterpart: systems that join and leave a
+ *                    running context (admission by device-memory footprint),
+ *                    and MD of any subset of its systems in one fused launch
+ *                    sequence -- the serving and batching forms
  */
 #ifndef PROTOMOL_AMD_H
 #define PROTOMOL_AMD_H
@@ -31,7 +36,7 @@
 extern "C" {
 #endif
 
-#define PM_ABI_VERSION 1
+#define PM_ABI_VERSION 2
 
 /* status codes */
 #define PM_OK 0
@@ -40,6 +45,7 @@ extern "C" {
 #define PM_EDEVICE -3   /* HIP runtime error */
 #define PM_ECAPACITY -4 /* a device-side capacity was exceeded (column, list) */
 #define PM_ENOTREADY -5 /* state not set yet */
+#define PM_EFULL -6     /* the context's device reservation has no room for the system */
 
 /* switching functions (protomol/switch/) */
 #define PM_SWITCH_NONE 0    /* UniversalSwitchingFunction */
@@ -139,10 +145,24 @@ int pm_abi_version(void);
 int pm_device_info(char *name, int32_t name_len, int32_t *n_cu, double *hbm_gib);
 
 /* Create a context holding n_systems independent systems (a batch of 1 for
- * the plain drop-in).  All systems share the force description. */
+ * the plain drop-in; 0 for a context that systems join later).  All systems
+ * share the force description.  System ids are 0 .. n_systems-1, then the
+ * ids pm_system_add returns (the lowest free one). */
 int pm_ctx_create(const pm_force_desc *force, int32_t n_systems, const pm_system_desc *systems,
                   pm_ctx **out);
 int pm_ctx_destroy(pm_ctx *ctx);
+/* Reserve `bytes` of device memory for the context's systems (once, before
+ * any system is added).  Afterwards every system is placed inside it and
+ * pm_system_add returns PM_EFULL when it does not fit. */
+int pm_ctx_reserve(pm_ctx *ctx, uint64_t bytes);
+/* Reserved and used bytes of the reservation (0, 0 without one). */
+int pm_ctx_memory(pm_ctx *ctx, uint64_t *reserved, uint64_t *used);
+/* Device bytes a system would take in this context (its admission price). */
+int pm_system_footprint(pm_ctx *ctx, const pm_system_desc *system, uint64_t *bytes);
+/* Add a system (copied; the description may be freed afterwards). */
+int pm_system_add(pm_ctx *ctx, const pm_system_desc *system, int32_t *id);
+/* Remove a system and give its device memory back. */
+int pm_system_remove(pm_ctx *ctx, int32_t id);
 /* Last error message for ctx (or the calling thread when ctx is NULL). */
 const char *pm_last_error(const pm_ctx *ctx);
 
@@ -151,8 +171,10 @@ const char *pm_last_error(const pm_ctx *ctx);
 int pm_evaluate(pm_ctx *ctx, int32_t system, const double *x, double *f, double *e);
 
 /* The same over all systems at once (one fused launch sequence); x, f are the
- * systems' arrays concatenated in order, e holds 36 slots per system. */
+ * systems' arrays concatenated in id order, e holds 36 slots per system. */
 int pm_evaluate_batch(pm_ctx *ctx, const double *x, double *f, double *e);
+/* The same over the listed systems, arrays concatenated in list order. */
+int pm_evaluate_systems(pm_ctx *ctx, int32_t n, const int32_t *ids, const double *x, double *f, double *e);
 
 /* Digest of the pair set the last evaluation of `system` used: count and
  * sum of splitmix64(lo | hi << 32) over pairs (lo < hi, original indices). */
@@ -167,8 +189,19 @@ int pm_md_set_state(pm_ctx *ctx, const double *x, const double *v);
 int pm_md_run(pm_ctx *ctx, int32_t n_steps, double dt_fs, int32_t energies_every);
 int pm_md_get_state(pm_ctx *ctx, double *x, double *v, double *f, double *e);
 
+/* The same for a subset of systems.  pm_md_run = begin + advance + end:
+ * begin makes the next kick of each listed system a half kick, advance runs
+ * n_steps (kick + drift, forces) -- energies_last != 0 also sums energies and
+ * virial at the last step --, end applies the closing half kick.  A system's
+ * results depend only on its own calls, not on which systems share them. */
+int pm_md_set_state_systems(pm_ctx *ctx, int32_t n, const int32_t *ids, const double *x, const double *v);
+int pm_md_begin(pm_ctx *ctx, int32_t n, const int32_t *ids);
+int pm_md_advance(pm_ctx *ctx, int32_t n, const int32_t *ids, int32_t n_steps, double dt_fs, int32_t energies_last);
+int pm_md_end(pm_ctx *ctx, int32_t n, const int32_t *ids, double dt_fs);
+int pm_md_get_state_systems(pm_ctx *ctx, int32_t n, const int32_t *ids, double *x, double *v, double *f, double *e);
+
 /* Statistics of the last call: stats[0] pair-list rebuilds (summed over
- * systems), [1] MD calls re-run after a pair-list capacity grew, [2] pairs evaluated (last
+ * systems), [1] 1 if an MD advance was re-run after a pair-list capacity grew, [2] pairs evaluated (last
  * evaluation, all systems), [3] device ms of the last call,
  * [4] pair-kernel ms summed over the call's launches (PM_FLAG_TIMING, or the
  * single launch of an evaluation), [5] pair-kernel launches timed,

@@ -13,16 +13,18 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libprotomol_amd.so")
 
-PM_OK, PM_EINVAL, PM_ENOMEM, PM_EDEVICE, PM_ECAPACITY, PM_ENOTREADY = 0, -1, -2, -3, -4, -5
+PM_OK, PM_EINVAL, PM_ENOMEM, PM_EDEVICE, PM_ECAPACITY, PM_ENOTREADY, PM_EFULL = 0, -1, -2, -3, -4, -5, -6
 SWITCH_NONE, SWITCH_CUTOFF, SWITCH_C1, SWITCH_C2, SWITCH_CN, SWITCH_CMPCNCN = range(6)
 TERM_NONE, TERM_LJ, TERM_COULOMB, TERM_EWALD_REAL = range(4)
 LJ_TABLE, LJ_PERATOM = 0, 1
 FLAG_VIRIAL, FLAG_GB, FLAG_BONDED, FLAG_TIMING = 1, 2, 4, 8
 NSLOTS = 36
 
-EXPORTED = ("pm_abi_version", "pm_device_info", "pm_ctx_create", "pm_ctx_destroy", "pm_last_error",
-            "pm_evaluate", "pm_evaluate_batch", "pm_pair_census", "pm_md_set_state", "pm_md_run",
-            "pm_md_get_state", "pm_stats")
+EXPORTED = ("pm_abi_version", "pm_device_info", "pm_ctx_create", "pm_ctx_destroy", "pm_ctx_reserve",
+            "pm_ctx_memory", "pm_system_footprint", "pm_system_add", "pm_system_remove", "pm_last_error",
+            "pm_evaluate", "pm_evaluate_batch", "pm_evaluate_systems", "pm_pair_census", "pm_md_set_state",
+            "pm_md_run", "pm_md_get_state", "pm_md_set_state_systems", "pm_md_begin", "pm_md_advance",
+            "pm_md_end", "pm_md_get_state_systems", "pm_stats")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -81,6 +83,18 @@ def lib():
         L.pm_md_run.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_int32]
         L.pm_md_get_state.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
         L.pm_stats.argtypes = [C.c_void_p, _dp, C.c_int32]
+        u64p = C.POINTER(C.c_uint64)
+        L.pm_ctx_reserve.argtypes = [C.c_void_p, C.c_uint64]
+        L.pm_ctx_memory.argtypes = [C.c_void_p, u64p, u64p]
+        L.pm_system_footprint.argtypes = [C.c_void_p, C.POINTER(SystemDesc), u64p]
+        L.pm_system_add.argtypes = [C.c_void_p, C.POINTER(SystemDesc), _ip]
+        L.pm_system_remove.argtypes = [C.c_void_p, C.c_int32]
+        L.pm_evaluate_systems.argtypes = [C.c_void_p, C.c_int32, _ip, _dp, _dp, _dp]
+        L.pm_md_set_state_systems.argtypes = [C.c_void_p, C.c_int32, _ip, _dp, _dp]
+        L.pm_md_begin.argtypes = [C.c_void_p, C.c_int32, _ip]
+        L.pm_md_advance.argtypes = [C.c_void_p, C.c_int32, _ip, C.c_int32, C.c_double, C.c_int32]
+        L.pm_md_end.argtypes = [C.c_void_p, C.c_int32, _ip, C.c_double]
+        L.pm_md_get_state_systems.argtypes = [C.c_void_p, C.c_int32, _ip, _dp, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -166,21 +180,62 @@ class _Keep:
         self.desc = d
 
 
-class Context:
-    """One device context holding one or more systems (pm_ctx)."""
+def _ids(ids):
+    return np.ascontiguousarray(np.asarray(ids, np.int32).reshape(-1))
 
-    def __init__(self, force: ForceDesc, systems):
-        self.systems = list(systems)
+
+def _cat(arrs):
+    return np.ascontiguousarray(np.concatenate([np.asarray(a, np.float64).reshape(-1, 3) for a in arrs]))
+
+
+class Context:
+    """One device context (pm_ctx) holding systems by id.  Systems passed at
+    creation get ids 0..n-1; ``add``/``remove`` let systems join and leave."""
+
+    def __init__(self, force: ForceDesc, systems=(), reserve_bytes: int | None = None):
         self.force = force
-        bonded = bool(force.flags & FLAG_BONDED)
-        self._keep = [_Keep(s, bonded) for s in self.systems]
-        arr = (SystemDesc * len(self.systems))(*[k.desc for k in self._keep])
+        self._bonded = bool(force.flags & FLAG_BONDED)
+        systems = list(systems)
+        keep = [_Keep(s, self._bonded) for s in systems]
+        arr = (SystemDesc * max(1, len(systems)))(*[k.desc for k in keep])
         h = C.c_void_p()
-        rc = lib().pm_ctx_create(C.byref(force), len(self.systems), arr, C.byref(h))
+        rc = lib().pm_ctx_create(C.byref(force), len(systems), arr, C.byref(h))
         if rc != PM_OK:
             raise PMError(rc, lib().pm_last_error(None).decode())
         self.h = h
-        self.offsets = np.cumsum([0] + [s.n for s in self.systems])
+        self.members = {k: s for k, s in enumerate(systems)}
+        if reserve_bytes:
+            check(lib().pm_ctx_reserve(self.h, int(reserve_bytes)), self.h)
+
+    # ---- membership
+    @property
+    def systems(self):
+        return [self.members[k] for k in sorted(self.members)]
+
+    @property
+    def offsets(self):
+        return np.cumsum([0] + [s.n for s in self.systems])
+
+    def footprint(self, system) -> int:
+        b = C.c_uint64()
+        check(lib().pm_system_footprint(self.h, C.byref(_Keep(system, self._bonded).desc), C.byref(b)), self.h)
+        return b.value
+
+    def memory(self):
+        r, u = C.c_uint64(), C.c_uint64()
+        check(lib().pm_ctx_memory(self.h, C.byref(r), C.byref(u)), self.h)
+        return r.value, u.value
+
+    def add(self, system) -> int:
+        """Add a system; raises PMError with code PM_EFULL when the reservation has no room."""
+        i = C.c_int32()
+        check(lib().pm_system_add(self.h, C.byref(_Keep(system, self._bonded).desc), C.byref(i)), self.h)
+        self.members[i.value] = system
+        return i.value
+
+    def remove(self, sid: int):
+        check(lib().pm_system_remove(self.h, sid), self.h)
+        del self.members[sid]
 
     def close(self):
         if getattr(self, "h", None):
@@ -199,7 +254,7 @@ class Context:
     def __exit__(self, *a):
         self.close()
 
-    # SystemForce::evaluate semantics: ADD into forces and energies
+    # ---- SystemForce::evaluate semantics: ADD into forces and energies
     def evaluate(self, system: int, x: np.ndarray, f: np.ndarray, e: np.ndarray):
         xx = np.ascontiguousarray(x, np.float64)
         assert f.dtype == np.float64 and f.flags.c_contiguous and e.dtype == np.float64
@@ -209,12 +264,22 @@ class Context:
         xx = np.ascontiguousarray(x, np.float64)
         check(lib().pm_evaluate_batch(self.h, ptr(xx), ptr(f), ptr(e)), self.h)
 
+    def evaluate_systems(self, ids, xs):
+        """Evaluate the listed systems in one fused launch; returns (forces per system, energies (k, 36))."""
+        ii = _ids(ids)
+        x = _cat(xs)
+        f = np.zeros_like(x)
+        e = np.zeros((len(ii), NSLOTS))
+        check(lib().pm_evaluate_systems(self.h, len(ii), ptr(ii, _ip), ptr(x), ptr(f), ptr(e)), self.h)
+        return self._split(ii, f), e
+
     def census(self, system: int, x: np.ndarray):
         xx = np.ascontiguousarray(x, np.float64)
         cnt, hsh = C.c_uint64(), C.c_uint64()
         check(lib().pm_pair_census(self.h, system, ptr(xx), C.byref(cnt), C.byref(hsh)), self.h)
         return cnt.value, hsh.value
 
+    # ---- MD over all systems
     def md_set_state(self, x, v):
         xx = np.ascontiguousarray(x, np.float64)
         vv = np.ascontiguousarray(v, np.float64)
@@ -228,9 +293,44 @@ class Context:
         x = np.zeros((n, 3))
         v = np.zeros((n, 3))
         f = np.zeros((n, 3))
-        e = np.zeros((len(self.systems), NSLOTS))
+        e = np.zeros((len(self.members), NSLOTS))
         check(lib().pm_md_get_state(self.h, ptr(x), ptr(v), ptr(f), ptr(e)), self.h)
         return x, v, f, e
+
+    # ---- MD over a subset (LeapfrogIntegrator::run = begin + advance + end)
+    def _split(self, ii, a):
+        out, off = [], 0
+        for sid in ii:
+            n = self.members[int(sid)].n
+            out.append(a[off:off + n])
+            off += n
+        return out
+
+    def set_state_systems(self, ids, xs, vs):
+        ii = _ids(ids)
+        x, v = _cat(xs), _cat(vs)
+        check(lib().pm_md_set_state_systems(self.h, len(ii), ptr(ii, _ip), ptr(x), ptr(v)), self.h)
+
+    def md_begin(self, ids):
+        ii = _ids(ids)
+        check(lib().pm_md_begin(self.h, len(ii), ptr(ii, _ip)), self.h)
+
+    def md_advance(self, ids, n_steps: int, dt_fs: float, energies_last: bool = False):
+        ii = _ids(ids)
+        check(lib().pm_md_advance(self.h, len(ii), ptr(ii, _ip), n_steps, dt_fs, 1 if energies_last else 0), self.h)
+
+    def md_end(self, ids, dt_fs: float):
+        ii = _ids(ids)
+        check(lib().pm_md_end(self.h, len(ii), ptr(ii, _ip), dt_fs), self.h)
+
+    def get_state_systems(self, ids):
+        """(x, v, f) lists per system and energies (k, 36)."""
+        ii = _ids(ids)
+        n = sum(self.members[int(k)].n for k in ii)
+        x, v, f = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+        e = np.zeros((len(ii), NSLOTS))
+        check(lib().pm_md_get_state_systems(self.h, len(ii), ptr(ii, _ip), ptr(x), ptr(v), ptr(f), ptr(e)), self.h)
+        return self._split(ii, x), self._split(ii, v), self._split(ii, f), e
 
     def stats(self):
         s = np.zeros(8)

@@ -1,5 +1,7 @@
-// pm_api.cpp -- host side of the C ABI: context setup, device memory,
-// launch sequencing and result export.  Compiled by hipcc for gfx950.
+// pm_api.cpp -- host side of the C ABI: contexts of systems that can come and
+// go, their device memory (optionally carved from one HBM reservation), the
+// launch tables of system subsets, launch sequencing and result export.
+// Compiled by hipcc for gfx950.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -8,6 +10,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <memory>
 #include <string>
 #include <utility>
 #include <vector>
@@ -25,6 +28,7 @@ namespace {
 
 thread_local std::string g_thread_err;
 constexpr double kInvTimeFactor = 0.02045482828087295;  // PMConstants.cpp:54
+constexpr size_t kAlign = 256;
 
 struct Fail {
   int code;
@@ -38,37 +42,106 @@ struct Fail {
       throw Fail{PM_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)};                 \
   } while (0)
 
-struct Allocs {
-  std::vector<void *> ptrs;
-  template <class T>
-  T *get(size_t n) {
-    if (n == 0) n = 1;
+inline size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+// One device reservation handed out first-fit in kAlign units, with
+// neighbouring free ranges merged on release.  Admission by HBM footprint
+// asks it for room; without a reservation blocks come from hipMalloc.
+struct Arena {
+  char *base = nullptr;
+  size_t cap = 0, used = 0;
+  std::map<size_t, size_t> free_;  // offset -> bytes
+
+  void init(size_t bytes) {
+    bytes = align_up(bytes);
     void *p = nullptr;
-    if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) throw Fail{PM_ENOMEM, "hipMalloc failed"};
-    ptrs.push_back(p);
-    if (hipMemset(p, 0, n * sizeof(T)) != hipSuccess) throw Fail{PM_EDEVICE, "hipMemset failed"};
-    return static_cast<T *>(p);
+    if (hipMalloc(&p, bytes) != hipSuccess) throw Fail{PM_ENOMEM, "hipMalloc of the reservation failed"};
+    base = static_cast<char *>(p);
+    cap = bytes;
+    used = 0;
+    free_.clear();
+    free_[0] = bytes;
   }
-  template <class T>
-  T *put(const T *h, size_t n) {
-    T *d = get<T>(n);
-    if (n && h && hipMemcpy(d, h, n * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
-      throw Fail{PM_EDEVICE, "hipMemcpy failed"};
-    return d;
+  void *alloc(size_t bytes) {
+    bytes = align_up(std::max<size_t>(bytes, 1));
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < bytes) continue;
+      const size_t off = it->first, rest = it->second - bytes;
+      free_.erase(it);
+      if (rest) free_[off + bytes] = rest;
+      used += bytes;
+      return base + off;
+    }
+    return nullptr;
   }
-  void release() {
-    for (void *p : ptrs) (void)hipFree(p);
-    ptrs.clear();
+  void release(void *p, size_t bytes) {
+    bytes = align_up(std::max<size_t>(bytes, 1));
+    size_t off = static_cast<char *>(p) - base;
+    used -= bytes;
+    auto next = free_.lower_bound(off);
+    if (next != free_.end() && off + bytes == next->first) {
+      bytes += next->second;
+      next = free_.erase(next);
+    }
+    if (next != free_.begin()) {
+      auto prev = std::prev(next);
+      if (prev->first + prev->second == off) {
+        prev->second += bytes;
+        return;
+      }
+    }
+    free_[off] = bytes;
+  }
+  void destroy() {
+    if (base) (void)hipFree(base);
+    base = nullptr;
+    cap = used = 0;
+    free_.clear();
   }
 };
 
+// Arrays carved out of one block at kAlign boundaries; with base == nullptr
+// it only measures (and every pointer it returns is null).
+struct Carver {
+  char *base = nullptr;
+  size_t off = 0;
+  template <class T>
+  T *take(size_t n) {
+    off = align_up(off);
+    T *p = base ? reinterpret_cast<T *>(base + off) : nullptr;
+    off += std::max<size_t>(n, 1) * sizeof(T);
+    return p;
+  }
+};
+
+
+// Host-side image of one system: validated inputs and the derived tables,
+// ready to be copied into its device block.
+struct SysPrep {
+  int n = 0;
+  std::vector<int> excl_start, excl_partner, excl_class, corr, loc_start, loc_item;
+  std::vector<double> invm;
+  std::vector<float> sig_half, eps_sqrt;
+  std::vector<float4> lj;
+  int n_corr = 0, n_bonds = 0, n_angles = 0, ncol = 1, ncx = 1, ncy = 1, rx = 0, ry = 0;
+  int cl_cap = 0, slot_cap = 0, list_cap = 0, n_gb_tiles = 0;
+  double ax = 0, ay = 0;
+  double coul_const = 0, vir_const = 0;
+};
+
 struct SysHost {
-  int n = 0, ncol = 0, cl_cap = 0, slot_cap = 0, list_cap = 0, n_corr = 0, n_bonded = 0;
+  int id = -1;
+  int n = 0, ncol = 0, cl_cap = 0, slot_cap = 0, list_cap = 0, n_corr = 0, n_bonded = 0, n_local = 0;
   int n_gb_tiles = 0;
   double coul_const = 0.0;  // Ewald point-self + charged-system energy (FP64, host)
   double vir_const = 0.0;   // charged-system virial diagonal
-  SysDev dev{};
-  Allocs main, lists;
+  SysDev dev{};             // host copy of the device record
+  void *block = nullptr;
+  size_t block_bytes = 0;
+  void *lists = nullptr;
+  size_t lists_bytes = 0;
+  int *ctl_snap = nullptr;  // control words saved at the start of an MD advance
+  bool md_ready = false;
 };
 
 // PM_SWITCH_* -> device record; returns the switch's cutoffSquared() in FP64
@@ -150,51 +223,6 @@ double build_switch(const pm_switch_desc &d, SwitchDev &s, bool ewald_hard, doub
   return cut2;
 }
 
-// per-system prefix record on the device: [0, count] for single-system launches
-struct Prefixes {
-  std::vector<int> atom, cl, col, slot, corr, bonded, gb;
-};
-
-}  // namespace
-
-struct pm_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  pm_force_desc desc{};
-  ForceDev P{};
-  bool gb = false;
-  double rc = 0.0, skin_md = 0.0, skin_eval = 0.0;
-  std::vector<SysHost> sys;
-  SysDev *d_sys = nullptr;
-  int *d_pref = nullptr;     // 7 prefix arrays of nsys+1 over all systems
-  int *d_single = nullptr;   // per system: 7 x [0, count]
-  Launch L;
-  std::vector<Launch> Ls;    // single-system launches
-  double *d_xaos = nullptr, *d_vaos = nullptr, *d_faos = nullptr;
-  int n_total = 0;
-  std::string err;
-  bool md_ready = false;
-  double stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  hipEvent_t ev0 = nullptr, ev1 = nullptr, evp0 = nullptr, evp1 = nullptr;
-  std::vector<hipEvent_t> evs;  // PM_FLAG_TIMING: start/stop per pair launch of an MD call
-  int evs_used = 0;
-  // captured MD steps (kick + drift, rebuild decision, conditional rebuild,
-  // forces) for a given time step; replayed between the first and last step
-  hipGraphExec_t gexec = nullptr;
-  double g_h = 0.0;
-  Allocs common;
-};
-
-namespace {
-
-void set_err(pm_ctx *c, const std::string &m) {
-  if (c) c->err = m;
-  g_thread_err = m;
-}
-
-void upload_sys(pm_ctx *c, int s) {
-  HIPCHECK(hipMemcpy(c->d_sys + s, &c->sys[s].dev, sizeof(SysDev), hipMemcpyHostToDevice));
-}
 
 int estimate_list_cap(double rho, double rlist, int cl_cap) {
   // clusters whose bounding box lies within rlist of an i-cluster box, halved
@@ -207,31 +235,385 @@ int estimate_list_cap(double rho, double rlist, int cl_cap) {
   return std::min(cap, 27 * cl_cap / 2 + 32);
 }
 
-void alloc_lists(SysHost &h) {
-  h.lists.release();
-  h.dev.list_cap = h.list_cap;
-  h.dev.list = h.lists.get<uint4>((size_t)h.cl_cap * h.list_cap);
-  h.dev.list_mask = h.lists.get<unsigned long long>(2 * (size_t)h.cl_cap * h.list_cap);
+// The launch table of a subset of a context's systems: copies of their device
+// records, prefix arrays over them, and a buffer for their packed scalars.
+struct LaunchSet {
+  std::vector<int> ids;
+  Launch L;
+  void *mem = nullptr;
+  long long *d_scal = nullptr;
+  long long *h_scal = nullptr;  // pinned
+  hipGraphExec_t gexec = nullptr;  // captured MD steps for time step g_h
+  double g_h = 0.0;
+  ~LaunchSet() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (h_scal) (void)hipHostFree(h_scal);
+    if (mem) (void)hipFree(mem);
+  }
+};
+
+}  // namespace
+
+struct pm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  pm_force_desc desc{};
+  ForceDev P{};
+  bool gb = false;
+  double rc = 0.0, skin_md = 0.0, skin_eval = 0.0;
+  std::vector<std::unique_ptr<SysHost>> sys;  // by id; null once removed
+  Arena arena;
+  std::map<std::vector<int>, std::unique_ptr<LaunchSet>> sets;
+  std::string err;
+  double stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, evp0 = nullptr, evp1 = nullptr;
+  std::vector<hipEvent_t> evs;  // PM_FLAG_TIMING: start/stop per pair launch of an MD call
+  int evs_used = 0;
+};
+
+namespace {
+
+void set_err(pm_ctx *c, const std::string &m) {
+  if (c) c->err = m;
+  g_thread_err = m;
 }
 
-void setup_system(pm_ctx *c, SysHost &h, const pm_system_desc &d) {
+void *dev_alloc(pm_ctx *c, size_t bytes) {
+  if (c->arena.base) {
+    void *p = c->arena.alloc(bytes);
+    if (!p) throw Fail{PM_EFULL, "the device reservation has no room for " + std::to_string(bytes) + " bytes"};
+    return p;
+  }
+  void *p = nullptr;
+  if (hipMalloc(&p, std::max<size_t>(bytes, 1)) != hipSuccess) throw Fail{PM_ENOMEM, "hipMalloc failed"};
+  return p;
+}
+
+void dev_free(pm_ctx *c, void *p, size_t bytes) {
+  if (!p) return;
+  if (c->arena.base) c->arena.release(p, bytes);
+  else (void)hipFree(p);
+}
+
+void drop_sets(pm_ctx *c) { c->sets.clear(); }
+
+std::vector<int> live_ids(const pm_ctx *c) {
+  std::vector<int> v;
+  for (size_t i = 0; i < c->sys.size(); ++i)
+    if (c->sys[i]) v.push_back((int)i);
+  return v;
+}
+
+SysHost &sys_of(pm_ctx *c, int id) {
+  if (id < 0 || id >= (int)c->sys.size() || !c->sys[id]) throw Fail{PM_EINVAL, "no system with id " + std::to_string(id)};
+  return *c->sys[id];
+}
+
+std::vector<int> check_ids(pm_ctx *c, int32_t n, const int32_t *ids) {
+  if (n < 1 || !ids) throw Fail{PM_EINVAL, "need >= 1 system id"};
+  std::vector<int> v(ids, ids + n);
+  std::vector<int> u = v;
+  std::sort(u.begin(), u.end());
+  if (std::adjacent_find(u.begin(), u.end()) != u.end()) throw Fail{PM_EINVAL, "repeated system id"};
+  for (int id : v) sys_of(c, id);
+  return v;
+}
+
+// ------------------------------------------------------------ system set-up
+
+// Validate a system description and build its host-side tables
+// (exclusions, Ewald correction pairs, per-atom bonded/correction items,
+// masses, LJ parameters, cluster grid and capacities).
+SysPrep prepare(const pm_ctx *c, const pm_system_desc &d) {
+  SysPrep p;
   const int n = d.n_atoms;
   if (n < 1) throw Fail{PM_EINVAL, "n_atoms must be >= 1"};
   if (!d.scaled_charge || !d.mass || !d.molecule) throw Fail{PM_EINVAL, "charge, mass and molecule are required"};
-  h.n = n;
+  p.n = n;
+  if (d.periodic) {
+    for (int k = 0; k < 3; ++k)
+      if (!(d.box[k] > 0)) throw Fail{PM_EINVAL, "box edges must be positive"};
+  }
+  // exclusion table: last add() of a pair wins (ExclusionTable.cpp:97-120);
+  // the Ewald correction walks the table's pairs with exact duplicates dropped
+  struct Ex {
+    int a, b, cls, order;
+  };
+  std::vector<Ex> ex;
+  ex.reserve(d.n_exclusions);
+  for (int k = 0; k < d.n_exclusions; ++k) {
+    int a = d.exclusions[3 * k], b = d.exclusions[3 * k + 1], cls = d.exclusions[3 * k + 2];
+    if (a < 0 || b < 0 || a >= n || b >= n) throw Fail{PM_EINVAL, "exclusion atom out of range"};
+    if (cls != PM_EXCL_FULL && cls != PM_EXCL_MODIFIED) throw Fail{PM_EINVAL, "bad exclusion class"};
+    if (a > b) std::swap(a, b);
+    if (a == b) continue;
+    ex.push_back({a, b, cls, k});
+  }
+  {
+    // correction pairs: first occurrence of each (a, b, class), in table order
+    std::vector<Ex> byk = ex;
+    std::stable_sort(byk.begin(), byk.end(), [](const Ex &x, const Ex &y) {
+      return x.a != y.a ? x.a < y.a : (x.b != y.b ? x.b < y.b : x.cls < y.cls);
+    });
+    std::vector<Ex> firsts;
+    for (size_t i = 0; i < byk.size(); ++i)
+      if (i == 0 || byk[i].a != byk[i - 1].a || byk[i].b != byk[i - 1].b || byk[i].cls != byk[i - 1].cls)
+        firsts.push_back(byk[i]);
+    std::sort(firsts.begin(), firsts.end(), [](const Ex &x, const Ex &y) { return x.order < y.order; });
+    const bool ewald = c->desc.term_b == PM_TERM_EWALD_REAL && c->desc.ewald_correction && !c->gb;
+    if (ewald)
+      for (const Ex &e : firsts) {
+        p.corr.push_back(e.a);
+        p.corr.push_back(e.b);
+        p.corr.push_back(e.cls);
+      }
+    p.n_corr = (int)p.corr.size() / 3;
+  }
+  // pair -> class of its last add
+  std::stable_sort(ex.begin(), ex.end(), [](const Ex &x, const Ex &y) { return x.a != y.a ? x.a < y.a : x.b < y.b; });
+  std::vector<Ex> last;
+  for (size_t i = 0; i < ex.size(); ++i)
+    if (i + 1 == ex.size() || ex[i + 1].a != ex[i].a || ex[i + 1].b != ex[i].b) last.push_back(ex[i]);
+  p.excl_start.assign(n + 1, 0);
+  for (const Ex &e : last) {
+    p.excl_start[e.a + 1]++;
+    p.excl_start[e.b + 1]++;
+  }
+  for (int i = 0; i < n; ++i) p.excl_start[i + 1] += p.excl_start[i];
+  p.excl_partner.resize(p.excl_start[n]);
+  p.excl_class.resize(p.excl_start[n]);
+  {
+    std::vector<int> fill(p.excl_start.begin(), p.excl_start.end() - 1);
+    for (const Ex &e : last) {
+      p.excl_partner[fill[e.a]] = e.b;
+      p.excl_class[fill[e.a]++] = e.cls;
+      p.excl_partner[fill[e.b]] = e.a;
+      p.excl_class[fill[e.b]++] = e.cls;
+    }
+  }
+  // bonded terms
+  p.n_bonds = (c->desc.flags & PM_FLAG_BONDED) ? d.n_bonds : 0;
+  p.n_angles = (c->desc.flags & PM_FLAG_BONDED) ? d.n_angles : 0;
+  if (p.n_bonds && (!d.bonds || !d.bond_k || !d.bond_r0)) throw Fail{PM_EINVAL, "bonds need atoms, k and r0"};
+  if (p.n_angles && (!d.angles || !d.angle_k || !d.angle_theta0)) throw Fail{PM_EINVAL, "angles need atoms, k and theta0"};
+  for (int k = 0; k < 2 * p.n_bonds; ++k)
+    if (d.bonds[k] < 0 || d.bonds[k] >= n) throw Fail{PM_EINVAL, "bond atom out of range"};
+  for (int k = 0; k < 3 * p.n_angles; ++k)
+    if (d.angles[k] < 0 || d.angles[k] >= n) throw Fail{PM_EINVAL, "angle atom out of range"};
+  // per-atom items of the owner-computes kernel: (index << 4) | (kind << 2) | role
+  {
+    std::vector<int> cnt(n + 1, 0);
+    for (int k = 0; k < p.n_bonds; ++k) for (int r = 0; r < 2; ++r) cnt[d.bonds[2 * k + r] + 1]++;
+    for (int k = 0; k < p.n_angles; ++k) for (int r = 0; r < 3; ++r) cnt[d.angles[3 * k + r] + 1]++;
+    for (int k = 0; k < p.n_corr; ++k) for (int r = 0; r < 2; ++r) cnt[p.corr[3 * k + r] + 1]++;
+    for (int i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+    p.loc_start = cnt;
+    p.loc_item.resize(cnt[n]);
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (int k = 0; k < p.n_bonds; ++k)
+      for (int r = 0; r < 2; ++r) p.loc_item[fill[d.bonds[2 * k + r]]++] = (k << 4) | (pm::kLocBond << 2) | r;
+    for (int k = 0; k < p.n_angles; ++k)
+      for (int r = 0; r < 3; ++r) p.loc_item[fill[d.angles[3 * k + r]]++] = (k << 4) | (pm::kLocAngle << 2) | r;
+    for (int k = 0; k < p.n_corr; ++k)
+      for (int r = 0; r < 2; ++r) p.loc_item[fill[p.corr[3 * k + r]]++] = (k << 4) | (pm::kLocCorr << 2) | r;
+  }
+  // per-atom data
+  p.invm.resize(n);
+  for (int i = 0; i < n; ++i) {
+    if (!(d.mass[i] > 0)) throw Fail{PM_EINVAL, "masses must be positive"};
+    p.invm[i] = 1.0 / d.mass[i];
+  }
+  if (d.lj_mode == PM_LJ_PERATOM) {
+    if (!d.lj_sigma || !d.lj_epsilon) throw Fail{PM_EINVAL, "per-atom LJ needs sigma and epsilon"};
+    p.sig_half.resize(n);
+    p.eps_sqrt.resize(n);
+    for (int i = 0; i < n; ++i) {
+      p.sig_half[i] = (float)(0.5 * d.lj_sigma[i]);
+      p.eps_sqrt[i] = (float)std::sqrt(d.lj_epsilon[i]);
+    }
+  } else {
+    if (d.n_types < 1 || !d.lj_table || !d.type) throw Fail{PM_EINVAL, "LJ table mode needs n_types, type and lj_table"};
+    if (d.n_types > 16 && !c->gb)
+      throw Fail{PM_EINVAL, "the cell-list pair kernel keeps the LJ table in LDS: at most 16 types (use PM_LJ_PERATOM)"};
+    for (int i = 0; i < n; ++i)
+      if (d.type[i] < 0 || d.type[i] >= d.n_types) throw Fail{PM_EINVAL, "atom type out of range"};
+    p.lj.resize((size_t)d.n_types * d.n_types);
+    for (size_t k = 0; k < p.lj.size(); ++k)
+      p.lj[k] = make_float4((float)d.lj_table[4 * k], (float)d.lj_table[4 * k + 1], (float)d.lj_table[4 * k + 2],
+                            (float)d.lj_table[4 * k + 3]);
+  }
+  if (c->gb) {
+    if (!d.gb_radius || !d.gb_scaling) throw Fail{PM_EINVAL, "GB needs gb_radius and gb_scaling"};
+    p.n_gb_tiles = pm::gb_tile_count(n);
+    p.cl_cap = 0;
+    p.slot_cap = n;  // forces are kept per atom: slot = atom
+    return p;
+  }
+  if (!d.periodic) throw Fail{PM_EINVAL, "the cell-list pair path needs periodic boundary conditions"};
+  // Ewald point-self and charged-system constants (NonbondedFullEwaldSystemForce.h:694-714)
+  if (c->desc.term_b == PM_TERM_EWALD_REAL && c->desc.ewald_correction) {
+    double q2 = 0.0, q = 0.0;
+    for (int i = 0; i < n; ++i) q2 += d.scaled_charge[i] * d.scaled_charge[i];
+    for (int i = 0; i < n; ++i) q += d.scaled_charge[i];
+    double alpha = c->desc.ewald_alpha;
+    double point = -q2 * alpha / std::sqrt(M_PI);
+    double V = std::fabs(d.box[0] * d.box[1] * d.box[2]);
+    double charged = 0.0;
+    if (std::fabs(q * 0.00268283) > 1.0e-5) charged = -M_PI / (2.0 * V * (alpha * alpha)) * q * q;
+    p.coul_const = point + charged;
+    p.vir_const = charged;
+  }
+  // x-y column grid: columns about (8/rho)^(1/3) wide so 8-atom clusters are compact
+  const double V = d.box[0] * d.box[1] * d.box[2];
+  const double rho = n / V;
+  const double a = std::cbrt(pm::kCluster / rho);
+  const double rlist = c->rc + std::max(c->skin_md, c->skin_eval);
+  for (int k = 0; k < 3; ++k)
+    if (d.box[k] < 2.0 * rlist)
+      throw Fail{PM_EINVAL, "box edge " + std::to_string(d.box[k]) + " is shorter than twice the pair-list radius " +
+                                std::to_string(rlist)};
+  p.ncx = std::max(1, (int)std::floor(d.box[0] / a));
+  p.ncy = std::max(1, (int)std::floor(d.box[1] / a));
+  p.ax = d.box[0] / p.ncx;
+  p.ay = d.box[1] / p.ncy;
+  p.ncol = p.ncx * p.ncy;
+  p.rx = (int)std::ceil(rlist / p.ax);
+  p.ry = (int)std::ceil(rlist / p.ay);
+  p.cl_cap = (n + pm::kCluster - 1) / pm::kCluster + p.ncol;
+  p.slot_cap = p.cl_cap * pm::kCluster;
+  p.list_cap = estimate_list_cap(rho, rlist, p.cl_cap);
+  return p;
+}
+
+// Lay out every device array of a system in one block (Carver measures with
+// a null base).  Returns the block size.
+size_t carve(const pm_ctx *c, const SysPrep &p, const pm_system_desc &d, Carver &cv, SysDev &S, int **ctl_snap) {
+  const int n = p.n;
+  S.excl_start = cv.take<int>(p.excl_start.size());
+  S.excl_partner = cv.take<int>(p.excl_partner.size());
+  S.excl_class = cv.take<int>(p.excl_class.size());
+  S.corr = cv.take<int>(p.corr.size());
+  S.loc_start = cv.take<int>(p.loc_start.size());
+  S.loc_item = cv.take<int>(p.loc_item.size());
+  S.charge = cv.take<double>(n);
+  S.inv_mass = cv.take<double>(n);
+  S.molecule = cv.take<int>(n);
+  S.type = cv.take<int>(n);
+  if (d.lj_mode == PM_LJ_PERATOM) {
+    S.sig_half = cv.take<float>(n);
+    S.eps_sqrt = cv.take<float>(n);
+  } else {
+    S.lj_table = cv.take<float4>(p.lj.size());
+  }
+  S.x = cv.take<double>(3 * (size_t)n);
+  S.v = cv.take<double>(3 * (size_t)n);
+  S.xref = cv.take<double>(3 * (size_t)n);
+  S.xaos = cv.take<double>(3 * (size_t)n);
+  S.vaos = cv.take<double>(3 * (size_t)n);
+  S.faos = cv.take<double>(3 * (size_t)n);
+  S.slot = cv.take<int>(n);
+  S.lbuild = cv.take<float4>(n);
+  S.bin_col = cv.take<int>(n);
+  S.bin_z = cv.take<float>(n);
+  S.colbuf = cv.take<unsigned long long>(n);
+  S.bonds = cv.take<int>(2 * (size_t)p.n_bonds);
+  S.bond_k = cv.take<double>(p.n_bonds);
+  S.bond_r0 = cv.take<double>(p.n_bonds);
+  S.angles = cv.take<int>(3 * (size_t)p.n_angles);
+  S.angle_k = cv.take<double>(p.n_angles);
+  S.angle_t0 = cv.take<double>(p.n_angles);
+  S.n_clusters = cv.take<int>(1);
+  S.acc = cv.take<long long>(pm::kAccN);
+  S.census = cv.take<unsigned long long>(2);
+  S.err = cv.take<unsigned>(1);
+  S.maxdisp2 = cv.take<unsigned>(1);
+  S.maxdisp_run = cv.take<unsigned>(1);
+  S.ctl = cv.take<int>(pm::kCtlN);
+  *ctl_snap = cv.take<int>(pm::kCtlN);
+  S.xq = cv.take<float4>(p.slot_cap);
+  S.fx = cv.take<long long>(p.slot_cap);
+  S.fy = cv.take<long long>(p.slot_cap);
+  S.fz = cv.take<long long>(p.slot_cap);
+  if (c->gb) {
+    S.gb_rho = cv.take<double>(n);
+    S.gb_S = cv.take<double>(n);
+    S.gb_born = cv.take<double>(n);
+    S.gb_drfac = cv.take<double>(n);
+    S.gb_partial = cv.take<double>(n);
+    S.gb_spare = cv.take<double>(n);
+    return cv.off;
+  }
+  S.col_count = cv.take<int>(p.ncol);
+  S.col_cursor = cv.take<int>(p.ncol);
+  S.col_atom_start = cv.take<int>(p.ncol);
+  S.col_cl_start = cv.take<int>(p.ncol);
+  S.col_ncl = cv.take<int>(p.ncol);
+  S.ljt = cv.take<int>(p.slot_cap);
+  S.ljp = cv.take<float2>(p.slot_cap);
+  S.orig = cv.take<int>(p.slot_cap);
+  S.smol = cv.take<int>(p.slot_cap);
+  S.cl_origin = cv.take<double>(3 * (size_t)p.cl_cap);
+  S.cl_lo = cv.take<float4>(p.cl_cap);
+  S.cl_hi = cv.take<float4>(p.cl_cap);
+  S.cl_col = cv.take<int>(p.cl_cap);
+  S.cl_nvalid = cv.take<int>(p.cl_cap);
+  S.list_len = cv.take<int>(p.cl_cap);
+  return cv.off;
+}
+
+size_t lists_bytes(int cl_cap, int list_cap) {
+  Carver cv;
+  cv.take<uint4>((size_t)cl_cap * list_cap);
+  cv.take<unsigned long long>(2 * (size_t)cl_cap * list_cap);
+  return cv.off;
+}
+
+void alloc_lists(pm_ctx *c, SysHost &h) {
+  if (h.cl_cap == 0) return;
+  const size_t bytes = lists_bytes(h.cl_cap, h.list_cap);
+  void *mem = dev_alloc(c, bytes);
+  dev_free(c, h.lists, h.lists_bytes);
+  h.lists = mem;
+  h.lists_bytes = bytes;
+  Carver cv;
+  cv.base = static_cast<char *>(mem);
+  h.dev.list_cap = h.list_cap;
+  h.dev.list = cv.take<uint4>((size_t)h.cl_cap * h.list_cap);
+  h.dev.list_mask = cv.take<unsigned long long>(2 * (size_t)h.cl_cap * h.list_cap);
+}
+
+uint64_t footprint(const pm_ctx *c, const pm_system_desc &d) {
+  SysPrep p = prepare(c, d);
+  Carver cv;
+  SysDev S{};
+  int *snap = nullptr;
+  size_t b = align_up(carve(c, p, d, cv, S, &snap));
+  if (!c->gb) b += align_up(lists_bytes(p.cl_cap, p.list_cap));
+  return b;
+}
+
+template <class T>
+void upload(T *dst, const T *src, size_t n) {
+  if (n && src) HIPCHECK(hipMemcpy(dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+}
+
+// add a system; returns its id
+int add_system(pm_ctx *c, const pm_system_desc &d) {
+  SysPrep p = prepare(c, d);
+  auto hp = std::make_unique<SysHost>();
+  SysHost &h = *hp;
   SysDev &S = h.dev;
+  const int n = p.n;
+  h.n = n;
   S.n_atoms = n;
   S.periodic = d.periodic ? 1 : 0;
   S.lj_mode = d.lj_mode;
   S.n_types = d.n_types;
   S.csf = (float)d.coulomb_scaling;
   S.lj_scaling = (float)d.lj_scaling;
-  Allocs &A = h.main;
-
   if (S.periodic) {
     // PeriodicBoundaryConditions::set (PeriodicBoundaryConditions.cpp:32-77), orthogonal
     const double Lx = d.box[0], Ly = d.box[1], Lz = d.box[2];
-    if (!(Lx > 0 && Ly > 0 && Lz > 0)) throw Fail{PM_EINVAL, "box edges must be positive"};
     double a1 = Ly * Lz - 0.0 * 0.0, d1 = Lx * a1 + 0.0 * 0.0 + 0.0 * 0.0;
     double a2 = Lz * Lx - 0.0 * 0.0, d2 = 0.0 * 0.0 + Ly * a2 + 0.0 * 0.0;
     double a3 = Lx * Ly - 0.0 * 0.0, d3 = 0.0 * 0.0 + 0.0 * 0.0 + Lz * a3;
@@ -247,276 +629,154 @@ void setup_system(pm_ctx *c, SysHost &h, const pm_system_desc &d) {
     double dx = (Lx * 0.5) * (Lx * 0.5), dy = (Ly * 0.5) * (Ly * 0.5), dz = (Lz * 0.5) * (Lz * 0.5);
     S.D = std::min(dx, std::min(dy, dz));
   }
+  S.n_corr = h.n_corr = p.n_corr;
+  S.n_bonds = p.n_bonds;
+  S.n_angles = p.n_angles;
+  h.n_bonded = p.n_bonds + p.n_angles;
+  h.n_local = (int)p.loc_item.size();
+  h.n_gb_tiles = p.n_gb_tiles;
+  h.coul_const = p.coul_const;
+  h.vir_const = p.vir_const;
+  S.ncx = p.ncx;
+  S.ncy = p.ncy;
+  S.ncol = h.ncol = p.ncol;
+  S.ax = p.ax;
+  S.ay = p.ay;
+  S.rx = p.rx;
+  S.ry = p.ry;
+  S.cl_cap = h.cl_cap = p.cl_cap;
+  S.slot_cap = h.slot_cap = p.slot_cap;
+  h.list_cap = p.list_cap;
 
-  // exclusion table: last add() of a pair wins (ExclusionTable.cpp:97-120)
-  std::map<std::pair<int, int>, int> ex;
-  std::vector<int> corr;
-  {
-    std::vector<std::pair<std::pair<int, int>, int>> seen;
-    for (int k = 0; k < d.n_exclusions; ++k) {
-      int a = d.exclusions[3 * k], b = d.exclusions[3 * k + 1], cls = d.exclusions[3 * k + 2];
-      if (a < 0 || b < 0 || a >= n || b >= n) throw Fail{PM_EINVAL, "exclusion atom out of range"};
-      if (cls != PM_EXCL_FULL && cls != PM_EXCL_MODIFIED) throw Fail{PM_EINVAL, "bad exclusion class"};
-      if (a > b) std::swap(a, b);
-      if (a == b) continue;
-      ex[{a, b}] = cls;
-      // the table of pairs the Ewald correction walks drops exact duplicates
-      bool dup = false;
-      for (auto &p : seen)
-        if (p.first.first == a && p.first.second == b && p.second == cls) { dup = true; break; }
-      if (!dup) {
-        seen.push_back({{a, b}, cls});
-        corr.push_back(a);
-        corr.push_back(b);
-        corr.push_back(cls);
-      }
+  Carver dry;
+  SysDev tmp{};
+  int *snap = nullptr;
+  h.block_bytes = carve(c, p, d, dry, tmp, &snap);
+  h.block = dev_alloc(c, h.block_bytes);
+  try {
+    Carver cv;
+    cv.base = static_cast<char *>(h.block);
+    carve(c, p, d, cv, S, &h.ctl_snap);
+    HIPCHECK(hipMemset(h.block, 0, h.block_bytes));
+    upload(const_cast<int *>(S.excl_start), p.excl_start.data(), p.excl_start.size());
+    upload(const_cast<int *>(S.excl_partner), p.excl_partner.data(), p.excl_partner.size());
+    upload(const_cast<int *>(S.excl_class), p.excl_class.data(), p.excl_class.size());
+    upload(const_cast<int *>(S.corr), p.corr.data(), p.corr.size());
+    upload(const_cast<int *>(S.loc_start), p.loc_start.data(), p.loc_start.size());
+    upload(const_cast<int *>(S.loc_item), p.loc_item.data(), p.loc_item.size());
+    upload(const_cast<double *>(S.charge), d.scaled_charge, n);
+    upload(const_cast<double *>(S.inv_mass), p.invm.data(), n);
+    upload(const_cast<int *>(S.molecule), d.molecule, n);
+    if (d.lj_mode == PM_LJ_PERATOM) {
+      upload(const_cast<float *>(S.sig_half), p.sig_half.data(), n);
+      upload(const_cast<float *>(S.eps_sqrt), p.eps_sqrt.data(), n);
+    } else {
+      upload(const_cast<int *>(S.type), d.type, n);
+      upload(const_cast<float4 *>(S.lj_table), p.lj.data(), p.lj.size());
     }
-  }
-  std::vector<int> cnt(n + 1, 0);
-  for (auto &kv : ex) { cnt[kv.first.first + 1]++; cnt[kv.first.second + 1]++; }
-  for (int i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
-  std::vector<int> partner(cnt[n]), pcls(cnt[n]), fill(cnt.begin(), cnt.end() - 1);
-  for (auto &kv : ex) {
-    int a = kv.first.first, b = kv.first.second;
-    partner[fill[a]] = b; pcls[fill[a]++] = kv.second;
-    partner[fill[b]] = a; pcls[fill[b]++] = kv.second;
-  }
-  S.excl_start = A.put(cnt.data(), cnt.size());
-  S.excl_partner = A.put(partner.data(), partner.size());
-  S.excl_class = A.put(pcls.data(), pcls.size());
-  const bool ewald = c->desc.term_b == PM_TERM_EWALD_REAL;
-  h.n_corr = (ewald && c->desc.ewald_correction) ? (int)corr.size() / 3 : 0;
-  S.n_corr = h.n_corr;
-  S.corr = A.put(corr.data(), corr.size());
-
-  // per-atom data
-  std::vector<double> invm(n);
-  for (int i = 0; i < n; ++i) {
-    if (!(d.mass[i] > 0)) throw Fail{PM_EINVAL, "masses must be positive"};
-    invm[i] = 1.0 / d.mass[i];
-  }
-  S.charge = A.put(d.scaled_charge, n);
-  S.inv_mass = A.put(invm.data(), n);
-  S.molecule = A.put(d.molecule, n);
-  if (d.lj_mode == PM_LJ_PERATOM) {
-    if (!d.lj_sigma || !d.lj_epsilon) throw Fail{PM_EINVAL, "per-atom LJ needs sigma and epsilon"};
-    std::vector<float> sh(n), es(n);
-    for (int i = 0; i < n; ++i) {
-      sh[i] = (float)(0.5 * d.lj_sigma[i]);
-      es[i] = (float)std::sqrt(d.lj_epsilon[i]);
+    upload(const_cast<int *>(S.bonds), d.bonds, 2 * (size_t)p.n_bonds);
+    upload(const_cast<double *>(S.bond_k), d.bond_k, p.n_bonds);
+    upload(const_cast<double *>(S.bond_r0), d.bond_r0, p.n_bonds);
+    upload(const_cast<int *>(S.angles), d.angles, 3 * (size_t)p.n_angles);
+    upload(const_cast<double *>(S.angle_k), d.angle_k, p.n_angles);
+    upload(const_cast<double *>(S.angle_t0), d.angle_theta0, p.n_angles);
+    if (c->gb) {
+      upload(const_cast<double *>(S.gb_rho), d.gb_radius, n);
+      upload(const_cast<double *>(S.gb_S), d.gb_scaling, n);
+      std::vector<int> ident(n);
+      for (int i = 0; i < n; ++i) ident[i] = i;
+      upload(S.slot, ident.data(), n);
+      S.ncx = S.ncy = S.ncol = 1;
+    } else {
+      alloc_lists(c, h);
     }
-    S.sig_half = A.put(sh.data(), n);
-    S.eps_sqrt = A.put(es.data(), n);
-    S.type = A.get<int>(n);
-  } else {
-    if (d.n_types < 1 || !d.lj_table || !d.type) throw Fail{PM_EINVAL, "LJ table mode needs n_types, type and lj_table"};
-    if (d.n_types > 16 && !c->gb)
-      throw Fail{PM_EINVAL, "the cell-list pair kernel keeps the LJ table in LDS: at most 16 types (use PM_LJ_PERATOM)"};
-    for (int i = 0; i < n; ++i)
-      if (d.type[i] < 0 || d.type[i] >= d.n_types) throw Fail{PM_EINVAL, "atom type out of range"};
-    std::vector<float4> t((size_t)d.n_types * d.n_types);
-    for (size_t k = 0; k < t.size(); ++k)
-      t[k] = make_float4((float)d.lj_table[4 * k], (float)d.lj_table[4 * k + 1], (float)d.lj_table[4 * k + 2],
-                         (float)d.lj_table[4 * k + 3]);
-    S.lj_table = A.put(t.data(), t.size());
-    S.type = A.put(d.type, n);
+  } catch (...) {
+    dev_free(c, h.lists, h.lists_bytes);
+    dev_free(c, h.block, h.block_bytes);
+    throw;
   }
-  S.x = A.get<double>(3 * (size_t)n);
-  S.v = A.get<double>(3 * (size_t)n);
-  S.xref = A.get<double>(3 * (size_t)n);
-  S.slot = A.get<int>(n);
-  S.lbuild = A.get<float4>(n);
-  S.bin_col = A.get<int>(n);
-  S.bin_z = A.get<float>(n);
-  S.colbuf = A.get<unsigned long long>(n);
-
-  // bonded terms
-  S.n_bonds = (c->desc.flags & PM_FLAG_BONDED) ? d.n_bonds : 0;
-  S.n_angles = (c->desc.flags & PM_FLAG_BONDED) ? d.n_angles : 0;
-  for (int k = 0; k < 2 * S.n_bonds; ++k)
-    if (d.bonds[k] < 0 || d.bonds[k] >= n) throw Fail{PM_EINVAL, "bond atom out of range"};
-  for (int k = 0; k < 3 * S.n_angles; ++k)
-    if (d.angles[k] < 0 || d.angles[k] >= n) throw Fail{PM_EINVAL, "angle atom out of range"};
-  S.bonds = A.put(d.bonds, 2 * (size_t)S.n_bonds);
-  S.bond_k = A.put(d.bond_k, S.n_bonds);
-  S.bond_r0 = A.put(d.bond_r0, S.n_bonds);
-  S.angles = A.put(d.angles, 3 * (size_t)S.n_angles);
-  S.angle_k = A.put(d.angle_k, S.n_angles);
-  S.angle_t0 = A.put(d.angle_theta0, S.n_angles);
-  h.n_bonded = S.n_bonds + S.n_angles;
-
-  // per-system scalars
-  S.n_clusters = A.get<int>(1);
-  S.acc = A.get<long long>(pm::kAccN);
-  S.census = A.get<unsigned long long>(2);
-  S.err = A.get<unsigned>(1);
-  S.maxdisp2 = A.get<unsigned>(1);
-  S.maxdisp_run = A.get<unsigned>(1);
-  S.ctl = A.get<int>(pm::kCtlN);
-
-  if (c->gb) {
-    if (!d.gb_radius || !d.gb_scaling) throw Fail{PM_EINVAL, "GB needs gb_radius and gb_scaling"};
-    pm::gb_setup_system(S, A.put(d.gb_radius, n), A.put(d.gb_scaling, n), A.get<double>(n), A.get<double>(n),
-                        A.get<double>(n), A.get<double>(n));
-    h.n_gb_tiles = pm::gb_tile_count(n);
-    // the cluster machinery is not used in vacuum; keep the arrays minimal
-    S.ncx = S.ncy = S.ncol = 1;
-    h.ncol = 1;
-    h.cl_cap = 0;
-    h.slot_cap = n;  // forces are kept per atom: slot = atom
-    S.cl_cap = 0;
-    S.slot_cap = n;
-    S.xq = A.get<float4>(n);
-    S.fx = A.get<long long>(n);
-    S.fy = A.get<long long>(n);
-    S.fz = A.get<long long>(n);
-    std::vector<int> ident(n);
-    for (int i = 0; i < n; ++i) ident[i] = i;
-    HIPCHECK(hipMemcpy(S.slot, ident.data(), n * sizeof(int), hipMemcpyHostToDevice));
-    return;
+  int id = -1;
+  for (size_t i = 0; i < c->sys.size(); ++i)
+    if (!c->sys[i]) { id = (int)i; break; }
+  if (id < 0) {
+    id = (int)c->sys.size();
+    c->sys.emplace_back();
   }
-
-  if (!S.periodic) throw Fail{PM_EINVAL, "the cell-list pair path needs periodic boundary conditions"};
-  // Ewald point-self and charged-system constants (NonbondedFullEwaldSystemForce.h:694-714)
-  if (ewald && c->desc.ewald_correction) {
-    double q2 = 0.0, q = 0.0;
-    for (int i = 0; i < n; ++i) q2 += d.scaled_charge[i] * d.scaled_charge[i];
-    for (int i = 0; i < n; ++i) q += d.scaled_charge[i];
-    double alpha = c->desc.ewald_alpha;
-    double point = -q2 * alpha / std::sqrt(M_PI);
-    double V = std::fabs(d.box[0] * d.box[1] * d.box[2]);
-    double charged = 0.0;
-    if (std::fabs(q * 0.00268283) > 1.0e-5) charged = -M_PI / (2.0 * V * (alpha * alpha)) * q * q;
-    h.coul_const = point + charged;
-    h.vir_const = charged;
-  }
-
-  // x-y column grid: columns about (8/rho)^(1/3) wide so 8-atom clusters are compact
-  const double V = d.box[0] * d.box[1] * d.box[2];
-  const double rho = n / V;
-  const double a = std::cbrt(pm::kCluster / rho);
-  double rlist = c->rc + std::max(c->skin_md, c->skin_eval);
-  for (int k = 0; k < 3; ++k)
-    if (d.box[k] < 2.0 * rlist)
-      throw Fail{PM_EINVAL, "box edge " + std::to_string(d.box[k]) + " is shorter than twice the pair-list radius " +
-                                std::to_string(rlist)};
-  S.ncx = std::max(1, (int)std::floor(d.box[0] / a));
-  S.ncy = std::max(1, (int)std::floor(d.box[1] / a));
-  S.ax = d.box[0] / S.ncx;
-  S.ay = d.box[1] / S.ncy;
-  S.ncol = S.ncx * S.ncy;
-  S.rx = (int)std::ceil(rlist / S.ax);
-  S.ry = (int)std::ceil(rlist / S.ay);
-  h.ncol = S.ncol;
-  h.cl_cap = (n + pm::kCluster - 1) / pm::kCluster + S.ncol;
-  h.slot_cap = h.cl_cap * pm::kCluster;
-  h.list_cap = estimate_list_cap(rho, rlist, h.cl_cap);
-  S.cl_cap = h.cl_cap;
-  S.slot_cap = h.slot_cap;
-  S.col_count = A.get<int>(S.ncol);
-  S.col_cursor = A.get<int>(S.ncol);
-  S.col_atom_start = A.get<int>(S.ncol);
-  S.col_cl_start = A.get<int>(S.ncol);
-  S.col_ncl = A.get<int>(S.ncol);
-  S.xq = A.get<float4>(h.slot_cap);
-  S.ljt = A.get<int>(h.slot_cap);
-  S.ljp = A.get<float2>(h.slot_cap);
-  S.orig = A.get<int>(h.slot_cap);
-  S.smol = A.get<int>(h.slot_cap);
-  S.fx = A.get<long long>(h.slot_cap);
-  S.fy = A.get<long long>(h.slot_cap);
-  S.fz = A.get<long long>(h.slot_cap);
-  S.cl_origin = A.get<double>(3 * (size_t)h.cl_cap);
-  S.cl_lo = A.get<float4>(h.cl_cap);
-  S.cl_hi = A.get<float4>(h.cl_cap);
-  S.cl_col = A.get<int>(h.cl_cap);
-  S.cl_nvalid = A.get<int>(h.cl_cap);
-  S.list_len = A.get<int>(h.cl_cap);
-  alloc_lists(h);
+  h.id = id;
+  c->sys[id] = std::move(hp);
+  drop_sets(c);
+  return id;
 }
 
-void build_launches(pm_ctx *c) {
-  const int ns = (int)c->sys.size();
-  Prefixes p;
-  auto push = [](std::vector<int> &v, int x) { v.push_back(v.empty() ? x : v.back() + x); };
-  p.atom = p.cl = p.col = p.slot = p.corr = p.bonded = p.gb = {0};
-  for (auto &h : c->sys) {
-    push(p.atom, h.n);
-    push(p.cl, h.cl_cap);
-    push(p.col, h.ncol);
-    push(p.slot, h.slot_cap);
-    push(p.corr, h.n_corr);
-    push(p.bonded, h.n_bonded);
-    push(p.gb, h.n_gb_tiles);
+void remove_system(pm_ctx *c, int id) {
+  SysHost &h = sys_of(c, id);
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  drop_sets(c);
+  dev_free(c, h.lists, h.lists_bytes);
+  dev_free(c, h.block, h.block_bytes);
+  c->sys[id].reset();
+}
+
+// ------------------------------------------------------------ launch sets
+
+LaunchSet &get_set(pm_ctx *c, const std::vector<int> &ids) {
+  auto it = c->sets.find(ids);
+  if (it != c->sets.end()) return *it->second;
+  if (c->sets.size() >= 64) c->sets.clear();
+  auto ls = std::make_unique<LaunchSet>();
+  ls->ids = ids;
+  const int ns = (int)ids.size();
+  std::vector<SysDev> table(ns);
+  std::vector<std::vector<int>> pre(8, std::vector<int>(1, 0));
+  for (int k = 0; k < ns; ++k) {
+    const SysHost &h = *c->sys[ids[k]];
+    table[k] = h.dev;
+    const int cnt[8] = {h.n, h.cl_cap, h.ncol, h.slot_cap, h.n_corr, h.n_bonded, h.n_gb_tiles, h.n_local};
+    for (int q = 0; q < 8; ++q) pre[q].push_back(pre[q].back() + cnt[q]);
   }
-  std::vector<int> all;
-  for (auto *v : {&p.atom, &p.cl, &p.col, &p.slot, &p.corr, &p.bonded, &p.gb}) all.insert(all.end(), v->begin(), v->end());
-  if (!c->d_pref) c->d_pref = c->common.get<int>(all.size());
-  HIPCHECK(hipMemcpy(c->d_pref, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice));
-  std::vector<int> single;
-  for (auto &h : c->sys)
-    for (int v : {h.n, h.cl_cap, h.ncol, h.slot_cap, h.n_corr, h.n_bonded, h.n_gb_tiles}) {
-      single.push_back(0);
-      single.push_back(v);
-    }
-  if (!c->d_single) c->d_single = c->common.get<int>(single.size());
-  HIPCHECK(hipMemcpy(c->d_single, single.data(), single.size() * sizeof(int), hipMemcpyHostToDevice));
-  Launch &L = c->L;
-  const int stride = ns + 1;
+  // one allocation: [SysDev x ns][8 prefix arrays of ns + 1][kScalN int64 x ns]
+  const size_t o_pre = align_up((size_t)ns * sizeof(SysDev));
+  const size_t o_scal = align_up(o_pre + 8 * (size_t)(ns + 1) * sizeof(int));
+  const size_t bytes = o_scal + (size_t)ns * pm::kScalN * sizeof(long long);
+  HIPCHECK(hipMalloc(&ls->mem, bytes));
+  char *base = static_cast<char *>(ls->mem);
+  SysDev *d_tab = reinterpret_cast<SysDev *>(base);
+  int *d_pre = reinterpret_cast<int *>(base + o_pre);
+  ls->d_scal = reinterpret_cast<long long *>(base + o_scal);
+  HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&ls->h_scal), (size_t)ns * pm::kScalN * sizeof(long long), 0));
+  std::vector<int> flat;
+  for (auto &v : pre) flat.insert(flat.end(), v.begin(), v.end());
+  HIPCHECK(hipMemcpy(d_tab, table.data(), ns * sizeof(SysDev), hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(d_pre, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice));
+  Launch &L = ls->L;
+  const int st = ns + 1;
   L.stream = c->stream;
-  L.sys = c->d_sys;
+  L.sys = d_tab;
   L.nsys = ns;
-  L.atom_prefix = c->d_pref;
-  L.cl_prefix = c->d_pref + stride;
-  L.col_prefix = c->d_pref + 2 * stride;
-  L.slot_prefix = c->d_pref + 3 * stride;
-  L.corr_prefix = c->d_pref + 4 * stride;
-  L.bonded_prefix = c->d_pref + 5 * stride;
-  L.gb_prefix = c->d_pref + 6 * stride;
-  L.n_atoms_total = p.atom.back();
-  L.n_cl_total = p.cl.back();
-  L.n_cols_total = p.col.back();
-  L.n_slots_total = p.slot.back();
-  L.n_corr_total = p.corr.back();
-  L.n_bonded_total = p.bonded.back();
-  L.n_gb_tiles_total = p.gb.back();
-  c->Ls.resize(ns);
-  for (int s = 0; s < ns; ++s) {
-    Launch &l = c->Ls[s];
-    const int *b = c->d_single + 14 * s;
-    const SysHost &h = c->sys[s];
-    l.stream = c->stream;
-    l.sys = c->d_sys + s;
-    l.nsys = 1;
-    l.atom_prefix = b;
-    l.cl_prefix = b + 2;
-    l.col_prefix = b + 4;
-    l.slot_prefix = b + 6;
-    l.corr_prefix = b + 8;
-    l.bonded_prefix = b + 10;
-    l.gb_prefix = b + 12;
-    l.n_atoms_total = h.n;
-    l.n_cl_total = h.cl_cap;
-    l.n_cols_total = h.ncol;
-    l.n_slots_total = h.slot_cap;
-    l.n_corr_total = h.n_corr;
-    l.n_bonded_total = h.n_bonded;
-    l.n_gb_tiles_total = h.n_gb_tiles;
-  }
+  L.atom_prefix = d_pre;
+  L.cl_prefix = d_pre + st;
+  L.col_prefix = d_pre + 2 * st;
+  L.slot_prefix = d_pre + 3 * st;
+  L.corr_prefix = d_pre + 4 * st;
+  L.bonded_prefix = d_pre + 5 * st;
+  L.gb_prefix = d_pre + 6 * st;
+  L.n_atoms_total = pre[0].back();
+  L.n_cl_total = pre[1].back();
+  L.n_cols_total = pre[2].back();
+  L.n_slots_total = pre[3].back();
+  L.n_corr_total = pre[4].back();
+  L.n_bonded_total = pre[5].back();
+  L.n_gb_tiles_total = pre[6].back();
+  L.n_local_total = pre[7].back();
+  LaunchSet &ref = *ls;
+  c->sets[ids] = std::move(ls);
+  return ref;
 }
 
-// one force evaluation of the systems in `L` at the positions already in S.x
-hipEvent_t next_event(pm_ctx *c) {
-  if (c->evs_used == (int)c->evs.size()) {
-    hipEvent_t e;
-    HIPCHECK(hipEventCreate(&e));
-    c->evs.push_back(e);
-  }
-  return c->evs[c->evs_used++];
-}
+// ------------------------------------------------------------ force evaluation
 
-// Rebuild policy of an MD run: the list (radius rc + skin) is rebuilt when
-// an atom has moved more than skin/2 from where it was built, or after
+// Rebuild policy of MD: the list (radius rc + skin) is rebuilt when an atom
+// has moved more than skin/2 from where it was built, or after
 // rebuild_interval steps when that is > 0.
 struct StepPolicy {
   float rlist = 0.f, lim2 = 0.f;
@@ -532,56 +792,67 @@ StepPolicy md_policy(const pm_ctx *c) {
   return p;
 }
 
-// one force evaluation of the systems in `L` at the positions already in S.x.
+StepPolicy eval_policy(const pm_ctx *c) {
+  StepPolicy p;
+  p.rlist = (float)(c->rc + c->skin_eval);
+  return p;
+}
+
+hipEvent_t next_event(pm_ctx *c) {
+  if (c->evs_used == (int)c->evs.size()) {
+    hipEvent_t e;
+    HIPCHECK(hipEventCreate(&e));
+    c->evs.push_back(e);
+  }
+  return c->evs[c->evs_used++];
+}
+
+// One force evaluation of the systems of L at the positions in S.x.
 // mode kStepForce rebuilds the list and zeroes the forces first; kStepAuto
-// (MD steps, whose kick has consumed and zeroed the forces) leaves the
-// rebuild decision to the device.
-void force_eval(pm_ctx *c, const Launch &L, int mode, const StepPolicy &pol, bool energies, bool census, bool timed) {
+// (MD steps, whose kick has consumed and zeroed the forces) leaves the rebuild
+// decision to the device.  kind: pm::kPair* (what the pair kernel adds).
+// gb_energies: the GB passes also sum energies (GB has no separate energy pass).
+void force_eval(pm_ctx *c, const Launch &L, int mode, const StepPolicy &pol, int kind, bool gb_energies, bool timed) {
   pm::launch_step_begin(L, mode, pol.lim2, pol.kmax);
   const bool pool = !timed && (c->desc.flags & PM_FLAG_TIMING);
   if (mode & pm::kStepForce) pm::launch_zero_forces(L);
+  const bool local_en = kind == pm::kPairForcesEnergies || (c->gb && gb_energies);
+  const int local = pm::kLocalForces | (local_en ? pm::kLocalEnergies : 0);
   if (c->gb) {
     if (timed) HIPCHECK(hipEventRecord(c->evp0, c->stream));
     if (pool) HIPCHECK(hipEventRecord(next_event(c), c->stream));
-    pm::launch_gb(L, c->P, energies);
+    pm::launch_gb(L, c->P, kind == pm::kPairForcesEnergies || gb_energies);
     if (pool) HIPCHECK(hipEventRecord(next_event(c), c->stream));
     if (timed) HIPCHECK(hipEventRecord(c->evp1, c->stream));
-    pm::launch_bonded(L, energies);
+    pm::launch_local(L, c->P, local);
     return;
   }
   pm::launch_rebuild(L, pol.rlist * pol.rlist);
   if (timed) HIPCHECK(hipEventRecord(c->evp0, c->stream));
   if (pool) HIPCHECK(hipEventRecord(next_event(c), c->stream));
-  if (!pm::launch_pair(L, c->P, energies, census)) throw Fail{PM_EINVAL, "unsupported term combination"};
+  if (!pm::launch_pair(L, c->P, kind)) throw Fail{PM_EINVAL, "unsupported term combination"};
   if (pool) HIPCHECK(hipEventRecord(next_event(c), c->stream));
   if (timed) HIPCHECK(hipEventRecord(c->evp1, c->stream));
-  pm::launch_ewald_corr(L, c->P, energies);
-  pm::launch_bonded(L, energies);
+  pm::launch_local(L, c->P, local);
 }
 
-struct Readback {
-  std::vector<long long> acc;
-  std::vector<unsigned long long> census;
-  std::vector<unsigned> err;
-};
+// energies and virial at the current positions with the current pair list
+// (no forces): the pair kernel's energy-only form and the local terms' sums
+void energy_pass(pm_ctx *c, const Launch &L) {
+  if (c->gb) return;  // GB sums energies inside its force passes
+  if (!pm::launch_pair(L, c->P, pm::kPairEnergies)) throw Fail{PM_EINVAL, "unsupported term combination"};
+  pm::launch_local(L, c->P, pm::kLocalEnergies);
+}
 
-Readback read_scalars(pm_ctx *c, int s0, int s1) {
-  Readback r;
-  r.acc.resize((size_t)(s1 - s0) * pm::kAccN);
-  r.census.resize(2 * (size_t)(s1 - s0));
-  r.err.resize(s1 - s0);
-  for (int s = s0; s < s1; ++s) {
-    const SysDev &S = c->sys[s].dev;
-    HIPCHECK(hipMemcpyAsync(&r.acc[(size_t)(s - s0) * pm::kAccN], S.acc, pm::kAccN * sizeof(long long), hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(hipMemcpyAsync(&r.census[2 * (size_t)(s - s0)], S.census, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(hipMemcpyAsync(&r.err[s - s0], S.err, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
-  }
+// copy the packed scalars of the set's systems to the host (synchronizes)
+void read_scalars(pm_ctx *c, LaunchSet &ls) {
+  pm::launch_gather_scalars(ls.L, ls.d_scal);
+  HIPCHECK(hipMemcpyAsync(ls.h_scal, ls.d_scal, ls.ids.size() * pm::kScalN * sizeof(long long),
+                          hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
-  return r;
 }
 
-void add_energies(pm_ctx *c, int s, const long long *acc, double *e) {
-  const SysHost &h = c->sys[s];
+void add_energies(const pm_ctx *c, const SysHost &h, const long long *acc, double *e) {
   const double k = pm::kEScaleInv;
   e[PM_E_COULOMB] += (double)acc[pm::kAccCoul] * k + h.coul_const;
   e[PM_E_LENNARDJONES] += (double)acc[pm::kAccLJ] * k;
@@ -598,28 +869,267 @@ void add_energies(pm_ctx *c, int s, const long long *acc, double *e) {
   }
 }
 
-// raise on device status bits; grows a full pair list and reports it
-bool check_errors(pm_ctx *c, int s0, const std::vector<unsigned> &err) {
-  bool regrow = false;
-  for (size_t k = 0; k < err.size(); ++k) {
-    const unsigned e = err[k];
+// Raise on device status bits; grow the pair lists that overflowed.
+// Returns the ids whose lists grew (their results must be recomputed).
+std::vector<int> check_errors(pm_ctx *c, LaunchSet &ls) {
+  std::vector<int> grown;
+  bool any = false;
+  for (size_t k = 0; k < ls.ids.size(); ++k) {
+    const unsigned e = (unsigned)ls.h_scal[k * pm::kScalN + pm::kScalErr];
     if (!e) continue;
-    const int s = s0 + (int)k;
-    HIPCHECK(hipMemset(c->sys[s].dev.err, 0, sizeof(unsigned)));
-    if (e & pm::kErrNonFinite) throw Fail{PM_EINVAL, "system " + std::to_string(s) + ": non-finite position"};
-    if (e & pm::kErrColumnCap)
-      throw Fail{PM_ECAPACITY, "system " + std::to_string(s) + ": more than " + std::to_string(pm::kSortCap) +
+    any = true;
+    const int id = ls.ids[k];
+    if (e & pm::kErrNonFinite) { pm::launch_clear_err(ls.L); throw Fail{PM_EINVAL, "system " + std::to_string(id) + ": non-finite position"}; }
+    if (e & pm::kErrColumnCap) {
+      pm::launch_clear_err(ls.L);
+      throw Fail{PM_ECAPACITY, "system " + std::to_string(id) + ": more than " + std::to_string(pm::kSortCap) +
                                    " atoms in one cell column"};
-    if (e & pm::kErrClusterCap) throw Fail{PM_ECAPACITY, "system " + std::to_string(s) + ": cluster capacity"};
-    if (e & pm::kErrListCap) {
-      SysHost &h = c->sys[s];
-      h.list_cap *= 2;
-      alloc_lists(h);
-      upload_sys(c, s);
-      regrow = true;
     }
+    if (e & pm::kErrClusterCap) { pm::launch_clear_err(ls.L); throw Fail{PM_ECAPACITY, "system " + std::to_string(id) + ": cluster capacity"}; }
+    if (e & pm::kErrListCap) grown.push_back(id);
   }
-  return regrow;
+  if (!any) return grown;
+  pm::launch_clear_err(ls.L);
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  for (int id : grown) {
+    SysHost &h = *c->sys[id];
+    h.list_cap *= 2;
+    alloc_lists(c, h);
+  }
+  drop_sets(c);  // the device records changed
+  return grown;
+}
+
+// H2D of concatenated AoS host arrays into the systems' staging buffers
+void put_staging(pm_ctx *c, const std::vector<int> &ids, const double *x, const double *v) {
+  size_t off = 0;
+  for (int id : ids) {
+    const SysHost &h = *c->sys[id];
+    const size_t b = 3 * (size_t)h.n * sizeof(double);
+    if (x) HIPCHECK(hipMemcpyAsync(h.dev.xaos, x + 3 * off, b, hipMemcpyHostToDevice, c->stream));
+    if (v) HIPCHECK(hipMemcpyAsync(h.dev.vaos, v + 3 * off, b, hipMemcpyHostToDevice, c->stream));
+    off += h.n;
+  }
+}
+
+void get_staging(pm_ctx *c, const std::vector<int> &ids, double *x, double *v, double *f) {
+  size_t off = 0;
+  for (int id : ids) {
+    const SysHost &h = *c->sys[id];
+    const size_t b = 3 * (size_t)h.n * sizeof(double);
+    if (x) HIPCHECK(hipMemcpyAsync(x + 3 * off, h.dev.xaos, b, hipMemcpyDeviceToHost, c->stream));
+    if (v) HIPCHECK(hipMemcpyAsync(v + 3 * off, h.dev.vaos, b, hipMemcpyDeviceToHost, c->stream));
+    if (f) HIPCHECK(hipMemcpyAsync(f + 3 * off, h.dev.faos, b, hipMemcpyDeviceToHost, c->stream));
+    off += h.n;
+  }
+}
+
+size_t atoms_of(const pm_ctx *c, const std::vector<int> &ids) {
+  size_t n = 0;
+  for (int id : ids) n += c->sys[id]->n;
+  return n;
+}
+
+// SystemForce::evaluate for the systems `ids` at host positions x: ADDS into f, e
+void evaluate_ids(pm_ctx *c, const std::vector<int> &ids, const double *x, double *f, double *e, bool census,
+                  uint64_t *count, uint64_t *hash) {
+  for (int id : ids) c->sys[id]->md_ready = false;  // the positions are overwritten
+  put_staging(c, ids, x, nullptr);
+  for (int attempt = 0;; ++attempt) {
+    LaunchSet &ls = get_set(c, ids);
+    HIPCHECK(hipEventRecord(c->ev0, c->stream));
+    pm::launch_import(ls.L, pm::kXferX);
+    force_eval(c, ls.L, pm::kStepForce, eval_policy(c), census ? pm::kPairCensus : pm::kPairForcesEnergies, false,
+               true);
+    HIPCHECK(hipEventRecord(c->ev1, c->stream));
+    read_scalars(c, ls);
+    if (!check_errors(c, ls).empty()) {
+      if (attempt > 4) throw Fail{PM_ECAPACITY, "pair list capacity kept overflowing"};
+      continue;
+    }
+    pm::launch_export(ls.L, pm::kXferF);
+    const size_t n = atoms_of(c, ids);
+    std::vector<double> fo(f ? 3 * n : 0);
+    get_staging(c, ids, nullptr, nullptr, f ? fo.data() : nullptr);
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    if (f)
+      for (size_t k = 0; k < fo.size(); ++k) f[k] += fo[k];
+    if (e)
+      for (size_t k = 0; k < ids.size(); ++k) add_energies(c, *c->sys[ids[k]], &ls.h_scal[k * pm::kScalN], e + k * PM_E_NSLOTS);
+    if (census) {
+      *count = (uint64_t)ls.h_scal[pm::kScalCensus];
+      *hash = (uint64_t)ls.h_scal[pm::kScalCensus + 1];
+    }
+    float ms = 0.f, pms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    HIPCHECK(hipEventElapsedTime(&pms, c->evp0, c->evp1));
+    c->stats[0] = 1 + attempt;
+    c->stats[3] = ms;
+    c->stats[4] = pms;
+    c->stats[5] = 1;
+    return;
+  }
+}
+
+// StandardIntegrator::initializeForces: positions/velocities in, F(x0) computed
+void set_state_ids(pm_ctx *c, const std::vector<int> &ids, const double *x, const double *v) {
+  put_staging(c, ids, x, v);
+  for (int attempt = 0;; ++attempt) {
+    LaunchSet &ls = get_set(c, ids);
+    pm::launch_import(ls.L, pm::kXferX | pm::kXferV);
+    force_eval(c, ls.L, pm::kStepForce, md_policy(c), pm::kPairForces, false, false);
+    read_scalars(c, ls);
+    if (!check_errors(c, ls).empty()) {
+      if (attempt > 4) throw Fail{PM_ECAPACITY, "pair list capacity kept overflowing"};
+      continue;
+    }
+    break;
+  }
+  for (int id : ids) c->sys[id]->md_ready = true;
+}
+
+void require_md(pm_ctx *c, const std::vector<int> &ids) {
+  for (int id : ids)
+    if (!c->sys[id]->md_ready) throw Fail{PM_ENOTREADY, "system " + std::to_string(id) + ": set the MD state first"};
+}
+
+constexpr int kGraphSteps = 10;
+
+// kGraphSteps MD steps of set ls for time step h, captured on first use.
+// Kernel arguments are the set's own device tables, which live as long as the
+// set does, so the graph stays valid with it.
+hipGraphExec_t md_graph(pm_ctx *c, LaunchSet &ls, double h, const StepPolicy &pol) {
+  if (ls.gexec && ls.g_h == h) return ls.gexec;
+  if (ls.gexec) {
+    HIPCHECK(hipGraphExecDestroy(ls.gexec));
+    ls.gexec = nullptr;
+  }
+  hipGraph_t g = nullptr;
+  HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  try {
+    for (int k = 0; k < kGraphSteps; ++k) {
+      pm::launch_kick_drift(ls.L, h);
+      force_eval(c, ls.L, pm::kStepAuto, pol, pm::kPairForces, false, false);
+    }
+  } catch (...) {
+    (void)hipStreamEndCapture(c->stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  HIPCHECK(hipStreamEndCapture(c->stream, &g));
+  hipError_t e = hipGraphInstantiate(&ls.gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    ls.gexec = nullptr;
+    throw Fail{PM_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e)};
+  }
+  ls.g_h = h;
+  return ls.gexec;
+}
+
+// LeapfrogIntegrator::run (LeapfrogIntegrator.cpp:89-104) in three parts:
+// md_begin arms the first kick as a half kick, md_advance runs n steps of
+// (kick + drift, forces), md_end applies the closing half kick.
+void md_begin(pm_ctx *c, const std::vector<int> &ids) {
+  require_md(c, ids);
+  pm::launch_md_arm(get_set(c, ids).L);
+}
+
+void md_advance(pm_ctx *c, const std::vector<int> &ids, int n_steps, double dt_fs, bool energies_last, int depth = 0) {
+  require_md(c, ids);
+  if (n_steps < 1) return;
+  if (!(dt_fs > 0.0)) throw Fail{PM_EINVAL, "dt must be > 0"};
+  const double h = dt_fs * kInvTimeFactor;
+  const StepPolicy pol = md_policy(c);
+  const bool timing = (c->desc.flags & PM_FLAG_TIMING) != 0;
+  {
+    LaunchSet &ls = get_set(c, ids);
+    // state at the start: a pair list that overflows during the advance
+    // (capacity grown) sends its system back here to run the advance again
+    pm::launch_export(ls.L, pm::kXferX | pm::kXferV);
+    for (int id : ids)
+      HIPCHECK(hipMemcpyAsync(c->sys[id]->ctl_snap, c->sys[id]->dev.ctl, pm::kCtlN * sizeof(int),
+                              hipMemcpyDeviceToDevice, c->stream));
+    if (depth == 0) {
+      HIPCHECK(hipEventRecord(c->ev0, c->stream));
+      c->evs_used = 0;
+    }
+    int i = 0;
+    if (!timing)
+      while (n_steps - i - (energies_last ? 1 : 0) >= kGraphSteps) {
+        HIPCHECK(hipGraphLaunch(md_graph(c, ls, h, pol), c->stream));
+        i += kGraphSteps;
+      }
+    for (; i < n_steps; ++i) {
+      const bool en = energies_last && i == n_steps - 1;
+      pm::launch_kick_drift(ls.L, h);
+      force_eval(c, ls.L, pm::kStepAuto, pol, pm::kPairForces, en, false);
+      if (en) energy_pass(c, ls.L);
+    }
+    if (depth == 0) HIPCHECK(hipEventRecord(c->ev1, c->stream));
+    read_scalars(c, ls);
+  }
+  LaunchSet &ls = get_set(c, ids);
+  std::vector<long long> scal(ls.h_scal, ls.h_scal + ids.size() * pm::kScalN);
+  const std::vector<int> grown = check_errors(c, ls);
+  if (!grown.empty()) {
+    if (depth > 4) throw Fail{PM_ECAPACITY, "pair list capacity kept overflowing"};
+    // restore the grown systems to the start of the advance and run it again
+    LaunchSet &rs = get_set(c, grown);
+    pm::launch_import(rs.L, pm::kXferX | pm::kXferV);
+    force_eval(c, rs.L, pm::kStepForce, pol, pm::kPairForces, false, false);
+    for (int id : grown)
+      HIPCHECK(hipMemcpyAsync(c->sys[id]->dev.ctl + pm::kCtlCount, c->sys[id]->ctl_snap + pm::kCtlCount,
+                              2 * sizeof(int), hipMemcpyDeviceToDevice, c->stream));
+    md_advance(c, grown, n_steps, dt_fs, energies_last, depth + 1);
+  }
+  if (depth == 0) {
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    LaunchSet &fs = get_set(c, ids);
+    read_scalars(c, fs);
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    double pair_ms = 0.0;
+    for (int k = 0; k + 1 < c->evs_used; k += 2) {
+      float t = 0.f;
+      HIPCHECK(hipEventElapsedTime(&t, c->evs[k], c->evs[k + 1]));
+      pair_ms += t;
+    }
+    unsigned worst = 0;
+    double rebuilds = 0;
+    for (size_t k = 0; k < ids.size(); ++k) {
+      worst = std::max(worst, (unsigned)fs.h_scal[k * pm::kScalN + pm::kScalMaxdispRun]);
+      rebuilds += (double)fs.h_scal[k * pm::kScalN + pm::kScalCtl + pm::kCtlCount];
+    }
+    float w;
+    std::memcpy(&w, &worst, sizeof w);
+    c->stats[0] = rebuilds;
+    c->stats[1] = grown.empty() ? 0 : 1;
+    c->stats[3] = ms;
+    c->stats[4] = pair_ms;
+    c->stats[5] = c->evs_used / 2;
+    c->stats[6] = std::sqrt((double)w);
+    c->stats[7] = c->skin_md;
+  }
+  (void)scal;
+}
+
+void md_end(pm_ctx *c, const std::vector<int> &ids, double dt_fs) {
+  require_md(c, ids);
+  // StandardIntegrator::doHalfKick (StandardIntegrator.cpp:80-92)
+  pm::launch_half_kick(get_set(c, ids).L, 0.5 * dt_fs * kInvTimeFactor);
+}
+
+void get_state_ids(pm_ctx *c, const std::vector<int> &ids, double *x, double *v, double *f, double *e) {
+  require_md(c, ids);
+  LaunchSet &ls = get_set(c, ids);
+  pm::launch_export(ls.L, pm::kXferX | pm::kXferV | pm::kXferF);
+  get_staging(c, ids, x, v, f);
+  read_scalars(c, ls);
+  if (e) {
+    std::memset(e, 0, sizeof(double) * PM_E_NSLOTS * ids.size());
+    for (size_t k = 0; k < ids.size(); ++k) add_energies(c, *c->sys[ids[k]], &ls.h_scal[k * pm::kScalN], e + k * PM_E_NSLOTS);
+  }
 }
 
 int run_guarded(pm_ctx *c, const std::function<void()> &fn) {
@@ -638,85 +1148,6 @@ int run_guarded(pm_ctx *c, const std::function<void()> &fn) {
   }
 }
 
-// evaluate systems [s0, s1) through launch L at host positions x; add into f, e
-void evaluate_range(pm_ctx *c, const Launch &L, int s0, int s1, const double *x, double *f, double *e, bool census,
-                    uint64_t *count, uint64_t *hash) {
-  // an evaluation overwrites the device positions: any MD state is gone
-  c->md_ready = false;
-  int off = 0;
-  for (int s = 0; s < s0; ++s) off += c->sys[s].n;
-  int n = 0;
-  for (int s = s0; s < s1; ++s) n += c->sys[s].n;
-  HIPCHECK(hipEventRecord(c->ev0, c->stream));
-  HIPCHECK(hipMemcpyAsync(c->d_xaos + 3 * (size_t)off, x, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  // the kernels index the staging buffer from the launch's first atom
-  pm::launch_import_x(L, c->d_xaos + 3 * (size_t)off);
-  for (int attempt = 0;; ++attempt) {
-    StepPolicy pol;
-    pol.rlist = (float)(c->rc + c->skin_eval);
-    force_eval(c, L, pm::kStepForce, pol, true, census, true);
-    HIPCHECK(hipEventRecord(c->ev1, c->stream));
-    Readback r = read_scalars(c, s0, s1);
-    if (check_errors(c, s0, r.err)) {
-      if (attempt > 4) throw Fail{PM_ECAPACITY, "pair list capacity kept overflowing"};
-      continue;
-    }
-    pm::launch_export_f(L, c->d_faos + 3 * (size_t)off);
-    std::vector<double> fo(3 * (size_t)n);
-    HIPCHECK(hipMemcpyAsync(fo.data(), c->d_faos + 3 * (size_t)off, fo.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(hipStreamSynchronize(c->stream));
-    if (f)
-      for (size_t k = 0; k < fo.size(); ++k) f[k] += fo[k];
-    if (e)
-      for (int s = s0; s < s1; ++s) add_energies(c, s, &r.acc[(size_t)(s - s0) * pm::kAccN], e + (size_t)(s - s0) * PM_E_NSLOTS);
-    if (census) {
-      *count = r.census[0];
-      *hash = r.census[1];
-    }
-    float ms = 0.f, pms = 0.f;
-    HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    HIPCHECK(hipEventElapsedTime(&pms, c->evp0, c->evp1));
-    c->stats[3] = ms;
-    c->stats[4] = pms;
-    c->stats[5] = 1;
-    c->stats[0] = 1 + attempt;
-    return;
-  }
-}
-
-constexpr int kGraphSteps = 10;
-
-// The graph of kGraphSteps MD steps (kick + drift, then forces with the
-// device's own rebuild decision) for time step h, captured on first use.
-// Kernel arguments are the context's fixed device tables, so a graph stays
-// valid when pair lists are regrown (only the tables' contents change).
-template <class Step>
-hipGraphExec_t md_graph(pm_ctx *c, double h, const StepPolicy &, Step &step) {
-  if (c->gexec && c->g_h == h) return c->gexec;
-  if (c->gexec) {
-    HIPCHECK(hipGraphExecDestroy(c->gexec));
-    c->gexec = nullptr;
-  }
-  hipGraph_t g = nullptr;
-  HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  try {
-    for (int k = 0; k < kGraphSteps; ++k) step(1, false);
-  } catch (...) {
-    (void)hipStreamEndCapture(c->stream, &g);
-    if (g) (void)hipGraphDestroy(g);
-    throw;
-  }
-  HIPCHECK(hipStreamEndCapture(c->stream, &g));
-  hipError_t e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(g);
-  if (e != hipSuccess) {
-    c->gexec = nullptr;
-    throw Fail{PM_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e)};
-  }
-  c->g_h = h;
-  return c->gexec;
-}
-
 }  // namespace
 
 extern "C" {
@@ -731,9 +1162,7 @@ int pm_device_info(char *name, int32_t name_len, int32_t *n_cu, double *hbm_gib)
     HIPCHECK(hipGetDevice(&dev));
     hipDeviceProp_t p;
     HIPCHECK(hipGetDeviceProperties(&p, dev));
-    if (name && name_len > 0) {
-      std::snprintf(name, (size_t)name_len, "%s (%s)", p.name, p.gcnArchName);
-    }
+    if (name && name_len > 0) std::snprintf(name, (size_t)name_len, "%s (%s)", p.name, p.gcnArchName);
     if (n_cu) *n_cu = p.multiProcessorCount;
     if (hbm_gib) *hbm_gib = (double)p.totalGlobalMem / (1024.0 * 1024.0 * 1024.0);
   });
@@ -744,7 +1173,7 @@ int pm_ctx_create(const pm_force_desc *force, int32_t n_systems, const pm_system
   *out = nullptr;
   pm_ctx *c = new pm_ctx();
   int rc = run_guarded(c, [&] {
-    if (!force || n_systems < 1 || !systems) throw Fail{PM_EINVAL, "need a force description and >= 1 system"};
+    if (!force || n_systems < 0 || (n_systems > 0 && !systems)) throw Fail{PM_EINVAL, "need a force description"};
     c->desc = *force;
     HIPCHECK(hipGetDevice(&c->device));
     HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -793,29 +1222,23 @@ int pm_ctx_create(const pm_force_desc *force, int32_t n_systems, const pm_system
       c->rc = std::sqrt(cmax);
       c->skin_md = force->skin > 0 ? force->skin : 2.0;
       c->skin_eval = 0.1;
-    }
-    if (!c->gb) {
+      if (force->rebuild_interval < 0) throw Fail{PM_EINVAL, "rebuild_interval must be >= 0"};
       // the pair list radius rc + skin must stay below half the smallest box
       // edge (one image per pair); a larger requested skin is reduced to fit
-      double minL = 1e300;
-      for (int s = 0; s < n_systems; ++s)
-        for (int k = 0; k < 3; ++k) minL = std::min(minL, systems[s].box[k]);
-      const double room = 0.5 * minL - c->rc - 0.01;
-      if (room < 0.05)
-        throw Fail{PM_EINVAL, "box edge " + std::to_string(minL) + " leaves no room for a pair list beyond rc " +
-                                  std::to_string(c->rc) + " (needs box > 2 rc)"};
-      c->skin_md = std::min(c->skin_md, room);
-      c->skin_eval = std::min(c->skin_eval, room);
+      // the systems given here
+      if (n_systems > 0) {
+        double minL = 1e300;
+        for (int s = 0; s < n_systems; ++s)
+          for (int k = 0; k < 3; ++k) minL = std::min(minL, systems[s].box[k]);
+        const double room = 0.5 * minL - c->rc - 0.01;
+        if (room < 0.05)
+          throw Fail{PM_EINVAL, "box edge " + std::to_string(minL) + " leaves no room for a pair list beyond rc " +
+                                    std::to_string(c->rc) + " (needs box > 2 rc)"};
+        c->skin_md = std::min(c->skin_md, room);
+        c->skin_eval = std::min(c->skin_eval, room);
+      }
     }
-    c->sys.resize(n_systems);
-    for (int s = 0; s < n_systems; ++s) setup_system(c, c->sys[s], systems[s]);
-    c->d_sys = c->common.get<SysDev>(n_systems);
-    for (int s = 0; s < n_systems; ++s) upload_sys(c, s);
-    for (auto &h : c->sys) c->n_total += h.n;
-    c->d_xaos = c->common.get<double>(3 * (size_t)c->n_total);
-    c->d_vaos = c->common.get<double>(3 * (size_t)c->n_total);
-    c->d_faos = c->common.get<double>(3 * (size_t)c->n_total);
-    build_launches(c);
+    for (int s = 0; s < n_systems; ++s) add_system(c, systems[s]);
     HIPCHECK(hipDeviceSynchronize());
   });
   if (rc != PM_OK) {
@@ -830,25 +1253,65 @@ int pm_ctx_create(const pm_force_desc *force, int32_t n_systems, const pm_system
 int pm_ctx_destroy(pm_ctx *c) {
   if (!c) return PM_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (auto &h : c->sys) {
-    h.lists.release();
-    h.main.release();
-  }
-  c->common.release();
+  c->sets.clear();
+  for (auto &h : c->sys)
+    if (h) {
+      dev_free(c, h->lists, h->lists_bytes);
+      dev_free(c, h->block, h->block_bytes);
+    }
+  c->sys.clear();
+  c->arena.destroy();
   for (hipEvent_t e : {c->ev0, c->ev1, c->evp0, c->evp1})
     if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : c->evs) (void)hipEventDestroy(e);
-  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return PM_OK;
 }
 
+int pm_ctx_reserve(pm_ctx *c, uint64_t bytes) {
+  if (!c) return PM_EINVAL;
+  return run_guarded(c, [&] {
+    if (!live_ids(c).empty() || c->arena.base) throw Fail{PM_EINVAL, "reserve before adding systems, once"};
+    if (bytes == 0) throw Fail{PM_EINVAL, "reservation of 0 bytes"};
+    c->arena.init(bytes);
+  });
+}
+
+int pm_ctx_memory(pm_ctx *c, uint64_t *reserved, uint64_t *used) {
+  if (!c) return PM_EINVAL;
+  if (reserved) *reserved = c->arena.cap;
+  if (used) *used = c->arena.used;
+  return PM_OK;
+}
+
+int pm_system_footprint(pm_ctx *c, const pm_system_desc *system, uint64_t *bytes) {
+  if (!c) return PM_EINVAL;
+  return run_guarded(c, [&] {
+    if (!system || !bytes) throw Fail{PM_EINVAL, "system and bytes required"};
+    *bytes = footprint(c, *system);
+  });
+}
+
+int pm_system_add(pm_ctx *c, const pm_system_desc *system, int32_t *id) {
+  if (!c) return PM_EINVAL;
+  return run_guarded(c, [&] {
+    if (!system || !id) throw Fail{PM_EINVAL, "system and id required"};
+    *id = add_system(c, *system);
+  });
+}
+
+int pm_system_remove(pm_ctx *c, int32_t id) {
+  if (!c) return PM_EINVAL;
+  return run_guarded(c, [&] { remove_system(c, id); });
+}
+
 int pm_evaluate(pm_ctx *c, int32_t system, const double *x, double *f, double *e) {
   if (!c) return PM_EINVAL;
   return run_guarded(c, [&] {
-    if (system < 0 || system >= (int)c->sys.size() || !x) throw Fail{PM_EINVAL, "bad system index or positions"};
-    evaluate_range(c, c->Ls[system], system, system + 1, x, f, e, false, nullptr, nullptr);
+    if (!x) throw Fail{PM_EINVAL, "positions required"};
+    sys_of(c, system);
+    evaluate_ids(c, {system}, x, f, e, false, nullptr, nullptr);
   });
 }
 
@@ -856,17 +1319,27 @@ int pm_evaluate_batch(pm_ctx *c, const double *x, double *f, double *e) {
   if (!c) return PM_EINVAL;
   return run_guarded(c, [&] {
     if (!x) throw Fail{PM_EINVAL, "positions required"};
-    evaluate_range(c, c->L, 0, (int)c->sys.size(), x, f, e, false, nullptr, nullptr);
+    const std::vector<int> ids = live_ids(c);
+    if (ids.empty()) throw Fail{PM_EINVAL, "no systems"};
+    evaluate_ids(c, ids, x, f, e, false, nullptr, nullptr);
+  });
+}
+
+int pm_evaluate_systems(pm_ctx *c, int32_t n, const int32_t *ids, const double *x, double *f, double *e) {
+  if (!c) return PM_EINVAL;
+  return run_guarded(c, [&] {
+    if (!x) throw Fail{PM_EINVAL, "positions required"};
+    evaluate_ids(c, check_ids(c, n, ids), x, f, e, false, nullptr, nullptr);
   });
 }
 
 int pm_pair_census(pm_ctx *c, int32_t system, const double *x, uint64_t *count, uint64_t *hash) {
   if (!c) return PM_EINVAL;
   return run_guarded(c, [&] {
-    if (system < 0 || system >= (int)c->sys.size() || !x || !count || !hash)
-      throw Fail{PM_EINVAL, "bad census arguments"};
+    if (!x || !count || !hash) throw Fail{PM_EINVAL, "bad census arguments"};
+    sys_of(c, system);
     if (c->gb) throw Fail{PM_EINVAL, "the all-pairs GB path evaluates every pair; no census"};
-    evaluate_range(c, c->Ls[system], system, system + 1, x, nullptr, nullptr, true, count, hash);
+    evaluate_ids(c, {system}, x, nullptr, nullptr, true, count, hash);
   });
 }
 
@@ -874,119 +1347,65 @@ int pm_md_set_state(pm_ctx *c, const double *x, const double *v) {
   if (!c) return PM_EINVAL;
   return run_guarded(c, [&] {
     if (!x || !v) throw Fail{PM_EINVAL, "positions and velocities required"};
-    const size_t bytes = 3 * (size_t)c->n_total * sizeof(double);
-    HIPCHECK(hipMemcpyAsync(c->d_xaos, x, bytes, hipMemcpyHostToDevice, c->stream));
-    HIPCHECK(hipMemcpyAsync(c->d_vaos, v, bytes, hipMemcpyHostToDevice, c->stream));
-    pm::launch_import_x(c->L, c->d_xaos);
-    pm::launch_import_v(c->L, c->d_vaos);
-    // StandardIntegrator::initializeForces: F(x0) before the first step
-    for (int attempt = 0;; ++attempt) {
-      force_eval(c, c->L, pm::kStepForce, md_policy(c), false, false, false);
-      Readback r = read_scalars(c, 0, (int)c->sys.size());
-      if (check_errors(c, 0, r.err)) {
-        if (attempt > 4) throw Fail{PM_ECAPACITY, "pair list capacity kept overflowing"};
-        continue;
-      }
-      break;
-    }
-    c->md_ready = true;
+    const std::vector<int> ids = live_ids(c);
+    if (ids.empty()) throw Fail{PM_EINVAL, "no systems"};
+    set_state_ids(c, ids, x, v);
+  });
+}
+
+int pm_md_set_state_systems(pm_ctx *c, int32_t n, const int32_t *ids, const double *x, const double *v) {
+  if (!c) return PM_EINVAL;
+  return run_guarded(c, [&] {
+    if (!x || !v) throw Fail{PM_EINVAL, "positions and velocities required"};
+    set_state_ids(c, check_ids(c, n, ids), x, v);
   });
 }
 
 int pm_md_run(pm_ctx *c, int32_t n_steps, double dt_fs, int32_t energies_every) {
   if (!c) return PM_EINVAL;
   return run_guarded(c, [&] {
-    if (!c->md_ready) throw Fail{PM_ENOTREADY, "pm_md_set_state first"};
+    const std::vector<int> ids = live_ids(c);
+    if (ids.empty()) throw Fail{PM_EINVAL, "no systems"};
+    require_md(c, ids);
     if (n_steps < 1) return;
     if (!(dt_fs > 0.0)) throw Fail{PM_EINVAL, "dt must be > 0"};
-    // LeapfrogIntegrator::run (LeapfrogIntegrator.cpp:89-104): half kick + drift,
-    // forces, then (kick + drift, forces) per step, closing half kick
-    const double h = dt_fs * kInvTimeFactor;
-    const StepPolicy pol = md_policy(c);
-    const bool timing = (c->desc.flags & PM_FLAG_TIMING) != 0;
-    const bool en = energies_every > 0;
-    const int ns = (int)c->sys.size();
-    // snapshot of x, v: a pair-list overflow (capacity grown) re-runs the call
-    pm::launch_export_xv(c->L, c->d_xaos, c->d_vaos);
-    for (int attempt = 0;; ++attempt) {
-      HIPCHECK(hipEventRecord(c->ev0, c->stream));
-      c->evs_used = 0;
-      auto step = [&](int i, bool energies) {
-        pm::launch_kick_drift(c->L, h, i == 0 ? 0 : 1);
-        force_eval(c, c->L, pm::kStepAuto | (i == 0 ? pm::kStepResetRun : 0), pol, energies, false, false);
-      };
-      int i = 0;
-      step(i++, en && n_steps == 1);
-      if (!timing) {
-        // the steps between the first and the last replay a captured graph
-        while (n_steps - 1 - i >= kGraphSteps) {
-          HIPCHECK(hipGraphLaunch(md_graph(c, h, pol, step), c->stream));
-          i += kGraphSteps;
-        }
-      }
-      for (; i < n_steps; ++i) step(i, en && i == n_steps - 1);
-      pm::launch_half_kick(c->L, 0.5 * dt_fs * kInvTimeFactor);
-      HIPCHECK(hipEventRecord(c->ev1, c->stream));
-      std::vector<unsigned> md(ns);
-      std::vector<int> ctl(pm::kCtlN * (size_t)ns);
-      for (int s = 0; s < ns; ++s) {
-        HIPCHECK(hipMemcpyAsync(&md[s], c->sys[s].dev.maxdisp_run, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
-        HIPCHECK(hipMemcpyAsync(&ctl[pm::kCtlN * (size_t)s], c->sys[s].dev.ctl, pm::kCtlN * sizeof(int),
-                                hipMemcpyDeviceToHost, c->stream));
-      }
-      Readback r = read_scalars(c, 0, ns);
-      if (check_errors(c, 0, r.err)) {
-        if (attempt > 4) throw Fail{PM_ECAPACITY, "pair list capacity kept overflowing"};
-        // lists regrown: restore the state of the call's start and run it again
-        pm::launch_import_x(c->L, c->d_xaos);
-        pm::launch_import_v(c->L, c->d_vaos);
-        force_eval(c, c->L, pm::kStepForce, pol, false, false, false);
-        continue;
-      }
-      float ms = 0.f;
-      HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-      double pair_ms = 0.0;
-      for (int k = 0; k + 1 < c->evs_used; k += 2) {
-        float t = 0.f;
-        HIPCHECK(hipEventElapsedTime(&t, c->evs[k], c->evs[k + 1]));
-        pair_ms += t;
-      }
-      unsigned worst = 0;
-      double rebuilds = 0;
-      for (int s = 0; s < ns; ++s) {
-        worst = std::max(worst, md[s]);
-        rebuilds += ctl[pm::kCtlN * (size_t)s + pm::kCtlCount];
-      }
-      float w;
-      std::memcpy(&w, &worst, sizeof w);
-      c->stats[0] = rebuilds;
-      c->stats[1] = attempt;
-      c->stats[3] = ms;
-      c->stats[4] = pair_ms;
-      c->stats[5] = c->evs_used / 2;
-      c->stats[6] = std::sqrt((double)w);
-      c->stats[7] = c->skin_md;
-      break;
-    }
+    md_begin(c, ids);
+    md_advance(c, ids, n_steps, dt_fs, energies_every > 0);
+    md_end(c, ids, dt_fs);
+    HIPCHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int pm_md_begin(pm_ctx *c, int32_t n, const int32_t *ids) {
+  if (!c) return PM_EINVAL;
+  return run_guarded(c, [&] { md_begin(c, check_ids(c, n, ids)); });
+}
+
+int pm_md_advance(pm_ctx *c, int32_t n, const int32_t *ids, int32_t n_steps, double dt_fs, int32_t energies_last) {
+  if (!c) return PM_EINVAL;
+  return run_guarded(c, [&] { md_advance(c, check_ids(c, n, ids), n_steps, dt_fs, energies_last != 0); });
+}
+
+int pm_md_end(pm_ctx *c, int32_t n, const int32_t *ids, double dt_fs) {
+  if (!c) return PM_EINVAL;
+  return run_guarded(c, [&] {
+    md_end(c, check_ids(c, n, ids), dt_fs);
+    HIPCHECK(hipStreamSynchronize(c->stream));
   });
 }
 
 int pm_md_get_state(pm_ctx *c, double *x, double *v, double *f, double *e) {
   if (!c) return PM_EINVAL;
   return run_guarded(c, [&] {
-    if (!c->md_ready) throw Fail{PM_ENOTREADY, "pm_md_set_state first"};
-    const size_t bytes = 3 * (size_t)c->n_total * sizeof(double);
-    pm::launch_export_xv(c->L, c->d_xaos, c->d_vaos);
-    pm::launch_export_f(c->L, c->d_faos);
-    if (x) HIPCHECK(hipMemcpyAsync(x, c->d_xaos, bytes, hipMemcpyDeviceToHost, c->stream));
-    if (v) HIPCHECK(hipMemcpyAsync(v, c->d_vaos, bytes, hipMemcpyDeviceToHost, c->stream));
-    if (f) HIPCHECK(hipMemcpyAsync(f, c->d_faos, bytes, hipMemcpyDeviceToHost, c->stream));
-    Readback r = read_scalars(c, 0, (int)c->sys.size());
-    if (e) {
-      std::memset(e, 0, sizeof(double) * PM_E_NSLOTS * c->sys.size());
-      for (size_t s = 0; s < c->sys.size(); ++s) add_energies(c, (int)s, &r.acc[s * pm::kAccN], e + s * PM_E_NSLOTS);
-    }
+    const std::vector<int> ids = live_ids(c);
+    if (ids.empty()) throw Fail{PM_EINVAL, "no systems"};
+    get_state_ids(c, ids, x, v, f, e);
   });
+}
+
+int pm_md_get_state_systems(pm_ctx *c, int32_t n, const int32_t *ids, double *x, double *v, double *f, double *e) {
+  if (!c) return PM_EINVAL;
+  return run_guarded(c, [&] { get_state_ids(c, check_ids(c, n, ids), x, v, f, e); });
 }
 
 int pm_stats(pm_ctx *c, double *stats, int32_t n) {

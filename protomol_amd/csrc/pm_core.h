@@ -30,11 +30,23 @@ enum { kAccLJ = 0, kAccCoul = 1, kAccVxx, kAccVxy, kAccVxz, kAccVyy, kAccVyz, kA
 // device-side status bits (per system)
 enum { kErrColumnCap = 1, kErrListCap = 2, kErrNonFinite = 4, kErrClusterCap = 8 };
 
+// items of the per-atom list of bonded/correction terms: (index << 4) | (kind << 2) | role
+enum { kLocBond = 0, kLocAngle = 1, kLocCorr = 2 };
+enum { kLocalForces = 1, kLocalEnergies = 2 };
+// host staging transfers (k_import / k_export)
+enum { kXferX = 1, kXferV = 2, kXferF = 4 };
+// packed per-system scalars (k_gather_scalars): acc[0..16), census[16..18), err,
+// ctl[19..23), largest squared displacement of the run (float bits)
+enum { kScalCensus = 16, kScalErr = 18, kScalCtl = 19, kScalMaxdispRun = 23, kScalN = 24 };
+// what a pair-kernel launch accumulates
+enum { kPairForces = 0, kPairForcesEnergies = 1, kPairEnergies = 2, kPairCensus = 3 };
+
 // per-system control words: [0] rebuild the pair list in this step (0/1),
-// [1] steps since the last rebuild, [2] rebuilds since the start of the run
-enum { kCtlRebuild = 0, kCtlSince = 1, kCtlCount = 2, kCtlN = 4 };
+// [1] steps since the last rebuild, [2] rebuilds since the start of the run,
+// [3] the next kick is the first of a run (a half kick)
+enum { kCtlRebuild = 0, kCtlSince = 1, kCtlCount = 2, kCtlFirst = 3, kCtlN = 4 };
 // k_step_begin modes (bit set)
-enum { kStepForce = 1, kStepAuto = 2, kStepResetRun = 4 };
+enum { kStepForce = 1, kStepAuto = 2 };
 
 struct SwitchDev {
   int kind;                // PM_SWITCH_*
@@ -78,7 +90,7 @@ struct SysDev {
   float lj_scaling;
   // original-order arrays
   double *x, *v, *xref;    // 3 * n each, SoA (x[0..n), y[n..2n), z[2n..3n))
-  double *f_out;           // 3 * n AoS export buffer
+  double *xaos, *vaos, *faos;  // 3 * n AoS FP64 staging for host transfers
   const double *charge, *inv_mass;
   const int *molecule, *type;
   const float *sig_half, *eps_sqrt; // PM_LJ_PERATOM: sigma/2, sqrt(epsilon)
@@ -114,6 +126,8 @@ struct SysDev {
   // Ewald correction pairs (unique exclusion table pairs)
   int n_corr;
   const int *corr;         // 3 per pair: a, b, class
+  const int *loc_start;    // n + 1: per-atom CSR of bonded/correction items
+  const int *loc_item;
   // GB
   const double *gb_rho, *gb_S;
   double *gb_born, *gb_drfac, *gb_partial, *gb_spare;

@@ -14,20 +14,20 @@ struct Launch {
   const int *atom_prefix = nullptr, *cl_prefix = nullptr, *col_prefix = nullptr, *slot_prefix = nullptr,
             *corr_prefix = nullptr, *bonded_prefix = nullptr, *gb_prefix = nullptr;
   int n_atoms_total = 0, n_cl_total = 0, n_cols_total = 0, n_slots_total = 0, n_corr_total = 0,
-      n_bonded_total = 0, n_gb_tiles_total = 0;
+      n_bonded_total = 0, n_gb_tiles_total = 0, n_local_total = 0;
 };
 
-void launch_import_x(const Launch &L, const double *xaos);
-void launch_import_v(const Launch &L, const double *vaos);
+void launch_import(const Launch &L, int what);
+void launch_export(const Launch &L, int what);
+void launch_gather_scalars(const Launch &L, long long *out);
+void launch_clear_err(const Launch &L);
 void launch_rebuild(const Launch &L, float rlist2);
-bool launch_pair(const Launch &L, const ForceDev &P, bool energies, bool census);
-void launch_ewald_corr(const Launch &L, const ForceDev &P, bool energies);
-void launch_bonded(const Launch &L, bool energies);
-void launch_kick_drift(const Launch &L, double h, int mode);
+bool launch_pair(const Launch &L, const ForceDev &P, int kind);
+void launch_local(const Launch &L, const ForceDev &P, int mode);
+void launch_kick_drift(const Launch &L, double h);
+void launch_md_arm(const Launch &L);
 void launch_half_kick(const Launch &L, double hh);
 void launch_zero_forces(const Launch &L);
 void launch_step_begin(const Launch &L, int mode, float lim2, int kmax);
-void launch_export_f(const Launch &L, double *out);
-void launch_export_xv(const Launch &L, double *xo, double *vo);
 
 }  // namespace pm

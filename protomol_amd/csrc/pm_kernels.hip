@@ -1,7 +1,7 @@
 // pm_kernels.hip -- gfx950 kernels of the nonbonded force path.
 //
 //   rebuild:  k_bin -> k_col_scan -> k_col_fill -> k_col_sort -> k_cluster_geom -> k_pairlist
-//   force:    k_pair<TermA, TermB, Energies, Census>, k_bonded, k_ewald_corr
+//   force:    k_pair<TermA, TermB, SwitchA, SwitchB, Kind>, k_local (bonded + Ewald correction)
 //   step:     k_kick_drift, k_half_kick, k_export_f, k_kinetic
 //
 // All kernels take the device table of SysDev records plus a prefix array
@@ -218,26 +218,17 @@ __device__ __forceinline__ void apply_switch(const SwitchDev &sw, float r2, floa
 
 // ------------------------------------------------------------- rebuild
 
-__global__ void k_import_x(SysDev *sys, const int *ap, int nsys, const double *xaos) {
-  int g = blockIdx.x * blockDim.x + threadIdx.x;
+// host staging (AoS FP64, per system) -> device state; what: kXfer* bits
+__global__ void k_import(SysDev *sys, const int *ap, int nsys, int what) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ap[nsys]) return;
-  int s = find_sys(ap, nsys, g);
+  const int s = find_sys(ap, nsys, g);
   SysDev &S = sys[s];
-  int a = g - ap[s], n = S.n_atoms;
-  S.x[a] = xaos[3 * (size_t)g];
-  S.x[n + a] = xaos[3 * (size_t)g + 1];
-  S.x[2 * n + a] = xaos[3 * (size_t)g + 2];
-}
-
-__global__ void k_import_v(SysDev *sys, const int *ap, int nsys, const double *vaos) {
-  int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= ap[nsys]) return;
-  int s = find_sys(ap, nsys, g);
-  SysDev &S = sys[s];
-  int a = g - ap[s], n = S.n_atoms;
-  S.v[a] = vaos[3 * (size_t)g];
-  S.v[n + a] = vaos[3 * (size_t)g + 1];
-  S.v[2 * n + a] = vaos[3 * (size_t)g + 2];
+  const int a = g - ap[s], n = S.n_atoms;
+  for (int d = 0; d < 3; ++d) {
+    if (what & kXferX) S.x[d * n + a] = S.xaos[3 * (size_t)a + d];
+    if (what & kXferV) S.v[d * n + a] = S.vaos[3 * (size_t)a + d];
+  }
 }
 
 // Lanes of a wave that share a (system, column) key are served by one atomic:
@@ -626,9 +617,12 @@ constexpr int kLJLds = 256;  // LJ table entries kept in LDS per wave (T <= 16)
 // accumulate in registers; j-forces are summed over the 8 i lanes with DPP and
 // land, like the i-forces, in 64-bit fixed point, so the total is independent
 // of the order in which waves finish.
-template <int TA, int TB, int SWA, int SWB, bool EN, bool CEN>
+template <int TA, int TB, int SWA, int SWB, int KIND>
 __global__ __launch_bounds__(256) void k_pair(const SysDev *__restrict__ sys, const int *__restrict__ clp, int nsys,
                                               ForceDev P) {
+  constexpr bool FORCES = KIND != kPairEnergies;
+  constexpr bool EN = KIND == kPairForcesEnergies || KIND == kPairEnergies;
+  constexpr bool CEN = KIND == kPairCensus;
   __shared__ float4 ljs[4][kLJLds];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
@@ -799,11 +793,13 @@ __global__ __launch_bounds__(256) void k_pair(const SysDev *__restrict__ sys, co
       }
     }
     // Newton's third law: j force = sum over the 8 i lanes of this j atom
-    const float gx = dpp_sum8(fx), gy = dpp_sum8(fy), gz = dpp_sum8(fz);
-    if (ii == 0 && jj < nvj) {
-      atomic_add_ll(&gfx[sj], to_fixed((double)gx, kFScale));
-      atomic_add_ll(&gfy[sj], to_fixed((double)gy, kFScale));
-      atomic_add_ll(&gfz[sj], to_fixed((double)gz, kFScale));
+    if (FORCES) {
+      const float gx = dpp_sum8(fx), gy = dpp_sum8(fy), gz = dpp_sum8(fz);
+      if (ii == 0 && jj < nvj) {
+        atomic_add_ll(&gfx[sj], to_fixed((double)gx, kFScale));
+        atomic_add_ll(&gfy[sj], to_fixed((double)gy, kFScale));
+        atomic_add_ll(&gfz[sj], to_fixed((double)gz, kFScale));
+      }
     }
   }
   // i force = sum over the 8 j lanes that share ii
@@ -816,7 +812,7 @@ __global__ __launch_bounds__(256) void k_pair(const SysDev *__restrict__ sys, co
   fix += __shfl_xor(fix, 32);
   fiy += __shfl_xor(fiy, 32);
   fiz += __shfl_xor(fiz, 32);
-  if (jj == 0 && vi) {
+  if (FORCES && jj == 0 && vi) {
     atomic_add_ll(&gfx[si], to_fixed((double)fix, kFScale));
     atomic_add_ll(&gfy[si], to_fixed((double)fiy, kFScale));
     atomic_add_ll(&gfz[si], to_fixed((double)fiz, kFScale));
@@ -867,124 +863,144 @@ __device__ __forceinline__ void min_image_vec(const SysDev &S, double *d) {
     for (int k = 0; k < 3; ++k) d[k] -= S.L[k] * rint(S.Lr[k] * d[k]);
 }
 
-__device__ __forceinline__ void add_force_fixed(const SysDev &S, int a, double fx, double fy, double fz) {
-  const int sl = S.slot[a];
-  atomic_add_ll(&S.fx[sl], to_fixed(fx, kFScale));
-  atomic_add_ll(&S.fy[sl], to_fixed(fy, kFScale));
-  atomic_add_ll(&S.fz[sl], to_fixed(fz, kFScale));
+// Sum of int64 values over the lanes of a wave (exact, order-free).
+__device__ __forceinline__ long long wave_sum_ll(long long v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
 }
 
-// NonbondedFullEwaldSystemForce::correctionTerm (NonbondedFullEwaldSystemForce.h:1097-1131)
-// in FP64, one thread per excluded pair
-__global__ void k_ewald_corr(SysDev *sys, const int *xp, int nsys, ForceDev P, int energies) {
-  int g = blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  double eacc = 0.0, v[6] = {0, 0, 0, 0, 0, 0};
-  int s = -1;
-  if (g < xp[nsys]) {
-    s = find_sys(xp, nsys, g);
+// Bonded terms and Ewald excluded-pair corrections, one thread per atom
+// ("owner computes"): the thread walks the terms its atom takes part in and
+// evaluates each in FP64 from that atom's side.  A term is evaluated once per
+// participant with the same expressions, so the forces on its atoms are
+// exactly opposite, and each atom's slot is updated by one thread only -- no
+// atomics, and the result does not depend on which other systems share the
+// launch.  Energies and virial (mode & kLocalEnergies) are counted by the
+// term's first atom.
+//   BondSystemForce::calcBond (force/bonded/BondSystemForce.h:77-108)
+//   AngleSystemForce::calcAngle (force/bonded/AngleSystemForce.h:66-140)
+//   NonbondedFullEwaldSystemForce::correctionTerm (NonbondedFullEwaldSystemForce.h:1097-1131)
+__global__ __launch_bounds__(256) void k_local(SysDev *sys, const int *ap, int nsys, ForceDev P, int mode) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool ok = g < ap[nsys];
+  int s = 0;
+  double f[3] = {0.0, 0.0, 0.0};
+  double eb = 0.0, ea = 0.0, ec = 0.0, v[6] = {0, 0, 0, 0, 0, 0};
+  if (ok) {
+    s = find_sys(ap, nsys, g);
     const SysDev &S = sys[s];
-    const int k = g - xp[s], n = S.n_atoms;
-    const int a1 = S.corr[3 * k], a2 = S.corr[3 * k + 1], cls = S.corr[3 * k + 2];
-    double d[3] = {S.x[a2] - S.x[a1], S.x[n + a2] - S.x[n + a1], S.x[2 * n + a2] - S.x[2 * n + a1]};
-    min_image_vec(S, d);
-    const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-    double qq = S.charge[a1] * S.charge[a2];
-    if (cls == PM_EXCL_MODIFIED) qq *= 1.0 - (double)S.csf;
-    const double r = sqrt(r2), rr = 1.0 / r;
-    const double e = erf(P.dalpha * r) * rr;  // C-library erf, as the reference on Linux
-    eacc = -qq * e;
-    const double fs = qq * (P.dtwo_alpha_pi * exp(-(P.dalpha * P.dalpha) * r2) - e) * rr * rr;
-    const double f[3] = {d[0] * fs, d[1] * fs, d[2] * fs};
-    add_force_fixed(S, a1, -f[0], -f[1], -f[2]);
-    add_force_fixed(S, a2, f[0], f[1], f[2]);
-    v[0] = f[0] * d[0]; v[1] = f[0] * d[1]; v[2] = f[0] * d[2];
-    v[3] = f[1] * d[1]; v[4] = f[1] * d[2]; v[5] = f[2] * d[2];
-  }
-  if (!energies || s < 0) return;
-  // per-thread atomics: exclusion pairs are few (3 per water)
-  const SysDev &S = sys[s];
-  atomic_add_ll(&S.acc[kAccCoul], to_fixed(eacc, kEScale));
-  atomic_add_ll(&S.acc[kAccVxx], to_fixed(v[0], kEScale));
-  atomic_add_ll(&S.acc[kAccVxy], to_fixed(v[1], kEScale));
-  atomic_add_ll(&S.acc[kAccVxz], to_fixed(v[2], kEScale));
-  atomic_add_ll(&S.acc[kAccVyy], to_fixed(v[3], kEScale));
-  atomic_add_ll(&S.acc[kAccVyz], to_fixed(v[4], kEScale));
-  atomic_add_ll(&S.acc[kAccVzz], to_fixed(v[5], kEScale));
-  (void)lane;
-}
-
-// BondSystemForce::calcBond (force/bonded/BondSystemForce.h:77-108) and
-// AngleSystemForce::calcAngle (AngleSystemForce.h:66-140), FP64, one thread per term
-__global__ void k_bonded(SysDev *sys, const int *tp, int nsys, int energies) {
-  int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= tp[nsys]) return;
-  int s = find_sys(tp, nsys, g);
-  const SysDev &S = sys[s];
-  const int k = g - tp[s], n = S.n_atoms;
-  if (k < S.n_bonds) {
-    const int a1 = S.bonds[2 * k], a2 = S.bonds[2 * k + 1];
-    double r12[3] = {S.x[a1] - S.x[a2], S.x[n + a1] - S.x[n + a2], S.x[2 * n + a1] - S.x[2 * n + a2]};
-    min_image_vec(S, r12);
-    const double r = sqrt(r12[0] * r12[0] + r12[1] * r12[1] + r12[2] * r12[2]);
-    const double dp = 2.0 * S.bond_k[k] * (r - S.bond_r0[k]);
-    const double c = -dp / r;
-    add_force_fixed(S, a1, r12[0] * c, r12[1] * c, r12[2] * c);
-    add_force_fixed(S, a2, -r12[0] * c, -r12[1] * c, -r12[2] * c);
-    if (energies) {
-      const double e = S.bond_k[k] * (r - S.bond_r0[k]) * (r - S.bond_r0[k]);
-      atomic_add_ll(&S.acc[kAccBond], to_fixed(e, kEScale));
-      atomic_add_ll(&S.acc[kAccVxx], to_fixed(r12[0] * c * r12[0], kEScale));
-      atomic_add_ll(&S.acc[kAccVxy], to_fixed(r12[0] * c * r12[1], kEScale));
-      atomic_add_ll(&S.acc[kAccVxz], to_fixed(r12[0] * c * r12[2], kEScale));
-      atomic_add_ll(&S.acc[kAccVyy], to_fixed(r12[1] * c * r12[1], kEScale));
-      atomic_add_ll(&S.acc[kAccVyz], to_fixed(r12[1] * c * r12[2], kEScale));
-      atomic_add_ll(&S.acc[kAccVzz], to_fixed(r12[2] * c * r12[2], kEScale));
+    const int a = g - ap[s], n = S.n_atoms;
+    const double *X = S.x;
+    for (int it = S.loc_start[a]; it < S.loc_start[a + 1]; ++it) {
+      const int item = S.loc_item[it];
+      const int kind = (item >> 2) & 3, role = item & 3, k = item >> 4;
+      if (kind == kLocBond) {
+        const int a1 = S.bonds[2 * k], a2 = S.bonds[2 * k + 1];
+        double r12[3] = {X[a1] - X[a2], X[n + a1] - X[n + a2], X[2 * n + a1] - X[2 * n + a2]};
+        min_image_vec(S, r12);
+        const double r = sqrt(r12[0] * r12[0] + r12[1] * r12[1] + r12[2] * r12[2]);
+        const double dp = 2.0 * S.bond_k[k] * (r - S.bond_r0[k]);
+        const double c = -dp / r;
+        const double sg = role == 0 ? 1.0 : -1.0;
+        for (int d = 0; d < 3; ++d) f[d] += sg * (r12[d] * c);
+        if (role == 0) {
+          eb += S.bond_k[k] * (r - S.bond_r0[k]) * (r - S.bond_r0[k]);
+          v[0] += r12[0] * c * r12[0];
+          v[1] += r12[0] * c * r12[1];
+          v[2] += r12[0] * c * r12[2];
+          v[3] += r12[1] * c * r12[1];
+          v[4] += r12[1] * c * r12[2];
+          v[5] += r12[2] * c * r12[2];
+        }
+      } else if (kind == kLocAngle) {
+        const int a1 = S.angles[3 * k], a2 = S.angles[3 * k + 1], a3 = S.angles[3 * k + 2];
+        double r12[3] = {X[a1] - X[a2], X[n + a1] - X[n + a2], X[2 * n + a1] - X[2 * n + a2]};
+        double r32[3] = {X[a3] - X[a2], X[n + a3] - X[n + a2], X[2 * n + a3] - X[2 * n + a2]};
+        min_image_vec(S, r12);
+        min_image_vec(S, r32);
+        const double d12 = sqrt(r12[0] * r12[0] + r12[1] * r12[1] + r12[2] * r12[2]);
+        const double d32 = sqrt(r32[0] * r32[0] + r32[1] * r32[1] + r32[2] * r32[2]);
+        const double cr0 = r12[1] * r32[2] - r12[2] * r32[1], cr1 = r12[2] * r32[0] - r12[0] * r32[2],
+                     cr2 = r12[0] * r32[1] - r12[1] * r32[0];
+        const double theta =
+            atan2(sqrt(cr0 * cr0 + cr1 * cr1 + cr2 * cr2), r12[0] * r32[0] + r12[1] * r32[1] + r12[2] * r32[2]);
+        const double st = sin(theta), ct = cos(theta);
+        const double dp = 2.0 * S.angle_k[k] * (theta - S.angle_t0[k]);
+        double f1[3], f3[3];
+        for (int d = 0; d < 3; ++d) {
+          const double u12 = r12[d] / d12, u32 = r32[d] / d32;
+          f1[d] = -((u12 * ct - u32) / (st * d12)) * dp;
+          f3[d] = -((u32 * ct - u12) / (st * d32)) * dp;
+        }
+        for (int d = 0; d < 3; ++d) f[d] += role == 0 ? f1[d] : (role == 2 ? f3[d] : -f1[d] - f3[d]);
+        if (role == 0) {
+          ea += S.angle_k[k] * (theta - S.angle_t0[k]) * (theta - S.angle_t0[k]);
+          v[0] += f1[0] * r12[0] + f3[0] * r32[0];
+          v[1] += f1[0] * r12[1] + f3[0] * r32[1];
+          v[2] += f1[0] * r12[2] + f3[0] * r32[2];
+          v[3] += f1[1] * r12[1] + f3[1] * r32[1];
+          v[4] += f1[1] * r12[2] + f3[1] * r32[2];
+          v[5] += f1[2] * r12[2] + f3[2] * r32[2];
+        }
+      } else {  // kLocCorr
+        const int a1 = S.corr[3 * k], a2 = S.corr[3 * k + 1], cls = S.corr[3 * k + 2];
+        double d[3] = {X[a2] - X[a1], X[n + a2] - X[n + a1], X[2 * n + a2] - X[2 * n + a1]};
+        min_image_vec(S, d);
+        const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        double qq = S.charge[a1] * S.charge[a2];
+        if (cls == PM_EXCL_MODIFIED) qq *= 1.0 - (double)S.csf;
+        const double r = sqrt(r2), rr = 1.0 / r;
+        const double e = erf(P.dalpha * r) * rr;  // C-library erf, as the reference on Linux
+        const double fs = qq * (P.dtwo_alpha_pi * exp(-(P.dalpha * P.dalpha) * r2) - e) * rr * rr;
+        const double sg = role == 0 ? -1.0 : 1.0;
+        for (int q = 0; q < 3; ++q) f[q] += sg * (d[q] * fs);
+        if (role == 0) {
+          ec += -qq * e;
+          v[0] += d[0] * fs * d[0];
+          v[1] += d[0] * fs * d[1];
+          v[2] += d[0] * fs * d[2];
+          v[3] += d[1] * fs * d[1];
+          v[4] += d[1] * fs * d[2];
+          v[5] += d[2] * fs * d[2];
+        }
+      }
     }
-    return;
+    if ((mode & kLocalForces) && S.loc_start[a + 1] > S.loc_start[a]) {
+      const int sl = S.slot[a];
+      S.fx[sl] += to_fixed(f[0], kFScale);
+      S.fy[sl] += to_fixed(f[1], kFScale);
+      S.fz[sl] += to_fixed(f[2], kFScale);
+    }
   }
-  const int q = k - S.n_bonds;
-  if (q >= S.n_angles) return;
-  const int a1 = S.angles[3 * q], a2 = S.angles[3 * q + 1], a3 = S.angles[3 * q + 2];
-  double r12[3] = {S.x[a1] - S.x[a2], S.x[n + a1] - S.x[n + a2], S.x[2 * n + a1] - S.x[2 * n + a2]};
-  double r32[3] = {S.x[a3] - S.x[a2], S.x[n + a3] - S.x[n + a2], S.x[2 * n + a3] - S.x[2 * n + a2]};
-  min_image_vec(S, r12);
-  min_image_vec(S, r32);
-  const double d12 = sqrt(r12[0] * r12[0] + r12[1] * r12[1] + r12[2] * r12[2]);
-  const double d32 = sqrt(r32[0] * r32[0] + r32[1] * r32[1] + r32[2] * r32[2]);
-  const double cr0 = r12[1] * r32[2] - r12[2] * r32[1], cr1 = r12[2] * r32[0] - r12[0] * r32[2],
-               cr2 = r12[0] * r32[1] - r12[1] * r32[0];
-  const double theta = atan2(sqrt(cr0 * cr0 + cr1 * cr1 + cr2 * cr2), r12[0] * r32[0] + r12[1] * r32[1] + r12[2] * r32[2]);
-  const double st = sin(theta), ct = cos(theta);
-  const double dp = 2.0 * S.angle_k[q] * (theta - S.angle_t0[q]);
-  double f1[3], f3[3];
-  for (int d = 0; d < 3; ++d) {
-    const double u12 = r12[d] / d12, u32 = r32[d] / d32;
-    f1[d] = -((u12 * ct - u32) / (st * d12)) * dp;
-    f3[d] = -((u32 * ct - u12) / (st * d32)) * dp;
-  }
-  add_force_fixed(S, a1, f1[0], f1[1], f1[2]);
-  add_force_fixed(S, a2, -f1[0] - f3[0], -f1[1] - f3[1], -f1[2] - f3[2]);
-  add_force_fixed(S, a3, f3[0], f3[1], f3[2]);
-  if (energies) {
-    const double e = S.angle_k[q] * (theta - S.angle_t0[q]) * (theta - S.angle_t0[q]);
-    atomic_add_ll(&S.acc[kAccAngle], to_fixed(e, kEScale));
-    atomic_add_ll(&S.acc[kAccVxx], to_fixed(f1[0] * r12[0] + f3[0] * r32[0], kEScale));
-    atomic_add_ll(&S.acc[kAccVxy], to_fixed(f1[0] * r12[1] + f3[0] * r32[1], kEScale));
-    atomic_add_ll(&S.acc[kAccVxz], to_fixed(f1[0] * r12[2] + f3[0] * r32[2], kEScale));
-    atomic_add_ll(&S.acc[kAccVyy], to_fixed(f1[1] * r12[1] + f3[1] * r32[1], kEScale));
-    atomic_add_ll(&S.acc[kAccVyz], to_fixed(f1[1] * r12[2] + f3[1] * r32[2], kEScale));
-    atomic_add_ll(&S.acc[kAccVzz], to_fixed(f1[2] * r12[2] + f3[2] * r32[2], kEScale));
+  if (!(mode & kLocalEnergies)) return;
+  // each lane's terms go to fixed point first; integer sums are exact, so the
+  // totals do not depend on how lanes are grouped
+  long long q[9] = {to_fixed(eb, kEScale), to_fixed(ea, kEScale), to_fixed(ec, kEScale),
+                    to_fixed(v[0], kEScale), to_fixed(v[1], kEScale), to_fixed(v[2], kEScale),
+                    to_fixed(v[3], kEScale), to_fixed(v[4], kEScale), to_fixed(v[5], kEScale)};
+  const int slots[9] = {kAccBond, kAccAngle, kAccCoul, kAccVxx, kAccVxy, kAccVxz, kAccVyy, kAccVyz, kAccVzz};
+  const int s0 = __shfl(s, 0);
+  const bool mixed = wave_ballot(ok && s != s0) != 0;
+  if (!mixed) {
+    for (int k = 0; k < 9; ++k) q[k] = wave_sum_ll(q[k]);
+    if ((threadIdx.x & 63) == 0)
+      for (int k = 0; k < 9; ++k)
+        if (q[k]) atomic_add_ll(&sys[s0].acc[slots[k]], q[k]);
+  } else if (ok) {
+    for (int k = 0; k < 9; ++k)
+      if (q[k]) atomic_add_ll(&sys[s].acc[slots[k]], q[k]);
   }
 }
 
 // ------------------------------------------------------------ integrator
 
-// mode 0: doHalfKickdoDrift (LeapfrogIntegrator.cpp:33-56)  v += F*h*0.5/m; x += v*h
-// mode 1: doKickdoDrift     (LeapfrogIntegrator.cpp:58-87)  v += F*h/m;     x += v*h
+// First step of a run (the system's kCtlFirst word is set):
+//   doHalfKickdoDrift (LeapfrogIntegrator.cpp:33-56)  v += F*h*0.5/m; x += v*h
+// later steps:
+//   doKickdoDrift     (LeapfrogIntegrator.cpp:58-87)  v += F*h/m;     x += v*h
 // The force buffer is consumed (zeroed) and the FP32 cluster-local
 // coordinates for the next force evaluation are written.
-__global__ __launch_bounds__(256) void k_kick_drift(SysDev *sys, const int *ap, int nsys, double h, int mode) {
+__global__ __launch_bounds__(256) void k_kick_drift(SysDev *sys, const int *ap, int nsys, double h) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   float d2 = 0.f;
   int s = 0;
@@ -994,6 +1010,7 @@ __global__ __launch_bounds__(256) void k_kick_drift(SysDev *sys, const int *ap, 
     SysDev &S = sys[s];
     const int a = g - ap[s], n = S.n_atoms;
     const int sl = S.slot[a];
+    const int mode = S.ctl[kCtlFirst] ? 0 : 1;
     const double m = 1.0 / S.inv_mass[a];
     const double F[3] = {(double)S.fx[sl] * kFScaleInv, (double)S.fy[sl] * kFScaleInv,
                          (double)S.fz[sl] * kFScaleInv};
@@ -1080,10 +1097,7 @@ __global__ void k_step_begin(SysDev *sys, int nsys, int mode, float lim2, int km
     S.census[0] = 0;
     S.census[1] = 0;
     int *ctl = S.ctl;
-    if (mode & kStepResetRun) {
-      *S.maxdisp_run = 0u;
-      ctl[kCtlCount] = 0;
-    }
+    ctl[kCtlFirst] = 0;  // the step's kick has run
     const unsigned d2 = *S.maxdisp2;
     bool rb = (mode & kStepForce) != 0;
     if (!rb && (mode & kStepAuto))
@@ -1100,39 +1114,64 @@ __global__ void k_step_begin(SysDev *sys, int nsys, int mode, float lim2, int km
   }
 }
 
-__global__ void k_export_f(SysDev *sys, const int *ap, int nsys, double *out) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= ap[nsys]) return;
-  const int s = find_sys(ap, nsys, g);
-  const SysDev &S = sys[s];
-  const int a = g - ap[s];
-  const int sl = S.slot[a];
-  out[3 * (size_t)g] = (double)S.fx[sl] * kFScaleInv;
-  out[3 * (size_t)g + 1] = (double)S.fy[sl] * kFScaleInv;
-  out[3 * (size_t)g + 2] = (double)S.fz[sl] * kFScaleInv;
-}
-
-__global__ void k_export_xv(SysDev *sys, const int *ap, int nsys, double *xo, double *vo) {
+// device state -> host staging (AoS FP64, per system); what: kXfer* bits
+__global__ void k_export(SysDev *sys, const int *ap, int nsys, int what) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ap[nsys]) return;
   const int s = find_sys(ap, nsys, g);
   const SysDev &S = sys[s];
   const int a = g - ap[s], n = S.n_atoms;
+  const int sl = S.slot[a];
   for (int d = 0; d < 3; ++d) {
-    if (xo) xo[3 * (size_t)g + d] = S.x[d * n + a];
-    if (vo) vo[3 * (size_t)g + d] = S.v[d * n + a];
+    if (what & kXferX) S.xaos[3 * (size_t)a + d] = S.x[d * n + a];
+    if (what & kXferV) S.vaos[3 * (size_t)a + d] = S.v[d * n + a];
   }
+  if (what & kXferF) {
+    S.faos[3 * (size_t)a] = (double)S.fx[sl] * kFScaleInv;
+    S.faos[3 * (size_t)a + 1] = (double)S.fy[sl] * kFScaleInv;
+    S.faos[3 * (size_t)a + 2] = (double)S.fz[sl] * kFScaleInv;
+  }
+}
+
+// per-system scalars of a launch, packed for one device-to-host copy:
+// kScalN int64 per system (see kScal*)
+__global__ void k_gather_scalars(const SysDev *sys, int nsys, long long *out) {
+  const int s = blockIdx.x;
+  if (s >= nsys) return;
+  const SysDev &S = sys[s];
+  long long *o = out + (size_t)s * kScalN;
+  for (int k = threadIdx.x; k < kScalN; k += blockDim.x) {
+    long long v = 0;
+    if (k < kAccN) v = S.acc[k];
+    else if (k < kScalCensus + 2) v = (long long)S.census[k - kScalCensus];
+    else if (k == kScalErr) v = *S.err;
+    else if (k >= kScalCtl && k < kScalCtl + kCtlN) v = S.ctl[k - kScalCtl];
+    else if (k == kScalMaxdispRun) v = *S.maxdisp_run;
+    o[k] = v;
+  }
+}
+
+// clear the error bits of the systems of a launch
+__global__ void k_clear_err(SysDev *sys, int nsys) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nsys) *sys[s].err = 0u;
 }
 
 // ------------------------------------------------------------- launchers
 
 static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
 
-void launch_import_x(const Launch &L, const double *xaos) {
-  if (L.n_atoms_total) hipLaunchKernelGGL(k_import_x, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys, xaos);
+void launch_import(const Launch &L, int what) {
+  if (L.n_atoms_total) hipLaunchKernelGGL(k_import, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys, what);
 }
-void launch_import_v(const Launch &L, const double *vaos) {
-  if (L.n_atoms_total) hipLaunchKernelGGL(k_import_v, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys, vaos);
+void launch_export(const Launch &L, int what) {
+  if (L.n_atoms_total) hipLaunchKernelGGL(k_export, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys, what);
+}
+void launch_gather_scalars(const Launch &L, long long *out) {
+  hipLaunchKernelGGL(k_gather_scalars, dim3(L.nsys), dim3(64), 0, L.stream, L.sys, L.nsys, out);
+}
+void launch_clear_err(const Launch &L) {
+  hipLaunchKernelGGL(k_clear_err, dim3(nblk(L.nsys, 64)), dim3(64), 0, L.stream, L.sys, L.nsys);
 }
 
 void launch_rebuild(const Launch &L, float rlist2) {
@@ -1146,44 +1185,53 @@ void launch_rebuild(const Launch &L, float rlist2) {
 }
 
 template <int TA, int TB, int SWA, int SWB>
-static void launch_pair_t(const Launch &L, const ForceDev &P, bool en, bool cen) {
+static void launch_pair_t(const Launch &L, const ForceDev &P, int kind) {
   dim3 g(nblk((long long)L.n_cl_total * kWave, 256)), b(256);
-  if (cen) hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, false, true>), g, b, 0, L.stream, L.sys, L.cl_prefix, L.nsys, P);
-  else if (en) hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, true, false>), g, b, 0, L.stream, L.sys, L.cl_prefix, L.nsys, P);
-  else hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, false, false>), g, b, 0, L.stream, L.sys, L.cl_prefix, L.nsys, P);
+  switch (kind) {
+  case kPairForces:
+    hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairForces>), g, b, 0, L.stream, L.sys, L.cl_prefix, L.nsys, P);
+    break;
+  case kPairForcesEnergies:
+    hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairForcesEnergies>), g, b, 0, L.stream, L.sys, L.cl_prefix, L.nsys, P);
+    break;
+  case kPairEnergies:
+    hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairEnergies>), g, b, 0, L.stream, L.sys, L.cl_prefix, L.nsys, P);
+    break;
+  default:
+    hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairCensus>), g, b, 0, L.stream, L.sys, L.cl_prefix, L.nsys, P);
+    break;
+  }
 }
 
 // C2 switches get a compile-time fast path; every other kind runs the generic one
 template <int TA, int TB>
-static void launch_pair_sw(const Launch &L, const ForceDev &P, bool en, bool cen) {
+static void launch_pair_sw(const Launch &L, const ForceDev &P, int kind) {
   const bool a2 = TA == PM_TERM_NONE || P.sw_a.kind == PM_SWITCH_C2;
   const bool b2 = TB != PM_TERM_COULOMB || P.sw_b.kind == PM_SWITCH_C2;
   constexpr int SB = TB == PM_TERM_COULOMB ? PM_SWITCH_C2 : PM_SWITCH_CUTOFF;
   constexpr int SBG = TB == PM_TERM_COULOMB ? kSwGeneric : PM_SWITCH_CUTOFF;
-  if (a2 && b2) launch_pair_t<TA, TB, PM_SWITCH_C2, SB>(L, P, en, cen);
-  else launch_pair_t<TA, TB, kSwGeneric, SBG>(L, P, en, cen);
+  if (a2 && b2) launch_pair_t<TA, TB, PM_SWITCH_C2, SB>(L, P, kind);
+  else launch_pair_t<TA, TB, kSwGeneric, SBG>(L, P, kind);
 }
 
-bool launch_pair(const Launch &L, const ForceDev &P, bool en, bool cen) {
+bool launch_pair(const Launch &L, const ForceDev &P, int kind) {
   if (!L.n_cl_total) return true;
   const int a = P.term_a, b = P.term_b;
-  if (a == PM_TERM_LJ && b == PM_TERM_NONE) launch_pair_sw<PM_TERM_LJ, PM_TERM_NONE>(L, P, en, cen);
-  else if (a == PM_TERM_COULOMB && b == PM_TERM_NONE) launch_pair_sw<PM_TERM_COULOMB, PM_TERM_NONE>(L, P, en, cen);
-  else if (a == PM_TERM_LJ && b == PM_TERM_COULOMB) launch_pair_sw<PM_TERM_LJ, PM_TERM_COULOMB>(L, P, en, cen);
-  else if (a == PM_TERM_LJ && b == PM_TERM_EWALD_REAL) launch_pair_sw<PM_TERM_LJ, PM_TERM_EWALD_REAL>(L, P, en, cen);
-  else if (a == PM_TERM_NONE && b == PM_TERM_EWALD_REAL) launch_pair_t<PM_TERM_NONE, PM_TERM_EWALD_REAL, PM_SWITCH_NONE, PM_SWITCH_CUTOFF>(L, P, en, cen);
+  if (a == PM_TERM_LJ && b == PM_TERM_NONE) launch_pair_sw<PM_TERM_LJ, PM_TERM_NONE>(L, P, kind);
+  else if (a == PM_TERM_COULOMB && b == PM_TERM_NONE) launch_pair_sw<PM_TERM_COULOMB, PM_TERM_NONE>(L, P, kind);
+  else if (a == PM_TERM_LJ && b == PM_TERM_COULOMB) launch_pair_sw<PM_TERM_LJ, PM_TERM_COULOMB>(L, P, kind);
+  else if (a == PM_TERM_LJ && b == PM_TERM_EWALD_REAL) launch_pair_sw<PM_TERM_LJ, PM_TERM_EWALD_REAL>(L, P, kind);
+  else if (a == PM_TERM_NONE && b == PM_TERM_EWALD_REAL) launch_pair_t<PM_TERM_NONE, PM_TERM_EWALD_REAL, PM_SWITCH_NONE, PM_SWITCH_CUTOFF>(L, P, kind);
   else return false;
   return true;
 }
 
-void launch_ewald_corr(const Launch &L, const ForceDev &P, bool en) {
-  if (L.n_corr_total) hipLaunchKernelGGL(k_ewald_corr, dim3(nblk(L.n_corr_total, 256)), dim3(256), 0, L.stream, L.sys, L.corr_prefix, L.nsys, P, en ? 1 : 0);
+void launch_local(const Launch &L, const ForceDev &P, int mode) {
+  if (L.n_atoms_total && L.n_local_total)
+    hipLaunchKernelGGL(k_local, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys, P, mode);
 }
-void launch_bonded(const Launch &L, bool en) {
-  if (L.n_bonded_total) hipLaunchKernelGGL(k_bonded, dim3(nblk(L.n_bonded_total, 256)), dim3(256), 0, L.stream, L.sys, L.bonded_prefix, L.nsys, en ? 1 : 0);
-}
-void launch_kick_drift(const Launch &L, double h, int mode) {
-  if (L.n_atoms_total) hipLaunchKernelGGL(k_kick_drift, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys, h, mode);
+void launch_kick_drift(const Launch &L, double h) {
+  if (L.n_atoms_total) hipLaunchKernelGGL(k_kick_drift, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys, h);
 }
 void launch_half_kick(const Launch &L, double hh) {
   if (L.n_atoms_total) hipLaunchKernelGGL(k_half_kick, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys, hh);
@@ -1191,14 +1239,22 @@ void launch_half_kick(const Launch &L, double hh) {
 void launch_zero_forces(const Launch &L) {
   if (L.n_slots_total) hipLaunchKernelGGL(k_zero_forces, dim3(nblk(L.n_slots_total, 256)), dim3(256), 0, L.stream, L.sys, L.slot_prefix, L.nsys);
 }
+// start of an MD run for the systems of the launch: the first kick is a half
+// kick, and the run's rebuild count and displacement maximum restart
+__global__ void k_md_arm(SysDev *sys, int nsys) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsys) return;
+  SysDev &S = sys[s];
+  S.ctl[kCtlFirst] = 1;
+  S.ctl[kCtlCount] = 0;
+  *S.maxdisp_run = 0u;
+}
+
+void launch_md_arm(const Launch &L) {
+  hipLaunchKernelGGL(k_md_arm, dim3(nblk(L.nsys, 64)), dim3(64), 0, L.stream, L.sys, L.nsys);
+}
+
 void launch_step_begin(const Launch &L, int mode, float lim2, int kmax) {
   hipLaunchKernelGGL(k_step_begin, dim3(L.nsys), dim3(64), 0, L.stream, L.sys, L.nsys, mode, lim2, kmax);
 }
-void launch_export_f(const Launch &L, double *out) {
-  if (L.n_atoms_total) hipLaunchKernelGGL(k_export_f, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys, out);
-}
-void launch_export_xv(const Launch &L, double *xo, double *vo) {
-  if (L.n_atoms_total) hipLaunchKernelGGL(k_export_xv, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys, xo, vo);
-}
-
 }  // namespace pm

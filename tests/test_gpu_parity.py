@@ -344,3 +344,85 @@ def test_md_batch_alone_vs_mix_bitwise():
         assert np.array_equal(xb[off:off + s.n], x) and np.array_equal(vb[off:off + s.n], v)
         assert np.array_equal(Fb[off:off + s.n], F) and np.array_equal(Eb[k], E[0])
         off += s.n
+
+
+def _water_set(sizes, seed0):
+    return [S.water_box(n, (n / 0.0334) ** (1 / 3), seed=seed0 + k, min_contact=2.0) for k, n in enumerate(sizes)]
+
+
+def test_pool_requests_alone_vs_mix_bitwise():
+    """Systems that join and leave a running context (MD advanced in chunks
+    together with whatever else is resident) end bitwise where a plain
+    LeapfrogIntegrator::run of the same length leaves them alone."""
+    print("case pool alone-vs-mix", flush=True)
+    systems = _water_set((600, 1100, 800), 80)
+    v0s = [S.maxwell_velocities(s, 300.0, seed=90 + k) for k, s in enumerate(systems)]
+    ta, sa, tb, sb = MODELS["c2c2"]
+    desc = force_desc(ta, sa, tb, sb, flags=L.FLAG_BONDED | L.FLAG_VIRIAL, skin=0.8, interval=0, ewald=True)
+    steps = (30, 20, 30)
+    alone = []
+    for k, s in enumerate(systems):
+        alone.append(_md(desc, [s], [v0s[k]], steps[k], 1.0))
+    with L.Context(desc, [], reserve_bytes=2 << 30) as ctx:
+        done, left = {}, {}
+
+        def admit(k):
+            sid = ctx.add(systems[k])
+            ctx.set_state_systems([sid], [systems[k].x], [v0s[k]])
+            ctx.md_begin([sid])
+            left[sid] = (k, steps[k])
+
+        admit(0)
+        chunk = 10
+        t = 0
+        while left or len(done) < 3:
+            if t == 1:
+                admit(1)
+            if t == 2:
+                admit(2)
+            ids = sorted(left)
+            last = [sid for sid in ids if left[sid][1] == chunk]
+            # all resident systems advance together; those finishing sum energies
+            ctx.md_advance(ids, chunk, 1.0, energies_last=bool(last))
+            for sid in ids:
+                left[sid] = (left[sid][0], left[sid][1] - chunk)
+            if last:
+                ctx.md_end(last, 1.0)
+                xs, vs, fs, es = ctx.get_state_systems(last)
+                for j, sid in enumerate(last):
+                    done[left[sid][0]] = (xs[j], vs[j], fs[j], es[j])
+                    del left[sid]
+                    ctx.remove(sid)
+            t += 1
+    for k in range(3):
+        xa, va, Fa, Ea, _ = alone[k]
+        x, v, F, E = done[k]
+        assert np.array_equal(x, xa) and np.array_equal(v, va) and np.array_equal(F, Fa), k
+        # energies of a finishing system are summed in the same last step; other
+        # systems summing theirs in that launch do not change its numbers
+        assert np.array_equal(E, Ea[0]), k
+
+
+def test_pool_admission_by_footprint():
+    print("case pool footprint", flush=True)
+    a, b, c = _water_set((700, 900, 650), 120)
+    ta, sa, tb, sb = MODELS["c2c2"]
+    desc = force_desc(ta, sa, tb, sb, flags=L.FLAG_BONDED, skin=0.8, interval=0, ewald=True)
+    with L.Context(desc) as probe:
+        fa, fb, fc = probe.footprint(a), probe.footprint(b), probe.footprint(c)
+    assert fc < fa
+    with L.Context(desc, [], reserve_bytes=fa + fb + (fc // 2)) as ctx:
+        ia = ctx.add(a)
+        ctx.add(b)
+        with pytest.raises(L.PMError) as err:
+            ctx.add(c)
+        assert err.value.code == L.PM_EFULL
+        ctx.remove(ia)
+        ic = ctx.add(c)
+        r, u = ctx.memory()
+        assert u <= r
+        f1, e1 = ctx.evaluate_systems([ic], [c.x])
+        f2 = np.zeros((c.n, 3))
+        e2 = np.zeros(L.NSLOTS)
+        ctx.evaluate(ic, c.x, f2, e2)
+        assert np.array_equal(f1[0], f2) and np.array_equal(e1[0], e2)
