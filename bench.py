@@ -1,16 +1,26 @@
 #!/usr/bin/env python3
-"""Headline benchmark: device-resident MD of 32,000 three-site waters
-(96,000 atoms, cubic box 98.6 A) with Lennard-Jones (C2 switch 8-10 A) plus
-Ewald real-space Coulomb (alpha 0.32 A^-1, Rc 10 A) and the harmonic water
-model, leapfrog at dt = 1 fs, pair list rebuilt every 10 steps (and sooner if
-an atom moves more than half the list buffer).  The start is a lattice with
-seeded orientations, relieved of H-H contacts under 2 A and equilibrated for
-a few hundred untimed steps with velocity rescaling to 300 K.
+"""Benchmarks of the nonbonded force path on one MI355X.
 
-Prints one JSON line: pair interactions/s (whole job), ns/day, the pair
-kernel's roofline figures (HIP events around every pair-kernel launch on
-the kernel's own stream) and the CPU baseline (the FP64 oracle on this
-host's cores, a bounded sample of the same workload).
+Headline (the JSON line's ``value``): device-resident MD of 32,000 three-site
+waters (96,000 atoms, cubic box 98.6 A) with Lennard-Jones (C2 switch
+8-10 A) plus Ewald real-space Coulomb (alpha 0.32 A^-1, Rc 10 A) and the
+harmonic water model, leapfrog at dt = 1 fs, pair list rebuilt every 10 steps
+(and sooner if an atom moves more than half the list buffer).  The start is a
+lattice with seeded orientations, relieved of H-H contacts under 2 A and
+equilibrated for a few hundred untimed steps with velocity rescaling to 300 K.
+
+Beside it, in the same line:
+  roofline      the pair kernel's FP32 rate (HIP events around every
+                pair-kernel launch on the kernel's own stream) against peak
+  cpu_baseline  the FP64 oracle (the reference algorithm) on this host's cores
+  batched       64 water boxes of 500-16,000 molecules (log-uniform), all in
+                one context: graph-replayed MD with autotuned launch shapes,
+                against the same systems run one per context (plain kernel)
+  served        a Poisson stream of requests (force evaluation or 200-step
+                MD segment, 50/50) on water boxes of 1,000-32,000 molecules:
+                continuous batching with admission by HBM footprint, against
+                the same requests served one at a time; p50/p99 latency
+  gb            32 GB/OBC bead chains of 5,000 atoms, all pairs, batched
 """
 from __future__ import annotations
 
@@ -20,6 +30,7 @@ import math
 import os
 import sys
 import time
+from multiprocessing import get_context
 
 import numpy as np
 
@@ -27,23 +38,26 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from protomol_amd import _lib as L  # noqa: E402
+from protomol_amd import autotune  # noqa: E402
+from protomol_amd import serve  # noqa: E402
 from protomol_amd import systems as S  # noqa: E402
 
 N_MOL, BOX, DT_FS, REBUILD = 32000, 98.6, 1.0, 10
 ALPHA, ACCURACY = 0.32, math.exp(-10.24)
+DENSITY = 0.0334  # molecules / A^3
 # algorithmic work per within-cutoff pair of the LJ(C2) + Ewald-real kernel:
 # distance 11, LJ 11, C2 switch 13, Ewald real 23, force combine/accumulate 10,
 # energies 2, virial 12 (add/sub/mul = 1, FMA = 2, each rsqrt/rcp/exp = 1)
 FLOP_PER_PAIR = 82
 PEAK_FP32_TFLOPS = 157.3
-PEAK_HBM_GBS = 8000.0
+WORKERS = 16
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def force_desc(skin, timing=True):
+def force_desc(skin, timing=False, rebuild=REBUILD, **shape):
     d = L.ForceDesc()
     d.term_a = L.TERM_LJ
     d.switch_a = L.switch(L.SWITCH_C2, 8.0, 10.0)
@@ -53,9 +67,23 @@ def force_desc(skin, timing=True):
     d.ewald_accuracy = ACCURACY
     d.ewald_correction = 1
     d.skin = skin
-    d.rebuild_interval = REBUILD
+    d.rebuild_interval = rebuild
     d.flags = L.FLAG_BONDED | (L.FLAG_TIMING if timing else 0)
+    for k, v in shape.items():
+        setattr(d, k, v)
     return d
+
+
+def _water(args):
+    n, seed = args
+    return S.water_box(n, (n / DENSITY) ** (1.0 / 3.0), seed=seed, min_contact=2.0)
+
+
+def make_waters(sizes, seed0):
+    """Water boxes at 0.0334 molecules/A^3, generated in parallel."""
+    jobs = [(n, seed0 + k) for k, n in enumerate(sizes)]
+    with get_context("fork").Pool(min(WORKERS, len(jobs))) as pool:
+        return pool.map(_water, jobs)
 
 
 def cpu_baseline(system, seconds):
@@ -67,7 +95,7 @@ def cpu_baseline(system, seconds):
     except Exception as e:  # the checker library is not built
         return {"value": None, "unit": "pair interactions/s", "cores": 0, "kind": "port",
                 "sample": f"unavailable: {e}"}
-    cores = max(1, min(os.cpu_count() or 1, 16))
+    cores = max(1, min(os.cpu_count() or 1, WORKERS))
     sw = O.make_switch(O.SW_C2, 8.0, 10.0)
     n_eval, t_tot, pairs = 0, 0.0, 0
     while t_tot < seconds or n_eval == 0:
@@ -87,29 +115,13 @@ def cpu_baseline(system, seconds):
             "ns_per_day": (1.0 / per) * DT_FS * 0.0864}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--skin", type=float, default=1.2)
-    ap.add_argument("--equil", type=int, default=300, help="untimed equilibration steps (rescaled to 300 K)")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(independent systems shard by placement, see DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
-
+def headline(args):
     log("building the 96,000-atom water box")
     system = S.water_box(N_MOL, BOX, seed=2024, min_contact=2.0)
     v0 = S.maxwell_velocities(system, 300.0, seed=2025)
-    name, ncu, hbm = L.device_info()
-    log(f"device: {name}, {ncu} CUs, {hbm:.0f} GiB")
-    with L.Context(force_desc(args.skin), [system]) as ctx:
+    with L.Context(force_desc(args.skin, timing=True), [system]) as ctx:
         ctx.md_set_state(system.x, v0)
-        seg = 25
+        seg, t_now = 25, 300.0
         for k in range(0, args.equil, seg):
             ctx.md_run(seg, DT_FS, energies=False)
             x, v, _, _ = ctx.md_get_state()
@@ -155,8 +167,175 @@ def main():
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
                      "kernel": "k_pair<LJ, EWALD_REAL>", "kernel_ms": pair_ms,
                      "flop_per_pair": FLOP_PER_PAIR},
-        "device": name,
     }
+    return out, system
+
+
+def batched(args):
+    """64 water boxes (log-uniform 500-16,000 molecules), MD in one context vs one context each."""
+    sizes = S.ensemble_sizes(args.batch_systems, 500, 16000, seed=77)
+    log(f"batched: generating {len(sizes)} water boxes ({sum(sizes) * 3} atoms)")
+    systems = make_waters(sizes, 500)
+    v0 = [S.maxwell_velocities(s, 300.0, seed=900 + k) for k, s in enumerate(systems)]
+    base = force_desc(args.skin)
+    log("batched: autotuning launch shapes on the batch")
+    best, table = autotune.tune(base, systems, v0, DT_FS, steps=20, warm=10, log=log)
+    tuned = autotune.apply(base, best)
+    steps = args.batch_steps
+    with L.Context(tuned, systems) as ctx:
+        ctx.md_set_state(np.concatenate([s.x for s in systems]), np.concatenate(v0))
+        ctx.md_run(20, DT_FS, energies=False)
+        t0 = time.perf_counter()
+        ctx.md_run(steps, DT_FS, energies=False)
+        t_batch = time.perf_counter() - t0
+        rebuilds = ctx.stats()["rebuilds"]
+    log(f"batched: {steps} steps of {len(systems)} systems in {t_batch:.2f} s")
+    # the plain kernel: each system in its own context, one after another
+    t_plain = 0.0
+    for k, s in enumerate(systems):
+        with L.Context(base, [s]) as ctx:
+            ctx.md_set_state(s.x, v0[k])
+            ctx.md_run(20, DT_FS, energies=False)
+            t0 = time.perf_counter()
+            ctx.md_run(steps, DT_FS, energies=False)
+            t_plain += time.perf_counter() - t0
+    log(f"batched: plain one-system-per-context total {t_plain:.2f} s")
+    n = len(systems)
+    return {"workload": f"{n} water boxes, {min(sizes)}-{max(sizes)} molecules (log-uniform 500-16,000), "
+                        f"{sum(sizes) * 3} atoms, LJ C2 + Ewald real + bond/angle, {steps} steps at 1 fs",
+            "system_steps_per_s": n * steps / t_batch, "plain_system_steps_per_s": n * steps / t_plain,
+            "speedup_vs_plain": t_plain / t_batch, "ms_per_step": 1e3 * t_batch / steps,
+            "atom_steps_per_s": sum(sizes) * 3 * steps / t_batch, "rebuilds": rebuilds,
+            "tuned": best, "tuning": [[p, ms] for p, ms in table]}
+
+
+def _requests(templates, n_req, rate_hz, steps, seed):
+    rng = np.random.default_rng(seed)
+    reqs, t = [], 0.0
+    for k in range(n_req):
+        s = templates[int(rng.integers(len(templates)))]
+        kind = "md" if rng.random() < 0.5 else "eval"
+        shift = rng.uniform(-0.5, 0.5, size=3) * s.box
+        x = s.x + shift
+        v = S.maxwell_velocities(s, 300.0, seed=10_000 + k) if kind == "md" else None
+        reqs.append(serve.Request(rid=k, kind=kind, system=s, x=x, v=v, steps=steps if kind == "md" else 0,
+                                  arrival=t))
+        if rate_hz:
+            t += rng.exponential(1.0 / rate_hz)
+    return reqs
+
+
+def served(args):
+    """Poisson request stream on water boxes of 1,000-32,000 molecules."""
+    n_templates = 16
+    sizes = S.ensemble_sizes(n_templates, 1000, 32000, seed=91)
+    log(f"served: generating {n_templates} water boxes for the request stream")
+    templates = make_waters(sizes, 700)
+    desc = force_desc(args.skin, rebuild=0)
+    n_req, steps = args.serve_requests, 200
+    reserve = int(args.serve_reserve_gib * (1 << 30))
+    # plain: the same requests one at a time (each alone on the device)
+    reqs = _requests(templates, n_req, 0.0, steps, seed=5)
+    log(f"served: plain, {n_req} requests one at a time")
+    with L.Context(desc, [], reserve_bytes=reserve) as ctx:
+        srv = serve.Server(ctx, dt_fs=DT_FS, chunk=steps)
+        t0 = time.perf_counter()
+        for r in reqs:
+            srv.submit(r)
+            while not srv.idle():
+                srv.tick()
+        t_plain = time.perf_counter() - t0
+    plain_rate = n_req / t_plain
+    log(f"served: plain {plain_rate:.1f} requests/s")
+    # continuous batching, every request waiting at t = 0 (capacity)
+    reqs = _requests(templates, n_req, 0.0, steps, seed=5)
+    with L.Context(desc, [], reserve_bytes=reserve) as ctx:
+        srv = serve.Server(ctx, dt_fs=DT_FS, chunk=args.serve_chunk)
+        t0 = time.perf_counter()
+        srv.run(reqs, max_seconds=900)
+        t_sat = time.perf_counter() - t0
+    sat_rate = n_req / t_sat
+    log(f"served: batched capacity {sat_rate:.1f} requests/s")
+    # Poisson arrivals at a fixed fraction of that capacity: latency
+    rate = args.serve_load * sat_rate
+    reqs = _requests(templates, n_req, rate, steps, seed=6)
+    with L.Context(desc, [], reserve_bytes=reserve) as ctx:
+        srv = serve.Server(ctx, dt_fs=DT_FS, chunk=args.serve_chunk)
+        t0 = time.perf_counter()
+        done = srv.run(reqs, max_seconds=900)
+        t_poi = time.perf_counter() - t0
+        ticks = srv.ticks
+    lat = serve.latency_summary(done)
+    log(f"served: Poisson {rate:.1f}/s -> p50 {lat['p50_ms']:.1f} ms, p99 {lat['p99_ms']:.1f} ms")
+    return {"workload": f"{n_req} requests, 50/50 force evaluation / {steps}-step MD segment, water boxes "
+                        f"{min(sizes)}-{max(sizes)} molecules ({n_templates} seeded boxes, each request "
+                        f"shifted and with its own velocities), LJ C2 + Ewald real + bond/angle",
+            "plain_requests_per_s": plain_rate, "batched_capacity_requests_per_s": sat_rate,
+            "speedup_vs_plain": sat_rate / plain_rate, "poisson_rate_per_s": rate,
+            "poisson_requests_per_s": n_req / t_poi, "latency": lat, "ticks": ticks,
+            "chunk_steps": args.serve_chunk, "reserve_gib": args.serve_reserve_gib}
+
+
+def gb_batch(args):
+    """32 GB/OBC bead chains of 5,000 atoms, all pairs, MD in one context."""
+    n_chain, n_atoms = args.gb_chains, 5000
+    log(f"gb: generating {n_chain} chains of {n_atoms} atoms")
+    with get_context("fork").Pool(min(WORKERS, n_chain)) as pool:
+        chains = pool.starmap(S.gb_chain, [(n_atoms, 3000 + k) for k in range(n_chain)])
+    d = L.ForceDesc()
+    d.gb_offset, d.gb_alpha, d.gb_beta, d.gb_gamma = 0.09, 1.0, 0.8, 4.85
+    d.gb_solute_dielec, d.gb_solvent_dielec = 1.0, 78.3
+    d.flags = L.FLAG_GB
+    v0 = [S.maxwell_velocities(c, 300.0, seed=4000 + k) for k, c in enumerate(chains)]
+    steps = args.gb_steps
+    with L.Context(d, chains) as ctx:
+        ctx.md_set_state(np.concatenate([c.x for c in chains]), np.concatenate(v0))
+        ctx.md_run(5, DT_FS, energies=False)
+        t0 = time.perf_counter()
+        ctx.md_run(steps, DT_FS, energies=False)
+        t = time.perf_counter() - t0
+    pairs = n_chain * n_atoms * (n_atoms - 1) // 2
+    log(f"gb: {steps} steps in {t:.2f} s")
+    return {"workload": f"{n_chain} chains x {n_atoms} atoms, GB/OBC + LJ + Coulomb, all pairs, {steps} steps at 1 fs",
+            "ms_per_step": 1e3 * t / steps, "system_steps_per_s": n_chain * steps / t,
+            "pair_interactions_per_s": pairs * steps / t}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--skin", type=float, default=1.2)
+    ap.add_argument("--equil", type=int, default=300, help="untimed equilibration steps (rescaled to 300 K)")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", default="batched,served,gb",
+                    help="comma list of extra forms beside the headline (batched, served, gb) or 'none'")
+    ap.add_argument("--only", default="", help="run one extra form alone: batched, served or gb")
+    ap.add_argument("--batch-systems", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-chunk", type=int, default=20)
+    ap.add_argument("--serve-load", type=float, default=0.5)
+    ap.add_argument("--serve-reserve-gib", type=float, default=24.0)
+    ap.add_argument("--gb-chains", type=int, default=32)
+    ap.add_argument("--gb-steps", type=int, default=200)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(independent systems shard by placement, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    name, ncu, hbm = L.device_info()
+    log(f"device: {name}, {ncu} CUs, {hbm:.0f} GiB")
+    forms = {"batched": batched, "served": served, "gb": gb_batch}
+    if args.only:
+        print(json.dumps({args.only: forms[args.only](args), "device": name}), flush=True)
+        return
+    out, system = headline(args)
+    out["device"] = name
+    for f in [f for f in args.forms.split(",") if f and f != "none"]:
+        out[f] = forms[f](args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(system, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -136,6 +136,9 @@ typedef struct pm_force_desc {
                                   (0 = only by the displacement test); it is always rebuilt
                                   when an atom has moved more than skin/2 since the build */
   uint32_t flags;              /* PM_FLAG_* */
+  /* launch shapes (0 = default; an autotuner may choose them per workload) */
+  double cell_scale;           /* x-y column width as a multiple of (8/density)^(1/3): 0.6 .. 2 */
+  int32_t pair_waves;          /* waves per workgroup of the pair kernel: 1, 2, 4 (default) or 8 */
 } pm_force_desc;
 
 typedef struct pm_ctx pm_ctx;

@@ -297,6 +297,14 @@ void dev_free(pm_ctx *c, void *p, size_t bytes) {
 
 void drop_sets(pm_ctx *c) { c->sets.clear(); }
 
+// forget the launch tables that hold a copy of system `id`
+void drop_sets_with(pm_ctx *c, int id) {
+  for (auto it = c->sets.begin(); it != c->sets.end();) {
+    if (std::find(it->first.begin(), it->first.end(), id) != it->first.end()) it = c->sets.erase(it);
+    else ++it;
+  }
+}
+
 std::vector<int> live_ids(const pm_ctx *c) {
   std::vector<int> v;
   for (size_t i = 0; i < c->sys.size(); ++i)
@@ -466,7 +474,8 @@ SysPrep prepare(const pm_ctx *c, const pm_system_desc &d) {
   // x-y column grid: columns about (8/rho)^(1/3) wide so 8-atom clusters are compact
   const double V = d.box[0] * d.box[1] * d.box[2];
   const double rho = n / V;
-  const double a = std::cbrt(pm::kCluster / rho);
+  const double scale = c->desc.cell_scale > 0 ? c->desc.cell_scale : 1.0;
+  const double a = scale * std::cbrt(pm::kCluster / rho);
   const double rlist = c->rc + std::max(c->skin_md, c->skin_eval);
   for (int k = 0; k < 3; ++k)
     if (d.box[k] < 2.0 * rlist)
@@ -704,14 +713,14 @@ int add_system(pm_ctx *c, const pm_system_desc &d) {
   }
   h.id = id;
   c->sys[id] = std::move(hp);
-  drop_sets(c);
+  drop_sets_with(c, id);  // a reused id must not find a stale table
   return id;
 }
 
 void remove_system(pm_ctx *c, int id) {
   SysHost &h = sys_of(c, id);
   HIPCHECK(hipStreamSynchronize(c->stream));
-  drop_sets(c);
+  drop_sets_with(c, id);
   dev_free(c, h.lists, h.lists_bytes);
   dev_free(c, h.block, h.block_bytes);
   c->sys[id].reset();
@@ -722,7 +731,10 @@ void remove_system(pm_ctx *c, int id) {
 LaunchSet &get_set(pm_ctx *c, const std::vector<int> &ids) {
   auto it = c->sets.find(ids);
   if (it != c->sets.end()) return *it->second;
-  if (c->sets.size() >= 64) c->sets.clear();
+  if (c->sets.size() >= 64) {
+    HIPCHECK(hipStreamSynchronize(c->stream));  // tables may be in use by queued work
+    c->sets.clear();
+  }
   auto ls = std::make_unique<LaunchSet>();
   ls->ids = ids;
   const int ns = (int)ids.size();
@@ -1184,6 +1196,11 @@ int pm_ctx_create(const pm_force_desc *force, int32_t n_systems, const pm_system
     ForceDev &P = c->P;
     std::memset(&P, 0, sizeof P);
     P.flags = force->flags;
+    P.pair_waves = force->pair_waves ? force->pair_waves : 4;
+    if (P.pair_waves != 1 && P.pair_waves != 2 && P.pair_waves != 4 && P.pair_waves != 8)
+      throw Fail{PM_EINVAL, "pair_waves must be 1, 2, 4 or 8"};
+    if (force->cell_scale != 0.0 && !(force->cell_scale >= 0.6 && force->cell_scale <= 2.0))
+      throw Fail{PM_EINVAL, "cell_scale must be 0 (default) or within 0.6 .. 2"};
     c->gb = (force->flags & PM_FLAG_GB) != 0;
     if (c->gb) {
       pm::gb_setup_force(P, *force);

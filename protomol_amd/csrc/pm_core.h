@@ -70,6 +70,7 @@ struct ForceDev {
   double dalpha, dtwo_alpha_pi;             // Ewald correction (FP64)
   float rlist2;
   unsigned flags;
+  int pair_waves;          // waves per workgroup of the pair kernel (1, 2, 4 or 8)
   // GB / OBC (ProtoMolApp.cpp:285-301, GBForce.cpp:11-12)
   double gb_offset, gb_a, gb_b, gb_g, gb_fac;  // fac = 1/eps_solute - 1/eps_solvent
 };

@@ -618,14 +618,15 @@ constexpr int kLJLds = 256;  // LJ table entries kept in LDS per wave (T <= 16)
 // land, like the i-forces, in 64-bit fixed point, so the total is independent
 // of the order in which waves finish.
 template <int TA, int TB, int SWA, int SWB, int KIND>
-__global__ __launch_bounds__(256) void k_pair(const SysDev *__restrict__ sys, const int *__restrict__ clp, int nsys,
+__global__ __launch_bounds__(512) void k_pair(const SysDev *__restrict__ sys, const int *__restrict__ clp, int nsys,
                                               ForceDev P) {
   constexpr bool FORCES = KIND != kPairEnergies;
   constexpr bool EN = KIND == kPairForcesEnergies || KIND == kPairEnergies;
   constexpr bool CEN = KIND == kPairCensus;
-  __shared__ float4 ljs[4][kLJLds];
+  extern __shared__ float4 lj_lds[];  // kLJLds entries per wave of the workgroup
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
+  float4 *ljs = lj_lds + wv * kLJLds;
   const int w = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   if (w >= clp[nsys]) return;
   const int s = __builtin_amdgcn_readfirstlane(find_sys(clp, nsys, w));
@@ -648,7 +649,7 @@ __global__ __launch_bounds__(256) void k_pair(const SysDev *__restrict__ sys, co
   const bool tab_lds = TA == PM_TERM_LJ && !peratom;  // T <= 16 (checked at context creation)
   const PM_GLOBAL float4 *gtab = gp(S->lj_table);
   if (tab_lds) {
-    for (int k = lane; k < T * T; k += kWave) ljs[wv][k] = gtab[k];
+    for (int k = lane; k < T * T; k += kWave) ljs[k] = gtab[k];
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -725,7 +726,7 @@ __global__ __launch_bounds__(256) void k_pair(const SysDev *__restrict__ sys, co
           B *= ljsc;
         }
       } else {
-        const float4 p = ljs[wv][tiT + tj];
+        const float4 p = ljs[tiT + tj];
         A = mod ? p.z : p.x;
         B = mod ? p.w : p.y;
       }
@@ -1186,19 +1187,24 @@ void launch_rebuild(const Launch &L, float rlist2) {
 
 template <int TA, int TB, int SWA, int SWB>
 static void launch_pair_t(const Launch &L, const ForceDev &P, int kind) {
-  dim3 g(nblk((long long)L.n_cl_total * kWave, 256)), b(256);
+  // one wave per i-cluster; P.pair_waves waves per workgroup, each with its
+  // own LDS copy of the LJ table
+  const int waves = P.pair_waves;
+  const int threads = kWave * waves;
+  dim3 g(nblk((long long)L.n_cl_total * kWave, threads)), b(threads);
+  const size_t lds = (size_t)waves * kLJLds * sizeof(float4);
   switch (kind) {
   case kPairForces:
-    hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairForces>), g, b, 0, L.stream, L.sys, L.cl_prefix, L.nsys, P);
+    hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairForces>), g, b, lds, L.stream, L.sys, L.cl_prefix, L.nsys, P);
     break;
   case kPairForcesEnergies:
-    hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairForcesEnergies>), g, b, 0, L.stream, L.sys, L.cl_prefix, L.nsys, P);
+    hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairForcesEnergies>), g, b, lds, L.stream, L.sys, L.cl_prefix, L.nsys, P);
     break;
   case kPairEnergies:
-    hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairEnergies>), g, b, 0, L.stream, L.sys, L.cl_prefix, L.nsys, P);
+    hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairEnergies>), g, b, lds, L.stream, L.sys, L.cl_prefix, L.nsys, P);
     break;
   default:
-    hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairCensus>), g, b, 0, L.stream, L.sys, L.cl_prefix, L.nsys, P);
+    hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairCensus>), g, b, lds, L.stream, L.sys, L.cl_prefix, L.nsys, P);
     break;
   }
 }
