@@ -119,7 +119,8 @@ def headline(args):
     log("building the 96,000-atom water box")
     system = S.water_box(N_MOL, BOX, seed=2024, min_contact=2.0)
     v0 = S.maxwell_velocities(system, 300.0, seed=2025)
-    with L.Context(force_desc(args.skin, timing=True), [system]) as ctx:
+    shape = {"cell_scale": args.cell_scale, "pair_waves": args.pair_waves}
+    with L.Context(force_desc(args.skin, timing=not args.no_timing, **shape), [system]) as ctx:
         ctx.md_set_state(system.x, v0)
         seg, t_now = 25, 300.0
         for k in range(0, args.equil, seg):
@@ -161,7 +162,10 @@ def headline(args):
                                "alpha 0.32 Rc 10 A + bond/angle, leapfrog dt 1 fs, pair list every 10 steps",
                    "atoms": system.n, "pairs_per_step": pairs, "skin_A": args.skin, "rebuild_interval": REBUILD,
                    "rebuilds": st["rebuilds"], "max_displacement_A": st["max_displacement"],
-                   "equilibration_steps": args.equil},
+                   "list_entries": st["list_entries"],
+                   "list_efficiency": pairs / max(1.0, 64.0 * st["list_entries"]),
+                   "equilibration_steps": args.equil, "cell_scale": args.cell_scale or 1.0,
+                   "pair_waves": args.pair_waves or 4},
         "ns_per_day": steps_s * DT_FS * 0.0864,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
@@ -234,9 +238,26 @@ def served(args):
     desc = force_desc(args.skin, rebuild=0)
     n_req, steps = args.serve_requests, 200
     reserve = int(args.serve_reserve_gib * (1 << 30))
-    # plain: the same requests one at a time (each alone on the device)
+    # plain: each request on its own through the drop-in API (a context per
+    # request: evaluate, or set state + LeapfrogIntegrator::run), one at a time
     reqs = _requests(templates, n_req, 0.0, steps, seed=5)
-    log(f"served: plain, {n_req} requests one at a time")
+    log(f"served: plain API, {n_req} requests one at a time")
+    t0 = time.perf_counter()
+    for r in reqs:
+        with L.Context(desc, [r.system]) as ctx:
+            if r.kind == "eval":
+                f = np.zeros_like(r.x)
+                e = np.zeros(L.NSLOTS)
+                ctx.evaluate(0, r.x, f, e)
+            else:
+                ctx.md_set_state(r.x, r.v)
+                ctx.md_run(steps, DT_FS, energies=True)
+                ctx.md_get_state()
+    t_plain = time.perf_counter() - t0
+    plain_rate = n_req / t_plain
+    log(f"served: plain API {plain_rate:.1f} requests/s")
+    # the same requests one at a time inside the server (no batching)
+    reqs = _requests(templates, n_req, 0.0, steps, seed=5)
     with L.Context(desc, [], reserve_bytes=reserve) as ctx:
         srv = serve.Server(ctx, dt_fs=DT_FS, chunk=steps)
         t0 = time.perf_counter()
@@ -244,9 +265,9 @@ def served(args):
             srv.submit(r)
             while not srv.idle():
                 srv.tick()
-        t_plain = time.perf_counter() - t0
-    plain_rate = n_req / t_plain
-    log(f"served: plain {plain_rate:.1f} requests/s")
+        t_serial = time.perf_counter() - t0
+    serial_rate = n_req / t_serial
+    log(f"served: one-at-a-time server {serial_rate:.1f} requests/s")
     # continuous batching, every request waiting at t = 0 (capacity)
     reqs = _requests(templates, n_req, 0.0, steps, seed=5)
     with L.Context(desc, [], reserve_bytes=reserve) as ctx:
@@ -270,8 +291,9 @@ def served(args):
     return {"workload": f"{n_req} requests, 50/50 force evaluation / {steps}-step MD segment, water boxes "
                         f"{min(sizes)}-{max(sizes)} molecules ({n_templates} seeded boxes, each request "
                         f"shifted and with its own velocities), LJ C2 + Ewald real + bond/angle",
-            "plain_requests_per_s": plain_rate, "batched_capacity_requests_per_s": sat_rate,
-            "speedup_vs_plain": sat_rate / plain_rate, "poisson_rate_per_s": rate,
+            "plain_requests_per_s": plain_rate, "one_at_a_time_requests_per_s": serial_rate,
+            "batched_capacity_requests_per_s": sat_rate, "speedup_vs_plain": sat_rate / plain_rate,
+            "speedup_vs_one_at_a_time": sat_rate / serial_rate, "poisson_rate_per_s": rate,
             "poisson_requests_per_s": n_req / t_poi, "latency": lat, "ticks": ticks,
             "chunk_steps": args.serve_chunk, "reserve_gib": args.serve_reserve_gib}
 
@@ -308,8 +330,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--skin", type=float, default=1.2)
     ap.add_argument("--equil", type=int, default=300, help="untimed equilibration steps (rescaled to 300 K)")
+    ap.add_argument("--cell-scale", type=float, default=0.0, help="column width factor (0 = default 1.0)")
+    ap.add_argument("--pair-waves", type=int, default=0, help="pair-kernel waves per workgroup (0 = default 4)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-timing", action="store_true",
+                    help="no per-launch HIP events (MD steps replay a captured graph; no roofline figure)")
     ap.add_argument("--forms", default="batched,served,gb",
                     help="comma list of extra forms beside the headline (batched, served, gb) or 'none'")
     ap.add_argument("--only", default="", help="run one extra form alone: batched, served or gb")

@@ -204,8 +204,8 @@ int pm_md_end(pm_ctx *ctx, int32_t n, const int32_t *ids, double dt_fs);
 int pm_md_get_state_systems(pm_ctx *ctx, int32_t n, const int32_t *ids, double *x, double *v, double *f, double *e);
 
 /* Statistics of the last call: stats[0] pair-list rebuilds (summed over
- * systems), [1] 1 if an MD advance was re-run after a pair-list capacity grew, [2] pairs evaluated (last
- * evaluation, all systems), [3] device ms of the last call,
+ * systems), [1] 1 if an MD advance was re-run after a pair-list capacity grew, [2] entries (8x8
+ * cluster pairs) of the pair lists at their last build, summed over systems, [3] device ms of the last call,
  * [4] pair-kernel ms summed over the call's launches (PM_FLAG_TIMING, or the
  * single launch of an evaluation), [5] pair-kernel launches timed,
  * [6] largest displacement (A) of an atom from its position at the pair-list

@@ -335,5 +335,5 @@ class Context:
     def stats(self):
         s = np.zeros(8)
         check(lib().pm_stats(self.h, ptr(s), 8), self.h)
-        return {"rebuilds": s[0], "regrow_reruns": s[1], "pairs": s[2], "device_ms": s[3], "pair_ms": s[4],
+        return {"rebuilds": s[0], "regrow_reruns": s[1], "list_entries": s[2], "device_ms": s[3], "pair_ms": s[4],
                 "pair_launches": s[5], "max_displacement": s[6], "skin": s[7]}

@@ -24,8 +24,8 @@ import numpy as np
 from . import _lib as L
 
 DEFAULT_GRID = {
-    "skin": (0.8, 1.2),
-    "cell_scale": (0.8, 1.0, 1.25),
+    "skin": (1.0, 1.3),
+    "cell_scale": (1.0, 1.25, 1.5),
     "pair_waves": (2, 4),
 }
 

@@ -408,22 +408,24 @@ SysPrep prepare(const pm_ctx *c, const pm_system_desc &d) {
     if (d.bonds[k] < 0 || d.bonds[k] >= n) throw Fail{PM_EINVAL, "bond atom out of range"};
   for (int k = 0; k < 3 * p.n_angles; ++k)
     if (d.angles[k] < 0 || d.angles[k] >= n) throw Fail{PM_EINVAL, "angle atom out of range"};
-  // per-atom items of the owner-computes kernel: (index << 4) | (kind << 2) | role
+  // per-atom items of the owner-computes kernel: (index << 2) | role, in a CSR
+  // keyed by (atom, kind) -- entry 3 * atom + kind -- so the kernel walks
+  // one kind at a time
   {
-    std::vector<int> cnt(n + 1, 0);
-    for (int k = 0; k < p.n_bonds; ++k) for (int r = 0; r < 2; ++r) cnt[d.bonds[2 * k + r] + 1]++;
-    for (int k = 0; k < p.n_angles; ++k) for (int r = 0; r < 3; ++r) cnt[d.angles[3 * k + r] + 1]++;
-    for (int k = 0; k < p.n_corr; ++k) for (int r = 0; r < 2; ++r) cnt[p.corr[3 * k + r] + 1]++;
-    for (int i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+    std::vector<int> cnt(3 * (size_t)n + 1, 0);
+    for (int k = 0; k < p.n_bonds; ++k) for (int r = 0; r < 2; ++r) cnt[3 * d.bonds[2 * k + r] + pm::kLocBond + 1]++;
+    for (int k = 0; k < p.n_angles; ++k) for (int r = 0; r < 3; ++r) cnt[3 * d.angles[3 * k + r] + pm::kLocAngle + 1]++;
+    for (int k = 0; k < p.n_corr; ++k) for (int r = 0; r < 2; ++r) cnt[3 * p.corr[3 * k + r] + pm::kLocCorr + 1]++;
+    for (size_t i = 0; i + 1 < cnt.size(); ++i) cnt[i + 1] += cnt[i];
     p.loc_start = cnt;
-    p.loc_item.resize(cnt[n]);
+    p.loc_item.resize(cnt.back());
     std::vector<int> fill(cnt.begin(), cnt.end() - 1);
     for (int k = 0; k < p.n_bonds; ++k)
-      for (int r = 0; r < 2; ++r) p.loc_item[fill[d.bonds[2 * k + r]]++] = (k << 4) | (pm::kLocBond << 2) | r;
+      for (int r = 0; r < 2; ++r) p.loc_item[fill[3 * d.bonds[2 * k + r] + pm::kLocBond]++] = (k << 2) | r;
     for (int k = 0; k < p.n_angles; ++k)
-      for (int r = 0; r < 3; ++r) p.loc_item[fill[d.angles[3 * k + r]]++] = (k << 4) | (pm::kLocAngle << 2) | r;
+      for (int r = 0; r < 3; ++r) p.loc_item[fill[3 * d.angles[3 * k + r] + pm::kLocAngle]++] = (k << 2) | r;
     for (int k = 0; k < p.n_corr; ++k)
-      for (int r = 0; r < 2; ++r) p.loc_item[fill[p.corr[3 * k + r]]++] = (k << 4) | (pm::kLocCorr << 2) | r;
+      for (int r = 0; r < 2; ++r) p.loc_item[fill[3 * p.corr[3 * k + r] + pm::kLocCorr]++] = (k << 2) | r;
   }
   // per-atom data
   p.invm.resize(n);
@@ -540,9 +542,12 @@ size_t carve(const pm_ctx *c, const SysPrep &p, const pm_system_desc &d, Carver 
   S.ctl = cv.take<int>(pm::kCtlN);
   *ctl_snap = cv.take<int>(pm::kCtlN);
   S.xq = cv.take<float4>(p.slot_cap);
-  S.fx = cv.take<long long>(p.slot_cap);
-  S.fy = cv.take<long long>(p.slot_cap);
-  S.fz = cv.take<long long>(p.slot_cap);
+  // x, y, z components of the fixed-point forces at a fixed stride, so a lane
+  // can address its component as fx + component * f_stride
+  S.f_stride = (int)(align_up((size_t)p.slot_cap * sizeof(long long)) / sizeof(long long));
+  S.fx = cv.take<long long>(3 * (size_t)S.f_stride);
+  S.fy = S.fx ? S.fx + S.f_stride : nullptr;
+  S.fz = S.fx ? S.fx + 2 * (size_t)S.f_stride : nullptr;
   if (c->gb) {
     S.gb_rho = cv.take<double>(n);
     S.gb_S = cv.take<double>(n);
@@ -976,6 +981,8 @@ void evaluate_ids(pm_ctx *c, const std::vector<int> &ids, const double *x, doubl
     HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     HIPCHECK(hipEventElapsedTime(&pms, c->evp0, c->evp1));
     c->stats[0] = 1 + attempt;
+    c->stats[2] = 0;
+    for (size_t k = 0; k < ids.size(); ++k) c->stats[2] += (double)ls.h_scal[k * pm::kScalN + pm::kScalCtl + pm::kCtlEntries];
     c->stats[3] = ms;
     c->stats[4] = pms;
     c->stats[5] = 1;
@@ -1108,11 +1115,13 @@ void md_advance(pm_ctx *c, const std::vector<int> &ids, int n_steps, double dt_f
       pair_ms += t;
     }
     unsigned worst = 0;
-    double rebuilds = 0;
+    double rebuilds = 0, entries = 0;
     for (size_t k = 0; k < ids.size(); ++k) {
       worst = std::max(worst, (unsigned)fs.h_scal[k * pm::kScalN + pm::kScalMaxdispRun]);
       rebuilds += (double)fs.h_scal[k * pm::kScalN + pm::kScalCtl + pm::kCtlCount];
+      entries += (double)fs.h_scal[k * pm::kScalN + pm::kScalCtl + pm::kCtlEntries];
     }
+    c->stats[2] = entries;
     float w;
     std::memcpy(&w, &worst, sizeof w);
     c->stats[0] = rebuilds;

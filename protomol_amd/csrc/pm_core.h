@@ -30,21 +30,23 @@ enum { kAccLJ = 0, kAccCoul = 1, kAccVxx, kAccVxy, kAccVxz, kAccVyy, kAccVyz, kA
 // device-side status bits (per system)
 enum { kErrColumnCap = 1, kErrListCap = 2, kErrNonFinite = 4, kErrClusterCap = 8 };
 
-// items of the per-atom list of bonded/correction terms: (index << 4) | (kind << 2) | role
+// items of the per-atom lists of bonded/correction terms, (index << 2) | role,
+// in a CSR keyed by (atom, kind)
 enum { kLocBond = 0, kLocAngle = 1, kLocCorr = 2 };
 enum { kLocalForces = 1, kLocalEnergies = 2 };
 // host staging transfers (k_import / k_export)
 enum { kXferX = 1, kXferV = 2, kXferF = 4 };
 // packed per-system scalars (k_gather_scalars): acc[0..16), census[16..18), err,
 // ctl[19..23), largest squared displacement of the run (float bits)
-enum { kScalCensus = 16, kScalErr = 18, kScalCtl = 19, kScalMaxdispRun = 23, kScalN = 24 };
+enum { kScalCensus = 16, kScalErr = 18, kScalCtl = 19, kScalMaxdispRun = 25, kScalN = 26 };
 // what a pair-kernel launch accumulates
 enum { kPairForces = 0, kPairForcesEnergies = 1, kPairEnergies = 2, kPairCensus = 3 };
 
 // per-system control words: [0] rebuild the pair list in this step (0/1),
 // [1] steps since the last rebuild, [2] rebuilds since the start of the run,
-// [3] the next kick is the first of a run (a half kick)
-enum { kCtlRebuild = 0, kCtlSince = 1, kCtlCount = 2, kCtlFirst = 3, kCtlN = 4 };
+// [3] the next kick is the first of a run (a half kick), [4] entries (8x8
+// cluster pairs) of the pair list at its last build
+enum { kCtlRebuild = 0, kCtlSince = 1, kCtlCount = 2, kCtlFirst = 3, kCtlEntries = 4, kCtlN = 6 };
 // k_step_begin modes (bit set)
 enum { kStepForce = 1, kStepAuto = 2 };
 
@@ -109,7 +111,8 @@ struct SysDev {
   float2 *ljp;             // per-atom mode (sigma/2, sqrt(eps))
   int *orig;               // slot -> atom (-1 padding)
   int *smol;               // slot -> molecule (-1 padding)
-  long long *fx, *fy, *fz; // fixed-point forces
+  long long *fx, *fy, *fz; // fixed-point forces; fy = fx + f_stride, fz = fx + 2 f_stride
+  int f_stride;
   // clusters
   double *cl_origin;       // 3 * cl_cap SoA
   float4 *cl_lo, *cl_hi;   // bounding boxes (w = unused)
@@ -127,7 +130,7 @@ struct SysDev {
   // Ewald correction pairs (unique exclusion table pairs)
   int n_corr;
   const int *corr;         // 3 per pair: a, b, class
-  const int *loc_start;    // n + 1: per-atom CSR of bonded/correction items
+  const int *loc_start;    // 3 n + 1: CSR of bonded/correction items by (atom, kind)
   const int *loc_item;
   // GB
   const double *gb_rho, *gb_S;

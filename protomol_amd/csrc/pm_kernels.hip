@@ -471,106 +471,204 @@ __device__ __forceinline__ int excl_lookup(const SysDev &S, int a, int b) {
   return 0;
 }
 
-// one wave per i-cluster: geometric half-shell cluster-pair list within rlist
-__global__ __launch_bounds__(256) void k_pairlist(SysDev *sys, const int *clp, int nsys, float rlist2) {
-  const int lane = threadIdx.x & 63;
-  const int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  if (w >= clp[nsys]) return;
-  const int s = find_sys(clp, nsys, w);
+// Cluster-pair list within rlist, half shell over columns.  One workgroup per
+// i column: the boxes of the j clusters in the column's window of neighbour
+// columns (periodic x/y image applied) are staged in LDS once, then each wave
+// tests its i clusters against them, one z image at a time.  Windows larger
+// than the LDS stage are done in several passes.
+constexpr int kListThreads = 1024;
+constexpr int kListWaves = kListThreads / kWave;
+constexpr int kCandCap = 1536;   // staged j clusters per pass
+constexpr int kWinCap = 128;     // neighbour columns per pass
+constexpr int kColCap = kSortCap / kCluster;  // clusters of one column
+static_assert(kColCap < kCandCap, "a column's clusters must fit one LDS stage");
+
+__global__ __launch_bounds__(kListThreads) void k_pairlist(SysDev *sys, const int *colp, int nsys, float rlist2) {
+  __shared__ float4 c_lo[kCandCap], c_hi[kCandCap];
+  __shared__ int c_id[kCandCap];             // cj | (nvalid - 1) << 28
+  __shared__ int w_col[kWinCap], w_start[kWinCap + 1], w_sxy[kWinCap];
+  __shared__ int n_win, next_q, i_count[kColCap];
+  const int gcol = blockIdx.x;
+  if (gcol >= colp[nsys]) return;
+  const int s = find_sys(colp, nsys, gcol);
   SysDev &S = sys[s];
-  const int ci = w - clp[s];
-  if (!S.ctl[kCtlRebuild] || ci >= *S.n_clusters) return;
-  const int coli = S.cl_col[ci];
+  if (!S.ctl[kCtlRebuild]) return;
+  const int coli = gcol - colp[s];
+  const int ci0 = S.col_cl_start[coli], nci = S.col_ncl[coli];
+  if (nci == 0) return;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  if (nci > kColCap) {  // the column sort has already flagged it
+    if (t == 0) atomicOr(S.err, (unsigned)kErrColumnCap);
+    return;
+  }
   const int cxi = coli % S.ncx, cyi = coli / S.ncx;
-  const float4 loi = S.cl_lo[ci], hii = S.cl_hi[ci];
-  const double oix = S.cl_origin[ci], oiy = S.cl_origin[S.cl_cap + ci], oiz = S.cl_origin[2 * S.cl_cap + ci];
   const float Lx = (float)S.L[0], Ly = (float)S.L[1], Lz = (float)S.L[2];
   const float ax = (float)S.ax, ay = (float)S.ay;
+  const float ix0 = cxi * ax, ix1 = (cxi + 1) * ax, iy0 = cyi * ay, iy1 = (cyi + 1) * ay;
   const int cap = S.list_cap;
-  uint4 *out = S.list + (size_t)ci * cap;
-  int count = 0;
-  for (int dy = -S.ry; dy <= S.ry; ++dy) {
-    const int cyu = cyi + dy;
-    const int sy = floordiv(cyu, S.ncy);
-    const int cyj = cyu - sy * S.ncy;
-    const float ylo = cyu * ay, yhi = (cyu + 1) * ay;
-    const float gy = fmaxf(0.f, fmaxf(ylo - hii.y, loi.y - yhi) - 0.01f);
-    if (gy * gy >= rlist2) continue;
-    for (int dx = -S.rx; dx <= S.rx; ++dx) {
-      if (dy < 0 || (dy == 0 && dx < 0)) continue;  // half shell over columns
-      const int cxu = cxi + dx;
-      const int sx = floordiv(cxu, S.ncx);
-      const int cxj = cxu - sx * S.ncx;
-      const float xlo = cxu * ax, xhi = (cxu + 1) * ax;
-      const float gx = fmaxf(0.f, fmaxf(xlo - hii.x, loi.x - xhi) - 0.01f);
-      if (gx * gx + gy * gy >= rlist2) continue;
-      const int colj = cyj * S.ncx + cxj;
-      const int c0 = S.col_cl_start[colj], nc = S.col_ncl[colj];
-      const bool samecol = dx == 0 && dy == 0;
-      const float sxL = sx * Lx, syL = sy * Ly;
+  const int wx = 2 * S.rx + 1, nwin_all = wx * (2 * S.ry + 1);
+  for (int k = t; k < nci; k += kListThreads) i_count[k] = 0;
+  if (t == 0) next_q = 0;
+  __syncthreads();
+  while (true) {
+    // choose this pass's neighbour columns (thread 0, in window order)
+    if (t == 0) {
+      int q = next_q, nw = 0, tot = 0;
+      w_start[0] = 0;
+      for (; q < nwin_all && nw < kWinCap; ++q) {
+        const int dy = q / wx - S.ry, dx = q % wx - S.rx;
+        if (dy < 0 || (dy == 0 && dx < 0)) continue;  // half shell over columns
+        const int cyu = cyi + dy, cxu = cxi + dx;
+        const float gy = fmaxf(0.f, fmaxf(cyu * ay - iy1, iy0 - (cyu + 1) * ay) - 0.01f);
+        const float gx = fmaxf(0.f, fmaxf(cxu * ax - ix1, ix0 - (cxu + 1) * ax) - 0.01f);
+        if (gx * gx + gy * gy >= rlist2) continue;
+        const int sy = floordiv(cyu, S.ncy), sx = floordiv(cxu, S.ncx);
+        const int colj = (cyu - sy * S.ncy) * S.ncx + (cxu - sx * S.ncx);
+        const int nc = S.col_ncl[colj];
+        if (tot + nc > kCandCap) break;  // next pass (nc <= kColCap < kCandCap: a pass never ends empty)
+        w_col[nw] = colj;
+        w_sxy[nw] = (sx + 1) | ((sy + 1) << 2) | ((dx == 0 && dy == 0) ? 16 : 0);
+        tot += nc;
+        w_start[++nw] = tot;
+      }
+      n_win = nw;
+      next_q = q;
+    }
+    __syncthreads();
+    const int nw = n_win, ncand = w_start[nw];
+    // stage the window's j-cluster boxes with their x/y image applied
+    for (int k = t; k < ncand; k += kListThreads) {
+      int w = 0;
+      while (w_start[w + 1] <= k) ++w;
+      const int cj = S.col_cl_start[w_col[w]] + (k - w_start[w]);
+      const int sxy = w_sxy[w];
+      const float sxL = ((sxy & 3) - 1) * Lx, syL = (((sxy >> 2) & 3) - 1) * Ly;
+      const int nvj = S.cl_nvalid[cj];
+      const bool empty = nvj == 0;
+      float4 lo = S.cl_lo[cj], hi = S.cl_hi[cj];
+      lo.x += sxL; hi.x += sxL;
+      lo.y += syL; hi.y += syL;
+      lo.w = __int_as_float(sxy);
+      if (empty) lo.x = 1e30f;  // never within range
+      c_lo[k] = lo;
+      c_hi[k] = hi;
+      c_id[k] = cj | (max(nvj - 1, 0) << 28);
+    }
+    __syncthreads();
+    // each wave: its i clusters against the staged boxes
+    for (int il = wv; il < nci; il += kListWaves) {
+      const int ci = ci0 + il;
+      if (S.cl_nvalid[ci] == 0) continue;
+      const float4 loi = S.cl_lo[ci], hii = S.cl_hi[ci];
+      uint4 *out = S.list + (size_t)ci * cap;
+      int count = i_count[il];
       for (int sz = -1; sz <= 1; ++sz) {
+        // a z image is a candidate only if the i box comes within rlist of
+        // that image of the wrapped frame [0, Lz)
+        if (sz == 1 && fmaxf(0.f, Lz - hii.z - 0.01f) * fmaxf(0.f, Lz - hii.z - 0.01f) >= rlist2) continue;
+        if (sz == -1 && fmaxf(0.f, loi.z - 0.01f) * fmaxf(0.f, loi.z - 0.01f) >= rlist2) continue;
         const float szL = sz * Lz;
-        for (int base = 0; base < nc; base += kWave) {
+        for (int base = 0; base < ncand; base += kWave) {
           const int k = base + lane;
-          const int cj = c0 + k;
           bool hit = false;
-          if (k < nc && !(samecol && !(sz > 0 || (sz == 0 && cj >= ci)))) {
-            const float4 loj = S.cl_lo[cj], hij = S.cl_hi[cj];
-            float ex = fmaxf(0.f, fmaxf(loj.x + sxL - hii.x, loi.x - (hij.x + sxL)));
-            float ey = fmaxf(0.f, fmaxf(loj.y + syL - hii.y, loi.y - (hij.y + syL)));
-            float ez = fmaxf(0.f, fmaxf(loj.z + szL - hii.z, loi.z - (hij.z + szL)));
-            hit = ex * ex + ey * ey + ez * ez < rlist2;
+          int cj = 0, sxy = 0, idw = 0;
+          if (k < ncand) {
+            const float4 loj = c_lo[k], hij = c_hi[k];
+            idw = c_id[k];
+            cj = idw & 0x7FFFFF;
+            sxy = __float_as_int(loj.w);
+            const bool samecol = (sxy & 16) != 0;
+            if (!(samecol && !(sz > 0 || (sz == 0 && cj >= ci)))) {
+              const float ex = fmaxf(0.f, fmaxf(loj.x - hii.x, loi.x - hij.x));
+              const float ey = fmaxf(0.f, fmaxf(loj.y - hii.y, loi.y - hij.y));
+              const float ez = fmaxf(0.f, fmaxf(loj.z + szL - hii.z, loi.z - (hij.z + szL)));
+              hit = ex * ex + ey * ey + ez * ez < rlist2;
+            }
           }
           const unsigned long long b = wave_ballot(hit);
           if (b == 0) continue;
           const int pos = count + __popcll(b & ((1ull << lane) - 1ull));
           if (hit && pos < cap) {
-            const int sh = (sx + 1) * 9 + (sy + 1) * 3 + (sz + 1);
-            const int nvj = S.cl_nvalid[cj];
-            const double ojx = S.cl_origin[cj] + (double)sx * S.L[0];
-            const double ojy = S.cl_origin[S.cl_cap + cj] + (double)sy * S.L[1];
-            const double ojz = S.cl_origin[2 * S.cl_cap + cj] + (double)sz * S.L[2];
-            uint4 e;
-            e.x = (unsigned)cj | ((unsigned)sh << 23) | ((unsigned)(nvj - 1) << 28);
-            e.y = __float_as_uint((float)(ojx - oix));
-            e.z = __float_as_uint((float)(ojy - oiy));
-            e.w = __float_as_uint((float)(ojz - oiz));
-            out[pos] = e;
+            // the image offset is filled in by the second pass below
+            const int sh = (sxy & 3) * 9 + ((sxy >> 2) & 3) * 3 + (sz + 1);
+            out[pos].x = (unsigned)cj | ((unsigned)sh << 23) | ((unsigned)idw & 0x70000000u);
           }
           count += __popcll(b);
         }
       }
+      if (lane == 0) i_count[il] = count;
     }
+    __syncthreads();
+    if (next_q >= nwin_all) break;
+    __syncthreads();
   }
-  if (count > cap) {
-    if (lane == 0) atomicOr(S.err, (unsigned)kErrListCap);
-    count = cap;
-  }
-  // exclusion masks for entries whose clusters share a molecule
-  const int ii = lane >> 3, jj = lane & 7;
-  const int si = ci * kCluster + ii;
-  const int mi = S.smol[si];
-  unsigned long long *mask = S.list_mask + 2 * (size_t)ci * cap;
-  for (int e = 0; e < count; ++e) {
-    const int cj = (int)(out[e].x & 0x7FFFFFu);
-    const int sj = cj * kCluster + jj;
-    const int mj = S.smol[sj];
-    const bool same = mi >= 0 && mi == mj;
-    if (wave_ballot(same) == 0) continue;
-    int cls = 0;
-    if (same) {
-      const int a = S.orig[si], b = S.orig[sj];
-      if (a != b) cls = excl_lookup(S, a, b);
+  // per i cluster: capacity, exclusion masks, length
+  for (int il = wv; il < nci; il += kListWaves) {
+    const int ci = ci0 + il;
+    int count = i_count[il];
+    if (count > cap) {
+      if (lane == 0) atomicOr(S.err, (unsigned)kErrListCap);
+      count = cap;
     }
-    const unsigned long long mf = wave_ballot(cls == PM_EXCL_FULL);
-    const unsigned long long mm = wave_ballot(cls == PM_EXCL_MODIFIED);
-    if ((mf | mm) != 0 && lane == 0) {
-      mask[2 * e] = mf;
-      mask[2 * e + 1] = mm;
-      out[e].x |= 0x80000000u;
+    if (lane == 0) atomicAdd(&S.ctl[kCtlEntries], count);
+    uint4 *out = S.list + (size_t)ci * cap;
+    const double oix = S.cl_origin[ci], oiy = S.cl_origin[S.cl_cap + ci], oiz = S.cl_origin[2 * S.cl_cap + ci];
+    const int ii = lane & 7, jj = lane >> 3;  // the pair kernel's lane layout
+    const int si = ci * kCluster + ii;
+    const int mi = S.smol[si];
+    const int mine = S.smol[ci * kCluster + (lane & 7)];  // lane k < 8 holds i atom k's molecule
+    int mol_i[kCluster];
+    for (int k = 0; k < kCluster; ++k) mol_i[k] = __shfl(mine, k);
+    unsigned long long *mask = S.list_mask + 2 * (size_t)ci * cap;
+    // 64 entries per pass (one per lane): the FP32 offset of the j cluster's
+    // origin (with its image) from the i cluster's, and a screen for a j atom
+    // whose molecule is one of the i cluster's
+    for (int e0 = 0; e0 < count; e0 += kWave) {
+      const int e = e0 + lane;
+      bool share = false;
+      if (e < count) {
+        const unsigned x = out[e].x;
+        const int cj = (int)(x & 0x7FFFFFu), sh = (int)((x >> 23) & 31u);
+        const int sx = sh / 9 - 1, sy = (sh / 3) % 3 - 1, sz = sh % 3 - 1;
+        const double ojx = S.cl_origin[cj] + (double)sx * S.L[0];
+        const double ojy = S.cl_origin[S.cl_cap + cj] + (double)sy * S.L[1];
+        const double ojz = S.cl_origin[2 * S.cl_cap + cj] + (double)sz * S.L[2];
+        const int4 m0 = *reinterpret_cast<const int4 *>(S.smol + cj * kCluster);
+        const int4 m1 = *reinterpret_cast<const int4 *>(S.smol + cj * kCluster + 4);
+        const int mj[kCluster] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
+        for (int a = 0; a < kCluster; ++a)
+          for (int b = 0; b < kCluster; ++b) share = share || (mj[a] >= 0 && mj[a] == mol_i[b]);
+        out[e].y = __float_as_uint((float)(ojx - oix));
+        out[e].z = __float_as_uint((float)(ojy - oiy));
+        out[e].w = __float_as_uint((float)(ojz - oiz));
+      }
+      // FULL/MODIFIED masks (bit = lane = jj * 8 + ii) for the entries that share a molecule
+      unsigned long long flagged = wave_ballot(share);
+      while (flagged) {
+        const int g = __ffsll((long long)flagged) - 1;
+        flagged &= flagged - 1;
+        const int ee = e0 + g;
+        const int cj = (int)(out[ee].x & 0x7FFFFFu);
+        const int sj = cj * kCluster + jj;
+        const int mj = S.smol[sj];
+        const bool same = mi >= 0 && mi == mj;
+        int cls = 0;
+        if (same) {
+          const int a = S.orig[si], b = S.orig[sj];
+          if (a != b) cls = excl_lookup(S, a, b);
+        }
+        const unsigned long long mf = wave_ballot(cls == PM_EXCL_FULL);
+        const unsigned long long mm = wave_ballot(cls == PM_EXCL_MODIFIED);
+        if ((mf | mm) != 0 && lane == 0) {
+          mask[2 * ee] = mf;
+          mask[2 * ee + 1] = mm;
+          out[ee].x |= 0x80000000u;
+        }
+      }
     }
+    if (lane == 0) S.list_len[ci] = count;
   }
-  if (lane == 0) S.list_len[ci] = count;
+  // clusters past the column's count keep an empty list
 }
 
 // ------------------------------------------------------------ pair force
@@ -578,7 +676,9 @@ __global__ __launch_bounds__(256) void k_pairlist(SysDev *sys, const int *clp, i
 // compile-time switch kinds for the hot path; kSwGeneric dispatches at run time
 constexpr int kSwGeneric = -1;
 
-template <int SW>
+// OUTER: the switch's cutoff may be shorter than the pair's inclusion cutoff
+// (OneAtomPairTwo with two different cutoffs), so S = 0 beyond it
+template <int SW, bool OUTER = true>
 __device__ __forceinline__ void apply_switch_t(const SwitchDev &sw, float r2, float &E, float &F) {
   if (SW == PM_SWITCH_C2) {  // C2SwitchingFunction.h:42-53
     const float c2 = sw.cut2 - r2;
@@ -586,7 +686,7 @@ __device__ __forceinline__ void apply_switch_t(const SwitchDev &sw, float r2, fl
     float S = sw.s1 * (c2 * c4);
     float dS = sw.s3 * (c2 * c2 - c4);
     const bool inner = r2 < sw.on2;
-    const bool outer = r2 > sw.cut2;  // a shorter switch inside OneAtomPairTwo's larger cutoff
+    const bool outer = OUTER && r2 > sw.cut2;
     S = inner ? 1.0f : (outer ? 0.0f : S);
     dS = (inner || outer) ? 0.0f : dS;
     F = F * S - E * dS;
@@ -607,6 +707,28 @@ __device__ __forceinline__ float dpp_sum8(float v) {
   return v;
 }
 
+// Sums of (x, y, z) over each group of 8 consecutive lanes, left in lane 0
+// (x), lane 1 (z) and lane 2 (y) of the group: halving steps over lane pairs
+// (quad swap), quad halves, then the group's two quads.  Fixed order, so the
+// result bits do not depend on anything but the inputs.
+__device__ __forceinline__ float dpp_sum8_scatter(float x, float y, float z, int g) {
+  const bool odd = g & 1;
+  // step 1 (lane ^ 1): even lanes keep x and y, odd lanes keep z
+  const float send1 = odd ? x : z;
+  const float r1 = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(send1), 0xB1, 0xF, 0xF, false));
+  const float r2 = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(y), 0xB1, 0xF, 0xF, false));
+  const float a = (odd ? z : x) + r1;  // even: x pair sum, odd: z pair sum
+  const float b = y + r2;              // even: y pair sum
+  // step 2 (lane ^ 2): lane 0 keeps x, lane 2 keeps y, lane 1 keeps z
+  const bool q0 = (g & 3) == 0;
+  const float send2 = q0 ? b : a;
+  const float r3 = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(send2), 0x4E, 0xF, 0xF, false));
+  const float c = ((g & 3) == 2 ? b : a) + r3;
+  // step 3: lanes 0..2 of the group add lanes 4..6 (row shift left by 4)
+  const float r4 = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(c), 0x104, 0xF, 0xF, false));
+  return c + r4;
+}
+
 constexpr int kLJLds = 256;  // LJ table entries kept in LDS per wave (T <= 16)
 
 // One wave per i-cluster.  Lane = (jj, ii) = (lane >> 3, lane & 7): i atom ii
@@ -617,7 +739,9 @@ constexpr int kLJLds = 256;  // LJ table entries kept in LDS per wave (T <= 16)
 // accumulate in registers; j-forces are summed over the 8 i lanes with DPP and
 // land, like the i-forces, in 64-bit fixed point, so the total is independent
 // of the order in which waves finish.
-template <int TA, int TB, int SWA, int SWB, int KIND>
+// SAME: both terms share one inclusion cutoff (one test per pair, and no
+// "outer" branch in the switch)
+template <int TA, int TB, int SWA, int SWB, int KIND, bool SAME>
 __global__ __launch_bounds__(512) void k_pair(const SysDev *__restrict__ sys, const int *__restrict__ clp, int nsys,
                                               ForceDev P) {
   constexpr bool FORCES = KIND != kPairEnergies;
@@ -641,8 +765,14 @@ __global__ __launch_bounds__(512) void k_pair(const SysDev *__restrict__ sys, co
   PM_GLOBAL long long *gfy = gp(S->fy);
   PM_GLOBAL long long *gfz = gp(S->fz);
   const int ii = lane & 7, jj = lane >> 3;
+  // j forces: lanes ii = 0, 1, 2 of each j group hold the x, z, y sums
+  PM_GLOBAL long long *gfc = gfx + (size_t)(ii == 1 ? 2 : (ii == 2 ? 1 : 0)) * S->f_stride;
   const int si = ci * kCluster + ii;
   const bool vi = ii < *cp(S->cl_nvalid + ci);
+  // wave-uniform lane masks (lane = jj * 8 + ii): real i atoms, and the i < j
+  // triangle of the cluster's pairs with itself
+  const unsigned long long vim = wave_ballot(vi);
+  const unsigned long long trim = wave_ballot(ii < jj);
   const float4 xi = xq[si];
   const bool peratom = S->lj_mode == PM_LJ_PERATOM;
   const int T = S->n_types;
@@ -688,26 +818,37 @@ __global__ __launch_bounds__(512) void k_pair(const SysDev *__restrict__ sys, co
     const float dy = (xj.y - xi.y) + __uint_as_float(ent.z);
     const float dz = (xj.z - xi.z) + __uint_as_float(ent.w);
     const float r2 = dx * dx + dy * dy + dz * dz;
-    bool act = vi && (jj < nvj);
-    if (cj == ci && sh == kShiftCenter) act = act && (ii < jj);
+    // pairs to evaluate, as a wave-uniform lane mask: real j atoms (lanes
+    // below 8 * nvj), real i atoms, each self pair once, FULL exclusions out
+    unsigned long long am = vim & (nvj == 8 ? ~0ull : ((1ull << (8 * nvj)) - 1ull));
+    if (cj == ci && sh == kShiftCenter) am &= trim;
     bool mod = false;
     if (w0 & 0x80000000u) {
+      // masks are indexed by lane = jj * 8 + ii
       const unsigned long long mf = M[2 * e], mm = M[2 * e + 1];
-      // masks are indexed by (i slot, j slot) as bit ii * 8 + jj
-      act = act && !((mf >> (ii * 8 + jj)) & 1ull);
-      mod = (mm >> (ii * 8 + jj)) & 1ull;
+      am &= ~mf;
+      mod = (mm >> lane) & 1ull;
     }
-    bool inA = (TA != PM_TERM_NONE) && r2 <= P.cutA2;
-    bool inB = (TB != PM_TERM_NONE) && r2 <= P.cutB2;
-    const bool nearA = (TA != PM_TERM_NONE) && fabsf(r2 - P.cutA2) <= P.guard;
-    const bool nearB = (TB != PM_TERM_NONE) && fabsf(r2 - P.cutB2) <= P.guard;
-    if (__builtin_expect(act && (nearA || nearB), 0)) {
-      const double r2d = min_image_r2(*S, orig[si], orig[sj]);
-      inA = (TA != PM_TERM_NONE) && r2d <= P.cutA2d;
-      inB = (TB != PM_TERM_NONE) && r2d <= P.cutB2d;
+    const bool act = (am >> lane) & 1ull;
+    bool inA, inB;
+    if (SAME) {
+      bool in = r2 <= P.cutA2;
+      if (__builtin_expect(act && fabsf(r2 - P.cutA2) <= P.guard, 0))
+        in = min_image_r2(*S, orig[si], orig[sj]) <= P.cutA2d;
+      inA = inB = in && act;
+    } else {
+      inA = (TA != PM_TERM_NONE) && r2 <= P.cutA2;
+      inB = (TB != PM_TERM_NONE) && r2 <= P.cutB2;
+      const bool nearA = (TA != PM_TERM_NONE) && fabsf(r2 - P.cutA2) <= P.guard;
+      const bool nearB = (TB != PM_TERM_NONE) && fabsf(r2 - P.cutB2) <= P.guard;
+      if (__builtin_expect(act && (nearA || nearB), 0)) {
+        const double r2d = min_image_r2(*S, orig[si], orig[sj]);
+        inA = (TA != PM_TERM_NONE) && r2d <= P.cutA2d;
+        inB = (TB != PM_TERM_NONE) && r2d <= P.cutB2d;
+      }
+      inA = inA && act;
+      inB = inB && act;
     }
-    inA = inA && act;
-    inB = inB && act;
     if (wave_ballot(inA || inB) == 0) continue;
 
     const float rinv = __builtin_amdgcn_rsqf(r2);
@@ -736,14 +877,14 @@ __global__ __launch_bounds__(512) void k_pair(const SysDev *__restrict__ sys, co
       const float r12A = A * r6 * r6;
       float E = r12A - r6B;
       float F = (12.0f * r12A - 6.0f * r6B) * r2inv;
-      apply_switch_t<SWA>(P.sw_a, r2, E, F);
+      apply_switch_t<SWA, !SAME>(P.sw_a, r2, E, F);
       fs += inA ? F : 0.f;
       ea = inA ? E : 0.f;
     } else if (TA == PM_TERM_COULOMB) {
       float E = xi.w * xj.w * rinv;
       if (mod) E *= csf;
       float F = E * r2inv;
-      apply_switch_t<SWA>(P.sw_a, r2, E, F);
+      apply_switch_t<SWA, !SAME>(P.sw_a, r2, E, F);
       fs += inA ? F : 0.f;
       ea = inA ? E : 0.f;
     }
@@ -752,7 +893,7 @@ __global__ __launch_bounds__(512) void k_pair(const SysDev *__restrict__ sys, co
       float E = xi.w * xj.w * rinv;
       if (mod) E *= csf;
       float F = E * r2inv;
-      apply_switch_t<SWB>(P.sw_b, r2, E, F);
+      apply_switch_t<SWB, !SAME>(P.sw_b, r2, E, F);
       fs += inB ? F : 0.f;
       eb = inB ? E : 0.f;
     } else if (TB == PM_TERM_EWALD_REAL) {
@@ -795,12 +936,8 @@ __global__ __launch_bounds__(512) void k_pair(const SysDev *__restrict__ sys, co
     }
     // Newton's third law: j force = sum over the 8 i lanes of this j atom
     if (FORCES) {
-      const float gx = dpp_sum8(fx), gy = dpp_sum8(fy), gz = dpp_sum8(fz);
-      if (ii == 0 && jj < nvj) {
-        atomic_add_ll(&gfx[sj], to_fixed((double)gx, kFScale));
-        atomic_add_ll(&gfy[sj], to_fixed((double)gy, kFScale));
-        atomic_add_ll(&gfz[sj], to_fixed((double)gz, kFScale));
-      }
+      const float g = dpp_sum8_scatter(fx, fy, fz, ii);
+      if (ii < 3 && jj < nvj) atomic_add_ll(&gfc[sj], to_fixed((double)g, kFScale));
     }
   }
   // i force = sum over the 8 j lanes that share ii
@@ -892,81 +1029,90 @@ __global__ __launch_bounds__(256) void k_local(SysDev *sys, const int *ap, int n
     const SysDev &S = sys[s];
     const int a = g - ap[s], n = S.n_atoms;
     const double *X = S.x;
-    for (int it = S.loc_start[a]; it < S.loc_start[a + 1]; ++it) {
+    const int *ls = S.loc_start + 3 * (size_t)a;
+    // one kind at a time, so lanes of a wave run the same term code together
+    for (int it = ls[kLocBond]; it < ls[kLocBond + 1]; ++it) {
       const int item = S.loc_item[it];
-      const int kind = (item >> 2) & 3, role = item & 3, k = item >> 4;
-      if (kind == kLocBond) {
-        const int a1 = S.bonds[2 * k], a2 = S.bonds[2 * k + 1];
-        double r12[3] = {X[a1] - X[a2], X[n + a1] - X[n + a2], X[2 * n + a1] - X[2 * n + a2]};
-        min_image_vec(S, r12);
-        const double r = sqrt(r12[0] * r12[0] + r12[1] * r12[1] + r12[2] * r12[2]);
-        const double dp = 2.0 * S.bond_k[k] * (r - S.bond_r0[k]);
-        const double c = -dp / r;
-        const double sg = role == 0 ? 1.0 : -1.0;
-        for (int d = 0; d < 3; ++d) f[d] += sg * (r12[d] * c);
-        if (role == 0) {
-          eb += S.bond_k[k] * (r - S.bond_r0[k]) * (r - S.bond_r0[k]);
-          v[0] += r12[0] * c * r12[0];
-          v[1] += r12[0] * c * r12[1];
-          v[2] += r12[0] * c * r12[2];
-          v[3] += r12[1] * c * r12[1];
-          v[4] += r12[1] * c * r12[2];
-          v[5] += r12[2] * c * r12[2];
-        }
-      } else if (kind == kLocAngle) {
-        const int a1 = S.angles[3 * k], a2 = S.angles[3 * k + 1], a3 = S.angles[3 * k + 2];
-        double r12[3] = {X[a1] - X[a2], X[n + a1] - X[n + a2], X[2 * n + a1] - X[2 * n + a2]};
-        double r32[3] = {X[a3] - X[a2], X[n + a3] - X[n + a2], X[2 * n + a3] - X[2 * n + a2]};
-        min_image_vec(S, r12);
-        min_image_vec(S, r32);
-        const double d12 = sqrt(r12[0] * r12[0] + r12[1] * r12[1] + r12[2] * r12[2]);
-        const double d32 = sqrt(r32[0] * r32[0] + r32[1] * r32[1] + r32[2] * r32[2]);
-        const double cr0 = r12[1] * r32[2] - r12[2] * r32[1], cr1 = r12[2] * r32[0] - r12[0] * r32[2],
-                     cr2 = r12[0] * r32[1] - r12[1] * r32[0];
-        const double theta =
-            atan2(sqrt(cr0 * cr0 + cr1 * cr1 + cr2 * cr2), r12[0] * r32[0] + r12[1] * r32[1] + r12[2] * r32[2]);
-        const double st = sin(theta), ct = cos(theta);
-        const double dp = 2.0 * S.angle_k[k] * (theta - S.angle_t0[k]);
-        double f1[3], f3[3];
-        for (int d = 0; d < 3; ++d) {
-          const double u12 = r12[d] / d12, u32 = r32[d] / d32;
-          f1[d] = -((u12 * ct - u32) / (st * d12)) * dp;
-          f3[d] = -((u32 * ct - u12) / (st * d32)) * dp;
-        }
-        for (int d = 0; d < 3; ++d) f[d] += role == 0 ? f1[d] : (role == 2 ? f3[d] : -f1[d] - f3[d]);
-        if (role == 0) {
-          ea += S.angle_k[k] * (theta - S.angle_t0[k]) * (theta - S.angle_t0[k]);
-          v[0] += f1[0] * r12[0] + f3[0] * r32[0];
-          v[1] += f1[0] * r12[1] + f3[0] * r32[1];
-          v[2] += f1[0] * r12[2] + f3[0] * r32[2];
-          v[3] += f1[1] * r12[1] + f3[1] * r32[1];
-          v[4] += f1[1] * r12[2] + f3[1] * r32[2];
-          v[5] += f1[2] * r12[2] + f3[2] * r32[2];
-        }
-      } else {  // kLocCorr
-        const int a1 = S.corr[3 * k], a2 = S.corr[3 * k + 1], cls = S.corr[3 * k + 2];
-        double d[3] = {X[a2] - X[a1], X[n + a2] - X[n + a1], X[2 * n + a2] - X[2 * n + a1]};
-        min_image_vec(S, d);
-        const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-        double qq = S.charge[a1] * S.charge[a2];
-        if (cls == PM_EXCL_MODIFIED) qq *= 1.0 - (double)S.csf;
-        const double r = sqrt(r2), rr = 1.0 / r;
-        const double e = erf(P.dalpha * r) * rr;  // C-library erf, as the reference on Linux
-        const double fs = qq * (P.dtwo_alpha_pi * exp(-(P.dalpha * P.dalpha) * r2) - e) * rr * rr;
-        const double sg = role == 0 ? -1.0 : 1.0;
-        for (int q = 0; q < 3; ++q) f[q] += sg * (d[q] * fs);
-        if (role == 0) {
-          ec += -qq * e;
-          v[0] += d[0] * fs * d[0];
-          v[1] += d[0] * fs * d[1];
-          v[2] += d[0] * fs * d[2];
-          v[3] += d[1] * fs * d[1];
-          v[4] += d[1] * fs * d[2];
-          v[5] += d[2] * fs * d[2];
-        }
+      const int role = item & 3, k = item >> 2;
+      const int a1 = S.bonds[2 * k], a2 = S.bonds[2 * k + 1];
+      double r12[3] = {X[a1] - X[a2], X[n + a1] - X[n + a2], X[2 * n + a1] - X[2 * n + a2]};
+      min_image_vec(S, r12);
+      const double r = sqrt(r12[0] * r12[0] + r12[1] * r12[1] + r12[2] * r12[2]);
+      const double dp = 2.0 * S.bond_k[k] * (r - S.bond_r0[k]);
+      const double c = -dp / r;
+      const double sg = role == 0 ? 1.0 : -1.0;
+      for (int d = 0; d < 3; ++d) f[d] += sg * (r12[d] * c);
+      if (role == 0) {
+        eb += S.bond_k[k] * (r - S.bond_r0[k]) * (r - S.bond_r0[k]);
+        v[0] += r12[0] * c * r12[0];
+        v[1] += r12[0] * c * r12[1];
+        v[2] += r12[0] * c * r12[2];
+        v[3] += r12[1] * c * r12[1];
+        v[4] += r12[1] * c * r12[2];
+        v[5] += r12[2] * c * r12[2];
       }
     }
-    if ((mode & kLocalForces) && S.loc_start[a + 1] > S.loc_start[a]) {
+    for (int it = ls[kLocAngle]; it < ls[kLocAngle + 1]; ++it) {
+      const int item = S.loc_item[it];
+      const int role = item & 3, k = item >> 2;
+      const int a1 = S.angles[3 * k], a2 = S.angles[3 * k + 1], a3 = S.angles[3 * k + 2];
+      double r12[3] = {X[a1] - X[a2], X[n + a1] - X[n + a2], X[2 * n + a1] - X[2 * n + a2]};
+      double r32[3] = {X[a3] - X[a2], X[n + a3] - X[n + a2], X[2 * n + a3] - X[2 * n + a2]};
+      min_image_vec(S, r12);
+      min_image_vec(S, r32);
+      const double d12 = sqrt(r12[0] * r12[0] + r12[1] * r12[1] + r12[2] * r12[2]);
+      const double d32 = sqrt(r32[0] * r32[0] + r32[1] * r32[1] + r32[2] * r32[2]);
+      const double cr0 = r12[1] * r32[2] - r12[2] * r32[1], cr1 = r12[2] * r32[0] - r12[0] * r32[2],
+                   cr2 = r12[0] * r32[1] - r12[1] * r32[0];
+      const double cn = sqrt(cr0 * cr0 + cr1 * cr1 + cr2 * cr2);
+      const double dt = r12[0] * r32[0] + r12[1] * r32[1] + r12[2] * r32[2];
+      const double theta = atan2(cn, dt);
+      // sin and cos of theta = atan2(|r12 x r32|, r12 . r32), from the same vectors
+      const double inv = 1.0 / (d12 * d32);
+      const double st = cn * inv, ct = dt * inv;
+      const double dp = 2.0 * S.angle_k[k] * (theta - S.angle_t0[k]);
+      double f1[3], f3[3];
+      for (int d = 0; d < 3; ++d) {
+        const double u12 = r12[d] / d12, u32 = r32[d] / d32;
+        f1[d] = -((u12 * ct - u32) / (st * d12)) * dp;
+        f3[d] = -((u32 * ct - u12) / (st * d32)) * dp;
+      }
+      for (int d = 0; d < 3; ++d) f[d] += role == 0 ? f1[d] : (role == 2 ? f3[d] : -f1[d] - f3[d]);
+      if (role == 0) {
+        ea += S.angle_k[k] * (theta - S.angle_t0[k]) * (theta - S.angle_t0[k]);
+        v[0] += f1[0] * r12[0] + f3[0] * r32[0];
+        v[1] += f1[0] * r12[1] + f3[0] * r32[1];
+        v[2] += f1[0] * r12[2] + f3[0] * r32[2];
+        v[3] += f1[1] * r12[1] + f3[1] * r32[1];
+        v[4] += f1[1] * r12[2] + f3[1] * r32[2];
+        v[5] += f1[2] * r12[2] + f3[2] * r32[2];
+      }
+    }
+    for (int it = ls[kLocCorr]; it < ls[kLocCorr + 1]; ++it) {
+      const int item = S.loc_item[it];
+      const int role = item & 3, k = item >> 2;
+      const int a1 = S.corr[3 * k], a2 = S.corr[3 * k + 1], cls = S.corr[3 * k + 2];
+      double d[3] = {X[a2] - X[a1], X[n + a2] - X[n + a1], X[2 * n + a2] - X[2 * n + a1]};
+      min_image_vec(S, d);
+      const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      double qq = S.charge[a1] * S.charge[a2];
+      if (cls == PM_EXCL_MODIFIED) qq *= 1.0 - (double)S.csf;
+      const double r = sqrt(r2), rr = 1.0 / r;
+      const double e = erf(P.dalpha * r) * rr;  // C-library erf, as the reference on Linux
+      const double fs = qq * (P.dtwo_alpha_pi * exp(-(P.dalpha * P.dalpha) * r2) - e) * rr * rr;
+      const double sg = role == 0 ? -1.0 : 1.0;
+      for (int q = 0; q < 3; ++q) f[q] += sg * (d[q] * fs);
+      if (role == 0) {
+        ec += -qq * e;
+        v[0] += d[0] * fs * d[0];
+        v[1] += d[0] * fs * d[1];
+        v[2] += d[0] * fs * d[2];
+        v[3] += d[1] * fs * d[1];
+        v[4] += d[1] * fs * d[2];
+        v[5] += d[2] * fs * d[2];
+      }
+    }
+    if ((mode & kLocalForces) && ls[3] > ls[0]) {
       const int sl = S.slot[a];
       S.fx[sl] += to_fixed(f[0], kFScale);
       S.fy[sl] += to_fixed(f[1], kFScale);
@@ -1109,6 +1255,7 @@ __global__ void k_step_begin(SysDev *sys, int nsys, int mode, float lim2, int km
     if (rb) {
       ctl[kCtlSince] = 0;
       ctl[kCtlCount] += 1;
+      ctl[kCtlEntries] = 0;
     } else {
       ctl[kCtlSince] += 1;
     }
@@ -1182,10 +1329,10 @@ void launch_rebuild(const Launch &L, float rlist2) {
   hipLaunchKernelGGL(k_col_fill, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys);
   hipLaunchKernelGGL(k_col_sort, dim3(L.n_cols_total), dim3(1024), 0, L.stream, L.sys, L.col_prefix, L.nsys);
   hipLaunchKernelGGL(k_cluster_geom, dim3(nblk(L.n_cl_total, 128)), dim3(128), 0, L.stream, L.sys, L.cl_prefix, L.nsys);
-  hipLaunchKernelGGL(k_pairlist, dim3(nblk((long long)L.n_cl_total * kWave, 256)), dim3(256), 0, L.stream, L.sys, L.cl_prefix, L.nsys, rlist2);
+  hipLaunchKernelGGL(k_pairlist, dim3(L.n_cols_total), dim3(kListThreads), 0, L.stream, L.sys, L.col_prefix, L.nsys, rlist2);
 }
 
-template <int TA, int TB, int SWA, int SWB>
+template <int TA, int TB, int SWA, int SWB, bool SAME = false>
 static void launch_pair_t(const Launch &L, const ForceDev &P, int kind) {
   // one wave per i-cluster; P.pair_waves waves per workgroup, each with its
   // own LDS copy of the LJ table
@@ -1195,16 +1342,16 @@ static void launch_pair_t(const Launch &L, const ForceDev &P, int kind) {
   const size_t lds = (size_t)waves * kLJLds * sizeof(float4);
   switch (kind) {
   case kPairForces:
-    hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairForces>), g, b, lds, L.stream, L.sys, L.cl_prefix, L.nsys, P);
+    hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairForces, SAME>), g, b, lds, L.stream, L.sys, L.cl_prefix, L.nsys, P);
     break;
   case kPairForcesEnergies:
-    hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairForcesEnergies>), g, b, lds, L.stream, L.sys, L.cl_prefix, L.nsys, P);
+    hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairForcesEnergies, SAME>), g, b, lds, L.stream, L.sys, L.cl_prefix, L.nsys, P);
     break;
   case kPairEnergies:
-    hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairEnergies>), g, b, lds, L.stream, L.sys, L.cl_prefix, L.nsys, P);
+    hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairEnergies, SAME>), g, b, lds, L.stream, L.sys, L.cl_prefix, L.nsys, P);
     break;
   default:
-    hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairCensus>), g, b, lds, L.stream, L.sys, L.cl_prefix, L.nsys, P);
+    hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairCensus, SAME>), g, b, lds, L.stream, L.sys, L.cl_prefix, L.nsys, P);
     break;
   }
 }
@@ -1216,7 +1363,9 @@ static void launch_pair_sw(const Launch &L, const ForceDev &P, int kind) {
   const bool b2 = TB != PM_TERM_COULOMB || P.sw_b.kind == PM_SWITCH_C2;
   constexpr int SB = TB == PM_TERM_COULOMB ? PM_SWITCH_C2 : PM_SWITCH_CUTOFF;
   constexpr int SBG = TB == PM_TERM_COULOMB ? kSwGeneric : PM_SWITCH_CUTOFF;
-  if (a2 && b2) launch_pair_t<TA, TB, PM_SWITCH_C2, SB>(L, P, kind);
+  const bool same = TA != PM_TERM_NONE && TB != PM_TERM_NONE && P.cutA2 == P.cutB2 && P.cutA2d == P.cutB2d;
+  if (a2 && b2 && same) launch_pair_t<TA, TB, PM_SWITCH_C2, SB, true>(L, P, kind);
+  else if (a2 && b2) launch_pair_t<TA, TB, PM_SWITCH_C2, SB>(L, P, kind);
   else launch_pair_t<TA, TB, kSwGeneric, SBG>(L, P, kind);
 }
 
