@@ -50,8 +50,12 @@ __device__ __forceinline__ int find_sys_gb(const int *__restrict__ prefix, int n
   return lo;
 }
 
-// contribution of atom j to the burial term of atom i (GBBornRadii.cpp:59-91)
-__device__ __forceinline__ float burial_term(float dist, float ori, float orj, float sj) {
+// reciprocal by the hardware instruction (1 ulp)
+__device__ __forceinline__ float rcp(float v) { return __builtin_amdgcn_rcpf(v); }
+
+// contribution of atom j to the burial term of atom i (GBBornRadii.cpp:59-91);
+// iori = 1/ori, idist = 1/dist
+__device__ __forceinline__ float burial_term(float dist, float idist, float ori, float iori, float orj, float sj) {
   const float sorj = sj * orj;
   float L, U;
   if (ori >= dist + sorj) {
@@ -62,16 +66,15 @@ __device__ __forceinline__ float burial_term(float dist, float ori, float orj, f
     L = ori > t ? ori : t;
     U = dist + sorj;
   }
-  const float C = ori < sorj - dist ? 2.0f * (1.0f / ori - 1.0f / L) : 0.0f;
-  const float iL = 1.0f / L, iU = 1.0f / U, iL2 = iL * iL, iU2 = iU * iU;
-  const float idist = 1.0f / dist;
+  const float iL = rcp(L), iU = rcp(U), iL2 = iL * iL, iU2 = iU * iU;
+  const float C = ori < sorj - dist ? 2.0f * (iori - iL) : 0.0f;
   return (iL - iU) + (0.25f * dist) * (iU2 - iL2) + (0.5f * idist) * __logf(L * iU) +
          (0.25f * sorj * sorj * idist) * (iL2 - iU2) + C;
 }
 
 // d(burial_i)/d(r_ij) (GBForce.cpp:55-126); the term (U/(4 d L))(dL/U - L dU/U^2)
-// of the reference is written in its equal form (dL/L - dU/U)/(4 d)
-__device__ __forceinline__ float dburial(float dist, float ori, float orj, float sj) {
+// of the reference is written in its equal form (dL/L - dU/U)/(4 d); idist = 1/dist
+__device__ __forceinline__ float dburial(float dist, float idist, float ori, float orj, float sj) {
   const float sorj = sj * orj;
   float L, U;
   if (ori >= dist + sorj) {
@@ -82,11 +85,10 @@ __device__ __forceinline__ float dburial(float dist, float ori, float orj, float
     L = ori > t ? ori : t;
     U = dist + sorj;
   }
-  const float iL = 1.0f / L, iU = 1.0f / U, iL2 = iL * iL, iU2 = iU * iU;
+  const float iL = rcp(L), iU = rcp(U), iL2 = iL * iL, iU2 = iU * iU;
   const float dL = ori <= dist - sorj ? 1.0f : 0.0f;
   const float dU = ori < dist + sorj ? 1.0f : 0.0f;
   const float dC = ori <= sorj - dist ? 2.0f * iL2 * dL : 0.0f;
-  const float idist = 1.0f / dist;
   const float st = sorj * idist;
   const float s2 = sorj * sorj;
   return -0.5f * dL * iL2 + 0.5f * dU * iU2 + 0.125f * (iU2 - iL2) +
@@ -97,8 +99,8 @@ __device__ __forceinline__ float dburial(float dist, float ori, float orj, float
 
 struct GBShared {
   double x[kGBBlock], y[kGBBlock], z[kGBBlock];
-  float q[kGBBlock], ori[kGBBlock], S[kGBBlock], B[kGBBlock], dr[kGBBlock], part[kGBBlock], sg[kGBBlock],
-      ep[kGBBlock];
+  float q[kGBBlock], ori[kGBBlock], S[kGBBlock], B[kGBBlock], iB[kGBBlock], qB2[kGBBlock], dr[kGBBlock],
+      part[kGBBlock], sg[kGBBlock], ep[kGBBlock];
 };
 
 __device__ void load_chunk(GBShared &sh, const SysDev &S, const ForceDev &P, int j0, int pass) {
@@ -111,8 +113,11 @@ __device__ void load_chunk(GBShared &sh, const SysDev &S, const ForceDev &P, int
     sh.ori[t] = (float)(S.gb_rho[j] - P.gb_offset);
     sh.S[t] = (float)S.gb_S[j];
     if (pass >= 2) {
-      sh.q[t] = (float)S.charge[j];
-      sh.B[t] = (float)S.gb_born[j];
+      const float q = (float)S.charge[j], B = (float)S.gb_born[j], iB = rcp(B);
+      sh.q[t] = q;
+      sh.B[t] = B;
+      sh.iB[t] = iB;
+      sh.qB2[t] = 0.5f * q * q * iB * iB;  // q^2 / (2 B^2)
     }
     if (pass >= 3) {
       sh.dr[t] = (float)S.gb_drfac[j];
@@ -157,11 +162,13 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_born(SysDev *sys, const int *gp
     __syncthreads();
     const int cnt = min(kGBBlock, n - j0);
     if (!vi) continue;
+    const float iori = rcp(ori);
     for (int k = 0; k < cnt; ++k) {
       if (j0 + k == i) continue;
       const float dx = (float)(sh.x[k] - xi), dy = (float)(sh.y[k] - yi), dz = (float)(sh.z[k] - zi);
-      const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
-      burial += (double)burial_term(dist, ori, sh.ori[k], sh.S[k]);
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      const float idist = rsqrtf(r2);
+      burial += (double)burial_term(r2 * idist, idist, ori, iori, sh.ori[k], sh.S[k]);
     }
   }
   if (!vi) return;
@@ -195,7 +202,7 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_partial(SysDev *sys, const int 
     qi = (float)S.charge[i];
     Bi = (float)S.gb_born[i];
   }
-  const float i4Bi = 0.25f / Bi;
+  const float i4Bi = 0.25f * rcp(Bi), iBi = rcp(Bi);
   double acc = 0.0;
   for (int j0 = 0; j0 < n; j0 += kGBBlock) {
     __syncthreads();
@@ -209,9 +216,8 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_partial(SysDev *sys, const int 
       const float r2 = dx * dx + dy * dy + dz * dz;
       const float Bj = sh.B[k];
       const float BB = Bi * Bj;
-      const float ex = __expf(-r2 / (4.0f * BB));
-      const float f = sqrtf(r2 + BB * ex);
-      const float inv = 1.0f / f;
+      const float ex = __expf(-r2 * (0.25f * iBi * sh.iB[k]));  // exp(-r^2 / (4 Bi Bj))
+      const float inv = rsqrtf(r2 + BB * ex);                     // 1 / f_GB
       const float part = qi * sh.q[k] * inv * inv * 0.5f * inv * ex;
       acc += (double)(part * (Bj + r2 * i4Bi));
     }
@@ -258,7 +264,8 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_force(SysDev *sys, const int *g
     if (ep < ee) nextx = S.excl_partner[ep];
   }
   const float fac = (float)P.gb_fac;
-  const float i4Bi = 0.25f / Bi;
+  const float iBi = rcp(Bi), i4Bi = 0.25f * iBi;
+  const float qiB2 = 0.5f * qi * qi * iBi * iBi;  // q_i^2 / (2 B_i^2)
   const float csf = S.csf;
   double fx = 0.0, fy = 0.0, fz = 0.0, egb = 0.0, elj = 0.0, eco = 0.0;
   for (int j0 = 0; j0 < n; j0 += kGBBlock) {
@@ -280,19 +287,18 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_force(SysDev *sys, const int *g
       const float r2 = dx * dx + dy * dy + dz * dz;
       const float rinv = rsqrtf(r2);
       const float dist = r2 * rinv;
-      const float Bj = sh.B[k];
+      const float Bj = sh.B[k], iBj = sh.iB[k];
       const float BB = Bi * Bj;
-      const float ex = __expf(-r2 / (4.0f * BB));
-      const float fgb = sqrtf(r2 + BB * ex);
-      const float inv = 1.0f / fgb;
+      const float ex = __expf(-r2 * (i4Bi * iBj));  // exp(-r^2 / (4 Bi Bj))
+      const float inv = rsqrtf(r2 + BB * ex);       // 1 / f_GB
       const float qq = qi * sh.q[k];
-      const float dRi = dri * dburial(dist, ori, sh.ori[k], sh.S[k]);
-      const float dRj = sh.dr[k] * dburial(dist, sh.ori[k], ori, Si);
+      const float dRi = dri * dburial(dist, rinv, ori, sh.ori[k], sh.S[k]);
+      const float dRj = sh.dr[k] * dburial(dist, rinv, sh.ori[k], ori, Si);
       const float c0 = qq * inv * inv * 0.5f * inv;
-      const float ai = Bj + r2 * i4Bi, bj = Bi + r2 * (0.25f / Bj);
+      const float ai = Bj + r2 * i4Bi, bj = Bi + r2 * (0.25f * iBj);
       float force = -c0 * ((2.0f * dist - 0.5f * ex * dist) + ex * dRi * ai + ex * dRj * bj) * rinv;
-      force -= 0.5f * qi * qi * dRi / (Bi * Bi) * rinv;
-      force -= 0.5f * sh.q[k] * sh.q[k] * dRj / (Bj * Bj) * rinv;
+      force -= qiB2 * dRi * rinv;
+      force -= sh.qB2[k] * dRj * rinv;
       const float part = c0 * ex;
       force -= (Pi - part * ai) * (dRi * rinv);
       force -= (sh.part[k] - part * bj) * (dRj * rinv);
