@@ -57,6 +57,21 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def pair_kernel_traffic():
+    """HBM bytes per launch of the pair kernel at this workload: FETCH_SIZE and
+    WRITE_SIZE from two rocprofv3 --pmc runs, each scaled by the counter's
+    reading for a device copy of known size (profiles/pmc_k_pair_water96k.txt)."""
+    path = os.path.join(ROOT, "profiles", "pmc_k_pair_water96k.txt")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch"):
+                    return float(line.split()[1])
+    except OSError:
+        pass
+    return None
+
+
 def force_desc(skin, timing=False, rebuild=REBUILD, **shape):
     d = L.ForceDesc()
     d.term_a = L.TERM_LJ
@@ -168,7 +183,7 @@ def headline(args):
                    "pair_waves": args.pair_waves or 4},
         "ns_per_day": steps_s * DT_FS * 0.0864,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": pair_kernel_traffic(),
                      "kernel": "k_pair<LJ, EWALD_REAL>", "kernel_ms": pair_ms,
                      "flop_per_pair": FLOP_PER_PAIR},
     }
