@@ -134,7 +134,7 @@ def headline(args):
     log("building the 96,000-atom water box")
     system = S.water_box(N_MOL, BOX, seed=2024, min_contact=2.0)
     v0 = S.maxwell_velocities(system, 300.0, seed=2025)
-    shape = {"cell_scale": args.cell_scale, "pair_waves": args.pair_waves}
+    shape = {"cell_scale": args.cell_scale, "pair_waves": args.pair_waves, "pair_split": args.pair_split}
     with L.Context(force_desc(args.skin, timing=not args.no_timing, **shape), [system]) as ctx:
         ctx.md_set_state(system.x, v0)
         seg, t_now = 25, 300.0
@@ -180,7 +180,7 @@ def headline(args):
                    "list_entries": st["list_entries"],
                    "list_efficiency": pairs / max(1.0, 64.0 * st["list_entries"]),
                    "equilibration_steps": args.equil, "cell_scale": args.cell_scale or 1.0,
-                   "pair_waves": args.pair_waves or 4},
+                   "pair_waves": args.pair_waves or 4, "pair_split": args.pair_split or 12},
         "ns_per_day": steps_s * DT_FS * 0.0864,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": pair_kernel_traffic(),
@@ -348,6 +348,7 @@ def main():
     ap.add_argument("--cell-scale", type=float, default=1.25,
                     help="column width factor (1.25: the autotuner's choice for water at 0.1 atoms/A^3)")
     ap.add_argument("--pair-waves", type=int, default=0, help="pair-kernel waves per workgroup (0 = default 4)")
+    ap.add_argument("--pair-split", type=int, default=0, help="waves per super-cluster list (0 = default 12)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true",

@@ -139,6 +139,7 @@ typedef struct pm_force_desc {
   /* launch shapes (0 = default; an autotuner may choose them per workload) */
   double cell_scale;           /* x-y column width as a multiple of (8/density)^(1/3): 0.6 .. 2 */
   int32_t pair_waves;          /* waves per workgroup of the pair kernel: 1, 2, 4 (default) or 8 */
+  int32_t pair_split;          /* waves sharing one super-cluster's pair list: 1 .. 64 (default 12) */
 } pm_force_desc;
 
 typedef struct pm_ctx pm_ctx;

@@ -51,7 +51,8 @@ class ForceDesc(C.Structure):
                 ("ewald_correction", C.c_int32), ("gb_offset", C.c_double), ("gb_alpha", C.c_double),
                 ("gb_beta", C.c_double), ("gb_gamma", C.c_double), ("gb_solute_dielec", C.c_double),
                 ("gb_solvent_dielec", C.c_double), ("skin", C.c_double), ("rebuild_interval", C.c_int32),
-                ("flags", C.c_uint32), ("cell_scale", C.c_double), ("pair_waves", C.c_int32)]
+                ("flags", C.c_uint32), ("cell_scale", C.c_double), ("pair_waves", C.c_int32),
+                ("pair_split", C.c_int32)]
 
 
 class PMError(RuntimeError):

@@ -6,6 +6,8 @@ force model) over a grid of launch shapes and returns the fastest:
   skin        pair-list buffer (A): a longer list per rebuild vs more rebuilds
   cell_scale  x-y column width (multiple of (8/density)^(1/3)): the shape of
               the 8-atom clusters, i.e. the cell tiles the pair kernel walks
+  pair_split  waves sharing one super-cluster's pair list (work per wave
+              against waves in flight)
   pair_waves  waves per workgroup of the pair kernel (1, 2, 4, 8)
 
 Results are kept in a JSON cache keyed by a signature of the workload, so a
@@ -25,8 +27,8 @@ from . import _lib as L
 
 DEFAULT_GRID = {
     "skin": (1.0, 1.3),
-    "cell_scale": (1.0, 1.25, 1.5),
-    "pair_waves": (2, 4),
+    "cell_scale": (1.0, 1.25),
+    "pair_split": (8, 12, 16),
 }
 
 

@@ -124,14 +124,14 @@ struct SysPrep {
   std::vector<float> sig_half, eps_sqrt;
   std::vector<float4> lj;
   int n_corr = 0, n_bonds = 0, n_angles = 0, ncol = 1, ncx = 1, ncy = 1, rx = 0, ry = 0;
-  int cl_cap = 0, slot_cap = 0, list_cap = 0, n_gb_tiles = 0;
+  int cl_cap = 0, slot_cap = 0, sc_cap = 0, list_cap = 0, n_gb_tiles = 0;
   double ax = 0, ay = 0;
   double coul_const = 0, vir_const = 0;
 };
 
 struct SysHost {
   int id = -1;
-  int n = 0, ncol = 0, cl_cap = 0, slot_cap = 0, list_cap = 0, n_corr = 0, n_bonded = 0, n_local = 0;
+  int n = 0, ncol = 0, cl_cap = 0, slot_cap = 0, sc_cap = 0, list_cap = 0, n_corr = 0, n_bonded = 0, n_local = 0;
   int n_gb_tiles = 0;
   double coul_const = 0.0;  // Ewald point-self + charged-system energy (FP64, host)
   double vir_const = 0.0;   // charged-system virial diagonal
@@ -224,12 +224,15 @@ double build_switch(const pm_switch_desc &d, SwitchDev &s, bool ewald_hard, doub
 }
 
 
-int estimate_list_cap(double rho, double rlist, int cl_cap) {
-  // clusters whose bounding box lies within rlist of an i-cluster box, halved
-  double b = std::cbrt(pm::kCluster / std::max(rho, 1e-6));
-  double w = 2.0 * b, r = rlist;
-  double vol = w * w * w + 6.0 * w * w * r + 3.0 * M_PI * w * r * r + 4.0 / 3.0 * M_PI * r * r * r;
-  double half = 0.5 * vol * rho / pm::kCluster;
+// entries of one super-cluster's list: j clusters whose box comes within
+// rlist of the box of kScCl z-consecutive clusters of a column (ax x ay
+// cross-section, height kScCl * hz), halved for the half shell, with margin
+int estimate_list_cap(double rho, double rlist, double ax, double ay, int cl_cap) {
+  const double hz = pm::kCluster / std::max(rho * ax * ay, 1e-9);  // height of one cluster
+  const double a = 2.0 * ax, b = 2.0 * ay, c = (pm::kScCl + 1) * hz, r = rlist;
+  const double vol = a * b * c + 2.0 * r * (a * b + b * c + c * a) + M_PI * r * r * (a + b + c) +
+                     4.0 / 3.0 * M_PI * r * r * r;
+  const double half = 0.5 * vol * rho / pm::kCluster;
   int cap = (int)(1.4 * half) + 32;
   cap = (cap + 31) / 32 * 32;
   return std::min(cap, 27 * cl_cap / 2 + 32);
@@ -492,7 +495,8 @@ SysPrep prepare(const pm_ctx *c, const pm_system_desc &d) {
   p.ry = (int)std::ceil(rlist / p.ay);
   p.cl_cap = (n + pm::kCluster - 1) / pm::kCluster + p.ncol;
   p.slot_cap = p.cl_cap * pm::kCluster;
-  p.list_cap = estimate_list_cap(rho, rlist, p.cl_cap);
+  p.sc_cap = p.cl_cap / pm::kScCl + p.ncol + 1;
+  p.list_cap = estimate_list_cap(rho, rlist, p.ax, p.ay, p.cl_cap);
   return p;
 }
 
@@ -571,20 +575,27 @@ size_t carve(const pm_ctx *c, const SysPrep &p, const pm_system_desc &d, Carver 
   S.cl_hi = cv.take<float4>(p.cl_cap);
   S.cl_col = cv.take<int>(p.cl_cap);
   S.cl_nvalid = cv.take<int>(p.cl_cap);
-  S.list_len = cv.take<int>(p.cl_cap);
+  S.col_sc_start = cv.take<int>(p.ncol);
+  S.col_nsc = cv.take<int>(p.ncol);
+  S.sc_col = cv.take<int>(p.sc_cap);
+  S.n_sc = cv.take<int>(1);
+  S.list_len = cv.take<int>(p.sc_cap);
   return cv.off;
 }
 
-size_t lists_bytes(int cl_cap, int list_cap) {
+// exclusion-mask slots per super-cluster: grows with the list capacity
+int mask_cap_for(int list_cap) { return std::max(pm::kMaskSlots, list_cap / 8); }
+
+size_t lists_bytes(int sc_cap, int list_cap) {
   Carver cv;
-  cv.take<uint4>((size_t)cl_cap * list_cap);
-  cv.take<unsigned long long>(2 * (size_t)cl_cap * list_cap);
+  cv.take<uint4>(2 * (size_t)sc_cap * list_cap);
+  cv.take<unsigned long long>((size_t)sc_cap * mask_cap_for(list_cap) * 2 * pm::kScCl);
   return cv.off;
 }
 
 void alloc_lists(pm_ctx *c, SysHost &h) {
   if (h.cl_cap == 0) return;
-  const size_t bytes = lists_bytes(h.cl_cap, h.list_cap);
+  const size_t bytes = lists_bytes(h.sc_cap, h.list_cap);
   void *mem = dev_alloc(c, bytes);
   dev_free(c, h.lists, h.lists_bytes);
   h.lists = mem;
@@ -592,8 +603,9 @@ void alloc_lists(pm_ctx *c, SysHost &h) {
   Carver cv;
   cv.base = static_cast<char *>(mem);
   h.dev.list_cap = h.list_cap;
-  h.dev.list = cv.take<uint4>((size_t)h.cl_cap * h.list_cap);
-  h.dev.list_mask = cv.take<unsigned long long>(2 * (size_t)h.cl_cap * h.list_cap);
+  h.dev.mask_cap = mask_cap_for(h.list_cap);
+  h.dev.list = cv.take<uint4>(2 * (size_t)h.sc_cap * h.list_cap);
+  h.dev.list_mask = cv.take<unsigned long long>((size_t)h.sc_cap * h.dev.mask_cap * 2 * pm::kScCl);
 }
 
 uint64_t footprint(const pm_ctx *c, const pm_system_desc &d) {
@@ -602,7 +614,7 @@ uint64_t footprint(const pm_ctx *c, const pm_system_desc &d) {
   SysDev S{};
   int *snap = nullptr;
   size_t b = align_up(carve(c, p, d, cv, S, &snap));
-  if (!c->gb) b += align_up(lists_bytes(p.cl_cap, p.list_cap));
+  if (!c->gb) b += align_up(lists_bytes(p.sc_cap, p.list_cap));
   return b;
 }
 
@@ -659,6 +671,7 @@ int add_system(pm_ctx *c, const pm_system_desc &d) {
   S.rx = p.rx;
   S.ry = p.ry;
   S.cl_cap = h.cl_cap = p.cl_cap;
+  S.sc_cap = h.sc_cap = p.sc_cap;
   S.slot_cap = h.slot_cap = p.slot_cap;
   h.list_cap = p.list_cap;
 
@@ -744,16 +757,17 @@ LaunchSet &get_set(pm_ctx *c, const std::vector<int> &ids) {
   ls->ids = ids;
   const int ns = (int)ids.size();
   std::vector<SysDev> table(ns);
-  std::vector<std::vector<int>> pre(8, std::vector<int>(1, 0));
+  constexpr int kPre = 9;
+  std::vector<std::vector<int>> pre(kPre, std::vector<int>(1, 0));
   for (int k = 0; k < ns; ++k) {
     const SysHost &h = *c->sys[ids[k]];
     table[k] = h.dev;
-    const int cnt[8] = {h.n, h.cl_cap, h.ncol, h.slot_cap, h.n_corr, h.n_bonded, h.n_gb_tiles, h.n_local};
-    for (int q = 0; q < 8; ++q) pre[q].push_back(pre[q].back() + cnt[q]);
+    const int cnt[kPre] = {h.n, h.cl_cap, h.ncol, h.slot_cap, h.n_corr, h.n_bonded, h.n_gb_tiles, h.n_local, h.sc_cap};
+    for (int q = 0; q < kPre; ++q) pre[q].push_back(pre[q].back() + cnt[q]);
   }
-  // one allocation: [SysDev x ns][8 prefix arrays of ns + 1][kScalN int64 x ns]
+  // one allocation: [SysDev x ns][kPre prefix arrays of ns + 1][kScalN int64 x ns]
   const size_t o_pre = align_up((size_t)ns * sizeof(SysDev));
-  const size_t o_scal = align_up(o_pre + 8 * (size_t)(ns + 1) * sizeof(int));
+  const size_t o_scal = align_up(o_pre + kPre * (size_t)(ns + 1) * sizeof(int));
   const size_t bytes = o_scal + (size_t)ns * pm::kScalN * sizeof(long long);
   HIPCHECK(hipMalloc(&ls->mem, bytes));
   char *base = static_cast<char *>(ls->mem);
@@ -777,6 +791,7 @@ LaunchSet &get_set(pm_ctx *c, const std::vector<int> &ids) {
   L.corr_prefix = d_pre + 4 * st;
   L.bonded_prefix = d_pre + 5 * st;
   L.gb_prefix = d_pre + 6 * st;
+  L.sc_prefix = d_pre + 8 * st;
   L.n_atoms_total = pre[0].back();
   L.n_cl_total = pre[1].back();
   L.n_cols_total = pre[2].back();
@@ -785,6 +800,7 @@ LaunchSet &get_set(pm_ctx *c, const std::vector<int> &ids) {
   L.n_bonded_total = pre[5].back();
   L.n_gb_tiles_total = pre[6].back();
   L.n_local_total = pre[7].back();
+  L.n_sc_total = pre[8].back();
   LaunchSet &ref = *ls;
   c->sets[ids] = std::move(ls);
   return ref;
@@ -1206,6 +1222,8 @@ int pm_ctx_create(const pm_force_desc *force, int32_t n_systems, const pm_system
     std::memset(&P, 0, sizeof P);
     P.flags = force->flags;
     P.pair_waves = force->pair_waves ? force->pair_waves : 4;
+    P.pair_split = force->pair_split ? force->pair_split : pm::kPairSplitDefault;
+    if (P.pair_split < 1 || P.pair_split > 64) throw Fail{PM_EINVAL, "pair_split must be 1 .. 64"};
     if (P.pair_waves != 1 && P.pair_waves != 2 && P.pair_waves != 4 && P.pair_waves != 8)
       throw Fail{PM_EINVAL, "pair_waves must be 1, 2, 4 or 8"};
     if (force->cell_scale != 0.0 && !(force->cell_scale >= 0.6 && force->cell_scale <= 2.0))

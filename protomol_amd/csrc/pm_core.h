@@ -15,6 +15,9 @@
 namespace pm {
 
 constexpr int kCluster = 8;        // atoms per cluster (one 8x8 tile = one wave64)
+constexpr int kScCl = 8;           // clusters per super-cluster (z-consecutive in one column)
+constexpr int kMaskSlots = 32;     // least exclusion-mask slots per super-cluster list
+constexpr int kPairSplitDefault = 12;  // waves sharing one super-cluster's list in the pair kernel
 constexpr int kWave = 64;
 constexpr int kSortCap = 4096;     // max atoms per x-y column (LDS bitonic sort)
 constexpr double kFScale = 4294967296.0;      // 2^32 fixed point for forces
@@ -73,6 +76,7 @@ struct ForceDev {
   float rlist2;
   unsigned flags;
   int pair_waves;          // waves per workgroup of the pair kernel (1, 2, 4 or 8)
+  int pair_split;          // waves sharing one super-cluster's list
   // GB / OBC (ProtoMolApp.cpp:285-301, GBForce.cpp:11-12)
   double gb_offset, gb_a, gb_b, gb_g, gb_fac;  // fac = 1/eps_solute - 1/eps_solvent
 };
@@ -102,6 +106,9 @@ struct SysDev {
   float4 *lbuild;          // local coordinates at the last rebuild (displacement check)
   // binning scratch
   int *col_count, *col_cursor, *col_atom_start, *col_cl_start, *col_ncl;
+  int *col_sc_start, *col_nsc;  // per column: first super-cluster, count
+  int *sc_col;             // super-cluster -> column
+  int sc_cap;
   int *bin_col;            // atom -> column
   float *bin_z;            // atom -> wrapped z
   unsigned long long *colbuf; // column-ordered (zkey << 32 | atom)
@@ -118,9 +125,15 @@ struct SysDev {
   float4 *cl_lo, *cl_hi;   // bounding boxes (w = unused)
   int *cl_col;             // cluster -> column
   int *cl_nvalid;
+  // pair list per super-cluster: list_cap entries of two uint4,
+  //   {cj | shift << 23 | (nvalid_j - 1) << 28 | has_masks << 31, FP32 offset of the j
+  //    cluster's origin + image from the super-cluster origin (x, y, z)},
+  //   {imask (i clusters of the super-cluster in range) | mask slot << 8, 0, 0, 0}
+  // and mask_cap slots of 16 lane masks (FULL, MODIFIED for each i cluster)
   int *list_len;
-  uint4 *list;             // cl_cap * list_cap entries
-  unsigned long long *list_mask; // 2 per entry: FULL, MODIFIED
+  uint4 *list;
+  unsigned long long *list_mask;
+  int mask_cap;            // mask slots per super-cluster
   // LJ table (table mode): n_types^2 float4 (A, B, A14, B14)
   const float4 *lj_table;
   // bonded
@@ -136,7 +149,7 @@ struct SysDev {
   const double *gb_rho, *gb_S;
   double *gb_born, *gb_drfac, *gb_partial, *gb_spare;
   // per-system scalars
-  int *n_clusters;
+  int *n_clusters, *n_sc;
   long long *acc;          // kAccN fixed-point energy/virial accumulators
   unsigned long long *census; // count, hash
   unsigned *err;

@@ -12,9 +12,9 @@ struct Launch {
   SysDev *sys = nullptr;
   int nsys = 0;
   const int *atom_prefix = nullptr, *cl_prefix = nullptr, *col_prefix = nullptr, *slot_prefix = nullptr,
-            *corr_prefix = nullptr, *bonded_prefix = nullptr, *gb_prefix = nullptr;
+            *corr_prefix = nullptr, *bonded_prefix = nullptr, *gb_prefix = nullptr, *sc_prefix = nullptr;
   int n_atoms_total = 0, n_cl_total = 0, n_cols_total = 0, n_slots_total = 0, n_corr_total = 0,
-      n_bonded_total = 0, n_gb_tiles_total = 0, n_local_total = 0;
+      n_bonded_total = 0, n_gb_tiles_total = 0, n_local_total = 0, n_sc_total = 0;
 };
 
 void launch_import(const Launch &L, int what);

@@ -75,6 +75,11 @@ __device__ __forceinline__ float wave_max_f(float v) {
   for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
   return v;
 }
+// Sum of int64 values over the lanes of a wave (exact, order-free).
+__device__ __forceinline__ long long wave_sum_ll(long long v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
 __device__ __forceinline__ double wave_sum_d(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;
@@ -279,45 +284,57 @@ __global__ void k_bin(SysDev *sys, const int *ap, int nsys) {
 __global__ __launch_bounds__(1024) void k_col_scan(SysDev *sys) {
   SysDev &S = sys[blockIdx.x];
   if (!S.ctl[kCtlRebuild]) return;
-  __shared__ int sa[1024], sc[1024];
+  __shared__ int sa[1024], sc[1024], ss[1024];
   const int t = threadIdx.x, nt = blockDim.x;
   const int ncol = S.ncol;
   const int chunk = (ncol + nt - 1) / nt;
   const int c0 = t * chunk, c1 = min(ncol, c0 + chunk);
-  int la = 0, lc = 0;
+  int la = 0, lc = 0, ls = 0;
   for (int c = c0; c < c1; ++c) {
-    int n = S.col_count[c];
+    const int n = S.col_count[c];
+    const int k = (n + kCluster - 1) / kCluster;
     la += n;
-    lc += (n + kCluster - 1) / kCluster;
+    lc += k;
+    ls += (k + kScCl - 1) / kScCl;
   }
   sa[t] = la;
   sc[t] = lc;
+  ss[t] = ls;
   __syncthreads();
   for (int o = 1; o < nt; o <<= 1) {
-    int va = t >= o ? sa[t - o] : 0, vc = t >= o ? sc[t - o] : 0;
+    const int va = t >= o ? sa[t - o] : 0, vc = t >= o ? sc[t - o] : 0, vs = t >= o ? ss[t - o] : 0;
     __syncthreads();
     sa[t] += va;
     sc[t] += vc;
+    ss[t] += vs;
     __syncthreads();
   }
-  int ba = sa[t] - la, bc = sc[t] - lc;
+  int ba = sa[t] - la, bc = sc[t] - lc, bs = ss[t] - ls;
   for (int c = c0; c < c1; ++c) {
-    int n = S.col_count[c];
-    int k = (n + kCluster - 1) / kCluster;
+    const int n = S.col_count[c];
+    const int k = (n + kCluster - 1) / kCluster;
+    const int q = (k + kScCl - 1) / kScCl;
     S.col_atom_start[c] = ba;
     S.col_cl_start[c] = bc;
     S.col_ncl[c] = k;
+    S.col_sc_start[c] = bs;
+    S.col_nsc[c] = q;
     S.col_cursor[c] = 0;
+    for (int j = 0; j < q; ++j)
+      if (bs + j < S.sc_cap) S.sc_col[bs + j] = c;
     ba += n;
     bc += k;
+    bs += q;
   }
   if (t == nt - 1) {
-    int tot = sc[t];
-    if (tot > S.cl_cap) {
+    int tot = sc[t], tsc = ss[t];
+    if (tot > S.cl_cap || tsc > S.sc_cap) {
       atomicOr(S.err, (unsigned)kErrClusterCap);
-      tot = S.cl_cap;
+      tot = min(tot, S.cl_cap);
+      tsc = min(tsc, S.sc_cap);
     }
     *S.n_clusters = tot;
+    *S.n_sc = tsc;
   }
 }
 
@@ -471,23 +488,35 @@ __device__ __forceinline__ int excl_lookup(const SysDev &S, int a, int b) {
   return 0;
 }
 
-// Cluster-pair list within rlist, half shell over columns.  One workgroup per
-// i column: the boxes of the j clusters in the column's window of neighbour
-// columns (periodic x/y image applied) are staged in LDS once, then each wave
-// tests its i clusters against them, one z image at a time.  Windows larger
-// than the LDS stage are done in several passes.
+// Super-cluster pair list within rlist, half shell over columns.  A
+// super-cluster is kScCl z-consecutive clusters of one column; its list holds
+// the j clusters within rlist of any of them, with an 8-bit mask of which.
+// One workgroup per column: the boxes of the j clusters in the column's window
+// of neighbour columns (periodic x/y image applied) are staged in LDS once;
+// each wave then tests its super-clusters' clusters against them, one z image
+// at a time.  Windows larger than the LDS stage take several passes.  A second
+// pass per super-cluster fills in the entries' FP32 image offsets and the
+// FULL/MODIFIED lane masks of cluster pairs that share a molecule.
 constexpr int kListThreads = 1024;
 constexpr int kListWaves = kListThreads / kWave;
-constexpr int kCandCap = 1536;   // staged j clusters per pass
+constexpr int kCandCap = 1280;   // staged j clusters per pass
 constexpr int kWinCap = 128;     // neighbour columns per pass
 constexpr int kColCap = kSortCap / kCluster;  // clusters of one column
+constexpr int kColSc = kColCap / kScCl;       // super-clusters of one column
 static_assert(kColCap < kCandCap, "a column's clusters must fit one LDS stage");
+
+struct ListScratch {  // per wave, in the candidate stage's LDS once pass A is over
+  int mol[kWave], at[kWave];
+  unsigned char rowF[kScCl][kScCl], rowM[kScCl][kScCl];
+};
 
 __global__ __launch_bounds__(kListThreads) void k_pairlist(SysDev *sys, const int *colp, int nsys, float rlist2) {
   __shared__ float4 c_lo[kCandCap], c_hi[kCandCap];
   __shared__ int c_id[kCandCap];             // cj | (nvalid - 1) << 28
   __shared__ int w_col[kWinCap], w_start[kWinCap + 1], w_sxy[kWinCap];
-  __shared__ int n_win, next_q, i_count[kColCap];
+  __shared__ int n_win, next_q, sc_count[kColSc];
+  __shared__ float4 ib_lo[kListWaves][kScCl], ib_hi[kListWaves][kScCl];
+  static_assert(sizeof(ListScratch) * kListWaves <= sizeof(float4) * kCandCap, "scratch fits the stage");
   const int gcol = blockIdx.x;
   if (gcol >= colp[nsys]) return;
   const int s = find_sys(colp, nsys, gcol);
@@ -501,18 +530,19 @@ __global__ __launch_bounds__(kListThreads) void k_pairlist(SysDev *sys, const in
     if (t == 0) atomicOr(S.err, (unsigned)kErrColumnCap);
     return;
   }
+  const int nsc = S.col_nsc[coli], sc0 = S.col_sc_start[coli];
   const int cxi = coli % S.ncx, cyi = coli / S.ncx;
   const float Lx = (float)S.L[0], Ly = (float)S.L[1], Lz = (float)S.L[2];
   const float ax = (float)S.ax, ay = (float)S.ay;
   const float ix0 = cxi * ax, ix1 = (cxi + 1) * ax, iy0 = cyi * ay, iy1 = (cyi + 1) * ay;
   const int cap = S.list_cap;
   const int wx = 2 * S.rx + 1, nwin_all = wx * (2 * S.ry + 1);
-  for (int k = t; k < nci; k += kListThreads) i_count[k] = 0;
+  for (int k = t; k < nsc; k += kListThreads) sc_count[k] = 0;
   if (t == 0) next_q = 0;
   __syncthreads();
+  // ---- pass A: entries (j cluster, image, i-cluster mask)
   while (true) {
-    // choose this pass's neighbour columns (thread 0, in window order)
-    if (t == 0) {
+    if (t == 0) {  // this pass's neighbour columns, in window order
       int q = next_q, nw = 0, tot = 0;
       w_start[0] = 0;
       for (; q < nwin_all && nw < kWinCap; ++q) {
@@ -536,7 +566,6 @@ __global__ __launch_bounds__(kListThreads) void k_pairlist(SysDev *sys, const in
     }
     __syncthreads();
     const int nw = n_win, ncand = w_start[nw];
-    // stage the window's j-cluster boxes with their x/y image applied
     for (int k = t; k < ncand; k += kListThreads) {
       int w = 0;
       while (w_start[w + 1] <= k) ++w;
@@ -544,33 +573,42 @@ __global__ __launch_bounds__(kListThreads) void k_pairlist(SysDev *sys, const in
       const int sxy = w_sxy[w];
       const float sxL = ((sxy & 3) - 1) * Lx, syL = (((sxy >> 2) & 3) - 1) * Ly;
       const int nvj = S.cl_nvalid[cj];
-      const bool empty = nvj == 0;
       float4 lo = S.cl_lo[cj], hi = S.cl_hi[cj];
       lo.x += sxL; hi.x += sxL;
       lo.y += syL; hi.y += syL;
       lo.w = __int_as_float(sxy);
-      if (empty) lo.x = 1e30f;  // never within range
+      if (nvj == 0) lo.x = 1e30f;  // never within range
       c_lo[k] = lo;
       c_hi[k] = hi;
       c_id[k] = cj | (max(nvj - 1, 0) << 28);
     }
     __syncthreads();
-    // each wave: its i clusters against the staged boxes
-    for (int il = wv; il < nci; il += kListWaves) {
-      const int ci = ci0 + il;
-      if (S.cl_nvalid[ci] == 0) continue;
-      const float4 loi = S.cl_lo[ci], hii = S.cl_hi[ci];
-      uint4 *out = S.list + (size_t)ci * cap;
-      int count = i_count[il];
+    for (int scl = wv; scl < nsc; scl += kListWaves) {
+      const int cl0 = ci0 + scl * kScCl, nic = min(kScCl, nci - scl * kScCl);
+      if (lane < kScCl) {
+        const bool ok = lane < nic && S.cl_nvalid[cl0 + lane] > 0;
+        ib_lo[wv][lane] = ok ? S.cl_lo[cl0 + lane] : make_float4(1e30f, 1e30f, 1e30f, 0.f);
+        ib_hi[wv][lane] = ok ? S.cl_hi[cl0 + lane] : make_float4(1e30f, 1e30f, 1e30f, 0.f);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      float zlo = 1e30f, zhi = -1e30f;
+      for (int c = 0; c < nic; ++c) {
+        zlo = fminf(zlo, ib_lo[wv][c].z);
+        zhi = fmaxf(zhi, ib_hi[wv][c].z);
+      }
+      uint4 *out = S.list + 2 * (size_t)(sc0 + scl) * cap;
+      int count = sc_count[scl];
       for (int sz = -1; sz <= 1; ++sz) {
-        // a z image is a candidate only if the i box comes within rlist of
-        // that image of the wrapped frame [0, Lz)
-        if (sz == 1 && fmaxf(0.f, Lz - hii.z - 0.01f) * fmaxf(0.f, Lz - hii.z - 0.01f) >= rlist2) continue;
-        if (sz == -1 && fmaxf(0.f, loi.z - 0.01f) * fmaxf(0.f, loi.z - 0.01f) >= rlist2) continue;
+        // a z image is a candidate only if the super-cluster's z range comes
+        // within rlist of that image of the wrapped frame [0, Lz)
+        if (sz == 1 && fmaxf(0.f, Lz - zhi - 0.01f) * fmaxf(0.f, Lz - zhi - 0.01f) >= rlist2) continue;
+        if (sz == -1 && fmaxf(0.f, zlo - 0.01f) * fmaxf(0.f, zlo - 0.01f) >= rlist2) continue;
         const float szL = sz * Lz;
         for (int base = 0; base < ncand; base += kWave) {
           const int k = base + lane;
-          bool hit = false;
+          unsigned im = 0;
           int cj = 0, sxy = 0, idw = 0;
           if (k < ncand) {
             const float4 loj = c_lo[k], hij = c_hi[k];
@@ -578,97 +616,154 @@ __global__ __launch_bounds__(kListThreads) void k_pairlist(SysDev *sys, const in
             cj = idw & 0x7FFFFF;
             sxy = __float_as_int(loj.w);
             const bool samecol = (sxy & 16) != 0;
-            if (!(samecol && !(sz > 0 || (sz == 0 && cj >= ci)))) {
+            for (int c = 0; c < nic; ++c) {
+              // each cluster pair once: within a column, j at or above i (z image 0)
+              if (samecol && !(sz > 0 || (sz == 0 && cj >= cl0 + c))) continue;
+              const float4 loi = ib_lo[wv][c], hii = ib_hi[wv][c];
               const float ex = fmaxf(0.f, fmaxf(loj.x - hii.x, loi.x - hij.x));
               const float ey = fmaxf(0.f, fmaxf(loj.y - hii.y, loi.y - hij.y));
               const float ez = fmaxf(0.f, fmaxf(loj.z + szL - hii.z, loi.z - (hij.z + szL)));
-              hit = ex * ex + ey * ey + ez * ez < rlist2;
+              if (ex * ex + ey * ey + ez * ez < rlist2) im |= 1u << c;
             }
           }
+          const bool hit = im != 0;
           const unsigned long long b = wave_ballot(hit);
           if (b == 0) continue;
           const int pos = count + __popcll(b & ((1ull << lane) - 1ull));
           if (hit && pos < cap) {
-            // the image offset is filled in by the second pass below
+            // the image offset and masks are filled in by pass B
             const int sh = (sxy & 3) * 9 + ((sxy >> 2) & 3) * 3 + (sz + 1);
-            out[pos].x = (unsigned)cj | ((unsigned)sh << 23) | ((unsigned)idw & 0x70000000u);
+            out[2 * pos].x = (unsigned)cj | ((unsigned)sh << 23) | ((unsigned)idw & 0x70000000u);
+            out[2 * pos + 1] = make_uint4(im, 0u, 0u, 0u);
           }
           count += __popcll(b);
         }
       }
-      if (lane == 0) i_count[il] = count;
+      if (lane == 0) sc_count[scl] = count;
     }
     __syncthreads();
     if (next_q >= nwin_all) break;
     __syncthreads();
   }
-  // per i cluster: capacity, exclusion masks, length
-  for (int il = wv; il < nci; il += kListWaves) {
-    const int ci = ci0 + il;
-    int count = i_count[il];
+  // ---- pass B, per super-cluster: offsets, exclusion masks, lengths
+  ListScratch &W = reinterpret_cast<ListScratch *>(c_lo)[wv];
+  long long pairs8 = 0;  // 8x8 cluster pairs, for the statistics
+  for (int scl = wv; scl < nsc; scl += kListWaves) {
+    const int sc = sc0 + scl;
+    const int cl0 = ci0 + scl * kScCl, nic = min(kScCl, nci - scl * kScCl);
+    int count = sc_count[scl];
     if (count > cap) {
       if (lane == 0) atomicOr(S.err, (unsigned)kErrListCap);
       count = cap;
     }
-    if (lane == 0) atomicAdd(&S.ctl[kCtlEntries], count);
-    uint4 *out = S.list + (size_t)ci * cap;
-    const double oix = S.cl_origin[ci], oiy = S.cl_origin[S.cl_cap + ci], oiz = S.cl_origin[2 * S.cl_cap + ci];
-    const int ii = lane & 7, jj = lane >> 3;  // the pair kernel's lane layout
-    const int si = ci * kCluster + ii;
-    const int mi = S.smol[si];
-    const int mine = S.smol[ci * kCluster + (lane & 7)];  // lane k < 8 holds i atom k's molecule
-    int mol_i[kCluster];
-    for (int k = 0; k < kCluster; ++k) mol_i[k] = __shfl(mine, k);
-    unsigned long long *mask = S.list_mask + 2 * (size_t)ci * cap;
-    // 64 entries per pass (one per lane): the FP32 offset of the j cluster's
-    // origin (with its image) from the i cluster's, and a screen for a j atom
-    // whose molecule is one of the i cluster's
+    uint4 *out = S.list + 2 * (size_t)sc * cap;
+    const double ox = S.cl_origin[cl0], oy = S.cl_origin[S.cl_cap + cl0], oz = S.cl_origin[2 * S.cl_cap + cl0];
+    {
+      const int c = lane >> 3, ii = lane & 7;
+      int mol = -1, at = -1;
+      if (c < nic) {
+        const int si = (cl0 + c) * kCluster + ii;
+        mol = S.smol[si];
+        at = S.orig[si];
+      }
+      W.mol[lane] = mol;
+      W.at[lane] = at;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    int mslot = 0;
+    unsigned long long *mbase = S.list_mask + (size_t)sc * S.mask_cap * (2 * kScCl);
     for (int e0 = 0; e0 < count; e0 += kWave) {
       const int e = e0 + lane;
-      bool share = false;
+      unsigned share = 0;
+      int cj = 0;
       if (e < count) {
-        const unsigned x = out[e].x;
-        const int cj = (int)(x & 0x7FFFFFu), sh = (int)((x >> 23) & 31u);
+        const unsigned x = out[2 * e].x;
+        const unsigned im = out[2 * e + 1].x & 0xFFu;
+        cj = (int)(x & 0x7FFFFFu);
+        const int sh = (int)((x >> 23) & 31u);
         const int sx = sh / 9 - 1, sy = (sh / 3) % 3 - 1, sz = sh % 3 - 1;
         const double ojx = S.cl_origin[cj] + (double)sx * S.L[0];
         const double ojy = S.cl_origin[S.cl_cap + cj] + (double)sy * S.L[1];
         const double ojz = S.cl_origin[2 * S.cl_cap + cj] + (double)sz * S.L[2];
+        out[2 * e].y = __float_as_uint((float)(ojx - ox));
+        out[2 * e].z = __float_as_uint((float)(ojy - oy));
+        out[2 * e].w = __float_as_uint((float)(ojz - oz));
+        pairs8 += __popc(im);
         const int4 m0 = *reinterpret_cast<const int4 *>(S.smol + cj * kCluster);
         const int4 m1 = *reinterpret_cast<const int4 *>(S.smol + cj * kCluster + 4);
         const int mj[kCluster] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
-        for (int a = 0; a < kCluster; ++a)
-          for (int b = 0; b < kCluster; ++b) share = share || (mj[a] >= 0 && mj[a] == mol_i[b]);
-        out[e].y = __float_as_uint((float)(ojx - oix));
-        out[e].z = __float_as_uint((float)(ojy - oiy));
-        out[e].w = __float_as_uint((float)(ojz - oiz));
+        for (int c = 0; c < kScCl; ++c) {
+          bool sh_c = false;
+          for (int b = 0; b < kCluster; ++b) {
+            const int mi = W.mol[c * kCluster + b];
+            for (int a = 0; a < kCluster; ++a) sh_c = sh_c || (mi >= 0 && mj[a] == mi);
+          }
+          if (sh_c && ((im >> c) & 1u)) share |= 1u << c;
+        }
       }
-      // FULL/MODIFIED masks (bit = lane = jj * 8 + ii) for the entries that share a molecule
-      unsigned long long flagged = wave_ballot(share);
+      unsigned long long flagged = wave_ballot(share != 0);
       while (flagged) {
         const int g = __ffsll((long long)flagged) - 1;
         flagged &= flagged - 1;
+        const unsigned sh_e = __shfl(share, g);
+        const int cj_e = __shfl(cj, g);
         const int ee = e0 + g;
-        const int cj = (int)(out[ee].x & 0x7FFFFFu);
-        const int sj = cj * kCluster + jj;
-        const int mj = S.smol[sj];
-        const bool same = mi >= 0 && mi == mj;
-        int cls = 0;
-        if (same) {
-          const int a = S.orig[si], b = S.orig[sj];
-          if (a != b) cls = excl_lookup(S, a, b);
+        if (mslot >= S.mask_cap) {
+          if (lane == 0) atomicOr(S.err, (unsigned)kErrListCap);
+          continue;
         }
-        const unsigned long long mf = wave_ballot(cls == PM_EXCL_FULL);
-        const unsigned long long mm = wave_ballot(cls == PM_EXCL_MODIFIED);
-        if ((mf | mm) != 0 && lane == 0) {
-          mask[2 * ee] = mf;
-          mask[2 * ee + 1] = mm;
-          out[ee].x |= 0x80000000u;
+        // lanes (k, jj): the k-th i cluster sharing a molecule with j cluster cj_e
+        const int k = lane >> 3, jj = lane & 7;
+        const int ncomb = __popc(sh_e);
+        int ck = -1;
+        {
+          unsigned m = sh_e;
+          for (int r = 0; r < k && m; ++r) m &= m - 1;
+          if (m && k < ncomb) ck = __ffs(m) - 1;
         }
+        unsigned rf = 0, rm = 0;
+        if (ck >= 0) {
+          const int sj = cj_e * kCluster + jj;
+          const int mj = S.smol[sj], bj = S.orig[sj];
+          for (int ii = 0; ii < kCluster; ++ii) {
+            const int mi = W.mol[ck * kCluster + ii], ai = W.at[ck * kCluster + ii];
+            if (mi >= 0 && mi == mj && ai != bj) {
+              const int cls = excl_lookup(S, ai, bj);
+              if (cls == PM_EXCL_FULL) rf |= 1u << ii;
+              if (cls == PM_EXCL_MODIFIED) rm |= 1u << ii;
+            }
+          }
+          W.rowF[k][jj] = (unsigned char)rf;
+          W.rowM[k][jj] = (unsigned char)rm;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // 16 words of the slot: FULL, MODIFIED lane masks (bit jj * 8 + ii) per i cluster
+        if (lane < 2 * kScCl) {
+          const int c = lane >> 1;
+          unsigned long long v = 0;
+          if ((sh_e >> c) & 1u) {
+            const int kc = __popc(sh_e & ((1u << c) - 1u));
+            v = (lane & 1) ? *reinterpret_cast<const unsigned long long *>(W.rowM[kc])
+                           : *reinterpret_cast<const unsigned long long *>(W.rowF[kc]);
+          }
+          mbase[(size_t)mslot * (2 * kScCl) + lane] = v;
+        }
+        if (lane == 0) {
+          out[2 * ee].x |= 0x80000000u;
+          out[2 * ee + 1].x |= (unsigned)mslot << 8;
+        }
+        ++mslot;
+        __builtin_amdgcn_wave_barrier();
       }
     }
-    if (lane == 0) S.list_len[ci] = count;
+    if (lane == 0) S.list_len[sc] = count;
   }
-  // clusters past the column's count keep an empty list
+  pairs8 = wave_sum_ll(pairs8);
+  if (lane == 0 && pairs8) atomicAdd(&S.ctl[kCtlEntries], (int)pairs8);
 }
 
 // ------------------------------------------------------------ pair force
@@ -730,33 +825,111 @@ __device__ __forceinline__ float dpp_sum8_scatter(float x, float y, float z, int
 }
 
 constexpr int kLJLds = 256;  // LJ table entries kept in LDS per wave (T <= 16)
+// LDS per wave of the pair kernel: the LJ table, then the super-cluster's 64
+// i atoms (position + charge, type | valid, atom index, per-atom LJ)
+constexpr int kPairWaveLds = kLJLds * 16 + kWave * (16 + 4 + 4 + 8);
 
-// One wave per i-cluster.  Lane = (jj, ii) = (lane >> 3, lane & 7): i atom ii
-// of the i-cluster against j atom jj of the current j-cluster, so one 8x8
-// cluster pair per step.  FP32 arithmetic on cluster-local coordinates; pairs
-// within `guard` of a cutoff are decided on the reference's FP64 minimal image
-// instead, so the within-cutoff pair set is exactly the FP64 one.  i-forces
-// accumulate in registers; j-forces are summed over the 8 i lanes with DPP and
-// land, like the i-forces, in 64-bit fixed point, so the total is independent
-// of the order in which waves finish.
+// One pair term set (OneAtomPair[Two]::doOneAtomPair, OneAtomPair.h:47-107):
+// the scalar force factor F/r for the pair, energies into ea / eb.
+template <int TA, int TB, int SWA, int SWB, bool SAME>
+__device__ __forceinline__ float pair_terms(const ForceDev &P, float r2, float qq, float A, float B, bool mod,
+                                            float csf, bool inA, bool inB, float &ea, float &eb) {
+  const float rinv = __builtin_amdgcn_rsqf(r2);
+  const float r2inv = rinv * rinv;
+  float fs = 0.f;
+  ea = 0.f;
+  eb = 0.f;
+  if (TA == PM_TERM_LJ) {
+    // LennardJonesForce::operator() (LennardJonesForce.h:45-58)
+    const float r6 = r2inv * r2inv * r2inv;
+    const float r6B = B * r6;
+    const float r12A = A * r6 * r6;
+    float E = r12A - r6B;
+    float F = (12.0f * r12A - 6.0f * r6B) * r2inv;
+    apply_switch_t<SWA, !SAME>(P.sw_a, r2, E, F);
+    fs += inA ? F : 0.f;
+    ea = inA ? E : 0.f;
+  } else if (TA == PM_TERM_COULOMB) {
+    float E = qq * rinv;
+    if (mod) E *= csf;
+    float F = E * r2inv;
+    apply_switch_t<SWA, !SAME>(P.sw_a, r2, E, F);
+    fs += inA ? F : 0.f;
+    ea = inA ? E : 0.f;
+  }
+  if (TB == PM_TERM_COULOMB) {
+    // CoulombForce::operator() (CoulombForce.h:29-40)
+    float E = qq * rinv;
+    if (mod) E *= csf;
+    float F = E * r2inv;
+    apply_switch_t<SWB, !SAME>(P.sw_b, r2, E, F);
+    fs += inB ? F : 0.f;
+    eb = inB ? E : 0.f;
+  } else if (TB == PM_TERM_EWALD_REAL) {
+    // NonbondedFullEwaldSystemForce::realTerm (NonbondedFullEwaldSystemForce.h:818-834):
+    // erfc(ar) by Abramowitz & Stegun 7.1.26, the approximation the reference uses
+    float q2 = qq;
+    if (mod) q2 *= csf;
+    const float r = r2 * rinv;
+    const float ar = P.alpha * r;
+    const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * ar);
+    const float poly =
+        t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
+    const float ee = q2 * __builtin_amdgcn_exp2f(-(ar * ar) * 1.4426950408889634f);
+    const float E = poly * ee * rinv;
+    const float F = (E + P.two_alpha_pi * ee) * r2inv;
+    fs += inB ? F : 0.f;
+    eb = inB ? E : 0.f;
+  }
+  return fs;
+}
+
+// P.pair_split waves share one super-cluster's list, each taking an equal
+// share of its entries.  An entry is one j cluster against the super-cluster's i
+// clusters set in its 8-bit mask; lane = (jj, ii) = (lane >> 3, lane & 7)
+// holds i atom ii of the current i cluster against j atom jj, so one 8x8
+// cluster pair per step.  The i atoms (positions relative to the
+// super-cluster origin) sit in LDS, the j cluster in registers across its i
+// clusters; FP32 arithmetic; pairs within `guard` of a cutoff are decided on
+// the reference's FP64 minimal image instead, so the within-cutoff pair set is
+// exactly the FP64 one.  i forces accumulate in registers per i cluster;
+// j forces are summed over the entry's i clusters and the 8 i lanes (DPP);
+// both land in 64-bit fixed point, so the total does not depend on the order
+// in which waves finish.
 // SAME: both terms share one inclusion cutoff (one test per pair, and no
 // "outer" branch in the switch)
+// waves per SIMD the register allocation must allow (6 measured best on
+// gfx950: a little spilling, more waves in flight)
+#ifndef PM_PAIR_MIN_WAVES
+#define PM_PAIR_MIN_WAVES 6
+#endif
 template <int TA, int TB, int SWA, int SWB, int KIND, bool SAME>
-__global__ __launch_bounds__(512) void k_pair(const SysDev *__restrict__ sys, const int *__restrict__ clp, int nsys,
+__global__ __launch_bounds__(512, PM_PAIR_MIN_WAVES) void k_pair(const SysDev *__restrict__ sys, const int *__restrict__ scp, int nsys,
                                               ForceDev P) {
   constexpr bool FORCES = KIND != kPairEnergies;
   constexpr bool EN = KIND == kPairForcesEnergies || KIND == kPairEnergies;
   constexpr bool CEN = KIND == kPairCensus;
-  extern __shared__ float4 lj_lds[];  // kLJLds entries per wave of the workgroup
+  extern __shared__ float4 pair_lds[];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
-  float4 *ljs = lj_lds + wv * kLJLds;
+  char *wb = reinterpret_cast<char *>(pair_lds) + (size_t)wv * kPairWaveLds;
+  float4 *ljs = reinterpret_cast<float4 *>(wb);
+  float4 *lx = reinterpret_cast<float4 *>(wb + kLJLds * 16);
+  int *linfo = reinterpret_cast<int *>(wb + kLJLds * 16 + kWave * 16);  // ti * T | valid << 30
+  int *lorig = linfo + kWave;
+  float2 *lpp = reinterpret_cast<float2 *>(lorig + kWave);
   const int w = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
-  if (w >= clp[nsys]) return;
-  const int s = __builtin_amdgcn_readfirstlane(find_sys(clp, nsys, w));
+  const int split = P.pair_split;
+  const int part = w % split, scg = w / split;
+  if (scg >= scp[nsys]) return;
+  const int s = __builtin_amdgcn_readfirstlane(find_sys(scp, nsys, scg));
   const SysDev *__restrict__ S = sys + s;
-  const int ci = w - clp[s];
-  if (ci >= *cp(S->n_clusters)) return;
+  const int sc = scg - scp[s];
+  if (sc >= *cp(S->n_sc)) return;
+  const int col = *cp(S->sc_col + sc);
+  const int kz = sc - *cp(S->col_sc_start + col);
+  const int cl0 = *cp(S->col_cl_start + col) + kz * kScCl;
+  const int nic = min(kScCl, *cp(S->col_ncl + col) - kz * kScCl);
   const PM_GLOBAL float4 *xq = gp(S->xq);
   const PM_GLOBAL int *ljt = gp(S->ljt);
   const PM_GLOBAL float2 *ljp = gp(S->ljp);
@@ -767,193 +940,189 @@ __global__ __launch_bounds__(512) void k_pair(const SysDev *__restrict__ sys, co
   const int ii = lane & 7, jj = lane >> 3;
   // j forces: lanes ii = 0, 1, 2 of each j group hold the x, z, y sums
   PM_GLOBAL long long *gfc = gfx + (size_t)(ii == 1 ? 2 : (ii == 2 ? 1 : 0)) * S->f_stride;
-  const int si = ci * kCluster + ii;
-  const bool vi = ii < *cp(S->cl_nvalid + ci);
-  // wave-uniform lane masks (lane = jj * 8 + ii): real i atoms, and the i < j
-  // triangle of the cluster's pairs with itself
-  const unsigned long long vim = wave_ballot(vi);
-  const unsigned long long trim = wave_ballot(ii < jj);
-  const float4 xi = xq[si];
   const bool peratom = S->lj_mode == PM_LJ_PERATOM;
   const int T = S->n_types;
-  const bool tab_lds = TA == PM_TERM_LJ && !peratom;  // T <= 16 (checked at context creation)
-  const PM_GLOBAL float4 *gtab = gp(S->lj_table);
-  if (tab_lds) {
+  // stage the super-cluster's i atoms: lane = c * 8 + atom
+  {
+    const int c = lane >> 3, a = lane & 7;
+    float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
+    int info = 0, og = -1;
+    float2 pp = make_float2(0.f, 0.f);
+    if (c < nic) {
+      const int cl = cl0 + c, si = cl * kCluster + a;
+      const float4 x = xq[si];
+      const double *O = S->cl_origin;
+      const int cc = S->cl_cap;
+      q = make_float4(x.x + (float)(O[cl] - O[cl0]), x.y + (float)(O[cc + cl] - O[cc + cl0]),
+                      x.z + (float)(O[2 * cc + cl] - O[2 * cc + cl0]), x.w);
+      const bool valid = a < S->cl_nvalid[cl];
+      info = (valid ? (1 << 30) : 0);
+      if (TA == PM_TERM_LJ && !peratom) info |= ljt[si] * T;
+      if (TA == PM_TERM_LJ && peratom) pp = ljp[si];
+      og = orig[si];
+    }
+    lx[lane] = q;
+    linfo[lane] = info;
+    lorig[lane] = og;
+    lpp[lane] = pp;
+  }
+  if (TA == PM_TERM_LJ && !peratom) {  // T <= 16 (checked at context creation)
+    const PM_GLOBAL float4 *gtab = gp(S->lj_table);
     for (int k = lane; k < T * T; k += kWave) ljs[k] = gtab[k];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
-  int tiT = 0;
-  float2 pi = make_float2(0.f, 0.f);
-  if (TA == PM_TERM_LJ) {
-    if (peratom) pi = ljp[si];
-    else tiT = ljt[si] * T;
-  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // wave-uniform lane masks (lane = jj * 8 + ii): real i atoms per i cluster,
+  // and the i < j triangle of a cluster's pairs with itself
+  unsigned long long vim[kScCl];
+#pragma unroll
+  for (int c = 0; c < kScCl; ++c) vim[c] = wave_ballot((linfo[c * kCluster + ii] >> 30) & 1);
+  const unsigned long long trim = wave_ballot(ii < jj);
   const float csf = S->csf, ljsc = S->lj_scaling;
-  int ai = 0;
-  if (CEN) ai = orig[si];
-  float fix = 0.f, fiy = 0.f, fiz = 0.f;
+  float fi[kScCl][3];
+#pragma unroll
+  for (int c = 0; c < kScCl; ++c) fi[c][0] = fi[c][1] = fi[c][2] = 0.f;
   double eA = 0.0, eB = 0.0;
   double vxx = 0.0, vxy = 0.0, vxz = 0.0, vyy = 0.0, vyz = 0.0, vzz = 0.0;
   unsigned long long ccount = 0, chash = 0;
-  const int len = *cp(S->list_len + ci);
-  const PM_CONST uint4 *Lst = cp(S->list + (size_t)ci * S->list_cap);
-  const PM_CONST unsigned long long *M = cp(S->list_mask + 2 * (size_t)ci * S->list_cap);
-  for (int e = 0; e < len; ++e) {
-    const uint4 ent = Lst[e];
+  const int len = *cp(S->list_len + sc);
+  const int e_begin = part * len / split, e_end = (part + 1) * len / split;
+  const PM_CONST uint4 *Lst = cp(S->list + 2 * (size_t)sc * S->list_cap);
+  const PM_CONST unsigned long long *Mb = cp(S->list_mask + (size_t)sc * S->mask_cap * (2 * kScCl));
+  for (int e = e_begin; e < e_end; ++e) {
+    const uint4 ent = Lst[2 * e];
+    const unsigned e1 = Lst[2 * e + 1].x;
     const unsigned w0 = ent.x;
     const int cj = (int)(w0 & 0x7FFFFFu);
     const int sh = (int)((w0 >> 23) & 31u);
     const int nvj = (int)((w0 >> 28) & 7u) + 1;
+    const unsigned imask = e1 & 0xFFu;
     const int sj = cj * kCluster + jj;
     const float4 xj = xq[sj];
+    const float xjx = xj.x + __uint_as_float(ent.y), xjy = xj.y + __uint_as_float(ent.z),
+                xjz = xj.z + __uint_as_float(ent.w);
     int tj = 0;
     float2 pj = make_float2(0.f, 0.f);
     if (TA == PM_TERM_LJ) {
       if (peratom) pj = ljp[sj];
       else tj = ljt[sj];
     }
-    const float dx = (xj.x - xi.x) + __uint_as_float(ent.y);
-    const float dy = (xj.y - xi.y) + __uint_as_float(ent.z);
-    const float dz = (xj.z - xi.z) + __uint_as_float(ent.w);
-    const float r2 = dx * dx + dy * dy + dz * dz;
-    // pairs to evaluate, as a wave-uniform lane mask: real j atoms (lanes
-    // below 8 * nvj), real i atoms, each self pair once, FULL exclusions out
-    unsigned long long am = vim & (nvj == 8 ? ~0ull : ((1ull << (8 * nvj)) - 1ull));
-    if (cj == ci && sh == kShiftCenter) am &= trim;
-    bool mod = false;
-    if (w0 & 0x80000000u) {
-      // masks are indexed by lane = jj * 8 + ii
-      const unsigned long long mf = M[2 * e], mm = M[2 * e + 1];
-      am &= ~mf;
-      mod = (mm >> lane) & 1ull;
-    }
-    const bool act = (am >> lane) & 1ull;
-    bool inA, inB;
-    if (SAME) {
-      bool in = r2 <= P.cutA2;
-      if (__builtin_expect(act && fabsf(r2 - P.cutA2) <= P.guard, 0))
-        in = min_image_r2(*S, orig[si], orig[sj]) <= P.cutA2d;
-      inA = inB = in && act;
-    } else {
-      inA = (TA != PM_TERM_NONE) && r2 <= P.cutA2;
-      inB = (TB != PM_TERM_NONE) && r2 <= P.cutB2;
-      const bool nearA = (TA != PM_TERM_NONE) && fabsf(r2 - P.cutA2) <= P.guard;
-      const bool nearB = (TB != PM_TERM_NONE) && fabsf(r2 - P.cutB2) <= P.guard;
-      if (__builtin_expect(act && (nearA || nearB), 0)) {
-        const double r2d = min_image_r2(*S, orig[si], orig[sj]);
-        inA = (TA != PM_TERM_NONE) && r2d <= P.cutA2d;
-        inB = (TB != PM_TERM_NONE) && r2d <= P.cutB2d;
+    const unsigned long long jm = nvj == 8 ? ~0ull : ((1ull << (8 * nvj)) - 1ull);
+    const int selfc = sh == kShiftCenter ? cj - cl0 : -1;  // i cluster that is cj itself
+    const bool flag = (w0 & 0x80000000u) != 0;
+    const PM_CONST unsigned long long *M = Mb + (size_t)(e1 >> 8) * (2 * kScCl);
+    float fjx = 0.f, fjy = 0.f, fjz = 0.f;
+#pragma unroll
+    for (int c = 0; c < kScCl; ++c) {
+      if (!((imask >> c) & 1u)) continue;
+      const float4 xi = lx[c * kCluster + ii];
+      const int info = linfo[c * kCluster + ii];
+      const float dx = xjx - xi.x, dy = xjy - xi.y, dz = xjz - xi.z;
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      unsigned long long am = vim[c] & jm;
+      if (c == selfc) am &= trim;
+      bool mod = false;
+      if (flag) {
+        am &= ~M[2 * c];
+        mod = (M[2 * c + 1] >> lane) & 1ull;
       }
-      inA = inA && act;
-      inB = inB && act;
-    }
-    if (wave_ballot(inA || inB) == 0) continue;
-
-    const float rinv = __builtin_amdgcn_rsqf(r2);
-    const float r2inv = rinv * rinv;
-    float fs = 0.f, ea = 0.f, eb = 0.f;
-    if (TA == PM_TERM_LJ) {
-      float A, B;
-      if (peratom) {
-        // GROMACS mixing (BuildTopology.cpp:650-659): sigma = (si+sj)/2, eps = sqrt(ei ej)
-        const float sg = pi.x + pj.x, ep = pi.y * pj.y;
-        const float sg2 = sg * sg, sg6 = sg2 * sg2 * sg2;
-        B = 4.0f * ep * sg6;
-        A = B * sg6;
-        if (mod) {
-          A *= ljsc;
-          B *= ljsc;
-        }
+      const bool act = (am >> lane) & 1ull;
+      bool inA, inB;
+      if (SAME) {
+        bool in = r2 <= P.cutA2;
+        if (__builtin_expect(act && fabsf(r2 - P.cutA2) <= P.guard, 0))
+          in = min_image_r2(*S, lorig[c * kCluster + ii], orig[sj]) <= P.cutA2d;
+        inA = inB = in && act;
       } else {
-        const float4 p = ljs[tiT + tj];
-        A = mod ? p.z : p.x;
-        B = mod ? p.w : p.y;
+        inA = (TA != PM_TERM_NONE) && r2 <= P.cutA2;
+        inB = (TB != PM_TERM_NONE) && r2 <= P.cutB2;
+        const bool nearA = (TA != PM_TERM_NONE) && fabsf(r2 - P.cutA2) <= P.guard;
+        const bool nearB = (TB != PM_TERM_NONE) && fabsf(r2 - P.cutB2) <= P.guard;
+        if (__builtin_expect(act && (nearA || nearB), 0)) {
+          const double r2d = min_image_r2(*S, lorig[c * kCluster + ii], orig[sj]);
+          inA = (TA != PM_TERM_NONE) && r2d <= P.cutA2d;
+          inB = (TB != PM_TERM_NONE) && r2d <= P.cutB2d;
+        }
+        inA = inA && act;
+        inB = inB && act;
       }
-      // LennardJonesForce::operator() (LennardJonesForce.h:45-58)
-      const float r6 = r2inv * r2inv * r2inv;
-      const float r6B = B * r6;
-      const float r12A = A * r6 * r6;
-      float E = r12A - r6B;
-      float F = (12.0f * r12A - 6.0f * r6B) * r2inv;
-      apply_switch_t<SWA, !SAME>(P.sw_a, r2, E, F);
-      fs += inA ? F : 0.f;
-      ea = inA ? E : 0.f;
-    } else if (TA == PM_TERM_COULOMB) {
-      float E = xi.w * xj.w * rinv;
-      if (mod) E *= csf;
-      float F = E * r2inv;
-      apply_switch_t<SWA, !SAME>(P.sw_a, r2, E, F);
-      fs += inA ? F : 0.f;
-      ea = inA ? E : 0.f;
-    }
-    if (TB == PM_TERM_COULOMB) {
-      // CoulombForce::operator() (CoulombForce.h:29-40)
-      float E = xi.w * xj.w * rinv;
-      if (mod) E *= csf;
-      float F = E * r2inv;
-      apply_switch_t<SWB, !SAME>(P.sw_b, r2, E, F);
-      fs += inB ? F : 0.f;
-      eb = inB ? E : 0.f;
-    } else if (TB == PM_TERM_EWALD_REAL) {
-      // NonbondedFullEwaldSystemForce::realTerm (NonbondedFullEwaldSystemForce.h:818-834):
-      // erfc(ar) by Abramowitz & Stegun 7.1.26, the approximation the reference uses
-      float qq = xi.w * xj.w;
-      if (mod) qq *= csf;
-      const float r = r2 * rinv;
-      const float ar = P.alpha * r;
-      const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * ar);
-      const float poly =
-          t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
-      const float ee = qq * __builtin_amdgcn_exp2f(-(ar * ar) * 1.4426950408889634f);
-      const float E = poly * ee * rinv;
-      const float F = (E + P.two_alpha_pi * ee) * r2inv;
-      fs += inB ? F : 0.f;
-      eb = inB ? E : 0.f;
-    }
-    const float fx = fs * dx, fy = fs * dy, fz = fs * dz;
-    fix -= fx;
-    fiy -= fy;
-    fiz -= fz;
-    if (EN) {
-      eA += (double)ea;
-      eB += (double)eb;
-      vxx += (double)(fx * dx);
-      vxy += (double)(fx * dy);
-      vxz += (double)(fx * dz);
-      vyy += (double)(fy * dy);
-      vyz += (double)(fy * dz);
-      vzz += (double)(fz * dz);
-    }
-    if (CEN) {
-      if (inA || inB) {
-        const int aj = orig[sj];
-        const unsigned lo = (unsigned)min(ai, aj), hi = (unsigned)max(ai, aj);
-        ccount += 1;
-        chash += pair_hash(lo, hi);
+      if (wave_ballot(inA || inB) == 0) continue;
+      float A = 0.f, B = 0.f;
+      if (TA == PM_TERM_LJ) {
+        if (peratom) {
+          // GROMACS mixing (BuildTopology.cpp:650-659): sigma = (si+sj)/2, eps = sqrt(ei ej)
+          const float2 pi = lpp[c * kCluster + ii];
+          const float sg = pi.x + pj.x, ep = pi.y * pj.y;
+          const float sg2 = sg * sg, sg6 = sg2 * sg2 * sg2;
+          B = 4.0f * ep * sg6;
+          A = B * sg6;
+          if (mod) {
+            A *= ljsc;
+            B *= ljsc;
+          }
+        } else {
+          const float4 p = ljs[(info & 0xFFFF) + tj];
+          A = mod ? p.z : p.x;
+          B = mod ? p.w : p.y;
+        }
+      }
+      float ea, eb;
+      const float fs = pair_terms<TA, TB, SWA, SWB, SAME>(P, r2, xi.w * xj.w, A, B, mod, csf, inA, inB, ea, eb);
+      const float fx = fs * dx, fy = fs * dy, fz = fs * dz;
+      fi[c][0] -= fx;
+      fi[c][1] -= fy;
+      fi[c][2] -= fz;
+      fjx += fx;
+      fjy += fy;
+      fjz += fz;
+      if (EN) {
+        eA += (double)ea;
+        eB += (double)eb;
+        vxx += (double)(fx * dx);
+        vxy += (double)(fx * dy);
+        vxz += (double)(fx * dz);
+        vyy += (double)(fy * dy);
+        vyz += (double)(fy * dz);
+        vzz += (double)(fz * dz);
+      }
+      if (CEN) {
+        if (inA || inB) {
+          const int ai = lorig[c * kCluster + ii], aj = orig[sj];
+          const unsigned lo = (unsigned)min(ai, aj), hi = (unsigned)max(ai, aj);
+          ccount += 1;
+          chash += pair_hash(lo, hi);
+        }
       }
     }
-    // Newton's third law: j force = sum over the 8 i lanes of this j atom
+    // Newton's third law: j force = sum over the i clusters and the 8 i lanes
     if (FORCES) {
-      const float g = dpp_sum8_scatter(fx, fy, fz, ii);
+      const float g = dpp_sum8_scatter(fjx, fjy, fjz, ii);
       if (ii < 3 && jj < nvj) atomic_add_ll(&gfc[sj], to_fixed((double)g, kFScale));
     }
   }
   // i force = sum over the 8 j lanes that share ii
-  fix += __shfl_xor(fix, 8);
-  fiy += __shfl_xor(fiy, 8);
-  fiz += __shfl_xor(fiz, 8);
-  fix += __shfl_xor(fix, 16);
-  fiy += __shfl_xor(fiy, 16);
-  fiz += __shfl_xor(fiz, 16);
-  fix += __shfl_xor(fix, 32);
-  fiy += __shfl_xor(fiy, 32);
-  fiz += __shfl_xor(fiz, 32);
-  if (FORCES && jj == 0 && vi) {
-    atomic_add_ll(&gfx[si], to_fixed((double)fix, kFScale));
-    atomic_add_ll(&gfy[si], to_fixed((double)fiy, kFScale));
-    atomic_add_ll(&gfz[si], to_fixed((double)fiz, kFScale));
+  if (FORCES) {
+#pragma unroll
+    for (int c = 0; c < kScCl; ++c) {
+      if (c >= nic) break;
+      float ax = fi[c][0], ay = fi[c][1], az = fi[c][2];
+      ax += __shfl_xor(ax, 8);
+      ay += __shfl_xor(ay, 8);
+      az += __shfl_xor(az, 8);
+      ax += __shfl_xor(ax, 16);
+      ay += __shfl_xor(ay, 16);
+      az += __shfl_xor(az, 16);
+      ax += __shfl_xor(ax, 32);
+      ay += __shfl_xor(ay, 32);
+      az += __shfl_xor(az, 32);
+      if (jj == 0 && ((vim[c] >> lane) & 1ull)) {
+        const int si = (cl0 + c) * kCluster + ii;
+        atomic_add_ll(&gfx[si], to_fixed((double)ax, kFScale));
+        atomic_add_ll(&gfy[si], to_fixed((double)ay, kFScale));
+        atomic_add_ll(&gfz[si], to_fixed((double)az, kFScale));
+      }
+    }
   }
   if (EN) {
     eA = wave_sum_d(eA);
@@ -1001,11 +1170,6 @@ __device__ __forceinline__ void min_image_vec(const SysDev &S, double *d) {
     for (int k = 0; k < 3; ++k) d[k] -= S.L[k] * rint(S.Lr[k] * d[k]);
 }
 
-// Sum of int64 values over the lanes of a wave (exact, order-free).
-__device__ __forceinline__ long long wave_sum_ll(long long v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
-}
 
 // Bonded terms and Ewald excluded-pair corrections, one thread per atom
 // ("owner computes"): the thread walks the terms its atom takes part in and
@@ -1334,24 +1498,24 @@ void launch_rebuild(const Launch &L, float rlist2) {
 
 template <int TA, int TB, int SWA, int SWB, bool SAME = false>
 static void launch_pair_t(const Launch &L, const ForceDev &P, int kind) {
-  // one wave per i-cluster; P.pair_waves waves per workgroup, each with its
-  // own LDS copy of the LJ table
+  // P.pair_split waves per super-cluster; P.pair_waves waves per workgroup,
+  // each with its own LDS region
   const int waves = P.pair_waves;
   const int threads = kWave * waves;
-  dim3 g(nblk((long long)L.n_cl_total * kWave, threads)), b(threads);
-  const size_t lds = (size_t)waves * kLJLds * sizeof(float4);
+  dim3 g(nblk((long long)L.n_sc_total * P.pair_split, waves)), b(threads);
+  const size_t lds = (size_t)waves * kPairWaveLds;
   switch (kind) {
   case kPairForces:
-    hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairForces, SAME>), g, b, lds, L.stream, L.sys, L.cl_prefix, L.nsys, P);
+    hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairForces, SAME>), g, b, lds, L.stream, L.sys, L.sc_prefix, L.nsys, P);
     break;
   case kPairForcesEnergies:
-    hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairForcesEnergies, SAME>), g, b, lds, L.stream, L.sys, L.cl_prefix, L.nsys, P);
+    hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairForcesEnergies, SAME>), g, b, lds, L.stream, L.sys, L.sc_prefix, L.nsys, P);
     break;
   case kPairEnergies:
-    hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairEnergies, SAME>), g, b, lds, L.stream, L.sys, L.cl_prefix, L.nsys, P);
+    hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairEnergies, SAME>), g, b, lds, L.stream, L.sys, L.sc_prefix, L.nsys, P);
     break;
   default:
-    hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairCensus, SAME>), g, b, lds, L.stream, L.sys, L.cl_prefix, L.nsys, P);
+    hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairCensus, SAME>), g, b, lds, L.stream, L.sys, L.sc_prefix, L.nsys, P);
     break;
   }
 }
@@ -1370,7 +1534,7 @@ static void launch_pair_sw(const Launch &L, const ForceDev &P, int kind) {
 }
 
 bool launch_pair(const Launch &L, const ForceDev &P, int kind) {
-  if (!L.n_cl_total) return true;
+  if (!L.n_sc_total) return true;
   const int a = P.term_a, b = P.term_b;
   if (a == PM_TERM_LJ && b == PM_TERM_NONE) launch_pair_sw<PM_TERM_LJ, PM_TERM_NONE>(L, P, kind);
   else if (a == PM_TERM_COULOMB && b == PM_TERM_NONE) launch_pair_sw<PM_TERM_COULOMB, PM_TERM_NONE>(L, P, kind);
