@@ -11,8 +11,9 @@
  *   pm_ctx_create    Topology<PBC,CubicCellManager> + ForceFactory::make of
  *                    NonbondedCutoffSystemForce<CCM, OneAtomPair[Two]<...>>
  *                    (factory/ForceFactory.cpp:21, module/NonbondedCutoffForceModule.cpp:56-88),
- *                    NonbondedFullEwaldSystemForce<PBC,CCM,true,false,corr,Cutoff>
- *                    (module/NonbondedFullElectrostaticForceModule.cpp:31-36) and the GB
+ *                    NonbondedFullEwaldSystemForce<PBC,CCM,true,false,corr,Cutoff> and,
+ *                    with PM_FLAG_EWALD_RECIPROCAL, <PBC,CCM,true,true,corr,Cutoff>
+ *                    ("CoulombEwald", module/NonbondedFullElectrostaticForceModule.cpp:31-36), and the GB
  *                    NonbondedSimpleFullSystemForce<OneAtomPairNoExclusion<VBC,Universal,GB*>>
  *                    (module/NonbondedSimpleFullForceModule.cpp:203-206)
  *   pm_evaluate      SystemForce::evaluate(topo, positions, forces, energies)
@@ -74,6 +75,9 @@ extern "C" {
 #define PM_FLAG_GB 2u         /* GBBornRadii -> GBPartialSum -> GBForce, all pairs, vacuum */
 #define PM_FLAG_BONDED 4u     /* harmonic Bond + Angle (force/bonded/{Bond,Angle}SystemForce.h) */
 #define PM_FLAG_TIMING 8u     /* time every pair-kernel launch with HIP events (pm_stats [4], [5]) */
+#define PM_FLAG_EWALD_RECIPROCAL 16u /* with PM_TERM_EWALD_REAL: add the reciprocal-space sum over
+                                        |k| < kc (NonbondedFullEwaldSystemForce::reciprocalTerm,
+                                        force/nonbonded/NonbondedFullEwaldSystemForce.h:873-1083) */
 
 /* ScalarStructure slots (type/ScalarStructure.h:26-67); energies[] has 36 */
 #define PM_E_COULOMB 0
@@ -210,7 +214,9 @@ int pm_md_get_state_systems(pm_ctx *ctx, int32_t n, const int32_t *ids, double *
  * [4] pair-kernel ms summed over the call's launches (PM_FLAG_TIMING, or the
  * single launch of an evaluation), [5] pair-kernel launches timed,
  * [6] largest displacement (A) of an atom from its position at the pair-list
- * build during the last MD call, [7] the pair-list skin (A) in use. */
+ * build during the last MD call, [7] the pair-list skin (A) in use,
+ * [8] reciprocal-space ms of the last evaluation (PM_FLAG_EWALD_RECIPROCAL),
+ * [9] reciprocal-space k-vectors, summed over the context's systems. */
 int pm_stats(pm_ctx *ctx, double *stats, int32_t n);
 
 #ifdef __cplusplus

@@ -83,6 +83,8 @@ def lib():
         L.orc_ewald_init.argtypes = [C.POINTER(OrcEwald), C.c_double, C.c_double, C.c_double, C.c_int, C.c_int]
         L.orc_ewald_real.argtypes = [C.POINTER(OrcSystem), C.POINTER(OrcEwald), _dp, _dp, _dp,
                                      C.POINTER(OrcCensus), C.c_int]
+        L.orc_ewald_kvectors.argtypes = [_dp, C.c_double, C.c_double, C.POINTER(_ip), _ip]
+        L.orc_ewald_reciprocal.argtypes = [C.POINTER(OrcSystem), C.POINTER(OrcEwald), _dp, _dp, _dp, C.c_int]
         L.orc_poly5.argtypes = [C.c_double]
         L.orc_poly5.restype = C.c_double
         L.orc_myerf.argtypes = [C.c_double]
@@ -178,6 +180,34 @@ def ewald_real(sys, alpha, accuracy, cellsize=5.0, correction=True, virial=False
     if rc != 0:
         raise RuntimeError(f"orc_ewald_real failed: {rc}")
     return f.reshape(-1, 3), e, (cen.count, cen.hash), ew.rc
+
+
+def ewald_kvectors(box, alpha, accuracy):
+    """(h, k, l) of every reciprocal-space vector, in the reference's order,
+    and (hmax, kmax, lmax, hklmax)."""
+    out = _ip()
+    hkl = np.zeros(4, np.int32)
+    boxa = np.ascontiguousarray(box, np.float64)
+    nk = lib().orc_ewald_kvectors(_p(boxa), alpha, accuracy, C.byref(out), _p(hkl, _ip))
+    if nk < 0:
+        raise RuntimeError("orc_ewald_kvectors failed")
+    kv = np.ctypeslib.as_array(out, shape=(max(nk, 1) * 4,))[: 4 * nk].reshape(-1, 4)[:, :3].copy()
+    C.CDLL(None).free(C.cast(out, C.c_void_p))
+    return kv, tuple(int(v) for v in hkl)
+
+
+def ewald_reciprocal(sys, alpha, accuracy, virial=False, nthreads=1, x=None):
+    """Reciprocal-space Ewald term alone: forces (n, 3), 36 energy slots."""
+    os_ = OracleSystem(sys)
+    ew = OrcEwald()
+    lib().orc_ewald_init(C.byref(ew), alpha, accuracy, 5.0, 0, 1 if virial else 0)
+    xx = np.ascontiguousarray(sys.x if x is None else x, np.float64).reshape(-1)
+    f = np.zeros(3 * sys.n)
+    e = np.zeros(NSLOTS)
+    rc = lib().orc_ewald_reciprocal(C.byref(os_.c), C.byref(ew), _p(xx), _p(f), _p(e), nthreads)
+    if rc != 0:
+        raise RuntimeError(f"orc_ewald_reciprocal failed: {rc}")
+    return f.reshape(-1, 3), e
 
 
 def census_bruteforce(sys, cut2, x=None):

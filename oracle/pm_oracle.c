@@ -12,6 +12,7 @@
 
 #include <math.h>
 #include <pthread.h>
+#include <limits.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -894,6 +895,201 @@ done:
   free(ctx); free(ctxp); free(fbuf); free(ebuf); free(cbuf); free(pairs); free(lat);
   free(ex.tab); cells_free(&cl);
   return rc;
+}
+
+/* ------------------------------------------------ Ewald reciprocal space */
+
+/* The k-vector list of NonbondedFullEwaldSystemForce::initialize
+ * (NonbondedFullEwaldSystemForce.h:604-667): h >= 0 half space, |k| < kc,
+ * in (h, k, l) loop order.  Returns the count (or -1), fills *out (malloc'd,
+ * 4 ints h, k, l, 0 per vector) and hkl[4] = {hmax, kmax, lmax, hklmax}. */
+int orc_ewald_kvectors(const double box[3], double alpha, double accuracy, int **out, int hkl[4]) {
+  orc_pbc p;
+  double org[3] = {0, 0, 0};
+  pbc_set(&p, box, org);
+  double LX = box[0], LY = box[1], LZ = box[2];
+  double pp = -log(accuracy);
+  double kc = 2.0 * alpha * sqrt(pp);
+  int hmax = (int)floor(kc / (2.0 * M_PI) * LX);
+  int kmax = (int)floor(kc / (2.0 * M_PI) * LY);
+  int lmax = (int)floor(kc / (2.0 * M_PI) * LZ);
+  int m = kmax > lmax ? kmax : lmax;
+  m = hmax > m ? hmax : m;
+  int hklmax = 2 > m + 1 ? 2 : m + 1;
+  hkl[0] = hmax; hkl[1] = kmax; hkl[2] = lmax; hkl[3] = hklmax;
+  double kc2 = kc * kc;
+  size_t cap = (size_t)(hmax + 1) * (2 * kmax + 1) * (2 * lmax + 1) + 1;
+  int *v = (int *)malloc(cap * 4 * sizeof(int));
+  if (!v) return -1;
+  int nk = 0;
+  for (int h = 0; h <= hmax; h++) {
+    double kx = 2.0 * M_PI * h * p.Lr[0];
+    for (int k = (h == 0 ? 0 : -kmax); k <= kmax; k++) {
+      double ky = 2.0 * M_PI * k * p.Lr[1];
+      for (int l = (h == 0 && k == 0 ? 1 : -lmax); l <= lmax; l++) {
+        double kz = 2.0 * M_PI * l * p.Lr[2];
+        if (kx * kx + ky * ky + kz * kz < kc2) {
+          v[4 * nk] = h; v[4 * nk + 1] = k; v[4 * nk + 2] = l; v[4 * nk + 3] = 0;
+          nk++;
+        }
+      }
+    }
+  }
+  *out = v;
+  return nk;
+}
+
+typedef struct recip_job {
+  const orc_system *sys;
+  const orc_ewald *ew;
+  const orc_pbc *pbc;
+  const int *kv;
+  const double *sincos;  /* per atom 2 * hklmax entries of (x, y, z) */
+  int hklmax, from, to;
+  double *f;             /* 3n, this job's forces */
+  double energy, vir[6];
+} recip_job;
+
+/* NonbondedFullEwaldSystemForce::reciprocalTerm k loop (NonbondedFullEwaldSystemForce.h:947-1051)
+ * over k-vectors [from, to): the (h, k) products are cached while h and k
+ * repeat, as the reference's myLastSinCos is. */
+static void *recip_thread(void *arg) {
+  recip_job *J = (recip_job *)arg;
+  const int n = J->sys->n, hm = J->hklmax;
+  const double *SC = J->sincos;
+  const orc_pbc *p = J->pbc;
+  double *last = (double *)malloc(2 * (size_t)n * sizeof(double));
+  double *sca = (double *)malloc(2 * (size_t)n * sizeof(double));
+  double alpha2r = 1.0 / (J->ew->alpha * J->ew->alpha);
+  double fac = 1.0 / (4.0 * J->ew->alpha * J->ew->alpha);
+  double V = p->L[0] * p->L[1] * p->L[2];
+  double Vr = 1.0 / V;
+  int lastH = INT_MAX, lastK = INT_MAX;
+  J->energy = 0.0;
+  memset(J->vir, 0, sizeof J->vir);
+  for (int q = J->from; q < J->to; q++) {
+    int ih = J->kv[4 * q], ik = J->kv[4 * q + 1], il = J->kv[4 * q + 2];
+    double k0 = 2.0 * M_PI * ih * p->Lr[0], k1 = 2.0 * M_PI * ik * p->Lr[1], k2 = 2.0 * M_PI * il * p->Lr[2];
+    double ksq = k0 * k0 + k1 * k1 + k2 * k2;
+    int ika = abs(ik), iks = ik < 0 ? -1 : 1, ila = abs(il), ils = il < 0 ? -1 : 1;
+    if (ih != lastH || ik != lastK) {
+      for (int i = 0; i < n; i++) {
+        const double *t = SC + (size_t)i * hm * 2 * 3;
+        double xsin = t[(ih * 2) * 3 + 0], xcos = t[(ih * 2 + 1) * 3 + 0];
+        double ysin = iks * t[(ika * 2) * 3 + 1], ycos = t[(ika * 2 + 1) * 3 + 1];
+        last[2 * i] = xsin * ycos + xcos * ysin;
+        last[2 * i + 1] = xcos * ycos - xsin * ysin;
+      }
+      lastH = ih;
+      lastK = ik;
+    }
+    double sumSin = 0.0, sumCos = 0.0;
+    for (int i = 0; i < n; i++) {
+      const double *t = SC + (size_t)i * hm * 2 * 3;
+      double xysin = last[2 * i], xycos = last[2 * i + 1];
+      double zsin = ils * t[(ila * 2) * 3 + 2], zcos = t[(ila * 2 + 1) * 3 + 2];
+      double sinA = xysin * zcos + xycos * zsin;
+      double cosA = xycos * zcos - xysin * zsin;
+      sca[2 * i] = sinA;
+      sca[2 * i + 1] = cosA;
+      sumSin += sinA;
+      sumCos += cosA;
+    }
+    double b = 1.0 / ksq * exp(-ksq * alpha2r / 4.0);
+    double e = b * (sumSin * sumSin + sumCos * sumCos);
+    J->energy += e;
+    if (J->ew->virial) {
+      double c = 2.0 * (1.0 / ksq + fac);
+      J->vir[0] += e * (1.0 - c * k0 * k0);
+      J->vir[1] -= e * c * k0 * k1;
+      J->vir[2] -= e * c * k0 * k2;
+      J->vir[3] += e * (1.0 - c * k1 * k1);
+      J->vir[4] -= e * c * k1 * k2;
+      J->vir[5] += e * (1.0 - c * k2 * k2);
+    }
+    double a = 8.0 * M_PI * Vr * b;
+    for (int i = 0; i < n; i++) {
+      double s = a * (sca[2 * i] * sumCos - sca[2 * i + 1] * sumSin);
+      J->f[3 * i] += k0 * s;
+      J->f[3 * i + 1] += k1 * s;
+      J->f[3 * i + 2] += k2 * s;
+    }
+  }
+  free(last);
+  free(sca);
+  return NULL;
+}
+
+/* NonbondedFullEwaldSystemForce<PBC,Cubic,false,true,false,Cutoff>::evaluate:
+ * the reciprocal-space term alone (NonbondedFullEwaldSystemForce.h:873-1083).
+ * Adds forces into f, c * energy into e[COULOMB] and the virial when
+ * ew->virial.  nthreads > 1 splits the k-vector range as parallelEvaluate
+ * splits it across ranks (:473-481). */
+int orc_ewald_reciprocal(const orc_system *sys, const orc_ewald *ew, const double *x, double *f, double *e,
+                         int nthreads) {
+  if (!sys->periodic) return -2;
+  if (nthreads < 1) nthreads = 1;
+  orc_pbc pbc;
+  pbc_set(&pbc, sys->box, sys->origin);
+  int *kv = NULL, hkl[4];
+  int nk = orc_ewald_kvectors(sys->box, ew->alpha, ew->accuracy, &kv, hkl);
+  if (nk < 0) return -1;
+  const int n = sys->n, hm = hkl[3];
+  double *SC = (double *)malloc((size_t)n * hm * 2 * 3 * sizeof(double));
+  if (!SC) { free(kv); return -1; }
+  /* sin/cos tables by the addition theorem (:900-945); x carries the charge */
+  for (int j = 0; j < n; j++) {
+    double r[3];
+    pbc_min_position(&pbc, sys->origin, x + 3 * j, r);
+    double qi = sys->charge[j];
+    double ax = r[0] * 2.0 * M_PI * pbc.Lr[0], ay = r[1] * 2.0 * M_PI * pbc.Lr[1], az = r[2] * 2.0 * M_PI * pbc.Lr[2];
+    double xs = sin(ax), ys = sin(ay), zs = sin(az), xc = cos(ax), yc = cos(ay), zc = cos(az);
+    double *t = SC + (size_t)j * hm * 2 * 3;
+    t[0] = 0.0; t[1] = 0.0; t[2] = 0.0;
+    t[3] = qi * 1.0; t[4] = 1.0; t[5] = 1.0;
+    t[6] = qi * xs; t[7] = ys; t[8] = zs;
+    t[9] = qi * xc; t[10] = yc; t[11] = zc;
+    for (int i = 4; i < 2 * hm; i += 2) {
+      double *o = t + i * 3, *p1 = t + (i - 1) * 3, *p2 = t + (i - 2) * 3;
+      o[0] = xs * p1[0] + xc * p2[0];
+      o[1] = ys * p1[1] + yc * p2[1];
+      o[2] = zs * p1[2] + zc * p2[2];
+      o[3] = xc * p1[0] - xs * p2[0];
+      o[4] = yc * p1[1] - ys * p2[1];
+      o[5] = zc * p1[2] - zs * p2[2];
+    }
+  }
+  recip_job *jobs = (recip_job *)calloc((size_t)nthreads, sizeof(recip_job));
+  pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nthreads);
+  double *fb = (double *)calloc((size_t)nthreads * 3 * (size_t)n, sizeof(double));
+  if (!jobs || !th || !fb) { free(jobs); free(th); free(fb); free(SC); free(kv); return -1; }
+  for (int t = 0; t < nthreads; t++) {
+    jobs[t].sys = sys; jobs[t].ew = ew; jobs[t].pbc = &pbc; jobs[t].kv = kv; jobs[t].sincos = SC;
+    jobs[t].hklmax = hm;
+    jobs[t].from = (int)((long long)nk * t / nthreads);
+    jobs[t].to = (int)((long long)nk * (t + 1) / nthreads);
+    jobs[t].f = fb + (size_t)t * 3 * n;
+  }
+  if (nthreads == 1) recip_thread(&jobs[0]);
+  else {
+    for (int t = 0; t < nthreads; t++) pthread_create(&th[t], NULL, recip_thread, &jobs[t]);
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+  }
+  double c = 4.0 * M_PI * (1.0 / (pbc.L[0] * pbc.L[1] * pbc.L[2]));
+  double recip = 0.0;
+  for (int t = 0; t < nthreads; t++) {
+    recip += c * jobs[t].energy;
+    for (size_t k = 0; k < 3 * (size_t)n; k++) f[k] += jobs[t].f[k];
+    if (ew->virial) {
+      const double *v = jobs[t].vir;
+      e[ORC_VIRIALXX] += c * v[0]; e[ORC_VIRIALXY] += c * v[1]; e[ORC_VIRIALXZ] += c * v[2];
+      e[ORC_VIRIALYX] += c * v[1]; e[ORC_VIRIALYY] += c * v[3]; e[ORC_VIRIALYZ] += c * v[4];
+      e[ORC_VIRIALZX] += c * v[2]; e[ORC_VIRIALZY] += c * v[4]; e[ORC_VIRIALZZ] += c * v[5];
+    }
+  }
+  e[ORC_COULOMB] += recip;
+  free(jobs); free(th); free(fb); free(SC); free(kv);
+  return 0;
 }
 
 /* ------------------------------------------------------------------- GB */

@@ -140,6 +140,14 @@ int orc_ewald_init(orc_ewald *ew, double alpha, double accuracy, double cellsize
 int orc_ewald_real(const orc_system *sys, const orc_ewald *ew, const double *x,
                    double *f, double *e, orc_census *census, int nthreads);
 
+/* The reciprocal-space k-vectors (NonbondedFullEwaldSystemForce.h:604-667):
+ * count, *out = malloc'd (h, k, l, 0) per vector, hkl = {hmax, kmax, lmax, hklmax}. */
+int orc_ewald_kvectors(const double box[3], double alpha, double accuracy, int **out, int hkl[4]);
+/* NonbondedFullEwaldSystemForce<PBC,Cubic,false,true,false,Cutoff>: the
+ * reciprocal-space term alone (:873-1083); adds into f and e. */
+int orc_ewald_reciprocal(const orc_system *sys, const orc_ewald *ew, const double *x,
+                         double *f, double *e, int nthreads);
+
 /* MathUtilities.cpp:12-49 */
 double orc_poly5(double ar);
 double orc_myerf(double x);

@@ -237,19 +237,27 @@ static int do_cutoff(Recs &r, PTopo &topo, Vector3DBlock &pos, Vector3DBlock &f,
   return -1;
 }
 
+template <bool REAL, bool RECIP, bool CORR>
+static void run_ewald(PTopo &topo, Vector3DBlock &pos, Vector3DBlock &f, ScalarStructure &e, double alpha,
+                      double acc) {
+  NonbondedFullEwaldSystemForce<PBC, CCM, REAL, RECIP, CORR, CutoffSwitchingFunction> force(alpha, acc, 0.0);
+  force.preProcess(&topo, &pos);
+  force.evaluate(&topo, &pos, &f, &e);
+  force.postProcess(&topo, &e, &f);
+}
+
 static int do_ewald(Recs &r, PTopo &topo, Vector3DBlock &pos, Vector3DBlock &f, ScalarStructure &e) {
   double alpha = getd(r, "ew_alpha", 0.32), acc = getd(r, "ew_accuracy", 1e-5);
   int corr = geti(r, "ew_correction", 1);
-  if (corr) {
-    NonbondedFullEwaldSystemForce<PBC, CCM, true, false, true, CutoffSwitchingFunction> force(alpha, acc, 0.0);
-    force.preProcess(&topo, &pos);
-    force.evaluate(&topo, &pos, &f, &e);
-    force.postProcess(&topo, &e, &f);
-  } else {
-    NonbondedFullEwaldSystemForce<PBC, CCM, true, false, false, CutoffSwitchingFunction> force(alpha, acc, 0.0);
-    force.preProcess(&topo, &pos);
-    force.evaluate(&topo, &pos, &f, &e);
-    force.postProcess(&topo, &e, &f);
+  // terms: 1 real, 2 reciprocal, 4 correction
+  int parts = geti(r, "ew_parts", corr ? 5 : 1);
+  switch (parts) {
+    case 1: run_ewald<true, false, false>(topo, pos, f, e, alpha, acc); break;
+    case 5: run_ewald<true, false, true>(topo, pos, f, e, alpha, acc); break;
+    case 2: run_ewald<false, true, false>(topo, pos, f, e, alpha, acc); break;
+    case 3: run_ewald<true, true, false>(topo, pos, f, e, alpha, acc); break;
+    case 7: run_ewald<true, true, true>(topo, pos, f, e, alpha, acc); break;
+    default: return -1;
   }
   topo.uncacheCellList();
   // the real-term pair set: r2 <= Rc^2 and not FULL-excluded, the same test
@@ -343,6 +351,9 @@ int main(int argc, char **argv) {
   ScalarStructure e;
   e.clear();
   e.virial(geti(r, "virial", 0) != 0);
+  // the molecular virial (on by default in ScalarStructure) is not part of
+  // the path under test
+  e.molecularVirial(geti(r, "molvirial", 0) != 0);
   std::vector<double> born;
   int rc = -1;
   if (mode == 3) rc = do_gb(r, pos, f, e, born);

@@ -17,7 +17,7 @@ PM_OK, PM_EINVAL, PM_ENOMEM, PM_EDEVICE, PM_ECAPACITY, PM_ENOTREADY, PM_EFULL = 
 SWITCH_NONE, SWITCH_CUTOFF, SWITCH_C1, SWITCH_C2, SWITCH_CN, SWITCH_CMPCNCN = range(6)
 TERM_NONE, TERM_LJ, TERM_COULOMB, TERM_EWALD_REAL = range(4)
 LJ_TABLE, LJ_PERATOM = 0, 1
-FLAG_VIRIAL, FLAG_GB, FLAG_BONDED, FLAG_TIMING = 1, 2, 4, 8
+FLAG_VIRIAL, FLAG_GB, FLAG_BONDED, FLAG_TIMING, FLAG_EWALD_RECIPROCAL = 1, 2, 4, 8, 16
 NSLOTS = 36
 
 EXPORTED = ("pm_abi_version", "pm_device_info", "pm_ctx_create", "pm_ctx_destroy", "pm_ctx_reserve",
@@ -334,7 +334,8 @@ class Context:
         return self._split(ii, x), self._split(ii, v), self._split(ii, f), e
 
     def stats(self):
-        s = np.zeros(8)
-        check(lib().pm_stats(self.h, ptr(s), 8), self.h)
+        s = np.zeros(10)
+        check(lib().pm_stats(self.h, ptr(s), 10), self.h)
         return {"rebuilds": s[0], "regrow_reruns": s[1], "list_entries": s[2], "device_ms": s[3], "pair_ms": s[4],
-                "pair_launches": s[5], "max_displacement": s[6], "skin": s[7]}
+                "pair_launches": s[5], "max_displacement": s[6], "skin": s[7], "reciprocal_ms": s[8],
+                "kvectors": s[9]}

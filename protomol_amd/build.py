@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libprotomol_amd.so")
-SOURCES = ["pm_kernels.hip", "pm_gb.hip", "pm_api.cpp"]
+SOURCES = ["pm_kernels.hip", "pm_gb.hip", "pm_ewald.hip", "pm_api.cpp"]
 ARCH = os.environ.get("PM_OFFLOAD_ARCH", "gfx950")
 
 # packed-FP32 instructions are kept out of the generated code on purpose

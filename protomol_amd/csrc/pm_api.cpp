@@ -127,12 +127,20 @@ struct SysPrep {
   int cl_cap = 0, slot_cap = 0, sc_cap = 0, list_cap = 0, n_gb_tiles = 0;
   double ax = 0, ay = 0;
   double coul_const = 0, vir_const = 0;
+  // Ewald reciprocal space
+  std::vector<int4> ew_rows, ew_tiles;
+  std::vector<double4> ew_kd;
+  std::vector<short2> ew_pts;
+  std::vector<int> ew_kmap, ew_fb_rows;
+  int ew_nk = 0, ew_xrows = 0, ew_yrows = 0, ew_zrows = 0, ew_chunk = 0, ew_nchunks = 0, ew_stride = 0;
+  double ew_c = 0, ew_a8 = 0, ew_fac = 0;
 };
 
 struct SysHost {
   int id = -1;
   int n = 0, ncol = 0, cl_cap = 0, slot_cap = 0, sc_cap = 0, list_cap = 0, n_corr = 0, n_bonded = 0, n_local = 0;
   int n_gb_tiles = 0;
+  pm::EwLaunch ew{};        // reciprocal-space launch shape (sys filled per launch set)
   double coul_const = 0.0;  // Ewald point-self + charged-system energy (FP64, host)
   double vir_const = 0.0;   // charged-system virial diagonal
   SysDev dev{};             // host copy of the device record
@@ -243,6 +251,7 @@ int estimate_list_cap(double rho, double rlist, double ax, double ay, int cl_cap
 struct LaunchSet {
   std::vector<int> ids;
   Launch L;
+  std::vector<pm::EwLaunch> ew;  // L.ew points here
   void *mem = nullptr;
   long long *d_scal = nullptr;
   long long *h_scal = nullptr;  // pinned
@@ -263,13 +272,14 @@ struct pm_ctx {
   pm_force_desc desc{};
   ForceDev P{};
   bool gb = false;
+  bool recip = false;       // PM_FLAG_EWALD_RECIPROCAL
   double rc = 0.0, skin_md = 0.0, skin_eval = 0.0;
   std::vector<std::unique_ptr<SysHost>> sys;  // by id; null once removed
   Arena arena;
   std::map<std::vector<int>, std::unique_ptr<LaunchSet>> sets;
   std::string err;
-  double stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  hipEvent_t ev0 = nullptr, ev1 = nullptr, evp0 = nullptr, evp1 = nullptr;
+  double stats[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, evp0 = nullptr, evp1 = nullptr, evr0 = nullptr, evr1 = nullptr;
   std::vector<hipEvent_t> evs;  // PM_FLAG_TIMING: start/stop per pair launch of an MD call
   int evs_used = 0;
 };
@@ -331,6 +341,104 @@ std::vector<int> check_ids(pm_ctx *c, int32_t n, const int32_t *ids) {
 }
 
 // ------------------------------------------------------------ system set-up
+
+// Reciprocal-space tables of NonbondedFullEwaldSystemForce::initialize
+// (NonbondedFullEwaldSystemForce.h:604-667): the k-vectors of the h >= 0 half
+// space with |k| < kc in the reference's (h, k, l) order and with its FP64
+// expressions, grouped into rows of l at fixed (h, k); the structure-factor
+// tiles (h blocks of 8 x 256 (k, l) points) and the force pass's row blocks.
+void ewald_setup(const pm_ctx *c, const pm_system_desc &d, SysPrep &p) {
+  constexpr int HB = 8, TA = 16, SFT = 256;
+  const double LX = d.box[0], LY = d.box[1], LZ = d.box[2];
+  double a1 = LY * LZ - 0.0 * 0.0, d1 = LX * a1 + 0.0 * 0.0 + 0.0 * 0.0;
+  double a2 = LZ * LX - 0.0 * 0.0, d2 = 0.0 * 0.0 + LY * a2 + 0.0 * 0.0;
+  double a3 = LX * LY - 0.0 * 0.0, d3 = 0.0 * 0.0 + 0.0 * 0.0 + LZ * a3;
+  const double Lr[3] = {a1 / d1, a2 / d2, a3 / d3};
+  const double alpha = c->desc.ewald_alpha;
+  const double pp = -std::log(c->desc.ewald_accuracy);
+  const double kc = 2.0 * alpha * std::sqrt(pp);
+  const int hmax = (int)std::floor(kc / (2.0 * M_PI) * LX);
+  const int kmax = (int)std::floor(kc / (2.0 * M_PI) * LY);
+  const int lmax = (int)std::floor(kc / (2.0 * M_PI) * LZ);
+  p.ew_xrows = hmax + 1;
+  p.ew_yrows = std::max(kmax, 1) + 1;
+  p.ew_zrows = std::max(lmax, 1) + 1;
+  p.ew_stride = p.ew_yrows + p.ew_zrows + HB;
+  if ((size_t)TA * p.ew_stride * sizeof(double2) > 65536)
+    throw Fail{PM_EINVAL, "reciprocal space: kmax + lmax = " + std::to_string(kmax + lmax) +
+                              " exceeds what the structure-factor pass stages in LDS (246)"};
+  const double kc2 = kc * kc;
+  const double ar = 1.0 / (alpha * alpha);
+  const int KW = 2 * kmax + 1, LW = 2 * lmax + 1;
+  std::vector<int> kidx((size_t)(hmax + 1) * KW * LW, -1);
+  p.ew_rows.clear();
+  p.ew_kd.clear();
+  for (int h = 0; h <= hmax; h++) {
+    const double kx = 2.0 * M_PI * h * Lr[0];
+    for (int k = (h == 0 ? 0 : -kmax); k <= kmax; k++) {
+      const double ky = 2.0 * M_PI * k * Lr[1];
+      bool row = false;
+      for (int l = (h == 0 && k == 0 ? 1 : -lmax); l <= lmax; l++) {
+        const double kz = 2.0 * M_PI * l * Lr[2];
+        if (kx * kx + ky * ky + kz * kz < kc2) {
+          const double ksq = kx * kx + ky * ky + kz * kz;
+          const double b = 1.0 / ksq * std::exp(-ksq * ar / 4.0);
+          const int q = (int)p.ew_kd.size();
+          if (!row) p.ew_rows.push_back(make_int4(h, k, l, q));
+          row = true;
+          kidx[((size_t)h * KW + (k + kmax)) * LW + (l + lmax)] = q;
+          p.ew_kd.push_back(make_double4(kx, ky, kz, b));
+        }
+      }
+    }
+  }
+  p.ew_nk = (int)p.ew_kd.size();
+  const int nrows = (int)p.ew_rows.size();
+  p.ew_rows.push_back(make_int4(0, 0, 0, p.ew_nk));
+  // structure-factor points and tiles per h block
+  p.ew_pts.clear();
+  p.ew_kmap.clear();
+  p.ew_tiles.clear();
+  for (int h0 = 0; h0 <= hmax; h0 += HB) {
+    const int nh = std::min(HB, hmax + 1 - h0);
+    const int first = (int)p.ew_pts.size();
+    for (int k = -kmax; k <= kmax; k++)
+      for (int l = -lmax; l <= lmax; l++) {
+        int m[HB];
+        bool any = false;
+        for (int t = 0; t < HB; ++t) {
+          m[t] = t < nh ? kidx[((size_t)(h0 + t) * KW + (k + kmax)) * LW + (l + lmax)] : -1;
+          any |= m[t] >= 0;
+        }
+        if (!any) continue;
+        p.ew_pts.push_back(make_short2((short)k, (short)l));
+        p.ew_kmap.insert(p.ew_kmap.end(), m, m + HB);
+      }
+    const int end = (int)p.ew_pts.size();
+    for (int t0 = first; t0 < end; t0 += SFT) p.ew_tiles.push_back(make_int4(h0, t0, end, nh));
+  }
+  // atom chunks: enough workgroups to fill the device
+  const int ntiles = (int)p.ew_tiles.size();
+  const int n = d.n_atoms;
+  int nch = std::max(1, std::min((2048 + ntiles - 1) / std::max(ntiles, 1), (n + TA - 1) / TA));
+  p.ew_chunk = ((n + nch - 1) / nch + TA - 1) / TA * TA;
+  p.ew_nchunks = (n + p.ew_chunk - 1) / p.ew_chunk;
+  // force pass: row blocks of about equal k count
+  const int waves = (n + 63) / 64;
+  const int nfb = std::max(1, std::min(nrows, (4096 + waves - 1) / waves));
+  p.ew_fb_rows.assign(1, 0);
+  long long accum = 0;
+  for (int r = 0; r < nrows; ++r) {
+    accum += p.ew_rows[r + 1].w - p.ew_rows[r].w;
+    const int b = (int)p.ew_fb_rows.size();
+    if (b < nfb && r + 1 < nrows && accum * nfb >= (long long)p.ew_nk * b) p.ew_fb_rows.push_back(r + 1);
+  }
+  p.ew_fb_rows.push_back(nrows);
+  const double V = LX * LY * LZ;
+  p.ew_c = 4.0 * M_PI * (1.0 / V);
+  p.ew_a8 = 8.0 * M_PI * (1.0 / V);
+  p.ew_fac = 1.0 / (4.0 * alpha * alpha);
+}
 
 // Validate a system description and build its host-side tables
 // (exclusions, Ewald correction pairs, per-atom bonded/correction items,
@@ -497,6 +605,7 @@ SysPrep prepare(const pm_ctx *c, const pm_system_desc &d) {
   p.slot_cap = p.cl_cap * pm::kCluster;
   p.sc_cap = p.cl_cap / pm::kScCl + p.ncol + 1;
   p.list_cap = estimate_list_cap(rho, rlist, p.ax, p.ay, p.cl_cap);
+  if (c->recip) ewald_setup(c, d, p);
   return p;
 }
 
@@ -580,6 +689,20 @@ size_t carve(const pm_ctx *c, const SysPrep &p, const pm_system_desc &d, Carver 
   S.sc_col = cv.take<int>(p.sc_cap);
   S.n_sc = cv.take<int>(1);
   S.list_len = cv.take<int>(p.sc_cap);
+  if (c->recip) {
+    S.ew_tx = cv.take<double2>((size_t)p.ew_xrows * n);
+    S.ew_ty = cv.take<double2>((size_t)p.ew_yrows * n);
+    S.ew_tz = cv.take<double2>((size_t)p.ew_zrows * n);
+    S.ew_rows = cv.take<int4>(p.ew_rows.size());
+    S.ew_kd = cv.take<double4>(p.ew_kd.size());
+    S.ew_tiles = cv.take<int4>(p.ew_tiles.size());
+    S.ew_pts = cv.take<short2>(p.ew_pts.size());
+    S.ew_kmap = cv.take<int>(p.ew_kmap.size());
+    S.ew_fb_rows = cv.take<int>(p.ew_fb_rows.size());
+    S.ew_psum = cv.take<double2>((size_t)p.ew_nchunks * p.ew_nk);
+    S.ew_sk = cv.take<double4>(p.ew_kd.size());
+    S.ew_pf = cv.take<double>((size_t)(p.ew_fb_rows.size() - 1) * 3 * n);
+  }
   return cv.off;
 }
 
@@ -674,6 +797,28 @@ int add_system(pm_ctx *c, const pm_system_desc &d) {
   S.sc_cap = h.sc_cap = p.sc_cap;
   S.slot_cap = h.slot_cap = p.slot_cap;
   h.list_cap = p.list_cap;
+  if (c->recip) {
+    S.ew_nk = p.ew_nk;
+    S.ew_nrows = (int)p.ew_rows.size() - 1;
+    S.ew_xrows = p.ew_xrows;
+    S.ew_yrows = p.ew_yrows;
+    S.ew_zrows = p.ew_zrows;
+    S.ew_chunk = p.ew_chunk;
+    S.ew_nchunks = p.ew_nchunks;
+    S.ew_ntiles = (int)p.ew_tiles.size();
+    S.ew_lds_stride = p.ew_stride;
+    S.ew_nfb = (int)p.ew_fb_rows.size() - 1;
+    for (int k = 0; k < 3; ++k) S.ew_org[k] = d.origin[k];
+    S.ew_c = p.ew_c;
+    S.ew_a8 = p.ew_a8;
+    S.ew_fac = p.ew_fac;
+    h.ew.n = n;
+    h.ew.nk = p.ew_nk;
+    h.ew.ntiles = S.ew_ntiles;
+    h.ew.nchunks = S.ew_nchunks;
+    h.ew.nfb = S.ew_nfb;
+    h.ew.lds_bytes = 16 * p.ew_stride * (int)sizeof(double2);
+  }
 
   Carver dry;
   SysDev tmp{};
@@ -716,6 +861,14 @@ int add_system(pm_ctx *c, const pm_system_desc &d) {
       S.ncx = S.ncy = S.ncol = 1;
     } else {
       alloc_lists(c, h);
+    }
+    if (c->recip) {
+      upload(const_cast<int4 *>(S.ew_rows), p.ew_rows.data(), p.ew_rows.size());
+      upload(const_cast<double4 *>(S.ew_kd), p.ew_kd.data(), p.ew_kd.size());
+      upload(const_cast<int4 *>(S.ew_tiles), p.ew_tiles.data(), p.ew_tiles.size());
+      upload(const_cast<short2 *>(S.ew_pts), p.ew_pts.data(), p.ew_pts.size());
+      upload(const_cast<int *>(S.ew_kmap), p.ew_kmap.data(), p.ew_kmap.size());
+      upload(const_cast<int *>(S.ew_fb_rows), p.ew_fb_rows.data(), p.ew_fb_rows.size());
     }
   } catch (...) {
     dev_free(c, h.lists, h.lists_bytes);
@@ -801,6 +954,15 @@ LaunchSet &get_set(pm_ctx *c, const std::vector<int> &ids) {
   L.n_gb_tiles_total = pre[6].back();
   L.n_local_total = pre[7].back();
   L.n_sc_total = pre[8].back();
+  if (c->recip) {
+    for (int k = 0; k < ns; ++k) {
+      pm::EwLaunch e = c->sys[ids[k]]->ew;
+      e.sys = d_tab + k;
+      ls->ew.push_back(e);
+    }
+    L.ew = ls->ew.data();
+    L.n_ew = (int)ls->ew.size();
+  }
   LaunchSet &ref = *ls;
   c->sets[ids] = std::move(ls);
   return ref;
@@ -866,6 +1028,11 @@ void force_eval(pm_ctx *c, const Launch &L, int mode, const StepPolicy &pol, int
   if (!pm::launch_pair(L, c->P, kind)) throw Fail{PM_EINVAL, "unsupported term combination"};
   if (pool) HIPCHECK(hipEventRecord(next_event(c), c->stream));
   if (timed) HIPCHECK(hipEventRecord(c->evp1, c->stream));
+  if (c->recip && kind != pm::kPairCensus) {
+    if (timed) HIPCHECK(hipEventRecord(c->evr0, c->stream));
+    pm::launch_ewald(L, pm::kEwForces | (kind == pm::kPairForcesEnergies ? pm::kEwEnergies : 0));
+    if (timed) HIPCHECK(hipEventRecord(c->evr1, c->stream));
+  }
   pm::launch_local(L, c->P, local);
 }
 
@@ -874,6 +1041,7 @@ void force_eval(pm_ctx *c, const Launch &L, int mode, const StepPolicy &pol, int
 void energy_pass(pm_ctx *c, const Launch &L) {
   if (c->gb) return;  // GB sums energies inside its force passes
   if (!pm::launch_pair(L, c->P, pm::kPairEnergies)) throw Fail{PM_EINVAL, "unsupported term combination"};
+  if (c->recip) pm::launch_ewald(L, pm::kEwEnergies);  // from the structure factors of the force pass
   pm::launch_local(L, c->P, pm::kLocalEnergies);
 }
 
@@ -1002,6 +1170,14 @@ void evaluate_ids(pm_ctx *c, const std::vector<int> &ids, const double *x, doubl
     c->stats[3] = ms;
     c->stats[4] = pms;
     c->stats[5] = 1;
+    c->stats[8] = 0.0;
+    c->stats[9] = 0.0;
+    if (c->recip && !census) {
+      float rms = 0.f;
+      HIPCHECK(hipEventElapsedTime(&rms, c->evr0, c->evr1));
+      c->stats[8] = rms;
+      for (int id : ids) c->stats[9] += c->sys[id]->ew.nk;
+    }
     return;
   }
 }
@@ -1218,6 +1394,8 @@ int pm_ctx_create(const pm_force_desc *force, int32_t n_systems, const pm_system
     HIPCHECK(hipEventCreate(&c->ev1));
     HIPCHECK(hipEventCreate(&c->evp0));
     HIPCHECK(hipEventCreate(&c->evp1));
+    HIPCHECK(hipEventCreate(&c->evr0));
+    HIPCHECK(hipEventCreate(&c->evr1));
     ForceDev &P = c->P;
     std::memset(&P, 0, sizeof P);
     P.flags = force->flags;
@@ -1229,6 +1407,9 @@ int pm_ctx_create(const pm_force_desc *force, int32_t n_systems, const pm_system
     if (force->cell_scale != 0.0 && !(force->cell_scale >= 0.6 && force->cell_scale <= 2.0))
       throw Fail{PM_EINVAL, "cell_scale must be 0 (default) or within 0.6 .. 2"};
     c->gb = (force->flags & PM_FLAG_GB) != 0;
+    c->recip = (force->flags & PM_FLAG_EWALD_RECIPROCAL) != 0;
+    if (c->recip && (c->gb || force->term_b != PM_TERM_EWALD_REAL))
+      throw Fail{PM_EINVAL, "PM_FLAG_EWALD_RECIPROCAL needs term_b = PM_TERM_EWALD_REAL (and no GB)"};
     if (c->gb) {
       pm::gb_setup_force(P, *force);
     } else {
@@ -1305,7 +1486,7 @@ int pm_ctx_destroy(pm_ctx *c) {
     }
   c->sys.clear();
   c->arena.destroy();
-  for (hipEvent_t e : {c->ev0, c->ev1, c->evp0, c->evp1})
+  for (hipEvent_t e : {c->ev0, c->ev1, c->evp0, c->evp1, c->evr0, c->evr1})
     if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : c->evs) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1454,7 +1635,7 @@ int pm_md_get_state_systems(pm_ctx *c, int32_t n, const int32_t *ids, double *x,
 
 int pm_stats(pm_ctx *c, double *stats, int32_t n) {
   if (!c || !stats) return PM_EINVAL;
-  for (int k = 0; k < n && k < 8; ++k) stats[k] = c->stats[k];
+  for (int k = 0; k < n && k < 10; ++k) stats[k] = c->stats[k];
   return PM_OK;
 }
 

@@ -6,8 +6,12 @@
 //   original atom order  x, v, xref (FP64 SoA), charge, inverse mass, slot
 //   cluster slot order   xq (float4: cluster-local x, y, z and scaled charge),
 //                        LJ type, original index, fixed-point force (3 x int64)
-//   clusters of 8 slots  FP64 origin, FP32 bounding box, pair list of
-//                        16-byte entries (j cluster, image shift, FP32 offset)
+//   clusters of 8 slots  FP64 origin, FP32 bounding box
+//   super-clusters       (8 z-consecutive clusters of a column) pair list of
+//                        32-byte entries (j cluster, image shift, FP32 offset,
+//                        i-cluster mask, exclusion-mask slot)
+//   Ewald k space        per-atom e^{i n 2 pi x / L} tables [n][atom], the
+//                        k-vector rows, structure-factor partial sums
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -156,6 +160,26 @@ struct SysDev {
   unsigned *maxdisp2;      // float bits of the largest squared displacement from the last rebuild (this step)
   unsigned *maxdisp_run;   // the same, folded over every step of the current run
   int *ctl;                // kCtlN control words
+  // Ewald reciprocal space (NonbondedFullEwaldSystemForce::reciprocalTerm):
+  // k = 2 pi (h/Lx, k/Ly, l/Lz), h >= 0 half space, |k| < kc, in the
+  // reference's (h, k, l) order; rows = runs of l at fixed (h, k)
+  int ew_nk, ew_nrows;
+  int ew_xrows, ew_yrows, ew_zrows;  // rows of the x, y, z tables (hmax+1, max(kmax,1)+1, max(lmax,1)+1)
+  int ew_chunk, ew_nchunks;          // structure factor: atoms per chunk, chunks
+  int ew_ntiles, ew_lds_stride;      // structure factor: (h block, 256 points) tiles; LDS row stride
+  int ew_nfb;                        // force pass: row blocks
+  double ew_org[3];                  // box origin (minimalPosition)
+  double ew_c, ew_a8, ew_fac;        // 4 pi / V, 8 pi / V, 1 / (4 alpha^2)
+  double2 *ew_tx, *ew_ty, *ew_tz;    // (cos, sin)(n 2 pi r / L) [n][atom]; x carries the charge
+  const int4 *ew_rows;               // h, k, first l, first k index (+ sentinel)
+  const double4 *ew_kd;              // per k: kx, ky, kz, b = exp(-k^2 / (4 alpha^2)) / k^2
+  const int4 *ew_tiles;              // first h, first point, end point of the h block, h count
+  const short2 *ew_pts;              // (k, l) points of each h block
+  const int *ew_kmap;                // per point and h of its block: k index or -1
+  const int *ew_fb_rows;             // first row of each force block (+ end)
+  double2 *ew_psum;                  // [chunk][k] partial sums of (cos, sin)
+  double4 *ew_sk;                    // per k: (8 pi / V) b sum cos, (8 pi / V) b sum sin, kz, b |S|^2
+  double *ew_pf;                     // [force block][3][atom] partial forces
 };
 
 }  // namespace pm

@@ -15,7 +15,16 @@ struct Launch {
             *corr_prefix = nullptr, *bonded_prefix = nullptr, *gb_prefix = nullptr, *sc_prefix = nullptr;
   int n_atoms_total = 0, n_cl_total = 0, n_cols_total = 0, n_slots_total = 0, n_corr_total = 0,
       n_bonded_total = 0, n_gb_tiles_total = 0, n_local_total = 0, n_sc_total = 0;
+  const struct EwLaunch *ew = nullptr;  // per system with a reciprocal-space term (host array)
+  int n_ew = 0;
 };
+
+// Host-side launch shape of one system's reciprocal-space passes
+struct EwLaunch {
+  const SysDev *sys = nullptr;  // the system's record in the device table
+  int n = 0, nk = 0, ntiles = 0, nchunks = 0, nfb = 0, lds_bytes = 0;
+};
+enum { kEwForces = 1, kEwEnergies = 2 };
 
 void launch_import(const Launch &L, int what);
 void launch_export(const Launch &L, int what);
@@ -29,5 +38,9 @@ void launch_md_arm(const Launch &L);
 void launch_half_kick(const Launch &L, double hh);
 void launch_zero_forces(const Launch &L);
 void launch_step_begin(const Launch &L, int mode, float lim2, int kmax);
+// reciprocal-space Ewald of the systems in L.ew: kEwForces (tables, structure
+// factors, forces) and/or kEwEnergies (energy and virial from the last
+// structure factors)
+void launch_ewald(const Launch &L, int mode);
 
 }  // namespace pm

@@ -213,6 +213,26 @@ def water_box_density(n_mol: int, seed: int, density: float = 0.0334) -> System:
     return water_box(n_mol, box, seed)
 
 
+def rock_salt(n_cells: int, spacing: float = 2.82, origin=(0.0, 0.0, 0.0), jitter: float = 0.0,
+              seed: int = 0) -> System:
+    """NaCl lattice of n_cells^3 conventional cells (8 ions each) with
+    nearest-neighbour distance `spacing`: charges +-1 e (scaled), LJ off, each
+    ion its own molecule.  Unjittered, its Coulomb energy is the Madelung
+    energy -N M q^2 / (2 spacing), M = 1.747564594633..."""
+    m = 2 * n_cells
+    idx = np.stack(np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij"), -1).reshape(-1, 3)
+    sign = np.where(idx.sum(1) % 2 == 0, 1.0, -1.0)
+    box = np.full(3, m * spacing)
+    org = np.asarray(origin, np.float64)
+    x = (idx + 0.5) * spacing - box / 2 + org
+    if jitter:
+        x = x + np.random.default_rng(seed).uniform(-jitter, jitter, x.shape)
+    n = len(x)
+    return System(x=x, charge=sign * SQRT_COULOMB, mass=np.where(sign > 0, 22.98977, 35.453),
+                  molecule=np.arange(n, dtype=np.int32), box=box, origin=org, periodic=True,
+                  type=np.zeros(n, np.int32), lj_table=np.zeros((1, 1, 4)), name=f"nacl{n}")
+
+
 def ensemble_sizes(count: int, lo: int, hi: int, seed: int) -> list[int]:
     """Molecule counts drawn log-uniform in [lo, hi] by seed."""
     rng = np.random.default_rng(seed)

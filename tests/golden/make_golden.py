@@ -74,6 +74,22 @@ def make_ewald(sysobj):
                         census=np.array(out["census"], dtype=np.uint64), alpha=0.32, accuracy=acc)
 
 
+def make_ewald_reciprocal(sysobj):
+    """NonbondedFullEwaldSystemForce with the reciprocal term: the full sum
+    (real + reciprocal + correction, ProtoMol's "CoulombEwald") and the
+    reciprocal term alone, both with the virial."""
+    acc = math.exp(-10.24)
+    out = {}
+    for parts, tag in ((7, "full"), (2, "recip")):
+        r = refio.system_records(sysobj)
+        r.update(mode=np.int32(2), ew_alpha=0.32, ew_accuracy=acc, ew_parts=np.int32(parts), cellsize=5.0,
+                 virial=np.int32(1))
+        o = refio.run(r)
+        out[f"f_{tag}"] = o["f"].reshape(-1, 3)
+        out[f"e_{tag}"] = o["e"]
+    np.savez_compressed(os.path.join(HERE, "ref_ewald_recip.npz"), x=sysobj.x, alpha=0.32, accuracy=acc, **out)
+
+
 def make_bonded(sysobj):
     r = refio.system_records(sysobj)
     r.update(mode=np.int32(4), bonds=sysobj.bonds.reshape(-1), bond_k=sysobj.bond_k, bond_r0=sysobj.bond_r0,
@@ -185,9 +201,13 @@ def make_water423():
 
 def main():
     w = water_fixture_system()
+    if sys.argv[1:] == ["ewald_recip"]:
+        make_ewald_reciprocal(w)
+        return
     for name, case in CUTOFF_CASES.items():
         make_cutoff(name, w, case)
     make_ewald(w)
+    make_ewald_reciprocal(w)
     make_bonded(w)
     make_gb(300, seed=7)
     make_water423()

@@ -426,3 +426,122 @@ def test_pool_admission_by_footprint():
         e2 = np.zeros(L.NSLOTS)
         ctx.evaluate(ic, c.x, f2, e2)
         assert np.array_equal(f1[0], f2) and np.array_equal(e1[0], e2)
+
+
+# ------------------------------------------------- Ewald reciprocal space
+
+RECIP = L.FLAG_EWALD_RECIPROCAL
+NOTERM = (0, 0, 0, 0, 0)
+
+
+def _random_ions(n, box, seed, min_dist=2.0):
+    """Neutral set of +-1 e ions, uniform in an orthogonal box, no pair
+    closer than min_dist (minimal image); each ion its own molecule."""
+    rng = np.random.default_rng(seed)
+    box = np.asarray(box, float)
+    pts = []
+    while len(pts) < n:
+        p = rng.uniform(-box / 2, box / 2)
+        if pts:
+            d = np.asarray(pts) - p
+            d -= box * np.rint(d / box)
+            if np.min((d * d).sum(1)) < min_dist ** 2:
+                continue
+        pts.append(p)
+    q = np.where(np.arange(n) % 2 == 0, 1.0, -1.0) * S.SQRT_COULOMB
+    return S.System(x=np.asarray(pts), charge=q, mass=np.full(n, 20.0), molecule=np.arange(n, dtype=np.int32),
+                    box=box, origin=np.zeros(3), periodic=True, type=np.zeros(n, np.int32),
+                    lj_table=np.zeros((1, 1, 4)), name=f"ions{n}")
+
+
+def test_ewald_full_vs_reference_golden():
+    """Real + reciprocal + correction ("CoulombEwald") against the
+    reference's own NonbondedFullEwaldSystemForce<PBC, Cubic, true, true, true>."""
+    print("case ewald full", flush=True)
+    g = T.golden("ref_ewald_recip.npz")
+    w = T.fixture_water()
+    desc = force_desc(L.TERM_NONE, NOTERM, 0, NOTERM, ewald=True, flags=L.FLAG_VIRIAL | RECIP)
+    with L.Context(desc, [w]) as ctx:
+        f, e = evaluate(ctx, 0, w.x)
+        st = ctx.stats()
+        cen = ctx.census(0, w.x)
+    kv, _ = O.ewald_kvectors(w.box, T.EWALD_ALPHA, T.EWALD_ACCURACY)
+    assert st["kvectors"] == len(kv)
+    assert T.rms_rel(f, g["f_full"]) <= RTOL_F
+    check_energies(e, g["e_full"], slots=(0,), npairs=cen[0])
+    check_virial(e, g["e_full"], npairs=cen[0])
+
+
+def _recip_by_difference(system, alpha=T.EWALD_ALPHA, acc=T.EWALD_ACCURACY):
+    """The device's reciprocal term: (real + reciprocal) minus real alone.
+    The real part is the same bits in both runs and the fixed-point force and
+    energy sums are integers, so the difference is the reciprocal term at
+    2^-32 (forces) and 2^-20 (energy, virial) resolution."""
+    out = []
+    for flags in (L.FLAG_VIRIAL | RECIP, L.FLAG_VIRIAL):
+        d = force_desc(L.TERM_NONE, NOTERM, 0, NOTERM, ewald=True, flags=flags, skin=1.0)
+        d.ewald_alpha, d.ewald_accuracy = alpha, acc
+        with L.Context(d, [system]) as ctx:
+            out.append(evaluate(ctx, 0, system.x))
+    return out[0][0] - out[1][0], out[0][1] - out[1][1]
+
+
+@pytest.mark.parametrize("case", ["water", "nacl", "ions_box"])
+def test_ewald_reciprocal_fp64_vs_oracle(case):
+    """The reciprocal term is FP64 end to end: forces, energy and virial
+    match the oracle (itself bit-exact against the reference's classes) to
+    the fixed-point resolution."""
+    print(f"case ewald reciprocal/{case}", flush=True)
+    alpha, acc = T.EWALD_ALPHA, T.EWALD_ACCURACY
+    if case == "water":
+        s = T.fixture_water()
+    elif case == "nacl":
+        s = S.rock_salt(5, 2.82, jitter=0.3, seed=4)
+        alpha, acc = 0.35, 1e-8
+    else:
+        s = _random_ions(900, (24.5, 26.0, 29.5), seed=6)
+    f, e = _recip_by_difference(s, alpha, acc)
+    fo, eo = O.ewald_reciprocal(s, alpha, acc, virial=True)
+    fmax = np.max(np.abs(fo))
+    assert np.max(np.abs(f - fo)) <= 1e-9 * fmax + 2.0 ** -30, np.max(np.abs(f - fo))
+    assert abs(e[0] - eo[0]) <= 1e-10 * abs(eo[0]) + 4e-6, (e[0], eo[0])
+    assert np.max(np.abs(e[7:16] - eo[7:16])) <= 1e-10 * np.max(np.abs(eo[7:16])) + 4e-6
+    # Newton: the reciprocal forces sum to zero
+    assert np.max(np.abs(f.sum(0))) <= 1e-9 * fmax + s.n * 2.0 ** -31
+
+
+def test_ewald_full_madelung_on_device():
+    """Rock salt: the device's real + reciprocal + self energy is the
+    Madelung energy (the real term's A&S erfc and FP32 pair arithmetic bound
+    the error)."""
+    print("case ewald madelung", flush=True)
+    s = S.rock_salt(5, 2.82)
+    d = force_desc(L.TERM_NONE, NOTERM, 0, NOTERM, ewald=True, flags=RECIP, skin=1.0)
+    d.ewald_alpha, d.ewald_accuracy = 0.35, 1e-8
+    with L.Context(d, [s]) as ctx:
+        f, e = evaluate(ctx, 0, s.x)
+    exact = -s.n * 1.747564594633182 * S.SQRT_COULOMB ** 2 / (2 * 2.82)
+    assert abs(e[0] - exact) <= 1e-5 * abs(exact), (e[0], exact)
+    assert np.max(np.abs(f)) < 1e-3
+
+
+def test_md_full_ewald_graph_and_batch_bitwise():
+    """MD with the reciprocal term: graph replay equals direct launches, and
+    each system of a batch ends bitwise where it ends alone."""
+    print("case md full ewald", flush=True)
+    systems = _water_set((520, 700), 40)
+    v0s = [S.maxwell_velocities(s, 300.0, seed=50 + k) for k, s in enumerate(systems)]
+    ta, sa, tb, sb = MODELS["c2c2"]
+    runs = []
+    for flags in (L.FLAG_BONDED | RECIP, L.FLAG_BONDED | RECIP | L.FLAG_TIMING):
+        desc = force_desc(ta, sa, tb, sb, flags=flags, skin=0.8, interval=0, ewald=True)
+        runs.append(_md(desc, systems, v0s, 23, 1.0))
+    (x1, v1, F1, E1, _), (x2, v2, F2, E2, _) = runs
+    assert np.array_equal(x1, x2) and np.array_equal(v1, v2) and np.array_equal(F1, F2) and np.array_equal(E1, E2)
+    desc = force_desc(ta, sa, tb, sb, flags=L.FLAG_BONDED | RECIP, skin=0.8, interval=0, ewald=True)
+    off = 0
+    for k, s in enumerate(systems):
+        x, v, F, E, _ = _md(desc, [s], [v0s[k]], 23, 1.0)
+        assert np.array_equal(x1[off:off + s.n], x) and np.array_equal(v1[off:off + s.n], v)
+        assert np.array_equal(F1[off:off + s.n], F) and np.array_equal(E1[k], E[0])
+        off += s.n

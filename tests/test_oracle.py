@@ -269,3 +269,113 @@ def test_newton_third_law_and_virial_identity():
     # unwrapped-consistent images only for small displacements; check symmetry instead
     V = e[O.VIRIALXX:O.VIRIALXX + 9].reshape(3, 3)
     assert np.allclose(V, V.T, atol=1e-9)
+
+
+# ---------------------------------------------------------- Ewald reciprocal
+
+MADELUNG_NACL = 1.747564594633182
+
+
+def test_ewald_reciprocal_matches_reference():
+    # NonbondedFullEwaldSystemForce with the reciprocal term, run by the
+    # reference's own classes: the reciprocal term alone and the full sum
+    g = T.golden("ref_ewald_recip.npz")
+    w = T.fixture_water()
+    assert np.array_equal(w.x, g["x"])
+    alpha, acc = float(g["alpha"]), float(g["accuracy"])
+    f, e = O.ewald_reciprocal(w, alpha, acc, virial=True)
+    # same tables, same k order, same sums: bit-exact
+    assert np.array_equal(f, g["f_recip"])
+    assert np.array_equal(e, g["e_recip"])
+    fr, er, _, _ = O.ewald_real(w, alpha, acc, cellsize=5.0, virial=True)
+    scale = np.max(np.abs(g["f_full"]))
+    assert np.max(np.abs(fr + f - g["f_full"])) < 1e-12 * scale
+    assert np.allclose(er + e, g["e_full"], rtol=1e-13, atol=1e-10)
+    # the k range split across threads (parallelEvaluate) sums the same terms
+    f4, e4 = O.ewald_reciprocal(w, alpha, acc, virial=True, nthreads=4)
+    assert np.max(np.abs(f4 - f)) < 1e-12 * scale
+    assert abs(e4[0] - e[0]) < 1e-11 * abs(e[0])
+
+
+def test_ewald_kvectors_half_space_and_cutoff():
+    box = np.array([22.5, 24.0, 27.0])
+    alpha, acc = 0.32, math.exp(-10.24)
+    kv, (hm, km, lm, hkl) = O.ewald_kvectors(box, alpha, acc)
+    kc = 2 * alpha * math.sqrt(10.24)
+    k = 2 * math.pi * kv / box
+    assert np.all((k ** 2).sum(1) < kc * kc)
+    # brute force over the full box of indices: exactly one of +-k is present
+    g = np.stack(np.meshgrid(np.arange(-hm, hm + 1), np.arange(-km, km + 1), np.arange(-lm, lm + 1),
+                             indexing="ij"), -1).reshape(-1, 3)
+    kk = ((2 * math.pi * g / box) ** 2).sum(1)
+    inside = g[(kk < kc * kc) & np.any(g != 0, axis=1)]
+    assert 2 * len(kv) == len(inside)
+    have = {tuple(v) for v in kv}
+    for v in inside:
+        assert (tuple(v) in have) != (tuple(-v) in have)
+    assert hkl == max(2, max(hm, km, lm) + 1)
+
+
+def _nacl_real_exact(s, alpha, rc):
+    """Real-space Ewald energy with the exact erfc, every image within rc."""
+    x, q, L = s.x, s.charge, s.box
+    e = 0.0
+    rng = range(-2, 3)
+    for a in range(s.n):
+        d = x - x[a]
+        d -= L * np.rint(d / L)
+        for i in rng:
+            for j in rng:
+                for k in rng:
+                    r = np.linalg.norm(d + L * np.array([i, j, k]), axis=1)
+                    m = (r < rc) & (r > 1e-12)
+                    e += 0.5 * q[a] * np.sum(q[m] * np.array([math.erfc(alpha * v) / v for v in r[m]]))
+    return e
+
+
+@pytest.mark.parametrize("alpha", [0.5, 0.8])
+def test_ewald_reciprocal_madelung(alpha):
+    # rock salt: real + reciprocal + self is the Madelung energy whatever
+    # alpha splits it; with the exact erfc in real space the reciprocal
+    # term is pinned to the k-space truncation error (exp(-p) = 1e-12)
+    s = S.rock_salt(2, 2.82)
+    acc = 1e-12
+    rc = math.sqrt(-math.log(acc)) / alpha
+    f, e = O.ewald_reciprocal(s, alpha, acc)
+    exact = -s.n * MADELUNG_NACL * S.SQRT_COULOMB ** 2 / (2 * 2.82)
+    self_e = -np.sum(s.charge ** 2) * alpha / math.sqrt(math.pi)
+    recip_expected = exact - _nacl_real_exact(s, alpha, rc) - self_e
+    assert abs(e[O.COULOMB] - recip_expected) < 1e-9 * abs(exact)
+    # a perfect lattice feels no force
+    assert np.max(np.abs(f)) < 1e-9
+    # the reference's real term (A&S erfc) closes the same sum to its 1.5e-7 bound
+    fr, er, _, _ = O.ewald_real(s, alpha, acc, cellsize=5.0, correction=True)
+    assert abs(er[O.COULOMB] + e[O.COULOMB] - exact) < 2e-6 * abs(exact)
+
+
+def test_ewald_reciprocal_fd_newton_virial():
+    s = S.rock_salt(1, 2.82, jitter=0.25, seed=3)
+    alpha, acc = 0.6, 1e-10
+    f, e = O.ewald_reciprocal(s, alpha, acc, virial=True)
+    # Newton: sum_i F_i = sum_k k a Im(S conj S) = 0
+    assert np.max(np.abs(f.sum(0))) < 1e-10 * np.max(np.abs(f))
+    h = 1e-5
+    for a in (0, 3, 6):
+        for k in range(3):
+            xp, xm = s.x.copy(), s.x.copy()
+            xp[a, k] += h
+            xm[a, k] -= h
+            ep = O.ewald_reciprocal(s, alpha, acc, x=xp)[1][0]
+            em = O.ewald_reciprocal(s, alpha, acc, x=xm)[1][0]
+            assert abs(-(ep - em) / (2 * h) - f[a, k]) < 1e-6 * max(1.0, np.max(np.abs(f)))
+    # virial diagonal = -dE/d(strain): scale the box and positions along one axis
+    import dataclasses
+    for k in range(3):
+        es = []
+        for lam in (1 + h, 1 - h):
+            sc = np.ones(3)
+            sc[k] = lam
+            s2 = dataclasses.replace(s, x=(s.x - s.origin) * sc + s.origin, box=s.box * sc)
+            es.append(O.ewald_reciprocal(s2, alpha, acc)[1][0])
+        w = -(es[0] - es[1]) / (2 * h)
+        assert abs(w - e[O.VIRIALXX + 4 * k]) < 1e-6 * max(1.0, abs(e[O.COULOMB]))
