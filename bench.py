@@ -21,6 +21,10 @@ Beside it, in the same line:
                 continuous batching with admission by HBM footprint, against
                 the same requests served one at a time; p50/p99 latency
   gb            32 GB/OBC bead chains of 5,000 atoms, all pairs, batched
+  ewald         the headline box with full Ewald (real + reciprocal +
+                correction, ProtoMol's CoulombEwald): MD step time, the
+                reciprocal-space passes' FP64 rate against peak, and the FP64
+                oracle's reciprocal term on this host's cores
 """
 from __future__ import annotations
 
@@ -50,6 +54,12 @@ DENSITY = 0.0334  # molecules / A^3
 # energies 2, virial 12 (add/sub/mul = 1, FMA = 2, each rsqrt/rcp/exp = 1)
 FLOP_PER_PAIR = 82
 PEAK_FP32_TFLOPS = 157.3
+# reciprocal-space work per (k-vector, atom) pair, counted on
+# NonbondedFullEwaldSystemForce::reciprocalTerm: structure factor 8 (sinA,
+# cosA from the (h, k) products and z phases 6, sums 2), force 10 (the
+# sin/cos cross term 3, times a 1, times k 3, accumulate 3)
+EW_FLOP_PER_KATOM = 18
+PEAK_FP64_TFLOPS = 78.6
 WORKERS = 16
 
 
@@ -338,6 +348,72 @@ def gb_batch(args):
             "pair_interactions_per_s": pairs * steps / t}
 
 
+def ewald_full(args):
+    """The 96,000-atom water box with the reciprocal-space sum added."""
+    system = getattr(args, "_system", None)
+    if system is None:
+        log("ewald: building the 96,000-atom water box")
+        system = S.water_box(N_MOL, BOX, seed=2024, min_contact=2.0)
+    v0 = S.maxwell_velocities(system, 300.0, seed=2025)
+    d = force_desc(args.skin, cell_scale=args.cell_scale)
+    d.flags |= L.FLAG_EWALD_RECIPROCAL
+    rec = []
+    with L.Context(d, [system]) as ctx:
+        f = np.zeros((system.n, 3))
+        e = np.zeros(L.NSLOTS)
+        for k in range(args.ewald_evals):
+            ctx.evaluate(0, system.x, f, e)
+            st = ctx.stats()
+            rec.append(st["reciprocal_ms"])
+            log(f"ewald: evaluation {k + 1}: reciprocal {st['reciprocal_ms']:.3f} ms, device {st['device_ms']:.3f} ms")
+        nk = int(st["kvectors"])
+        ctx.md_set_state(system.x, v0)
+        ctx.md_run(3, DT_FS, energies=False)
+        t0 = time.perf_counter()
+        ctx.md_run(args.ewald_steps, DT_FS, energies=False)
+        t = time.perf_counter() - t0
+    log(f"ewald: {args.ewald_steps} MD steps in {t:.2f} s")
+    rms = float(np.median(rec[1:] if len(rec) > 1 else rec))
+    work = float(nk) * system.n
+    achieved = work * EW_FLOP_PER_KATOM / (rms * 1e-3) / 1e12
+    out = {"workload": "water 32,000 mol / 96,000 atoms, box 98.6 A, LJ C2 8-10 A + full Ewald alpha 0.32 "
+                       "(Rc 10 A, kc 2.05 A^-1) + bond/angle, leapfrog dt 1 fs",
+           "kvectors": nk, "ms_per_step": 1e3 * t / args.ewald_steps, "reciprocal_ms": rms,
+           "katom_pairs_per_s": work / (rms * 1e-3),
+           "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / PEAK_FP64_TFLOPS,
+                        "kernel": "k_ew_tables + k_ew_sf + k_ew_ssum + k_ew_force + k_ew_fsum (FP64)",
+                        "flop_per_katom": EW_FLOP_PER_KATOM}}
+    if not args.no_cpu:
+        out["cpu_baseline"] = ewald_cpu_baseline(args.cpu_seconds)
+    return out
+
+
+def ewald_cpu_baseline(seconds):
+    """The oracle's reciprocal term (the reference's algorithm) on this host,
+    on a bounded sample: 8,000 waters (24,000 atoms, same alpha and kc)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    try:
+        import oracle as O
+    except Exception as e:
+        return {"value": None, "unit": "(k-vector, atom) pairs/s", "cores": 0, "kind": "port",
+                "sample": f"unavailable: {e}"}
+    cores = max(1, min(os.cpu_count() or 1, WORKERS))
+    w = S.water_box(8000, (8000 / DENSITY) ** (1.0 / 3.0), seed=11, min_contact=2.0)
+    nk = len(O.ewald_kvectors(w.box, ALPHA, ACCURACY)[0])
+    n_eval, t_tot = 0, 0.0
+    while t_tot < seconds or n_eval == 0:
+        t0 = time.perf_counter()
+        O.ewald_reciprocal(w, ALPHA, ACCURACY, nthreads=cores)
+        t_tot += time.perf_counter() - t0
+        n_eval += 1
+        log(f"ewald cpu baseline: evaluation {n_eval} done, {t_tot:.1f} s")
+    per = t_tot / n_eval
+    return {"value": nk * w.n / per, "unit": "(k-vector, atom) pairs/s", "cores": cores, "kind": "port",
+            "sample": f"{n_eval} reciprocal-term evaluation(s) of 8,000 waters ({w.n} atoms, {nk} k-vectors), "
+                      f"{per:.3f} s each on {cores} threads"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -353,9 +429,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true",
                     help="no per-launch HIP events (MD steps replay a captured graph; no roofline figure)")
-    ap.add_argument("--forms", default="batched,served,gb",
-                    help="comma list of extra forms beside the headline (batched, served, gb) or 'none'")
-    ap.add_argument("--only", default="", help="run one extra form alone: batched, served or gb")
+    ap.add_argument("--forms", default="batched,served,gb,ewald",
+                    help="comma list of extra forms beside the headline (batched, served, gb, ewald) or 'none'")
+    ap.add_argument("--only", default="", help="run one extra form alone: batched, served, gb or ewald")
     ap.add_argument("--batch-systems", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--serve-requests", type=int, default=400)
@@ -364,6 +440,8 @@ def main():
     ap.add_argument("--serve-reserve-gib", type=float, default=24.0)
     ap.add_argument("--gb-chains", type=int, default=32)
     ap.add_argument("--gb-steps", type=int, default=200)
+    ap.add_argument("--ewald-evals", type=int, default=4)
+    ap.add_argument("--ewald-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -371,12 +449,13 @@ def main():
         sys.exit(2)
     name, ncu, hbm = L.device_info()
     log(f"device: {name}, {ncu} CUs, {hbm:.0f} GiB")
-    forms = {"batched": batched, "served": served, "gb": gb_batch}
+    forms = {"batched": batched, "served": served, "gb": gb_batch, "ewald": ewald_full}
     if args.only:
         print(json.dumps({args.only: forms[args.only](args), "device": name}), flush=True)
         return
     out, system = headline(args)
     out["device"] = name
+    args._system = system
     for f in [f for f in args.forms.split(",") if f and f != "none"]:
         out[f] = forms[f](args)
     if not args.no_cpu:

@@ -132,7 +132,7 @@ struct SysPrep {
   std::vector<double4> ew_kd;
   std::vector<short2> ew_pts;
   std::vector<int> ew_kmap, ew_fb_rows;
-  int ew_nk = 0, ew_xrows = 0, ew_yrows = 0, ew_zrows = 0, ew_chunk = 0, ew_nchunks = 0, ew_stride = 0;
+  int ew_nk = 0, ew_xrows = 0, ew_yrows = 0, ew_zrows = 0, ew_chunk = 0, ew_nchunks = 0;
   double ew_c = 0, ew_a8 = 0, ew_fac = 0;
 };
 
@@ -346,9 +346,10 @@ std::vector<int> check_ids(pm_ctx *c, int32_t n, const int32_t *ids) {
 // (NonbondedFullEwaldSystemForce.h:604-667): the k-vectors of the h >= 0 half
 // space with |k| < kc in the reference's (h, k, l) order and with its FP64
 // expressions, grouped into rows of l at fixed (h, k); the structure-factor
-// tiles (h blocks of 8 x 256 (k, l) points) and the force pass's row blocks.
+// tiles (h blocks of 8 x kEwSFPoints (k, l) points) and the force pass's row
+// blocks.
 void ewald_setup(const pm_ctx *c, const pm_system_desc &d, SysPrep &p) {
-  constexpr int HB = 8, TA = 16, SFT = 256;
+  constexpr int HB = 8, SFT = pm::kEwSFPoints;
   const double LX = d.box[0], LY = d.box[1], LZ = d.box[2];
   double a1 = LY * LZ - 0.0 * 0.0, d1 = LX * a1 + 0.0 * 0.0 + 0.0 * 0.0;
   double a2 = LZ * LX - 0.0 * 0.0, d2 = 0.0 * 0.0 + LY * a2 + 0.0 * 0.0;
@@ -360,13 +361,9 @@ void ewald_setup(const pm_ctx *c, const pm_system_desc &d, SysPrep &p) {
   const int hmax = (int)std::floor(kc / (2.0 * M_PI) * LX);
   const int kmax = (int)std::floor(kc / (2.0 * M_PI) * LY);
   const int lmax = (int)std::floor(kc / (2.0 * M_PI) * LZ);
-  p.ew_xrows = hmax + 1;
+  p.ew_xrows = (hmax + 1 + HB - 1) / HB * HB;
   p.ew_yrows = std::max(kmax, 1) + 1;
   p.ew_zrows = std::max(lmax, 1) + 1;
-  p.ew_stride = p.ew_yrows + p.ew_zrows + HB;
-  if ((size_t)TA * p.ew_stride * sizeof(double2) > 65536)
-    throw Fail{PM_EINVAL, "reciprocal space: kmax + lmax = " + std::to_string(kmax + lmax) +
-                              " exceeds what the structure-factor pass stages in LDS (246)"};
   const double kc2 = kc * kc;
   const double ar = 1.0 / (alpha * alpha);
   const int KW = 2 * kmax + 1, LW = 2 * lmax + 1;
@@ -417,15 +414,17 @@ void ewald_setup(const pm_ctx *c, const pm_system_desc &d, SysPrep &p) {
     const int end = (int)p.ew_pts.size();
     for (int t0 = first; t0 < end; t0 += SFT) p.ew_tiles.push_back(make_int4(h0, t0, end, nh));
   }
-  // atom chunks: enough workgroups to fill the device
+  // atom chunks (multiples of 4 atoms): enough workgroups to fill the device
   const int ntiles = (int)p.ew_tiles.size();
   const int n = d.n_atoms;
-  int nch = std::max(1, std::min((2048 + ntiles - 1) / std::max(ntiles, 1), (n + TA - 1) / TA));
+  constexpr int TA = 4;
+  int nch = std::max(1, std::min((4096 + ntiles - 1) / std::max(ntiles, 1), (n + 63) / 64));
   p.ew_chunk = ((n + nch - 1) / nch + TA - 1) / TA * TA;
   p.ew_nchunks = (n + p.ew_chunk - 1) / p.ew_chunk;
-  // force pass: row blocks of about equal k count
-  const int waves = (n + 63) / 64;
-  const int nfb = std::max(1, std::min(nrows, (4096 + waves - 1) / waves));
+  // force pass: row blocks of about equal k count, enough waves to hide the
+  // latency of the per-k weights' scalar loads
+  const int waves = (n + 127) / 128;  // two atoms per thread
+  const int nfb = std::max(1, std::min(nrows, (12288 + waves - 1) / waves));
   p.ew_fb_rows.assign(1, 0);
   long long accum = 0;
   for (int r = 0; r < nrows; ++r) {
@@ -647,7 +646,7 @@ size_t carve(const pm_ctx *c, const SysPrep &p, const pm_system_desc &d, Carver 
   S.angle_k = cv.take<double>(p.n_angles);
   S.angle_t0 = cv.take<double>(p.n_angles);
   S.n_clusters = cv.take<int>(1);
-  S.acc = cv.take<long long>(pm::kAccN);
+  S.acc = cv.take<long long>((size_t)pm::kAccN * pm::kAccCopies);
   S.census = cv.take<unsigned long long>(2);
   S.err = cv.take<unsigned>(1);
   S.maxdisp2 = cv.take<unsigned>(1);
@@ -690,9 +689,7 @@ size_t carve(const pm_ctx *c, const SysPrep &p, const pm_system_desc &d, Carver 
   S.n_sc = cv.take<int>(1);
   S.list_len = cv.take<int>(p.sc_cap);
   if (c->recip) {
-    S.ew_tx = cv.take<double2>((size_t)p.ew_xrows * n);
-    S.ew_ty = cv.take<double2>((size_t)p.ew_yrows * n);
-    S.ew_tz = cv.take<double2>((size_t)p.ew_zrows * n);
+    S.ew_tab = cv.take<double2>((size_t)(p.ew_xrows + p.ew_yrows + p.ew_zrows) * n);
     S.ew_rows = cv.take<int4>(p.ew_rows.size());
     S.ew_kd = cv.take<double4>(p.ew_kd.size());
     S.ew_tiles = cv.take<int4>(p.ew_tiles.size());
@@ -806,7 +803,7 @@ int add_system(pm_ctx *c, const pm_system_desc &d) {
     S.ew_chunk = p.ew_chunk;
     S.ew_nchunks = p.ew_nchunks;
     S.ew_ntiles = (int)p.ew_tiles.size();
-    S.ew_lds_stride = p.ew_stride;
+    S.ew_rl = p.ew_xrows + p.ew_yrows + p.ew_zrows;
     S.ew_nfb = (int)p.ew_fb_rows.size() - 1;
     for (int k = 0; k < 3; ++k) S.ew_org[k] = d.origin[k];
     S.ew_c = p.ew_c;
@@ -817,7 +814,7 @@ int add_system(pm_ctx *c, const pm_system_desc &d) {
     h.ew.ntiles = S.ew_ntiles;
     h.ew.nchunks = S.ew_nchunks;
     h.ew.nfb = S.ew_nfb;
-    h.ew.lds_bytes = 16 * p.ew_stride * (int)sizeof(double2);
+    h.ew.lds_bytes = 0;
   }
 
   Carver dry;

@@ -10,7 +10,7 @@
 //   super-clusters       (8 z-consecutive clusters of a column) pair list of
 //                        32-byte entries (j cluster, image shift, FP32 offset,
 //                        i-cluster mask, exclusion-mask slot)
-//   Ewald k space        per-atom e^{i n 2 pi x / L} tables [n][atom], the
+//   Ewald k space        per-atom records of e^{i n 2 pi x / L}, the
 //                        k-vector rows, structure-factor partial sums
 #pragma once
 #include <hip/hip_runtime.h>
@@ -23,6 +23,17 @@ constexpr int kScCl = 8;           // clusters per super-cluster (z-consecutive 
 constexpr int kMaskSlots = 32;     // least exclusion-mask slots per super-cluster list
 constexpr int kPairSplitDefault = 12;  // waves sharing one super-cluster's list in the pair kernel
 constexpr int kWave = 64;
+// Ewald structure-factor tiling: waves per workgroup x 16-point column tiles
+// per wave (the workgroup's points = waves x tiles x 16)
+#ifndef PM_EW_SF_WAVES
+#define PM_EW_SF_WAVES 4
+#endif
+#ifndef PM_EW_SF_TILES
+#define PM_EW_SF_TILES 4
+#endif
+constexpr int kEwSFWaves = PM_EW_SF_WAVES;
+constexpr int kEwSFTiles = PM_EW_SF_TILES;
+constexpr int kEwSFPoints = kEwSFWaves * kEwSFTiles * 16;
 constexpr int kSortCap = 4096;     // max atoms per x-y column (LDS bitonic sort)
 constexpr double kFScale = 4294967296.0;      // 2^32 fixed point for forces
 constexpr double kFScaleInv = 1.0 / 4294967296.0;
@@ -33,6 +44,14 @@ constexpr int kShiftCenter = 13;   // (0,0,0) image in the 27-shift encoding
 // energy accumulator slots (int64 fixed point, per system)
 enum { kAccLJ = 0, kAccCoul = 1, kAccVxx, kAccVxy, kAccVxz, kAccVyy, kAccVyz, kAccVzz,
        kAccBond, kAccAngle, kAccN = 16 };
+// The accumulators exist in kAccCopies copies of one 128-byte line each; a
+// wave adds into copy (wave index mod kAccCopies), k_gather_scalars sums the
+// copies.  Integer sums are exact, so the totals do not depend on the split,
+// and thousands of waves no longer queue for the atomics of one cache line.
+constexpr int kAccCopies = 64;
+__host__ __device__ inline long long *acc_copy(long long *acc, int w) {
+  return acc + (size_t)(w & (kAccCopies - 1)) * kAccN;
+}
 
 // device-side status bits (per system)
 enum { kErrColumnCap = 1, kErrListCap = 2, kErrNonFinite = 4, kErrClusterCap = 8 };
@@ -154,7 +173,7 @@ struct SysDev {
   double *gb_born, *gb_drfac, *gb_partial, *gb_spare;
   // per-system scalars
   int *n_clusters, *n_sc;
-  long long *acc;          // kAccN fixed-point energy/virial accumulators
+  long long *acc;          // kAccCopies x kAccN fixed-point energy/virial accumulators
   unsigned long long *census; // count, hash
   unsigned *err;
   unsigned *maxdisp2;      // float bits of the largest squared displacement from the last rebuild (this step)
@@ -164,13 +183,15 @@ struct SysDev {
   // k = 2 pi (h/Lx, k/Ly, l/Lz), h >= 0 half space, |k| < kc, in the
   // reference's (h, k, l) order; rows = runs of l at fixed (h, k)
   int ew_nk, ew_nrows;
-  int ew_xrows, ew_yrows, ew_zrows;  // rows of the x, y, z tables (hmax+1, max(kmax,1)+1, max(lmax,1)+1)
+  int ew_xrows, ew_yrows, ew_zrows;  // rows of the x, y, z tables (hmax+1 up to a multiple of 8,
+                                     // max(kmax,1)+1, max(lmax,1)+1)
   int ew_chunk, ew_nchunks;          // structure factor: atoms per chunk, chunks
-  int ew_ntiles, ew_lds_stride;      // structure factor: (h block, 256 points) tiles; LDS row stride
+  int ew_ntiles, ew_rl;              // structure factor: (h block, kEwSFPoints points) tiles; record length
   int ew_nfb;                        // force pass: row blocks
   double ew_org[3];                  // box origin (minimalPosition)
   double ew_c, ew_a8, ew_fac;        // 4 pi / V, 8 pi / V, 1 / (4 alpha^2)
-  double2 *ew_tx, *ew_ty, *ew_tz;    // (cos, sin)(n 2 pi r / L) [n][atom]; x carries the charge
+  double2 *ew_tab;                   // per atom (cos, sin)(n 2 pi r / L), n = 0.. of x, y, z in turn
+                                     // (ew_rl entries); x carries the charge
   const int4 *ew_rows;               // h, k, first l, first k index (+ sentinel)
   const double4 *ew_kd;              // per k: kx, ky, kz, b = exp(-k^2 / (4 alpha^2)) / k^2
   const int4 *ew_tiles;              // first h, first point, end point of the h block, h count

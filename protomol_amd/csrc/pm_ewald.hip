@@ -8,20 +8,25 @@
 //   F_j  = sum_k k (8 pi / V) b_k (sin_j(k) Re S - cos_j(k) Im S),
 // with b_k = exp(-k^2 / (4 alpha^2)) / k^2.  e^{i k.r_j} factors into
 // per-dimension phases X_j(h) Y_j(k) Z_j(l), which the reference tabulates
-// per atom by the addition theorem (:900-945); so does k_ew_tables here.
+// per atom by the addition theorem (:900-945); so does k_ew_tables here,
+// into one record per atom: X(0..xrows) | Y(0..yrows) | Z(0..zrows).
 //
 // All in FP64: on this part FP64 vector FMA issues at the rate of FP32, and
 // without transcendentals inside the sums nothing is gained in FP32.
 //   k_ew_tables  one thread per atom: minimalPosition, sin/cos, recurrences
-//   k_ew_sf      S(k) partial sums: a lane owns one (k, l) point and 8 h
-//                values (one h block), atoms stream through LDS in tiles of
-//                16, so S needs no cross-lane reduction; one partial per atom
-//                chunk, summed in chunk order by k_ew_ssum (deterministic)
+//   k_ew_sf      S(k) partial sums as a complex matrix product on the FP64
+//                matrix cores: S(h, p) = sum_j X_j(h) W_j(p) over atoms j,
+//                with p = (k, l) points and W_j(p) = Y_j(k) Z_j(l) formed in
+//                registers; rows = (h, re/im) of an h block of 8, columns =
+//                16 points, 4 atoms per step, operands read straight from
+//                the atoms' records.  One partial per atom chunk, summed in
+//                chunk order by k_ew_ssum (deterministic)
 //   k_ew_ssum    S(k), the per-k force weights and b |S|^2
 //   k_ew_force   one thread per atom over a block of k rows: the phases of a
 //                row follow from the previous row's by recurrence (no table
-//                reads inside a sweep), the per-k weights are wave-uniform
-//                (scalar loads); partial forces per row block
+//                reads inside a sweep) and run along the row by the
+//                three-term cosine recurrence, the per-k weights are
+//                wave-uniform (scalar loads); partial forces per row block
 //   k_ew_fsum    forces summed over row blocks in order, into the 64-bit
 //                fixed-point force slots
 //   k_ew_energy  energy and virial, one workgroup, fixed summation order
@@ -45,9 +50,9 @@ __device__ __forceinline__ const EW_CONST T *ew_cp(const T *p) {
 }
 
 constexpr int kEwHB = 8;        // h values per structure-factor block
-constexpr int kEwTA = 16;       // atoms per LDS tile of the structure-factor pass
-constexpr int kEwSFThreads = 256;
 constexpr int kEwFThreads = 256;
+
+typedef double ew_d4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -55,9 +60,9 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 __device__ __forceinline__ double2 cmulc(double2 a, double2 b) {  // a * conj(b)
   return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
 }
-// table entry for a signed index: e^{-i n t} = conj(e^{i n t})
-__device__ __forceinline__ double2 ew_phase(const double2 *tab, int n_atoms, int idx, int j) {
-  const double2 v = tab[(size_t)(idx < 0 ? -idx : idx) * n_atoms + j];
+// record entry for a signed index: e^{-i n t} = conj(e^{i n t})
+__device__ __forceinline__ double2 ew_phase(const double2 *rec, int idx) {
+  const double2 v = rec[idx < 0 ? -idx : idx];
   return idx < 0 ? make_double2(v.x, -v.y) : v;
 }
 
@@ -80,88 +85,123 @@ __global__ __launch_bounds__(256) void k_ew_tables(const SysDev *__restrict__ sy
       for (int k = 0; k < 3; ++k) d[k] = d[k] - S.L[k] * rint(S.Lr[k] * d[k]);
   }
   const double q = S.charge[j];
-  double2 *tab[3] = {S.ew_tx, S.ew_ty, S.ew_tz};
+  double2 *rec = S.ew_tab + (size_t)j * S.ew_rl;
+  double2 *sec[3] = {rec, rec + S.ew_xrows, rec + S.ew_xrows + S.ew_yrows};
   const int rows[3] = {S.ew_xrows, S.ew_yrows, S.ew_zrows};
   for (int k = 0; k < 3; ++k) {
     const double t = d[k] * 2.0 * M_PI * S.Lr[k];
     double s1, c1;
     sincos(t, &s1, &c1);
     const double w = k == 0 ? q : 1.0;
-    double2 *T = tab[k] + j;
-    double ps = 0.0 * w, pc = w * 1.0;  // n = 0
-    T[0] = make_double2(pc, ps);
+    double2 *T = sec[k];
+    T[0] = make_double2(w * 1.0, 0.0);  // n = 0
     if (rows[k] < 2) continue;
     double cs = w * s1, cc = w * c1;    // n = 1
-    T[(size_t)n] = make_double2(cc, cs);
+    T[1] = make_double2(cc, cs);
     for (int m = 2; m < rows[k]; ++m) {
       const double ns = s1 * cc + c1 * cs;
       const double nc = c1 * cc - s1 * cs;
       cs = ns;
       cc = nc;
-      T[(size_t)m * n] = make_double2(cc, cs);
+      T[m] = make_double2(cc, cs);
     }
-    (void)ps;
-    (void)pc;
   }
 }
 
-// Structure-factor partial sums.  Block = (tile, chunk): the tile's 256
-// lanes own (k, l) points of one h block (up to 8 h values), the chunk is a
-// contiguous range of atoms.  Atom tiles of 16 are staged in LDS as
-// [atom][row] with rows Y(0..), Z(0..), X(h0..h0+7).
-__global__ __launch_bounds__(kEwSFThreads) void k_ew_sf(const SysDev *__restrict__ sys) {
-  extern __shared__ double2 ew_lds[];
+// Structure-factor partial sums.  Block = (tile, chunk): the tile's
+// kEwSFPoints (k, l) points of one h block, the chunk a contiguous range of
+// atoms.
+// v_mfma_f64_16x16x4f64 computes D += A B with A 16 x 4 (lane l: row
+// l % 16, column l / 16), B 4 x 16 (lane l: row l / 16, column l % 16) and
+// D 16 x 16 (lane l, register r: row 4 r + l / 16, column l % 16).  Here
+// A = x phases of 4 atoms, B = Re W or Im W of those atoms at 16 points, and
+// A row i carries (h, part) = ((i & 3) + 4 (i >> 3), (i >> 2) & 1), so lane
+// l (q = l / 16) ends with Dr, Di at (q, re), (q, im), (q + 4, re),
+// (q + 4, im) -- both parts of h = q and q + 4 -- and
+//   Re S(h, p) = Dr(h, re) - Di(h, im),  Im S(h, p) = Di(h, re) + Dr(h, im).
+// Atoms past the chunk's end enter with zero weight.
+// Workgroups are dealt to the 8 XCDs round robin; every tile of one atom
+// chunk is placed on the same XCD (block = ((chunk / 8) tiles + tile) 8 +
+// chunk % 8), so that chunk's phase records are fetched into one L2.
+__global__ __launch_bounds__(kEwSFWaves * 64) void k_ew_sf(const SysDev *__restrict__ sys) {
   const SysDev &S = *sys;
   const int n = S.n_atoms;
   const int nt = S.ew_ntiles;
-  const int tile = blockIdx.x % nt, chunk = blockIdx.x / nt;
-  const int4 T = S.ew_tiles[tile];
-  const int K1 = S.ew_yrows, L1 = S.ew_zrows, RS = S.ew_lds_stride;
-  const int pt = T.y + (int)threadIdx.x;
-  const bool live = pt < T.z;
-  const short2 kl = S.ew_pts[live ? pt : T.y];
-  const int ka = kl.x < 0 ? -kl.x : kl.x, la = kl.y < 0 ? -kl.y : kl.y;
-  const double sy = kl.x < 0 ? -1.0 : 1.0, sz = kl.y < 0 ? -1.0 : 1.0;
-  double ar[kEwHB], ai[kEwHB];
+  const int b = blockIdx.x;
+  const int chunk = (b / (8 * nt)) * 8 + (b & 7), tile = (b >> 3) % nt;
+  if (chunk >= S.ew_nchunks) return;
+  const int4 T = ew_cp(S.ew_tiles)[tile];   // scalar loads: the tile is the same for the workgroup
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int col = lane & 15, quad = lane >> 4;
+  const int RL = S.ew_rl, YO = S.ew_xrows, ZO = S.ew_xrows + S.ew_yrows;
+  int ky[kEwSFTiles], kz[kEwSFTiles], pidx[kEwSFTiles];
+  double sy[kEwSFTiles], sz[kEwSFTiles];
 #pragma unroll
-  for (int h = 0; h < kEwHB; ++h) ar[h] = ai[h] = 0.0;
+  for (int t = 0; t < kEwSFTiles; ++t) {
+    const int p = T.y + (wv * kEwSFTiles + t) * 16 + col;
+    pidx[t] = p < T.z ? p : -1;
+    const short2 kl = S.ew_pts[p < T.z ? p : T.y];
+    ky[t] = YO + (kl.x < 0 ? -kl.x : kl.x);
+    kz[t] = ZO + (kl.y < 0 ? -kl.y : kl.y);
+    sy[t] = kl.x < 0 ? -1.0 : 1.0;
+    sz[t] = kl.y < 0 ? -1.0 : 1.0;
+  }
+  const int ai = lane & 15;
+  const int xrow = T.x + (ai & 3) + 4 * (ai >> 3), xpart = (ai >> 2) & 1;
+  ew_d4 dr[kEwSFTiles], di[kEwSFTiles];
+#pragma unroll
+  for (int t = 0; t < kEwSFTiles; ++t) dr[t] = di[t] = (ew_d4){0.0, 0.0, 0.0, 0.0};
   const int a0 = chunk * S.ew_chunk;
   const int a1 = min(n, a0 + S.ew_chunk);
-  const int nrow = K1 + L1 + kEwHB;
-  for (int t0 = a0; t0 < a1; t0 += kEwTA) {
-    const int na = min(kEwTA, a1 - t0);
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < nrow * kEwTA; idx += kEwSFThreads) {
-      const int r = idx / kEwTA, a = idx - r * kEwTA;
-      double2 v = make_double2(0.0, 0.0);
-      if (a < na) {
-        if (r < K1) v = S.ew_ty[(size_t)r * n + t0 + a];
-        else if (r < K1 + L1) v = S.ew_tz[(size_t)(r - K1) * n + t0 + a];
-        else if (r - K1 - L1 < T.w) v = S.ew_tx[(size_t)(T.x + r - K1 - L1) * n + t0 + a];
-      }
-      ew_lds[a * RS + r] = v;
-    }
-    __syncthreads();
-    for (int a = 0; a < na; ++a) {
-      const double2 *A = ew_lds + a * RS;
-      const double2 Y = A[ka], Z = A[K1 + la];
-      const double yi = sy * Y.y, zi = sz * Z.y;
-      const double wr = Y.x * Z.x - yi * zi, wi = Y.x * zi + yi * Z.x;
+  // operands of the next atom quad are loaded while the current one's
+  // products run (register double buffer)
+  double2 cx, cy[kEwSFTiles], cz[kEwSFTiles];
+  {
+    const double2 *rec = S.ew_tab + (size_t)min(a0 + quad, a1 - 1) * RL;
+    cx = rec[xrow];
 #pragma unroll
-      for (int h = 0; h < kEwHB; ++h) {
-        const double2 X = A[K1 + L1 + h];
-        ar[h] += X.x * wr - X.y * wi;
-        ai[h] += X.x * wi + X.y * wr;
-      }
+    for (int t = 0; t < kEwSFTiles; ++t) {
+      cy[t] = rec[ky[t]];
+      cz[t] = rec[kz[t]];
     }
   }
-  if (!live) return;
+  for (int a = a0; a < a1; a += 4) {
+    double2 nx, ny[kEwSFTiles], nz[kEwSFTiles];
+    {
+      const double2 *rec = S.ew_tab + (size_t)min(a + 4 + quad, a1 - 1) * RL;
+      nx = rec[xrow];
+#pragma unroll
+      for (int t = 0; t < kEwSFTiles; ++t) {
+        ny[t] = rec[ky[t]];
+        nz[t] = rec[kz[t]];
+      }
+    }
+    const double av = a + quad < a1 ? (xpart ? cx.y : cx.x) : 0.0;
+#pragma unroll
+    for (int t = 0; t < kEwSFTiles; ++t) {
+      const double yi = sy[t] * cy[t].y, zi = sz[t] * cz[t].y;
+      const double wr = cy[t].x * cz[t].x - yi * zi, wi = cy[t].x * zi + yi * cz[t].x;
+      dr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, wr, dr[t], 0, 0, 0);
+      di[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, wi, di[t], 0, 0, 0);
+    }
+    cx = nx;
+#pragma unroll
+    for (int t = 0; t < kEwSFTiles; ++t) {
+      cy[t] = ny[t];
+      cz[t] = nz[t];
+    }
+  }
   double2 *out = S.ew_psum + (size_t)chunk * S.ew_nk;
 #pragma unroll
-  for (int h = 0; h < kEwHB; ++h) {
-    if (h >= T.w) break;
-    const int k = S.ew_kmap[(size_t)pt * kEwHB + h];
-    if (k >= 0) out[k] = make_double2(ar[h], ai[h]);
+  for (int t = 0; t < kEwSFTiles; ++t) {
+    if (pidx[t] < 0) continue;
+#pragma unroll
+    for (int hh = 0; hh < 2; ++hh) {
+      const int h = quad + 4 * hh;
+      if (h >= T.w) continue;
+      const int k = S.ew_kmap[(size_t)pidx[t] * kEwHB + h];
+      if (k >= 0) out[k] = make_double2(dr[t][2 * hh] - di[t][2 * hh + 1], di[t][2 * hh] + dr[t][2 * hh + 1]);
+    }
   }
 }
 
@@ -182,65 +222,118 @@ __global__ __launch_bounds__(256) void k_ew_ssum(const SysDev *__restrict__ sys)
   S.ew_sk[k] = make_double4(a * c, a * s, kd.z, e);
 }
 
-// Forces of atom j (one per thread) over the k rows of block blockIdx.y.
-// Within a block, rows of one h run over consecutive k and their first l
-// moves by a few steps: Y and Z(first l) follow by multiplying with the
-// atom's unit phases; anything else reloads from the tables.
+// Forces of two atoms per thread (j and j + kEwFThreads of the workgroup's
+// 2 kEwFThreads) over the k rows of block blockIdx.y: two independent chains
+// per wave-uniform weight load.  Within a block, rows of one h run over
+// consecutive k and their first l moves by a few steps: Y and Z(first l)
+// follow by multiplying with the atom's unit phases; anything else reloads
+// from the atom's record.
+constexpr int kEwFAtoms = 2;
 __global__ __launch_bounds__(kEwFThreads) void k_ew_force(const SysDev *__restrict__ sys) {
   const SysDev &S = *sys;
   const int n = S.n_atoms;
-  const int j0 = blockIdx.x * kEwFThreads + (int)threadIdx.x;
-  const bool live = j0 < n;
-  const int j = live ? j0 : n - 1;
   const int b = blockIdx.y;
   const EW_CONST int *fbr = ew_cp(S.ew_fb_rows);
   const EW_CONST int4 *rows = ew_cp(S.ew_rows);
   const EW_CONST double4 *sk = ew_cp(S.ew_sk);
   const EW_CONST double4 *kd = ew_cp(S.ew_kd);
   const int r0 = fbr[b], r1 = fbr[b + 1];
-  const double2 ey1 = S.ew_ty[(size_t)n + j], ez1 = S.ew_tz[(size_t)n + j];
-  double fx = 0.0, fy = 0.0, fz = 0.0;
+  int jj[kEwFAtoms];
+  const double2 *recx[kEwFAtoms], *recy[kEwFAtoms], *recz[kEwFAtoms];
+  double2 ey1[kEwFAtoms], ez1[kEwFAtoms], X[kEwFAtoms], Y[kEwFAtoms], Zl[kEwFAtoms];
+  double c2[kEwFAtoms], fx[kEwFAtoms], fy[kEwFAtoms], fz[kEwFAtoms];
+#pragma unroll
+  for (int u = 0; u < kEwFAtoms; ++u) {
+    jj[u] = blockIdx.x * (kEwFAtoms * kEwFThreads) + u * kEwFThreads + (int)threadIdx.x;
+    const int j = min(jj[u], n - 1);
+    recx[u] = S.ew_tab + (size_t)j * S.ew_rl;
+    recy[u] = recx[u] + S.ew_xrows;
+    recz[u] = recy[u] + S.ew_yrows;
+    ey1[u] = recy[u][1];
+    ez1[u] = recz[u][1];
+    c2[u] = 2.0 * ez1[u].x;
+    fx[u] = fy[u] = fz[u] = 0.0;
+    X[u] = Y[u] = Zl[u] = make_double2(0.0, 0.0);
+  }
   int ph = -1, pk = 0, plo = 0;
-  double2 X = make_double2(0.0, 0.0), Y = X, Zl = X;
   for (int r = r0; r < r1; ++r) {
     const int4 R = rows[r];
     const int kend = rows[r + 1].w;
-    if (R.x != ph) {
-      X = S.ew_tx[(size_t)R.x * n + j];
-      Y = ew_phase(S.ew_ty, n, R.y, j);
-      Zl = ew_phase(S.ew_tz, n, R.z, j);
-    } else {
-      if (R.y == pk + 1) Y = cmul(Y, ey1);
-      else Y = ew_phase(S.ew_ty, n, R.y, j);
-      const int dl = R.z - plo;
-      if (dl >= -4 && dl <= 4) {
-        for (int m = 0; m < dl; ++m) Zl = cmul(Zl, ez1);
-        for (int m = 0; m < -dl; ++m) Zl = cmulc(Zl, ez1);
+    const bool newh = R.x != ph, stepk = R.y == pk + 1;
+    const int dl = R.z - plo;
+#pragma unroll
+    for (int u = 0; u < kEwFAtoms; ++u) {
+      if (newh) {
+        X[u] = recx[u][R.x];
+        Y[u] = ew_phase(recy[u], R.y);
+        Zl[u] = ew_phase(recz[u], R.z);
       } else {
-        Zl = ew_phase(S.ew_tz, n, R.z, j);
+        Y[u] = stepk ? cmul(Y[u], ey1[u]) : ew_phase(recy[u], R.y);
+        if (dl >= -4 && dl <= 4) {
+          for (int m = 0; m < dl; ++m) Zl[u] = cmul(Zl[u], ez1[u]);
+          for (int m = 0; m < -dl; ++m) Zl[u] = cmulc(Zl[u], ez1[u]);
+        } else {
+          Zl[u] = ew_phase(recz[u], R.z);
+        }
       }
     }
     ph = R.x;
     pk = R.y;
     plo = R.z;
-    double2 e = cmul(cmul(X, Y), Zl);
-    double rs = 0.0;
-    for (int q = R.w; q < kend; ++q) {
+    // e(l) along the row by the three-term recurrence
+    // e(l + 1) = 2 cos(t_z) e(l) - e(l - 1), two FMAs per step
+    double2 e[kEwFAtoms], ep[kEwFAtoms];
+    double rs[kEwFAtoms];
+#pragma unroll
+    for (int u = 0; u < kEwFAtoms; ++u) {
+      e[u] = cmul(cmul(X[u], Y[u]), Zl[u]);
+      ep[u] = cmulc(e[u], ez1[u]);  // e(first l - 1)
+      rs[u] = 0.0;
+    }
+    int q = R.w;
+    for (; q + 3 < kend; q += 4) {  // four k per pass: one wait for their weights
+      double4 w[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) w[v] = sk[q + v];
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int u = 0; u < kEwFAtoms; ++u) {
+          const double t = e[u].y * w[v].x - e[u].x * w[v].y;
+          const double2 en = make_double2(c2[u] * e[u].x - ep[u].x, c2[u] * e[u].y - ep[u].y);
+          ep[u] = e[u];
+          e[u] = en;
+          fz[u] += w[v].z * t;
+          rs[u] += t;
+        }
+    }
+    for (; q < kend; ++q) {
       const double4 w = sk[q];
-      const double t = e.y * w.x - e.x * w.y;
-      fz += w.z * t;
-      rs += t;
-      e = cmul(e, ez1);
+#pragma unroll
+      for (int u = 0; u < kEwFAtoms; ++u) {
+        const double t = e[u].y * w.x - e[u].x * w.y;
+        const double2 en = make_double2(c2[u] * e[u].x - ep[u].x, c2[u] * e[u].y - ep[u].y);
+        ep[u] = e[u];
+        e[u] = en;
+        fz[u] += w.z * t;
+        rs[u] += t;
+      }
     }
     const double4 k0 = kd[R.w];
-    fx += k0.x * rs;
-    fy += k0.y * rs;
+#pragma unroll
+    for (int u = 0; u < kEwFAtoms; ++u) {
+      fx[u] += k0.x * rs[u];
+      fy[u] += k0.y * rs[u];
+    }
   }
-  if (!live) return;
   double *pf = S.ew_pf + (size_t)b * 3 * n;
-  pf[j] = fx;
-  pf[(size_t)n + j] = fy;
-  pf[2 * (size_t)n + j] = fz;
+#pragma unroll
+  for (int u = 0; u < kEwFAtoms; ++u) {
+    if (jj[u] >= n) continue;
+    pf[jj[u]] = fx[u];
+    pf[(size_t)n + jj[u]] = fy[u];
+    pf[2 * (size_t)n + jj[u]] = fz[u];
+  }
 }
 
 // sum of the row blocks' partial forces, in block order, into the fixed-point
@@ -302,10 +395,11 @@ void launch_ewald(const Launch &L, int mode) {
     if (!E.n || !E.nk) continue;
     if (mode & kEwForces) {
       hipLaunchKernelGGL(k_ew_tables, dim3((E.n + 255) / 256), dim3(256), 0, L.stream, E.sys);
-      hipLaunchKernelGGL(k_ew_sf, dim3(E.ntiles * E.nchunks), dim3(kEwSFThreads), E.lds_bytes, L.stream, E.sys);
+      hipLaunchKernelGGL(k_ew_sf, dim3(E.ntiles * ((E.nchunks + 7) / 8) * 8), dim3(kEwSFWaves * 64), 0, L.stream,
+                         E.sys);
       hipLaunchKernelGGL(k_ew_ssum, dim3((E.nk + 255) / 256), dim3(256), 0, L.stream, E.sys);
-      hipLaunchKernelGGL(k_ew_force, dim3((E.n + kEwFThreads - 1) / kEwFThreads, E.nfb), dim3(kEwFThreads), 0,
-                         L.stream, E.sys);
+      hipLaunchKernelGGL(k_ew_force, dim3((E.n + kEwFAtoms * kEwFThreads - 1) / (kEwFAtoms * kEwFThreads), E.nfb),
+                         dim3(kEwFThreads), 0, L.stream, E.sys);
       hipLaunchKernelGGL(k_ew_fsum, dim3((E.n + 255) / 256), dim3(256), 0, L.stream, E.sys);
     }
     if (mode & kEwEnergies) hipLaunchKernelGGL(k_ew_energy, dim3(1), dim3(1024), 0, L.stream, E.sys);

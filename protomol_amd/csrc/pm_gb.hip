@@ -358,8 +358,9 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_force(SysDev *sys, const int *g
     double t[3] = {0, 0, 0};
     for (int c = 0; c < 3; ++c)
       for (int w = 0; w < kGBBlock / 64; ++w) t[c] += red[c][w];
-    atomicAdd(reinterpret_cast<unsigned long long *>(&S.acc[kAccCoul]), (unsigned long long)__double2ll_rn((t[0] + t[2]) * kEScale));
-    atomicAdd(reinterpret_cast<unsigned long long *>(&S.acc[kAccLJ]), (unsigned long long)__double2ll_rn(t[1] * kEScale));
+    long long *acc = acc_copy(S.acc, (int)blockIdx.x);
+    atomicAdd(reinterpret_cast<unsigned long long *>(&acc[kAccCoul]), (unsigned long long)__double2ll_rn((t[0] + t[2]) * kEScale));
+    atomicAdd(reinterpret_cast<unsigned long long *>(&acc[kAccLJ]), (unsigned long long)__double2ll_rn(t[1] * kEScale));
   }
 }
 

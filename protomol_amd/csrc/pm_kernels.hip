@@ -898,13 +898,18 @@ __device__ __forceinline__ float pair_terms(const ForceDev &P, float r2, float q
 // in which waves finish.
 // SAME: both terms share one inclusion cutoff (one test per pair, and no
 // "outer" branch in the switch)
-// waves per SIMD the register allocation must allow (6 measured best on
-// gfx950: a little spilling, more waves in flight)
+// waves per SIMD the register allocation must allow: 6 for the forces-only
+// kernel (measured best on gfx950: a little spilling, more waves in flight);
+// the kinds that also carry FP64 energy and virial sums (or the census) get
+// the registers of 4 waves, so those sums never spill inside the entry loop
 #ifndef PM_PAIR_MIN_WAVES
 #define PM_PAIR_MIN_WAVES 6
 #endif
+#ifndef PM_PAIR_MIN_WAVES_EN
+#define PM_PAIR_MIN_WAVES_EN 4
+#endif
 template <int TA, int TB, int SWA, int SWB, int KIND, bool SAME>
-__global__ __launch_bounds__(512, PM_PAIR_MIN_WAVES) void k_pair(const SysDev *__restrict__ sys, const int *__restrict__ scp, int nsys,
+__global__ __launch_bounds__(512, (KIND == kPairForces ? PM_PAIR_MIN_WAVES : PM_PAIR_MIN_WAVES_EN)) void k_pair(const SysDev *__restrict__ sys, const int *__restrict__ scp, int nsys,
                                               ForceDev P) {
   constexpr bool FORCES = KIND != kPairEnergies;
   constexpr bool EN = KIND == kPairForcesEnergies || KIND == kPairEnergies;
@@ -1134,7 +1139,7 @@ __global__ __launch_bounds__(512, PM_PAIR_MIN_WAVES) void k_pair(const SysDev *_
     vyz = wave_sum_d(vyz);
     vzz = wave_sum_d(vzz);
     if (lane == 0) {
-      PM_GLOBAL long long *acc = gp(S->acc);
+      PM_GLOBAL long long *acc = gp(acc_copy(S->acc, w));
       const int slotA = TA == PM_TERM_LJ ? kAccLJ : kAccCoul;
       atomic_add_ll(&acc[slotA], to_fixed(eA, kEScale));
       atomic_add_ll(&acc[kAccCoul], to_fixed(eB, kEScale));
@@ -1292,14 +1297,15 @@ __global__ __launch_bounds__(256) void k_local(SysDev *sys, const int *ap, int n
   const int slots[9] = {kAccBond, kAccAngle, kAccCoul, kAccVxx, kAccVxy, kAccVxz, kAccVyy, kAccVyz, kAccVzz};
   const int s0 = __shfl(s, 0);
   const bool mixed = wave_ballot(ok && s != s0) != 0;
+  const int wid = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (!mixed) {
     for (int k = 0; k < 9; ++k) q[k] = wave_sum_ll(q[k]);
     if ((threadIdx.x & 63) == 0)
       for (int k = 0; k < 9; ++k)
-        if (q[k]) atomic_add_ll(&sys[s0].acc[slots[k]], q[k]);
+        if (q[k]) atomic_add_ll(&acc_copy(sys[s0].acc, wid)[slots[k]], q[k]);
   } else if (ok) {
     for (int k = 0; k < 9; ++k)
-      if (q[k]) atomic_add_ll(&sys[s].acc[slots[k]], q[k]);
+      if (q[k]) atomic_add_ll(&acc_copy(sys[s].acc, wid)[slots[k]], q[k]);
   }
 }
 
@@ -1403,7 +1409,7 @@ __global__ void k_step_begin(SysDev *sys, int nsys, int mode, float lim2, int km
   const int s = blockIdx.x;
   if (s >= nsys) return;
   SysDev &S = sys[s];
-  for (int k = threadIdx.x; k < kAccN; k += blockDim.x) S.acc[k] = 0;
+  for (int k = threadIdx.x; k < kAccN * kAccCopies; k += blockDim.x) S.acc[k] = 0;
   if (threadIdx.x == 0) {
     S.census[0] = 0;
     S.census[1] = 0;
@@ -1454,7 +1460,8 @@ __global__ void k_gather_scalars(const SysDev *sys, int nsys, long long *out) {
   long long *o = out + (size_t)s * kScalN;
   for (int k = threadIdx.x; k < kScalN; k += blockDim.x) {
     long long v = 0;
-    if (k < kAccN) v = S.acc[k];
+    if (k < kAccN)
+      for (int c = 0; c < kAccCopies; ++c) v += S.acc[(size_t)c * kAccN + k];
     else if (k < kScalCensus + 2) v = (long long)S.census[k - kScalCensus];
     else if (k == kScalErr) v = *S.err;
     else if (k >= kScalCtl && k < kScalCtl + kCtlN) v = S.ctl[k - kScalCtl];
