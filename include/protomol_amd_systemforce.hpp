@@ -8,6 +8,8 @@
 //       (force/nonbonded/NonbondedCutoffSystemForce.h:45-55)
 //   NonbondedEwaldRealSystemForceHIP  for NonbondedFullEwaldSystemForce<PBC, Cubic,
 //       true, false, correction, Cutoff> (force/nonbonded/NonbondedFullEwaldSystemForce.h:329-398)
+//   NonbondedFullEwaldSystemForceHIP  for NonbondedFullEwaldSystemForce<PBC, Cubic,
+//       true, true, correction, Cutoff> (the same, with reciprocalTerm :873-1083)
 //
 // evaluate() keeps the reference's semantics: it ADDS into forces and into
 // the energy/virial slots of the ScalarStructure; a non-zero status from the
@@ -164,6 +166,31 @@ namespace ProtoMol {
       return d;
     }
     virtual Force *doMake(const std::vector<Value> &) const { return new NonbondedEwaldRealSystemForceHIP(*this); }
+  };
+
+  // The full Ewald sum: real space + reciprocal space (+ correction, self and
+  // charged-system terms), NonbondedFullEwaldSystemForce<PBC, Cubic, true,
+  // true, correction, Cutoff> ("CoulombEwald")
+  class NonbondedFullEwaldSystemForceHIP : public PmSystemForceBase {
+  public:
+    NonbondedFullEwaldSystemForceHIP(double alpha, double accuracy, bool correction)
+      : PmSystemForceBase(describe(alpha, accuracy, correction)) {}
+    virtual std::string getKeyword() const { return "FullEwaldHIP"; }
+    virtual std::string getIdNoAlias() const { return "CoulombEwald -algorithm FullEwaldHIP"; }
+    virtual void getParameters(std::vector<Parameter> &) const {}
+  private:
+    static pm_force_desc describe(double alpha, double accuracy, bool correction) {
+      pm_force_desc d = pm_force_desc();
+      d.term_a = PM_TERM_NONE;
+      d.term_b = PM_TERM_EWALD_REAL;
+      d.ewald_alpha = alpha;
+      d.ewald_accuracy = accuracy;
+      d.ewald_correction = correction ? 1 : 0;
+      d.rebuild_interval = 1;
+      d.flags = PM_FLAG_EWALD_RECIPROCAL;
+      return d;
+    }
+    virtual Force *doMake(const std::vector<Value> &) const { return new NonbondedFullEwaldSystemForceHIP(*this); }
   };
 
 }  // namespace ProtoMol
