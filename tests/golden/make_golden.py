@@ -199,10 +199,24 @@ def make_water423():
                         switchon=0.1, cellsize=6.5)
 
 
+def copy_water_expected():
+    """The reference's own expected output files of water_CHARMM_PERIODIC_L
+    (data the reference's test holds), for the output-format checks."""
+    import shutil
+    dst = os.path.join(HERE, "water423_expected")
+    os.makedirs(dst, exist_ok=True)
+    for ext in ("forces", "pos", "vel", "energy", "energy.header", "dcd"):
+        shutil.copyfile(os.path.join(REFTESTS, f"expected/water_CHARMM_PERIODIC_L.{ext}"),
+                        os.path.join(dst, f"water_CHARMM_PERIODIC_L.{ext}"))
+
+
 def main():
     w = water_fixture_system()
     if sys.argv[1:] == ["ewald_recip"]:
         make_ewald_reciprocal(w)
+        return
+    if sys.argv[1:] == ["expected"]:
+        copy_water_expected()
         return
     for name, case in CUTOFF_CASES.items():
         make_cutoff(name, w, case)
@@ -211,6 +225,7 @@ def main():
     make_bonded(w)
     make_gb(300, seed=7)
     make_water423()
+    copy_water_expected()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -283,6 +283,63 @@ def test_water423_reference_regression_on_device():
     assert T.rms_rel(F, d["forces200"]) < 1e-3
 
 
+def test_water423_outputs_on_device(tmp_path):
+    """The reference's water regression run on the HIP path, written through
+    ProtoMol's output formats (XYZ force trajectory, final XYZ positions and
+    velocities, allEnergiesFile, DCD) and read back against the reference's
+    own expected files of that test."""
+    import os
+    from protomol_amd import io as pio
+    print("case water423 outputs", flush=True)
+    exp = os.path.join(os.path.dirname(__file__), "golden", "water423_expected", "water_CHARMM_PERIODIC_L")
+    sysw, d = T.water423_system()
+    names = pio.read_xyz_frames(exp + ".pos")[0][0]
+    desc = force_desc(L.TERM_LJ, (3, 0.1, 6.5, 0.0, 0.0), L.TERM_COULOMB, (2, 0.0, 6.5, 0.0, 0.0),
+                      flags=L.FLAG_BONDED, skin=1.0, interval=5, cutoff=6.5)
+    ftraj = pio.XYZTrajectoryWriter(str(tmp_path / "w.forces"))
+    dcd = pio.DCDWriter(str(tmp_path / "w.dcd"))
+    ew = pio.EnergiesWriter(str(tmp_path / "w.energy"))
+
+    def energies_line(t_fs, x, v, e):
+        ke = S.kinetic_energy(sysw, v)
+        pot = e[0] + e[1] + e[2] + e[3]
+        ew.write(t_fs, pot, ke, pot + ke, S.temperature(sysw, v), e[2], e[3], 0.0, 0.0, e[1], e[0], 0.0,
+                 float(np.prod(sysw.box)))
+
+    with L.Context(desc, [sysw]) as ctx:
+        f0, e0 = evaluate(ctx, 0, d["x0"])
+        ftraj.write(f0, names)
+        dcd.write(d["x0"])
+        energies_line(0.0, d["x0"], d["v0"], e0)
+        ctx.md_set_state(d["x0"], d["v0"])
+        ctx.md_run(int(d["steps"]), float(d["dt"]), energies=True)
+        x, v, F, E = ctx.md_get_state()
+    ftraj.write(F, names)
+    dcd.write(x)
+    energies_line(float(d["steps"]) * float(d["dt"]), x, v, E[0])
+    pio.write_xyz(str(tmp_path / "w.pos"), x, names)
+    pio.write_xyz(str(tmp_path / "w.vel"), v, names)
+    # read back what was written, against the reference's expected files
+    fr, fe = pio.read_xyz_frames(str(tmp_path / "w.forces")), pio.read_xyz_frames(exp + ".forces")
+    assert len(fr) == len(fe) == 2 and fr[0][0] == fe[0][0]
+    assert T.rms_rel(fr[0][1], fe[0][1]) <= RTOL_F and T.rms_rel(fr[1][1], fe[1][1]) < 1e-3
+    assert np.max(np.abs(pio.read_xyz_frames(str(tmp_path / "w.pos"))[0][1] - pio.read_xyz_frames(exp + ".pos")[0][1])) < 2e-3
+    assert T.rms_rel(pio.read_xyz_frames(str(tmp_path / "w.vel"))[0][1], pio.read_xyz_frames(exp + ".vel")[0][1]) < 1e-2
+    hdr, frames = pio.read_dcd(str(tmp_path / "w.dcd"))
+    hdr_e, frames_e = pio.read_dcd(exp + ".dcd")
+    assert hdr["sets"] == hdr_e["sets"] == 2 and hdr["atoms"] == hdr_e["atoms"]
+    assert np.max(np.abs(frames[1] - frames_e[1])) < 2e-3
+    rows = [[float(t) for t in ln.split()] for ln in open(str(tmp_path / "w.energy")) if ln.strip()]
+    rows_e = [[float(t) for t in ln.split()] for ln in open(exp + ".energy") if ln.strip()]
+    assert open(str(tmp_path / "w.energy.header")).read() == open(exp + ".energy.header").read()
+    for r, re_ in zip(rows, rows_e):
+        assert r[0] == re_[0] and r[12] == re_[12]
+        # time 0: same state, FP32 pair arithmetic; 100 fs: after 200 steps
+        tol = 1e-3 if r[0] == 0.0 else 2e-2
+        for k in (1, 2, 3, 5, 6, 9, 10):
+            assert abs(r[k] - re_[k]) <= tol * max(1.0, abs(re_[k])), (k, r[k], re_[k])
+
+
 def _md(desc, systems, v0s, steps, dt):
     with L.Context(desc, systems) as ctx:
         ctx.md_set_state(np.concatenate([s.x for s in systems]), np.concatenate(v0s))
