@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define PM_ABI_VERSION 2
+#define PM_ABI_VERSION 3
 
 /* status codes */
 #define PM_OK 0
@@ -75,6 +75,8 @@ extern "C" {
 #define PM_FLAG_GB 2u         /* GBBornRadii -> GBPartialSum -> GBForce, all pairs, vacuum */
 #define PM_FLAG_BONDED 4u     /* harmonic Bond + Angle (force/bonded/{Bond,Angle}SystemForce.h) */
 #define PM_FLAG_TIMING 8u     /* time every pair-kernel launch with HIP events (pm_stats [4], [5]) */
+#define PM_FLAG_GB_ACE 32u   /* with PM_FLAG_GB: add GBACEForce, the ACE non-polar solvation term
+                                 (force/GB/GBACEForce.cpp:14-170), parameters gb_ace_* */
 #define PM_FLAG_EWALD_RECIPROCAL 16u /* with PM_TERM_EWALD_REAL: add the reciprocal-space sum over
                                         |k| < kc (NonbondedFullEwaldSystemForce::reciprocalTerm,
                                         force/nonbonded/NonbondedFullEwaldSystemForce.h:873-1083) */
@@ -144,6 +146,8 @@ typedef struct pm_force_desc {
   double cell_scale;           /* x-y column width as a multiple of (8/density)^(1/3): 0.6 .. 2 */
   int32_t pair_waves;          /* waves per workgroup of the pair kernel: 1, 2, 4 (default) or 8 */
   int32_t pair_split;          /* waves sharing one super-cluster's pair list: 1 .. 64 (default 12) */
+  double gb_ace_sigma;         /* GBACEForce -solvationparam (kcal/mol/A^2; 0 = 2.26/418.4) */
+  double gb_ace_rho_s;         /* GBACEForce -watersphereradius (A; 0 = 1.4) */
 } pm_force_desc;
 
 typedef struct pm_ctx pm_ctx;
@@ -207,6 +211,16 @@ int pm_md_begin(pm_ctx *ctx, int32_t n, const int32_t *ids);
 int pm_md_advance(pm_ctx *ctx, int32_t n, const int32_t *ids, int32_t n_steps, double dt_fs, int32_t energies_last);
 int pm_md_end(pm_ctx *ctx, int32_t n, const int32_t *ids, double dt_fs);
 int pm_md_get_state_systems(pm_ctx *ctx, int32_t n, const int32_t *ids, double *x, double *v, double *f, double *e);
+
+/* LangevinLeapfrogIntegrator (integrator/base/LangevinLeapfrogIntegrator.cpp:
+ * doHalfKick/doDrift, the STS step half kick, drift, forces, half kick) for
+ * every later pm_md_* call of the context: temperature in K, gamma in 1/ps
+ * (the reference's "gamma" parameter; 0 = plain leapfrog).  The noise of
+ * atom a at the k-th half kick of its system is Philox-4x32-10 of (a, k) under
+ * the system's seed, so a system's trajectory depends only on its own calls.
+ * pm_system_set_seed gives one system its own seed (default: this seed). */
+int pm_md_set_langevin(pm_ctx *ctx, double temperature, double gamma_per_ps, uint32_t seed);
+int pm_system_set_seed(pm_ctx *ctx, int32_t id, uint32_t seed);
 
 /* Statistics of the last call: stats[0] pair-list rebuilds (summed over
  * systems), [1] 1 if an MD advance was re-run after a pair-list capacity grew, [2] entries (8x8

@@ -61,7 +61,8 @@ class OrcEwald(C.Structure):
 class OrcGB(C.Structure):
     _fields_ = [("vdw_radius", _dp), ("scaling", _dp), ("offset", C.c_double),
                 ("alpha_obc", C.c_double), ("beta_obc", C.c_double), ("gamma_obc", C.c_double),
-                ("solute_dielec", C.c_double), ("solvent_dielec", C.c_double), ("block_size", C.c_int)]
+                ("solute_dielec", C.c_double), ("solvent_dielec", C.c_double), ("block_size", C.c_int),
+                ("ace", C.c_int), ("ace_sigma", C.c_double), ("ace_rho_s", C.c_double)]
 
 
 _lib = None
@@ -218,7 +219,12 @@ def census_bruteforce(sys, cut2, x=None):
     return cen.count, cen.hash
 
 
-def gb_evaluate(sys, offset=0.09, a=1.0, b=0.8, g=4.85, eps_in=1.0, eps_out=78.3, x=None):
+ACE_SIGMA = 2.26 / 418.4   # GBACEForce -solvationparam default (GBACEForce.cpp:218-219)
+ACE_RHO_S = 1.4            # -watersphereradius default
+
+
+def gb_evaluate(sys, offset=0.09, a=1.0, b=0.8, g=4.85, eps_in=1.0, eps_out=78.3, x=None, ace=False,
+                ace_sigma=ACE_SIGMA, ace_rho_s=ACE_RHO_S):
     os_ = OracleSystem(sys)
     rho = np.ascontiguousarray(sys.gb_radius, np.float64)
     S = np.ascontiguousarray(sys.gb_scaling, np.float64)
@@ -228,6 +234,8 @@ def gb_evaluate(sys, offset=0.09, a=1.0, b=0.8, g=4.85, eps_in=1.0, eps_out=78.3
     gb.offset, gb.alpha_obc, gb.beta_obc, gb.gamma_obc = offset, a, b, g
     gb.solute_dielec, gb.solvent_dielec = eps_in, eps_out
     gb.block_size = 64
+    gb.ace = 1 if ace else 0
+    gb.ace_sigma, gb.ace_rho_s = ace_sigma, ace_rho_s
     xx = np.ascontiguousarray(sys.x if x is None else x, np.float64).reshape(-1)
     f = np.zeros(3 * sys.n)
     e = np.zeros(NSLOTS)

@@ -1234,6 +1234,34 @@ int orc_gb_evaluate(const orc_system *sys, const orc_gb *gb, const double *x, do
   for (int i = 0; i < n; i++) tot += self[i];
   e[ORC_COULOMB] += egb;
   e[ORC_COULOMB] += tot;
+  if (gb->ace) {
+    /* GBACEForce::operator() (GBACEForce.cpp:14-157), the same all-pairs walk;
+     * PI is the reference's own constant 3.14169 (GBACEForce.cpp:7) */
+    const double PIa = 3.14169, sg = gb->ace_sigma, rs = gb->ace_rho_s;
+    const double c1 = 24 * PIa * sg;
+    GB_FOR_PAIRS(n, bs, {
+      double d[3] = {x[3 * j] - x[3 * i], x[3 * j + 1] - x[3 * i + 1], x[3 * j + 2] - x[3 * i + 2]};
+      double dist = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+      double ori = rho[i] - gb->offset, orj = rho[j] - gb->offset;
+      double dRi = gb_dR(dist, rho[i], ori, orj, S[j], born[i], psi[i], gb->alpha_obc, gb->beta_obc, gb->gamma_obc);
+      double dRj = gb_dR(dist, rho[j], orj, ori, S[i], born[j], psi[j], gb->alpha_obc, gb->beta_obc, gb->gamma_obc);
+      double ci = (rho[i] + rs) * (rho[i] + rs) * pow(rho[i], 6);
+      double cj = (rho[j] + rs) * (rho[j] + rs) * pow(rho[j], 6);
+      double bi7 = pow(born[i], 7), bj7 = pow(born[j], 7);
+      double force = c1 * (ci * (1 / bi7) * dRi * (1 / dist) + cj * (1 / bj7) * dRj * (1 / dist));
+      double fij[3] = {d[0] * force, d[1] * force, d[2] * force};
+      f[3 * i] -= fij[0]; f[3 * i + 1] -= fij[1]; f[3 * i + 2] -= fij[2];
+      f[3 * j] += fij[0]; f[3 * j + 1] += fij[1]; f[3 * j + 2] += fij[2];
+    })
+    /* GBACEForce::postProcess (GBACEForce.cpp:160-170): sum of the ACE
+     * potentials 4 PI sigma (rho_i + rho_s)^2 (rho_i / R_i)^6 */
+    double ace = 0.0;
+    for (int i = 0; i < n; i++) {
+      double p1 = rho[i] / born[i];
+      ace += 4 * PIa * sg * (rho[i] + rs) * (rho[i] + rs) * pow(p1, 6);
+    }
+    e[ORC_COULOMB] += ace;
+  }
   if (born_out) memcpy(born_out, born, sizeof(double) * (size_t)n);
   free(burial); free(born); free(psi); free(partial); free(self);
   return 0;

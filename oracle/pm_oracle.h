@@ -111,6 +111,8 @@ typedef struct orc_gb {
   double alpha_obc, beta_obc, gamma_obc;
   double solute_dielec, solvent_dielec;
   int block_size;            /* NonbondedSimpleFullSystemForce default 64 */
+  int ace;                   /* add GBACEForce (force/GB/GBACEForce.cpp) */
+  double ace_sigma, ace_rho_s; /* -solvationparam, -watersphereradius */
 } orc_gb;
 
 uint64_t orc_pair_hash(uint32_t lo, uint32_t hi);

@@ -25,6 +25,7 @@
 #include <protomol/force/GB/GBBornRadii.h>
 #include <protomol/force/GB/GBPartialSum.h>
 #include <protomol/force/GB/GBForce.h>
+#include <protomol/force/GB/GBACEForce.h>
 #include <protomol/force/bonded/BondSystemForce.h>
 #include <protomol/force/bonded/AngleSystemForce.h>
 #include <protomol/switch/C1SwitchingFunction.h>
@@ -304,6 +305,10 @@ static int do_gb(Recs &r, Vector3DBlock &pos, Vector3DBlock &f, ScalarStructure 
   run_full(topo, pos, f, e, OneAtomPairNoExclusion<VBC, U, GBBornRadii>(GBBornRadii(), U()));
   run_full(topo, pos, f, e, OneAtomPairNoExclusion<VBC, U, GBPartialSum>(GBPartialSum(), U()));
   run_full(topo, pos, f, e, OneAtomPairNoExclusion<VBC, U, GBForce>(GBForce(gp[4], gp[5]), U()));
+  if (geti(r, "gb_ace", 0))
+    run_full(topo, pos, f, e,
+             OneAtomPairNoExclusion<VBC, U, GBACEForce>(GBACEForce(getd(r, "ace_sigma", 2.26 / 418.4),
+                                                                   getd(r, "ace_rho_s", 1.4)), U()));
   born.resize(n);
   for (int i = 0; i < n; i++) born[i] = topo.atoms[i].myGBSA_T->bornRad;
   if (geti(r, "gb_ljcoul", 0)) {

@@ -17,14 +17,14 @@ PM_OK, PM_EINVAL, PM_ENOMEM, PM_EDEVICE, PM_ECAPACITY, PM_ENOTREADY, PM_EFULL = 
 SWITCH_NONE, SWITCH_CUTOFF, SWITCH_C1, SWITCH_C2, SWITCH_CN, SWITCH_CMPCNCN = range(6)
 TERM_NONE, TERM_LJ, TERM_COULOMB, TERM_EWALD_REAL = range(4)
 LJ_TABLE, LJ_PERATOM = 0, 1
-FLAG_VIRIAL, FLAG_GB, FLAG_BONDED, FLAG_TIMING, FLAG_EWALD_RECIPROCAL = 1, 2, 4, 8, 16
+FLAG_VIRIAL, FLAG_GB, FLAG_BONDED, FLAG_TIMING, FLAG_EWALD_RECIPROCAL, FLAG_GB_ACE = 1, 2, 4, 8, 16, 32
 NSLOTS = 36
 
 EXPORTED = ("pm_abi_version", "pm_device_info", "pm_ctx_create", "pm_ctx_destroy", "pm_ctx_reserve",
             "pm_ctx_memory", "pm_system_footprint", "pm_system_add", "pm_system_remove", "pm_last_error",
             "pm_evaluate", "pm_evaluate_batch", "pm_evaluate_systems", "pm_pair_census", "pm_md_set_state",
             "pm_md_run", "pm_md_get_state", "pm_md_set_state_systems", "pm_md_begin", "pm_md_advance",
-            "pm_md_end", "pm_md_get_state_systems", "pm_stats")
+            "pm_md_end", "pm_md_get_state_systems", "pm_md_set_langevin", "pm_system_set_seed", "pm_stats")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -52,7 +52,7 @@ class ForceDesc(C.Structure):
                 ("gb_beta", C.c_double), ("gb_gamma", C.c_double), ("gb_solute_dielec", C.c_double),
                 ("gb_solvent_dielec", C.c_double), ("skin", C.c_double), ("rebuild_interval", C.c_int32),
                 ("flags", C.c_uint32), ("cell_scale", C.c_double), ("pair_waves", C.c_int32),
-                ("pair_split", C.c_int32)]
+                ("pair_split", C.c_int32), ("gb_ace_sigma", C.c_double), ("gb_ace_rho_s", C.c_double)]
 
 
 class PMError(RuntimeError):
@@ -96,6 +96,8 @@ def lib():
         L.pm_md_advance.argtypes = [C.c_void_p, C.c_int32, _ip, C.c_int32, C.c_double, C.c_int32]
         L.pm_md_end.argtypes = [C.c_void_p, C.c_int32, _ip, C.c_double]
         L.pm_md_get_state_systems.argtypes = [C.c_void_p, C.c_int32, _ip, _dp, _dp, _dp, _dp]
+        L.pm_md_set_langevin.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_uint32]
+        L.pm_system_set_seed.argtypes = [C.c_void_p, C.c_int32, C.c_uint32]
         _lib = L
     return _lib
 
@@ -332,6 +334,13 @@ class Context:
         e = np.zeros((len(ii), NSLOTS))
         check(lib().pm_md_get_state_systems(self.h, len(ii), ptr(ii, _ip), ptr(x), ptr(v), ptr(f), ptr(e)), self.h)
         return self._split(ii, x), self._split(ii, v), self._split(ii, f), e
+
+    def set_langevin(self, temperature: float, gamma_per_ps: float, seed: int = 1234):
+        """LangevinLeapfrogIntegrator for later MD calls (gamma 0: plain leapfrog)."""
+        check(lib().pm_md_set_langevin(self.h, temperature, gamma_per_ps, seed), self.h)
+
+    def set_seed(self, sid: int, seed: int):
+        check(lib().pm_system_set_seed(self.h, sid, seed), self.h)
 
     def stats(self):
         s = np.zeros(10)

@@ -28,6 +28,7 @@ namespace {
 
 thread_local std::string g_thread_err;
 constexpr double kInvTimeFactor = 0.02045482828087295;  // PMConstants.cpp:54
+constexpr double kBoltzmann = 0.001987191;              // PMConstants.cpp:70
 constexpr size_t kAlign = 256;
 
 struct Fail {
@@ -257,6 +258,7 @@ struct LaunchSet {
   long long *h_scal = nullptr;  // pinned
   hipGraphExec_t gexec = nullptr;  // captured MD steps for time step g_h
   double g_h = 0.0;
+  pm::LangevinDev g_lv{};          // ... and these Langevin constants
   ~LaunchSet() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (h_scal) (void)hipHostFree(h_scal);
@@ -273,6 +275,9 @@ struct pm_ctx {
   ForceDev P{};
   bool gb = false;
   bool recip = false;       // PM_FLAG_EWALD_RECIPROCAL
+  // LangevinLeapfrogIntegrator (gamma_ps = 0: plain leapfrog)
+  double lv_temperature = 0.0, lv_gamma_ps = 0.0;
+  unsigned lv_seed = 1234;
   double rc = 0.0, skin_md = 0.0, skin_eval = 0.0;
   std::vector<std::unique_ptr<SysHost>> sys;  // by id; null once removed
   Arena arena;
@@ -752,6 +757,7 @@ int add_system(pm_ctx *c, const pm_system_desc &d) {
   const int n = p.n;
   h.n = n;
   S.n_atoms = n;
+  S.lv_seed = c->lv_seed;
   S.periodic = d.periodic ? 1 : 0;
   S.lj_mode = d.lj_mode;
   S.n_types = d.n_types;
@@ -1185,6 +1191,10 @@ void set_state_ids(pm_ctx *c, const std::vector<int> &ids, const double *x, cons
   for (int attempt = 0;; ++attempt) {
     LaunchSet &ls = get_set(c, ids);
     pm::launch_import(ls.L, pm::kXferX | pm::kXferV);
+    // a new state starts its Langevin noise stream at counter 0 (the force
+    // evaluation below advances it to 1)
+    for (int id : ids)
+      HIPCHECK(hipMemsetAsync(c->sys[id]->dev.ctl + pm::kCtlNoise, 0, sizeof(int), c->stream));
     force_eval(c, ls.L, pm::kStepForce, md_policy(c), pm::kPairForces, false, false);
     read_scalars(c, ls);
     if (!check_errors(c, ls).empty()) {
@@ -1206,8 +1216,28 @@ constexpr int kGraphSteps = 10;
 // kGraphSteps MD steps of set ls for time step h, captured on first use.
 // Kernel arguments are the set's own device tables, which live as long as the
 // set does, so the graph stays valid with it.
+// LangevinLeapfrogIntegrator constants for time step dt_fs
+// (LangevinLeapfrogIntegrator.cpp:24-26, 46-54)
+pm::LangevinDev lv_params(const pm_ctx *c, double dt_fs) {
+  pm::LangevinDev lv;
+  if (!(c->lv_gamma_ps > 0.0)) return lv;
+  const double g = c->lv_gamma_ps / (1000 * kInvTimeFactor);  // myGamma
+  const double dt = dt_fs * kInvTimeFactor;
+  lv.on = 1;
+  lv.fdt = (1.0 - std::exp(-0.5 * g * dt)) / g;
+  lv.vdt = std::exp(-0.5 * g * dt);
+  lv.ndt = std::sqrt((1.0 - std::exp(-g * dt)) / (2.0 * g));
+  lv.fc = 2 * kBoltzmann * c->lv_temperature * g;
+  return lv;
+}
+
+bool same_lv(const pm::LangevinDev &a, const pm::LangevinDev &b) {
+  return a.on == b.on && a.vdt == b.vdt && a.fdt == b.fdt && a.ndt == b.ndt && a.fc == b.fc;
+}
+
 hipGraphExec_t md_graph(pm_ctx *c, LaunchSet &ls, double h, const StepPolicy &pol) {
-  if (ls.gexec && ls.g_h == h) return ls.gexec;
+  const pm::LangevinDev lv = lv_params(c, h / kInvTimeFactor);
+  if (ls.gexec && ls.g_h == h && same_lv(ls.g_lv, lv)) return ls.gexec;
   if (ls.gexec) {
     HIPCHECK(hipGraphExecDestroy(ls.gexec));
     ls.gexec = nullptr;
@@ -1216,7 +1246,7 @@ hipGraphExec_t md_graph(pm_ctx *c, LaunchSet &ls, double h, const StepPolicy &po
   HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   try {
     for (int k = 0; k < kGraphSteps; ++k) {
-      pm::launch_kick_drift(ls.L, h);
+      pm::launch_kick_drift(ls.L, h, lv);
       force_eval(c, ls.L, pm::kStepAuto, pol, pm::kPairForces, false, false);
     }
   } catch (...) {
@@ -1232,6 +1262,7 @@ hipGraphExec_t md_graph(pm_ctx *c, LaunchSet &ls, double h, const StepPolicy &po
     throw Fail{PM_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e)};
   }
   ls.g_h = h;
+  ls.g_lv = lv;
   return ls.gexec;
 }
 
@@ -1270,7 +1301,7 @@ void md_advance(pm_ctx *c, const std::vector<int> &ids, int n_steps, double dt_f
       }
     for (; i < n_steps; ++i) {
       const bool en = energies_last && i == n_steps - 1;
-      pm::launch_kick_drift(ls.L, h);
+      pm::launch_kick_drift(ls.L, h, lv_params(c, dt_fs));
       force_eval(c, ls.L, pm::kStepAuto, pol, pm::kPairForces, en, false);
       if (en) energy_pass(c, ls.L);
     }
@@ -1326,8 +1357,9 @@ void md_advance(pm_ctx *c, const std::vector<int> &ids, int n_steps, double dt_f
 
 void md_end(pm_ctx *c, const std::vector<int> &ids, double dt_fs) {
   require_md(c, ids);
-  // StandardIntegrator::doHalfKick (StandardIntegrator.cpp:80-92)
-  pm::launch_half_kick(get_set(c, ids).L, 0.5 * dt_fs * kInvTimeFactor);
+  // StandardIntegrator::doHalfKick (StandardIntegrator.cpp:80-92), or the
+  // Langevin half kick
+  pm::launch_half_kick(get_set(c, ids).L, 0.5 * dt_fs * kInvTimeFactor, lv_params(c, dt_fs));
 }
 
 void get_state_ids(pm_ctx *c, const std::vector<int> &ids, double *x, double *v, double *f, double *e) {
@@ -1405,6 +1437,7 @@ int pm_ctx_create(const pm_force_desc *force, int32_t n_systems, const pm_system
       throw Fail{PM_EINVAL, "cell_scale must be 0 (default) or within 0.6 .. 2"};
     c->gb = (force->flags & PM_FLAG_GB) != 0;
     c->recip = (force->flags & PM_FLAG_EWALD_RECIPROCAL) != 0;
+    if ((force->flags & PM_FLAG_GB_ACE) && !c->gb) throw Fail{PM_EINVAL, "PM_FLAG_GB_ACE needs PM_FLAG_GB"};
     if (c->recip && (c->gb || force->term_b != PM_TERM_EWALD_REAL))
       throw Fail{PM_EINVAL, "PM_FLAG_EWALD_RECIPROCAL needs term_b = PM_TERM_EWALD_REAL (and no GB)"};
     if (c->gb) {
@@ -1628,6 +1661,30 @@ int pm_md_get_state(pm_ctx *c, double *x, double *v, double *f, double *e) {
 int pm_md_get_state_systems(pm_ctx *c, int32_t n, const int32_t *ids, double *x, double *v, double *f, double *e) {
   if (!c) return PM_EINVAL;
   return run_guarded(c, [&] { get_state_ids(c, check_ids(c, n, ids), x, v, f, e); });
+}
+
+int pm_md_set_langevin(pm_ctx *c, double temperature, double gamma_per_ps, uint32_t seed) {
+  if (!c) return PM_EINVAL;
+  return run_guarded(c, [&] {
+    if (!(temperature >= 0.0) || !(gamma_per_ps >= 0.0)) throw Fail{PM_EINVAL, "temperature and gamma must be >= 0"};
+    c->lv_temperature = temperature;
+    c->lv_gamma_ps = gamma_per_ps;
+    c->lv_seed = seed;
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    for (auto &h : c->sys)
+      if (h) h->dev.lv_seed = seed;
+    drop_sets(c);
+  });
+}
+
+int pm_system_set_seed(pm_ctx *c, int32_t id, uint32_t seed) {
+  if (!c) return PM_EINVAL;
+  return run_guarded(c, [&] {
+    SysHost &h = sys_of(c, id);
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    h.dev.lv_seed = seed;
+    drop_sets_with(c, id);
+  });
 }
 
 int pm_stats(pm_ctx *c, double *stats, int32_t n) {

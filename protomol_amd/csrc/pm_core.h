@@ -71,8 +71,9 @@ enum { kPairForces = 0, kPairForcesEnergies = 1, kPairEnergies = 2, kPairCensus 
 // per-system control words: [0] rebuild the pair list in this step (0/1),
 // [1] steps since the last rebuild, [2] rebuilds since the start of the run,
 // [3] the next kick is the first of a run (a half kick), [4] entries (8x8
-// cluster pairs) of the pair list at its last build
-enum { kCtlRebuild = 0, kCtlSince = 1, kCtlCount = 2, kCtlFirst = 3, kCtlEntries = 4, kCtlN = 6 };
+// cluster pairs) of the pair list at its last build, [5] force evaluations
+// so far (the counter of the Langevin noise stream)
+enum { kCtlRebuild = 0, kCtlSince = 1, kCtlCount = 2, kCtlFirst = 3, kCtlEntries = 4, kCtlNoise = 5, kCtlN = 6 };
 // k_step_begin modes (bit set)
 enum { kStepForce = 1, kStepAuto = 2 };
 
@@ -102,6 +103,10 @@ struct ForceDev {
   int pair_split;          // waves sharing one super-cluster's list
   // GB / OBC (ProtoMolApp.cpp:285-301, GBForce.cpp:11-12)
   double gb_offset, gb_a, gb_b, gb_g, gb_fac;  // fac = 1/eps_solute - 1/eps_solvent
+  // GBACEForce (GBACEForce.cpp): sigma, rho_s; c1 = 24 PI sigma with the
+  // reference's PI = 3.14169
+  int gb_ace;
+  double gb_ace_sigma, gb_ace_rho_s, gb_ace_c1;
 };
 
 // One system as the kernels see it.  All pointers are device pointers owned
@@ -179,6 +184,7 @@ struct SysDev {
   unsigned *maxdisp2;      // float bits of the largest squared displacement from the last rebuild (this step)
   unsigned *maxdisp_run;   // the same, folded over every step of the current run
   int *ctl;                // kCtlN control words
+  unsigned lv_seed;        // Langevin noise stream key
   // Ewald reciprocal space (NonbondedFullEwaldSystemForce::reciprocalTerm):
   // k = 2 pi (h/Lx, k/Ly, l/Lz), h >= 0 half space, |k| < kc, in the
   // reference's (h, k, l) order; rows = runs of l at fixed (h, k)

@@ -16,6 +16,9 @@
 
 namespace pm {
 
+// the value GBACEForce.cpp:7 defines as PI (sic), used by its energy and force
+constexpr double kAcePi = 3.14169;
+
 void gb_setup_force(ForceDev &P, const pm_force_desc &d) {
   P.gb_offset = d.gb_offset > 0 ? d.gb_offset : 0.09;
   P.gb_a = d.gb_alpha;
@@ -24,6 +27,10 @@ void gb_setup_force(ForceDev &P, const pm_force_desc &d) {
   const double ein = d.gb_solute_dielec > 0 ? d.gb_solute_dielec : 1.0;
   const double eout = d.gb_solvent_dielec > 0 ? d.gb_solvent_dielec : 80.0;
   P.gb_fac = (1.0 / ein) - (1.0 / eout);
+  P.gb_ace = (d.flags & PM_FLAG_GB_ACE) ? 1 : 0;
+  P.gb_ace_sigma = d.gb_ace_sigma > 0 ? d.gb_ace_sigma : 2.26 / 418.4;  // GBACEForce.cpp:218-222 defaults
+  P.gb_ace_rho_s = d.gb_ace_rho_s > 0 ? d.gb_ace_rho_s : 1.4;
+  P.gb_ace_c1 = 24 * kAcePi * P.gb_ace_sigma;
   P.term_a = PM_TERM_LJ;
   P.term_b = PM_TERM_COULOMB;
 }
@@ -100,7 +107,7 @@ __device__ __forceinline__ float dburial(float dist, float idist, float ori, flo
 struct GBShared {
   double x[kGBBlock], y[kGBBlock], z[kGBBlock];
   float q[kGBBlock], ori[kGBBlock], S[kGBBlock], B[kGBBlock], iB[kGBBlock], qB2[kGBBlock], dr[kGBBlock],
-      part[kGBBlock], sg[kGBBlock], ep[kGBBlock];
+      part[kGBBlock], sg[kGBBlock], ep[kGBBlock], ak[kGBBlock];
 };
 
 __device__ void load_chunk(GBShared &sh, const SysDev &S, const ForceDev &P, int j0, int pass) {
@@ -122,6 +129,7 @@ __device__ void load_chunk(GBShared &sh, const SysDev &S, const ForceDev &P, int
     if (pass >= 3) {
       sh.dr[t] = (float)S.gb_drfac[j];
       sh.part[t] = (float)S.gb_partial[j];
+      sh.ak[t] = P.gb_ace ? (float)S.gb_spare[j] : 0.f;
       if (S.lj_mode == PM_LJ_PERATOM) {
         sh.sg[t] = S.sig_half[j];
         sh.ep[t] = S.eps_sqrt[j];
@@ -180,6 +188,11 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_born(SysDev *sys, const int *gp
   const double tderv = P.gb_a - 2.0 * P.gb_b * psi + 3.0 * P.gb_g * psi * psi;
   S.gb_born[i] = B;
   S.gb_drfac[i] = B * B * orid * (1.0 - th * th) * tderv * (1.0 / rho);
+  // ACE: c_i / R_i^7 with c_i = (rho_i + rho_s)^2 rho_i^6 (GBACEForce.cpp:52-54, 155)
+  if (P.gb_ace) {
+    const double rs = rho + P.gb_ace_rho_s, r3 = rho * rho * rho, iB = 1.0 / B, iB3 = iB * iB * iB;
+    S.gb_spare[i] = rs * rs * (r3 * r3) * (iB3 * iB3 * iB);
+  }
 }
 
 // pass 2: partial sums (GBPartialSum.cpp:11-42): sum over l != i of
@@ -267,6 +280,8 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_force(SysDev *sys, const int *g
   const float iBi = rcp(Bi), i4Bi = 0.25f * iBi;
   const float qiB2 = 0.5f * qi * qi * iBi * iBi;  // q_i^2 / (2 B_i^2)
   const float csf = S.csf;
+  const float aki = (vi && P.gb_ace) ? (float)S.gb_spare[i] : 0.f;
+  const float ace_c1 = (float)P.gb_ace_c1;
   double fx = 0.0, fy = 0.0, fz = 0.0, egb = 0.0, elj = 0.0, eco = 0.0;
   for (int j0 = 0; j0 < n; j0 += kGBBlock) {
     __syncthreads();
@@ -303,6 +318,8 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_force(SysDev *sys, const int *g
       force -= (Pi - part * ai) * (dRi * rinv);
       force -= (sh.part[k] - part * bj) * (dRj * rinv);
       force *= fac;
+      // GBACEForce.cpp:155: c1 (c_i R_i^-7 dR_i/dr + c_j R_j^-7 dR_j/dr) / r
+      if (P.gb_ace) force += ace_c1 * (aki * dRi + sh.ak[k] * dRj) * rinv;
       float fs = force;
       const bool upper = j > i;
       if (energies && upper) egb += (double)(-qq * inv * fac);
@@ -343,6 +360,11 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_force(SysDev *sys, const int *g
     atomicAdd(reinterpret_cast<unsigned long long *>(&S.fy[i]), (unsigned long long)__double2ll_rn(fy * kFScale));
     atomicAdd(reinterpret_cast<unsigned long long *>(&S.fz[i]), (unsigned long long)__double2ll_rn(fz * kFScale));
     if (energies) egb += -0.5 * (double)S.charge[i] * (double)S.charge[i] * (1.0 / S.gb_born[i]) * P.gb_fac;
+    if (energies && P.gb_ace) {
+      // GBACEForce.cpp:40-50, 160-170: 4 PI sigma (rho_i + rho_s)^2 (rho_i / R_i)^6
+      const double rho = S.gb_rho[i], rs = rho + P.gb_ace_rho_s, p = rho / S.gb_born[i], p3 = p * p * p;
+      egb += 4 * kAcePi * P.gb_ace_sigma * rs * rs * (p3 * p3);
+    }
   }
   if (!energies) return;
   // block reduction in a fixed order, then fixed-point totals

@@ -33,9 +33,16 @@ void launch_clear_err(const Launch &L);
 void launch_rebuild(const Launch &L, float rlist2);
 bool launch_pair(const Launch &L, const ForceDev &P, int kind);
 void launch_local(const Launch &L, const ForceDev &P, int mode);
-void launch_kick_drift(const Launch &L, double h);
+// LangevinLeapfrogIntegrator::doHalfKick constants for one time step
+// (integrator/base/LangevinLeapfrogIntegrator.cpp:46-70); on = 0: the plain
+// leapfrog kicks
+struct LangevinDev {
+  int on = 0;
+  double vdt = 1.0, fdt = 0.0, ndt = 0.0, fc = 0.0;  // e^{-g dt/2}, (1 - e^{-g dt/2})/g, noise scale, 2 kB T g
+};
+void launch_kick_drift(const Launch &L, double h, const LangevinDev &lv);
 void launch_md_arm(const Launch &L);
-void launch_half_kick(const Launch &L, double hh);
+void launch_half_kick(const Launch &L, double hh, const LangevinDev &lv);
 void launch_zero_forces(const Launch &L);
 void launch_step_begin(const Launch &L, int mode, float lim2, int kmax);
 // reciprocal-space Ewald of the systems in L.ew: kEwForces (tables, structure

@@ -1317,7 +1317,47 @@ __global__ __launch_bounds__(256) void k_local(SysDev *sys, const int *ap, int n
 //   doKickdoDrift     (LeapfrogIntegrator.cpp:58-87)  v += F*h/m;     x += v*h
 // The force buffer is consumed (zeroed) and the FP32 cluster-local
 // coordinates for the next force evaluation are written.
-__global__ __launch_bounds__(256) void k_kick_drift(SysDev *sys, const int *ap, int nsys, double h) {
+// Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+// easy as 1, 2, 3", SC 2011): a counter-based generator, so atom a's noise
+// at kick k of its system depends on (seed, a, k) only
+__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint2 k) {
+  for (int r = 0; r < 10; ++r) {
+    const unsigned hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+    const unsigned hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+    c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
+    k.x += 0x9E3779B9u;
+    k.y += 0xBB67AE85u;
+  }
+  return c;
+}
+
+// three N(0, 1) deviates for atom a at noise counter ctr (Box-Muller on the
+// four 32-bit outputs, u = (x + 1/2) 2^-32 in (0, 1))
+__device__ __forceinline__ void lv_gauss(unsigned seed, int a, unsigned ctr, double g[3]) {
+  const uint4 r = philox4x32_10(make_uint4((unsigned)a, ctr, 0x4C414E47u, 0u), make_uint2(seed, 0x50524F54u));
+  const double s = 2.3283064365386963e-10;  // 2^-32
+  const double u0 = ((double)r.x + 0.5) * s, u1 = ((double)r.y + 0.5) * s;
+  const double u2 = ((double)r.z + 0.5) * s, u3 = ((double)r.w + 0.5) * s;
+  const double m0 = sqrt(-2.0 * log(u0)), m1 = sqrt(-2.0 * log(u2));
+  double s0, c0, s1, c1;
+  sincospi(2.0 * u1, &s0, &c0);
+  sincospi(2.0 * u3, &s1, &c1);
+  g[0] = m0 * c0;
+  g[1] = m0 * s0;
+  g[2] = m1 * c1;
+}
+
+// one Langevin half kick (LangevinLeapfrogIntegrator.cpp:46-70):
+// v = v e^{-g dt/2} + F (1 - e^{-g dt/2}) / (g m) + N(0,1) sqrt(2 kB T g / m) ndt
+__device__ __forceinline__ void lv_half(double v[3], const double F[3], double m, const LangevinDev &lv,
+                                        unsigned seed, int a, unsigned ctr) {
+  double g[3];
+  lv_gauss(seed, a, ctr, g);
+  const double sf = sqrt(lv.fc / m);
+  for (int d = 0; d < 3; ++d) v[d] = v[d] * lv.vdt + F[d] * lv.fdt / m + g[d] * sf * lv.ndt;
+}
+
+__global__ __launch_bounds__(256) void k_kick_drift(SysDev *sys, const int *ap, int nsys, double h, LangevinDev lv) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   float d2 = 0.f;
   int s = 0;
@@ -1335,9 +1375,19 @@ __global__ __launch_bounds__(256) void k_kick_drift(SysDev *sys, const int *ap, 
     S.fy[sl] = 0;
     S.fz[sl] = 0;
     float l[3];
+    double vl[3];
+    if (lv.on) {
+      // the closing half kick of the previous step (noise 2c) unless this is
+      // the first step of a run, then the opening half kick of this one (2c + 1)
+      const unsigned c = (unsigned)S.ctl[kCtlNoise];
+      for (int d = 0; d < 3; ++d) vl[d] = S.v[d * n + a];
+      if (mode == 1) lv_half(vl, F, m, lv, S.lv_seed, a, 2u * c);
+      lv_half(vl, F, m, lv, S.lv_seed, a, 2u * c + 1u);
+    }
     for (int d = 0; d < 3; ++d) {
       double vv = S.v[d * n + a];
-      if (mode == 0) vv += F[d] * h * 0.5 / m;
+      if (lv.on) vv = vl[d];
+      else if (mode == 0) vv += F[d] * h * 0.5 / m;
       else vv += F[d] * h / m;
       S.v[d * n + a] = vv;
       const double xx = S.x[d * n + a] + vv * h;
@@ -1375,7 +1425,7 @@ __global__ __launch_bounds__(256) void k_kick_drift(SysDev *sys, const int *ap, 
 }
 
 // StandardIntegrator::doHalfKick (StandardIntegrator.cpp:80-92): h = 0.5*dt*ITF, v += F*h/m
-__global__ void k_half_kick(SysDev *sys, const int *ap, int nsys, double hh) {
+__global__ void k_half_kick(SysDev *sys, const int *ap, int nsys, double hh, LangevinDev lv) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ap[nsys]) return;
   const int s = find_sys(ap, nsys, g);
@@ -1385,6 +1435,13 @@ __global__ void k_half_kick(SysDev *sys, const int *ap, int nsys, double hh) {
   const double m = 1.0 / S.inv_mass[a];
   const double F[3] = {(double)S.fx[sl] * kFScaleInv, (double)S.fy[sl] * kFScaleInv,
                        (double)S.fz[sl] * kFScaleInv};
+  if (lv.on) {
+    // the closing half kick of the last step: noise counter 2c
+    double v[3] = {S.v[a], S.v[n + a], S.v[2 * n + a]};
+    lv_half(v, F, m, lv, S.lv_seed, a, 2u * (unsigned)S.ctl[kCtlNoise]);
+    for (int d = 0; d < 3; ++d) S.v[d * n + a] = v[d];
+    return;
+  }
   for (int d = 0; d < 3; ++d) S.v[d * n + a] += F[d] * hh / m;
 }
 
@@ -1415,6 +1472,7 @@ __global__ void k_step_begin(SysDev *sys, int nsys, int mode, float lim2, int km
     S.census[1] = 0;
     int *ctl = S.ctl;
     ctl[kCtlFirst] = 0;  // the step's kick has run
+    ctl[kCtlNoise] += 1;
     const unsigned d2 = *S.maxdisp2;
     bool rb = (mode & kStepForce) != 0;
     if (!rb && (mode & kStepAuto))
@@ -1556,11 +1614,11 @@ void launch_local(const Launch &L, const ForceDev &P, int mode) {
   if (L.n_atoms_total && L.n_local_total)
     hipLaunchKernelGGL(k_local, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys, P, mode);
 }
-void launch_kick_drift(const Launch &L, double h) {
-  if (L.n_atoms_total) hipLaunchKernelGGL(k_kick_drift, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys, h);
+void launch_kick_drift(const Launch &L, double h, const LangevinDev &lv) {
+  if (L.n_atoms_total) hipLaunchKernelGGL(k_kick_drift, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys, h, lv);
 }
-void launch_half_kick(const Launch &L, double hh) {
-  if (L.n_atoms_total) hipLaunchKernelGGL(k_half_kick, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys, hh);
+void launch_half_kick(const Launch &L, double hh, const LangevinDev &lv) {
+  if (L.n_atoms_total) hipLaunchKernelGGL(k_half_kick, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys, hh, lv);
 }
 void launch_zero_forces(const Launch &L) {
   if (L.n_slots_total) hipLaunchKernelGGL(k_zero_forces, dim3(nblk(L.n_slots_total, 256)), dim3(256), 0, L.stream, L.sys, L.slot_prefix, L.nsys);

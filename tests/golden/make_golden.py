@@ -109,6 +109,22 @@ def make_gb(n, seed):
                         born=out["born"], params=params, seed=seed)
 
 
+def make_gb_ace(n, seed):
+    """GB + GBACEForce (ACE non-polar solvation) on a chain, without the
+    LJ/Coulomb terms: GBBornRadii, GBPartialSum, GBForce, GBACEForce."""
+    g = S.gb_chain(n, seed=seed)
+    params = np.array([0.09, 1.0, 0.8, 4.85, 1.0, 78.3])
+    r = refio.system_records(g)
+    r.update(mode=np.int32(3), gb_rho=g.gb_radius, gb_S=g.gb_scaling, gb_params=params, cellsize=np.int32(5),
+             gb_ljcoul=np.int32(0), gb_ace=np.int32(1), ace_sigma=2.26 / 418.4, ace_rho_s=1.4)
+    out = refio.run(r)
+    r["gb_ace"] = np.int32(0)
+    base = refio.run(r)
+    np.savez_compressed(os.path.join(HERE, f"ref_gb_ace{n}.npz"), x=g.x, f=out["f"].reshape(-1, 3), e=out["e"],
+                        f_noace=base["f"].reshape(-1, 3), e_noace=base["e"], born=out["born"], params=params,
+                        seed=seed, ace_sigma=2.26 / 418.4, ace_rho_s=1.4)
+
+
 # ------------------------------------------------------------- water_423
 
 def _pdb_xyz(path):
@@ -218,12 +234,16 @@ def main():
     if sys.argv[1:] == ["expected"]:
         copy_water_expected()
         return
+    if sys.argv[1:] == ["gb_ace"]:
+        make_gb_ace(200, seed=13)
+        return
     for name, case in CUTOFF_CASES.items():
         make_cutoff(name, w, case)
     make_ewald(w)
     make_ewald_reciprocal(w)
     make_bonded(w)
     make_gb(300, seed=7)
+    make_gb_ace(200, seed=13)
     make_water423()
     copy_water_expected()
     for f in sorted(os.listdir(HERE)):

@@ -69,3 +69,63 @@ def leapfrog_oracle(force_fn, x, v, mass, dt_fs, n_steps):
         F, E = force_fn(x)
     v = v + F * (0.5 * dt_fs * S.INV_TIMEFACTOR) / m
     return x, v, F, E
+
+
+# ------------------------------------------------ Langevin noise and oracle
+
+_PH_M0, _PH_M1, _PH_W0, _PH_W1 = 0xD2511F53, 0xCD9E8D57, 0x9E3779B9, 0xBB67AE85
+
+
+def philox4x32_10(c0, c1, c2, c3, k0, k1):
+    """Philox-4x32-10 (Salmon et al., SC 2011) on uint64 numpy arrays holding
+    32-bit words (vectorised over the counter)."""
+    c = [np.asarray(v, np.uint64) & 0xFFFFFFFF for v in (c0, c1, c2, c3)]
+    k = [np.uint64(k0), np.uint64(k1)]
+    m32 = np.uint64(0xFFFFFFFF)
+    for _ in range(10):
+        p0 = np.uint64(_PH_M0) * c[0]
+        p1 = np.uint64(_PH_M1) * c[2]
+        c = [(p1 >> np.uint64(32)) ^ c[1] ^ k[0], p1 & m32, (p0 >> np.uint64(32)) ^ c[3] ^ k[1], p0 & m32]
+        k = [(k[0] + np.uint64(_PH_W0)) & m32, (k[1] + np.uint64(_PH_W1)) & m32]
+    return c
+
+
+def langevin_gauss(seed, n, ctr):
+    """The three N(0, 1) deviates per atom of one Langevin half kick: Box-Muller
+    on Philox-4x32-10 of (atom, ctr, 'LANG', 0) under key (seed, 'PROT')."""
+    a = np.arange(n, dtype=np.uint64)
+    r = philox4x32_10(a, np.full(n, ctr, np.uint64), np.full(n, 0x4C414E47, np.uint64), np.zeros(n, np.uint64),
+                      seed, 0x50524F54)
+    u = [(x.astype(np.float64) + 0.5) * 2.0 ** -32 for x in r]
+    m0, m1 = np.sqrt(-2.0 * np.log(u[0])), np.sqrt(-2.0 * np.log(u[2]))
+    return np.stack([m0 * np.cos(2 * np.pi * u[1]), m0 * np.sin(2 * np.pi * u[1]), m1 * np.cos(2 * np.pi * u[3])], 1)
+
+
+def langevin_oracle(force_fn, x, v, mass, dt_fs, n_steps, temperature, gamma_ps, seed, ctr0=1):
+    """LangevinLeapfrogIntegrator (STSIntegrator::run: half kick, drift,
+    forces, half kick; doHalfKick LangevinLeapfrogIntegrator.cpp:46-70) in
+    numpy FP64, with the device's noise stream; ctr0 = the system's force
+    evaluations before the first kick."""
+    h = dt_fs * S.INV_TIMEFACTOR
+    g = gamma_ps / (1000 * S.INV_TIMEFACTOR)
+    fdt = (1.0 - math.exp(-0.5 * g * h)) / g
+    vdt = math.exp(-0.5 * g * h)
+    ndt = math.sqrt((1.0 - math.exp(-g * h)) / (2.0 * g))
+    fc = 2 * S.BOLTZMANN * temperature * g
+    m = mass[:, None]
+    n = len(mass)
+
+    def half(v, F, k):
+        return v * vdt + F * fdt / m + langevin_gauss(seed, n, k) * np.sqrt(fc / m) * ndt
+
+    F, E = force_fn(x)
+    c = ctr0
+    for i in range(n_steps):
+        if i > 0:
+            v = half(v, F, 2 * c)
+        v = half(v, F, 2 * c + 1)
+        x = x + v * h
+        F, E = force_fn(x)
+        c += 1
+    v = half(v, F, 2 * c)
+    return x, v, F, E

@@ -215,6 +215,39 @@ def test_gb_vs_reference_golden():
     check_energies(e, g["e"], npairs=3 * 300 * 299 // 2)
 
 
+def test_gb_ace_vs_reference_golden():
+    """GBACEForce (ACE non-polar solvation) on the device: the ACE part (GB +
+    ACE minus GB, exact in fixed point) against the reference's classes, and
+    a batch with ACE against the oracle."""
+    print("case gb ace", flush=True)
+    g = T.golden("ref_gb_ace200.npz")
+    c = S.gb_chain(200, seed=int(g["seed"]))
+    assert np.array_equal(c.x, g["x"])
+    out = []
+    for flags in (L.FLAG_GB_ACE, 0):
+        with L.Context(gb_desc(flags), [c]) as ctx:
+            out.append(evaluate(ctx, 0, c.x))
+    fa, ea = out[0][0] - out[1][0], out[0][1] - out[1][1]
+    ref_f, ref_e = g["f"] - g["f_noace"], g["e"] - g["e_noace"]
+    assert T.rms_rel(fa, ref_f) <= RTOL_F
+    assert abs(ea[0] - ref_e[0]) <= 1e-6 * abs(ref_e[0]) + 2e-6
+    chains = [S.gb_chain(n, seed=50 + n) for n in (190, 420)]
+    with L.Context(gb_desc(L.FLAG_GB_ACE), chains) as ctx:
+        x = np.concatenate([ch.x for ch in chains])
+        f = np.zeros_like(x)
+        e = np.zeros((len(chains), L.NSLOTS))
+        ctx.evaluate_batch(x, f, e)
+    off = 0
+    from test_oracle import _vacuum_lj_coulomb
+    for k, ch in enumerate(chains):
+        fo, eo, _ = O.gb_evaluate(ch, ace=True)
+        fl, el = _vacuum_lj_coulomb(ch)
+        assert T.rms_rel(f[off:off + ch.n], fo + fl) <= RTOL_F
+        npairs = 3 * ch.n * (ch.n - 1) // 2
+        assert abs(e[k][0] - (eo[0] + el[0])) <= RTOL_E * abs(eo[0] + el[0]) + 1e-7 * npairs
+        off += ch.n
+
+
 def test_gb_batch_vs_oracle():
     print("case gb batch", flush=True)
     chains = [S.gb_chain(n, seed=40 + n) for n in (257, 600, 1031)]
@@ -601,4 +634,81 @@ def test_md_full_ewald_graph_and_batch_bitwise():
         x, v, F, E, _ = _md(desc, [s], [v0s[k]], 23, 1.0)
         assert np.array_equal(x1[off:off + s.n], x) and np.array_equal(v1[off:off + s.n], v)
         assert np.array_equal(F1[off:off + s.n], F) and np.array_equal(E1[k], E[0])
+        off += s.n
+
+
+# ------------------------------------------------ Langevin leapfrog
+
+def test_langevin_noise_and_kicks_match_oracle_exactly():
+    """Force-free atoms: the device's Langevin run (Philox noise, damping,
+    drift) equals the numpy restatement to FP64 rounding."""
+    print("case langevin free", flush=True)
+    s = S.rock_salt(5, 2.82)
+    s.charge = np.zeros(s.n)
+    d = force_desc(L.TERM_NONE, NOTERM, 0, NOTERM, ewald=True, flags=0, skin=1.0)
+    v0 = S.maxwell_velocities(s, 300.0, seed=2)
+    zero = lambda x: (np.zeros_like(x), np.zeros(36))
+    steps, dt, temp, gamma, seed = 12, 1.0, 300.0, 20.0, 77
+    with L.Context(d, [s]) as ctx:
+        ctx.set_langevin(temp, gamma, seed)
+        ctx.md_set_state(s.x, v0)
+        ctx.md_run(steps, dt, energies=False)
+        x, v, F, _ = ctx.md_get_state()
+    assert np.max(np.abs(F)) == 0.0
+    xo, vo, _, _ = T.langevin_oracle(zero, s.x.copy(), v0.copy(), s.mass, dt, steps, temp, gamma, seed)
+    assert np.max(np.abs(v - vo)) <= 1e-12 * np.max(np.abs(vo))
+    assert np.max(np.abs(x - xo)) <= 1e-11
+
+
+def test_langevin_water_vs_oracle():
+    """Langevin MD of water on the device against the oracle's forces and the
+    numpy LangevinLeapfrogIntegrator with the same noise stream."""
+    print("case langevin water", flush=True)
+    w = S.water_box(343, 22.5, seed=5)
+    v0 = S.maxwell_velocities(w, 300.0, seed=1)
+    sw = O.make_switch(O.SW_C2, 8.0, 10.0)
+
+    def forces(x):
+        f, e, _ = O.nonbonded_cutoff(w, O.TERM_LJ, sw, O.TERM_COULOMB, sw, x=x, cellsize=5.0)
+        fb, eb = O.bonded(w, x=x)
+        return f + fb, e + eb
+
+    steps, dt, temp, gamma, seed = 20, 0.5, 300.0, 5.0, 4242
+    xo, vo, Fo, _ = T.langevin_oracle(forces, w.x.copy(), v0.copy(), w.mass, dt, steps, temp, gamma, seed)
+    ta, sa, tb, sb = MODELS["c2c2"]
+    with L.Context(force_desc(ta, sa, tb, sb, flags=L.FLAG_BONDED, skin=1.5, interval=5), [w]) as ctx:
+        ctx.set_langevin(temp, gamma, seed)
+        ctx.md_set_state(w.x, v0)
+        ctx.md_run(steps, dt, energies=False)
+        x, v, F, _ = ctx.md_get_state()
+    assert np.max(np.abs(x - xo)) < 1e-4
+    assert T.rms_rel(v, vo) < 1e-4
+    assert T.rms_rel(F, Fo) < 1e-4
+
+
+def test_langevin_batch_alone_vs_mix_bitwise():
+    """Per-system seeds: each system of a batched Langevin run ends bitwise
+    where it ends alone."""
+    print("case langevin batch", flush=True)
+    systems = _water_set((520, 700), 30)
+    v0s = [S.maxwell_velocities(s, 300.0, seed=60 + k) for k, s in enumerate(systems)]
+    ta, sa, tb, sb = MODELS["c2c2"]
+    desc = force_desc(ta, sa, tb, sb, flags=L.FLAG_BONDED, skin=0.8, interval=0, ewald=True)
+    seeds = (101, 202)
+    with L.Context(desc, systems) as ctx:
+        ctx.set_langevin(310.0, 2.0, 1)
+        for k in range(2):
+            ctx.set_seed(k, seeds[k])
+        ctx.md_set_state(np.concatenate([s.x for s in systems]), np.concatenate(v0s))
+        ctx.md_run(25, 1.0, energies=True)
+        xb, vb, Fb, Eb = ctx.md_get_state()
+    off = 0
+    for k, s in enumerate(systems):
+        with L.Context(desc, [s]) as ctx:
+            ctx.set_langevin(310.0, 2.0, seeds[k])
+            ctx.md_set_state(s.x, v0s[k])
+            ctx.md_run(25, 1.0, energies=True)
+            x, v, F, E = ctx.md_get_state()
+        assert np.array_equal(xb[off:off + s.n], x) and np.array_equal(vb[off:off + s.n], v)
+        assert np.array_equal(Fb[off:off + s.n], F) and np.array_equal(Eb[k], E[0])
         off += s.n

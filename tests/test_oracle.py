@@ -260,6 +260,38 @@ def test_gb_isolated_atom_and_fd():
             assert abs(-(ep - em) / (2 * h) - f[a, k]) < 1e-5 * max(1.0, abs(f[a, k]))
 
 
+def test_gb_ace_matches_reference():
+    # GBBornRadii -> GBPartialSum -> GBForce -> GBACEForce, the reference's classes
+    g = T.golden("ref_gb_ace200.npz")
+    c = S.gb_chain(200, seed=int(g["seed"]))
+    assert np.array_equal(c.x, g["x"])
+    f, e, born = O.gb_evaluate(c, ace=True, ace_sigma=float(g["ace_sigma"]), ace_rho_s=float(g["ace_rho_s"]))
+    assert np.max(np.abs(f - g["f"])) <= 1e-13 * np.max(np.abs(g["f"]))
+    assert abs(e[O.COULOMB] - g["e"][O.COULOMB]) <= 1e-12 * abs(g["e"][O.COULOMB])
+    f0, e0, _ = O.gb_evaluate(c)
+    assert np.array_equal(f0, g["f_noace"]) and np.array_equal(e0, g["e_noace"])
+    # the ACE energy is the closed form sum 4 PI sigma (rho + rho_s)^2 (rho / R)^6
+    # (PI = 3.14169 as the reference writes it)
+    rho = c.gb_radius
+    ace = np.sum(4 * 3.14169 * float(g["ace_sigma"]) * (rho + 1.4) ** 2 * (rho / born) ** 6)
+    assert abs((e[0] - e0[0]) - ace) <= 1e-10 * abs(ace)
+
+
+def test_gb_ace_finite_difference():
+    c = S.gb_chain(12, seed=3)
+    f, e, _ = O.gb_evaluate(c, ace=True)
+    h = 1e-6
+    for a in (0, 5, 11):
+        for k in range(3):
+            xp = c.x.copy()
+            xp[a, k] += h
+            xm = c.x.copy()
+            xm[a, k] -= h
+            ep = O.gb_evaluate(c, x=xp, ace=True)[1][0]
+            em = O.gb_evaluate(c, x=xm, ace=True)[1][0]
+            assert abs(-(ep - em) / (2 * h) - f[a, k]) < 1e-5 * max(1.0, abs(f[a, k]))
+
+
 def test_newton_third_law_and_virial_identity():
     w = S.water_box(64, 21.0, seed=9)
     sw = O.make_switch(O.SW_C2, 8.0, 10.0)
