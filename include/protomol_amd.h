@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define PM_ABI_VERSION 3
+#define PM_ABI_VERSION 4
 
 /* status codes */
 #define PM_OK 0
@@ -80,6 +80,12 @@ extern "C" {
 #define PM_FLAG_EWALD_RECIPROCAL 16u /* with PM_TERM_EWALD_REAL: add the reciprocal-space sum over
                                         |k| < kc (NonbondedFullEwaldSystemForce::reciprocalTerm,
                                         force/nonbonded/NonbondedFullEwaldSystemForce.h:873-1083) */
+#define PM_FLAG_MOLVIRIAL 64u /* ScalarStructure::molecularVirial() on: pairs of two molecules add
+                                 f (x) (R_J - R_I) of the molecules' centres of mass to the
+                                 MOLVIRIAL slots (force/OneAtomPair.h:95-103, OneAtomPairTwo.h:87-96,
+                                 OneAtomPairNoExclusion.h:73-81, NonbondedFullEwaldSystemForce.h:
+                                 839-848, 1039-1078) and f (x) d to the VIRIAL slots even with
+                                 PM_FLAG_VIRIAL off, as the reference's addVirial does */
 
 /* ScalarStructure slots (type/ScalarStructure.h:26-67); energies[] has 36 */
 #define PM_E_COULOMB 0
@@ -87,6 +93,7 @@ extern "C" {
 #define PM_E_BOND 2
 #define PM_E_ANGLE 3
 #define PM_E_VIRIALXX 7
+#define PM_E_MOLVIRIALXX 16
 #define PM_E_NSLOTS 36
 
 typedef struct pm_switch_desc {

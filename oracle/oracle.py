@@ -55,14 +55,15 @@ class OrcCensus(C.Structure):
 class OrcEwald(C.Structure):
     _fields_ = [("alpha", C.c_double), ("accuracy", C.c_double), ("rc", C.c_double), ("rc2", C.c_double),
                 ("two_alpha_pi", C.c_double), ("cellsize", C.c_double), ("correction", C.c_int),
-                ("virial", C.c_int)]
+                ("virial", C.c_int), ("molvirial", C.c_int)]
 
 
 class OrcGB(C.Structure):
     _fields_ = [("vdw_radius", _dp), ("scaling", _dp), ("offset", C.c_double),
                 ("alpha_obc", C.c_double), ("beta_obc", C.c_double), ("gamma_obc", C.c_double),
                 ("solute_dielec", C.c_double), ("solvent_dielec", C.c_double), ("block_size", C.c_int),
-                ("ace", C.c_int), ("ace_sigma", C.c_double), ("ace_rho_s", C.c_double)]
+                ("ace", C.c_int), ("ace_sigma", C.c_double), ("ace_rho_s", C.c_double),
+                ("virial", C.c_int), ("molvirial", C.c_int)]
 
 
 _lib = None
@@ -91,7 +92,10 @@ def lib():
         L.orc_myerf.argtypes = [C.c_double]
         L.orc_myerf.restype = C.c_double
         L.orc_gb_evaluate.argtypes = [C.POINTER(OrcSystem), C.POINTER(OrcGB), _dp, _dp, _dp, _dp]
-        L.orc_bonded.argtypes = [C.POINTER(OrcSystem), C.c_int, _ip, _dp, _dp, C.c_int, _ip, _dp, _dp, _dp, _dp, _dp]
+        L.orc_bonded.argtypes = [C.POINTER(OrcSystem), C.c_int, _ip, _dp, _dp, C.c_int, _ip, _dp, _dp, _dp, _dp, _dp,
+                                 C.c_int]
+        L.orc_n_molecules.argtypes = [C.POINTER(OrcSystem)]
+        L.orc_molecule_com.argtypes = [C.POINTER(OrcSystem), _dp, _dp]
         L.orc_pair_hash.argtypes = [C.c_uint32, C.c_uint32]
         L.orc_pair_hash.restype = C.c_uint64
         _lib = L
@@ -145,7 +149,7 @@ def switch_eval(sw, r2):
 
 
 def nonbonded_cutoff(sys, term_a, sw_a, term_b=TERM_NONE, sw_b=None, algorithm_cutoff=None, cellsize=5.0,
-                     virial=False, nthreads=1, x=None, census=True):
+                     virial=False, nthreads=1, x=None, census=True, molvirial=False):
     """NonbondedCutoffSystemForce::evaluate on fresh force/energy buffers.
     Returns (forces (n,3), energies[36], (count, hash))."""
     os_ = OracleSystem(sys)
@@ -158,6 +162,7 @@ def nonbonded_cutoff(sys, term_a, sw_a, term_b=TERM_NONE, sw_b=None, algorithm_c
     nb.algorithm_cutoff = algorithm_cutoff if algorithm_cutoff else cut
     nb.cellsize = cellsize
     nb.virial = 1 if virial else 0
+    nb.molvirial = 1 if molvirial else 0
     xx = np.ascontiguousarray(sys.x if x is None else x, np.float64).reshape(-1)
     f = np.zeros(3 * sys.n)
     e = np.zeros(NSLOTS)
@@ -169,10 +174,12 @@ def nonbonded_cutoff(sys, term_a, sw_a, term_b=TERM_NONE, sw_b=None, algorithm_c
     return f.reshape(-1, 3), e, (cen.count, cen.hash)
 
 
-def ewald_real(sys, alpha, accuracy, cellsize=5.0, correction=True, virial=False, nthreads=1, x=None):
+def ewald_real(sys, alpha, accuracy, cellsize=5.0, correction=True, virial=False, nthreads=1, x=None,
+               molvirial=False):
     os_ = OracleSystem(sys)
     ew = OrcEwald()
     lib().orc_ewald_init(C.byref(ew), alpha, accuracy, cellsize, 1 if correction else 0, 1 if virial else 0)
+    ew.molvirial = 1 if molvirial else 0
     xx = np.ascontiguousarray(sys.x if x is None else x, np.float64).reshape(-1)
     f = np.zeros(3 * sys.n)
     e = np.zeros(NSLOTS)
@@ -197,11 +204,12 @@ def ewald_kvectors(box, alpha, accuracy):
     return kv, tuple(int(v) for v in hkl)
 
 
-def ewald_reciprocal(sys, alpha, accuracy, virial=False, nthreads=1, x=None):
+def ewald_reciprocal(sys, alpha, accuracy, virial=False, nthreads=1, x=None, molvirial=False):
     """Reciprocal-space Ewald term alone: forces (n, 3), 36 energy slots."""
     os_ = OracleSystem(sys)
     ew = OrcEwald()
     lib().orc_ewald_init(C.byref(ew), alpha, accuracy, 5.0, 0, 1 if virial else 0)
+    ew.molvirial = 1 if molvirial else 0
     xx = np.ascontiguousarray(sys.x if x is None else x, np.float64).reshape(-1)
     f = np.zeros(3 * sys.n)
     e = np.zeros(NSLOTS)
@@ -224,7 +232,7 @@ ACE_RHO_S = 1.4            # -watersphereradius default
 
 
 def gb_evaluate(sys, offset=0.09, a=1.0, b=0.8, g=4.85, eps_in=1.0, eps_out=78.3, x=None, ace=False,
-                ace_sigma=ACE_SIGMA, ace_rho_s=ACE_RHO_S):
+                ace_sigma=ACE_SIGMA, ace_rho_s=ACE_RHO_S, virial=False, molvirial=False):
     os_ = OracleSystem(sys)
     rho = np.ascontiguousarray(sys.gb_radius, np.float64)
     S = np.ascontiguousarray(sys.gb_scaling, np.float64)
@@ -236,6 +244,7 @@ def gb_evaluate(sys, offset=0.09, a=1.0, b=0.8, g=4.85, eps_in=1.0, eps_out=78.3
     gb.block_size = 64
     gb.ace = 1 if ace else 0
     gb.ace_sigma, gb.ace_rho_s = ace_sigma, ace_rho_s
+    gb.virial, gb.molvirial = (1 if virial else 0), (1 if molvirial else 0)
     xx = np.ascontiguousarray(sys.x if x is None else x, np.float64).reshape(-1)
     f = np.zeros(3 * sys.n)
     e = np.zeros(NSLOTS)
@@ -246,7 +255,7 @@ def gb_evaluate(sys, offset=0.09, a=1.0, b=0.8, g=4.85, eps_in=1.0, eps_out=78.3
     return f.reshape(-1, 3), e, born
 
 
-def bonded(sys, x=None):
+def bonded(sys, x=None, virial=False):
     os_ = OracleSystem(sys)
     bonds = np.ascontiguousarray(sys.bonds, np.int32)
     angles = np.ascontiguousarray(sys.angles, np.int32)
@@ -256,8 +265,18 @@ def bonded(sys, x=None):
     f = np.zeros(3 * sys.n)
     e = np.zeros(NSLOTS)
     lib().orc_bonded(C.byref(os_.c), len(bonds), _p(bonds, _ip), _p(bk), _p(br), len(angles), _p(angles, _ip),
-                     _p(ak), _p(at), _p(xx), _p(f), _p(e))
+                     _p(ak), _p(at), _p(xx), _p(f), _p(e), 1 if virial else 0)
     return f.reshape(-1, 3), e
+
+
+def molecule_com(sys, x=None):
+    """Centres of mass of the molecules, (n_molecules, 3)."""
+    os_ = OracleSystem(sys)
+    xx = np.ascontiguousarray(sys.x if x is None else x, np.float64).reshape(-1)
+    nm = lib().orc_n_molecules(C.byref(os_.c))
+    com = np.zeros(3 * max(nm, 1))
+    lib().orc_molecule_com(C.byref(os_.c), _p(xx), _p(com))
+    return com[:3 * nm].reshape(-1, 3)
 
 
 def pair_hash(lo, hi):

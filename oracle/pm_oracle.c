@@ -342,6 +342,67 @@ static inline void add_virial(double *e, const double *f, const double *d) {
   e[ORC_VIRIALZZ] += f[2] * d[2];
 }
 
+static inline void add_molvirial(double *e, const double *f, const double *d) {
+  /* ScalarStructure::addMolVirial (ScalarStructure.cpp:56-75) */
+  double xy = f[0] * d[1], xz = f[0] * d[2], yz = f[1] * d[2];
+  e[ORC_MOLVIRIALXX] += f[0] * d[0];
+  e[ORC_MOLVIRIALXY] += xy;
+  e[ORC_MOLVIRIALXZ] += xz;
+  e[ORC_MOLVIRIALYX] += xy;
+  e[ORC_MOLVIRIALYY] += f[1] * d[1];
+  e[ORC_MOLVIRIALYZ] += yz;
+  e[ORC_MOLVIRIALZX] += xz;
+  e[ORC_MOLVIRIALZY] += yz;
+  e[ORC_MOLVIRIALZZ] += f[2] * d[2];
+}
+
+int orc_n_molecules(const orc_system *sys) {
+  int m = 0;
+  for (int i = 0; i < sys->n; i++)
+    if (sys->molecule[i] + 1 > m) m = sys->molecule[i] + 1;
+  return m;
+}
+
+void orc_molecule_com(const orc_system *sys, const double *x, double *com) {
+  /* molecularCenterOfMass (TopologyUtilities.cpp:645-674) for every molecule
+   * at once: the atoms of a molecule are visited in index order, the first
+   * one starts the sum, the rest are added with Kahan's compensation */
+  int nm = orc_n_molecules(sys);
+  double *sm = (double *)calloc((size_t)(nm > 0 ? nm : 1), sizeof(double));
+  double *tc = (double *)calloc((size_t)(nm > 0 ? nm : 1) * 3, sizeof(double));
+  char *seen = (char *)calloc((size_t)(nm > 0 ? nm : 1), 1);
+  for (int k = 0; k < 3 * nm; k++) com[k] = 0.0;
+  for (int i = 0; i < sys->n; i++) {
+    int m = sys->molecule[i];
+    double mass = sys->mass[i];
+    double *c = com + 3 * m, *t = tc + 3 * m;
+    if (!seen[m]) {
+      seen[m] = 1;
+      sm[m] = mass;
+      for (int k = 0; k < 3; k++) c[k] = x[3 * i + k] * mass;
+      continue;
+    }
+    sm[m] += mass;
+    for (int k = 0; k < 3; k++) {
+      double tx = x[3 * i + k] * mass;
+      double ty = tx - t[k];
+      double tt = c[k] + ty;
+      t[k] = (tt - c[k]) - ty;
+      c[k] = tt;
+    }
+  }
+  for (int m = 0; m < nm; m++)
+    if (seen[m])
+      for (int k = 0; k < 3; k++) com[3 * m + k] = com[3 * m + k] / sm[m];
+  free(sm); free(tc); free(seen);
+}
+
+/* minimalDifference of two molecules' centres (the pair terms' comDiff) */
+static inline void com_diff(const orc_pbc *p, const double *ci, const double *cj, double *d) {
+  if (p) pbc_min_diff(p, ci, cj, d);
+  else for (int k = 0; k < 3; k++) d[k] = cj[k] - ci[k];
+}
+
 typedef struct pair_ctx {
   const orc_system *sys;
   const orc_nonbonded *nb;
@@ -352,6 +413,7 @@ typedef struct pair_ctx {
   double *e;
   orc_census *census;
   double cut2;
+  const double *com;   /* molecular centres of mass (molecular virial), 3 per molecule */
 } pair_ctx;
 
 /* OneAtomPair::doOneAtomPair (OneAtomPair.h:47-107) and
@@ -395,7 +457,14 @@ static inline void one_atom_pair(pair_ctx *c, int i, int j) {
   double fij[3] = {d[0] * fs, d[1] * fs, d[2] * fs};
   c->f[3 * i] -= fij[0]; c->f[3 * i + 1] -= fij[1]; c->f[3 * i + 2] -= fij[2];
   c->f[3 * j] += fij[0]; c->f[3 * j + 1] += fij[1]; c->f[3 * j + 2] += fij[2];
-  if (nb->virial) add_virial(c->e, fij, d);
+  /* OneAtomPair.h:95-103: pairs of two molecules add the atomic and the
+   * molecular virial when the molecular one is on, whatever virial() says */
+  if (!same && nb->molvirial) {
+    double cd[3];
+    com_diff(c->pbc, c->com + 3 * sys->molecule[i], c->com + 3 * sys->molecule[j], cd);
+    add_virial(c->e, fij, d);
+    add_molvirial(c->e, fij, cd);
+  } else if (nb->virial) add_virial(c->e, fij, d);
 }
 
 /* ------------------------------------------------------------ cell list */
@@ -630,9 +699,16 @@ int orc_nonbonded_cutoff(const orc_system *sys, const orc_nonbonded *nb, const d
   double *fbuf = nthreads > 1 ? (double *)calloc((size_t)nthreads * 3 * (size_t)sys->n, sizeof(double)) : NULL;
   double *ebuf = (double *)calloc((size_t)nthreads * ORC_NSLOTS, sizeof(double));
   orc_census *cbuf = (orc_census *)calloc((size_t)nthreads, sizeof(orc_census));
+  double *com = NULL;
   int rc = 0;
   if (!ctx || !ctxp || !ebuf || !cbuf || (nthreads > 1 && !fbuf)) { rc = -1; goto done; }
+  if (nb->molvirial) {
+    com = (double *)calloc((size_t)orc_n_molecules(sys) * 3 + 3, sizeof(double));
+    if (!com) { rc = -1; goto done; }
+    orc_molecule_com(sys, x, com);
+  }
   for (int t = 0; t < nthreads; t++) {
+    ctx[t].com = com;
     ctx[t].sys = sys; ctx[t].nb = nb; ctx[t].pbc = &pbc; ctx[t].excl = &ex; ctx[t].x = x;
     ctx[t].f = nthreads > 1 ? fbuf + (size_t)t * 3 * sys->n : f;
     ctx[t].e = nthreads > 1 ? ebuf + (size_t)t * ORC_NSLOTS : e;
@@ -651,7 +727,7 @@ int orc_nonbonded_cutoff(const orc_system *sys, const orc_nonbonded *nb, const d
   if (census)
     for (int t = 0; t < nthreads; t++) { census->count += cbuf[t].count; census->hash += cbuf[t].hash; }
 done:
-  free(ctx); free(ctxp); free(fbuf); free(ebuf); free(cbuf); free(pairs);
+  free(ctx); free(ctxp); free(fbuf); free(ebuf); free(cbuf); free(pairs); free(com);
   free(ex.tab); cells_free(&cl);
   return rc;
 }
@@ -732,6 +808,7 @@ typedef struct ewald_ctx {
   orc_census *census;
   const double *lattice; /* nlat*3, lattice[0] = 0 */
   int nlat;
+  const double *com;     /* molecular centres of mass (molecular virial) */
 } ewald_ctx;
 
 /* NonbondedFullEwaldSystemForce::realTerm inner loops (NonbondedFullEwaldSystemForce.h:787-850) */
@@ -774,7 +851,13 @@ static void ewald_cellpair(void *vctx, const orc_cells *cl, int ci, int cj) {
         double fij[3] = {rij[0] * force, rij[1] * force, rij[2] * force};
         fi[0] -= fij[0]; fi[1] -= fij[1]; fi[2] -= fij[2];
         c->f[3 * j] += fij[0]; c->f[3 * j + 1] += fij[1]; c->f[3 * j + 2] += fij[2];
-        if (c->ew->virial) add_virial(c->e, fij, rij);
+        /* :839-848 */
+        if (!same && c->ew->molvirial) {
+          double cd[3];
+          com_diff(c->pbc, c->com + 3 * mi, c->com + 3 * mj, cd);
+          add_virial(c->e, fij, rij);
+          add_molvirial(c->e, fij, cd);
+        } else if (c->ew->virial) add_virial(c->e, fij, rij);
         ex = ORC_EXCL_NONE;
       }
     }
@@ -869,8 +952,15 @@ int orc_ewald_real(const orc_system *sys, const orc_ewald *ew, const double *x, 
   double *fbuf = nthreads > 1 ? (double *)calloc((size_t)nthreads * 3 * (size_t)sys->n, sizeof(double)) : NULL;
   double *ebuf = (double *)calloc((size_t)nthreads * ORC_NSLOTS, sizeof(double));
   orc_census *cbuf = (orc_census *)calloc((size_t)nthreads, sizeof(orc_census));
+  double *com = NULL;
   if (!ctx || !ctxp || !ebuf || !cbuf || (nthreads > 1 && !fbuf)) { rc = -1; goto done; }
+  if (ew->molvirial) {
+    com = (double *)calloc((size_t)orc_n_molecules(sys) * 3 + 3, sizeof(double));
+    if (!com) { rc = -1; goto done; }
+    orc_molecule_com(sys, x, com);
+  }
   for (int t = 0; t < nthreads; t++) {
+    ctx[t].com = com;
     ctx[t].sys = sys; ctx[t].ew = ew; ctx[t].pbc = &pbc; ctx[t].excl = &ex; ctx[t].x = x;
     ctx[t].f = nthreads > 1 ? fbuf + (size_t)t * 3 * sys->n : f;
     ctx[t].e = nthreads > 1 ? ebuf + (size_t)t * ORC_NSLOTS : e;
@@ -892,7 +982,7 @@ int orc_ewald_real(const orc_system *sys, const orc_ewald *ew, const double *x, 
     for (int t = 0; t < nthreads; t++) { census->count += cbuf[t].count; census->hash += cbuf[t].hash; }
   e[ORC_COULOMB] += real + 0.0 + intra + 0.0 + point + charged;
 done:
-  free(ctx); free(ctxp); free(fbuf); free(ebuf); free(cbuf); free(pairs); free(lat);
+  free(ctx); free(ctxp); free(fbuf); free(ebuf); free(cbuf); free(pairs); free(lat); free(com);
   free(ex.tab); cells_free(&cl);
   return rc;
 }
@@ -1080,12 +1170,38 @@ int orc_ewald_reciprocal(const orc_system *sys, const orc_ewald *ew, const doubl
   for (int t = 0; t < nthreads; t++) {
     recip += c * jobs[t].energy;
     for (size_t k = 0; k < 3 * (size_t)n; k++) f[k] += jobs[t].f[k];
+    /* :1055-1078: the tensor of the k sum goes to the atomic virial and, with
+     * the molecular virial on, to that one too (it is summed only when the
+     * atomic virial is on, :1017) */
+    const double *v = jobs[t].vir;
     if (ew->virial) {
-      const double *v = jobs[t].vir;
       e[ORC_VIRIALXX] += c * v[0]; e[ORC_VIRIALXY] += c * v[1]; e[ORC_VIRIALXZ] += c * v[2];
       e[ORC_VIRIALYX] += c * v[1]; e[ORC_VIRIALYY] += c * v[3]; e[ORC_VIRIALYZ] += c * v[4];
       e[ORC_VIRIALZX] += c * v[2]; e[ORC_VIRIALZY] += c * v[4]; e[ORC_VIRIALZZ] += c * v[5];
     }
+    if (ew->molvirial && ew->virial) {
+      e[ORC_MOLVIRIALXX] += c * v[0]; e[ORC_MOLVIRIALXY] += c * v[1]; e[ORC_MOLVIRIALXZ] += c * v[2];
+      e[ORC_MOLVIRIALYX] += c * v[1]; e[ORC_MOLVIRIALYY] += c * v[3]; e[ORC_MOLVIRIALYZ] += c * v[4];
+      e[ORC_MOLVIRIALZX] += c * v[2]; e[ORC_MOLVIRIALZY] += c * v[4]; e[ORC_MOLVIRIALZZ] += c * v[5];
+    }
+  }
+  if (ew->molvirial) {
+    /* :1039-1049: each atom's reciprocal force times the vector from its
+     * minimal position to its molecule's centre; the reference adds it per k
+     * vector, here once with the atom's force summed over k */
+    double *com = (double *)calloc((size_t)orc_n_molecules(sys) * 3 + 3, sizeof(double));
+    double *fr = (double *)calloc(3 * (size_t)n + 3, sizeof(double));
+    if (!com || !fr) { free(com); free(fr); free(jobs); free(th); free(fb); free(SC); free(kv); return -1; }
+    orc_molecule_com(sys, x, com);
+    for (int t = 0; t < nthreads; t++)
+      for (size_t k = 0; k < 3 * (size_t)n; k++) fr[k] += jobs[t].f[k];
+    for (int i = 0; i < n; i++) {
+      double ria[3], mri[3];
+      pbc_min_position(&pbc, sys->origin, x + 3 * i, ria);
+      pbc_min_diff(&pbc, ria, com + 3 * sys->molecule[i], mri);
+      add_molvirial(e, fr + 3 * i, mri);
+    }
+    free(com); free(fr);
   }
   e[ORC_COULOMB] += recip;
   free(jobs); free(th); free(fb); free(SC); free(kv);
@@ -1164,6 +1280,21 @@ int orc_gb_evaluate(const orc_system *sys, const orc_gb *gb, const double *x, do
     free(burial); free(born); free(psi); free(partial); free(self); return -1;
   }
   const double *rho = gb->vdw_radius, *S = gb->scaling;
+  double *com = NULL;
+  if (gb->molvirial) {
+    com = (double *)calloc((size_t)orc_n_molecules(sys) * 3 + 3, sizeof(double));
+    if (!com) { free(burial); free(born); free(psi); free(partial); free(self); return -1; }
+    orc_molecule_com(sys, x, com);
+  }
+  /* the virials of a pair, as OneAtomPairNoExclusion.h:73-81 adds them (the
+   * Born-radius and partial-sum passes carry no force, so add nothing) */
+#define GB_PAIR_VIRIAL(fij, d)                                                     \
+  if (gb->molvirial && sys->molecule[i] != sys->molecule[j]) {                     \
+    double cd[3];                                                                  \
+    com_diff(NULL, com + 3 * sys->molecule[i], com + 3 * sys->molecule[j], cd);   \
+    add_virial(e, fij, d);                                                         \
+    add_molvirial(e, fij, cd);                                                     \
+  } else if (gb->virial) add_virial(e, fij, d);
   /* pass 1: GBBornRadii::operator() (GBBornRadii.cpp:15-128) */
   GB_FOR_PAIRS(n, bs, {
     double d[3] = {x[3 * j] - x[3 * i], x[3 * j + 1] - x[3 * i + 1], x[3 * j + 2] - x[3 * i + 2]};
@@ -1228,6 +1359,7 @@ int orc_gb_evaluate(const orc_system *sys, const orc_gb *gb, const double *x, do
     double fij[3] = {d[0] * force, d[1] * force, d[2] * force};
     f[3 * i] -= fij[0]; f[3 * i + 1] -= fij[1]; f[3 * i + 2] -= fij[2];
     f[3 * j] += fij[0]; f[3 * j + 1] += fij[1]; f[3 * j + 2] += fij[2];
+    GB_PAIR_VIRIAL(fij, d)
   })
   /* GBForce::postProcess (GBForce.cpp:230-242) */
   double tot = 0.0;
@@ -1252,6 +1384,7 @@ int orc_gb_evaluate(const orc_system *sys, const orc_gb *gb, const double *x, do
       double fij[3] = {d[0] * force, d[1] * force, d[2] * force};
       f[3 * i] -= fij[0]; f[3 * i + 1] -= fij[1]; f[3 * i + 2] -= fij[2];
       f[3 * j] += fij[0]; f[3 * j + 1] += fij[1]; f[3 * j + 2] += fij[2];
+      GB_PAIR_VIRIAL(fij, d)
     })
     /* GBACEForce::postProcess (GBACEForce.cpp:160-170): sum of the ACE
      * potentials 4 PI sigma (rho_i + rho_s)^2 (rho_i / R_i)^6 */
@@ -1262,8 +1395,9 @@ int orc_gb_evaluate(const orc_system *sys, const orc_gb *gb, const double *x, do
     }
     e[ORC_COULOMB] += ace;
   }
+#undef GB_PAIR_VIRIAL
   if (born_out) memcpy(born_out, born, sizeof(double) * (size_t)n);
-  free(burial); free(born); free(psi); free(partial); free(self);
+  free(burial); free(born); free(psi); free(partial); free(self); free(com);
   return 0;
 }
 
@@ -1271,7 +1405,7 @@ int orc_gb_evaluate(const orc_system *sys, const orc_gb *gb, const double *x, do
 
 void orc_bonded(const orc_system *sys, int nbonds, const int *bonds, const double *bond_k,
                 const double *bond_r0, int nangles, const int *angles, const double *angle_k,
-                const double *angle_t0, const double *x, double *f, double *e) {
+                const double *angle_t0, const double *x, double *f, double *e, int virial) {
   orc_pbc pbc;
   if (sys->periodic) pbc_set(&pbc, sys->box, sys->origin);
   /* BondSystemForce::calcBond (BondSystemForce.h:77-108) */
@@ -1285,6 +1419,7 @@ void orc_bonded(const orc_system *sys, int nbonds, const int *bonds, const doubl
     double f1[3] = {r12[0] * (-dp / r), r12[1] * (-dp / r), r12[2] * (-dp / r)};
     for (int k = 0; k < 3; k++) { f[3 * a1 + k] += f1[k]; f[3 * a2 + k] -= f1[k]; }
     e[ORC_BOND] += bond_k[b] * (r - bond_r0[b]) * (r - bond_r0[b]);
+    if (virial) add_virial(e, f1, r12);  /* BondSystemForce.h:121-122 */
   }
   /* AngleSystemForce::calcAngle (AngleSystemForce.h:66-140), no Urey-Bradley */
   for (int a = 0; a < nangles; a++) {
@@ -1317,6 +1452,20 @@ void orc_bonded(const orc_system *sys, int nbonds, const int *bonds, const doubl
       f[3 * a3 + k] += f3[k];
     }
     e[ORC_ANGLE] += angle_k[a] * (theta - angle_t0[a]) * (theta - angle_t0[a]);
+    if (virial) {  /* AngleSystemForce.h:155-172 */
+      double xy = f1[0] * r12[1] + f3[0] * r32[1];
+      double xz = f1[0] * r12[2] + f3[0] * r32[2];
+      double yz = f1[1] * r12[2] + f3[1] * r32[2];
+      e[ORC_VIRIALXX] += f1[0] * r12[0] + f3[0] * r32[0];
+      e[ORC_VIRIALXY] += xy;
+      e[ORC_VIRIALXZ] += xz;
+      e[ORC_VIRIALYX] += xy;
+      e[ORC_VIRIALYY] += f1[1] * r12[1] + f3[1] * r32[1];
+      e[ORC_VIRIALYZ] += yz;
+      e[ORC_VIRIALZX] += xz;
+      e[ORC_VIRIALZY] += yz;
+      e[ORC_VIRIALZZ] += f1[2] * r12[2] + f3[2] * r32[2];
+    }
   }
 }
 

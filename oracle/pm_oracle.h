@@ -27,7 +27,9 @@ enum {
   ORC_VIRIALXX = 7, ORC_VIRIALXY = 8, ORC_VIRIALXZ = 9,
   ORC_VIRIALYX = 10, ORC_VIRIALYY = 11, ORC_VIRIALYZ = 12,
   ORC_VIRIALZX = 13, ORC_VIRIALZY = 14, ORC_VIRIALZZ = 15,
-  ORC_MOLVIRIALXX = 16,
+  ORC_MOLVIRIALXX = 16, ORC_MOLVIRIALXY = 17, ORC_MOLVIRIALXZ = 18,
+  ORC_MOLVIRIALYX = 19, ORC_MOLVIRIALYY = 20, ORC_MOLVIRIALYZ = 21,
+  ORC_MOLVIRIALZX = 22, ORC_MOLVIRIALZY = 23, ORC_MOLVIRIALZZ = 24,
   ORC_NSLOTS = 36
 };
 
@@ -101,6 +103,7 @@ typedef struct orc_ewald {
   double cellsize;
   int correction;            /* add correction + point self + charged system */
   int virial;
+  int molvirial;             /* ScalarStructure::molecularVirial() */
 } orc_ewald;
 
 /* GB OBC parameters (ProtoMolApp.cpp:285-301, GBForce.cpp:11-12). */
@@ -113,6 +116,7 @@ typedef struct orc_gb {
   int block_size;            /* NonbondedSimpleFullSystemForce default 64 */
   int ace;                   /* add GBACEForce (force/GB/GBACEForce.cpp) */
   double ace_sigma, ace_rho_s; /* -solvationparam, -watersphereradius */
+  int virial, molvirial;     /* ScalarStructure::virial() / molecularVirial() */
 } orc_gb;
 
 uint64_t orc_pair_hash(uint32_t lo, uint32_t hi);
@@ -120,6 +124,13 @@ uint64_t orc_pair_hash(uint32_t lo, uint32_t hi);
 void orc_switch_init(orc_switch *s, int kind, double switchon, double cutoff,
                      double switchoff, double order);
 void orc_switch_eval(const orc_switch *s, double r2, double *value, double *deriv);
+
+/* Molecule count (largest molecule id + 1) and the molecules' centres of
+ * mass, com[3 m + k] (buildMolecularCenterOfMass, TopologyUtilities.cpp:677-684,
+ * with molecularCenterOfMass :645-674: Kahan-compensated mass-weighted sums
+ * over the molecule's atoms in index order). */
+int orc_n_molecules(const orc_system *sys);
+void orc_molecule_com(const orc_system *sys, const double *x, double *com);
 
 /* Minimal image difference, PeriodicBoundaryConditions.h:72-110 (orthogonal). */
 double orc_min_diff(const orc_system *sys, const double *c1, const double *c2, double *d);
@@ -160,11 +171,12 @@ int orc_gb_evaluate(const orc_system *sys, const orc_gb *gb, const double *x,
                     double *f, double *e, double *born_out);
 
 /* Harmonic bonds/angles (force/bonded/BondSystemForce.h, AngleSystemForce.h).
- * bonds: nb*2 atoms, bond_k, bond_r0; angles: na*3 atoms, angle_k, angle_t0 (rad). */
+ * bonds: nb*2 atoms, bond_k, bond_r0; angles: na*3 atoms, angle_k, angle_t0 (rad).
+ * virial: add the atomic virial (BondSystemForce.h:121-122, AngleSystemForce.h:155-172). */
 void orc_bonded(const orc_system *sys, int nbonds, const int *bonds, const double *bond_k,
                 const double *bond_r0, int nangles, const int *angles,
                 const double *angle_k, const double *angle_t0, const double *x,
-                double *f, double *e);
+                double *f, double *e, int virial);
 
 /* NR ran1 (Random.cpp:33-68) and Box-Muller Gaussian (MathUtilities.cpp:206-236). */
 typedef struct orc_rng {

@@ -14,6 +14,7 @@
 #include <protomol/topology/CubicCellManager.h>
 #include <protomol/topology/CellListEnumeratorPeriodicBoundaries.h>
 #include <protomol/topology/CellListEnumeratorStandard.h>
+#include <protomol/topology/TopologyUtilities.h>
 #include <protomol/force/nonbonded/NonbondedCutoffSystemForce.h>
 #include <protomol/force/nonbonded/NonbondedFullEwaldSystemForce.h>
 #include <protomol/force/nonbonded/NonbondedSimpleFullSystemForce.h>
@@ -168,6 +169,13 @@ static void fill_common(T &topo, Recs &r, int n) {
   topo.exclusions.optimize();
 }
 
+// the molecules' centres of mass, as the integrators build them before each
+// force evaluation (STSIntegrator.cpp:51); the molecular virial reads them
+template <class T>
+static void molecule_positions(T &topo, Recs &r, Vector3DBlock &pos) {
+  if (geti(r, "molvirial", 0)) buildMolecularCenterOfMass(&pos, &topo);
+}
+
 static C1SwitchingFunction mkC1(const double *s) { return C1SwitchingFunction(s[2]); }
 static C2SwitchingFunction mkC2(const double *s) { return C2SwitchingFunction(s[1], s[2]); }
 static CnSwitchingFunction mkCn(const double *s) { return CnSwitchingFunction(s[4], s[1], s[3], s[2]); }
@@ -280,6 +288,7 @@ static int do_gb(Recs &r, Vector3DBlock &pos, Vector3DBlock &f, ScalarStructure 
   int n = (int)pos.size();
   VTopo topo(1.0, ExclusionType(ExclusionType::ONE3), VBC(), CCM(geti(r, "cellsize", 5)), 16.0f);
   fill_common(topo, r, n);
+  molecule_positions(topo, r, pos);
   const std::vector<double> &gp = r["gb_params"].d;  // offset, a, b, g, eps_in, eps_out
   topo.doGBSAOpenMM = 1;
   topo.obcType = 2;
@@ -356,8 +365,7 @@ int main(int argc, char **argv) {
   ScalarStructure e;
   e.clear();
   e.virial(geti(r, "virial", 0) != 0);
-  // the molecular virial (on by default in ScalarStructure) is not part of
-  // the path under test
+  // the molecular virial is on by default in ScalarStructure; the record says
   e.molecularVirial(geti(r, "molvirial", 0) != 0);
   std::vector<double> born;
   int rc = -1;
@@ -367,6 +375,7 @@ int main(int argc, char **argv) {
     PBC bc(Vector3D(box[0], 0, 0), Vector3D(0, box[1], 0), Vector3D(0, 0, box[2]), Vector3D(org[0], org[1], org[2]));
     PTopo topo(getd(r, "csf", 1.0), ExclusionType(ExclusionType::ONE3), bc, CCM(getd(r, "cellsize", 5.0)), 16.0f);
     fill_common(topo, r, n);
+    molecule_positions(topo, r, pos);
     if (mode == 1) rc = do_cutoff(r, topo, pos, f, e);
     else if (mode == 2) rc = do_ewald(r, topo, pos, f, e);
     else if (mode == 4) rc = do_bonded(r, topo, pos, f, e);

@@ -135,6 +135,9 @@ struct SysPrep {
   std::vector<int> ew_kmap, ew_fb_rows;
   int ew_nk = 0, ew_xrows = 0, ew_yrows = 0, ew_zrows = 0, ew_chunk = 0, ew_nchunks = 0;
   double ew_c = 0, ew_a8 = 0, ew_fac = 0;
+  // molecules (molecular virial): CSR of their atoms in index order
+  int n_mol = 0;
+  std::vector<int> mol_start, mol_atoms;
 };
 
 struct SysHost {
@@ -275,6 +278,7 @@ struct pm_ctx {
   ForceDev P{};
   bool gb = false;
   bool recip = false;       // PM_FLAG_EWALD_RECIPROCAL
+  bool molvir = false;      // PM_FLAG_MOLVIRIAL
   // LangevinLeapfrogIntegrator (gamma_ps = 0: plain leapfrog)
   double lv_temperature = 0.0, lv_gamma_ps = 0.0;
   unsigned lv_seed = 1234;
@@ -494,9 +498,13 @@ SysPrep prepare(const pm_ctx *c, const pm_system_desc &d) {
   }
   // pair -> class of its last add
   std::stable_sort(ex.begin(), ex.end(), [](const Ex &x, const Ex &y) { return x.a != y.a ? x.a < y.a : x.b < y.b; });
+  // the pair terms look exclusions up only for atoms of one molecule
+  // (OneAtomPair.h:64-68), so a pair of two molecules is never excluded
   std::vector<Ex> last;
   for (size_t i = 0; i < ex.size(); ++i)
-    if (i + 1 == ex.size() || ex[i + 1].a != ex[i].a || ex[i + 1].b != ex[i].b) last.push_back(ex[i]);
+    if ((i + 1 == ex.size() || ex[i + 1].a != ex[i].a || ex[i + 1].b != ex[i].b) &&
+        d.molecule[ex[i].a] == d.molecule[ex[i].b])
+      last.push_back(ex[i]);
   p.excl_start.assign(n + 1, 0);
   for (const Ex &e : last) {
     p.excl_start[e.a + 1]++;
@@ -547,6 +555,20 @@ SysPrep prepare(const pm_ctx *c, const pm_system_desc &d) {
   for (int i = 0; i < n; ++i) {
     if (!(d.mass[i] > 0)) throw Fail{PM_EINVAL, "masses must be positive"};
     p.invm[i] = 1.0 / d.mass[i];
+  }
+  if (c->molvir) {
+    // molecule ids index the molecules (GenericTopology::molecules)
+    for (int i = 0; i < n; ++i) {
+      if (d.molecule[i] < 0) throw Fail{PM_EINVAL, "molecule ids must be >= 0"};
+      p.n_mol = std::max(p.n_mol, d.molecule[i] + 1);
+    }
+    if (p.n_mol > n) throw Fail{PM_EINVAL, "molecule ids must be < n_atoms"};
+    p.mol_start.assign(p.n_mol + 1, 0);
+    for (int i = 0; i < n; ++i) p.mol_start[d.molecule[i] + 1]++;
+    for (int m = 0; m < p.n_mol; ++m) p.mol_start[m + 1] += p.mol_start[m];
+    p.mol_atoms.resize(n);
+    std::vector<int> fill(p.mol_start.begin(), p.mol_start.end() - 1);
+    for (int i = 0; i < n; ++i) p.mol_atoms[fill[d.molecule[i]]++] = i;
   }
   if (d.lj_mode == PM_LJ_PERATOM) {
     if (!d.lj_sigma || !d.lj_epsilon) throw Fail{PM_EINVAL, "per-atom LJ needs sigma and epsilon"};
@@ -665,6 +687,13 @@ size_t carve(const pm_ctx *c, const SysPrep &p, const pm_system_desc &d, Carver 
   S.fx = cv.take<long long>(3 * (size_t)S.f_stride);
   S.fy = S.fx ? S.fx + S.f_stride : nullptr;
   S.fz = S.fx ? S.fx + 2 * (size_t)S.f_stride : nullptr;
+  if (c->molvir) {
+    S.n_mol = p.n_mol;
+    S.mol_start = cv.take<int>(p.mol_start.size());
+    S.mol_atoms = cv.take<int>(p.mol_atoms.size());
+    S.mass = cv.take<double>(n);
+    S.com = cv.take<double>(3 * (size_t)p.n_mol);
+  }
   if (c->gb) {
     S.gb_rho = cv.take<double>(n);
     S.gb_S = cv.take<double>(n);
@@ -842,6 +871,11 @@ int add_system(pm_ctx *c, const pm_system_desc &d) {
     upload(const_cast<double *>(S.charge), d.scaled_charge, n);
     upload(const_cast<double *>(S.inv_mass), p.invm.data(), n);
     upload(const_cast<int *>(S.molecule), d.molecule, n);
+    if (c->molvir) {
+      upload(const_cast<int *>(S.mol_start), p.mol_start.data(), p.mol_start.size());
+      upload(const_cast<int *>(S.mol_atoms), p.mol_atoms.data(), p.mol_atoms.size());
+      upload(const_cast<double *>(S.mass), d.mass, n);
+    }
     if (d.lj_mode == PM_LJ_PERATOM) {
       upload(const_cast<float *>(S.sig_half), p.sig_half.data(), n);
       upload(const_cast<float *>(S.eps_sqrt), p.eps_sqrt.data(), n);
@@ -957,6 +991,7 @@ LaunchSet &get_set(pm_ctx *c, const std::vector<int> &ids) {
   L.n_gb_tiles_total = pre[6].back();
   L.n_local_total = pre[7].back();
   L.n_sc_total = pre[8].back();
+  for (int k = 0; k < ns; ++k) L.max_mol = std::max(L.max_mol, c->sys[ids[k]]->dev.n_mol);
   if (c->recip) {
     for (int k = 0; k < ns; ++k) {
       pm::EwLaunch e = c->sys[ids[k]]->ew;
@@ -1010,16 +1045,24 @@ hipEvent_t next_event(pm_ctx *c) {
 // (MD steps, whose kick has consumed and zeroed the forces) leaves the rebuild
 // decision to the device.  kind: pm::kPair* (what the pair kernel adds).
 // gb_energies: the GB passes also sum energies (GB has no separate energy pass).
+// reciprocal-space energy-pass flags: the virial bits the context asks for
+int ewald_energy_mode(const pm_ctx *c) {
+  return pm::kEwEnergies | (c->P.virial ? pm::kEwVirial : 0) | (c->molvir ? pm::kEwMolVirial : 0);
+}
+
 void force_eval(pm_ctx *c, const Launch &L, int mode, const StepPolicy &pol, int kind, bool gb_energies, bool timed) {
   pm::launch_step_begin(L, mode, pol.lim2, pol.kmax);
   const bool pool = !timed && (c->desc.flags & PM_FLAG_TIMING);
   if (mode & pm::kStepForce) pm::launch_zero_forces(L);
-  const bool local_en = kind == pm::kPairForcesEnergies || (c->gb && gb_energies);
+  const bool en = kind == pm::kPairForcesEnergies;
+  const bool local_en = en || (c->gb && gb_energies);
   const int local = pm::kLocalForces | (local_en ? pm::kLocalEnergies : 0);
+  // the molecular virial reads the molecules' centres at these positions
+  if (c->molvir && local_en) pm::launch_mol_com(L);
   if (c->gb) {
     if (timed) HIPCHECK(hipEventRecord(c->evp0, c->stream));
     if (pool) HIPCHECK(hipEventRecord(next_event(c), c->stream));
-    pm::launch_gb(L, c->P, kind == pm::kPairForcesEnergies || gb_energies);
+    pm::launch_gb(L, c->P, en || gb_energies);
     if (pool) HIPCHECK(hipEventRecord(next_event(c), c->stream));
     if (timed) HIPCHECK(hipEventRecord(c->evp1, c->stream));
     pm::launch_local(L, c->P, local);
@@ -1028,12 +1071,13 @@ void force_eval(pm_ctx *c, const Launch &L, int mode, const StepPolicy &pol, int
   pm::launch_rebuild(L, pol.rlist * pol.rlist);
   if (timed) HIPCHECK(hipEventRecord(c->evp0, c->stream));
   if (pool) HIPCHECK(hipEventRecord(next_event(c), c->stream));
-  if (!pm::launch_pair(L, c->P, kind)) throw Fail{PM_EINVAL, "unsupported term combination"};
+  if (!pm::launch_pair(L, c->P, en && c->molvir ? pm::kPairForcesEnergiesMV : kind))
+    throw Fail{PM_EINVAL, "unsupported term combination"};
   if (pool) HIPCHECK(hipEventRecord(next_event(c), c->stream));
   if (timed) HIPCHECK(hipEventRecord(c->evp1, c->stream));
   if (c->recip && kind != pm::kPairCensus) {
     if (timed) HIPCHECK(hipEventRecord(c->evr0, c->stream));
-    pm::launch_ewald(L, pm::kEwForces | (kind == pm::kPairForcesEnergies ? pm::kEwEnergies : 0));
+    pm::launch_ewald(L, pm::kEwForces | (en ? ewald_energy_mode(c) : 0));
     if (timed) HIPCHECK(hipEventRecord(c->evr1, c->stream));
   }
   pm::launch_local(L, c->P, local);
@@ -1043,8 +1087,11 @@ void force_eval(pm_ctx *c, const Launch &L, int mode, const StepPolicy &pol, int
 // (no forces): the pair kernel's energy-only form and the local terms' sums
 void energy_pass(pm_ctx *c, const Launch &L) {
   if (c->gb) return;  // GB sums energies inside its force passes
-  if (!pm::launch_pair(L, c->P, pm::kPairEnergies)) throw Fail{PM_EINVAL, "unsupported term combination"};
-  if (c->recip) pm::launch_ewald(L, pm::kEwEnergies);  // from the structure factors of the force pass
+  if (c->molvir) pm::launch_mol_com(L);
+  if (!pm::launch_pair(L, c->P, c->molvir ? pm::kPairEnergiesMV : pm::kPairEnergies))
+    throw Fail{PM_EINVAL, "unsupported term combination"};
+  // from the structure factors and partial forces of the force pass
+  if (c->recip) pm::launch_ewald(L, ewald_energy_mode(c));
   pm::launch_local(L, c->P, pm::kLocalEnergies);
 }
 
@@ -1062,11 +1109,25 @@ void add_energies(const pm_ctx *c, const SysHost &h, const long long *acc, doubl
   e[PM_E_LENNARDJONES] += (double)acc[pm::kAccLJ] * k;
   e[PM_E_BOND] += (double)acc[pm::kAccBond] * k;
   e[PM_E_ANGLE] += (double)acc[pm::kAccAngle] * k;
-  if (c->desc.flags & PM_FLAG_VIRIAL) {
-    const double xx = (double)acc[pm::kAccVxx] * k + h.vir_const, xy = (double)acc[pm::kAccVxy] * k,
-                 xz = (double)acc[pm::kAccVxz] * k, yy = (double)acc[pm::kAccVyy] * k + h.vir_const,
-                 yz = (double)acc[pm::kAccVyz] * k, zz = (double)acc[pm::kAccVzz] * k + h.vir_const;
+  // the atomic virial slots: everything with the virial on; with only the
+  // molecular virial on, the pairs of two molecules (the reference's
+  // three-argument addVirial writes both)
+  const bool vir = (c->desc.flags & PM_FLAG_VIRIAL) != 0;
+  if (vir || c->molvir) {
+    const double vc = vir ? h.vir_const : 0.0;
+    const double xx = (double)acc[pm::kAccVxx] * k + vc, xy = (double)acc[pm::kAccVxy] * k,
+                 xz = (double)acc[pm::kAccVxz] * k, yy = (double)acc[pm::kAccVyy] * k + vc,
+                 yz = (double)acc[pm::kAccVyz] * k, zz = (double)acc[pm::kAccVzz] * k + vc;
     double *v = e + PM_E_VIRIALXX;
+    v[0] += xx; v[1] += xy; v[2] += xz;
+    v[3] += xy; v[4] += yy; v[5] += yz;
+    v[6] += xz; v[7] += yz; v[8] += zz;
+  }
+  if (c->molvir) {
+    const double xx = (double)acc[pm::kAccMVxx] * k, xy = (double)acc[pm::kAccMVxy] * k,
+                 xz = (double)acc[pm::kAccMVxz] * k, yy = (double)acc[pm::kAccMVyy] * k,
+                 yz = (double)acc[pm::kAccMVyz] * k, zz = (double)acc[pm::kAccMVzz] * k;
+    double *v = e + PM_E_MOLVIRIALXX;
     v[0] += xx; v[1] += xy; v[2] += xz;
     v[3] += xy; v[4] += yy; v[5] += yz;
     v[6] += xz; v[7] += yz; v[8] += zz;
@@ -1437,6 +1498,9 @@ int pm_ctx_create(const pm_force_desc *force, int32_t n_systems, const pm_system
       throw Fail{PM_EINVAL, "cell_scale must be 0 (default) or within 0.6 .. 2"};
     c->gb = (force->flags & PM_FLAG_GB) != 0;
     c->recip = (force->flags & PM_FLAG_EWALD_RECIPROCAL) != 0;
+    c->molvir = (force->flags & PM_FLAG_MOLVIRIAL) != 0;
+    P.virial = (force->flags & PM_FLAG_VIRIAL) ? 1 : 0;
+    P.molvirial = c->molvir ? 1 : 0;
     if ((force->flags & PM_FLAG_GB_ACE) && !c->gb) throw Fail{PM_EINVAL, "PM_FLAG_GB_ACE needs PM_FLAG_GB"};
     if (c->recip && (c->gb || force->term_b != PM_TERM_EWALD_REAL))
       throw Fail{PM_EINVAL, "PM_FLAG_EWALD_RECIPROCAL needs term_b = PM_TERM_EWALD_REAL (and no GB)"};

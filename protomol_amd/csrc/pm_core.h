@@ -43,7 +43,7 @@ constexpr int kShiftCenter = 13;   // (0,0,0) image in the 27-shift encoding
 
 // energy accumulator slots (int64 fixed point, per system)
 enum { kAccLJ = 0, kAccCoul = 1, kAccVxx, kAccVxy, kAccVxz, kAccVyy, kAccVyz, kAccVzz,
-       kAccBond, kAccAngle, kAccN = 16 };
+       kAccBond, kAccAngle, kAccMVxx, kAccMVxy, kAccMVxz, kAccMVyy, kAccMVyz, kAccMVzz, kAccN = 16 };
 // The accumulators exist in kAccCopies copies of one 128-byte line each; a
 // wave adds into copy (wave index mod kAccCopies), k_gather_scalars sums the
 // copies.  Integer sums are exact, so the totals do not depend on the split,
@@ -65,8 +65,10 @@ enum { kXferX = 1, kXferV = 2, kXferF = 4 };
 // packed per-system scalars (k_gather_scalars): acc[0..16), census[16..18), err,
 // ctl[19..23), largest squared displacement of the run (float bits)
 enum { kScalCensus = 16, kScalErr = 18, kScalCtl = 19, kScalMaxdispRun = 25, kScalN = 26 };
-// what a pair-kernel launch accumulates
-enum { kPairForces = 0, kPairForcesEnergies = 1, kPairEnergies = 2, kPairCensus = 3 };
+// what a pair-kernel launch accumulates; the MV kinds also sum the molecular
+// virial (and the atomic one as the reference does with it on)
+enum { kPairForces = 0, kPairForcesEnergies = 1, kPairEnergies = 2, kPairCensus = 3, kPairForcesEnergiesMV = 4,
+       kPairEnergiesMV = 5 };
 
 // per-system control words: [0] rebuild the pair list in this step (0/1),
 // [1] steps since the last rebuild, [2] rebuilds since the start of the run,
@@ -107,6 +109,7 @@ struct ForceDev {
   // reference's PI = 3.14169
   int gb_ace;
   double gb_ace_sigma, gb_ace_rho_s, gb_ace_c1;
+  int virial, molvirial;   // ScalarStructure::virial() / molecularVirial()
 };
 
 // One system as the kernels see it.  All pointers are device pointers owned
@@ -185,6 +188,12 @@ struct SysDev {
   unsigned *maxdisp_run;   // the same, folded over every step of the current run
   int *ctl;                // kCtlN control words
   unsigned lv_seed;        // Langevin noise stream key
+  // molecules (molecular virial): CSR of their atoms in index order, centres
+  // of mass (SoA, 3 x n_mol) at the positions of the current evaluation
+  int n_mol;
+  const int *mol_start, *mol_atoms;
+  const double *mass;      // n, the masses the centres are weighted by
+  double *com;
   // Ewald reciprocal space (NonbondedFullEwaldSystemForce::reciprocalTerm):
   // k = 2 pi (h/Lx, k/Ly, l/Lz), h >= 0 half space, |k| < kc, in the
   // reference's (h, k, l) order; rows = runs of l at fixed (h, k)

@@ -69,13 +69,10 @@ __device__ __forceinline__ double2 ew_phase(const double2 *rec, int idx) {
 // Per-atom phase tables, as NonbondedFullEwaldSystemForce.h:900-945:
 // r = minimalPosition(x) (PeriodicBoundaryConditions.h:113-147), t = r 2 pi / L,
 // (cos, sin)(n t) by the addition theorem, the x table scaled by the charge.
-__global__ __launch_bounds__(256) void k_ew_tables(const SysDev *__restrict__ sys) {
+// PeriodicBoundaryConditions: the wrap shared by minimalPosition (:113-147,
+// after the origin is taken off) and minimalDifference (:72-110)
+__device__ __forceinline__ void ew_wrap(const SysDev &S, double *d) {
 #pragma clang fp contract(off)
-  const SysDev &S = *sys;
-  const int n = S.n_atoms;
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  double d[3] = {S.x[j] - S.ew_org[0], S.x[n + j] - S.ew_org[1], S.x[2 * n + j] - S.ew_org[2]};
   if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] > S.D) {
     for (int k = 0; k < 3; ++k) {
       if (d[k] < -S.H2[k]) d[k] += S.H[k];
@@ -84,6 +81,24 @@ __global__ __launch_bounds__(256) void k_ew_tables(const SysDev *__restrict__ sy
     if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] > S.D)
       for (int k = 0; k < 3; ++k) d[k] = d[k] - S.L[k] * rint(S.Lr[k] * d[k]);
   }
+}
+
+__device__ __forceinline__ void ew_min_position(const SysDev &S, int j, double *d) {
+  const int n = S.n_atoms;
+  d[0] = S.x[j] - S.ew_org[0];
+  d[1] = S.x[n + j] - S.ew_org[1];
+  d[2] = S.x[2 * n + j] - S.ew_org[2];
+  ew_wrap(S, d);
+}
+
+__global__ __launch_bounds__(256) void k_ew_tables(const SysDev *__restrict__ sys) {
+#pragma clang fp contract(off)
+  const SysDev &S = *sys;
+  const int n = S.n_atoms;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double d[3];
+  ew_min_position(S, j, d);
   const double q = S.charge[j];
   double2 *rec = S.ew_tab + (size_t)j * S.ew_rl;
   double2 *sec[3] = {rec, rec + S.ew_xrows, rec + S.ew_xrows + S.ew_yrows};
@@ -352,9 +367,44 @@ __global__ __launch_bounds__(256) void k_ew_fsum(const SysDev *__restrict__ sys)
   S.fz[sl] += __double2ll_rn(f[2] * kFScale);
 }
 
+// The reciprocal term's share of the molecular virial
+// (NonbondedFullEwaldSystemForce.h:1039-1049): each atom's reciprocal force
+// (summed over k here, per k there) times minimalDifference(minimalPosition(x),
+// R) to its molecule's centre R.  One thread per atom; the per-atom products
+// go to fixed point before they are summed, so the total does not depend on
+// the grouping.
+__global__ __launch_bounds__(256) void k_ew_molvir(const SysDev *__restrict__ sys) {
+  const SysDev &S = *sys;
+  const int n = S.n_atoms;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  long long q[6] = {0, 0, 0, 0, 0, 0};
+  if (j < n) {
+    double f[3] = {0.0, 0.0, 0.0};
+    for (int b = 0; b < S.ew_nfb; ++b)
+      for (int k = 0; k < 3; ++k) f[k] += S.ew_pf[((size_t)b * 3 + k) * n + j];
+    double d[3];
+    ew_min_position(S, j, d);
+    const int m = S.molecule[j], nm = S.n_mol;
+    double r[3] = {S.com[m] - d[0], S.com[nm + m] - d[1], S.com[2 * nm + m] - d[2]};
+    ew_wrap(S, r);
+    const double v[6] = {f[0] * r[0], f[0] * r[1], f[0] * r[2], f[1] * r[1], f[1] * r[2], f[2] * r[2]};
+    for (int k = 0; k < 6; ++k) q[k] = __double2ll_rn(v[k] * kEScale);
+  }
+  for (int k = 0; k < 6; ++k)
+    for (int o = 32; o > 0; o >>= 1) q[k] += __shfl_xor(q[k], o);
+  if ((threadIdx.x & 63) == 0) {
+    long long *acc = acc_copy(S.acc, (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    const int slot[6] = {kAccMVxx, kAccMVxy, kAccMVxz, kAccMVyy, kAccMVyz, kAccMVzz};
+    for (int k = 0; k < 6; ++k)
+      if (q[k]) atomicAdd(reinterpret_cast<unsigned long long *>(&acc[slot[k]]), static_cast<unsigned long long>(q[k]));
+  }
+}
+
 // c sum_k b |S|^2 and the virial (NonbondedFullEwaldSystemForce.h:1028-1038,
-// 1053-1067): one workgroup, strided partial sums then a fixed tree
-__global__ __launch_bounds__(1024) void k_ew_energy(const SysDev *__restrict__ sys) {
+// 1053-1078): one workgroup, strided partial sums then a fixed tree.  The
+// virial goes to the atomic slots with kEwVirial, and also to the molecular
+// ones with kEwMolVirial (the reference sums it only with the atomic virial on)
+__global__ __launch_bounds__(1024) void k_ew_energy(const SysDev *__restrict__ sys, int mode) {
   __shared__ double red[7][1024];
   const SysDev &S = *sys;
   const int t = threadIdx.x;
@@ -383,9 +433,13 @@ __global__ __launch_bounds__(1024) void k_ew_energy(const SysDev *__restrict__ s
     long long *acc = S.acc;
     const double c = S.ew_c;
     const int slot[7] = {kAccCoul, kAccVxx, kAccVxy, kAccVxz, kAccVyy, kAccVyz, kAccVzz};
-    for (int m = 0; m < 7; ++m)
-      atomicAdd(reinterpret_cast<unsigned long long *>(&acc[slot[m]]),
-                static_cast<unsigned long long>(__double2ll_rn(c * red[m][0] * kEScale)));
+    const int mslot[7] = {-1, kAccMVxx, kAccMVxy, kAccMVxz, kAccMVyy, kAccMVyz, kAccMVzz};
+    for (int m = 0; m < 7; ++m) {
+      const unsigned long long q = static_cast<unsigned long long>(__double2ll_rn(c * red[m][0] * kEScale));
+      if (m == 0 || (mode & kEwVirial)) atomicAdd(reinterpret_cast<unsigned long long *>(&acc[slot[m]]), q);
+      if (m > 0 && (mode & kEwVirial) && (mode & kEwMolVirial))
+        atomicAdd(reinterpret_cast<unsigned long long *>(&acc[mslot[m]]), q);
+    }
   }
 }
 
@@ -402,7 +456,9 @@ void launch_ewald(const Launch &L, int mode) {
                          dim3(kEwFThreads), 0, L.stream, E.sys);
       hipLaunchKernelGGL(k_ew_fsum, dim3((E.n + 255) / 256), dim3(256), 0, L.stream, E.sys);
     }
-    if (mode & kEwEnergies) hipLaunchKernelGGL(k_ew_energy, dim3(1), dim3(1024), 0, L.stream, E.sys);
+    if (mode & kEwMolVirial)
+      hipLaunchKernelGGL(k_ew_molvir, dim3((E.n + 255) / 256), dim3(256), 0, L.stream, E.sys);
+    if (mode & kEwEnergies) hipLaunchKernelGGL(k_ew_energy, dim3(1), dim3(1024), 0, L.stream, E.sys, mode);
   }
 }
 

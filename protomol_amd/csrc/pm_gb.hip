@@ -108,6 +108,7 @@ struct GBShared {
   double x[kGBBlock], y[kGBBlock], z[kGBBlock];
   float q[kGBBlock], ori[kGBBlock], S[kGBBlock], B[kGBBlock], iB[kGBBlock], qB2[kGBBlock], dr[kGBBlock],
       part[kGBBlock], sg[kGBBlock], ep[kGBBlock], ak[kGBBlock];
+  int mol[kGBBlock];
 };
 
 __device__ void load_chunk(GBShared &sh, const SysDev &S, const ForceDev &P, int j0, int pass) {
@@ -127,6 +128,7 @@ __device__ void load_chunk(GBShared &sh, const SysDev &S, const ForceDev &P, int
       sh.qB2[t] = 0.5f * q * q * iB * iB;  // q^2 / (2 B^2)
     }
     if (pass >= 3) {
+      sh.mol[t] = S.molecule[j];
       sh.dr[t] = (float)S.gb_drfac[j];
       sh.part[t] = (float)S.gb_partial[j];
       sh.ak[t] = P.gb_ace ? (float)S.gb_spare[j] : 0.f;
@@ -240,10 +242,15 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_partial(SysDev *sys, const int 
 
 // pass 3: GB pair force and energy (GBForce.cpp:14-175), self energy
 // (GBForce.cpp:50-60, 230-242), plus the all-pairs Lennard-Jones and
-// Coulomb terms (OneAtomPair<Vacuum, Universal, ...> with the exclusion table)
+// Coulomb terms (OneAtomPair<Vacuum, Universal, ...> with the exclusion table).
+// VIR: with the energies, the virials of each pair i < j, counted by i's lane
+// (OneAtomPairNoExclusion.h:73-81, OneAtomPair.h:95-103): f (x) d to the
+// atomic virial, and f (x) (R_J - R_I) to the molecular one for pairs of two
+// molecules
+template <bool VIR>
 __global__ __launch_bounds__(kGBBlock) void k_gb_force(SysDev *sys, const int *gp, int nsys, ForceDev P, int energies) {
   __shared__ GBShared sh;
-  __shared__ double red[4][kGBBlock / 64];
+  __shared__ double red[VIR ? 15 : 3][kGBBlock / 64];
   const int tile = blockIdx.x;
   if (tile >= gp[nsys]) return;
   const int s = find_sys_gb(gp, nsys, tile);
@@ -283,6 +290,8 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_force(SysDev *sys, const int *g
   const float aki = (vi && P.gb_ace) ? (float)S.gb_spare[i] : 0.f;
   const float ace_c1 = (float)P.gb_ace_c1;
   double fx = 0.0, fy = 0.0, fz = 0.0, egb = 0.0, elj = 0.0, eco = 0.0;
+  double vir[6] = {0, 0, 0, 0, 0, 0}, mvir[6] = {0, 0, 0, 0, 0, 0};
+  const int moli = vi ? S.molecule[i] : 0;
   for (int j0 = 0; j0 < n; j0 += kGBBlock) {
     __syncthreads();
     load_chunk(sh, S, P, j0, 3);
@@ -353,6 +362,29 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_force(SysDev *sys, const int *g
       fx -= (double)(fs * dx);
       fy -= (double)(fs * dy);
       fz -= (double)(fs * dz);
+      if (VIR && energies && upper) {
+        const float px = fs * dx, py = fs * dy, pz = fs * dz;
+        const bool other = sh.mol[k] != moli;
+        if (other || P.virial) {
+          vir[0] += (double)(px * dx);
+          vir[1] += (double)(px * dy);
+          vir[2] += (double)(px * dz);
+          vir[3] += (double)(py * dy);
+          vir[4] += (double)(py * dz);
+          vir[5] += (double)(pz * dz);
+        }
+        if (other && P.molvirial) {
+          const int mj = sh.mol[k], nm = S.n_mol;
+          const double cx = S.com[mj] - S.com[moli], cy = S.com[nm + mj] - S.com[nm + moli],
+                       cz = S.com[2 * nm + mj] - S.com[2 * nm + moli];
+          mvir[0] += (double)px * cx;
+          mvir[1] += (double)px * cy;
+          mvir[2] += (double)px * cz;
+          mvir[3] += (double)py * cy;
+          mvir[4] += (double)py * cz;
+          mvir[5] += (double)pz * cz;
+        }
+      }
     }
   }
   if (vi) {
@@ -368,21 +400,38 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_force(SysDev *sys, const int *g
   }
   if (!energies) return;
   // block reduction in a fixed order, then fixed-point totals
-  double v[3] = {egb, elj, eco};
+  constexpr int NV = VIR ? 15 : 3;
+  double v[NV];
+  v[0] = egb;
+  v[1] = elj;
+  v[2] = eco;
+  if (VIR)
+    for (int c = 0; c < 6; ++c) {
+      v[3 + c] = vir[c];
+      v[9 + c] = mvir[c];
+    }
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int c = 0; c < 3; ++c) {
+  for (int c = 0; c < NV; ++c) {
     double t = v[c];
     for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
     if (lane == 0) red[c][wv] = t;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double t[3] = {0, 0, 0};
-    for (int c = 0; c < 3; ++c)
+    double t[NV];
+    for (int c = 0; c < NV; ++c) {
+      t[c] = 0.0;
       for (int w = 0; w < kGBBlock / 64; ++w) t[c] += red[c][w];
+    }
     long long *acc = acc_copy(S.acc, (int)blockIdx.x);
     atomicAdd(reinterpret_cast<unsigned long long *>(&acc[kAccCoul]), (unsigned long long)__double2ll_rn((t[0] + t[2]) * kEScale));
     atomicAdd(reinterpret_cast<unsigned long long *>(&acc[kAccLJ]), (unsigned long long)__double2ll_rn(t[1] * kEScale));
+    if (VIR) {
+      const int vs[12] = {kAccVxx, kAccVxy, kAccVxz, kAccVyy, kAccVyz, kAccVzz,
+                          kAccMVxx, kAccMVxy, kAccMVxz, kAccMVyy, kAccMVyz, kAccMVzz};
+      for (int c = 0; c < 12; ++c)
+        atomicAdd(reinterpret_cast<unsigned long long *>(&acc[vs[c]]), (unsigned long long)__double2ll_rn(t[3 + c] * kEScale));
+    }
   }
 }
 
@@ -391,7 +440,10 @@ void launch_gb(const Launch &L, const ForceDev &P, bool energies) {
   dim3 g(L.n_gb_tiles_total), b(kGBBlock);
   hipLaunchKernelGGL(k_gb_born, g, b, 0, L.stream, L.sys, L.gb_prefix, L.nsys, P);
   hipLaunchKernelGGL(k_gb_partial, g, b, 0, L.stream, L.sys, L.gb_prefix, L.nsys, P);
-  hipLaunchKernelGGL(k_gb_force, g, b, 0, L.stream, L.sys, L.gb_prefix, L.nsys, P, energies ? 1 : 0);
+  if (energies && (P.virial || P.molvirial))
+    hipLaunchKernelGGL(k_gb_force<true>, g, b, 0, L.stream, L.sys, L.gb_prefix, L.nsys, P, 1);
+  else
+    hipLaunchKernelGGL(k_gb_force<false>, g, b, 0, L.stream, L.sys, L.gb_prefix, L.nsys, P, energies ? 1 : 0);
 }
 
 }  // namespace pm

@@ -17,6 +17,7 @@ struct Launch {
       n_bonded_total = 0, n_gb_tiles_total = 0, n_local_total = 0, n_sc_total = 0;
   const struct EwLaunch *ew = nullptr;  // per system with a reciprocal-space term (host array)
   int n_ew = 0;
+  int max_mol = 0;                      // most molecules of a system (molecular virial; 0 = off)
 };
 
 // Host-side launch shape of one system's reciprocal-space passes
@@ -24,7 +25,11 @@ struct EwLaunch {
   const SysDev *sys = nullptr;  // the system's record in the device table
   int n = 0, nk = 0, ntiles = 0, nchunks = 0, nfb = 0, lds_bytes = 0;
 };
-enum { kEwForces = 1, kEwEnergies = 2 };
+// kEwVirial: the energies pass also adds the k-sum virial (to the atomic
+// slots, and to the molecular ones with kEwMolVirial); kEwMolVirial: add each
+// atom's reciprocal force (x) (its molecule's centre - its minimal position)
+// to the molecular virial
+enum { kEwForces = 1, kEwEnergies = 2, kEwVirial = 4, kEwMolVirial = 8 };
 
 void launch_import(const Launch &L, int what);
 void launch_export(const Launch &L, int what);
@@ -44,6 +49,8 @@ void launch_kick_drift(const Launch &L, double h, const LangevinDev &lv);
 void launch_md_arm(const Launch &L);
 void launch_half_kick(const Launch &L, double hh, const LangevinDev &lv);
 void launch_zero_forces(const Launch &L);
+// the molecules' centres of mass at the current positions (molecular virial)
+void launch_mol_com(const Launch &L);
 void launch_step_begin(const Launch &L, int mode, float lim2, int kmax);
 // reciprocal-space Ewald of the systems in L.ew: kEwForces (tables, structure
 // factors, forces) and/or kEwEnergies (energy and virial from the last

@@ -123,6 +123,20 @@ __device__ __forceinline__ double min_image_r2(const SysDev &S, int a, int b) {
   return r2;
 }
 
+// the same procedure on a difference vector in place (bonded terms, the
+// molecular centres of the molecular virial)
+__device__ __forceinline__ void min_image_vec(const SysDev &S, double *d) {
+  double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  if (!S.periodic || r2 <= S.D) return;
+  for (int k = 0; k < 3; ++k) {
+    if (d[k] < -S.H2[k]) d[k] += S.H[k];
+    else if (d[k] > S.H2[k]) d[k] -= S.H[k];
+  }
+  r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  if (r2 > S.D)
+    for (int k = 0; k < 3; ++k) d[k] -= S.L[k] * rint(S.Lr[k] * d[k]);
+}
+
 // Cn coefficient tables (CnSwitchingFunction.cpp:9-31), rows = order 2,3,4,(5),6
 __constant__ double kSwCoef[5][7] = {{10., -15., 6., 0, 0, 0, 0},
                                      {35., -84., 70., -20., 0, 0, 0},
@@ -911,8 +925,9 @@ __device__ __forceinline__ float pair_terms(const ForceDev &P, float r2, float q
 template <int TA, int TB, int SWA, int SWB, int KIND, bool SAME>
 __global__ __launch_bounds__(512, (KIND == kPairForces ? PM_PAIR_MIN_WAVES : PM_PAIR_MIN_WAVES_EN)) void k_pair(const SysDev *__restrict__ sys, const int *__restrict__ scp, int nsys,
                                               ForceDev P) {
-  constexpr bool FORCES = KIND != kPairEnergies;
-  constexpr bool EN = KIND == kPairForcesEnergies || KIND == kPairEnergies;
+  constexpr bool MV = KIND == kPairForcesEnergiesMV || KIND == kPairEnergiesMV;
+  constexpr bool FORCES = KIND != kPairEnergies && KIND != kPairEnergiesMV;
+  constexpr bool EN = KIND == kPairForcesEnergies || KIND == kPairEnergies || MV;
   constexpr bool CEN = KIND == kPairCensus;
   extern __shared__ float4 pair_lds[];
   const int lane = threadIdx.x & 63;
@@ -990,6 +1005,7 @@ __global__ __launch_bounds__(512, (KIND == kPairForces ? PM_PAIR_MIN_WAVES : PM_
   for (int c = 0; c < kScCl; ++c) fi[c][0] = fi[c][1] = fi[c][2] = 0.f;
   double eA = 0.0, eB = 0.0;
   double vxx = 0.0, vxy = 0.0, vxz = 0.0, vyy = 0.0, vyz = 0.0, vzz = 0.0;
+  double mxx = 0.0, mxy = 0.0, mxz = 0.0, myy = 0.0, myz = 0.0, mzz = 0.0;
   unsigned long long ccount = 0, chash = 0;
   const int len = *cp(S->list_len + sc);
   const int e_begin = part * len / split, e_end = (part + 1) * len / split;
@@ -1084,12 +1100,37 @@ __global__ __launch_bounds__(512, (KIND == kPairForces ? PM_PAIR_MIN_WAVES : PM_
       if (EN) {
         eA += (double)ea;
         eB += (double)eb;
-        vxx += (double)(fx * dx);
-        vxy += (double)(fx * dy);
-        vxz += (double)(fx * dz);
-        vyy += (double)(fy * dy);
-        vyz += (double)(fy * dz);
-        vzz += (double)(fz * dz);
+        bool atomic = true;
+        if (MV) {
+          // OneAtomPair.h:95-103: a pair of two molecules adds f (x) (R_J - R_I)
+          // to the molecular virial and f (x) d to the atomic one; a pair
+          // inside a molecule adds to the atomic virial when that is on
+          const bool live = inA || inB;
+          const int mi = live ? S->smol[(cl0 + c) * kCluster + ii] : 0;
+          const int mj = live ? S->smol[sj] : 0;
+          const bool other = live && mi != mj;
+          atomic = other || P.virial;
+          if (other) {
+            const int nm = S->n_mol;
+            double cd[3] = {S->com[mj] - S->com[mi], S->com[nm + mj] - S->com[nm + mi],
+                            S->com[2 * nm + mj] - S->com[2 * nm + mi]};
+            min_image_vec(*S, cd);
+            mxx += (double)fx * cd[0];
+            mxy += (double)fx * cd[1];
+            mxz += (double)fx * cd[2];
+            myy += (double)fy * cd[1];
+            myz += (double)fy * cd[2];
+            mzz += (double)fz * cd[2];
+          }
+        }
+        if (atomic) {
+          vxx += (double)(fx * dx);
+          vxy += (double)(fx * dy);
+          vxz += (double)(fx * dz);
+          vyy += (double)(fy * dy);
+          vyz += (double)(fy * dz);
+          vzz += (double)(fz * dz);
+        }
       }
       if (CEN) {
         if (inA || inB) {
@@ -1150,6 +1191,23 @@ __global__ __launch_bounds__(512, (KIND == kPairForces ? PM_PAIR_MIN_WAVES : PM_
       atomic_add_ll(&acc[kAccVyz], to_fixed(vyz, kEScale));
       atomic_add_ll(&acc[kAccVzz], to_fixed(vzz, kEScale));
     }
+    if (MV) {
+      mxx = wave_sum_d(mxx);
+      mxy = wave_sum_d(mxy);
+      mxz = wave_sum_d(mxz);
+      myy = wave_sum_d(myy);
+      myz = wave_sum_d(myz);
+      mzz = wave_sum_d(mzz);
+      if (lane == 0) {
+        PM_GLOBAL long long *acc = gp(acc_copy(S->acc, w));
+        atomic_add_ll(&acc[kAccMVxx], to_fixed(mxx, kEScale));
+        atomic_add_ll(&acc[kAccMVxy], to_fixed(mxy, kEScale));
+        atomic_add_ll(&acc[kAccMVxz], to_fixed(mxz, kEScale));
+        atomic_add_ll(&acc[kAccMVyy], to_fixed(myy, kEScale));
+        atomic_add_ll(&acc[kAccMVyz], to_fixed(myz, kEScale));
+        atomic_add_ll(&acc[kAccMVzz], to_fixed(mzz, kEScale));
+      }
+    }
   }
   if (CEN) {
     ccount = wave_sum_u64(ccount);
@@ -1163,17 +1221,6 @@ __global__ __launch_bounds__(512, (KIND == kPairForces ? PM_PAIR_MIN_WAVES : PM_
 
 // ------------------------------------------------------- FP64 side terms
 
-__device__ __forceinline__ void min_image_vec(const SysDev &S, double *d) {
-  double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-  if (!S.periodic || r2 <= S.D) return;
-  for (int k = 0; k < 3; ++k) {
-    if (d[k] < -S.H2[k]) d[k] += S.H[k];
-    else if (d[k] > S.H2[k]) d[k] -= S.H[k];
-  }
-  r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-  if (r2 > S.D)
-    for (int k = 0; k < 3; ++k) d[k] -= S.L[k] * rint(S.Lr[k] * d[k]);
-}
 
 
 // Bonded terms and Ewald excluded-pair corrections, one thread per atom
@@ -1289,6 +1336,10 @@ __global__ __launch_bounds__(256) void k_local(SysDev *sys, const int *ap, int n
     }
   }
   if (!(mode & kLocalEnergies)) return;
+  // bonded and correction terms add to the atomic virial only when it is on
+  // (BondSystemForce.h:121, AngleSystemForce.h:155, NonbondedFullEwaldSystemForce.h:1125)
+  if (!P.virial)
+    for (int k = 0; k < 6; ++k) v[k] = 0.0;
   // each lane's terms go to fixed point first; integer sums are exact, so the
   // totals do not depend on how lanes are grouped
   long long q[9] = {to_fixed(eb, kEScale), to_fixed(ea, kEScale), to_fixed(ec, kEScale),
@@ -1579,6 +1630,14 @@ static void launch_pair_t(const Launch &L, const ForceDev &P, int kind) {
   case kPairEnergies:
     hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairEnergies, SAME>), g, b, lds, L.stream, L.sys, L.sc_prefix, L.nsys, P);
     break;
+  case kPairForcesEnergiesMV:
+    hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairForcesEnergiesMV, SAME>), g, b, lds, L.stream, L.sys, L.sc_prefix,
+                       L.nsys, P);
+    break;
+  case kPairEnergiesMV:
+    hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairEnergiesMV, SAME>), g, b, lds, L.stream, L.sys, L.sc_prefix,
+                       L.nsys, P);
+    break;
   default:
     hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairCensus, SAME>), g, b, lds, L.stream, L.sys, L.sc_prefix, L.nsys, P);
     break;
@@ -1620,6 +1679,41 @@ void launch_kick_drift(const Launch &L, double h, const LangevinDev &lv) {
 void launch_half_kick(const Launch &L, double hh, const LangevinDev &lv) {
   if (L.n_atoms_total) hipLaunchKernelGGL(k_half_kick, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys, hh, lv);
 }
+// buildMolecularCenterOfMass (TopologyUtilities.cpp:677-684) with
+// molecularCenterOfMass (:645-674): one thread per molecule, the atoms in
+// index order, Kahan-compensated mass-weighted sums, FP64
+__global__ __launch_bounds__(256) void k_mol_com(SysDev *sys) {
+#pragma clang fp contract(off)
+  const SysDev &S = sys[blockIdx.y];
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= S.n_mol) return;
+  const int n = S.n_atoms, a0 = S.mol_start[m], a1 = S.mol_start[m + 1];
+  double c[3] = {0.0, 0.0, 0.0}, t[3] = {0.0, 0.0, 0.0}, sm = 0.0;
+  for (int k = a0; k < a1; ++k) {
+    const int a = S.mol_atoms[k];
+    const double mass = S.mass[a];
+    if (k == a0) {
+      sm = mass;
+      for (int q = 0; q < 3; ++q) c[q] = S.x[q * n + a] * mass;
+      continue;
+    }
+    sm += mass;
+    for (int q = 0; q < 3; ++q) {
+      const double tx = S.x[q * n + a] * mass;
+      const double ty = tx - t[q];
+      const double tt = c[q] + ty;
+      t[q] = (tt - c[q]) - ty;
+      c[q] = tt;
+    }
+  }
+  for (int q = 0; q < 3; ++q) S.com[q * S.n_mol + m] = a1 > a0 ? c[q] / sm : 0.0;
+}
+
+void launch_mol_com(const Launch &L) {
+  if (L.max_mol > 0)
+    hipLaunchKernelGGL(k_mol_com, dim3(nblk(L.max_mol, 256), L.nsys), dim3(256), 0, L.stream, L.sys);
+}
+
 void launch_zero_forces(const Launch &L) {
   if (L.n_slots_total) hipLaunchKernelGGL(k_zero_forces, dim3(nblk(L.n_slots_total, 256)), dim3(256), 0, L.stream, L.sys, L.slot_prefix, L.nsys);
 }

@@ -125,6 +125,53 @@ def make_gb_ace(n, seed):
                         seed=seed, ace_sigma=2.26 / 418.4, ace_rho_s=1.4)
 
 
+def gb_multi_chain(n, seed, per):
+    """A GB chain cut into molecules of `per` atoms (the pair terms see
+    pairs of different molecules, so the molecular virial has work)."""
+    g = S.gb_chain(n, seed=seed)
+    g.molecule = (np.arange(n) // per).astype(np.int32)
+    return g
+
+
+def make_molvirial(w):
+    """The atomic and molecular virials (ScalarStructure::molecularVirial on)
+    of the reference's classes: LJ + Coulomb C2 cutoff pairs and the full
+    Ewald sum on the water box, bonded terms, and GB + LJ + Coulomb on a chain
+    cut into molecules, each with the atomic virial on and off."""
+    acc = math.exp(-10.24)
+    out = {"x": w.x, "alpha": 0.32, "accuracy": acc}
+    ta, tb, sa, sb = CUTOFF_CASES["c2c2"]
+    for vir in (1, 0):
+        r = refio.system_records(w)
+        r.update(mode=np.int32(1), term_a=np.int32(ta), term_b=np.int32(tb), sw_a=np.array(sa, float),
+                 sw_b=np.array(sb, float), alg_cutoff=10.0, cellsize=5.0, virial=np.int32(vir),
+                 molvirial=np.int32(1))
+        o = refio.run(r)
+        out[f"f_cut_v{vir}"], out[f"e_cut_v{vir}"] = o["f"].reshape(-1, 3), o["e"]
+        for parts, tag in ((7, "ew"), (2, "rec")):
+            r = refio.system_records(w)
+            r.update(mode=np.int32(2), ew_alpha=0.32, ew_accuracy=acc, ew_parts=np.int32(parts), cellsize=5.0,
+                     virial=np.int32(vir), molvirial=np.int32(1))
+            o = refio.run(r)
+            out[f"f_{tag}_v{vir}"], out[f"e_{tag}_v{vir}"] = o["f"].reshape(-1, 3), o["e"]
+    r = refio.system_records(w)
+    r.update(mode=np.int32(4), bonds=w.bonds.reshape(-1), bond_k=w.bond_k, bond_r0=w.bond_r0,
+             angles=w.angles.reshape(-1), angle_k=w.angle_k, angle_t0=w.angle_t0, cellsize=5.0,
+             virial=np.int32(1))
+    o = refio.run(r)
+    out["e_bonded_v1"] = o["e"]
+    g = gb_multi_chain(200, 17, 50)
+    params = np.array([0.09, 1.0, 0.8, 4.85, 1.0, 78.3])
+    for vir in (1, 0):
+        r = refio.system_records(g)
+        r.update(mode=np.int32(3), gb_rho=g.gb_radius, gb_S=g.gb_scaling, gb_params=params, cellsize=np.int32(5),
+                 gb_ljcoul=np.int32(1), virial=np.int32(vir), molvirial=np.int32(1))
+        o = refio.run(r)
+        out[f"f_gb_v{vir}"], out[f"e_gb_v{vir}"] = o["f"].reshape(-1, 3), o["e"]
+    out.update(gb_x=g.x, gb_seed=17, gb_per=50, gb_params=params)
+    np.savez_compressed(os.path.join(HERE, "ref_molvirial.npz"), **out)
+
+
 # ------------------------------------------------------------- water_423
 
 def _pdb_xyz(path):
@@ -237,6 +284,9 @@ def main():
     if sys.argv[1:] == ["gb_ace"]:
         make_gb_ace(200, seed=13)
         return
+    if sys.argv[1:] == ["molvirial"]:
+        make_molvirial(w)
+        return
     for name, case in CUTOFF_CASES.items():
         make_cutoff(name, w, case)
     make_ewald(w)
@@ -244,6 +294,7 @@ def main():
     make_bonded(w)
     make_gb(300, seed=7)
     make_gb_ace(200, seed=13)
+    make_molvirial(w)
     make_water423()
     copy_water_expected()
     for f in sorted(os.listdir(HERE)):

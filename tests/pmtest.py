@@ -129,3 +129,56 @@ def langevin_oracle(force_fn, x, v, mass, dt_fs, n_steps, temperature, gamma_ps,
         c += 1
     v = half(v, F, 2 * c)
     return x, v, F, E
+
+
+def vacuum_lj_coulomb(c, virial=False, molvirial=False, x=None):
+    """All-pairs LJ (per-atom GROMACS mixing) + Coulomb in vacuum, as
+    OneAtomPair<Vacuum, Universal, ...> adds them (OneAtomPair.h:47-107):
+    exclusions only between atoms of one molecule; with the molecular virial
+    on, pairs of two molecules add f (x) (R_J - R_I) to MOLVIRIAL and f (x) d
+    to VIRIAL, other pairs add f (x) d only when the atomic virial is on.
+    Returns forces (n, 3) and 36 slots."""
+    import oracle as O
+    x = c.x if x is None else x
+    n = c.n
+    mol = np.asarray(c.molecule)
+    d = x[None, :, :] - x[:, None, :]
+    r2 = np.sum(d * d, axis=2)
+    iu = np.triu_indices(n, 1)
+    excl = {}
+    for a, b, cls in c.exclusions:
+        if mol[a] == mol[b]:
+            excl[(min(a, b), max(a, b))] = cls
+    cls = np.array([excl.get((i, j), 0) for i, j in zip(*iu)])
+    keep = cls != 1
+    i, j, cls = iu[0][keep], iu[1][keep], cls[keep]
+    rr = 1.0 / r2[i, j]
+    sg = 0.5 * (c.lj_sigma[i] + c.lj_sigma[j])
+    ep = np.sqrt(c.lj_epsilon[i] * c.lj_epsilon[j])
+    A, B = 4 * ep * sg ** 12, 4 * ep * sg ** 6
+    A = np.where(cls == 2, A * c.lj_scaling, A)
+    B = np.where(cls == 2, B * c.lj_scaling, B)
+    r6 = rr ** 3
+    elj = A * r6 * r6 - B * r6
+    flj = (12 * A * r6 * r6 - 6 * B * r6) * rr
+    ec = c.charge[i] * c.charge[j] * np.sqrt(rr)
+    ec = np.where(cls == 2, ec * c.coulomb_scaling, ec)
+    fc = ec * rr
+    fij = d[i, j] * (flj + fc)[:, None]
+    f = np.zeros((n, 3))
+    np.add.at(f, i, -fij)
+    np.add.at(f, j, fij)
+    e = np.zeros(36)
+    e[0] = ec.sum()
+    e[1] = elj.sum()
+    other = mol[i] != mol[j]
+    if molvirial:
+        com = O.molecule_com(c, x=x)
+        cd = com[mol[j]] - com[mol[i]]
+        w = np.einsum("pa,pb->ab", fij[other], cd[other])
+        # ScalarStructure::addVirial fills the lower triangle with the upper one
+        w[1, 0], w[2, 0], w[2, 1] = w[0, 1], w[0, 2], w[1, 2]
+        e[16:25] += w.reshape(-1)
+    atomic = (other & molvirial) | (np.ones_like(other) if virial else np.zeros_like(other))
+    e[7:16] += np.einsum("pa,pb->ab", fij[atomic], d[i, j][atomic]).reshape(-1)
+    return f, e

@@ -712,3 +712,76 @@ def test_langevin_batch_alone_vs_mix_bitwise():
         assert np.array_equal(xb[off:off + s.n], x) and np.array_equal(vb[off:off + s.n], v)
         assert np.array_equal(Fb[off:off + s.n], F) and np.array_equal(Eb[k], E[0])
         off += s.n
+
+
+def check_tensor(v, r, npairs):
+    tol = RTOL_E * max(1.0, np.max(np.abs(r))) + 1e-7 * npairs
+    assert np.max(np.abs(v - r)) <= tol, (v, r, tol)
+
+
+def test_molvirial_cutoff_and_ewald_vs_reference_golden():
+    """The molecular virial (PM_FLAG_MOLVIRIAL) with the atomic virial on and
+    off: LJ + Coulomb C2 pairs and the full Ewald sum against the
+    reference's own classes."""
+    print("case molvirial cutoff/ewald", flush=True)
+    g = T.golden("ref_molvirial.npz")
+    w = T.fixture_water()
+    ta, sa, tb, sb = MODELS["c2c2"]
+    for vir in (1, 0):
+        flags = L.FLAG_MOLVIRIAL | (L.FLAG_VIRIAL if vir else 0)
+        with L.Context(force_desc(ta, sa, tb, sb, flags=flags), [w]) as ctx:
+            f, e = evaluate(ctx, 0, w.x)
+            cen = ctx.census(0, w.x)
+        ref = g[f"e_cut_v{vir}"]
+        assert T.rms_rel(f, g[f"f_cut_v{vir}"]) <= RTOL_F
+        check_energies(e, ref, npairs=cen[0])
+        check_tensor(e[7:16], ref[7:16], cen[0])
+        check_tensor(e[16:25], ref[16:25], cen[0])
+        desc = force_desc(L.TERM_NONE, NOTERM, 0, NOTERM, ewald=True, flags=flags | RECIP)
+        with L.Context(desc, [w]) as ctx:
+            f, e = evaluate(ctx, 0, w.x)
+            cen = ctx.census(0, w.x)
+        ref = g[f"e_ew_v{vir}"]
+        assert T.rms_rel(f, g[f"f_ew_v{vir}"]) <= RTOL_F
+        check_energies(e, ref, slots=(0,), npairs=cen[0])
+        check_tensor(e[7:16], ref[7:16], cen[0])
+        check_tensor(e[16:25], ref[16:25], cen[0])
+
+
+def test_molvirial_gb_vs_reference_golden():
+    """GB + LJ + Coulomb on a chain cut into molecules: the atomic and the
+    molecular virial against the reference's classes."""
+    print("case molvirial gb", flush=True)
+    g = T.golden("ref_molvirial.npz")
+    c = S.gb_chain(200, seed=int(g["gb_seed"]))
+    c.molecule = (np.arange(c.n) // int(g["gb_per"])).astype(np.int32)
+    npairs = 3 * c.n * (c.n - 1) // 2
+    for vir in (1, 0):
+        flags = L.FLAG_MOLVIRIAL | (L.FLAG_VIRIAL if vir else 0)
+        with L.Context(gb_desc(flags), [c]) as ctx:
+            f, e = evaluate(ctx, 0, c.x)
+        ref = g[f"e_gb_v{vir}"]
+        assert T.rms_rel(f, g[f"f_gb_v{vir}"]) <= RTOL_F
+        check_energies(e, ref, npairs=npairs)
+        check_tensor(e[7:16], ref[7:16], npairs)
+        check_tensor(e[16:25], ref[16:25], npairs)
+
+
+def test_molvirial_md_energy_pass_matches_evaluate():
+    """MD energies (the separate energy pass) carry the same molecular virial
+    as a direct evaluation at the final positions, with bonded terms and the
+    full Ewald sum."""
+    print("case molvirial md", flush=True)
+    w = S.water_box(343, 22.5, seed=5)
+    v0 = S.maxwell_velocities(w, 300.0, seed=1)
+    flags = L.FLAG_MOLVIRIAL | L.FLAG_VIRIAL | L.FLAG_BONDED | RECIP
+    desc = force_desc(L.TERM_LJ, MODELS["c2c2"][1], 0, NOTERM, ewald=True, flags=flags, skin=1.5, interval=5)
+    with L.Context(desc, [w]) as ctx:
+        ctx.md_set_state(w.x, v0)
+        ctx.md_run(20, 0.5, energies=True)
+        x, v, F, E = ctx.md_get_state()
+        f2, e2 = evaluate(ctx, 0, x)
+    assert np.any(E[0][16:25] != 0)
+    check_tensor(E[0][7:16], e2[7:16], 200000)
+    check_tensor(E[0][16:25], e2[16:25], 200000)
+    assert T.rms_rel(F, f2) <= RTOL_F
