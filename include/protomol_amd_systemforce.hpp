@@ -90,12 +90,16 @@ namespace ProtoMol {
 
   class PmSystemForceBase : public SystemForce {
   public:
-    explicit PmSystemForceBase(const pm_force_desc &d) : myDesc(d), myCtx(0), myTopo(0) {}
+    explicit PmSystemForceBase(const pm_force_desc &d)
+      : myDesc(d), myCtx(0), myTopo(0), myVirial(false), myMolVirial(false) {}
     virtual ~PmSystemForceBase() { if (myCtx) pm_ctx_destroy(myCtx); }
 
+    // the virial flags follow the ScalarStructure (ScalarStructure::virial(),
+    // molecularVirial()); a change rebuilds the context
     virtual void evaluate(const GenericTopology *topo, const Vector3DBlock *positions,
                           Vector3DBlock *forces, ScalarStructure *energies) {
-      ensure(topo);
+      const bool vir = energies->virial(), mol = energies->molecularVirial();
+      ensure(topo, vir, mol);
       std::vector<double> e(PM_E_NSLOTS, 0.0);
       const double *x = positions->size() ? (*positions)[0].c : 0;
       double *f = forces->size() ? (*forces)[0].c : 0;
@@ -103,9 +107,12 @@ namespace ProtoMol {
       if (rc != PM_OK) THROW(std::string("protomol_amd: ") + pm_last_error(myCtx));
       (*energies)[ScalarStructure::COULOMB] += e[PM_E_COULOMB];
       (*energies)[ScalarStructure::LENNARDJONES] += e[PM_E_LENNARDJONES];
-      if (energies->virial())
+      if (vir || mol)
         for (int k = 0; k < 9; k++)
           (*energies)[(ScalarStructure::Index)(ScalarStructure::VIRIALXX + k)] += e[PM_E_VIRIALXX + k];
+      if (mol)
+        for (int k = 0; k < 9; k++)
+          (*energies)[(ScalarStructure::Index)(ScalarStructure::MOLVIRIALXX + k)] += e[PM_E_MOLVIRIALXX + k];
     }
     // one device evaluates everything; the MPI split has nothing to divide
     virtual void parallelEvaluate(const GenericTopology *topo, const Vector3DBlock *positions,
@@ -120,19 +127,23 @@ namespace ProtoMol {
     }
 
   protected:
-    void ensure(const GenericTopology *topo) {
-      if (myCtx && myTopo == topo) return;
+    void ensure(const GenericTopology *topo, bool vir, bool mol) {
+      if (myCtx && myTopo == topo && myVirial == vir && myMolVirial == mol) return;
       uncache();
       PmTopologyImage img(topo);
       pm_force_desc d = myDesc;
-      d.flags |= PM_FLAG_VIRIAL;
+      if (vir) d.flags |= PM_FLAG_VIRIAL;
+      if (mol) d.flags |= PM_FLAG_MOLVIRIAL;
       int rc = pm_ctx_create(&d, 1, &img.desc, &myCtx);
       if (rc != PM_OK) THROW(std::string("protomol_amd: ") + pm_last_error(0));
       myTopo = topo;
+      myVirial = vir;
+      myMolVirial = mol;
     }
     pm_force_desc myDesc;
     pm_ctx *myCtx;
     const GenericTopology *myTopo;
+    bool myVirial, myMolVirial;
   };
 
   // LennardJones and/or Coulomb with a switching function, cell-list cutoff
