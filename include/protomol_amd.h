@@ -196,7 +196,9 @@ int pm_evaluate_batch(pm_ctx *ctx, const double *x, double *f, double *e);
 int pm_evaluate_systems(pm_ctx *ctx, int32_t n, const int32_t *ids, const double *x, double *f, double *e);
 
 /* Digest of the pair set the last evaluation of `system` used: count and
- * sum of splitmix64(lo | hi << 32) over pairs (lo < hi, original indices). */
+ * sum of splitmix64(lo | hi << 32) over pairs (lo < hi, original indices).
+ * x == NULL: the pair set that the MD state's current pair list yields at
+ * its positions, without a rebuild (the set the last MD step evaluated). */
 int pm_pair_census(pm_ctx *ctx, int32_t system, const double *x, uint64_t *count, uint64_t *hash);
 
 /* Device-resident leapfrog.  Set x, v (concatenated over systems), run

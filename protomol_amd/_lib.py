@@ -276,8 +276,10 @@ class Context:
         check(lib().pm_evaluate_systems(self.h, len(ii), ptr(ii, _ip), ptr(x), ptr(f), ptr(e)), self.h)
         return self._split(ii, f), e
 
-    def census(self, system: int, x: np.ndarray):
-        xx = np.ascontiguousarray(x, np.float64)
+    def census(self, system: int, x):
+        """(count, hash) of the pair set at positions x (a fresh list), or with
+        x None of the MD state's current list at its positions (no rebuild)."""
+        xx = None if x is None else np.ascontiguousarray(x, np.float64)
         cnt, hsh = C.c_uint64(), C.c_uint64()
         check(lib().pm_pair_census(self.h, system, ptr(xx), C.byref(cnt), C.byref(hsh)), self.h)
         return cnt.value, hsh.value

@@ -1655,10 +1655,21 @@ int pm_evaluate_systems(pm_ctx *c, int32_t n, const int32_t *ids, const double *
 int pm_pair_census(pm_ctx *c, int32_t system, const double *x, uint64_t *count, uint64_t *hash) {
   if (!c) return PM_EINVAL;
   return run_guarded(c, [&] {
-    if (!x || !count || !hash) throw Fail{PM_EINVAL, "bad census arguments"};
+    if (!count || !hash) throw Fail{PM_EINVAL, "bad census arguments"};
     sys_of(c, system);
     if (c->gb) throw Fail{PM_EINVAL, "the all-pairs GB path evaluates every pair; no census"};
-    evaluate_ids(c, {system}, x, nullptr, nullptr, true, count, hash);
+    if (x) {
+      evaluate_ids(c, {system}, x, nullptr, nullptr, true, count, hash);
+      return;
+    }
+    // the list the MD state holds, at its positions: no rebuild
+    require_md(c, {system});
+    LaunchSet &ls = get_set(c, {system});
+    HIPCHECK(hipMemsetAsync(c->sys[system]->dev.census, 0, 2 * sizeof(unsigned long long), c->stream));
+    if (!pm::launch_pair(ls.L, c->P, pm::kPairCensus)) throw Fail{PM_EINVAL, "unsupported term combination"};
+    read_scalars(c, ls);
+    *count = (uint64_t)ls.h_scal[pm::kScalCensus];
+    *hash = (uint64_t)ls.h_scal[pm::kScalCensus + 1];
   });
 }
 

@@ -785,3 +785,24 @@ def test_molvirial_md_energy_pass_matches_evaluate():
     check_tensor(E[0][7:16], e2[7:16], 200000)
     check_tensor(E[0][16:25], e2[16:25], 200000)
     assert T.rms_rel(F, f2) <= RTOL_F
+
+
+def test_md_live_list_pair_set_exact():
+    """The pair list an MD run keeps between rebuilds (radius rc + skin,
+    rebuilt when an atom has moved skin/2) yields exactly the FP64
+    within-cutoff pair set at every checked state: census of the live list,
+    no rebuild, against the oracle's brute force."""
+    print("case md live list census", flush=True)
+    w = S.water_box(343, 22.5, seed=5)
+    v0 = S.maxwell_velocities(w, 600.0, seed=1)
+    ta, sa, tb, sb = MODELS["c2c2"]
+    d = force_desc(ta, sa, tb, sb, flags=L.FLAG_BONDED, skin=0.6, interval=0)
+    rebuilds = 0
+    with L.Context(d, [w]) as ctx:
+        ctx.md_set_state(w.x, v0)
+        for k in range(6):
+            ctx.md_run(7, 1.0, energies=False)
+            rebuilds += ctx.stats()["rebuilds"]
+            x, _, _, _ = ctx.md_get_state()
+            assert ctx.census(0, None) == O.census_bruteforce(w, 100.0, x=x), k
+    assert rebuilds >= 2
