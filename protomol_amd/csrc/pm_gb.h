@@ -5,9 +5,12 @@
 
 namespace pm {
 
-constexpr int kGBBlock = 256;  // i atoms per workgroup (one per lane), j staged in LDS
+constexpr int kGBBlock = 256;  // threads per workgroup = j atoms staged in LDS per chunk
+constexpr int kGBI = 64;       // i atoms per workgroup (one per lane of each wave)
+constexpr int kGBParts = kGBBlock / kGBI;  // waves per workgroup: each takes a quarter of every chunk
+static_assert(kGBParts == 4, "the partial sums are added as a fixed 4-term expression");
 
-inline int gb_tile_count(int n) { return (n + kGBBlock - 1) / kGBBlock; }
+inline int gb_tile_count(int n) { return (n + kGBI - 1) / kGBI; }
 
 void gb_setup_force(ForceDev &P, const pm_force_desc &d);
 void gb_setup_system(SysDev &S, const double *rho, const double *scaling, double *born, double *drfac,

@@ -3,12 +3,14 @@
 // ProtoMol evaluates GB as three dependent all-pairs passes over
 // NonbondedSimpleFullSystemForce (GBBornRadii -> GBPartialSum -> GBForce) and
 // stores ten N x N intermediate arrays per system (topology/Atom.h:62-83).
-// Here each pass recomputes what it needs instead: one workgroup owns 256
-// i atoms (one per lane) and streams every j atom through LDS, so an atom's
-// Born radius, partial sum and force are complete in its own lane.  No N x N
-// storage, no atomics inside a pass, a fixed summation order per atom
-// (bitwise reproducible), FP32 pair arithmetic on FP64 coordinate differences,
-// FP64 accumulation and FP64 per-atom Born-radius algebra.
+// Here each pass recomputes what it needs instead: one workgroup owns 64
+// i atoms (one per lane of each of its four waves) and streams every j atom
+// through LDS; wave p takes the p-th quarter of every j chunk, and the four
+// partial sums of an i atom are added in wave order, so its Born radius,
+// partial sum and force are complete in the workgroup.  No N x N storage, no
+// atomics inside a pass, a fixed summation order per atom (bitwise
+// reproducible), FP32 pair arithmetic on FP64 coordinate differences, FP64
+// accumulation and FP64 per-atom Born-radius algebra.
 #include <hip/hip_runtime.h>
 #include <math.h>
 
@@ -104,42 +106,46 @@ __device__ __forceinline__ float dburial(float dist, float idist, float ori, flo
          0.25f * s2 * idist * iU2 * iU * dU - 0.25f * s2 * idist * iL2 * iL * dL + dC;
 }
 
+// A chunk of j atoms in LDS, one record of four 16-byte words per atom so a
+// pair reads its j data with a few wide loads:
+//   pos  x, y, z (FP64)
+//   a    rho~ = rho - offset, S, B (Born radius), 1/B
+//   b    q, q^2 / (2 B^2), dR factor, partial sum
+//   c    sigma/2 (or the LJ type's bits), sqrt(eps), ACE c / R^7, molecule's bits
 struct GBShared {
-  double x[kGBBlock], y[kGBBlock], z[kGBBlock];
-  float q[kGBBlock], ori[kGBBlock], S[kGBBlock], B[kGBBlock], iB[kGBBlock], qB2[kGBBlock], dr[kGBBlock],
-      part[kGBBlock], sg[kGBBlock], ep[kGBBlock], ak[kGBBlock];
-  int mol[kGBBlock];
+  double4 pos[kGBBlock];
+  float4 a[kGBBlock], b[kGBBlock], c[kGBBlock];
 };
 
 __device__ void load_chunk(GBShared &sh, const SysDev &S, const ForceDev &P, int j0, int pass) {
   const int t = threadIdx.x, n = S.n_atoms;
   const int j = j0 + t;
   if (j < n) {
-    sh.x[t] = S.x[j];
-    sh.y[t] = S.x[n + j];
-    sh.z[t] = S.x[2 * n + j];
-    sh.ori[t] = (float)(S.gb_rho[j] - P.gb_offset);
-    sh.S[t] = (float)S.gb_S[j];
+    sh.pos[t] = make_double4(S.x[j], S.x[n + j], S.x[2 * n + j], 0.0);
+    float4 a = make_float4((float)(S.gb_rho[j] - P.gb_offset), (float)S.gb_S[j], 1.f, 1.f);
     if (pass >= 2) {
       const float q = (float)S.charge[j], B = (float)S.gb_born[j], iB = rcp(B);
-      sh.q[t] = q;
-      sh.B[t] = B;
-      sh.iB[t] = iB;
-      sh.qB2[t] = 0.5f * q * q * iB * iB;  // q^2 / (2 B^2)
-    }
-    if (pass >= 3) {
-      sh.mol[t] = S.molecule[j];
-      sh.dr[t] = (float)S.gb_drfac[j];
-      sh.part[t] = (float)S.gb_partial[j];
-      sh.ak[t] = P.gb_ace ? (float)S.gb_spare[j] : 0.f;
-      if (S.lj_mode == PM_LJ_PERATOM) {
-        sh.sg[t] = S.sig_half[j];
-        sh.ep[t] = S.eps_sqrt[j];
-      } else {
-        sh.sg[t] = __int_as_float(S.type[j]);
-        sh.ep[t] = 0.f;
+      a.z = B;
+      a.w = iB;
+      float4 b = make_float4(q, 0.5f * q * q * iB * iB, 0.f, 0.f);  // q^2 / (2 B^2)
+      if (pass >= 3) {
+        b.z = (float)S.gb_drfac[j];
+        b.w = (float)S.gb_partial[j];
+        float4 c;
+        if (S.lj_mode == PM_LJ_PERATOM) {
+          c.x = S.sig_half[j];
+          c.y = S.eps_sqrt[j];
+        } else {
+          c.x = __int_as_float(S.type[j]);
+          c.y = 0.f;
+        }
+        c.z = P.gb_ace ? (float)S.gb_spare[j] : 0.f;
+        c.w = __int_as_float(S.molecule[j]);
+        sh.c[t] = c;
       }
+      sh.b[t] = b;
     }
+    sh.a[t] = a;
   }
 }
 
@@ -150,12 +156,14 @@ __device__ void load_chunk(GBShared &sh, const SysDev &S, const ForceDev &P, int
 // (GBForce.cpp:126: R_i^2 rho~_i (1 - tanh^2) dtanh/dpsi / rho_i)
 __global__ __launch_bounds__(kGBBlock) void k_gb_born(SysDev *sys, const int *gp, int nsys, ForceDev P) {
   __shared__ GBShared sh;
+  __shared__ double red[kGBParts][kGBI];
   const int tile = blockIdx.x;
   if (tile >= gp[nsys]) return;
   const int s = find_sys_gb(gp, nsys, tile);
   const SysDev &S = sys[s];
   const int n = S.n_atoms;
-  const int i = (tile - gp[s]) * kGBBlock + threadIdx.x;
+  const int lane = threadIdx.x & (kGBI - 1), part = threadIdx.x / kGBI;
+  const int i = (tile - gp[s]) * kGBI + lane;
   const bool vi = i < n;
   double xi = 0, yi = 0, zi = 0;
   float ori = 1.f;
@@ -173,15 +181,21 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_born(SysDev *sys, const int *gp
     const int cnt = min(kGBBlock, n - j0);
     if (!vi) continue;
     const float iori = rcp(ori);
-    for (int k = 0; k < cnt; ++k) {
+    const int k1 = min(cnt, (part + 1) * kGBI);
+    for (int k = part * kGBI; k < k1; ++k) {
       if (j0 + k == i) continue;
-      const float dx = (float)(sh.x[k] - xi), dy = (float)(sh.y[k] - yi), dz = (float)(sh.z[k] - zi);
+      const double4 pj = sh.pos[k];
+      const float4 aj = sh.a[k];
+      const float dx = (float)(pj.x - xi), dy = (float)(pj.y - yi), dz = (float)(pj.z - zi);
       const float r2 = dx * dx + dy * dy + dz * dz;
       const float idist = rsqrtf(r2);
-      burial += (double)burial_term(r2 * idist, idist, ori, iori, sh.ori[k], sh.S[k]);
+      burial += (double)burial_term(r2 * idist, idist, ori, iori, aj.x, aj.y);
     }
   }
-  if (!vi) return;
+  red[part][lane] = burial;
+  __syncthreads();
+  if (part != 0 || !vi) return;
+  burial = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
   const double rho = S.gb_rho[i], orid = rho - P.gb_offset;
   const double psi = 0.5 * burial * orid;
   const double tp = P.gb_a * psi - P.gb_b * psi * psi + P.gb_g * psi * psi * psi;
@@ -201,12 +215,14 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_born(SysDev *sys, const int *gp
 // q_i q_l f^-3 / 2 e^{-r^2/4RiRl} (R_l + r^2 / 4 R_i)
 __global__ __launch_bounds__(kGBBlock) void k_gb_partial(SysDev *sys, const int *gp, int nsys, ForceDev P) {
   __shared__ GBShared sh;
+  __shared__ double red[kGBParts][kGBI];
   const int tile = blockIdx.x;
   if (tile >= gp[nsys]) return;
   const int s = find_sys_gb(gp, nsys, tile);
   const SysDev &S = sys[s];
   const int n = S.n_atoms;
-  const int i = (tile - gp[s]) * kGBBlock + threadIdx.x;
+  const int lane = threadIdx.x & (kGBI - 1), part = threadIdx.x / kGBI;
+  const int i = (tile - gp[s]) * kGBI + lane;
   const bool vi = i < n;
   double xi = 0, yi = 0, zi = 0;
   float qi = 0.f, Bi = 1.f;
@@ -225,19 +241,25 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_partial(SysDev *sys, const int 
     __syncthreads();
     const int cnt = min(kGBBlock, n - j0);
     if (!vi) continue;
-    for (int k = 0; k < cnt; ++k) {
+    const int k1 = min(cnt, (part + 1) * kGBI);
+    for (int k = part * kGBI; k < k1; ++k) {
       if (j0 + k == i) continue;
-      const float dx = (float)(sh.x[k] - xi), dy = (float)(sh.y[k] - yi), dz = (float)(sh.z[k] - zi);
+      const double4 pj = sh.pos[k];
+      const float4 aj = sh.a[k];
+      const float qj = sh.b[k].x;
+      const float dx = (float)(pj.x - xi), dy = (float)(pj.y - yi), dz = (float)(pj.z - zi);
       const float r2 = dx * dx + dy * dy + dz * dz;
-      const float Bj = sh.B[k];
+      const float Bj = aj.z;
       const float BB = Bi * Bj;
-      const float ex = __expf(-r2 * (0.25f * iBi * sh.iB[k]));  // exp(-r^2 / (4 Bi Bj))
-      const float inv = rsqrtf(r2 + BB * ex);                     // 1 / f_GB
-      const float part = qi * sh.q[k] * inv * inv * 0.5f * inv * ex;
+      const float ex = __expf(-r2 * (0.25f * iBi * aj.w));  // exp(-r^2 / (4 Bi Bj))
+      const float inv = rsqrtf(r2 + BB * ex);                 // 1 / f_GB
+      const float part = qi * qj * inv * inv * 0.5f * inv * ex;
       acc += (double)(part * (Bj + r2 * i4Bi));
     }
   }
-  if (vi) S.gb_partial[i] = acc;
+  red[part][lane] = acc;
+  __syncthreads();
+  if (part == 0 && vi) S.gb_partial[i] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
 }
 
 // pass 3: GB pair force and energy (GBForce.cpp:14-175), self energy
@@ -251,12 +273,14 @@ template <bool VIR>
 __global__ __launch_bounds__(kGBBlock) void k_gb_force(SysDev *sys, const int *gp, int nsys, ForceDev P, int energies) {
   __shared__ GBShared sh;
   __shared__ double red[VIR ? 15 : 3][kGBBlock / 64];
+  __shared__ double fred[3][kGBParts][kGBI];
   const int tile = blockIdx.x;
   if (tile >= gp[nsys]) return;
   const int s = find_sys_gb(gp, nsys, tile);
   const SysDev &S = sys[s];
   const int n = S.n_atoms;
-  const int i = (tile - gp[s]) * kGBBlock + threadIdx.x;
+  const int lane = threadIdx.x & (kGBI - 1), part = threadIdx.x / kGBI;
+  const int i = (tile - gp[s]) * kGBI + lane;
   const bool vi = i < n;
   const bool peratom = S.lj_mode == PM_LJ_PERATOM;
   double xi = 0, yi = 0, zi = 0;
@@ -298,7 +322,8 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_force(SysDev *sys, const int *g
     __syncthreads();
     const int cnt = min(kGBBlock, n - j0);
     if (!vi) continue;
-    for (int k = 0; k < cnt; ++k) {
+    const int k1 = min(cnt, (part + 1) * kGBI);
+    for (int k = part * kGBI; k < k1; ++k) {
       const int j = j0 + k;
       if (j == i) continue;
       int cls = 0;
@@ -307,35 +332,37 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_force(SysDev *sys, const int *g
         nextx = ep < ee ? S.excl_partner[ep] : 0x7fffffff;
       }
       if (j == nextx) cls = S.excl_class[ep];
-      const float dx = (float)(sh.x[k] - xi), dy = (float)(sh.y[k] - yi), dz = (float)(sh.z[k] - zi);
+      const double4 pj = sh.pos[k];
+      const float4 aj = sh.a[k], bj4 = sh.b[k], cj4 = sh.c[k];
+      const float dx = (float)(pj.x - xi), dy = (float)(pj.y - yi), dz = (float)(pj.z - zi);
       const float r2 = dx * dx + dy * dy + dz * dz;
       const float rinv = rsqrtf(r2);
       const float dist = r2 * rinv;
-      const float Bj = sh.B[k], iBj = sh.iB[k];
+      const float Bj = aj.z, iBj = aj.w;
       const float BB = Bi * Bj;
       const float ex = __expf(-r2 * (i4Bi * iBj));  // exp(-r^2 / (4 Bi Bj))
       const float inv = rsqrtf(r2 + BB * ex);       // 1 / f_GB
-      const float qq = qi * sh.q[k];
-      const float dRi = dri * dburial(dist, rinv, ori, sh.ori[k], sh.S[k]);
-      const float dRj = sh.dr[k] * dburial(dist, rinv, sh.ori[k], ori, Si);
+      const float qq = qi * bj4.x;
+      const float dRi = dri * dburial(dist, rinv, ori, aj.x, aj.y);
+      const float dRj = bj4.z * dburial(dist, rinv, aj.x, ori, Si);
       const float c0 = qq * inv * inv * 0.5f * inv;
       const float ai = Bj + r2 * i4Bi, bj = Bi + r2 * (0.25f * iBj);
       float force = -c0 * ((2.0f * dist - 0.5f * ex * dist) + ex * dRi * ai + ex * dRj * bj) * rinv;
       force -= qiB2 * dRi * rinv;
-      force -= sh.qB2[k] * dRj * rinv;
+      force -= bj4.y * dRj * rinv;
       const float part = c0 * ex;
       force -= (Pi - part * ai) * (dRi * rinv);
-      force -= (sh.part[k] - part * bj) * (dRj * rinv);
+      force -= (bj4.w - part * bj) * (dRj * rinv);
       force *= fac;
       // GBACEForce.cpp:155: c1 (c_i R_i^-7 dR_i/dr + c_j R_j^-7 dR_j/dr) / r
-      if (P.gb_ace) force += ace_c1 * (aki * dRi + sh.ak[k] * dRj) * rinv;
+      if (P.gb_ace) force += ace_c1 * (aki * dRi + cj4.z * dRj) * rinv;
       float fs = force;
       const bool upper = j > i;
       if (energies && upper) egb += (double)(-qq * inv * fac);
       if (cls != PM_EXCL_FULL) {
         float A, Bc;
         if (peratom) {
-          const float sg = sgi + sh.sg[k], e2 = epi * sh.ep[k];
+          const float sg = sgi + cj4.x, e2 = epi * cj4.y;
           const float sg2 = sg * sg, sg6 = sg2 * sg2 * sg2;
           Bc = 4.0f * e2 * sg6;
           A = Bc * sg6;
@@ -344,7 +371,7 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_force(SysDev *sys, const int *g
             Bc *= S.lj_scaling;
           }
         } else {
-          const float4 p = S.lj_table[ti * S.n_types + __float_as_int(sh.sg[k])];
+          const float4 p = S.lj_table[ti * S.n_types + __float_as_int(cj4.x)];
           A = cls == PM_EXCL_MODIFIED ? p.z : p.x;
           Bc = cls == PM_EXCL_MODIFIED ? p.w : p.y;
         }
@@ -364,7 +391,7 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_force(SysDev *sys, const int *g
       fz -= (double)(fs * dz);
       if (VIR && energies && upper) {
         const float px = fs * dx, py = fs * dy, pz = fs * dz;
-        const bool other = sh.mol[k] != moli;
+        const bool other = __float_as_int(cj4.w) != moli;
         if (other || P.virial) {
           vir[0] += (double)(px * dx);
           vir[1] += (double)(px * dy);
@@ -374,7 +401,7 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_force(SysDev *sys, const int *g
           vir[5] += (double)(pz * dz);
         }
         if (other && P.molvirial) {
-          const int mj = sh.mol[k], nm = S.n_mol;
+          const int mj = __float_as_int(cj4.w), nm = S.n_mol;
           const double cx = S.com[mj] - S.com[moli], cy = S.com[nm + mj] - S.com[nm + moli],
                        cz = S.com[2 * nm + mj] - S.com[2 * nm + moli];
           mvir[0] += (double)px * cx;
@@ -387,7 +414,14 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_force(SysDev *sys, const int *g
       }
     }
   }
-  if (vi) {
+  fred[0][part][lane] = fx;
+  fred[1][part][lane] = fy;
+  fred[2][part][lane] = fz;
+  __syncthreads();
+  if (vi && part == 0) {
+    fx = ((fred[0][0][lane] + fred[0][1][lane]) + fred[0][2][lane]) + fred[0][3][lane];
+    fy = ((fred[1][0][lane] + fred[1][1][lane]) + fred[1][2][lane]) + fred[1][3][lane];
+    fz = ((fred[2][0][lane] + fred[2][1][lane]) + fred[2][2][lane]) + fred[2][3][lane];
     atomicAdd(reinterpret_cast<unsigned long long *>(&S.fx[i]), (unsigned long long)__double2ll_rn(fx * kFScale));
     atomicAdd(reinterpret_cast<unsigned long long *>(&S.fy[i]), (unsigned long long)__double2ll_rn(fy * kFScale));
     atomicAdd(reinterpret_cast<unsigned long long *>(&S.fz[i]), (unsigned long long)__double2ll_rn(fz * kFScale));
@@ -410,11 +444,11 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_force(SysDev *sys, const int *g
       v[3 + c] = vir[c];
       v[9 + c] = mvir[c];
     }
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int wv = threadIdx.x >> 6;
   for (int c = 0; c < NV; ++c) {
     double t = v[c];
     for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
-    if (lane == 0) red[c][wv] = t;
+    if ((threadIdx.x & 63) == 0) red[c][wv] = t;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
