@@ -143,7 +143,7 @@ struct SysPrep {
 struct SysHost {
   int id = -1;
   int n = 0, ncol = 0, cl_cap = 0, slot_cap = 0, sc_cap = 0, list_cap = 0, n_corr = 0, n_bonded = 0, n_local = 0;
-  int n_gb_tiles = 0;
+  int n_gb_tiles = 0, n_gb_pairs = 0;  // GB: 64-atom tiles, tile pairs (it <= jt)
   pm::EwLaunch ew{};        // reciprocal-space launch shape (sys filled per launch set)
   double coul_const = 0.0;  // Ewald point-self + charged-system energy (FP64, host)
   double vir_const = 0.0;   // charged-system virial diagonal
@@ -701,6 +701,10 @@ size_t carve(const pm_ctx *c, const SysPrep &p, const pm_system_desc &d, Carver 
     S.gb_drfac = cv.take<double>(n);
     S.gb_partial = cv.take<double>(n);
     S.gb_spare = cv.take<double>(n);
+    // per tile pair (it <= jt): the i-side and j-side FP64 force partials of
+    // its 64 + 64 atoms, summed per atom in a fixed order by k_gb_fsum
+    S.gb_nt = p.n_gb_tiles;
+    S.gb_fpart = cv.take<double>((size_t)pm::gb_pair_count(p.n_gb_tiles) * pm::kGBPairDoubles);
     return cv.off;
   }
   S.col_count = cv.take<int>(p.ncol);
@@ -816,6 +820,7 @@ int add_system(pm_ctx *c, const pm_system_desc &d) {
   h.n_bonded = p.n_bonds + p.n_angles;
   h.n_local = (int)p.loc_item.size();
   h.n_gb_tiles = p.n_gb_tiles;
+  h.n_gb_pairs = c->gb ? pm::gb_pair_count(p.n_gb_tiles) : 0;
   h.coul_const = p.coul_const;
   h.vir_const = p.vir_const;
   S.ncx = p.ncx;
@@ -947,12 +952,13 @@ LaunchSet &get_set(pm_ctx *c, const std::vector<int> &ids) {
   ls->ids = ids;
   const int ns = (int)ids.size();
   std::vector<SysDev> table(ns);
-  constexpr int kPre = 9;
+  constexpr int kPre = 10;
   std::vector<std::vector<int>> pre(kPre, std::vector<int>(1, 0));
   for (int k = 0; k < ns; ++k) {
     const SysHost &h = *c->sys[ids[k]];
     table[k] = h.dev;
-    const int cnt[kPre] = {h.n, h.cl_cap, h.ncol, h.slot_cap, h.n_corr, h.n_bonded, h.n_gb_tiles, h.n_local, h.sc_cap};
+    const int cnt[kPre] = {h.n, h.cl_cap, h.ncol, h.slot_cap, h.n_corr, h.n_bonded, h.n_gb_tiles, h.n_local, h.sc_cap,
+                           h.n_gb_pairs};
     for (int q = 0; q < kPre; ++q) pre[q].push_back(pre[q].back() + cnt[q]);
   }
   // one allocation: [SysDev x ns][kPre prefix arrays of ns + 1][kScalN int64 x ns]
@@ -981,6 +987,7 @@ LaunchSet &get_set(pm_ctx *c, const std::vector<int> &ids) {
   L.corr_prefix = d_pre + 4 * st;
   L.bonded_prefix = d_pre + 5 * st;
   L.gb_prefix = d_pre + 6 * st;
+  L.gbp_prefix = d_pre + 9 * st;
   L.sc_prefix = d_pre + 8 * st;
   L.n_atoms_total = pre[0].back();
   L.n_cl_total = pre[1].back();
@@ -989,6 +996,7 @@ LaunchSet &get_set(pm_ctx *c, const std::vector<int> &ids) {
   L.n_corr_total = pre[4].back();
   L.n_bonded_total = pre[5].back();
   L.n_gb_tiles_total = pre[6].back();
+  L.n_gb_pairs_total = pre[9].back();
   L.n_local_total = pre[7].back();
   L.n_sc_total = pre[8].back();
   for (int k = 0; k < ns; ++k) L.max_mol = std::max(L.max_mol, c->sys[ids[k]]->dev.n_mol);

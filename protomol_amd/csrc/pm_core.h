@@ -179,6 +179,8 @@ struct SysDev {
   // GB
   const double *gb_rho, *gb_S;
   double *gb_born, *gb_drfac, *gb_partial, *gb_spare;
+  int gb_nt;               // 64-atom tiles
+  double *gb_fpart;        // per tile pair (it <= jt): [i side, j side][x, y, z][64] FP64 force partials
   // per-system scalars
   int *n_clusters, *n_sc;
   long long *acc;          // kAccCopies x kAccN fixed-point energy/virial accumulators

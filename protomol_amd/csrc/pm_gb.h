@@ -11,6 +11,9 @@ constexpr int kGBParts = kGBBlock / kGBI;  // waves per workgroup: each takes a 
 static_assert(kGBParts == 4, "the partial sums are added as a fixed 4-term expression");
 
 inline int gb_tile_count(int n) { return (n + kGBI - 1) / kGBI; }
+// the force pass visits each unordered pair once: tile pairs (it, jt), it <= jt
+inline int gb_pair_count(int nt) { return nt * (nt + 1) / 2; }
+constexpr int kGBPairDoubles = 2 * 3 * kGBI;  // partial forces per tile pair
 
 void gb_setup_force(ForceDev &P, const pm_force_desc &d);
 void gb_setup_system(SysDev &S, const double *rho, const double *scaling, double *born, double *drfac,

@@ -59,6 +59,8 @@ __device__ __forceinline__ int find_sys_gb(const int *__restrict__ prefix, int n
   return lo;
 }
 
+__device__ __forceinline__ unsigned long long wave_ballot_gb(bool p) { return __ballot(p); }
+
 // reciprocal by the hardware instruction (1 ulp)
 __device__ __forceinline__ float rcp(float v) { return __builtin_amdgcn_rcpf(v); }
 
@@ -262,31 +264,68 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_partial(SysDev *sys, const int 
   if (part == 0 && vi) S.gb_partial[i] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
 }
 
-// pass 3: GB pair force and energy (GBForce.cpp:14-175), self energy
-// (GBForce.cpp:50-60, 230-242), plus the all-pairs Lennard-Jones and
-// Coulomb terms (OneAtomPair<Vacuum, Universal, ...> with the exclusion table).
-// VIR: with the energies, the virials of each pair i < j, counted by i's lane
-// (OneAtomPairNoExclusion.h:73-81, OneAtomPair.h:95-103): f (x) d to the
-// atomic virial, and f (x) (R_J - R_I) to the molecular one for pairs of two
-// molecules
+// pass 3: GB pair force and energy (GBForce.cpp:14-175) plus the all-pairs
+// Lennard-Jones and Coulomb terms (OneAtomPair<Vacuum, Universal, ...> with
+// the exclusion table), each unordered pair once, as the reference's pair
+// loop visits it.  One wave per 64 x 64 tile pair (it <= jt): lane l holds i
+// atom it*64+l in registers; at step r it meets j atom jt*64+((l+r)&63), read
+// from the wave's copy of the j tile in LDS, and adds the pair force to its
+// own i sum and to a j sum that travels one lane down after each step, so
+// after the sweep every j sum sits in a known lane -- no atomics.  A diagonal
+// tile takes steps 1..32 (step 32 on lanes 0..31 only), which visits each of
+// its pairs once.  Both sides go to the tile pair's slot of gb_fpart;
+// k_gb_fsum adds an atom's partials in a fixed order.
+// VIR: with the energies, the virials of each pair (OneAtomPairNoExclusion.h:
+// 73-81, OneAtomPair.h:95-103): f (x) d to the atomic virial, and f (x)
+// (R_J - R_I) to the molecular one for pairs of two molecules.
+constexpr int kGBPWaves = 4;
 template <bool VIR>
-__global__ __launch_bounds__(kGBBlock) void k_gb_force(SysDev *sys, const int *gp, int nsys, ForceDev P, int energies) {
-  __shared__ GBShared sh;
-  __shared__ double red[VIR ? 15 : 3][kGBBlock / 64];
-  __shared__ double fred[3][kGBParts][kGBI];
-  const int tile = blockIdx.x;
-  if (tile >= gp[nsys]) return;
-  const int s = find_sys_gb(gp, nsys, tile);
-  const SysDev &S = sys[s];
-  const int n = S.n_atoms;
-  const int lane = threadIdx.x & (kGBI - 1), part = threadIdx.x / kGBI;
-  const int i = (tile - gp[s]) * kGBI + lane;
-  const bool vi = i < n;
+__global__ __launch_bounds__(kGBPWaves * 64) void k_gb_pairs(SysDev *sys, const int *pp, int nsys, ForceDev P,
+                                                             int energies) {
+  __shared__ double4 jpos[kGBPWaves][kGBI];
+  __shared__ float4 ja[kGBPWaves][kGBI], jb[kGBPWaves][kGBI], jc[kGBPWaves][kGBI];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int w = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (w >= pp[nsys]) return;
+  const int s = find_sys_gb(pp, nsys, w);
+  SysDev &S = sys[s];
+  const int n = S.n_atoms, nt = S.gb_nt;
+  int p = w - pp[s], it = 0;
+  while (p >= nt - it) {  // row it holds tile pairs (it, it..nt-1)
+    p -= nt - it;
+    ++it;
+  }
+  const int jt = it + p;
+  const bool diag = it == jt;
   const bool peratom = S.lj_mode == PM_LJ_PERATOM;
+  // the j tile into this wave's LDS records
+  {
+    const int j = jt * kGBI + lane;
+    double4 pos = make_double4(0.0, 0.0, 0.0, 0.0);
+    float4 a = make_float4(1.f, 1.f, 1.f, 1.f), b = make_float4(0.f, 0.f, 0.f, 0.f), c = b;
+    if (j < n) {
+      pos = make_double4(S.x[j], S.x[n + j], S.x[2 * n + j], 0.0);
+      const float q = (float)S.charge[j], B = (float)S.gb_born[j], iB = rcp(B);
+      a = make_float4((float)(S.gb_rho[j] - P.gb_offset), (float)S.gb_S[j], B, iB);
+      b = make_float4(q, 0.5f * q * q * iB * iB, (float)S.gb_drfac[j], (float)S.gb_partial[j]);
+      c.x = peratom ? S.sig_half[j] : __int_as_float(S.type[j]);
+      c.y = peratom ? S.eps_sqrt[j] : 0.f;
+      c.z = P.gb_ace ? (float)S.gb_spare[j] : 0.f;
+      c.w = __int_as_float(S.molecule[j]);
+    }
+    jpos[wv][lane] = pos;
+    ja[wv][lane] = a;
+    jb[wv][lane] = b;
+    jc[wv][lane] = c;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int i = it * kGBI + lane;
+  const bool vi = i < n;
   double xi = 0, yi = 0, zi = 0;
-  float qi = 0.f, Bi = 1.f, ori = 1.f, Si = 1.f, dri = 0.f, Pi = 0.f, sgi = 0.f, epi = 0.f;
-  int ti = 0;
-  int ep = 0, ee = 0, nextx = 0x7fffffff;
+  float qi = 0.f, Bi = 1.f, ori = 1.f, Si = 1.f, dri = 0.f, Pi = 0.f, sgi = 0.f, epi = 0.f, aki = 0.f;
+  int ti = 0, moli = 0, ep0 = 0, ee = 0, pmin = 0x7fffffff, pmax = -1;
   if (vi) {
     xi = S.x[i];
     yi = S.x[n + i];
@@ -303,37 +342,36 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_force(SysDev *sys, const int *g
     } else {
       ti = S.type[i];
     }
-    ep = S.excl_start[i];
+    aki = P.gb_ace ? (float)S.gb_spare[i] : 0.f;
+    moli = S.molecule[i];
+    ep0 = S.excl_start[i];
     ee = S.excl_start[i + 1];
-    if (ep < ee) nextx = S.excl_partner[ep];
+    if (ep0 < ee) {  // partners are sorted
+      pmin = S.excl_partner[ep0];
+      pmax = S.excl_partner[ee - 1];
+    }
   }
+  // does any pair of this tile pair carry an exclusion? (wave-uniform)
+  const bool any_ex = wave_ballot_gb(pmax >= jt * kGBI && pmin < (jt + 1) * kGBI) != 0;
   const float fac = (float)P.gb_fac;
   const float iBi = rcp(Bi), i4Bi = 0.25f * iBi;
   const float qiB2 = 0.5f * qi * qi * iBi * iBi;  // q_i^2 / (2 B_i^2)
   const float csf = S.csf;
-  const float aki = (vi && P.gb_ace) ? (float)S.gb_spare[i] : 0.f;
   const float ace_c1 = (float)P.gb_ace_c1;
-  double fx = 0.0, fy = 0.0, fz = 0.0, egb = 0.0, elj = 0.0, eco = 0.0;
+  double fx = 0.0, fy = 0.0, fz = 0.0, gx = 0.0, gy = 0.0, gz = 0.0, egb = 0.0, elj = 0.0, eco = 0.0;
   double vir[6] = {0, 0, 0, 0, 0, 0}, mvir[6] = {0, 0, 0, 0, 0, 0};
-  const int moli = vi ? S.molecule[i] : 0;
-  for (int j0 = 0; j0 < n; j0 += kGBBlock) {
-    __syncthreads();
-    load_chunk(sh, S, P, j0, 3);
-    __syncthreads();
-    const int cnt = min(kGBBlock, n - j0);
-    if (!vi) continue;
-    const int k1 = min(cnt, (part + 1) * kGBI);
-    for (int k = part * kGBI; k < k1; ++k) {
-      const int j = j0 + k;
-      if (j == i) continue;
+  const int r0 = diag ? 1 : 0, r1 = diag ? 32 : 63;
+  for (int r = r0; r <= r1; ++r) {
+    const int jl = (lane + r) & 63;
+    const int j = jt * kGBI + jl;
+    const bool ok = vi && j < n && !(diag && r == 32 && lane >= 32);
+    if (ok) {
       int cls = 0;
-      while (j > nextx) {  // partners are sorted; skip ones passed already
-        ++ep;
-        nextx = ep < ee ? S.excl_partner[ep] : 0x7fffffff;
-      }
-      if (j == nextx) cls = S.excl_class[ep];
-      const double4 pj = sh.pos[k];
-      const float4 aj = sh.a[k], bj4 = sh.b[k], cj4 = sh.c[k];
+      if (any_ex)
+        for (int q = ep0; q < ee; ++q)
+          if (S.excl_partner[q] == j) cls = S.excl_class[q];
+      const double4 pj = jpos[wv][jl];
+      const float4 aj = ja[wv][jl], bj4 = jb[wv][jl], cj4 = jc[wv][jl];
       const float dx = (float)(pj.x - xi), dy = (float)(pj.y - yi), dz = (float)(pj.z - zi);
       const float r2 = dx * dx + dy * dy + dz * dz;
       const float rinv = rsqrtf(r2);
@@ -357,8 +395,7 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_force(SysDev *sys, const int *g
       // GBACEForce.cpp:155: c1 (c_i R_i^-7 dR_i/dr + c_j R_j^-7 dR_j/dr) / r
       if (P.gb_ace) force += ace_c1 * (aki * dRi + cj4.z * dRj) * rinv;
       float fs = force;
-      const bool upper = j > i;
-      if (energies && upper) egb += (double)(-qq * inv * fac);
+      if (energies) egb += (double)(-qq * inv * fac);
       if (cls != PM_EXCL_FULL) {
         float A, Bc;
         if (peratom) {
@@ -371,9 +408,9 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_force(SysDev *sys, const int *g
             Bc *= S.lj_scaling;
           }
         } else {
-          const float4 p = S.lj_table[ti * S.n_types + __float_as_int(cj4.x)];
-          A = cls == PM_EXCL_MODIFIED ? p.z : p.x;
-          Bc = cls == PM_EXCL_MODIFIED ? p.w : p.y;
+          const float4 q4 = S.lj_table[ti * S.n_types + __float_as_int(cj4.x)];
+          A = cls == PM_EXCL_MODIFIED ? q4.z : q4.x;
+          Bc = cls == PM_EXCL_MODIFIED ? q4.w : q4.y;
         }
         const float r2inv = rinv * rinv;
         const float r6 = r2inv * r2inv * r2inv;
@@ -381,16 +418,19 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_force(SysDev *sys, const int *g
         float ec = qq * rinv;
         if (cls == PM_EXCL_MODIFIED) ec *= csf;
         fs += (12.0f * r12A - 6.0f * r6B) * r2inv + ec * r2inv;
-        if (energies && upper) {
+        if (energies) {
           elj += (double)(r12A - r6B);
           eco += (double)ec;
         }
       }
-      fx -= (double)(fs * dx);
-      fy -= (double)(fs * dy);
-      fz -= (double)(fs * dz);
-      if (VIR && energies && upper) {
-        const float px = fs * dx, py = fs * dy, pz = fs * dz;
+      const float px = fs * dx, py = fs * dy, pz = fs * dz;
+      fx -= (double)px;
+      fy -= (double)py;
+      fz -= (double)pz;
+      gx += (double)px;
+      gy += (double)py;
+      gz += (double)pz;
+      if (VIR && energies) {
         const bool other = __float_as_int(cj4.w) != moli;
         if (other || P.virial) {
           vir[0] += (double)(px * dx);
@@ -413,27 +453,21 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_force(SysDev *sys, const int *g
         }
       }
     }
+    // the j sums move one lane down: lane l then holds the sum of j (l + r + 1) & 63
+    gx = __shfl(gx, (lane + 1) & 63);
+    gy = __shfl(gy, (lane + 1) & 63);
+    gz = __shfl(gz, (lane + 1) & 63);
   }
-  fred[0][part][lane] = fx;
-  fred[1][part][lane] = fy;
-  fred[2][part][lane] = fz;
-  __syncthreads();
-  if (vi && part == 0) {
-    fx = ((fred[0][0][lane] + fred[0][1][lane]) + fred[0][2][lane]) + fred[0][3][lane];
-    fy = ((fred[1][0][lane] + fred[1][1][lane]) + fred[1][2][lane]) + fred[1][3][lane];
-    fz = ((fred[2][0][lane] + fred[2][1][lane]) + fred[2][2][lane]) + fred[2][3][lane];
-    atomicAdd(reinterpret_cast<unsigned long long *>(&S.fx[i]), (unsigned long long)__double2ll_rn(fx * kFScale));
-    atomicAdd(reinterpret_cast<unsigned long long *>(&S.fy[i]), (unsigned long long)__double2ll_rn(fy * kFScale));
-    atomicAdd(reinterpret_cast<unsigned long long *>(&S.fz[i]), (unsigned long long)__double2ll_rn(fz * kFScale));
-    if (energies) egb += -0.5 * (double)S.charge[i] * (double)S.charge[i] * (1.0 / S.gb_born[i]) * P.gb_fac;
-    if (energies && P.gb_ace) {
-      // GBACEForce.cpp:40-50, 160-170: 4 PI sigma (rho_i + rho_s)^2 (rho_i / R_i)^6
-      const double rho = S.gb_rho[i], rs = rho + P.gb_ace_rho_s, p = rho / S.gb_born[i], p3 = p * p * p;
-      egb += 4 * kAcePi * P.gb_ace_sigma * rs * rs * (p3 * p3);
-    }
-  }
+  double *slot = S.gb_fpart + (size_t)(w - pp[s]) * kGBPairDoubles;
+  slot[lane] = fx;
+  slot[kGBI + lane] = fy;
+  slot[2 * kGBI + lane] = fz;
+  const int jf = (lane + r1 + 1) & 63;
+  slot[3 * kGBI + jf] = gx;
+  slot[4 * kGBI + jf] = gy;
+  slot[5 * kGBI + jf] = gz;
   if (!energies) return;
-  // block reduction in a fixed order, then fixed-point totals
+  // wave sums in a fixed order, then fixed-point totals
   constexpr int NV = VIR ? 15 : 3;
   double v[NV];
   v[0] = egb;
@@ -444,29 +478,62 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_force(SysDev *sys, const int *g
       v[3 + c] = vir[c];
       v[9 + c] = mvir[c];
     }
-  const int wv = threadIdx.x >> 6;
-  for (int c = 0; c < NV; ++c) {
-    double t = v[c];
-    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
-    if ((threadIdx.x & 63) == 0) red[c][wv] = t;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t[NV];
-    for (int c = 0; c < NV; ++c) {
-      t[c] = 0.0;
-      for (int w = 0; w < kGBBlock / 64; ++w) t[c] += red[c][w];
-    }
-    long long *acc = acc_copy(S.acc, (int)blockIdx.x);
-    atomicAdd(reinterpret_cast<unsigned long long *>(&acc[kAccCoul]), (unsigned long long)__double2ll_rn((t[0] + t[2]) * kEScale));
-    atomicAdd(reinterpret_cast<unsigned long long *>(&acc[kAccLJ]), (unsigned long long)__double2ll_rn(t[1] * kEScale));
+  for (int c = 0; c < NV; ++c)
+    for (int o = 32; o > 0; o >>= 1) v[c] += __shfl_xor(v[c], o);
+  if (lane == 0) {
+    long long *acc = acc_copy(S.acc, w);
+    atomicAdd(reinterpret_cast<unsigned long long *>(&acc[kAccCoul]), (unsigned long long)__double2ll_rn((v[0] + v[2]) * kEScale));
+    atomicAdd(reinterpret_cast<unsigned long long *>(&acc[kAccLJ]), (unsigned long long)__double2ll_rn(v[1] * kEScale));
     if (VIR) {
       const int vs[12] = {kAccVxx, kAccVxy, kAccVxz, kAccVyy, kAccVyz, kAccVzz,
                           kAccMVxx, kAccMVxy, kAccMVxz, kAccMVyy, kAccMVyz, kAccMVzz};
       for (int c = 0; c < 12; ++c)
-        atomicAdd(reinterpret_cast<unsigned long long *>(&acc[vs[c]]), (unsigned long long)__double2ll_rn(t[3 + c] * kEScale));
+        atomicAdd(reinterpret_cast<unsigned long long *>(&acc[vs[c]]), (unsigned long long)__double2ll_rn(v[3 + c] * kEScale));
     }
   }
+}
+
+// Per atom: its force partials of every tile pair, in tile order (the
+// j side of (u, t) for u < t, both sides of (t, t), the i side of (t, u) for
+// u > t), into the fixed-point slots; with the energies, the GB self energy
+// (GBForce.cpp:50-60, 230-242) and the ACE energy (GBACEForce.cpp:40-50,
+// 160-170) of the atom.
+__global__ __launch_bounds__(kGBI) void k_gb_fsum(SysDev *sys, const int *gp, int nsys, ForceDev P, int energies) {
+  const int tile = blockIdx.x;
+  if (tile >= gp[nsys]) return;
+  const int s = find_sys_gb(gp, nsys, tile);
+  SysDev &S = sys[s];
+  const int n = S.n_atoms, nt = S.gb_nt;
+  const int t = tile - gp[s], lane = threadIdx.x;
+  const int i = t * kGBI + lane;
+  double e = 0.0;
+  if (i < n) {
+    double f[3] = {0.0, 0.0, 0.0};
+    for (int u = 0; u < nt; ++u) {
+      const int a = min(u, t), b = max(u, t);
+      const double *slot = S.gb_fpart + (size_t)(a * nt - a * (a - 1) / 2 + (b - a)) * kGBPairDoubles;
+      for (int c = 0; c < 3; ++c) {
+        if (u >= t) f[c] += slot[c * kGBI + lane];        // i side of (t, u)
+        if (u <= t) f[c] += slot[(3 + c) * kGBI + lane];  // j side of (u, t)
+      }
+    }
+    S.fx[i] += __double2ll_rn(f[0] * kFScale);
+    S.fy[i] += __double2ll_rn(f[1] * kFScale);
+    S.fz[i] += __double2ll_rn(f[2] * kFScale);
+    if (energies) {
+      e = -0.5 * (double)S.charge[i] * (double)S.charge[i] * (1.0 / S.gb_born[i]) * P.gb_fac;
+      if (P.gb_ace) {
+        // 4 PI sigma (rho_i + rho_s)^2 (rho_i / R_i)^6
+        const double rho = S.gb_rho[i], rs = rho + P.gb_ace_rho_s, p = rho / S.gb_born[i], p3 = p * p * p;
+        e += 4 * kAcePi * P.gb_ace_sigma * rs * rs * (p3 * p3);
+      }
+    }
+  }
+  if (!energies) return;
+  for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o);
+  if (lane == 0)
+    atomicAdd(reinterpret_cast<unsigned long long *>(&acc_copy(S.acc, tile)[kAccCoul]),
+              (unsigned long long)__double2ll_rn(e * kEScale));
 }
 
 void launch_gb(const Launch &L, const ForceDev &P, bool energies) {
@@ -474,10 +541,13 @@ void launch_gb(const Launch &L, const ForceDev &P, bool energies) {
   dim3 g(L.n_gb_tiles_total), b(kGBBlock);
   hipLaunchKernelGGL(k_gb_born, g, b, 0, L.stream, L.sys, L.gb_prefix, L.nsys, P);
   hipLaunchKernelGGL(k_gb_partial, g, b, 0, L.stream, L.sys, L.gb_prefix, L.nsys, P);
+  dim3 gp2((L.n_gb_pairs_total + kGBPWaves - 1) / kGBPWaves), bp(kGBPWaves * 64);
   if (energies && (P.virial || P.molvirial))
-    hipLaunchKernelGGL(k_gb_force<true>, g, b, 0, L.stream, L.sys, L.gb_prefix, L.nsys, P, 1);
+    hipLaunchKernelGGL(k_gb_pairs<true>, gp2, bp, 0, L.stream, L.sys, L.gbp_prefix, L.nsys, P, 1);
   else
-    hipLaunchKernelGGL(k_gb_force<false>, g, b, 0, L.stream, L.sys, L.gb_prefix, L.nsys, P, energies ? 1 : 0);
+    hipLaunchKernelGGL(k_gb_pairs<false>, gp2, bp, 0, L.stream, L.sys, L.gbp_prefix, L.nsys, P, energies ? 1 : 0);
+  hipLaunchKernelGGL(k_gb_fsum, dim3(L.n_gb_tiles_total), dim3(kGBI), 0, L.stream, L.sys, L.gb_prefix, L.nsys, P,
+                     energies ? 1 : 0);
 }
 
 }  // namespace pm

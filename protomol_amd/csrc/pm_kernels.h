@@ -12,9 +12,10 @@ struct Launch {
   SysDev *sys = nullptr;
   int nsys = 0;
   const int *atom_prefix = nullptr, *cl_prefix = nullptr, *col_prefix = nullptr, *slot_prefix = nullptr,
-            *corr_prefix = nullptr, *bonded_prefix = nullptr, *gb_prefix = nullptr, *sc_prefix = nullptr;
+            *corr_prefix = nullptr, *bonded_prefix = nullptr, *gb_prefix = nullptr, *sc_prefix = nullptr,
+            *gbp_prefix = nullptr;  // GB tile pairs
   int n_atoms_total = 0, n_cl_total = 0, n_cols_total = 0, n_slots_total = 0, n_corr_total = 0,
-      n_bonded_total = 0, n_gb_tiles_total = 0, n_local_total = 0, n_sc_total = 0;
+      n_bonded_total = 0, n_gb_tiles_total = 0, n_local_total = 0, n_sc_total = 0, n_gb_pairs_total = 0;
   const struct EwLaunch *ew = nullptr;  // per system with a reciprocal-space term (host array)
   int n_ew = 0;
   int max_mol = 0;                      // most molecules of a system (molecular virial; 0 = off)
