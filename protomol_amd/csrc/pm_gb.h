@@ -5,10 +5,7 @@
 
 namespace pm {
 
-constexpr int kGBBlock = 256;  // threads per workgroup = j atoms staged in LDS per chunk
-constexpr int kGBI = 64;       // i atoms per workgroup (one per lane of each wave)
-constexpr int kGBParts = kGBBlock / kGBI;  // waves per workgroup: each takes a quarter of every chunk
-static_assert(kGBParts == 4, "the partial sums are added as a fixed 4-term expression");
+constexpr int kGBI = 64;  // atoms per tile (one per lane)
 
 inline int gb_tile_count(int n) { return (n + kGBI - 1) / kGBI; }
 // the force pass visits each unordered pair once: tile pairs (it, jt), it <= jt

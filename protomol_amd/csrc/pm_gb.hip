@@ -3,14 +3,12 @@
 // ProtoMol evaluates GB as three dependent all-pairs passes over
 // NonbondedSimpleFullSystemForce (GBBornRadii -> GBPartialSum -> GBForce) and
 // stores ten N x N intermediate arrays per system (topology/Atom.h:62-83).
-// Here each pass recomputes what it needs instead: one workgroup owns 64
-// i atoms (one per lane of each of its four waves) and streams every j atom
-// through LDS; wave p takes the p-th quarter of every j chunk, and the four
-// partial sums of an i atom are added in wave order, so its Born radius,
-// partial sum and force are complete in the workgroup.  No N x N storage, no
-// atomics inside a pass, a fixed summation order per atom (bitwise
-// reproducible), FP32 pair arithmetic on FP64 coordinate differences, FP64
-// accumulation and FP64 per-atom Born-radius algebra.
+// Here each pass recomputes what it needs instead, visiting each unordered
+// pair once (64 x 64 tile pairs, one wave each) and keeping per tile pair
+// only the 64 + 64 per-atom partial sums, which a per-atom kernel adds in a
+// fixed order.  No N x N pair storage, no atomics, a fixed summation order
+// per atom (bitwise reproducible), FP32 pair arithmetic on FP64 coordinate
+// differences, FP64 accumulation and FP64 per-atom Born-radius algebra.
 #include <hip/hip_runtime.h>
 #include <math.h>
 
@@ -108,98 +106,151 @@ __device__ __forceinline__ float dburial(float dist, float idist, float ori, flo
          0.25f * s2 * idist * iU2 * iU * dU - 0.25f * s2 * idist * iL2 * iL * dL + dC;
 }
 
-// A chunk of j atoms in LDS, one record of four 16-byte words per atom so a
-// pair reads its j data with a few wide loads:
-//   pos  x, y, z (FP64)
-//   a    rho~ = rho - offset, S, B (Born radius), 1/B
-//   b    q, q^2 / (2 B^2), dR factor, partial sum
-//   c    sigma/2 (or the LJ type's bits), sqrt(eps), ACE c / R^7, molecule's bits
-struct GBShared {
-  double4 pos[kGBBlock];
-  float4 a[kGBBlock], b[kGBBlock], c[kGBBlock];
-};
-
-__device__ void load_chunk(GBShared &sh, const SysDev &S, const ForceDev &P, int j0, int pass) {
-  const int t = threadIdx.x, n = S.n_atoms;
-  const int j = j0 + t;
-  if (j < n) {
-    sh.pos[t] = make_double4(S.x[j], S.x[n + j], S.x[2 * n + j], 0.0);
-    float4 a = make_float4((float)(S.gb_rho[j] - P.gb_offset), (float)S.gb_S[j], 1.f, 1.f);
-    if (pass >= 2) {
-      const float q = (float)S.charge[j], B = (float)S.gb_born[j], iB = rcp(B);
-      a.z = B;
-      a.w = iB;
-      float4 b = make_float4(q, 0.5f * q * q * iB * iB, 0.f, 0.f);  // q^2 / (2 B^2)
-      if (pass >= 3) {
-        b.z = (float)S.gb_drfac[j];
-        b.w = (float)S.gb_partial[j];
-        float4 c;
-        if (S.lj_mode == PM_LJ_PERATOM) {
-          c.x = S.sig_half[j];
-          c.y = S.eps_sqrt[j];
-        } else {
-          c.x = __int_as_float(S.type[j]);
-          c.y = 0.f;
-        }
-        c.z = P.gb_ace ? (float)S.gb_spare[j] : 0.f;
-        c.w = __int_as_float(S.molecule[j]);
-        sh.c[t] = c;
-      }
-      sh.b[t] = b;
-    }
-    sh.a[t] = a;
-  }
-}
-
 }  // namespace
 
-// pass 1: burial sums, then the OBC Born radius (GBBornRadii::postProcess,
-// GBBornRadii.cpp:145-173) and the per-atom factor of dR_i/dr_ij
-// (GBForce.cpp:126: R_i^2 rho~_i (1 - tanh^2) dtanh/dpsi / rho_i)
-__global__ __launch_bounds__(kGBBlock) void k_gb_born(SysDev *sys, const int *gp, int nsys, ForceDev P) {
-  __shared__ GBShared sh;
-  __shared__ double red[kGBParts][kGBI];
-  const int tile = blockIdx.x;
-  if (tile >= gp[nsys]) return;
-  const int s = find_sys_gb(gp, nsys, tile);
+// The three passes visit each unordered pair once, in 64 x 64 tile pairs
+// (it <= jt), one wave per tile pair: lane l holds i atom it*64+l in
+// registers; at step r it meets j atom jt*64+((l+r)&63), read from the wave's
+// copy of the j tile in LDS, and adds the pair's terms to its own i sum and
+// to a j sum that travels one lane down after each step, so after the sweep
+// every j sum sits in a known lane -- no atomics.  A diagonal tile takes
+// steps 1..32 (step 32 on lanes 0..31 only), which visits each of its pairs
+// once.  Both sides go to the tile pair's slot of gb_fpart, and a per-atom
+// kernel adds an atom's partials over the tile pairs in a fixed order.
+constexpr int kGBPWaves = 4;
+
+// tile pair (it, jt) of the wave's index p within its system (row-major over it <= jt)
+__device__ __forceinline__ void gb_tile_pair(int p, int nt, int &it, int &jt) {
+  it = 0;
+  while (p >= nt - it) {
+    p -= nt - it;
+    ++it;
+  }
+  jt = it + p;
+}
+
+// slot of tile pair (a, b), a <= b
+__device__ __forceinline__ double *gb_slot(const SysDev &S, int a, int b) {
+  return S.gb_fpart + (size_t)(a * S.gb_nt - a * (a - 1) / 2 + (b - a)) * kGBPairDoubles;
+}
+
+// an atom's partials over the tile pairs, in tile order: the j side of (u, t)
+// for u < t, both sides of (t, t), the i side of (t, u) for u > t; side
+// offsets: i side at `io`, j side at `jo` within a slot
+__device__ __forceinline__ double gb_atom_sum(const SysDev &S, int t, int lane, int io, int jo) {
+  double v = 0.0;
+  for (int u = 0; u < S.gb_nt; ++u) {
+    const double *slot = gb_slot(S, min(u, t), max(u, t));
+    if (u >= t) v += slot[io + lane];
+    if (u <= t) v += slot[jo + lane];
+  }
+  return v;
+}
+
+// passes 1 and 2: PASS 1 the burial terms of both atoms of a pair
+// (GBBornRadii.cpp:15-128), PASS 2 their partial-sum terms
+// (GBPartialSum.cpp:11-42: q_i q_l f^-3 / 2 e^{-r^2/4RiRl} (R_l + r^2 / 4 R_i))
+template <int PASS>
+__global__ __launch_bounds__(kGBPWaves * 64) void k_gb_sym(SysDev *sys, const int *pp, int nsys, ForceDev P) {
+  __shared__ double4 jpos[kGBPWaves][kGBI];
+  __shared__ float4 ja[kGBPWaves][kGBI];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int w = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (w >= pp[nsys]) return;
+  const int s = find_sys_gb(pp, nsys, w);
   const SysDev &S = sys[s];
   const int n = S.n_atoms;
-  const int lane = threadIdx.x & (kGBI - 1), part = threadIdx.x / kGBI;
-  const int i = (tile - gp[s]) * kGBI + lane;
+  int it, jt;
+  gb_tile_pair(w - pp[s], S.gb_nt, it, jt);
+  const bool diag = it == jt;
+  {
+    const int j = jt * kGBI + lane;
+    double4 pos = make_double4(0.0, 0.0, 0.0, 0.0);
+    float4 a = make_float4(1.f, 1.f, 1.f, 1.f);
+    if (j < n) {
+      pos = make_double4(S.x[j], S.x[n + j], S.x[2 * n + j], 0.0);
+      if (PASS == 1) {
+        const float o = (float)(S.gb_rho[j] - P.gb_offset);
+        a = make_float4(o, (float)S.gb_S[j], rcp(o), 0.f);
+      } else {
+        const float B = (float)S.gb_born[j];
+        a = make_float4((float)S.charge[j], B, rcp(B), 0.f);
+      }
+    }
+    jpos[wv][lane] = pos;
+    ja[wv][lane] = a;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int i = it * kGBI + lane;
   const bool vi = i < n;
   double xi = 0, yi = 0, zi = 0;
-  float ori = 1.f;
+  float u0 = 1.f, u1 = 1.f, u2 = 1.f;  // PASS 1: rho~, S, 1/rho~; PASS 2: q, B, 1/B
   if (vi) {
     xi = S.x[i];
     yi = S.x[n + i];
     zi = S.x[2 * n + i];
-    ori = (float)(S.gb_rho[i] - P.gb_offset);
-  }
-  double burial = 0.0;
-  for (int j0 = 0; j0 < n; j0 += kGBBlock) {
-    __syncthreads();
-    load_chunk(sh, S, P, j0, 1);
-    __syncthreads();
-    const int cnt = min(kGBBlock, n - j0);
-    if (!vi) continue;
-    const float iori = rcp(ori);
-    const int k1 = min(cnt, (part + 1) * kGBI);
-    for (int k = part * kGBI; k < k1; ++k) {
-      if (j0 + k == i) continue;
-      const double4 pj = sh.pos[k];
-      const float4 aj = sh.a[k];
-      const float dx = (float)(pj.x - xi), dy = (float)(pj.y - yi), dz = (float)(pj.z - zi);
-      const float r2 = dx * dx + dy * dy + dz * dz;
-      const float idist = rsqrtf(r2);
-      burial += (double)burial_term(r2 * idist, idist, ori, iori, aj.x, aj.y);
+    if (PASS == 1) {
+      u0 = (float)(S.gb_rho[i] - P.gb_offset);
+      u1 = (float)S.gb_S[i];
+      u2 = rcp(u0);
+    } else {
+      u0 = (float)S.charge[i];
+      u1 = (float)S.gb_born[i];
+      u2 = rcp(u1);
     }
   }
-  red[part][lane] = burial;
-  __syncthreads();
-  if (part != 0 || !vi) return;
-  burial = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+  double si = 0.0, sj = 0.0;
+  const int r0 = diag ? 1 : 0, r1 = diag ? 32 : 63;
+  for (int r = r0; r <= r1; ++r) {
+    const int jl = (lane + r) & 63;
+    const int j = jt * kGBI + jl;
+    if (vi && j < n && !(diag && r == 32 && lane >= 32)) {
+      const double4 pj = jpos[wv][jl];
+      const float4 aj = ja[wv][jl];
+      const float dx = (float)(pj.x - xi), dy = (float)(pj.y - yi), dz = (float)(pj.z - zi);
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      if (PASS == 1) {
+        const float idist = rsqrtf(r2), dist = r2 * idist;
+        si += (double)burial_term(dist, idist, u0, u2, aj.x, aj.y);
+        sj += (double)burial_term(dist, idist, aj.x, aj.z, u0, u1);
+      } else {
+        const float Bj = aj.y;
+        const float ex = __expf(-r2 * (0.25f * u2 * aj.z));  // exp(-r^2 / (4 Bi Bj))
+        const float inv = rsqrtf(r2 + u1 * Bj * ex);          // 1 / f_GB
+        const float part = u0 * aj.x * inv * inv * 0.5f * inv * ex;
+        si += (double)(part * (Bj + r2 * (0.25f * u2)));
+        sj += (double)(part * (u1 + r2 * (0.25f * aj.z)));
+      }
+    }
+    sj = __shfl(sj, (lane + 1) & 63);  // lane l now holds the sum of j (l + r + 1) & 63
+  }
+  double *slot = S.gb_fpart + (size_t)(w - pp[s]) * kGBPairDoubles;
+  slot[lane] = si;
+  slot[kGBI + ((lane + r1 + 1) & 63)] = sj;
+}
+
+// per atom after pass 1: the burial sum, then the OBC Born radius
+// (GBBornRadii::postProcess, GBBornRadii.cpp:145-173) and the per-atom factor
+// of dR_i/dr_ij (GBForce.cpp:126: R_i^2 rho~_i (1 - tanh^2) dtanh/dpsi / rho_i);
+// after pass 2: the partial sum
+template <int PASS>
+__global__ __launch_bounds__(kGBI) void k_gb_sums(SysDev *sys, const int *gp, int nsys, ForceDev P) {
+  const int tile = blockIdx.x;
+  if (tile >= gp[nsys]) return;
+  const int s = find_sys_gb(gp, nsys, tile);
+  SysDev &S = sys[s];
+  const int t = tile - gp[s], lane = threadIdx.x;
+  const int i = t * kGBI + lane;
+  if (i >= S.n_atoms) return;
+  const double v = gb_atom_sum(S, t, lane, 0, kGBI);
+  if (PASS == 2) {
+    S.gb_partial[i] = v;
+    return;
+  }
   const double rho = S.gb_rho[i], orid = rho - P.gb_offset;
-  const double psi = 0.5 * burial * orid;
+  const double psi = 0.5 * v * orid;
   const double tp = P.gb_a * psi - P.gb_b * psi * psi + P.gb_g * psi * psi * psi;
   const double th = tanh(tp);
   const double B = 1.0 / ((1.0 / orid) - (1.0 / rho) * th);
@@ -211,57 +262,6 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_born(SysDev *sys, const int *gp
     const double rs = rho + P.gb_ace_rho_s, r3 = rho * rho * rho, iB = 1.0 / B, iB3 = iB * iB * iB;
     S.gb_spare[i] = rs * rs * (r3 * r3) * (iB3 * iB3 * iB);
   }
-}
-
-// pass 2: partial sums (GBPartialSum.cpp:11-42): sum over l != i of
-// q_i q_l f^-3 / 2 e^{-r^2/4RiRl} (R_l + r^2 / 4 R_i)
-__global__ __launch_bounds__(kGBBlock) void k_gb_partial(SysDev *sys, const int *gp, int nsys, ForceDev P) {
-  __shared__ GBShared sh;
-  __shared__ double red[kGBParts][kGBI];
-  const int tile = blockIdx.x;
-  if (tile >= gp[nsys]) return;
-  const int s = find_sys_gb(gp, nsys, tile);
-  const SysDev &S = sys[s];
-  const int n = S.n_atoms;
-  const int lane = threadIdx.x & (kGBI - 1), part = threadIdx.x / kGBI;
-  const int i = (tile - gp[s]) * kGBI + lane;
-  const bool vi = i < n;
-  double xi = 0, yi = 0, zi = 0;
-  float qi = 0.f, Bi = 1.f;
-  if (vi) {
-    xi = S.x[i];
-    yi = S.x[n + i];
-    zi = S.x[2 * n + i];
-    qi = (float)S.charge[i];
-    Bi = (float)S.gb_born[i];
-  }
-  const float i4Bi = 0.25f * rcp(Bi), iBi = rcp(Bi);
-  double acc = 0.0;
-  for (int j0 = 0; j0 < n; j0 += kGBBlock) {
-    __syncthreads();
-    load_chunk(sh, S, P, j0, 2);
-    __syncthreads();
-    const int cnt = min(kGBBlock, n - j0);
-    if (!vi) continue;
-    const int k1 = min(cnt, (part + 1) * kGBI);
-    for (int k = part * kGBI; k < k1; ++k) {
-      if (j0 + k == i) continue;
-      const double4 pj = sh.pos[k];
-      const float4 aj = sh.a[k];
-      const float qj = sh.b[k].x;
-      const float dx = (float)(pj.x - xi), dy = (float)(pj.y - yi), dz = (float)(pj.z - zi);
-      const float r2 = dx * dx + dy * dy + dz * dz;
-      const float Bj = aj.z;
-      const float BB = Bi * Bj;
-      const float ex = __expf(-r2 * (0.25f * iBi * aj.w));  // exp(-r^2 / (4 Bi Bj))
-      const float inv = rsqrtf(r2 + BB * ex);                 // 1 / f_GB
-      const float part = qi * qj * inv * inv * 0.5f * inv * ex;
-      acc += (double)(part * (Bj + r2 * i4Bi));
-    }
-  }
-  red[part][lane] = acc;
-  __syncthreads();
-  if (part == 0 && vi) S.gb_partial[i] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
 }
 
 // pass 3: GB pair force and energy (GBForce.cpp:14-175) plus the all-pairs
@@ -278,7 +278,6 @@ __global__ __launch_bounds__(kGBBlock) void k_gb_partial(SysDev *sys, const int 
 // VIR: with the energies, the virials of each pair (OneAtomPairNoExclusion.h:
 // 73-81, OneAtomPair.h:95-103): f (x) d to the atomic virial, and f (x)
 // (R_J - R_I) to the molecular one for pairs of two molecules.
-constexpr int kGBPWaves = 4;
 template <bool VIR>
 __global__ __launch_bounds__(kGBPWaves * 64) void k_gb_pairs(SysDev *sys, const int *pp, int nsys, ForceDev P,
                                                              int energies) {
@@ -290,12 +289,8 @@ __global__ __launch_bounds__(kGBPWaves * 64) void k_gb_pairs(SysDev *sys, const 
   const int s = find_sys_gb(pp, nsys, w);
   SysDev &S = sys[s];
   const int n = S.n_atoms, nt = S.gb_nt;
-  int p = w - pp[s], it = 0;
-  while (p >= nt - it) {  // row it holds tile pairs (it, it..nt-1)
-    p -= nt - it;
-    ++it;
-  }
-  const int jt = it + p;
+  int it, jt;
+  gb_tile_pair(w - pp[s], nt, it, jt);
   const bool diag = it == jt;
   const bool peratom = S.lj_mode == PM_LJ_PERATOM;
   // the j tile into this wave's LDS records
@@ -508,15 +503,8 @@ __global__ __launch_bounds__(kGBI) void k_gb_fsum(SysDev *sys, const int *gp, in
   const int i = t * kGBI + lane;
   double e = 0.0;
   if (i < n) {
-    double f[3] = {0.0, 0.0, 0.0};
-    for (int u = 0; u < nt; ++u) {
-      const int a = min(u, t), b = max(u, t);
-      const double *slot = S.gb_fpart + (size_t)(a * nt - a * (a - 1) / 2 + (b - a)) * kGBPairDoubles;
-      for (int c = 0; c < 3; ++c) {
-        if (u >= t) f[c] += slot[c * kGBI + lane];        // i side of (t, u)
-        if (u <= t) f[c] += slot[(3 + c) * kGBI + lane];  // j side of (u, t)
-      }
-    }
+    double f[3];
+    for (int c = 0; c < 3; ++c) f[c] = gb_atom_sum(S, t, lane, c * kGBI, (3 + c) * kGBI);
     S.fx[i] += __double2ll_rn(f[0] * kFScale);
     S.fy[i] += __double2ll_rn(f[1] * kFScale);
     S.fz[i] += __double2ll_rn(f[2] * kFScale);
@@ -538,10 +526,12 @@ __global__ __launch_bounds__(kGBI) void k_gb_fsum(SysDev *sys, const int *gp, in
 
 void launch_gb(const Launch &L, const ForceDev &P, bool energies) {
   if (!L.n_gb_tiles_total) return;
-  dim3 g(L.n_gb_tiles_total), b(kGBBlock);
-  hipLaunchKernelGGL(k_gb_born, g, b, 0, L.stream, L.sys, L.gb_prefix, L.nsys, P);
-  hipLaunchKernelGGL(k_gb_partial, g, b, 0, L.stream, L.sys, L.gb_prefix, L.nsys, P);
+  dim3 gt(L.n_gb_tiles_total), bt(kGBI);
   dim3 gp2((L.n_gb_pairs_total + kGBPWaves - 1) / kGBPWaves), bp(kGBPWaves * 64);
+  hipLaunchKernelGGL(k_gb_sym<1>, gp2, bp, 0, L.stream, L.sys, L.gbp_prefix, L.nsys, P);
+  hipLaunchKernelGGL(k_gb_sums<1>, gt, bt, 0, L.stream, L.sys, L.gb_prefix, L.nsys, P);
+  hipLaunchKernelGGL(k_gb_sym<2>, gp2, bp, 0, L.stream, L.sys, L.gbp_prefix, L.nsys, P);
+  hipLaunchKernelGGL(k_gb_sums<2>, gt, bt, 0, L.stream, L.sys, L.gb_prefix, L.nsys, P);
   if (energies && (P.virial || P.molvirial))
     hipLaunchKernelGGL(k_gb_pairs<true>, gp2, bp, 0, L.stream, L.sys, L.gbp_prefix, L.nsys, P, 1);
   else
