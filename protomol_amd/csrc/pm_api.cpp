@@ -1532,6 +1532,8 @@ int pm_ctx_create(const pm_force_desc *force, int32_t n_systems, const pm_system
         cB2 = build_switch(force->switch_b, P.sw_b, true, rc);
         cB2 = rc * rc;
         P.alpha = (float)force->ewald_alpha;
+        P.alpha_p = (float)(0.3275911 * force->ewald_alpha);
+        P.alpha2_log2e = (float)(-force->ewald_alpha * force->ewald_alpha / std::log(2.0));
         P.two_alpha_pi = (float)(2.0 * force->ewald_alpha / std::sqrt(M_PI));
         P.dalpha = force->ewald_alpha;
         P.dtwo_alpha_pi = 2.0 * force->ewald_alpha / std::sqrt(M_PI);

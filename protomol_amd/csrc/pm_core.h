@@ -97,7 +97,9 @@ struct ForceDev {
   float cutA2, cutB2;      // inclusion cutoff^2 per term (FP32 decision)
   double cutA2d, cutB2d;   // the same in FP64 (guard-band recheck)
   float guard;             // |r2 - cut2| <= guard  ->  FP64 recheck
-  float alpha, alpha2_log2e, two_alpha_pi;  // Ewald real space
+  // Ewald real space: alpha, 0.3275911 alpha (the A&S 7.1.26 t argument),
+  // -alpha^2 log2(e) (exp2 argument per r^2), 2 alpha / sqrt(pi)
+  float alpha, alpha_p, alpha2_log2e, two_alpha_pi;
   double dalpha, dtwo_alpha_pi;             // Ewald correction (FP64)
   float rlist2;
   unsigned flags;

@@ -70,6 +70,10 @@ __device__ __forceinline__ unsigned long long pair_hash(unsigned lo, unsigned hi
 }
 
 __device__ __forceinline__ unsigned long long wave_ballot(bool p) { return __ballot(p); }
+// lane mask of a predicate over the wave (all 64 lanes active), and a lane's
+// bit of such a mask; both stay in scalar registers where the compiler can
+__device__ __forceinline__ unsigned long long lane_mask(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+__device__ __forceinline__ bool lane_bit(unsigned long long m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
 
 __device__ __forceinline__ float wave_max_f(float v) {
   for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
@@ -861,14 +865,14 @@ __device__ __forceinline__ float pair_terms(const ForceDev &P, float r2, float q
     float E = r12A - r6B;
     float F = (12.0f * r12A - 6.0f * r6B) * r2inv;
     apply_switch_t<SWA, !SAME>(P.sw_a, r2, E, F);
-    fs += inA ? F : 0.f;
+    fs = inA ? F : 0.f;
     ea = inA ? E : 0.f;
   } else if (TA == PM_TERM_COULOMB) {
     float E = qq * rinv;
     if (mod) E *= csf;
     float F = E * r2inv;
     apply_switch_t<SWA, !SAME>(P.sw_a, r2, E, F);
-    fs += inA ? F : 0.f;
+    fs = inA ? F : 0.f;
     ea = inA ? E : 0.f;
   }
   if (TB == PM_TERM_COULOMB) {
@@ -882,14 +886,15 @@ __device__ __forceinline__ float pair_terms(const ForceDev &P, float r2, float q
   } else if (TB == PM_TERM_EWALD_REAL) {
     // NonbondedFullEwaldSystemForce::realTerm (NonbondedFullEwaldSystemForce.h:818-834):
     // erfc(ar) by Abramowitz & Stegun 7.1.26, the approximation the reference uses
+    // with t = 1 / (1 + p a r) and exp(-(a r)^2) = 2^(r^2 * (-a^2 log2 e)),
+    // the constants p a and -a^2 log2 e taken from the host
     float q2 = qq;
     if (mod) q2 *= csf;
     const float r = r2 * rinv;
-    const float ar = P.alpha * r;
-    const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * ar);
+    const float t = __builtin_amdgcn_rcpf(fmaf(P.alpha_p, r, 1.0f));
     const float poly =
         t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
-    const float ee = q2 * __builtin_amdgcn_exp2f(-(ar * ar) * 1.4426950408889634f);
+    const float ee = q2 * __builtin_amdgcn_exp2f(r2 * P.alpha2_log2e);
     const float E = poly * ee * rinv;
     const float F = (E + P.two_alpha_pi * ee) * r2inv;
     fs += inB ? F : 0.f;
@@ -1041,34 +1046,43 @@ __global__ __launch_bounds__(512, (KIND == kPairForces ? PM_PAIR_MIN_WAVES : PM_
       const int info = linfo[c * kCluster + ii];
       const float dx = xjx - xi.x, dy = xjy - xi.y, dz = xjz - xi.z;
       const float r2 = dx * dx + dy * dy + dz * dz;
+      // the pair tests as wave lane masks (scalar ANDs, no per-lane shifts)
       unsigned long long am = vim[c] & jm;
       if (c == selfc) am &= trim;
       bool mod = false;
       if (flag) {
         am &= ~M[2 * c];
-        mod = (M[2 * c + 1] >> lane) & 1ull;
+        mod = lane_bit(M[2 * c + 1]);
       }
-      const bool act = (am >> lane) & 1ull;
-      bool inA, inB;
+      unsigned long long mA, mB;
       if (SAME) {
-        bool in = r2 <= P.cutA2;
-        if (__builtin_expect(act && fabsf(r2 - P.cutA2) <= P.guard, 0))
-          in = min_image_r2(*S, lorig[c * kCluster + ii], orig[sj]) <= P.cutA2d;
-        inA = inB = in && act;
+        mA = lane_mask(r2 <= P.cutA2) & am;
+        const unsigned long long near = lane_mask(fabsf(r2 - P.cutA2) <= P.guard) & am;
+        if (__builtin_expect(near != 0, 0)) {
+          bool in = lane_bit(mA);
+          if (lane_bit(near)) in = min_image_r2(*S, lorig[c * kCluster + ii], orig[sj]) <= P.cutA2d;
+          mA = lane_mask(in);
+        }
+        mB = mA;
       } else {
-        inA = (TA != PM_TERM_NONE) && r2 <= P.cutA2;
-        inB = (TB != PM_TERM_NONE) && r2 <= P.cutB2;
+        mA = TA != PM_TERM_NONE ? lane_mask(r2 <= P.cutA2) & am : 0ull;
+        mB = TB != PM_TERM_NONE ? lane_mask(r2 <= P.cutB2) & am : 0ull;
         const bool nearA = (TA != PM_TERM_NONE) && fabsf(r2 - P.cutA2) <= P.guard;
         const bool nearB = (TB != PM_TERM_NONE) && fabsf(r2 - P.cutB2) <= P.guard;
-        if (__builtin_expect(act && (nearA || nearB), 0)) {
-          const double r2d = min_image_r2(*S, lorig[c * kCluster + ii], orig[sj]);
-          inA = (TA != PM_TERM_NONE) && r2d <= P.cutA2d;
-          inB = (TB != PM_TERM_NONE) && r2d <= P.cutB2d;
+        const unsigned long long near = lane_mask(nearA || nearB) & am;
+        if (__builtin_expect(near != 0, 0)) {
+          bool a = lane_bit(mA), b = lane_bit(mB);
+          if (lane_bit(near)) {
+            const double r2d = min_image_r2(*S, lorig[c * kCluster + ii], orig[sj]);
+            a = (TA != PM_TERM_NONE) && r2d <= P.cutA2d;
+            b = (TB != PM_TERM_NONE) && r2d <= P.cutB2d;
+          }
+          mA = lane_mask(a);
+          mB = lane_mask(b);
         }
-        inA = inA && act;
-        inB = inB && act;
       }
-      if (wave_ballot(inA || inB) == 0) continue;
+      if ((mA | mB) == 0) continue;
+      const bool inA = lane_bit(mA), inB = lane_bit(mB);
       float A = 0.f, B = 0.f;
       if (TA == PM_TERM_LJ) {
         if (peratom) {
@@ -1162,7 +1176,7 @@ __global__ __launch_bounds__(512, (KIND == kPairForces ? PM_PAIR_MIN_WAVES : PM_
       ax += __shfl_xor(ax, 32);
       ay += __shfl_xor(ay, 32);
       az += __shfl_xor(az, 32);
-      if (jj == 0 && ((vim[c] >> lane) & 1ull)) {
+      if (jj == 0 && lane_bit(vim[c])) {
         const int si = (cl0 + c) * kCluster + ii;
         atomic_add_ll(&gfx[si], to_fixed((double)ax, kFScale));
         atomic_add_ll(&gfy[si], to_fixed((double)ay, kFScale));
