@@ -848,28 +848,28 @@ constexpr int kLJLds = 256;  // LJ table entries kept in LDS per wave (T <= 16)
 constexpr int kPairWaveLds = kLJLds * 16 + kWave * (16 + 4 + 4 + 8);
 
 // One pair term set (OneAtomPair[Two]::doOneAtomPair, OneAtomPair.h:47-107):
-// the scalar force factor F/r for the pair, energies into ea / eb.
+// the scalar force factor F/r for the pair, energies into ea / eb.  qq is the
+// charge product with the modified-pair scale already applied; A, B are the
+// pair's LJ coefficients and A12 = 12 A, B6 = 6 B.
 template <int TA, int TB, int SWA, int SWB, bool SAME>
-__device__ __forceinline__ float pair_terms(const ForceDev &P, float r2, float qq, float A, float B, bool mod,
-                                            float csf, bool inA, bool inB, float &ea, float &eb) {
+__device__ __forceinline__ float pair_terms(const ForceDev &P, float r2, float qq, float A, float B, float A12,
+                                            float B6, bool inA, bool inB, float &ea, float &eb) {
   const float rinv = __builtin_amdgcn_rsqf(r2);
   const float r2inv = rinv * rinv;
   float fs = 0.f;
   ea = 0.f;
   eb = 0.f;
   if (TA == PM_TERM_LJ) {
-    // LennardJonesForce::operator() (LennardJonesForce.h:45-58)
+    // LennardJonesForce::operator() (LennardJonesForce.h:45-58):
+    // E = A r^-12 - B r^-6 = r^-6 (A r^-6 - B), F r = r^-6 (12 A r^-6 - 6 B)
     const float r6 = r2inv * r2inv * r2inv;
-    const float r6B = B * r6;
-    const float r12A = A * r6 * r6;
-    float E = r12A - r6B;
-    float F = (12.0f * r12A - 6.0f * r6B) * r2inv;
+    float E = r6 * fmaf(A, r6, -B);
+    float F = r6 * fmaf(A12, r6, -B6) * r2inv;
     apply_switch_t<SWA, !SAME>(P.sw_a, r2, E, F);
     fs = inA ? F : 0.f;
     ea = inA ? E : 0.f;
   } else if (TA == PM_TERM_COULOMB) {
     float E = qq * rinv;
-    if (mod) E *= csf;
     float F = E * r2inv;
     apply_switch_t<SWA, !SAME>(P.sw_a, r2, E, F);
     fs = inA ? F : 0.f;
@@ -878,23 +878,20 @@ __device__ __forceinline__ float pair_terms(const ForceDev &P, float r2, float q
   if (TB == PM_TERM_COULOMB) {
     // CoulombForce::operator() (CoulombForce.h:29-40)
     float E = qq * rinv;
-    if (mod) E *= csf;
     float F = E * r2inv;
     apply_switch_t<SWB, !SAME>(P.sw_b, r2, E, F);
     fs += inB ? F : 0.f;
     eb = inB ? E : 0.f;
   } else if (TB == PM_TERM_EWALD_REAL) {
     // NonbondedFullEwaldSystemForce::realTerm (NonbondedFullEwaldSystemForce.h:818-834):
-    // erfc(ar) by Abramowitz & Stegun 7.1.26, the approximation the reference uses
+    // erfc(ar) by Abramowitz & Stegun 7.1.26, the approximation the reference uses,
     // with t = 1 / (1 + p a r) and exp(-(a r)^2) = 2^(r^2 * (-a^2 log2 e)),
     // the constants p a and -a^2 log2 e taken from the host
-    float q2 = qq;
-    if (mod) q2 *= csf;
     const float r = r2 * rinv;
     const float t = __builtin_amdgcn_rcpf(fmaf(P.alpha_p, r, 1.0f));
     const float poly =
         t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
-    const float ee = q2 * __builtin_amdgcn_exp2f(r2 * P.alpha2_log2e);
+    const float ee = qq * __builtin_amdgcn_exp2f(r2 * P.alpha2_log2e);
     const float E = poly * ee * rinv;
     const float F = (E + P.two_alpha_pi * ee) * r2inv;
     fs += inB ? F : 0.f;
@@ -940,7 +937,7 @@ __global__ __launch_bounds__(512, (KIND == kPairForces ? PM_PAIR_MIN_WAVES : PM_
   char *wb = reinterpret_cast<char *>(pair_lds) + (size_t)wv * kPairWaveLds;
   float4 *ljs = reinterpret_cast<float4 *>(wb);
   float4 *lx = reinterpret_cast<float4 *>(wb + kLJLds * 16);
-  int *linfo = reinterpret_cast<int *>(wb + kLJLds * 16 + kWave * 16);  // ti * T | valid << 30
+  int *linfo = reinterpret_cast<int *>(wb + kLJLds * 16 + kWave * 16);  // byte offset of row ti of the LJ table
   int *lorig = linfo + kWave;
   float2 *lpp = reinterpret_cast<float2 *>(lorig + kWave);
   const int w = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
@@ -968,10 +965,12 @@ __global__ __launch_bounds__(512, (KIND == kPairForces ? PM_PAIR_MIN_WAVES : PM_
   const bool peratom = S->lj_mode == PM_LJ_PERATOM;
   const int T = S->n_types;
   // stage the super-cluster's i atoms: lane = c * 8 + atom
+  unsigned long long valid_ca;  // bit c * 8 + atom: a real atom
   {
     const int c = lane >> 3, a = lane & 7;
     float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
     int info = 0, og = -1;
+    bool valid = false;
     float2 pp = make_float2(0.f, 0.f);
     if (c < nic) {
       const int cl = cl0 + c, si = cl * kCluster + a;
@@ -980,9 +979,8 @@ __global__ __launch_bounds__(512, (KIND == kPairForces ? PM_PAIR_MIN_WAVES : PM_
       const int cc = S->cl_cap;
       q = make_float4(x.x + (float)(O[cl] - O[cl0]), x.y + (float)(O[cc + cl] - O[cc + cl0]),
                       x.z + (float)(O[2 * cc + cl] - O[2 * cc + cl0]), x.w);
-      const bool valid = a < S->cl_nvalid[cl];
-      info = (valid ? (1 << 30) : 0);
-      if (TA == PM_TERM_LJ && !peratom) info |= ljt[si] * T;
+      valid = a < S->cl_nvalid[cl];
+      if (TA == PM_TERM_LJ && !peratom) info = ljt[si] * T * 16;
       if (TA == PM_TERM_LJ && peratom) pp = ljp[si];
       og = orig[si];
     }
@@ -990,10 +988,16 @@ __global__ __launch_bounds__(512, (KIND == kPairForces ? PM_PAIR_MIN_WAVES : PM_
     linfo[lane] = info;
     lorig[lane] = og;
     lpp[lane] = pp;
+    valid_ca = lane_mask(valid);
   }
-  if (TA == PM_TERM_LJ && !peratom) {  // T <= 16 (checked at context creation)
-    const PM_GLOBAL float4 *gtab = gp(S->lj_table);
-    for (int k = lane; k < T * T; k += kWave) ljs[k] = gtab[k];
+  // LJ table (T <= 16, checked at context creation) as (A, B, 12 A, 6 B); the
+  // modified-pair coefficients stay in the global table
+  const PM_GLOBAL float4 *gtab = gp(S->lj_table);
+  if (TA == PM_TERM_LJ && !peratom) {
+    for (int k = lane; k < T * T; k += kWave) {
+      const float4 g = gtab[k];
+      ljs[k] = make_float4(g.x, g.y, 12.0f * g.x, 6.0f * g.y);
+    }
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -1002,7 +1006,7 @@ __global__ __launch_bounds__(512, (KIND == kPairForces ? PM_PAIR_MIN_WAVES : PM_
   // and the i < j triangle of a cluster's pairs with itself
   unsigned long long vim[kScCl];
 #pragma unroll
-  for (int c = 0; c < kScCl; ++c) vim[c] = wave_ballot((linfo[c * kCluster + ii] >> 30) & 1);
+  for (int c = 0; c < kScCl; ++c) vim[c] = ((valid_ca >> (8 * c)) & 0xFFull) * 0x0101010101010101ull;
   const unsigned long long trim = wave_ballot(ii < jj);
   const float csf = S->csf, ljsc = S->lj_scaling;
   float fi[kScCl][3];
@@ -1034,6 +1038,7 @@ __global__ __launch_bounds__(512, (KIND == kPairForces ? PM_PAIR_MIN_WAVES : PM_
       if (peratom) pj = ljp[sj];
       else tj = ljt[sj];
     }
+    const char *ljrow = reinterpret_cast<const char *>(ljs) + tj * 16;  // + row offset = (ti, tj) entry
     const unsigned long long jm = nvj == 8 ? ~0ull : ((1ull << (8 * nvj)) - 1ull);
     const int selfc = sh == kShiftCenter ? cj - cl0 : -1;  // i cluster that is cj itself
     const bool flag = (w0 & 0x80000000u) != 0;
@@ -1049,10 +1054,10 @@ __global__ __launch_bounds__(512, (KIND == kPairForces ? PM_PAIR_MIN_WAVES : PM_
       // the pair tests as wave lane masks (scalar ANDs, no per-lane shifts)
       unsigned long long am = vim[c] & jm;
       if (c == selfc) am &= trim;
-      bool mod = false;
+      unsigned long long mm = 0;  // modified (1-4) pairs
       if (flag) {
         am &= ~M[2 * c];
-        mod = lane_bit(M[2 * c + 1]);
+        mm = M[2 * c + 1];
       }
       unsigned long long mA, mB;
       if (SAME) {
@@ -1083,7 +1088,8 @@ __global__ __launch_bounds__(512, (KIND == kPairForces ? PM_PAIR_MIN_WAVES : PM_
       }
       if ((mA | mB) == 0) continue;
       const bool inA = lane_bit(mA), inB = lane_bit(mB);
-      float A = 0.f, B = 0.f;
+      float qq = xi.w * xj.w;
+      float A = 0.f, B = 0.f, A12 = 0.f, B6 = 0.f;
       if (TA == PM_TERM_LJ) {
         if (peratom) {
           // GROMACS mixing (BuildTopology.cpp:650-659): sigma = (si+sj)/2, eps = sqrt(ei ej)
@@ -1092,18 +1098,37 @@ __global__ __launch_bounds__(512, (KIND == kPairForces ? PM_PAIR_MIN_WAVES : PM_
           const float sg2 = sg * sg, sg6 = sg2 * sg2 * sg2;
           B = 4.0f * ep * sg6;
           A = B * sg6;
-          if (mod) {
-            A *= ljsc;
-            B *= ljsc;
-          }
+          A12 = 12.0f * A;
+          B6 = 6.0f * B;
         } else {
-          const float4 p = ljs[(info & 0xFFFF) + tj];
-          A = mod ? p.z : p.x;
-          B = mod ? p.w : p.y;
+          const float4 p = *reinterpret_cast<const float4 *>(ljrow + info);
+          A = p.x;
+          B = p.y;
+          A12 = p.z;
+          B6 = p.w;
+        }
+      }
+      // modified pairs (rare: a wave-uniform test first): scaled charges
+      // and the 1-4 LJ coefficients
+      if (__builtin_expect(mm != 0, 0)) {
+        if (lane_bit(mm)) {
+          qq *= csf;
+          if (TA == PM_TERM_LJ) {
+            if (peratom) {
+              A *= ljsc;
+              B *= ljsc;
+            } else {
+              const float4 g = gtab[(info >> 4) + tj];
+              A = g.z;
+              B = g.w;
+            }
+            A12 = 12.0f * A;
+            B6 = 6.0f * B;
+          }
         }
       }
       float ea, eb;
-      const float fs = pair_terms<TA, TB, SWA, SWB, SAME>(P, r2, xi.w * xj.w, A, B, mod, csf, inA, inB, ea, eb);
+      const float fs = pair_terms<TA, TB, SWA, SWB, SAME>(P, r2, qq, A, B, A12, B6, inA, inB, ea, eb);
       const float fx = fs * dx, fy = fs * dy, fz = fs * dz;
       fi[c][0] -= fx;
       fi[c][1] -= fy;

@@ -213,6 +213,75 @@ def water_box_density(n_mol: int, seed: int, density: float = 0.0334) -> System:
     return water_box(n_mol, box, seed)
 
 
+# united-atom alkane chain (TraPPE-like CH3 / CH2 sites): bonds 1.54 A,
+# angles 114 deg; CH3 sigma 3.75 A eps 0.195, CH2 sigma 3.95 A eps 0.0914
+# kcal/mol; 1-4 pairs with their own LJ coefficients (sigma 0.9x, eps 0.5x)
+# and the Coulomb scale 1/1.2
+CHAIN_BOND = 1.54
+CHAIN_THETA = math.radians(114.0)
+CHAIN_SIGMA = (3.75, 3.95)
+CHAIN_EPS = (0.195, 0.0914)
+
+
+def chain_box(n_mol: int, box: float, seed: int, length: int = 4, per_atom: bool = False,
+              atom_noise: float = 0.05) -> System:
+    """Periodic box of ``n_mol`` united-atom chains of ``length`` sites on a
+    cubic lattice, random orientations, planar zig-zag geometry plus Gaussian
+    ``atom_noise`` (A).  Exclusions: 1-2 and 1-3 pairs FULL, 1-4 pairs
+    MODIFIED, so the pair path's scaled (1-4) branch is exercised: charges
+    +-0.25 e alternate along the chain and ``coulomb_scaling`` is 1/1.2.
+    ``per_atom`` gives per-atom sigma / epsilon with GROMACS mixing and
+    ``lj_scaling`` 0.5 instead of the type table (whose 1-4 entries use
+    sigma 0.9x and epsilon 0.5x)."""
+    rng = np.random.default_rng(seed)
+    m = 1
+    while m ** 3 < n_mol:
+        m += 1
+    a = box / m
+    sites = np.stack(np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij"), -1).reshape(-1, 3)
+    if n_mol < m ** 3:
+        sites = sites[np.sort(rng.choice(m ** 3, size=n_mol, replace=False))]
+    centers = (sites + 0.5) * a - 0.5 * box
+    half = 0.5 * (math.pi - CHAIN_THETA)
+    k = np.arange(length)
+    local = np.stack([k * CHAIN_BOND * math.cos(half), (k % 2) * CHAIN_BOND * math.sin(half), np.zeros(length)], 1)
+    local -= local.mean(axis=0)
+    R = _random_rotations(rng, n_mol)
+    x = (centers[:, None, :] + np.einsum("mij,aj->mai", R, local)).reshape(-1, 3)
+    x = x + rng.normal(0.0, atom_noise, size=x.shape)
+    n = n_mol * length
+    end = (k == 0) | (k == length - 1)
+    typ = np.tile(np.where(end, 0, 1).astype(np.int32), n_mol)
+    charge = np.tile(np.where(k % 2 == 0, 0.25, -0.25) * SQRT_COULOMB, n_mol)
+    if length % 2:
+        charge[length - 1::length] = 0.0
+    mass = np.tile(np.where(end, 15.035, 14.027), n_mol)
+    mol = np.repeat(np.arange(n_mol, dtype=np.int32), length)
+    base = length * np.arange(n_mol, dtype=np.int32)
+    ex = []
+    for d, cls in ((1, EXCL_FULL), (2, EXCL_FULL), (3, EXCL_MODIFIED)):
+        for i in range(length - d):
+            ex.append(np.stack([base + i, base + i + d, np.full(n_mol, cls)], 1))
+    ex = np.concatenate(ex).astype(np.int32)
+    bonds = np.concatenate([np.stack([base + i, base + i + 1], 1) for i in range(length - 1)]).astype(np.int32)
+    kw = dict(x=x, charge=charge, mass=mass, molecule=mol, box=np.full(3, float(box)), origin=np.zeros(3),
+              periodic=True, exclusions=ex, coulomb_scaling=1.0 / 1.2, bonds=bonds,
+              name=f"chain{length}x{n_mol}")
+    if per_atom:
+        sig = np.array(CHAIN_SIGMA)[typ]
+        eps = np.array(CHAIN_EPS)[typ]
+        return System(lj_sigma=sig, lj_epsilon=eps, lj_scaling=0.5, **kw)
+    table = np.zeros((2, 2, 4))
+    for ti in range(2):
+        for tj in range(2):
+            sg = 0.5 * (CHAIN_SIGMA[ti] + CHAIN_SIGMA[tj])
+            ep = math.sqrt(CHAIN_EPS[ti] * CHAIN_EPS[tj])
+            A, B = 4 * ep * sg ** 12, 4 * ep * sg ** 6
+            A14, B14 = 4 * 0.5 * ep * (0.9 * sg) ** 12, 4 * 0.5 * ep * (0.9 * sg) ** 6
+            table[ti, tj] = (A, B, A14, B14)
+    return System(type=typ, lj_table=table, **kw)
+
+
 def rock_salt(n_cells: int, spacing: float = 2.82, origin=(0.0, 0.0, 0.0), jitter: float = 0.0,
               seed: int = 0) -> System:
     """NaCl lattice of n_cells^3 conventional cells (8 ions each) with

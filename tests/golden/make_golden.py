@@ -133,6 +133,36 @@ def gb_multi_chain(n, seed, per):
     return g
 
 
+def chain_fixture_system(per_atom=False):
+    return S.chain_box(512, 52.0, seed=3, per_atom=per_atom)
+
+
+def make_modified():
+    """Systems with MODIFIED (1-4) exclusions: united-atom chains, once with
+    the type table (its own A14 / B14) and once with per-atom LJ and
+    lj_scaling, both with coulomb_scaling 1/1.2.  LJ + Coulomb C2 cutoff pairs
+    (forces, energies, virial, pair census) and the Ewald real-space sum with
+    its correction term."""
+    acc = math.exp(-10.24)
+    ta, tb, sa, sb = CUTOFF_CASES["c2c2"]
+    out = {"alpha": 0.32, "accuracy": acc}
+    for tag, per_atom in (("table", False), ("atom", True)):
+        c = chain_fixture_system(per_atom)
+        out[f"x_{tag}"] = c.x
+        r = refio.system_records(c)
+        r.update(mode=np.int32(1), term_a=np.int32(ta), term_b=np.int32(tb), sw_a=np.array(sa, float),
+                 sw_b=np.array(sb, float), alg_cutoff=10.0, cellsize=5.0, virial=np.int32(1))
+        o = refio.run(r)
+        out[f"f_cut_{tag}"], out[f"e_cut_{tag}"] = o["f"].reshape(-1, 3), o["e"]
+        out[f"census_{tag}"] = np.array(o["census"], dtype=np.uint64)
+        r = refio.system_records(c)
+        r.update(mode=np.int32(2), ew_alpha=0.32, ew_accuracy=acc, ew_correction=np.int32(1), cellsize=5.0,
+                 virial=np.int32(1))
+        o = refio.run(r)
+        out[f"f_ew_{tag}"], out[f"e_ew_{tag}"] = o["f"].reshape(-1, 3), o["e"]
+    np.savez_compressed(os.path.join(HERE, "ref_modified.npz"), **out)
+
+
 def make_molvirial(w):
     """The atomic and molecular virials (ScalarStructure::molecularVirial on)
     of the reference's classes: LJ + Coulomb C2 cutoff pairs and the full
@@ -287,6 +317,9 @@ def main():
     if sys.argv[1:] == ["molvirial"]:
         make_molvirial(w)
         return
+    if sys.argv[1:] == ["modified"]:
+        make_modified()
+        return
     for name, case in CUTOFF_CASES.items():
         make_cutoff(name, w, case)
     make_ewald(w)
@@ -295,6 +328,7 @@ def main():
     make_gb(300, seed=7)
     make_gb_ace(200, seed=13)
     make_molvirial(w)
+    make_modified()
     make_water423()
     copy_water_expected()
     for f in sorted(os.listdir(HERE)):
