@@ -794,16 +794,21 @@ constexpr int kSwGeneric = -1;
 template <int SW, bool OUTER = true>
 __device__ __forceinline__ void apply_switch_t(const SwitchDev &sw, float r2, float &E, float &F) {
   if (SW == PM_SWITCH_C2) {  // C2SwitchingFunction.h:42-53
+    // inside r_on the switch is 1 with zero derivative: the unswitched values
+    // are kept as they are (one select on each result)
     const float c2 = sw.cut2 - r2;
     const float c4 = c2 * (sw.s2 + 2.0f * r2);
-    float S = sw.s1 * (c2 * c4);
-    float dS = sw.s3 * (c2 * c2 - c4);
+    const float S = sw.s1 * (c2 * c4);
+    const float dS = sw.s3 * (c2 * c2 - c4);
     const bool inner = r2 < sw.on2;
-    const bool outer = OUTER && r2 > sw.cut2;
-    S = inner ? 1.0f : (outer ? 0.0f : S);
-    dS = (inner || outer) ? 0.0f : dS;
-    F = F * S - E * dS;
-    E = E * S;
+    float Fs = F * S - E * dS, Es = E * S;
+    if (OUTER) {
+      const bool outer = r2 > sw.cut2;
+      Fs = outer ? 0.0f : Fs;
+      Es = outer ? 0.0f : Es;
+    }
+    F = inner ? F : Fs;
+    E = inner ? E : Es;
   } else if (SW == PM_SWITCH_CUTOFF || SW == PM_SWITCH_NONE) {
     // inside the inclusion cutoff the value is 1 (CutoffSwitchingFunction.h:41-44)
   } else {
@@ -891,11 +896,11 @@ __device__ __forceinline__ float pair_terms(const ForceDev &P, float r2, float q
     const float t = __builtin_amdgcn_rcpf(fmaf(P.alpha_p, r, 1.0f));
     const float poly =
         t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
+    // E = q^2 erfc(a r) / r, F r = (E + 2 a / sqrt(pi) q^2 exp(-(a r)^2)) / r^2
     const float ee = qq * __builtin_amdgcn_exp2f(r2 * P.alpha2_log2e);
-    const float E = poly * ee * rinv;
-    const float F = (E + P.two_alpha_pi * ee) * r2inv;
+    const float F = ee * fmaf(poly, rinv, P.two_alpha_pi) * r2inv;
     fs += inB ? F : 0.f;
-    eb = inB ? E : 0.f;
+    eb = inB ? poly * ee * rinv : 0.f;
   }
   return fs;
 }
