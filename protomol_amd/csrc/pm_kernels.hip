@@ -1049,6 +1049,7 @@ __global__ __launch_bounds__(512, (KIND == kPairForces ? PM_PAIR_MIN_WAVES : PM_
     const bool flag = (w0 & 0x80000000u) != 0;
     const PM_CONST unsigned long long *M = Mb + (size_t)(e1 >> 8) * (2 * kScCl);
     float fjx = 0.f, fjy = 0.f, fjz = 0.f;
+    bool any = false;  // wave-uniform: some i cluster of the entry had a pair inside the cutoff
 #pragma unroll
     for (int c = 0; c < kScCl; ++c) {
       if (!((imask >> c) & 1u)) continue;
@@ -1092,6 +1093,7 @@ __global__ __launch_bounds__(512, (KIND == kPairForces ? PM_PAIR_MIN_WAVES : PM_
         }
       }
       if ((mA | mB) == 0) continue;
+      any = true;
       const bool inA = lane_bit(mA), inB = lane_bit(mB);
       float qq = xi.w * xj.w;
       float A = 0.f, B = 0.f, A12 = 0.f, B6 = 0.f;
@@ -1186,7 +1188,8 @@ __global__ __launch_bounds__(512, (KIND == kPairForces ? PM_PAIR_MIN_WAVES : PM_
       }
     }
     // Newton's third law: j force = sum over the i clusters and the 8 i lanes
-    if (FORCES) {
+    // (nothing to add when no pair of the entry was inside the cutoff)
+    if (FORCES && any) {
       const float g = dpp_sum8_scatter(fjx, fjy, fjz, ii);
       if (ii < 3 && jj < nvj) atomic_add_ll(&gfc[sj], to_fixed((double)g, kFScale));
     }
