@@ -81,9 +81,15 @@ __device__ __forceinline__ float burial_term(float dist, float idist, float ori,
          (0.25f * sorj * sorj * idist) * (iL2 - iU2) + C;
 }
 
-// d(burial_i)/d(r_ij) (GBForce.cpp:55-126); the term (U/(4 d L))(dL/U - L dU/U^2)
-// of the reference is written in its equal form (dL/L - dU/U)/(4 d); idist = 1/dist
-__device__ __forceinline__ float dburial(float dist, float idist, float ori, float orj, float sj) {
+// d(burial_i)/d(r_ij) (GBForce.cpp:55-126).  The reference's sum of ten
+// terms is grouped by 1/L and 1/U (the same terms, fewer operations):
+//   [dL] g(1/L) - [dU] g(1/U) + (1 + s^2/d^2)/8 (1/U^2 - 1/L^2)
+//     - log(L/U) / (4 d^2) + [rho~_i <= s - d] [dL] 2/L^2
+// with g(x) = x (1/(4d) - x/2 + (d^2 - s^2) x^2 / (4d)), s = S_j rho~_j,
+// dL = [rho~_i <= d - s], dU = [rho~_i < d + s]; the reference's term
+// (U/(4 d L))(dL/U - L dU/U^2) is (dL/L - dU/U)/(4 d), the x/(4d) part of g.
+// r2 = d^2, id4 = 1/(4d), q4 = 1/(4d^2)
+__device__ __forceinline__ float dburial(float dist, float r2, float id4, float q4, float ori, float orj, float sj) {
   const float sorj = sj * orj;
   float L, U;
   if (ori >= dist + sorj) {
@@ -94,16 +100,16 @@ __device__ __forceinline__ float dburial(float dist, float idist, float ori, flo
     L = ori > t ? ori : t;
     U = dist + sorj;
   }
-  const float iL = rcp(L), iU = rcp(U), iL2 = iL * iL, iU2 = iU * iU;
-  const float dL = ori <= dist - sorj ? 1.0f : 0.0f;
-  const float dU = ori < dist + sorj ? 1.0f : 0.0f;
-  const float dC = ori <= sorj - dist ? 2.0f * iL2 * dL : 0.0f;
-  const float st = sorj * idist;
+  const float iL = rcp(L), iU = rcp(U), iL2 = iL * iL;
+  const bool dL = ori <= dist - sorj, dU = ori < dist + sorj;
   const float s2 = sorj * sorj;
-  return -0.5f * dL * iL2 + 0.5f * dU * iU2 + 0.125f * (iU2 - iL2) +
-         0.125f * dist * (2.0f * iL2 * iL * dL - 2.0f * iU2 * iU * dU) -
-         0.25f * idist * idist * __logf(L * iU) + 0.25f * idist * (dL * iL - dU * iU) - 0.125f * st * st * (iL2 - iU2) +
-         0.25f * s2 * idist * iU2 * iU * dU - 0.25f * s2 * idist * iL2 * iL * dL + dC;
+  const float k4 = (r2 - s2) * id4;
+  const float gL = iL * fmaf(iL, fmaf(k4, iL, -0.5f), id4);
+  const float gU = iU * fmaf(iU, fmaf(k4, iU, -0.5f), id4);
+  float T = (dL ? gL : 0.0f) - (dU ? gU : 0.0f);
+  T = fmaf(fmaf(0.5f * s2, q4, 0.125f), fmaf(iU, iU, -iL2), T);
+  T = fmaf(-q4, __logf(L * iU), T);
+  return (dL && ori <= sorj - dist) ? T + 2.0f * iL2 : T;
 }
 
 }  // namespace
@@ -275,12 +281,12 @@ __global__ __launch_bounds__(kGBI) void k_gb_sums(SysDev *sys, const int *gp, in
 // tile takes steps 1..32 (step 32 on lanes 0..31 only), which visits each of
 // its pairs once.  Both sides go to the tile pair's slot of gb_fpart;
 // k_gb_fsum adds an atom's partials in a fixed order.
-// VIR: with the energies, the virials of each pair (OneAtomPairNoExclusion.h:
+// EN: energies; VIR: with the energies, the virials of each pair (OneAtomPairNoExclusion.h:
 // 73-81, OneAtomPair.h:95-103): f (x) d to the atomic virial, and f (x)
 // (R_J - R_I) to the molecular one for pairs of two molecules.
-template <bool VIR>
-__global__ __launch_bounds__(kGBPWaves * 64) void k_gb_pairs(SysDev *sys, const int *pp, int nsys, ForceDev P,
-                                                             int energies) {
+template <bool VIR, bool EN>
+__global__ __launch_bounds__(kGBPWaves * 64) void k_gb_pairs(SysDev *sys, const int *pp, int nsys, ForceDev P) {
+  constexpr bool energies = EN;
   __shared__ double4 jpos[kGBPWaves][kGBI];
   __shared__ float4 ja[kGBPWaves][kGBI], jb[kGBPWaves][kGBI], jc[kGBPWaves][kGBI];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -359,8 +365,11 @@ __global__ __launch_bounds__(kGBPWaves * 64) void k_gb_pairs(SysDev *sys, const 
   for (int r = r0; r <= r1; ++r) {
     const int jl = (lane + r) & 63;
     const int j = jt * kGBI + jl;
+    // every lane evaluates (padding records sit far away, so the math stays
+    // finite); a lane without a pair adds zeros, and the accumulations need
+    // no per-lane predication
     const bool ok = vi && j < n && !(diag && r == 32 && lane >= 32);
-    if (ok) {
+    {
       int cls = 0;
       if (any_ex)
         for (int q = ep0; q < ee; ++q)
@@ -368,7 +377,7 @@ __global__ __launch_bounds__(kGBPWaves * 64) void k_gb_pairs(SysDev *sys, const 
       const double4 pj = jpos[wv][jl];
       const float4 aj = ja[wv][jl], bj4 = jb[wv][jl], cj4 = jc[wv][jl];
       const float dx = (float)(pj.x - xi), dy = (float)(pj.y - yi), dz = (float)(pj.z - zi);
-      const float r2 = dx * dx + dy * dy + dz * dz;
+      const float r2 = ok ? dx * dx + dy * dy + dz * dz : 1.0e8f;
       const float rinv = rsqrtf(r2);
       const float dist = r2 * rinv;
       const float Bj = aj.z, iBj = aj.w;
@@ -376,8 +385,9 @@ __global__ __launch_bounds__(kGBPWaves * 64) void k_gb_pairs(SysDev *sys, const 
       const float ex = __expf(-r2 * (i4Bi * iBj));  // exp(-r^2 / (4 Bi Bj))
       const float inv = rsqrtf(r2 + BB * ex);       // 1 / f_GB
       const float qq = qi * bj4.x;
-      const float dRi = dri * dburial(dist, rinv, ori, aj.x, aj.y);
-      const float dRj = bj4.z * dburial(dist, rinv, aj.x, ori, Si);
+      const float id4 = 0.25f * rinv, q4 = id4 * rinv;
+      const float dRi = dri * dburial(dist, r2, id4, q4, ori, aj.x, aj.y);
+      const float dRj = bj4.z * dburial(dist, r2, id4, q4, aj.x, ori, Si);
       const float c0 = qq * inv * inv * 0.5f * inv;
       const float ai = Bj + r2 * i4Bi, bj = Bi + r2 * (0.25f * iBj);
       float force = -c0 * ((2.0f * dist - 0.5f * ex * dist) + ex * dRi * ai + ex * dRj * bj) * rinv;
@@ -390,8 +400,8 @@ __global__ __launch_bounds__(kGBPWaves * 64) void k_gb_pairs(SysDev *sys, const 
       // GBACEForce.cpp:155: c1 (c_i R_i^-7 dR_i/dr + c_j R_j^-7 dR_j/dr) / r
       if (P.gb_ace) force += ace_c1 * (aki * dRi + cj4.z * dRj) * rinv;
       float fs = force;
-      if (energies) egb += (double)(-qq * inv * fac);
-      if (cls != PM_EXCL_FULL) {
+      if (energies) egb += ok ? (double)(-qq * inv * fac) : 0.0;
+      if (ok && cls != PM_EXCL_FULL) {
         float A, Bc;
         if (peratom) {
           const float sg = sgi + cj4.x, e2 = epi * cj4.y;
@@ -418,6 +428,7 @@ __global__ __launch_bounds__(kGBPWaves * 64) void k_gb_pairs(SysDev *sys, const 
           eco += (double)ec;
         }
       }
+      fs = ok ? fs : 0.0f;
       const float px = fs * dx, py = fs * dy, pz = fs * dz;
       fx -= (double)px;
       fy -= (double)py;
@@ -426,7 +437,7 @@ __global__ __launch_bounds__(kGBPWaves * 64) void k_gb_pairs(SysDev *sys, const 
       gy += (double)py;
       gz += (double)pz;
       if (VIR && energies) {
-        const bool other = __float_as_int(cj4.w) != moli;
+        const bool other = ok && __float_as_int(cj4.w) != moli;
         if (other || P.virial) {
           vir[0] += (double)(px * dx);
           vir[1] += (double)(px * dy);
@@ -533,9 +544,11 @@ void launch_gb(const Launch &L, const ForceDev &P, bool energies) {
   hipLaunchKernelGGL(k_gb_sym<2>, gp2, bp, 0, L.stream, L.sys, L.gbp_prefix, L.nsys, P);
   hipLaunchKernelGGL(k_gb_sums<2>, gt, bt, 0, L.stream, L.sys, L.gb_prefix, L.nsys, P);
   if (energies && (P.virial || P.molvirial))
-    hipLaunchKernelGGL(k_gb_pairs<true>, gp2, bp, 0, L.stream, L.sys, L.gbp_prefix, L.nsys, P, 1);
+    hipLaunchKernelGGL((k_gb_pairs<true, true>), gp2, bp, 0, L.stream, L.sys, L.gbp_prefix, L.nsys, P);
+  else if (energies)
+    hipLaunchKernelGGL((k_gb_pairs<false, true>), gp2, bp, 0, L.stream, L.sys, L.gbp_prefix, L.nsys, P);
   else
-    hipLaunchKernelGGL(k_gb_pairs<false>, gp2, bp, 0, L.stream, L.sys, L.gbp_prefix, L.nsys, P, energies ? 1 : 0);
+    hipLaunchKernelGGL((k_gb_pairs<false, false>), gp2, bp, 0, L.stream, L.sys, L.gbp_prefix, L.nsys, P);
   hipLaunchKernelGGL(k_gb_fsum, dim3(L.n_gb_tiles_total), dim3(kGBI), 0, L.stream, L.sys, L.gb_prefix, L.nsys, P,
                      energies ? 1 : 0);
 }
