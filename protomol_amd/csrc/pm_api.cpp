@@ -121,6 +121,7 @@ struct Carver {
 struct SysPrep {
   int n = 0;
   std::vector<int> excl_start, excl_partner, excl_class, corr, loc_start, loc_item;
+  std::vector<unsigned long long> excl_win;  // per atom: FULL, MODIFIED partner bits (b - a + 31), far flag
   std::vector<double> invm;
   std::vector<float> sig_half, eps_sqrt;
   std::vector<float4> lj;
@@ -522,6 +523,17 @@ SysPrep prepare(const pm_ctx *c, const pm_system_desc &d) {
       p.excl_class[fill[e.b]++] = e.cls;
     }
   }
+  // the same exclusions as bit masks over the atom-index window a - 31 .. a + 31
+  // (bit b - a + 31 of the FULL and MODIFIED words); bit 63 of the FULL word
+  // marks an atom with a partner outside the window, whose lookups use the CSR
+  p.excl_win.assign(2 * (size_t)n, 0ull);
+  for (int a = 0; a < n; ++a)
+    for (int k = p.excl_start[a]; k < p.excl_start[a + 1]; ++k) {
+      const int dd = p.excl_partner[k] - a;
+      if (dd < -31 || dd > 31) p.excl_win[2 * (size_t)a] |= 1ull << 63;
+      else if (p.excl_class[k] == PM_EXCL_FULL) p.excl_win[2 * (size_t)a] |= 1ull << (dd + 31);
+      else if (p.excl_class[k] == PM_EXCL_MODIFIED) p.excl_win[2 * (size_t)a + 1] |= 1ull << (dd + 31);
+    }
   // bonded terms
   p.n_bonds = (c->desc.flags & PM_FLAG_BONDED) ? d.n_bonds : 0;
   p.n_angles = (c->desc.flags & PM_FLAG_BONDED) ? d.n_angles : 0;
@@ -642,6 +654,7 @@ size_t carve(const pm_ctx *c, const SysPrep &p, const pm_system_desc &d, Carver 
   S.excl_start = cv.take<int>(p.excl_start.size());
   S.excl_partner = cv.take<int>(p.excl_partner.size());
   S.excl_class = cv.take<int>(p.excl_class.size());
+  S.excl_win = cv.take<unsigned long long>(p.excl_win.size());
   S.corr = cv.take<int>(p.corr.size());
   S.loc_start = cv.take<int>(p.loc_start.size());
   S.loc_item = cv.take<int>(p.loc_item.size());
@@ -870,6 +883,7 @@ int add_system(pm_ctx *c, const pm_system_desc &d) {
     upload(const_cast<int *>(S.excl_start), p.excl_start.data(), p.excl_start.size());
     upload(const_cast<int *>(S.excl_partner), p.excl_partner.data(), p.excl_partner.size());
     upload(const_cast<int *>(S.excl_class), p.excl_class.data(), p.excl_class.size());
+    upload(const_cast<unsigned long long *>(S.excl_win), p.excl_win.data(), p.excl_win.size());
     upload(const_cast<int *>(S.corr), p.corr.data(), p.corr.size());
     upload(const_cast<int *>(S.loc_start), p.loc_start.data(), p.loc_start.size());
     upload(const_cast<int *>(S.loc_item), p.loc_item.data(), p.loc_item.size());

@@ -135,6 +135,7 @@ struct SysDev {
   const int *molecule, *type;
   const float *sig_half, *eps_sqrt; // PM_LJ_PERATOM: sigma/2, sqrt(epsilon)
   const int *excl_start, *excl_partner, *excl_class;  // CSR of exclusions (both directions)
+  const unsigned long long *excl_win;  // 2 per atom: FULL / MODIFIED bits over partners a-31..a+31; FULL bit 63: far partner
   int *slot;               // atom -> slot
   float4 *lbuild;          // local coordinates at the last rebuild (displacement check)
   // binning scratch

@@ -745,10 +745,19 @@ __global__ __launch_bounds__(kListThreads) void k_pairlist(SysDev *sys, const in
         if (ck >= 0) {
           const int sj = cj_e * kCluster + jj;
           const int mj = S.smol[sj], bj = S.orig[sj];
+          // j atom's exclusions as index-window bits (one load pair per lane);
+          // an atom with a partner outside the window looks up the CSR
+          const unsigned long long wf = bj >= 0 ? S.excl_win[2 * (size_t)bj] : 0ull;
+          const unsigned long long wm = bj >= 0 ? S.excl_win[2 * (size_t)bj + 1] : 0ull;
+          const bool far = (wf >> 63) != 0;
           for (int ii = 0; ii < kCluster; ++ii) {
             const int mi = W.mol[ck * kCluster + ii], ai = W.at[ck * kCluster + ii];
             if (mi >= 0 && mi == mj && ai != bj) {
-              const int cls = excl_lookup(S, ai, bj);
+              const int dd = ai - bj + 31;
+              const bool inw = dd >= 0 && dd < 63;
+              const int cls = far ? excl_lookup(S, ai, bj)
+                                  : (inw && ((wf >> dd) & 1ull) ? PM_EXCL_FULL
+                                     : (inw && ((wm >> dd) & 1ull) ? PM_EXCL_MODIFIED : 0));
               if (cls == PM_EXCL_FULL) rf |= 1u << ii;
               if (cls == PM_EXCL_MODIFIED) rm |= 1u << ii;
             }
