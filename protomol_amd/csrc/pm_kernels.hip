@@ -857,8 +857,9 @@ __device__ __forceinline__ float dpp_sum8_scatter(float x, float y, float z, int
 }
 
 constexpr int kLJLds = 256;  // LJ table entries kept in LDS per wave (T <= 16)
-// LDS per wave of the pair kernel: the LJ table, then the super-cluster's 64
-// i atoms (position + charge, type | valid, atom index, per-atom LJ)
+// LDS per wave of the pair kernel: the LJ table as (A, B, 12 A, 6 B), then the
+// super-cluster's 64 i atoms (position + charge, byte offset of the atom's
+// LJ table row, atom index, per-atom LJ)
 constexpr int kPairWaveLds = kLJLds * 16 + kWave * (16 + 4 + 4 + 8);
 
 // One pair term set (OneAtomPair[Two]::doOneAtomPair, OneAtomPair.h:47-107):
