@@ -434,7 +434,7 @@ def main():
     ap.add_argument("--only", default="", help="run one extra form alone: batched, served, gb or ewald")
     ap.add_argument("--batch-systems", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=500)
-    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-requests", type=int, default=2000, help="requests per served phase (the stream's 2,000)")
     ap.add_argument("--serve-chunk", type=int, default=20)
     ap.add_argument("--serve-load", type=float, default=0.5)
     ap.add_argument("--serve-reserve-gib", type=float, default=24.0)
