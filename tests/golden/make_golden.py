@@ -163,6 +163,28 @@ def make_modified():
     np.savez_compressed(os.path.join(HERE, "ref_modified.npz"), **out)
 
 
+def gb_chain_modified(n=200, seed=7):
+    """A GB chain whose 1-4 pairs are MODIFIED, with lj_scaling 0.5 and
+    coulomb_scaling 1/1.2 (the scaled LJ and Coulomb of the GB systems' pair
+    terms)."""
+    g = S.gb_chain(n, seed=seed)
+    ex14 = np.stack([np.arange(n - 3), np.arange(3, n), np.full(n - 3, S.EXCL_MODIFIED)], 1)
+    g.exclusions = np.concatenate([g.exclusions, ex14]).astype(np.int32)
+    g.lj_scaling, g.coulomb_scaling = 0.5, 1.0 / 1.2
+    return g
+
+
+def make_gb_modified(n=200, seed=7):
+    g = gb_chain_modified(n, seed)
+    params = np.array([0.09, 1.0, 0.8, 4.85, 1.0, 78.3])
+    r = refio.system_records(g)
+    r.update(mode=np.int32(3), gb_rho=g.gb_radius, gb_S=g.gb_scaling, gb_params=params, cellsize=np.int32(5),
+             gb_ljcoul=np.int32(1))
+    out = refio.run(r)
+    np.savez_compressed(os.path.join(HERE, "ref_gb_modified.npz"), x=g.x, f=out["f"].reshape(-1, 3), e=out["e"],
+                        params=params, n=n, seed=seed)
+
+
 def make_molvirial(w):
     """The atomic and molecular virials (ScalarStructure::molecularVirial on)
     of the reference's classes: LJ + Coulomb C2 cutoff pairs and the full
@@ -319,6 +341,7 @@ def main():
         return
     if sys.argv[1:] == ["modified"]:
         make_modified()
+        make_gb_modified()
         return
     for name, case in CUTOFF_CASES.items():
         make_cutoff(name, w, case)
@@ -329,6 +352,7 @@ def main():
     make_gb_ace(200, seed=13)
     make_molvirial(w)
     make_modified()
+    make_gb_modified()
     make_water423()
     copy_water_expected()
     for f in sorted(os.listdir(HERE)):

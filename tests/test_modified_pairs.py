@@ -114,3 +114,48 @@ def test_modified_pairs_vs_reference_golden(tag, model):
         assert abs(e[s] - ref_e[s]) <= tol, (s, e[s], ref_e[s], tol)
     vtol = RTOL_E * max(1.0, np.max(np.abs(ref_e[7:16]))) + 1e-7 * cen[0]
     assert np.max(np.abs(e[7:16] - ref_e[7:16])) <= vtol
+
+
+GB_PARAMS = dict(offset=0.09, a=1.0, b=0.8, g=4.85, eps_in=1.0, eps_out=78.3)
+
+
+def gb_chain_modified(n, seed):
+    # the chain of make_golden.gb_chain_modified: 1-4 pairs MODIFIED,
+    # lj_scaling 0.5, coulomb_scaling 1/1.2
+    g = S.gb_chain(n, seed=seed)
+    ex14 = np.stack([np.arange(n - 3), np.arange(3, n), np.full(n - 3, S.EXCL_MODIFIED)], 1)
+    g.exclusions = np.concatenate([g.exclusions, ex14]).astype(np.int32)
+    g.lj_scaling, g.coulomb_scaling = 0.5, 1.0 / 1.2
+    return g
+
+
+def test_oracle_gb_modified_matches_reference():
+    # GB (GBBornRadii, GBPartialSum, GBForce) plus the scaled LJ / Coulomb of
+    # the MODIFIED pairs against the reference's classes
+    g = T.golden("ref_gb_modified.npz")
+    c = gb_chain_modified(int(g["n"]), int(g["seed"]))
+    assert np.array_equal(c.x, g["x"])
+    f, e, _ = O.gb_evaluate(c, **GB_PARAMS)
+    fl, el = T.vacuum_lj_coulomb(c)
+    assert close(f + fl, g["f"], rel=1e-11)
+    assert close(e + el, g["e"], rel=1e-11)
+
+
+@pytest.mark.gpu
+def test_gb_modified_pairs_vs_reference_golden():
+    print("case modified/gb", flush=True)
+    g = T.golden("ref_gb_modified.npz")
+    c = gb_chain_modified(int(g["n"]), int(g["seed"]))
+    d = L.ForceDesc()
+    d.gb_offset, d.gb_alpha, d.gb_beta, d.gb_gamma = 0.09, 1.0, 0.8, 4.85
+    d.gb_solute_dielec, d.gb_solvent_dielec = 1.0, 78.3
+    d.flags = L.FLAG_GB
+    with L.Context(d, [c]) as ctx:
+        f = np.zeros_like(c.x)
+        e = np.zeros(L.NSLOTS)
+        ctx.evaluate(0, c.x, f, e)
+    assert T.rms_rel(f, g["f"]) <= RTOL_F
+    npairs = 3 * c.n * (c.n - 1) // 2
+    for s in (0, 1):
+        tol = RTOL_E * max(1.0, abs(g["e"][s])) + 1e-7 * npairs
+        assert abs(e[s] - g["e"][s]) <= tol, (s, e[s], g["e"][s], tol)
