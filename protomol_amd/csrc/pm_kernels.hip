@@ -1708,7 +1708,11 @@ static void launch_pair_sw(const Launch &L, const ForceDev &P, int kind) {
   const bool b2 = TB != PM_TERM_COULOMB || P.sw_b.kind == PM_SWITCH_C2;
   constexpr int SB = TB == PM_TERM_COULOMB ? PM_SWITCH_C2 : PM_SWITCH_CUTOFF;
   constexpr int SBG = TB == PM_TERM_COULOMB ? kSwGeneric : PM_SWITCH_CUTOFF;
-  const bool same = TA != PM_TERM_NONE && TB != PM_TERM_NONE && P.cutA2 == P.cutB2 && P.cutA2d == P.cutB2d;
+  // one inclusion cutoff for both terms, and no switch that ends before it
+  // (OneAtomPairTwo includes a pair within the larger switch cutoff; the
+  // shorter switch must then be zeroed beyond its own cutoff, the OUTER form)
+  const bool same = TA != PM_TERM_NONE && TB != PM_TERM_NONE && P.cutA2 == P.cutB2 && P.cutA2d == P.cutB2d &&
+                    P.sw_a.cut2 >= P.cutA2 && (TB != PM_TERM_COULOMB || P.sw_b.cut2 >= P.cutB2);
   if (a2 && b2 && same) launch_pair_t<TA, TB, PM_SWITCH_C2, SB, true>(L, P, kind);
   else if (a2 && b2) launch_pair_t<TA, TB, PM_SWITCH_C2, SB>(L, P, kind);
   else launch_pair_t<TA, TB, kSwGeneric, SBG>(L, P, kind);

@@ -43,6 +43,8 @@ CUTOFF_CASES = {
     "cmpcncn": (1, 2, (4, 7.0, 10.0, 9.0, 2.0), (4, 7.0, 10.0, 9.0, 2.0)),
     "lj_c2": (1, 0, (2, 8.0, 10.0, 0.0, 0.0), None),
     "coul_c1": (2, 0, (1, 0.0, 10.0, 0.0, 0.0), None),
+    # OneAtomPairTwo with two cutoffs: LJ switched off at 8 A, Coulomb at 10 A
+    "c2c2_two": (1, 2, (2, 6.0, 8.0, 0.0, 0.0), (2, 8.0, 10.0, 0.0, 0.0)),
 }
 
 
@@ -338,6 +340,9 @@ def main():
         return
     if sys.argv[1:] == ["molvirial"]:
         make_molvirial(w)
+        return
+    if sys.argv[1:] == ["cutoff_two"]:
+        make_cutoff("c2c2_two", w, CUTOFF_CASES["c2c2_two"])
         return
     if sys.argv[1:] == ["modified"]:
         make_modified()

@@ -31,6 +31,7 @@ MODELS = {
     "cmpcncn": (L.TERM_LJ, (5, 7.0, 10.0, 9.0, 2.0), L.TERM_COULOMB, (5, 7.0, 10.0, 9.0, 2.0)),
     "lj_c2": (L.TERM_LJ, (3, 8.0, 10.0, 0.0, 0.0), L.TERM_NONE, (0, 0, 0, 0, 0)),
     "coul_c1": (L.TERM_COULOMB, (2, 0.0, 10.0, 0.0, 0.0), L.TERM_NONE, (0, 0, 0, 0, 0)),
+    "c2c2_two": (L.TERM_LJ, (3, 6.0, 8.0, 0.0, 0.0), L.TERM_COULOMB, (3, 8.0, 10.0, 0.0, 0.0)),
 }
 
 

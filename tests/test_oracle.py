@@ -13,7 +13,7 @@ import oracle as O
 import pmtest as T
 from protomol_amd import systems as S
 
-CUTOFF_CASES = ["c2c2", "c2c1", "cncn", "cmpcncn", "lj_c2", "coul_c1"]
+CUTOFF_CASES = ["c2c2", "c2c1", "cncn", "cmpcncn", "lj_c2", "coul_c1", "c2c2_two"]
 
 
 def _switch_from_driver(v):
@@ -32,7 +32,7 @@ def test_cutoff_matches_reference(case):
     f, e, cen = O.nonbonded_cutoff(w, ta, swa, tb, swb, algorithm_cutoff=10.0, cellsize=5.0, virial=True)
     # same cell walk, same arithmetic: bit-exact except the Cn family's
     # pow()-based coefficients (rounding differences of order 1e-12 relative)
-    tol = 0.0 if case in ("c2c2", "c2c1", "lj_c2", "coul_c1") else 1e-11
+    tol = 0.0 if case in ("c2c2", "c2c1", "lj_c2", "coul_c1", "c2c2_two") else 1e-11
     assert np.max(np.abs(f - g["f"])) <= tol * max(1.0, np.max(np.abs(g["f"])))
     assert np.allclose(e, g["e"], rtol=tol, atol=tol * np.max(np.abs(g["e"])))
     assert (int(cen[0]), int(cen[1])) == (int(g["census"][0]), int(g["census"][1]))
