@@ -108,12 +108,15 @@ def test_ewald_vs_reference_golden():
     check_virial(e, g["e"], npairs=cen[0])
 
 
-def test_lj_plus_ewald_fused_vs_oracle():
-    print("case lj+ewald fused", flush=True)
+@pytest.mark.parametrize("lj_on,lj_cut", [(8.0, 10.0), (6.0, 8.0)])
+def test_lj_plus_ewald_fused_vs_oracle(lj_on, lj_cut):
+    # LJ (C2) and the Ewald real term in one pass; with the LJ switch ending
+    # at 8 A the two terms keep their own inclusion cutoffs (8 and 10 A)
+    print(f"case lj+ewald fused, LJ C2 {lj_on}-{lj_cut}", flush=True)
     w = S.water_box(512, 25.0, seed=11)
-    fl, el, cl = O.nonbonded_cutoff(w, O.TERM_LJ, O.make_switch(O.SW_C2, 8.0, 10.0), cellsize=5.0, virial=True)
+    fl, el, cl = O.nonbonded_cutoff(w, O.TERM_LJ, O.make_switch(O.SW_C2, lj_on, lj_cut), cellsize=5.0, virial=True)
     fe, ee, ce, _ = O.ewald_real(w, T.EWALD_ALPHA, T.EWALD_ACCURACY, cellsize=5.0, virial=True)
-    with L.Context(force_desc(L.TERM_LJ, T.C2_8_10, 0, (0, 0, 0, 0, 0), ewald=True), [w]) as ctx:
+    with L.Context(force_desc(L.TERM_LJ, (3, lj_on, lj_cut, 0.0, 0.0), 0, (0, 0, 0, 0, 0), ewald=True), [w]) as ctx:
         f, e = evaluate(ctx, 0, w.x)
     assert T.rms_rel(f, fl + fe) <= RTOL_F
     check_energies(e, el + ee, npairs=ce[0])
