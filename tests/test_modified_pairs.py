@@ -159,3 +159,17 @@ def test_gb_modified_pairs_vs_reference_golden():
     for s in (0, 1):
         tol = RTOL_E * max(1.0, abs(g["e"][s])) + 1e-7 * npairs
         assert abs(e[s] - g["e"][s]) <= tol, (s, e[s], g["e"][s], tol)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tag", ["table", "atom"])
+def test_modified_pairs_md_forces_vs_reference_golden(tag):
+    # the MD path's forces-only pair kernel (F(x0) of pm_md_set_state) on the
+    # same MODIFIED-pair chains
+    print(f"case modified/md/{tag}", flush=True)
+    g = T.golden("ref_modified.npz")
+    c = chains(tag)
+    with L.Context(_desc(False), [c]) as ctx:
+        ctx.md_set_state(c.x, np.zeros_like(c.x))
+        _, _, f, _ = ctx.md_get_state()
+    assert T.rms_rel(f, g[f"f_cut_{tag}"]) <= RTOL_F
