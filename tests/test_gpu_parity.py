@@ -95,6 +95,19 @@ def test_cutoff_vs_reference_golden(model):
     check_virial(e, g["e"], npairs=cen[0])
 
 
+@pytest.mark.parametrize("model", ["c2c2", "c2c2_two", "cncn"])
+def test_md_forces_vs_reference_golden(model):
+    # the MD path's forces-only pair kernels (F(x0) of pm_md_set_state)
+    print(f"case md forces/{model}", flush=True)
+    g = T.golden(f"ref_cutoff_{model}.npz")
+    w = T.fixture_water()
+    ta, sa, tb, sb = MODELS[model]
+    with L.Context(force_desc(ta, sa, tb, sb), [w]) as ctx:
+        ctx.md_set_state(w.x, np.zeros_like(w.x))
+        _, _, f, _ = ctx.md_get_state()
+    assert T.rms_rel(f, g["f"]) <= RTOL_F
+
+
 def test_ewald_vs_reference_golden():
     print("case ewald", flush=True)
     g = T.golden("ref_ewald.npz")
