@@ -176,6 +176,21 @@ def gb_chain_modified(n=200, seed=7):
     return g
 
 
+def gb_chain_table(n=200, seed=7):
+    """The MODIFIED-pair GB chain with a 4-type LJ table (types i mod 4, own
+    A14 / B14) in place of per-atom LJ."""
+    g = gb_chain_modified(n, seed)
+    sig, eps = np.array([3.0, 3.4, 3.8, 2.6]), np.array([0.08, 0.12, 0.18, 0.05])
+    tab = np.zeros((4, 4, 4))
+    for a in range(4):
+        for b in range(4):
+            sg, ep = 0.5 * (sig[a] + sig[b]), math.sqrt(eps[a] * eps[b])
+            tab[a, b] = (4 * ep * sg ** 12, 4 * ep * sg ** 6, 2 * ep * (0.9 * sg) ** 12, 2 * ep * (0.9 * sg) ** 6)
+    g.type = (np.arange(n) % 4).astype(np.int32)
+    g.lj_table, g.lj_sigma, g.lj_epsilon = tab, None, None
+    return g
+
+
 def make_gb_modified(n=200, seed=7):
     g = gb_chain_modified(n, seed)
     params = np.array([0.09, 1.0, 0.8, 4.85, 1.0, 78.3])
@@ -183,8 +198,14 @@ def make_gb_modified(n=200, seed=7):
     r.update(mode=np.int32(3), gb_rho=g.gb_radius, gb_S=g.gb_scaling, gb_params=params, cellsize=np.int32(5),
              gb_ljcoul=np.int32(1))
     out = refio.run(r)
-    np.savez_compressed(os.path.join(HERE, "ref_gb_modified.npz"), x=g.x, f=out["f"].reshape(-1, 3), e=out["e"],
-                        params=params, n=n, seed=seed)
+    res = {"x": g.x, "f": out["f"].reshape(-1, 3), "e": out["e"], "params": params, "n": n, "seed": seed}
+    t = gb_chain_table(n, seed)
+    r = refio.system_records(t)
+    r.update(mode=np.int32(3), gb_rho=t.gb_radius, gb_S=t.gb_scaling, gb_params=params, cellsize=np.int32(5),
+             gb_ljcoul=np.int32(1))
+    out = refio.run(r)
+    res.update(f_table=out["f"].reshape(-1, 3), e_table=out["e"])
+    np.savez_compressed(os.path.join(HERE, "ref_gb_modified.npz"), **res)
 
 
 def make_molvirial(w):

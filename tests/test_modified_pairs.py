@@ -141,11 +141,29 @@ def test_oracle_gb_modified_matches_reference():
     assert close(e + el, g["e"], rel=1e-11)
 
 
+def gb_chain_table(n, seed):
+    # the chain of make_golden.gb_chain_table: a 4-type LJ table with its own
+    # A14 / B14 in place of per-atom LJ
+    g = gb_chain_modified(n, seed)
+    sig, eps = np.array([3.0, 3.4, 3.8, 2.6]), np.array([0.08, 0.12, 0.18, 0.05])
+    tab = np.zeros((4, 4, 4))
+    for a in range(4):
+        for b in range(4):
+            sg, ep = 0.5 * (sig[a] + sig[b]), math.sqrt(eps[a] * eps[b])
+            tab[a, b] = (4 * ep * sg ** 12, 4 * ep * sg ** 6, 2 * ep * (0.9 * sg) ** 12, 2 * ep * (0.9 * sg) ** 6)
+    g.type = (np.arange(n) % 4).astype(np.int32)
+    g.lj_table, g.lj_sigma, g.lj_epsilon = tab, None, None
+    return g
+
+
 @pytest.mark.gpu
-def test_gb_modified_pairs_vs_reference_golden():
-    print("case modified/gb", flush=True)
+@pytest.mark.parametrize("lj", ["atom", "table"])
+def test_gb_modified_pairs_vs_reference_golden(lj):
+    print(f"case modified/gb/{lj}", flush=True)
     g = T.golden("ref_gb_modified.npz")
-    c = gb_chain_modified(int(g["n"]), int(g["seed"]))
+    make = gb_chain_modified if lj == "atom" else gb_chain_table
+    c = make(int(g["n"]), int(g["seed"]))
+    ref_f, ref_e = (g["f"], g["e"]) if lj == "atom" else (g["f_table"], g["e_table"])
     d = L.ForceDesc()
     d.gb_offset, d.gb_alpha, d.gb_beta, d.gb_gamma = 0.09, 1.0, 0.8, 4.85
     d.gb_solute_dielec, d.gb_solvent_dielec = 1.0, 78.3
@@ -154,11 +172,11 @@ def test_gb_modified_pairs_vs_reference_golden():
         f = np.zeros_like(c.x)
         e = np.zeros(L.NSLOTS)
         ctx.evaluate(0, c.x, f, e)
-    assert T.rms_rel(f, g["f"]) <= RTOL_F
+    assert T.rms_rel(f, ref_f) <= RTOL_F
     npairs = 3 * c.n * (c.n - 1) // 2
     for s in (0, 1):
-        tol = RTOL_E * max(1.0, abs(g["e"][s])) + 1e-7 * npairs
-        assert abs(e[s] - g["e"][s]) <= tol, (s, e[s], g["e"][s], tol)
+        tol = RTOL_E * max(1.0, abs(ref_e[s])) + 1e-7 * npairs
+        assert abs(e[s] - ref_e[s]) <= tol, (s, e[s], ref_e[s], tol)
 
 
 @pytest.mark.gpu
