@@ -45,6 +45,8 @@ CUTOFF_CASES = {
     "coul_c1": (2, 0, (1, 0.0, 10.0, 0.0, 0.0), None),
     # OneAtomPairTwo with two cutoffs: LJ switched off at 8 A, Coulomb at 10 A
     "c2c2_two": (1, 2, (2, 6.0, 8.0, 0.0, 0.0), (2, 8.0, 10.0, 0.0, 0.0)),
+    # two cutoffs through the run-time switch path: LJ C2 ending at 8 A, Coulomb C1 at 10 A
+    "c2c1_two": (1, 2, (2, 6.0, 8.0, 0.0, 0.0), (1, 0.0, 10.0, 0.0, 0.0)),
 }
 
 
@@ -364,6 +366,7 @@ def main():
         return
     if sys.argv[1:] == ["cutoff_two"]:
         make_cutoff("c2c2_two", w, CUTOFF_CASES["c2c2_two"])
+        make_cutoff("c2c1_two", w, CUTOFF_CASES["c2c1_two"])
         return
     if sys.argv[1:] == ["modified"]:
         make_modified()

@@ -32,6 +32,7 @@ MODELS = {
     "lj_c2": (L.TERM_LJ, (3, 8.0, 10.0, 0.0, 0.0), L.TERM_NONE, (0, 0, 0, 0, 0)),
     "coul_c1": (L.TERM_COULOMB, (2, 0.0, 10.0, 0.0, 0.0), L.TERM_NONE, (0, 0, 0, 0, 0)),
     "c2c2_two": (L.TERM_LJ, (3, 6.0, 8.0, 0.0, 0.0), L.TERM_COULOMB, (3, 8.0, 10.0, 0.0, 0.0)),
+    "c2c1_two": (L.TERM_LJ, (3, 6.0, 8.0, 0.0, 0.0), L.TERM_COULOMB, (2, 0.0, 10.0, 0.0, 0.0)),
 }
 
 

@@ -13,7 +13,7 @@ import oracle as O
 import pmtest as T
 from protomol_amd import systems as S
 
-CUTOFF_CASES = ["c2c2", "c2c1", "cncn", "cmpcncn", "lj_c2", "coul_c1", "c2c2_two"]
+CUTOFF_CASES = ["c2c2", "c2c1", "cncn", "cmpcncn", "lj_c2", "coul_c1", "c2c2_two", "c2c1_two"]
 
 
 def _switch_from_driver(v):
