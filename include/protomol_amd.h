@@ -20,6 +20,8 @@
  *                    (force/system/SystemForce.h:32-35) -- ADDS into forces and energies
  *   pm_md_*          LeapfrogIntegrator::run (integrator/leapfrog/LeapfrogIntegrator.cpp:89-104)
  *                    with the force group evaluated on the device every step
+ *   pm_random_velocity  randomVelocity (topology/TopologyUtilities.cpp:21-48) after
+ *                    MainModule's seeding (module/MainModule.cpp:88-102)
  *   pm_pair_census   no reference counterpart: order-independent digest of the
  *                    evaluated pair set, for bit-exact pair-set checks
  *   pm_ctx_reserve, pm_system_*, pm_*_systems, pm_md_begin/advance/end
@@ -230,6 +232,18 @@ int pm_md_get_state_systems(pm_ctx *ctx, int32_t n, const int32_t *ids, double *
  * pm_system_set_seed gives one system its own seed (default: this seed). */
 int pm_md_set_langevin(pm_ctx *ctx, double temperature, double gamma_per_ps, uint32_t seed);
 int pm_system_set_seed(pm_ctx *ctx, int32_t id, uint32_t seed);
+
+/* The velocity start of a run, as ProtoMol draws it (host only, no device
+ * work): randomNumber(seed, randomtype) seeds the generator and returns one
+ * number (MainModule::configure, module/MainModule.cpp:88-102), then
+ * randomVelocity(temperature, topology, velocities, seed)
+ * (topology/TopologyUtilities.cpp:21-48) sets v = sqrt(kB T / m) times three
+ * randomGaussianNumber deviates per atom (base/MathUtilities.cpp:206-236).
+ * randomtype 0 (ProtoMol's default): the POSIX drand48 generator after
+ * srand48(seed), which randomNumber uses on Linux (MathUtilities.cpp:175-183);
+ * 1: ProtoMol's Random (base/Random.cpp:33-68).  v: 3n AoS FP64 (written). */
+int pm_random_velocity(int32_t n, const double *mass, double temperature, uint32_t seed, int32_t randomtype,
+                       double *v);
 
 /* Statistics of the last call: stats[0] pair-list rebuilds (summed over
  * systems), [1] 1 if an MD advance was re-run after a pair-list capacity grew, [2] entries (8x8

@@ -66,6 +66,11 @@ class OrcGB(C.Structure):
                 ("virial", C.c_int), ("molvirial", C.c_int)]
 
 
+class OrcRng(C.Structure):
+    _fields_ = [("type", C.c_int), ("x48", C.c_ushort * 3), ("iv", C.c_int32 * 32), ("iy", C.c_int32),
+                ("idum", C.c_int32), ("have_spare", C.c_int), ("spare", C.c_double)]
+
+
 _lib = None
 
 
@@ -89,8 +94,14 @@ def lib():
         L.orc_ewald_reciprocal.argtypes = [C.POINTER(OrcSystem), C.POINTER(OrcEwald), _dp, _dp, _dp, C.c_int]
         L.orc_poly5.argtypes = [C.c_double]
         L.orc_poly5.restype = C.c_double
-        L.orc_myerf.argtypes = [C.c_double]
-        L.orc_myerf.restype = C.c_double
+        L.orc_rng_seed.argtypes = [C.POINTER(OrcRng), C.c_uint32, C.c_int]
+        L.orc_rng_uniform.argtypes = [C.POINTER(OrcRng)]
+        L.orc_rng_uniform.restype = C.c_double
+        L.orc_rng_gaussian.argtypes = [C.POINTER(OrcRng)]
+        L.orc_rng_gaussian.restype = C.c_double
+        L.orc_random_velocity.argtypes = [C.POINTER(OrcSystem), C.c_double, C.POINTER(OrcRng), _dp]
+        L.orc_kinetic_energy.argtypes = [C.POINTER(OrcSystem), _dp]
+        L.orc_kinetic_energy.restype = C.c_double
         L.orc_gb_evaluate.argtypes = [C.POINTER(OrcSystem), C.POINTER(OrcGB), _dp, _dp, _dp, _dp]
         L.orc_bonded.argtypes = [C.POINTER(OrcSystem), C.c_int, _ip, _dp, _dp, C.c_int, _ip, _dp, _dp, _dp, _dp, _dp,
                                  C.c_int]
@@ -175,7 +186,7 @@ def nonbonded_cutoff(sys, term_a, sw_a, term_b=TERM_NONE, sw_b=None, algorithm_c
 
 
 def ewald_real(sys, alpha, accuracy, cellsize=5.0, correction=True, virial=False, nthreads=1, x=None,
-               molvirial=False):
+               molvirial=False, census=True):
     os_ = OracleSystem(sys)
     ew = OrcEwald()
     lib().orc_ewald_init(C.byref(ew), alpha, accuracy, cellsize, 1 if correction else 0, 1 if virial else 0)
@@ -184,7 +195,8 @@ def ewald_real(sys, alpha, accuracy, cellsize=5.0, correction=True, virial=False
     f = np.zeros(3 * sys.n)
     e = np.zeros(NSLOTS)
     cen = OrcCensus()
-    rc = lib().orc_ewald_real(C.byref(os_.c), C.byref(ew), _p(xx), _p(f), _p(e), C.byref(cen), nthreads)
+    rc = lib().orc_ewald_real(C.byref(os_.c), C.byref(ew), _p(xx), _p(f), _p(e), C.byref(cen) if census else None,
+                              nthreads)
     if rc != 0:
         raise RuntimeError(f"orc_ewald_real failed: {rc}")
     return f.reshape(-1, 3), e, (cen.count, cen.hash), ew.rc
@@ -277,6 +289,31 @@ def molecule_com(sys, x=None):
     com = np.zeros(3 * max(nm, 1))
     lib().orc_molecule_com(C.byref(os_.c), _p(xx), _p(com))
     return com[:3 * nm].reshape(-1, 3)
+
+
+def velocity_start(sys, temperature, seed, randomtype=0, ndraw=0):
+    """The velocity start of a ProtoMol run: randomNumber(seed, randomtype)
+    seeds the generator and returns one number (MainModule.cpp:88-102), then
+    randomVelocity(temperature, ...) (TopologyUtilities.cpp:21-48); then
+    `ndraw` further uniform and then Gaussian numbers of the same streams.
+    Returns (first number, velocities (n, 3), uniforms, gaussians)."""
+    os_ = OracleSystem(sys)
+    r = OrcRng()
+    L = lib()
+    L.orc_rng_seed(C.byref(r), seed, randomtype)
+    u0 = L.orc_rng_uniform(C.byref(r))
+    v = np.zeros(3 * sys.n)
+    L.orc_random_velocity(C.byref(os_.c), temperature, C.byref(r), _p(v))
+    u = np.array([L.orc_rng_uniform(C.byref(r)) for _ in range(ndraw)])
+    g = np.array([L.orc_rng_gaussian(C.byref(r)) for _ in range(ndraw)])
+    return u0, v.reshape(-1, 3), u, g
+
+
+def kinetic_energy(sys, v):
+    """kineticEnergy (TopologyUtilities.cpp:163-171)"""
+    os_ = OracleSystem(sys)
+    vv = np.ascontiguousarray(v, np.float64).reshape(-1)
+    return lib().orc_kinetic_energy(C.byref(os_.c), _p(vv))
 
 
 def pair_hash(lo, hi):

@@ -31,6 +31,21 @@ uint64_t orc_pair_hash(uint32_t lo, uint32_t hi) {
   return z ^ (z >> 31);
 }
 
+/* Per-thread accumulators of the threaded walks: one 128-byte-aligned block
+ * per thread, so no two threads ever write into one cache line (the energy
+ * slots and the census are updated once per pair). */
+typedef struct orc_acc {
+  double e[ORC_NSLOTS];
+  orc_census census;
+  double real_energy;
+} __attribute__((aligned(128))) orc_acc;
+
+static orc_acc *acc_alloc(int nthreads) {
+  orc_acc *a = (orc_acc *)aligned_alloc(128, sizeof(orc_acc) * (size_t)nthreads);
+  if (a) memset(a, 0, sizeof(orc_acc) * (size_t)nthreads);
+  return a;
+}
+
 static inline void census_add(orc_census *c, int a, int b) {
   if (!c) return;
   uint32_t lo = (uint32_t)(a < b ? a : b), hi = (uint32_t)(a < b ? b : a);
@@ -697,11 +712,10 @@ int orc_nonbonded_cutoff(const orc_system *sys, const orc_nonbonded *nb, const d
   pair_ctx *ctx = (pair_ctx *)calloc((size_t)nthreads, sizeof(pair_ctx));
   void **ctxp = (void **)calloc((size_t)nthreads, sizeof(void *));
   double *fbuf = nthreads > 1 ? (double *)calloc((size_t)nthreads * 3 * (size_t)sys->n, sizeof(double)) : NULL;
-  double *ebuf = (double *)calloc((size_t)nthreads * ORC_NSLOTS, sizeof(double));
-  orc_census *cbuf = (orc_census *)calloc((size_t)nthreads, sizeof(orc_census));
+  orc_acc *acc = acc_alloc(nthreads);
   double *com = NULL;
   int rc = 0;
-  if (!ctx || !ctxp || !ebuf || !cbuf || (nthreads > 1 && !fbuf)) { rc = -1; goto done; }
+  if (!ctx || !ctxp || !acc || (nthreads > 1 && !fbuf)) { rc = -1; goto done; }
   if (nb->molvirial) {
     com = (double *)calloc((size_t)orc_n_molecules(sys) * 3 + 3, sizeof(double));
     if (!com) { rc = -1; goto done; }
@@ -711,8 +725,8 @@ int orc_nonbonded_cutoff(const orc_system *sys, const orc_nonbonded *nb, const d
     ctx[t].com = com;
     ctx[t].sys = sys; ctx[t].nb = nb; ctx[t].pbc = &pbc; ctx[t].excl = &ex; ctx[t].x = x;
     ctx[t].f = nthreads > 1 ? fbuf + (size_t)t * 3 * sys->n : f;
-    ctx[t].e = nthreads > 1 ? ebuf + (size_t)t * ORC_NSLOTS : e;
-    ctx[t].census = census ? &cbuf[t] : NULL;
+    ctx[t].e = nthreads > 1 ? acc[t].e : e;
+    ctx[t].census = census ? &acc[t].census : NULL;
     ctx[t].cut2 = cut2;
     ctxp[t] = &ctx[t];
   }
@@ -721,13 +735,13 @@ int orc_nonbonded_cutoff(const orc_system *sys, const orc_nonbonded *nb, const d
     for (int t = 0; t < nthreads; t++) {
       const double *ft = fbuf + (size_t)t * 3 * sys->n;
       for (size_t k = 0; k < 3 * (size_t)sys->n; k++) f[k] += ft[k];
-      for (int s = 0; s < ORC_NSLOTS; s++) e[s] += ebuf[t * ORC_NSLOTS + s];
+      for (int s = 0; s < ORC_NSLOTS; s++) e[s] += acc[t].e[s];
     }
   }
   if (census)
-    for (int t = 0; t < nthreads; t++) { census->count += cbuf[t].count; census->hash += cbuf[t].hash; }
+    for (int t = 0; t < nthreads; t++) { census->count += acc[t].census.count; census->hash += acc[t].census.hash; }
 done:
-  free(ctx); free(ctxp); free(fbuf); free(ebuf); free(cbuf); free(pairs); free(com);
+  free(ctx); free(ctxp); free(fbuf); free(acc); free(pairs); free(com);
   free(ex.tab); cells_free(&cl);
   return rc;
 }
@@ -763,23 +777,6 @@ double orc_poly5(double ar) {
 }
 
 /* MathUtilities.cpp:12-36 (Numerical Recipes erfc, fractional error < 1.2e-7) */
-double orc_myerf(double x) {
-  double erfcc = 0.0;
-  if (x < 9.0) {
-    double z = fabs(x);
-    double t = 1.0 / (1.0 + 0.5 * z);
-    erfcc = t * exp(-z * z - 1.26551223 +
-                    t * (1.00002368 +
-                         t * (.37409196 +
-                              t * (.09678418 +
-                                   t * (-.18628806 +
-                                        t * (.27886807 +
-                                             t * (-1.13520398 + t * (1.48851587 + t * (-.82215223 + t * .17087277)))))))));
-    if (x < 0.0) erfcc = 2.0 - erfcc;
-  }
-  return 1.0 - erfcc;
-}
-
 int orc_ewald_init(orc_ewald *ew, double alpha, double accuracy, double cellsize, int correction,
                    int virial) {
   /* NonbondedFullEwaldSystemForce::initialize (NonbondedFullEwaldSystemForce.h:610-619) */
@@ -804,7 +801,7 @@ typedef struct ewald_ctx {
   const double *x;
   double *f;
   double *e;
-  double real_energy;
+  double *real_energy;
   orc_census *census;
   const double *lattice; /* nlat*3, lattice[0] = 0 */
   int nlat;
@@ -847,7 +844,7 @@ static void ewald_cellpair(void *vctx, const orc_cells *cl, int ci, int cj) {
         double sv = r2 > c->ew->rc2 ? 0.0 : 1.0;
         force = force * sv - energy * 0.0;
         energy = energy * sv;
-        c->real_energy += energy;
+        *c->real_energy += energy;
         double fij[3] = {rij[0] * force, rij[1] * force, rij[2] * force};
         fi[0] -= fij[0]; fi[1] -= fij[1]; fi[2] -= fij[2];
         c->f[3 * j] += fij[0]; c->f[3 * j + 1] += fij[1]; c->f[3 * j + 2] += fij[2];
@@ -863,6 +860,36 @@ static void ewald_cellpair(void *vctx, const orc_cells *cl, int ci, int cj) {
     }
     c->f[3 * i] += fi[0]; c->f[3 * i + 1] += fi[1]; c->f[3 * i + 2] += fi[2];
   }
+}
+
+/* first[k] = 1 when exclusion record k is the first of its (unordered pair,
+ * class) in table order: the correction term walks each such pair once. */
+typedef struct excl_key { int a, b, cls, k; } excl_key;
+static int cmp_excl_key(const void *x, const void *y) {
+  const excl_key *u = (const excl_key *)x, *v = (const excl_key *)y;
+  if (u->a != v->a) return u->a < v->a ? -1 : 1;
+  if (u->b != v->b) return u->b < v->b ? -1 : 1;
+  if (u->cls != v->cls) return u->cls < v->cls ? -1 : 1;
+  return u->k < v->k ? -1 : (u->k > v->k ? 1 : 0);
+}
+static char *excl_first_occurrence(const orc_system *sys) {
+  int n = sys->nexcl;
+  char *first = (char *)calloc((size_t)(n > 0 ? n : 1), 1);
+  excl_key *keys = (excl_key *)malloc(sizeof(excl_key) * (size_t)(n > 0 ? n : 1));
+  if (!first || !keys) { free(first); free(keys); return NULL; }
+  for (int k = 0; k < n; k++) {
+    int a = sys->excl[3 * k], b = sys->excl[3 * k + 1];
+    keys[k].a = a < b ? a : b;
+    keys[k].b = a < b ? b : a;
+    keys[k].cls = sys->excl[3 * k + 2];
+    keys[k].k = k;
+  }
+  qsort(keys, (size_t)n, sizeof(excl_key), cmp_excl_key);
+  for (int t = 0; t < n; t++)
+    if (t == 0 || keys[t].a != keys[t - 1].a || keys[t].b != keys[t - 1].b || keys[t].cls != keys[t - 1].cls)
+      first[keys[t].k] = 1;
+  free(keys);
+  return first;
 }
 
 static int cmp_norm(const void *a, const void *b) {
@@ -906,17 +933,13 @@ int orc_ewald_real(const orc_system *sys, const orc_ewald *ew, const double *x, 
      * ExclusionTable's pair table in add() order, duplicates dropped.  Off
      * Windows, MathUtilities.h:154-161 declares no ProtoMol::erf, so the
      * unqualified erf() there is the C library's. */
+    char *first = excl_first_occurrence(sys);
+    if (!first && sys->nexcl > 0) { free(ex.tab); free(lat); return -1; }
     for (int k = 0; k < sys->nexcl; k++) {
       int a1 = sys->excl[3 * k], a2 = sys->excl[3 * k + 1], cls = sys->excl[3 * k + 2];
       if (cls == ORC_EXCL_NONE || a1 == a2) continue;
       if (a1 > a2) { int s = a1; a1 = a2; a2 = s; }
-      int dup = 0;
-      for (int t = 0; t < k; t++) {
-        int b1 = sys->excl[3 * t], b2 = sys->excl[3 * t + 1];
-        if (b1 > b2) { int s = b1; b1 = b2; b2 = s; }
-        if (b1 == a1 && b2 == a2 && sys->excl[3 * t + 2] == cls) { dup = 1; break; }
-      }
-      if (dup) continue;
+      if (!first[k]) continue;
       double rij[3];
       double r2 = pbc_min_diff(&pbc, x + 3 * a1, x + 3 * a2, rij);
       double qq = sys->charge[a1] * sys->charge[a2];
@@ -931,6 +954,7 @@ int orc_ewald_real(const orc_system *sys, const orc_ewald *ew, const double *x, 
       f[3 * a2] += fij[0]; f[3 * a2 + 1] += fij[1]; f[3 * a2 + 2] += fij[2];
       if (ew->virial) add_virial(e, fij, rij);
     }
+    free(first);
     double q2 = 0.0, q = 0.0;
     for (int i = 0; i < sys->n; i++) q2 += sys->charge[i] * sys->charge[i];
     point = -q2 * ew->alpha / sqrt(M_PI);
@@ -950,10 +974,9 @@ int orc_ewald_real(const orc_system *sys, const orc_ewald *ew, const double *x, 
   ewald_ctx *ctx = (ewald_ctx *)calloc((size_t)nthreads, sizeof(ewald_ctx));
   void **ctxp = (void **)calloc((size_t)nthreads, sizeof(void *));
   double *fbuf = nthreads > 1 ? (double *)calloc((size_t)nthreads * 3 * (size_t)sys->n, sizeof(double)) : NULL;
-  double *ebuf = (double *)calloc((size_t)nthreads * ORC_NSLOTS, sizeof(double));
-  orc_census *cbuf = (orc_census *)calloc((size_t)nthreads, sizeof(orc_census));
+  orc_acc *acc = acc_alloc(nthreads);
   double *com = NULL;
-  if (!ctx || !ctxp || !ebuf || !cbuf || (nthreads > 1 && !fbuf)) { rc = -1; goto done; }
+  if (!ctx || !ctxp || !acc || (nthreads > 1 && !fbuf)) { rc = -1; goto done; }
   if (ew->molvirial) {
     com = (double *)calloc((size_t)orc_n_molecules(sys) * 3 + 3, sizeof(double));
     if (!com) { rc = -1; goto done; }
@@ -963,26 +986,27 @@ int orc_ewald_real(const orc_system *sys, const orc_ewald *ew, const double *x, 
     ctx[t].com = com;
     ctx[t].sys = sys; ctx[t].ew = ew; ctx[t].pbc = &pbc; ctx[t].excl = &ex; ctx[t].x = x;
     ctx[t].f = nthreads > 1 ? fbuf + (size_t)t * 3 * sys->n : f;
-    ctx[t].e = nthreads > 1 ? ebuf + (size_t)t * ORC_NSLOTS : e;
-    ctx[t].census = census ? &cbuf[t] : NULL;
+    ctx[t].e = nthreads > 1 ? acc[t].e : e;
+    ctx[t].census = census ? &acc[t].census : NULL;
+    ctx[t].real_energy = &acc[t].real_energy;
     ctx[t].lattice = lat; ctx[t].nlat = nlat;
     ctxp[t] = &ctx[t];
   }
   rc = run_walk(&cl, pairs, np, nne, nthreads, ctxp, ewald_cellpair);
   double real = 0.0;
-  for (int t = 0; t < nthreads; t++) real += ctx[t].real_energy;
+  for (int t = 0; t < nthreads; t++) real += acc[t].real_energy;
   if (nthreads > 1) {
     for (int t = 0; t < nthreads; t++) {
       const double *ft = fbuf + (size_t)t * 3 * sys->n;
       for (size_t k = 0; k < 3 * (size_t)sys->n; k++) f[k] += ft[k];
-      for (int s = 0; s < ORC_NSLOTS; s++) e[s] += ebuf[t * ORC_NSLOTS + s];
+      for (int s = 0; s < ORC_NSLOTS; s++) e[s] += acc[t].e[s];
     }
   }
   if (census)
-    for (int t = 0; t < nthreads; t++) { census->count += cbuf[t].count; census->hash += cbuf[t].hash; }
+    for (int t = 0; t < nthreads; t++) { census->count += acc[t].census.count; census->hash += acc[t].census.hash; }
   e[ORC_COULOMB] += real + 0.0 + intra + 0.0 + point + charged;
 done:
-  free(ctx); free(ctxp); free(fbuf); free(ebuf); free(cbuf); free(pairs); free(lat); free(com);
+  free(ctx); free(ctxp); free(fbuf); free(acc); free(pairs); free(lat); free(com);
   free(ex.tab); cells_free(&cl);
   return rc;
 }
@@ -1471,8 +1495,12 @@ void orc_bonded(const orc_system *sys, int nbonds, const int *bonds, const doubl
 
 /* ---------------------------------------------------------- RNG, velocity */
 
-/* Random::seed / Random::rand (Random.cpp:33-68): Park-Miller minimal
- * standard generator (Schrage) with a Bays-Durham shuffle of 32. */
+/* randomNumber(seed, randomtype) (MathUtilities.cpp:153-187): randomtype 1 is
+ * ProtoMol's Random class (Random.cpp:33-68), the Park-Miller minimal standard
+ * generator (Schrage's method) with a Bays-Durham shuffle of 32; randomtype 0
+ * (the default, MainModule.cpp:53) is the C library's drand48 after
+ * srand48(seed) on Linux -- here erand48 on a state of our own, the same
+ * generator. */
 #define RNG_IA 16807
 #define RNG_IM 2147483647
 #define RNG_IQ 127773
@@ -1480,9 +1508,14 @@ void orc_bonded(const orc_system *sys, int nbonds, const int *bonds, const doubl
 #define RNG_NTAB 32
 #define RNG_NDIV (1 + (RNG_IM - 1) / RNG_NTAB)
 
-void orc_rng_seed(orc_rng *r, uint32_t seed) {
+void orc_rng_seed(orc_rng *r, uint32_t seed, int randomtype) {
   memset(r, 0, sizeof(*r));
-  int32_t idum = (int32_t)seed;
+  r->type = randomtype ? 1 : 0;
+  /* srand48: X = seed << 16 | 0x330E (the low 32 bits of a long seed) */
+  r->x48[0] = 0x330E;
+  r->x48[1] = (unsigned short)(seed & 0xFFFFu);
+  r->x48[2] = (unsigned short)(seed >> 16);
+  int32_t idum = (int32_t)seed; /* Random::seed takes an unsigned, stores it in an int */
   if (idum < 1) idum = 1;
   for (int j = RNG_NTAB + 7; j >= 0; j--) {
     int32_t k = idum / RNG_IQ;
@@ -1495,6 +1528,7 @@ void orc_rng_seed(orc_rng *r, uint32_t seed) {
 }
 
 double orc_rng_uniform(orc_rng *r) {
+  if (r->type == 0) return erand48(r->x48);
   int32_t k = r->idum / RNG_IQ;
   r->idum = RNG_IA * (r->idum - k * RNG_IQ) - RNG_IR * k;
   if (r->idum < 0) r->idum += RNG_IM;
@@ -1506,7 +1540,8 @@ double orc_rng_uniform(orc_rng *r) {
   return t > less1 ? less1 : t;
 }
 
-/* randomGaussianNumber (MathUtilities.cpp:206-236): polar Box-Muller */
+/* randomGaussianNumber (MathUtilities.cpp:206-236): polar Box-Muller, the
+ * second deviate of each pair kept for the next call */
 double orc_rng_gaussian(orc_rng *r) {
   if (r->have_spare) { r->have_spare = 0; return r->spare; }
   double rr = 2.0, v1 = 0, v2 = 0;
@@ -1521,18 +1556,19 @@ double orc_rng_gaussian(orc_rng *r) {
   return v2 * fac;
 }
 
-void orc_random_velocity(const orc_system *sys, double temperature, uint32_t seed, double *v) {
+/* randomVelocity (TopologyUtilities.cpp:21-48): sqrt(kB T / m) times three
+ * deviates per atom, drawn from r */
+void orc_random_velocity(const orc_system *sys, double temperature, orc_rng *r, double *v) {
   const double kbT = temperature * 0.001987191; /* PMConstants.cpp:70 */
-  orc_rng r;
-  orc_rng_seed(&r, seed);
   for (int i = 0; i < sys->n; i++) {
     double s = sqrt(kbT / sys->mass[i]);
-    v[3 * i] = s * orc_rng_gaussian(&r);
-    v[3 * i + 1] = s * orc_rng_gaussian(&r);
-    v[3 * i + 2] = s * orc_rng_gaussian(&r);
+    v[3 * i] = s * orc_rng_gaussian(r);
+    v[3 * i + 1] = s * orc_rng_gaussian(r);
+    v[3 * i + 2] = s * orc_rng_gaussian(r);
   }
 }
 
+/* kineticEnergy (TopologyUtilities.cpp:163-171) */
 double orc_kinetic_energy(const orc_system *sys, const double *v) {
   double ke = 0.0;
   for (int i = 0; i < sys->n; i++)

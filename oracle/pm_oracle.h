@@ -161,9 +161,8 @@ int orc_ewald_kvectors(const double box[3], double alpha, double accuracy, int *
 int orc_ewald_reciprocal(const orc_system *sys, const orc_ewald *ew, const double *x,
                          double *f, double *e, int nthreads);
 
-/* MathUtilities.cpp:12-49 */
+/* MathUtilities.cpp:39-49 */
 double orc_poly5(double ar);
-double orc_myerf(double x);
 
 /* GB: GBBornRadii -> GBPartialSum -> GBForce, all pairs, vacuum.
  * born_out (n, may be NULL) receives the Born radii. */
@@ -178,18 +177,22 @@ void orc_bonded(const orc_system *sys, int nbonds, const int *bonds, const doubl
                 const double *angle_k, const double *angle_t0, const double *x,
                 double *f, double *e, int virial);
 
-/* NR ran1 (Random.cpp:33-68) and Box-Muller Gaussian (MathUtilities.cpp:206-236). */
+/* randomNumber (MathUtilities.cpp:153-187): randomtype 1 = Random
+ * (Random.cpp:33-68), 0 = drand48 after srand48(seed); randomGaussianNumber
+ * (MathUtilities.cpp:206-236) keeps the second deviate of each pair. */
 typedef struct orc_rng {
+  int type;
+  unsigned short x48[3];
   int32_t iv[32];
   int32_t iy, idum;
   int have_spare;
   double spare;
 } orc_rng;
-void orc_rng_seed(orc_rng *r, uint32_t seed);
+void orc_rng_seed(orc_rng *r, uint32_t seed, int randomtype);
 double orc_rng_uniform(orc_rng *r);
 double orc_rng_gaussian(orc_rng *r);
-/* TopologyUtilities.cpp:21-48 */
-void orc_random_velocity(const orc_system *sys, double temperature, uint32_t seed, double *v);
+/* TopologyUtilities.cpp:21-48, drawing from r */
+void orc_random_velocity(const orc_system *sys, double temperature, orc_rng *r, double *v);
 /* TopologyUtilities.cpp:163-171 */
 double orc_kinetic_energy(const orc_system *sys, const double *v);
 

@@ -1,5 +1,7 @@
 // ref_driver.cpp -- runs ProtoMol's own force classes on a system read from a
-// record file and writes forces, energies and the evaluated pair census.
+// record file and writes forces, energies and the evaluated pair census (and,
+// in mode 5, its velocity start: random number streams, randomVelocity and
+// kineticEnergy).
 //
 // TEST INFRASTRUCTURE ONLY.  Built by oracle/Makefile against the ProtoMol
 // sources where they lie (never copied); the binary lands in oracle/_ref/ and
@@ -352,6 +354,31 @@ static int do_bonded(Recs &r, PTopo &topo, Vector3DBlock &pos, Vector3DBlock &f,
   return 0;
 }
 
+// mode 5: the velocity start of a run as MainModule draws it.  configure()
+// seeds the generator with randomNumber(seed, randomtype), which also returns
+// one number (MainModule.cpp:88-102); postBuild() then calls
+// randomVelocity(temperature, topology, velocities, seed)
+// (MainModule.cpp:143-147, TopologyUtilities.cpp:21-48).  Outputs: that first
+// number, the velocities, kineticEnergy and temperature of them, and `ndraw`
+// further randomNumber and then randomGaussianNumber values of the streams.
+static int do_velocity(Recs &r, int n, Vector3DBlock &v, std::vector<double> &out_u0, std::vector<double> &out_ke,
+                       std::vector<double> &out_u, std::vector<double> &out_g) {
+  VTopo topo(1.0, ExclusionType(ExclusionType::ONE3), VBC(), CCM(5.0), 16.0f);
+  topo.atoms.resize(n);
+  for (int i = 0; i < n; i++) topo.atoms[i].scaledMass = r["mass"].d[i];
+  topo.degreesOfFreedom = 3 * n - 3;  // BuildTopology.cpp:121
+  const unsigned seed = (unsigned)geti(r, "seed", 1234);
+  const unsigned rtype = (unsigned)geti(r, "randomtype", 0);
+  out_u0.push_back(randomNumber(seed, rtype));
+  randomVelocity(getd(r, "temperature", 300.0), &topo, &v, seed);
+  out_ke.push_back(kineticEnergy(&topo, &v));
+  out_ke.push_back(temperature(&topo, &v));
+  const int nd = geti(r, "ndraw", 0);
+  for (int k = 0; k < nd; k++) out_u.push_back(randomNumber(seed));
+  for (int k = 0; k < nd; k++) out_g.push_back(randomGaussianNumber(seed));
+  return 0;
+}
+
 int main(int argc, char **argv) {
   if (argc != 3) { fprintf(stderr, "usage: ref_driver in.rec out.rec\n"); return 2; }
   Report::report << Report::donthint;
@@ -367,9 +394,10 @@ int main(int argc, char **argv) {
   e.virial(geti(r, "virial", 0) != 0);
   // the molecular virial is on by default in ScalarStructure; the record says
   e.molecularVirial(geti(r, "molvirial", 0) != 0);
-  std::vector<double> born;
+  std::vector<double> born, u0, ke, uu, gg;
   int rc = -1;
   if (mode == 3) rc = do_gb(r, pos, f, e, born);
+  else if (mode == 5) rc = do_velocity(r, n, f, u0, ke, uu, gg);
   else {
     const std::vector<double> &box = r["box"].d, &org = r["origin"].d;
     PBC bc(Vector3D(box[0], 0, 0), Vector3D(0, box[1], 0), Vector3D(0, 0, box[2]), Vector3D(org[0], org[1], org[2]));
@@ -394,6 +422,12 @@ int main(int argc, char **argv) {
   double hh[2] = {(double)(g_hash >> 32), (double)(g_hash & 0xffffffffull)};
   write_rec_d(fp, "census_hash", hh, 2);
   if (!born.empty()) write_rec_d(fp, "born", &born[0], (long long)born.size());
+  if (mode == 5) {
+    write_rec_d(fp, "u0", &u0[0], 1);
+    write_rec_d(fp, "ke_t", &ke[0], 2);
+    write_rec_d(fp, "u", uu.empty() ? NULL : &uu[0], (long long)uu.size());
+    write_rec_d(fp, "g", gg.empty() ? NULL : &gg[0], (long long)gg.size());
+  }
   fclose(fp);
   return 0;
 }

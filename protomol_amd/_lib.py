@@ -24,7 +24,8 @@ EXPORTED = ("pm_abi_version", "pm_device_info", "pm_ctx_create", "pm_ctx_destroy
             "pm_ctx_memory", "pm_system_footprint", "pm_system_add", "pm_system_remove", "pm_last_error",
             "pm_evaluate", "pm_evaluate_batch", "pm_evaluate_systems", "pm_pair_census", "pm_md_set_state",
             "pm_md_run", "pm_md_get_state", "pm_md_set_state_systems", "pm_md_begin", "pm_md_advance",
-            "pm_md_end", "pm_md_get_state_systems", "pm_md_set_langevin", "pm_system_set_seed", "pm_stats")
+            "pm_md_end", "pm_md_get_state_systems", "pm_md_set_langevin", "pm_system_set_seed", "pm_stats",
+            "pm_random_velocity")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -98,6 +99,7 @@ def lib():
         L.pm_md_get_state_systems.argtypes = [C.c_void_p, C.c_int32, _ip, _dp, _dp, _dp, _dp]
         L.pm_md_set_langevin.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_uint32]
         L.pm_system_set_seed.argtypes = [C.c_void_p, C.c_int32, C.c_uint32]
+        L.pm_random_velocity.argtypes = [C.c_int32, _dp, C.c_double, C.c_uint32, C.c_int32, _dp]
         _lib = L
     return _lib
 
@@ -118,6 +120,15 @@ def device_info():
     hbm = C.c_double()
     check(lib().pm_device_info(name, 256, C.byref(ncu), C.byref(hbm)))
     return name.value.decode(), ncu.value, hbm.value
+
+
+def random_velocity(mass, temperature: float, seed: int, randomtype: int = 0) -> np.ndarray:
+    """ProtoMol's velocity start (pm_random_velocity): (n, 3) velocities."""
+    m = np.ascontiguousarray(mass, np.float64).reshape(-1)
+    v = np.zeros((len(m), 3))
+    check(lib().pm_random_velocity(len(m), ptr(m), float(temperature), int(seed) & 0xFFFFFFFF, int(randomtype),
+                                   ptr(v)))
+    return v
 
 
 def switch(kind=SWITCH_NONE, switchon=0.0, cutoff=0.0, switchoff=0.0, order=0.0) -> SwitchDesc:

@@ -1457,6 +1457,72 @@ void get_state_ids(pm_ctx *c, const std::vector<int> &ids, double *x, double *v,
   }
 }
 
+// randomNumber (MathUtilities.cpp:153-187) with both of its generators.
+// randomtype 0: the POSIX drand48 sequence -- X <- (0x5DEECE66D X + 11) mod 2^48
+// from X = seed << 16 | 0x330E (srand48), each draw X / 2^48 -- which the
+// reference calls on Linux; randomtype 1: ProtoMol's Random, the Park-Miller
+// minimal standard generator (Schrage's method) behind a Bays-Durham shuffle
+// table of 32 (Random.cpp:33-68).  gaussian() is randomGaussianNumber
+// (MathUtilities.cpp:206-236): polar Box-Muller, the second deviate of each
+// pair returned by the next call.
+class RefRandom {
+ public:
+  RefRandom(uint32_t seed, int randomtype) : type_(randomtype ? 1 : 0) {
+    x48_ = ((uint64_t)seed << 16) | 0x330Eu;
+    int32_t idum = (int32_t)seed;  // Random::seed keeps an unsigned seed in an int
+    if (idum < 1) idum = 1;
+    for (int j = kTab + 7; j >= 0; j--) {  // 8 warm-ups, then the table
+      idum = step(idum);
+      if (j < kTab) tab_[j] = idum;
+    }
+    last_ = tab_[0];
+    idum_ = idum;
+  }
+  double uniform() {
+    if (type_ == 0) {
+      x48_ = (0x5DEECE66Dull * x48_ + 11u) & ((1ull << 48) - 1);
+      return std::ldexp((double)x48_, -48);
+    }
+    idum_ = step(idum_);
+    const int j = last_ / (1 + (kMod - 1) / kTab);
+    last_ = tab_[j];
+    tab_[j] = idum_;
+    const double t = (1.0 / kMod) * last_, less_than_one = 1.0 - 3.0e-16;
+    return t > less_than_one ? less_than_one : t;
+  }
+  double gaussian() {
+    if (spare_ok_) {
+      spare_ok_ = false;
+      return spare_;
+    }
+    double r = 2.0, v1 = 0.0, v2 = 0.0;
+    while (r >= 1.0 || r < 1.523e-8) {
+      v1 = 2.0 * uniform() - 1.0;
+      v2 = 2.0 * uniform() - 1.0;
+      r = v1 * v1 + v2 * v2;
+    }
+    const double fac = std::sqrt(-2.0 * std::log(r) / r);
+    spare_ = v1 * fac;
+    spare_ok_ = true;
+    return v2 * fac;
+  }
+
+ private:
+  static constexpr int kTab = 32;
+  static constexpr int32_t kMod = 2147483647;
+  static int32_t step(int32_t x) {  // 16807 x mod (2^31 - 1) without overflow
+    const int32_t k = x / 127773;
+    x = 16807 * (x - k * 127773) - 2836 * k;
+    return x < 0 ? x + kMod : x;
+  }
+  int type_;
+  uint64_t x48_;
+  int32_t tab_[kTab];
+  int32_t last_ = 0, idum_ = 0;
+  bool spare_ok_ = false;
+  double spare_ = 0.0;
+};
+
 int run_guarded(pm_ctx *c, const std::function<void()> &fn) {
   try {
     fn();
@@ -1783,6 +1849,23 @@ int pm_system_set_seed(pm_ctx *c, int32_t id, uint32_t seed) {
     HIPCHECK(hipStreamSynchronize(c->stream));
     h.dev.lv_seed = seed;
     drop_sets_with(c, id);
+  });
+}
+
+int pm_random_velocity(int32_t n, const double *mass, double temperature, uint32_t seed, int32_t randomtype,
+                       double *v) {
+  return run_guarded(nullptr, [&] {
+    if (n < 0 || (n > 0 && (!mass || !v))) throw Fail{PM_EINVAL, "masses and velocities required"};
+    if (!(temperature >= 0.0)) throw Fail{PM_EINVAL, "Kelvin temperature cannot be negative"};
+    if (randomtype != 0 && randomtype != 1) throw Fail{PM_EINVAL, "randomtype must be 0 or 1"};
+    RefRandom rng(seed, randomtype);
+    (void)rng.uniform();  // the number randomNumber(seed, randomtype) returns while seeding
+    const double kbt = temperature * kBoltzmann;
+    for (int32_t i = 0; i < n; ++i) {
+      if (!(mass[i] > 0.0)) throw Fail{PM_EINVAL, "masses must be positive"};
+      const double s = std::sqrt(kbt / mass[i]);
+      for (int d = 0; d < 3; ++d) v[3 * (size_t)i + d] = s * rng.gaussian();
+    }
   });
 }
 

@@ -380,12 +380,13 @@ def gb_chain(n_atoms: int, seed: int, bond: float = 1.53, min_dist: float = 2.5)
                   gb_radius=rho, gb_scaling=gbs, name=f"gbchain{n_atoms}")
 
 
-def maxwell_velocities(sys: System, temperature: float, seed: int) -> np.ndarray:
+def maxwell_velocities(sys: System, temperature: float, seed: int, randomtype: int = 0) -> np.ndarray:
     """Seeded Maxwell velocities, sigma = sqrt(kB T / m) per component, in
-    ProtoMol velocity units (TopologyUtilities.cpp:21-48)."""
-    rng = np.random.default_rng(seed)
-    s = np.sqrt(temperature * BOLTZMANN / sys.mass)
-    return rng.normal(size=(sys.n, 3)) * s[:, None]
+    ProtoMol velocity units: the velocity start of a ProtoMol run with this
+    seed and randomtype (MainModule.cpp:88-102, 143-147;
+    TopologyUtilities.cpp:21-48), drawn by the library's pm_random_velocity."""
+    from protomol_amd import _lib
+    return _lib.random_velocity(sys.mass, temperature, seed, randomtype)
 
 
 def kinetic_energy(sys: System, v: np.ndarray) -> float:

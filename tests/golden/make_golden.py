@@ -10,6 +10,8 @@ Two kinds of fixtures:
                    sources) on seeded synthetic systems: forces, the 36
                    ScalarStructure slots, the evaluated pair-set census and,
                    for GB, the Born radii.
+  ref_velocity.npz the reference's velocity start (random number streams of
+                   both generator types, randomVelocity, kineticEnergy)
   water423.npz     the data files of the reference's regression test
                    test/tests/water_CHARMM_PERIODIC_L.conf: PSF/PAR/PDB
                    inputs and the committed expected outputs (forces at step 0
@@ -350,8 +352,37 @@ def copy_water_expected():
                         os.path.join(dst, f"water_CHARMM_PERIODIC_L.{ext}"))
 
 
+# velocity starts: (tag, randomtype, seed, temperature); the system's masses
+# are those of the water fixture with every 7th atom made heavier
+VELOCITY_CASES = (("t0", 0, 1234, 300.0), ("t1", 1, 4242, 310.0), ("t0_big_seed", 0, 3000000000, 250.0))
+
+
+def make_velocity():
+    """The reference's velocity start (MainModule's randomNumber(seed,
+    randomtype) seeding, randomVelocity, kineticEnergy, temperature) and
+    further uniform and Gaussian draws, for both random-number types."""
+    w = water_fixture_system()
+    mass = w.mass.copy()
+    mass[::7] *= 3.5
+    out = {"mass": mass}
+    for tag, rtype, seed, temp in VELOCITY_CASES:
+        r = {"n": np.int32(w.n), "x": np.zeros(3 * w.n), "mass": mass, "mode": np.int32(5),
+             "seed": np.int32(np.uint32(seed).view(np.int32)), "randomtype": np.int32(rtype),
+             "temperature": temp, "ndraw": np.int32(4000)}
+        o = refio.run(r)
+        out[f"{tag}_v"] = o["f"].reshape(-1, 3)
+        out[f"{tag}_u0"] = o["u0"][0]
+        out[f"{tag}_ke"], out[f"{tag}_temp"] = o["ke_t"][0], o["ke_t"][1]
+        out[f"{tag}_u"], out[f"{tag}_g"] = o["u"], o["g"]
+        out[f"{tag}_params"] = np.array([rtype, seed, temp], dtype=np.float64)
+    np.savez_compressed(os.path.join(HERE, "ref_velocity.npz"), **out)
+
+
 def main():
     w = water_fixture_system()
+    if sys.argv[1:] == ["velocity"]:
+        make_velocity()
+        return
     if sys.argv[1:] == ["ewald_recip"]:
         make_ewald_reciprocal(w)
         return
@@ -384,6 +415,7 @@ def main():
     make_gb_modified()
     make_water423()
     copy_water_expected()
+    make_velocity()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -54,3 +54,22 @@ def test_fluctuation_dissipation():
     _, v, _, _ = T.langevin_oracle(zero, s.x.copy(), np.zeros((s.n, 3)), s.mass, 2.0, 200, temp, 50.0, seed=9)
     var = v.var()
     assert abs(var / (S.BOLTZMANN * temp / 16.0) - 1.0) < 0.02
+
+
+def test_constant_force_exact_ornstein_uhlenbeck_mean():
+    # T = 0 and a constant force: dv/dt = F/m - gamma v has the exact solution
+    # v(t) = v0 e^{-gamma t} + F / (gamma m) (1 - e^{-gamma t}), and every half
+    # kick of LangevinLeapfrogIntegrator::doHalfKick is that solution over
+    # dt/2, so any number of steps lands on it (terminal velocity F/(gamma m))
+    s = _free(3, mass=14.0)
+    F = np.array([[3.0, -1.5, 0.25], [0.0, 7.0, -2.0], [-4.0, 0.5, 1.0]])
+    const = lambda x: (F.copy(), np.zeros(36))
+    v0 = np.array([[0.1, 0.0, -0.2], [0.0, 0.0, 0.0], [-0.3, 0.2, 0.05]])
+    gamma, dt = 37.0, 2.0
+    for steps in (1, 7, 400):
+        _, v, _, _ = T.langevin_oracle(const, s.x.copy(), v0.copy(), s.mass, dt, steps, 0.0, gamma, seed=5)
+        g_int = gamma / (1000.0 * S.INV_TIMEFACTOR)  # 1 / internal time unit
+        t_int = steps * dt * S.INV_TIMEFACTOR
+        decay = math.exp(-g_int * t_int)
+        want = v0 * decay + F / (g_int * s.mass[:, None]) * (1.0 - decay)
+        assert np.allclose(v, want, rtol=1e-12, atol=1e-15), (steps, v, want)
