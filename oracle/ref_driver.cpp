@@ -182,6 +182,7 @@ static C1SwitchingFunction mkC1(const double *s) { return C1SwitchingFunction(s[
 static C2SwitchingFunction mkC2(const double *s) { return C2SwitchingFunction(s[1], s[2]); }
 static CnSwitchingFunction mkCn(const double *s) { return CnSwitchingFunction(s[4], s[1], s[3], s[2]); }
 static CmpCnCnSwitchingFunction mkCmp(const double *s) { return CmpCnCnSwitchingFunction(s[1], s[2], s[3], s[4]); }
+static CutoffSwitchingFunction mkCut(const double *s) { return CutoffSwitchingFunction(s[2]); }
 
 template <class Pair>
 static void run_cutoff(PTopo &topo, Vector3DBlock &pos, Vector3DBlock &f, ScalarStructure &e, double cutoff,
@@ -205,7 +206,7 @@ static void census_run(PTopo &topo, Vector3DBlock &pos, double cutoff, const S &
   run_cutoff(topo, pos, f, e, cutoff, CP(CensusForce(), sw));
 }
 
-// kinds: 1 C1, 2 C2, 3 Cn, 4 CmpCnCn (switch vector: kind, switchon, cutoff, switchoff, order)
+// kinds: 1 C1, 2 C2, 3 Cn, 4 CmpCnCn, 5 Cutoff (switch vector: kind, switchon, cutoff, switchoff, order)
 static int do_cutoff(Recs &r, PTopo &topo, Vector3DBlock &pos, Vector3DBlock &f, ScalarStructure &e) {
   int ta = geti(r, "term_a", 1), tb = geti(r, "term_b", 0);
   const double *sa = &r["sw_a"].d[0];
@@ -226,6 +227,7 @@ static int do_cutoff(Recs &r, PTopo &topo, Vector3DBlock &pos, Vector3DBlock &f,
     ONE(2, C2SwitchingFunction, mkC2)
     ONE(3, CnSwitchingFunction, mkCn)
     ONE(4, CmpCnCnSwitchingFunction, mkCmp)
+    ONE(5, CutoffSwitchingFunction, mkCut)
 #undef ONE
     return -1;
   }
@@ -244,6 +246,11 @@ static int do_cutoff(Recs &r, PTopo &topo, Vector3DBlock &pos, Vector3DBlock &f,
   TWO(2, 3, C2SwitchingFunction, CnSwitchingFunction, mkC2, mkCn)
   TWO(3, 3, CnSwitchingFunction, CnSwitchingFunction, mkCn, mkCn)
   TWO(4, 4, CmpCnCnSwitchingFunction, CmpCnCnSwitchingFunction, mkCmp, mkCmp)
+  // Cutoff switches (MODIFY = 0): not among the module's registrations, but
+  // OneAtomPairTwo's own semantics for them (OneAtomPairTwo.h:66)
+  TWO(5, 5, CutoffSwitchingFunction, CutoffSwitchingFunction, mkCut, mkCut)
+  TWO(5, 2, CutoffSwitchingFunction, C2SwitchingFunction, mkCut, mkC2)
+  TWO(2, 5, C2SwitchingFunction, CutoffSwitchingFunction, mkC2, mkCut)
 #undef TWO
   return -1;
 }

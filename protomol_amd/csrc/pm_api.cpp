@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -643,6 +644,9 @@ SysPrep prepare(const pm_ctx *c, const pm_system_desc &d) {
   p.slot_cap = p.cl_cap * pm::kCluster;
   p.sc_cap = p.cl_cap / pm::kScCl + p.ncol + 1;
   p.list_cap = estimate_list_cap(rho, rlist, p.ax, p.ay, p.cl_cap);
+  // test knob: a smaller first list capacity (entries per super-cluster), so
+  // tests can drive the overflow-and-regrow path; the lists grow as usual
+  if (const char *cap = std::getenv("PM_DEBUG_LIST_CAP")) p.list_cap = std::max(1, std::atoi(cap));
   if (c->recip) ewald_setup(c, d, p);
   return p;
 }
@@ -1158,6 +1162,10 @@ void add_energies(const pm_ctx *c, const SysHost &h, const long long *acc, doubl
 
 // Raise on device status bits; grow the pair lists that overflowed.
 // Returns the ids whose lists grew (their results must be recomputed).
+// A list overflow is handled first: a system whose list overflowed may have
+// lost pairs (or exclusion masks) in the steps after it, so a non-finite
+// position it reports at the same time is not yet an error -- the grown,
+// re-run call decides.
 std::vector<int> check_errors(pm_ctx *c, LaunchSet &ls) {
   std::vector<int> grown;
   bool any = false;
@@ -1166,6 +1174,10 @@ std::vector<int> check_errors(pm_ctx *c, LaunchSet &ls) {
     if (!e) continue;
     any = true;
     const int id = ls.ids[k];
+    if (e & pm::kErrListCap) {
+      grown.push_back(id);
+      continue;
+    }
     if (e & pm::kErrNonFinite) { pm::launch_clear_err(ls.L); throw Fail{PM_EINVAL, "system " + std::to_string(id) + ": non-finite position"}; }
     if (e & pm::kErrColumnCap) {
       pm::launch_clear_err(ls.L);
@@ -1173,7 +1185,6 @@ std::vector<int> check_errors(pm_ctx *c, LaunchSet &ls) {
                                    " atoms in one cell column"};
     }
     if (e & pm::kErrClusterCap) { pm::launch_clear_err(ls.L); throw Fail{PM_ECAPACITY, "system " + std::to_string(id) + ": cluster capacity"}; }
-    if (e & pm::kErrListCap) grown.push_back(id);
   }
   if (!any) return grown;
   pm::launch_clear_err(ls.L);
@@ -1396,13 +1407,22 @@ void md_advance(pm_ctx *c, const std::vector<int> &ids, int n_steps, double dt_f
   const std::vector<int> grown = check_errors(c, ls);
   if (!grown.empty()) {
     if (depth > 4) throw Fail{PM_ECAPACITY, "pair list capacity kept overflowing"};
-    // restore the grown systems to the start of the advance and run it again
+    // restore the grown systems to the start of the advance and run it again:
+    // positions and velocities, forces at them from a fresh list, and the
+    // control words that evaluation advanced -- the run's rebuild count, the
+    // first-kick flag and the Langevin noise counter -- so the re-run draws
+    // the noise the first attempt drew.  The steps-since-rebuild word stays
+    // at the 0 of the fresh list.
     LaunchSet &rs = get_set(c, grown);
     pm::launch_import(rs.L, pm::kXferX | pm::kXferV);
     force_eval(c, rs.L, pm::kStepForce, pol, pm::kPairForces, false, false);
-    for (int id : grown)
+    for (int id : grown) {
+      static_assert(pm::kCtlFirst == pm::kCtlCount + 1, "count and first-kick words are adjacent");
       HIPCHECK(hipMemcpyAsync(c->sys[id]->dev.ctl + pm::kCtlCount, c->sys[id]->ctl_snap + pm::kCtlCount,
                               2 * sizeof(int), hipMemcpyDeviceToDevice, c->stream));
+      HIPCHECK(hipMemcpyAsync(c->sys[id]->dev.ctl + pm::kCtlNoise, c->sys[id]->ctl_snap + pm::kCtlNoise,
+                              sizeof(int), hipMemcpyDeviceToDevice, c->stream));
+    }
     md_advance(c, grown, n_steps, dt_fs, energies_last, depth + 1);
   }
   if (depth == 0) {
@@ -1619,8 +1639,22 @@ int pm_ctx_create(const pm_force_desc *force, int32_t n_systems, const pm_system
         P.dtwo_alpha_pi = 2.0 * force->ewald_alpha / std::sqrt(M_PI);
       } else if (tb == PM_TERM_COULOMB) {
         cB2 = build_switch(force->switch_b, P.sw_b, false, 0);
-        // OneAtomPairTwo tests the larger of the two switch cutoffs (OneAtomPairTwo.h:25-27)
-        cA2 = cB2 = std::max(cA2, cB2);
+        // OneAtomPairTwo includes a pair within the larger of the two switch
+        // cutoffs (OneAtomPairTwo.h:25-27, 41-44) and applies both switches when
+        // either modifies the values (:66-76).  A CutoffSwitchingFunction
+        // (MODIFY = 0) then zeroes its own term beyond its cutoff, a step that
+        // must fall where the FP64 test puts it: that term keeps its own
+        // cutoff as its inclusion cutoff (FP64-checked like the other).
+        // Otherwise a term is included out to the larger cutoff and its
+        // switch takes it to zero.
+        const double cmax = std::max(cA2, cB2);
+        const bool modA = P.sw_a.kind != PM_SWITCH_CUTOFF, modB = P.sw_b.kind != PM_SWITCH_CUTOFF;
+        if (modA || modB) {
+          if (modA) cA2 = cmax;
+          if (modB) cB2 = cmax;
+        } else {
+          cA2 = cB2 = cmax;
+        }
       }
       P.cutA2d = cA2;
       P.cutB2d = cB2;

@@ -220,12 +220,11 @@ __device__ __forceinline__ void apply_switch(const SwitchDev &sw, float r2, floa
     E = E * S;
     break;
   }
-  case PM_SWITCH_CUTOFF: {  // CutoffSwitchingFunction.h:41-44
-    float S = r2 > sw.cut2 ? 0.0f : 1.0f;
-    F = F * S;
-    E = E * S;
+  case PM_SWITCH_CUTOFF:
+    // CutoffSwitchingFunction (CutoffSwitchingFunction.h:41-44) is 1 up to its
+    // cutoff, which is this term's inclusion cutoff: the FP64-checked lane
+    // masks have already decided it, so nothing is left to scale
     break;
-  }
   case PM_SWITCH_CN:
   case PM_SWITCH_CMPCNCN: {
     double S, dS;

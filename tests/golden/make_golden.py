@@ -49,6 +49,13 @@ CUTOFF_CASES = {
     "c2c2_two": (1, 2, (2, 6.0, 8.0, 0.0, 0.0), (2, 8.0, 10.0, 0.0, 0.0)),
     # two cutoffs through the run-time switch path: LJ C2 ending at 8 A, Coulomb C1 at 10 A
     "c2c1_two": (1, 2, (2, 6.0, 8.0, 0.0, 0.0), (1, 0.0, 10.0, 0.0, 0.0)),
+    # CutoffSwitchingFunction (does not modify values): alone, both terms with
+    # two cutoffs (no switch modifies: both out to 10 A), and beside a C2
+    # switch (both switches applied: the Cutoff term stops at its own cutoff)
+    "coul_cut": (2, 0, (5, 0.0, 10.0, 0.0, 0.0), None),
+    "cut_cut_two": (1, 2, (5, 0.0, 8.0, 0.0, 0.0), (5, 0.0, 10.0, 0.0, 0.0)),
+    "cut_c2_two": (1, 2, (5, 0.0, 8.0, 0.0, 0.0), (2, 8.0, 10.0, 0.0, 0.0)),
+    "c2_cut_two": (1, 2, (2, 6.0, 8.0, 0.0, 0.0), (5, 0.0, 10.0, 0.0, 0.0)),
 }
 
 
@@ -382,6 +389,10 @@ def main():
     w = water_fixture_system()
     if sys.argv[1:] == ["velocity"]:
         make_velocity()
+        return
+    if sys.argv[1:] == ["cutoff_switch"]:
+        for name in ("coul_cut", "cut_cut_two", "cut_c2_two", "c2_cut_two"):
+            make_cutoff(name, w, CUTOFF_CASES[name])
         return
     if sys.argv[1:] == ["ewald_recip"]:
         make_ewald_reciprocal(w)

@@ -33,6 +33,12 @@ MODELS = {
     "coul_c1": (L.TERM_COULOMB, (2, 0.0, 10.0, 0.0, 0.0), L.TERM_NONE, (0, 0, 0, 0, 0)),
     "c2c2_two": (L.TERM_LJ, (3, 6.0, 8.0, 0.0, 0.0), L.TERM_COULOMB, (3, 8.0, 10.0, 0.0, 0.0)),
     "c2c1_two": (L.TERM_LJ, (3, 6.0, 8.0, 0.0, 0.0), L.TERM_COULOMB, (2, 0.0, 10.0, 0.0, 0.0)),
+    # CutoffSwitchingFunction: alone; on both terms (no switch modifies, both
+    # out to 10 A); beside a C2 switch (the Cutoff term stops at its own cutoff)
+    "coul_cut": (L.TERM_COULOMB, (1, 0.0, 10.0, 0.0, 0.0), L.TERM_NONE, (0, 0, 0, 0, 0)),
+    "cut_cut_two": (L.TERM_LJ, (1, 0.0, 8.0, 0.0, 0.0), L.TERM_COULOMB, (1, 0.0, 10.0, 0.0, 0.0)),
+    "cut_c2_two": (L.TERM_LJ, (1, 0.0, 8.0, 0.0, 0.0), L.TERM_COULOMB, (3, 8.0, 10.0, 0.0, 0.0)),
+    "c2_cut_two": (L.TERM_LJ, (3, 6.0, 8.0, 0.0, 0.0), L.TERM_COULOMB, (1, 0.0, 10.0, 0.0, 0.0)),
 }
 
 
@@ -96,7 +102,7 @@ def test_cutoff_vs_reference_golden(model):
     check_virial(e, g["e"], npairs=cen[0])
 
 
-@pytest.mark.parametrize("model", ["c2c2", "c2c2_two", "cncn"])
+@pytest.mark.parametrize("model", ["c2c2", "c2c2_two", "cncn", "cut_c2_two"])
 def test_md_forces_vs_reference_golden(model):
     # the MD path's forces-only pair kernels (F(x0) of pm_md_set_state)
     print(f"case md forces/{model}", flush=True)
@@ -824,3 +830,41 @@ def test_md_live_list_pair_set_exact():
             x, _, _, _ = ctx.md_get_state()
             assert ctx.census(0, None) == O.census_bruteforce(w, 100.0, x=x), k
     assert rebuilds >= 2
+
+
+def test_md_list_regrow_rerun_is_bitwise_unconstrained():
+    """A pair list that overflows during an MD advance is grown and the
+    advance re-run from its start; right after the state is set, that re-run
+    is bitwise the run with room from the start -- positions, velocities,
+    forces and the Langevin noise stream.  PM_DEBUG_LIST_CAP shrinks the first
+    list capacity; the test looks for a value whose overflow comes during the
+    advance (not at the state's own list build)."""
+    import os
+    print("case md list regrow", flush=True)
+    w = S.water_box(343, 22.5, seed=8)
+    v0 = S.maxwell_velocities(w, 600.0, seed=3)
+    ta, sa, tb, sb = MODELS["c2c2"]
+    desc = force_desc(ta, sa, tb, sb, flags=L.FLAG_BONDED, skin=1.0, interval=4)
+
+    def run():
+        with L.Context(desc, [w]) as ctx:
+            ctx.set_langevin(600.0, 5.0, seed=17)
+            ctx.md_set_state(w.x, v0)
+            ctx.md_run(100, 1.0, energies=True)
+            return ctx.md_get_state(), ctx.stats()
+
+    (x0, vv0, f0, e0), _ = run()
+    hit = None
+    try:
+        for cap in range(8, 600):
+            os.environ["PM_DEBUG_LIST_CAP"] = str(cap)
+            (x1, v1, f1, e1), st = run()
+            if st["regrow_reruns"]:
+                hit = cap
+                break
+    finally:
+        os.environ.pop("PM_DEBUG_LIST_CAP", None)
+    assert hit is not None, "no capacity overflowed during the advance"
+    print(f"  overflow during the advance at list capacity {hit}", flush=True)
+    assert np.array_equal(x1, x0) and np.array_equal(v1, vv0) and np.array_equal(f1, f0)
+    assert np.array_equal(e1, e0)

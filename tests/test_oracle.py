@@ -13,7 +13,8 @@ import oracle as O
 import pmtest as T
 from protomol_amd import systems as S
 
-CUTOFF_CASES = ["c2c2", "c2c1", "cncn", "cmpcncn", "lj_c2", "coul_c1", "c2c2_two", "c2c1_two"]
+CUTOFF_CASES = ["c2c2", "c2c1", "cncn", "cmpcncn", "lj_c2", "coul_c1", "c2c2_two", "c2c1_two",
+                "coul_cut", "cut_cut_two", "cut_c2_two", "c2_cut_two"]
 
 
 def _switch_from_driver(v):
@@ -32,7 +33,7 @@ def test_cutoff_matches_reference(case):
     f, e, cen = O.nonbonded_cutoff(w, ta, swa, tb, swb, algorithm_cutoff=10.0, cellsize=5.0, virial=True)
     # same cell walk, same arithmetic: bit-exact except the Cn family's
     # pow()-based coefficients (rounding differences of order 1e-12 relative)
-    tol = 0.0 if case in ("c2c2", "c2c1", "lj_c2", "coul_c1", "c2c2_two") else 1e-11
+    tol = 1e-11 if case in ("cncn", "cmpcncn", "c2c1_two") else 0.0
     assert np.max(np.abs(f - g["f"])) <= tol * max(1.0, np.max(np.abs(g["f"])))
     assert np.allclose(e, g["e"], rtol=tol, atol=tol * np.max(np.abs(g["e"])))
     assert (int(cen[0]), int(cen[1])) == (int(g["census"][0]), int(g["census"][1]))
@@ -411,3 +412,23 @@ def test_ewald_reciprocal_fd_newton_virial():
             es.append(O.ewald_reciprocal(s2, alpha, acc)[1][0])
         w = -(es[0] - es[1]) / (2 * h)
         assert abs(w - e[O.VIRIALXX + 4 * k]) < 1e-6 * max(1.0, abs(e[O.COULOMB]))
+
+
+def test_cutoff_switch_does_not_modify():
+    # CutoffSwitchingFunction has MODIFY = 0 (CutoffSwitchingFunction.h:20-21):
+    # with Cutoff on both terms, OneAtomPairTwo applies no switch, so LJ is
+    # evaluated out to the larger cutoff (10 A), exactly as LJ and Coulomb
+    # with Cutoff switches at 10 A each; beside a C2 switch the LJ term
+    # stops at its own 8 A (the LJ-only evaluation at 8 A)
+    w = T.fixture_water()
+    cut8, cut10 = O.make_switch(O.SW_CUTOFF, 0.0, 8.0), O.make_switch(O.SW_CUTOFF, 0.0, 10.0)
+    c2 = O.make_switch(O.SW_C2, 8.0, 10.0)
+    f, e, _ = O.nonbonded_cutoff(w, O.TERM_LJ, cut8, O.TERM_COULOMB, cut10, cellsize=5.0)
+    fl, el, _ = O.nonbonded_cutoff(w, O.TERM_LJ, cut10, cellsize=5.0)
+    fc, ec, _ = O.nonbonded_cutoff(w, O.TERM_COULOMB, cut10, cellsize=5.0)
+    assert np.allclose(f, fl + fc, rtol=0, atol=1e-10) and abs(e[1] - el[1]) < 1e-9 * abs(el[1])
+    f2, e2, _ = O.nonbonded_cutoff(w, O.TERM_LJ, cut8, O.TERM_COULOMB, c2, cellsize=5.0)
+    fl8, el8, _ = O.nonbonded_cutoff(w, O.TERM_LJ, cut8, cellsize=5.0)
+    fc2, ec2, _ = O.nonbonded_cutoff(w, O.TERM_COULOMB, c2, cellsize=5.0)
+    assert np.allclose(f2, fl8 + fc2, rtol=0, atol=1e-10) and abs(e2[1] - el8[1]) < 1e-9 * abs(el8[1])
+    assert abs(el[1] - el8[1]) > 1.0  # the two readings differ
