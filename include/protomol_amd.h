@@ -205,9 +205,11 @@ int pm_pair_census(pm_ctx *ctx, int32_t system, const double *x, uint64_t *count
 
 /* Device-resident leapfrog.  Set x, v (concatenated over systems), run
  * n_steps of LeapfrogIntegrator::run at dt_fs, read back state.  energies_every
- * > 0 evaluates energies and virial at the last step (pm_md_get_state reads
- * them).  The steps between the first and the last replay a captured HIP
- * graph unless PM_FLAG_TIMING is set. */
+ * > 0 evaluates energies and virial on every energies_every-th step (an
+ * output frequency) and at the last step (pm_md_get_state reads the last);
+ * 0: forces only.  Whole groups of 10 steps replay a captured HIP graph
+ * unless PM_FLAG_TIMING is set or an energy step falls inside a group
+ * (energies_every other than 10 or >= n_steps). */
 int pm_md_set_state(pm_ctx *ctx, const double *x, const double *v);
 int pm_md_run(pm_ctx *ctx, int32_t n_steps, double dt_fs, int32_t energies_every);
 int pm_md_get_state(pm_ctx *ctx, double *x, double *v, double *f, double *e);

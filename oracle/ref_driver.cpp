@@ -44,6 +44,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <ctime>
 #include <map>
 #include <string>
 #include <vector>
@@ -115,6 +116,7 @@ static int geti(Recs &r, const char *k, int def) {
 // A force functor with the LennardJonesForce interface that records every
 // pair OneAtomPair hands to it (after the cutoff test and exclusion check).
 static unsigned long long g_count = 0, g_hash = 0;
+static bool g_census = true;  // record "census" 0: skip the census pass (timing runs)
 static unsigned long long pair_hash(unsigned lo, unsigned hi) {
   unsigned long long z = ((unsigned long long)hi << 32) | lo;
   z += 0x9E3779B97F4A7C15ull;
@@ -196,6 +198,7 @@ static void run_cutoff(PTopo &topo, Vector3DBlock &pos, Vector3DBlock &f, Scalar
 
 template <class S>
 static void census_run(PTopo &topo, Vector3DBlock &pos, double cutoff, const S &sw) {
+  if (!g_census) return;
   typedef OneAtomPair<PBC, S, CensusForce> CP;
   Vector3DBlock f(pos.size());
   f.zero();
@@ -403,17 +406,35 @@ int main(int argc, char **argv) {
   e.molecularVirial(geti(r, "molvirial", 0) != 0);
   std::vector<double> born, u0, ke, uu, gg;
   int rc = -1;
-  if (mode == 3) rc = do_gb(r, pos, f, e, born);
-  else if (mode == 5) rc = do_velocity(r, n, f, u0, ke, uu, gg);
+  // "repeat" > 1 evaluates the force that many times (the forces add up) and
+  // "seconds" reports the wall time of those evaluations alone: the
+  // reference's own CPU path, timed (single-threaded, as it runs)
+  const int repeat = geti(r, "repeat", 1);
+  g_census = geti(r, "census", 1) != 0;
+  double seconds = 0.0;
+  if (mode == 3) {
+    timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    rc = do_gb(r, pos, f, e, born);
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    seconds = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+  } else if (mode == 5) rc = do_velocity(r, n, f, u0, ke, uu, gg);
   else {
     const std::vector<double> &box = r["box"].d, &org = r["origin"].d;
     PBC bc(Vector3D(box[0], 0, 0), Vector3D(0, box[1], 0), Vector3D(0, 0, box[2]), Vector3D(org[0], org[1], org[2]));
     PTopo topo(getd(r, "csf", 1.0), ExclusionType(ExclusionType::ONE3), bc, CCM(getd(r, "cellsize", 5.0)), 16.0f);
     fill_common(topo, r, n);
     molecule_positions(topo, r, pos);
-    if (mode == 1) rc = do_cutoff(r, topo, pos, f, e);
-    else if (mode == 2) rc = do_ewald(r, topo, pos, f, e);
-    else if (mode == 4) rc = do_bonded(r, topo, pos, f, e);
+    timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    for (int k = 0; k < repeat; k++) {
+      if (mode == 1) rc = do_cutoff(r, topo, pos, f, e);
+      else if (mode == 2) rc = do_ewald(r, topo, pos, f, e);
+      else if (mode == 4) rc = do_bonded(r, topo, pos, f, e);
+      if (rc) break;
+    }
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    seconds = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
   }
   if (rc) { fprintf(stderr, "unsupported configuration\n"); return 3; }
   FILE *fp = fopen(argv[2], "wb");
@@ -429,6 +450,7 @@ int main(int argc, char **argv) {
   double hh[2] = {(double)(g_hash >> 32), (double)(g_hash & 0xffffffffull)};
   write_rec_d(fp, "census_hash", hh, 2);
   if (!born.empty()) write_rec_d(fp, "born", &born[0], (long long)born.size());
+  write_rec_d(fp, "seconds", &seconds, 1);
   if (mode == 5) {
     write_rec_d(fp, "u0", &u0[0], 1);
     write_rec_d(fp, "ke_t", &ke[0], 2);

@@ -301,8 +301,11 @@ class Context:
         vv = np.ascontiguousarray(v, np.float64)
         check(lib().pm_md_set_state(self.h, ptr(xx), ptr(vv)), self.h)
 
-    def md_run(self, n_steps: int, dt_fs: float, energies: bool = True):
-        check(lib().pm_md_run(self.h, n_steps, dt_fs, 1 if energies else 0), self.h)
+    def md_run(self, n_steps: int, dt_fs: float, energies: bool = True, every: int = 0):
+        """LeapfrogIntegrator::run of n_steps.  energies: energies and virial
+        at the last step; every > 0: also on every every-th step."""
+        cadence = every if every > 0 else (n_steps if energies else 0)
+        check(lib().pm_md_run(self.h, n_steps, dt_fs, cadence), self.h)
 
     def md_get_state(self):
         n = int(self.offsets[-1])

@@ -28,7 +28,8 @@ from . import _lib as L
 DEFAULT_GRID = {
     "skin": (1.0, 1.3),
     "cell_scale": (1.0, 1.25),
-    "pair_split": (8, 12, 16),
+    "pair_split": (8, 12),
+    "pair_waves": (2, 4, 8),
 }
 
 

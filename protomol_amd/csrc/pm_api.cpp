@@ -261,11 +261,14 @@ struct LaunchSet {
   void *mem = nullptr;
   long long *d_scal = nullptr;
   long long *h_scal = nullptr;  // pinned
-  hipGraphExec_t gexec = nullptr;  // captured MD steps for time step g_h
-  double g_h = 0.0;
-  pm::LangevinDev g_lv{};          // ... and these Langevin constants
+  // captured MD steps for time step g_h and these Langevin constants: [0]
+  // forces only, [1] with the energy pass on the last of the steps
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  double g_h[2] = {0.0, 0.0};
+  pm::LangevinDev g_lv[2]{};
   ~LaunchSet() {
-    if (gexec) (void)hipGraphExecDestroy(gexec);
+    for (hipGraphExec_t g : gexec)
+      if (g) (void)hipGraphExecDestroy(g);
     if (h_scal) (void)hipHostFree(h_scal);
     if (mem) (void)hipFree(mem);
   }
@@ -1329,19 +1332,23 @@ bool same_lv(const pm::LangevinDev &a, const pm::LangevinDev &b) {
   return a.on == b.on && a.vdt == b.vdt && a.fdt == b.fdt && a.ndt == b.ndt && a.fc == b.fc;
 }
 
-hipGraphExec_t md_graph(pm_ctx *c, LaunchSet &ls, double h, const StepPolicy &pol) {
+hipGraphExec_t md_graph(pm_ctx *c, LaunchSet &ls, double h, const StepPolicy &pol, bool energies) {
   const pm::LangevinDev lv = lv_params(c, h / kInvTimeFactor);
-  if (ls.gexec && ls.g_h == h && same_lv(ls.g_lv, lv)) return ls.gexec;
-  if (ls.gexec) {
-    HIPCHECK(hipGraphExecDestroy(ls.gexec));
-    ls.gexec = nullptr;
+  const int g_id = energies ? 1 : 0;
+  hipGraphExec_t &gx = ls.gexec[g_id];
+  if (gx && ls.g_h[g_id] == h && same_lv(ls.g_lv[g_id], lv)) return gx;
+  if (gx) {
+    HIPCHECK(hipGraphExecDestroy(gx));
+    gx = nullptr;
   }
   hipGraph_t g = nullptr;
   HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   try {
     for (int k = 0; k < kGraphSteps; ++k) {
+      const bool en = energies && k == kGraphSteps - 1;
       pm::launch_kick_drift(ls.L, h, lv);
-      force_eval(c, ls.L, pm::kStepAuto, pol, pm::kPairForces, false, false);
+      force_eval(c, ls.L, pm::kStepAuto, pol, pm::kPairForces, en, false);
+      if (en) energy_pass(c, ls.L);
     }
   } catch (...) {
     (void)hipStreamEndCapture(c->stream, &g);
@@ -1349,15 +1356,15 @@ hipGraphExec_t md_graph(pm_ctx *c, LaunchSet &ls, double h, const StepPolicy &po
     throw;
   }
   HIPCHECK(hipStreamEndCapture(c->stream, &g));
-  hipError_t e = hipGraphInstantiate(&ls.gexec, g, nullptr, nullptr, 0);
+  hipError_t e = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) {
-    ls.gexec = nullptr;
+    gx = nullptr;
     throw Fail{PM_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e)};
   }
-  ls.g_h = h;
-  ls.g_lv = lv;
-  return ls.gexec;
+  ls.g_h[g_id] = h;
+  ls.g_lv[g_id] = lv;
+  return gx;
 }
 
 // LeapfrogIntegrator::run (LeapfrogIntegrator.cpp:89-104) in three parts:
@@ -1368,7 +1375,12 @@ void md_begin(pm_ctx *c, const std::vector<int> &ids) {
   pm::launch_md_arm(get_set(c, ids).L);
 }
 
-void md_advance(pm_ctx *c, const std::vector<int> &ids, int n_steps, double dt_fs, bool energies_last, int depth = 0) {
+// every > 0: the energy pass also runs on every every-th step of the advance
+// (an output frequency); energies_last: on the last step.  The energies of
+// the last step that ran the pass are what pm_md_get_state reads when that
+// step is the last one.
+void md_advance(pm_ctx *c, const std::vector<int> &ids, int n_steps, double dt_fs, bool energies_last, int every = 0,
+                int depth = 0) {
   require_md(c, ids);
   if (n_steps < 1) return;
   if (!(dt_fs > 0.0)) throw Fail{PM_EINVAL, "dt must be > 0"};
@@ -1387,14 +1399,19 @@ void md_advance(pm_ctx *c, const std::vector<int> &ids, int n_steps, double dt_f
       HIPCHECK(hipEventRecord(c->ev0, c->stream));
       c->evs_used = 0;
     }
+    if (every >= n_steps) every = 0;  // no energy step before the last
+    auto energy_step = [&](int i) { return (energies_last && i == n_steps - 1) || (every > 0 && (i + 1) % every == 0); };
     int i = 0;
-    if (!timing)
+    // whole graphs of kGraphSteps steps while the energy steps fall where a
+    // graph has them: nowhere, or (every = kGraphSteps) on its last step
+    const bool graphs = !timing && (every == 0 || every == kGraphSteps);
+    if (graphs)
       while (n_steps - i - (energies_last ? 1 : 0) >= kGraphSteps) {
-        HIPCHECK(hipGraphLaunch(md_graph(c, ls, h, pol), c->stream));
+        HIPCHECK(hipGraphLaunch(md_graph(c, ls, h, pol, every == kGraphSteps), c->stream));
         i += kGraphSteps;
       }
     for (; i < n_steps; ++i) {
-      const bool en = energies_last && i == n_steps - 1;
+      const bool en = energy_step(i);
       pm::launch_kick_drift(ls.L, h, lv_params(c, dt_fs));
       force_eval(c, ls.L, pm::kStepAuto, pol, pm::kPairForces, en, false);
       if (en) energy_pass(c, ls.L);
@@ -1423,7 +1440,7 @@ void md_advance(pm_ctx *c, const std::vector<int> &ids, int n_steps, double dt_f
       HIPCHECK(hipMemcpyAsync(c->sys[id]->dev.ctl + pm::kCtlNoise, c->sys[id]->ctl_snap + pm::kCtlNoise,
                               sizeof(int), hipMemcpyDeviceToDevice, c->stream));
     }
-    md_advance(c, grown, n_steps, dt_fs, energies_last, depth + 1);
+    md_advance(c, grown, n_steps, dt_fs, energies_last, every, depth + 1);
   }
   if (depth == 0) {
     HIPCHECK(hipStreamSynchronize(c->stream));
@@ -1824,7 +1841,7 @@ int pm_md_run(pm_ctx *c, int32_t n_steps, double dt_fs, int32_t energies_every) 
     if (n_steps < 1) return;
     if (!(dt_fs > 0.0)) throw Fail{PM_EINVAL, "dt must be > 0"};
     md_begin(c, ids);
-    md_advance(c, ids, n_steps, dt_fs, energies_every > 0);
+    md_advance(c, ids, n_steps, dt_fs, energies_every > 0, std::max(0, energies_every));
     md_end(c, ids, dt_fs);
     HIPCHECK(hipStreamSynchronize(c->stream));
   });

@@ -314,12 +314,20 @@ _ELEMENT_MASS = {"C": 12.011, "N": 14.007, "O": 15.999, "H": 1.008, "S": 32.06, 
 _ELEMENT_GBS = {"H": 0.85, "C": 0.72, "N": 0.79, "O": 0.85, "P": 0.86, "S": 0.96}
 
 
+CHAIN_KB = 310.0  # kcal/mol/A^2, the CHARMM aliphatic C-C bond constant
+CHAIN_KA = 50.0   # kcal/mol/rad^2
+
+
 def gb_chain(n_atoms: int, seed: int, bond: float = 1.53, min_dist: float = 2.5) -> System:
     """Bead polymer in vacuum: seeded self-avoiding random walk of ``bond``-A
     steps (non-bonded beads kept at least ``min_dist`` apart), charges
     N(0, 0.3) e shifted to net zero, LJ sigma U[2.5, 4.0] A, epsilon
     U[0.05, 0.2] kcal/mol (per-atom GROMACS mixing), intrinsic Born radii
-    U[1.2, 2.0] A, 1-2 and 1-3 exclusions along the chain."""
+    U[1.2, 2.0] A, 1-2 and 1-3 exclusions along the chain.  The chain's
+    covalent geometry holds it together in MD: harmonic bonds (310 kcal/mol/A^2,
+    rest length ``bond``) and angles (50 kcal/mol/rad^2, each at its start
+    value, so the start is unstrained); without them the excluded bonded
+    neighbours, which feel no Lennard-Jones repulsion, drift onto each other."""
     rng = np.random.default_rng(seed)
     pos = np.zeros((n_atoms, 3))
     cell = min_dist
@@ -374,9 +382,14 @@ def gb_chain(n_atoms: int, seed: int, bond: float = 1.53, min_dist: float = 2.5)
     angles = np.stack([idx[:-2], idx[1:-1], idx[2:]], 1)
     ex = np.concatenate([np.concatenate([bonds, np.full((len(bonds), 1), EXCL_FULL)], 1),
                          np.concatenate([angles[:, [0, 2]], np.full((len(angles), 1), EXCL_FULL)], 1)]).astype(np.int32)
+    u = pos[angles[:, 0]] - pos[angles[:, 1]]
+    w = pos[angles[:, 2]] - pos[angles[:, 1]]
+    theta0 = np.arctan2(np.linalg.norm(np.cross(u, w), axis=1), np.sum(u * w, axis=1))
     return System(x=pos, charge=q * SQRT_COULOMB, mass=mass, molecule=np.zeros(n_atoms, np.int32),
                   box=np.zeros(3), origin=np.zeros(3), periodic=False, lj_sigma=sigma, lj_epsilon=eps,
                   lj_scaling=1.0, exclusions=ex, bonds=bonds.astype(np.int32),
+                  bond_k=np.full(len(bonds), CHAIN_KB), bond_r0=np.full(len(bonds), bond),
+                  angles=angles.astype(np.int32), angle_k=np.full(len(angles), CHAIN_KA), angle_t0=theta0,
                   gb_radius=rho, gb_scaling=gbs, name=f"gbchain{n_atoms}")
 
 

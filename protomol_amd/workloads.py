@@ -15,7 +15,8 @@ full-size parity tests so both run the same inputs.
   served      water boxes of 1,000-32,000 molecules (log-uniform), the
               water32k force model, as a request stream's systems
   chains32    32 GB/OBC bead chains of 5,000 atoms, all pairs (Born radii,
-              partial sums, GB + LJ + Coulomb)
+              partial sums, GB + LJ + Coulomb) plus the chains' bonds and
+              angles
 
 Water boxes are lattice starts with seeded orientations whose H-H contacts
 under 2 A are relieved (systems.water_box, min_contact=2.0).
@@ -121,9 +122,11 @@ def desc_water32k(skin=1.2, rebuild=10, flags=0, **shape) -> L.ForceDesc:
 
 
 def desc_chains(flags=0) -> L.ForceDesc:
-    """GB/OBC (offset 0.09, OBC2 alpha 1, beta 0.8, gamma 4.85, dielectrics 1 / 78.3) + LJ + Coulomb."""
+    """GB/OBC (offset 0.09, OBC2 alpha 1, beta 0.8, gamma 4.85, dielectrics
+    1 / 78.3) + LJ + Coulomb, all pairs, and the chains' harmonic bonds and
+    angles."""
     d = L.ForceDesc()
     d.gb_offset, d.gb_alpha, d.gb_beta, d.gb_gamma = 0.09, 1.0, 0.8, 4.85
     d.gb_solute_dielec, d.gb_solvent_dielec = 1.0, 78.3
-    d.flags = L.FLAG_GB | flags
+    d.flags = L.FLAG_GB | L.FLAG_BONDED | flags
     return d

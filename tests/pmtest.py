@@ -183,62 +183,3 @@ def vacuum_lj_coulomb(c, virial=False, molvirial=False, x=None):
     e[7:16] += np.einsum("pa,pb->ab", fij[atomic], d[i, j][atomic]).reshape(-1)
     return f, e
 
-
-def vacuum_lj_coulomb_blocked(c, x=None, block=512):
-    """The same sum as vacuum_lj_coulomb (forces and the LJ / Coulomb slots,
-    no virial) for large chains: every pair in row blocks, then the excluded
-    pairs taken back out (FULL) or rescaled (MODIFIED).  FP64."""
-    x = np.asarray(c.x if x is None else x, np.float64)
-    n = c.n
-    q = np.asarray(c.charge, np.float64)
-    sh, se = 0.5 * np.asarray(c.lj_sigma), np.sqrt(np.asarray(c.lj_epsilon))
-    f = np.zeros((n, 3))
-    elj = ecou = 0.0
-
-    def terms(i, j, d):
-        rr = 1.0 / np.sum(d * d, axis=-1)
-        sg = sh[i] + sh[j]
-        ep = se[i] * se[j]
-        s6 = sg ** 6
-        B = 4.0 * ep * s6
-        A = B * s6
-        r6 = rr ** 3
-        el = A * r6 * r6 - B * r6
-        fl = (12 * A * r6 * r6 - 6 * B * r6) * rr
-        ec = q[i] * q[j] * np.sqrt(rr)
-        return el, fl, ec, ec * rr
-
-    for b0 in range(0, n, block):
-        i = np.arange(b0, min(n, b0 + block))[:, None]
-        j = np.arange(n)[None, :]
-        d = x[None, :, :] - x[i[:, 0], None, :]
-        mask = j > i
-        el, fl, ec, fc = terms(i, j, np.where(mask[..., None], d, 1.0))
-        el, ec = np.where(mask, el, 0.0), np.where(mask, ec, 0.0)
-        fs = np.where(mask, fl + fc, 0.0)
-        fij = d * fs[..., None]
-        f[i[:, 0]] -= fij.sum(axis=1)
-        f += fij.sum(axis=0)
-        elj += el.sum()
-        ecou += ec.sum()
-    mol = np.asarray(c.molecule)
-    last = {}
-    for a, b, cls in c.exclusions:
-        if mol[a] == mol[b] and a != b:
-            last[(min(a, b), max(a, b))] = cls
-    if last:
-        pairs = np.array([(a, b, k) for (a, b), k in last.items()])
-        i, j, cls = pairs[:, 0], pairs[:, 1], pairs[:, 2]
-        d = x[j] - x[i]
-        el, fl, ec, fc = terms(i, j, d)
-        # FULL: remove; MODIFIED: scale LJ by lj_scaling and Coulomb by coulomb_scaling
-        wl = np.where(cls == 1, -1.0, c.lj_scaling - 1.0)
-        wc = np.where(cls == 1, -1.0, c.coulomb_scaling - 1.0)
-        fij = d * (wl * fl + wc * fc)[:, None]
-        np.add.at(f, i, -fij)
-        np.add.at(f, j, fij)
-        elj += float(np.sum(wl * el))
-        ecou += float(np.sum(wc * ec))
-    e = np.zeros(36)
-    e[0], e[1] = ecou, elj
-    return f, e

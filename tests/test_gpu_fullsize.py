@@ -281,7 +281,28 @@ def test_chains32_batch_vs_alone_bitwise_and_oracle():
     npairs = 3 * n * (n - 1) // 2
     for k in (0, len(chains) - 1):
         fo, eo, _ = O.gb_evaluate(chains[k])
-        fl, el = T.vacuum_lj_coulomb_blocked(chains[k])
-        check_forces(f"chain {k}", fb[k * n:(k + 1) * n], fo + fl)
+        fl, el = O.vacuum_lj_coulomb(chains[k])
+        fbd, ebd = O.bonded(chains[k])
+        check_forces(f"chain {k}", fb[k * n:(k + 1) * n], fo + fl + fbd)
         assert abs(eb[k][0] - (eo[0] + el[0])) <= 1e-5 * abs(eo[0] + el[0]) + 1e-7 * npairs
         assert abs(eb[k][1] - el[1]) <= 1e-5 * max(1.0, abs(el[1])) + 1e-7 * npairs
+        assert abs(eb[k][2] - ebd[2]) <= 1e-9 * max(1.0, abs(ebd[2])) and abs(eb[k][3] - ebd[3]) <= 1e-6
+
+
+def test_chains_md_200_steps_forces_vs_oracle():
+    """The forces the step loop holds after 200 steps at 1 fs (the benchmarked
+    run) against FP64 at those positions: the chain stays a chain (no
+    overlapping beads), where FP32 pair arithmetic is accurate."""
+    print("case chains md state (200 steps)", flush=True)
+    c = S.gb_chain(5000, 3000)
+    v0 = S.maxwell_velocities(c, 300.0, seed=4000)
+    with L.Context(W.desc_chains(), [c]) as ctx:
+        ctx.md_set_state(c.x, v0)
+        ctx.md_run(200, 1.0, energies=True)
+        x, v, F, E = ctx.md_get_state()
+    fo, _, _ = O.gb_evaluate(c, x=x)
+    fl, _ = O.vacuum_lj_coulomb(c, x=x)
+    fbd, _ = O.bonded(c, x=x)
+    d = np.linalg.norm(x[1:] - x[:-1], axis=1)
+    print(f"  bond lengths after 200 steps: {d.min():.3f}-{d.max():.3f} A", flush=True)
+    check_forces("chain after 200 steps", F, fo + fl + fbd)

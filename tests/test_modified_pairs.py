@@ -139,6 +139,10 @@ def test_oracle_gb_modified_matches_reference():
     fl, el = T.vacuum_lj_coulomb(c)
     assert close(f + fl, g["f"], rel=1e-11)
     assert close(e + el, g["e"], rel=1e-11)
+    # the blocked all-pairs sum (MODIFIED pairs rescaled after the fact)
+    fb, eb = O.vacuum_lj_coulomb(c, block=29)
+    assert close(f + fb, g["f"], rel=1e-11)
+    assert close(e[:2] + eb[:2], g["e"][:2], rel=1e-11)
 
 
 def gb_chain_table(n, seed):

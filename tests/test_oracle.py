@@ -78,6 +78,10 @@ def test_gb_matches_reference():
     assert np.array_equal(born, g["born"])
     assert np.max(np.abs(f + fl - g["f"])) < 1e-9
     assert abs(e[0] + el[0] - g["e"][0]) < 1e-9 and abs(el[1] - g["e"][1]) < 1e-9
+    # the blocked all-pairs sum the large-chain checks use, on the same fixture
+    fb, eb = O.vacuum_lj_coulomb(c, block=37)
+    assert np.max(np.abs(f + fb - g["f"])) < 1e-9
+    assert abs(e[0] + eb[0] - g["e"][0]) < 1e-9 and abs(eb[1] - g["e"][1]) < 1e-9
 
 
 def _vacuum_lj_coulomb(c):
