@@ -235,6 +235,21 @@ int pm_md_get_state_systems(pm_ctx *ctx, int32_t n, const int32_t *ids, double *
 int pm_md_set_langevin(pm_ctx *ctx, double temperature, double gamma_per_ps, uint32_t seed);
 int pm_system_set_seed(pm_ctx *ctx, int32_t id, uint32_t seed);
 
+/* No counterpart in the reference: pm_evaluate for callers that cannot
+ * allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc).  One system, no context: the force description and system as
+ * for pm_ctx_create, positions x (3n AoS FP64, host); ADDS forces into f and
+ * energies into e[36] as pm_evaluate does.  Every device array -- the
+ * system's tables, cell and pair lists, the launch table -- is carved out of
+ * a static device array of 64 MiB that each GPU holds until the process ends
+ * (about 0.8 KB per atom of water at a 10 A cutoff: some 75,000 atoms fit); a
+ * system that needs more returns PM_ECAPACITY.  No reciprocal-space term
+ * (PM_FLAG_EWALD_RECIPROCAL returns PM_EINVAL).  Runs on the null stream of
+ * the current device and returns when done.  All calls on one GPU share its
+ * array: calls that overlap on one GPU race (serialise them). */
+int pm_evaluate_static(const pm_force_desc *force, const pm_system_desc *system, const double *x, double *f,
+                       double *e);
+
 /* The velocity start of a run, as ProtoMol draws it (host only, no device
  * work): randomNumber(seed, randomtype) seeds the generator and returns one
  * number (MainModule::configure, module/MainModule.cpp:88-102), then

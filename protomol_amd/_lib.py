@@ -25,7 +25,7 @@ EXPORTED = ("pm_abi_version", "pm_device_info", "pm_ctx_create", "pm_ctx_destroy
             "pm_evaluate", "pm_evaluate_batch", "pm_evaluate_systems", "pm_pair_census", "pm_md_set_state",
             "pm_md_run", "pm_md_get_state", "pm_md_set_state_systems", "pm_md_begin", "pm_md_advance",
             "pm_md_end", "pm_md_get_state_systems", "pm_md_set_langevin", "pm_system_set_seed", "pm_stats",
-            "pm_random_velocity")
+            "pm_random_velocity", "pm_evaluate_static")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -100,6 +100,7 @@ def lib():
         L.pm_md_set_langevin.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_uint32]
         L.pm_system_set_seed.argtypes = [C.c_void_p, C.c_int32, C.c_uint32]
         L.pm_random_velocity.argtypes = [C.c_int32, _dp, C.c_double, C.c_uint32, C.c_int32, _dp]
+        L.pm_evaluate_static.argtypes = [C.POINTER(ForceDesc), C.POINTER(SystemDesc), _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -129,6 +130,15 @@ def random_velocity(mass, temperature: float, seed: int, randomtype: int = 0) ->
     check(lib().pm_random_velocity(len(m), ptr(m), float(temperature), int(seed) & 0xFFFFFFFF, int(randomtype),
                                    ptr(v)))
     return v
+
+
+def evaluate_static(force: ForceDesc, system, x, f, e):
+    """pm_evaluate_static: one evaluation with no context and no device
+    allocation (ADDS into f and e)."""
+    keep = _Keep(system, bool(force.flags & FLAG_BONDED))
+    xx = np.ascontiguousarray(x, np.float64)
+    assert f.dtype == np.float64 and f.flags.c_contiguous and e.dtype == np.float64
+    check(lib().pm_evaluate_static(C.byref(force), C.byref(keep.desc), ptr(xx), ptr(f), ptr(e)))
 
 
 def switch(kind=SWITCH_NONE, switchon=0.0, cutoff=0.0, switchoff=0.0, order=0.0) -> SwitchDesc:

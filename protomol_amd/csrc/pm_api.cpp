@@ -12,6 +12,7 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -771,6 +772,16 @@ size_t lists_bytes(int sc_cap, int list_cap) {
   return cv.off;
 }
 
+// the pair lists of capacity h.list_cap at mem (lists_bytes of them)
+void place_lists(SysHost &h, void *mem) {
+  Carver cv;
+  cv.base = static_cast<char *>(mem);
+  h.dev.list_cap = h.list_cap;
+  h.dev.mask_cap = mask_cap_for(h.list_cap);
+  h.dev.list = cv.take<uint4>(2 * (size_t)h.sc_cap * h.list_cap);
+  h.dev.list_mask = cv.take<unsigned long long>((size_t)h.sc_cap * h.dev.mask_cap * 2 * pm::kScCl);
+}
+
 void alloc_lists(pm_ctx *c, SysHost &h) {
   if (h.cl_cap == 0) return;
   const size_t bytes = lists_bytes(h.sc_cap, h.list_cap);
@@ -778,12 +789,7 @@ void alloc_lists(pm_ctx *c, SysHost &h) {
   dev_free(c, h.lists, h.lists_bytes);
   h.lists = mem;
   h.lists_bytes = bytes;
-  Carver cv;
-  cv.base = static_cast<char *>(mem);
-  h.dev.list_cap = h.list_cap;
-  h.dev.mask_cap = mask_cap_for(h.list_cap);
-  h.dev.list = cv.take<uint4>(2 * (size_t)h.sc_cap * h.list_cap);
-  h.dev.list_mask = cv.take<unsigned long long>((size_t)h.sc_cap * h.dev.mask_cap * 2 * pm::kScCl);
+  place_lists(h, mem);
 }
 
 uint64_t footprint(const pm_ctx *c, const pm_system_desc &d) {
@@ -801,11 +807,9 @@ void upload(T *dst, const T *src, size_t n) {
   if (n && src) HIPCHECK(hipMemcpy(dst, src, n * sizeof(T), hipMemcpyHostToDevice));
 }
 
-// add a system; returns its id
-int add_system(pm_ctx *c, const pm_system_desc &d) {
-  SysPrep p = prepare(c, d);
-  auto hp = std::make_unique<SysHost>();
-  SysHost &h = *hp;
+// The host record of a system and the scalars of its device record, from
+// its prepared tables (no device memory yet).
+void init_system(const pm_ctx *c, const pm_system_desc &d, const SysPrep &p, SysHost &h) {
   SysDev &S = h.dev;
   const int n = p.n;
   h.n = n;
@@ -876,63 +880,80 @@ int add_system(pm_ctx *c, const pm_system_desc &d) {
     h.ew.nfb = S.ew_nfb;
     h.ew.lds_bytes = 0;
   }
-
   Carver dry;
   SysDev tmp{};
   int *snap = nullptr;
   h.block_bytes = carve(c, p, d, dry, tmp, &snap);
+}
+
+// Lay the system's arrays out in h.block, clear them (a fill kernel when
+// `fill`, else a memset) and upload its tables.
+void fill_block(pm_ctx *c, const pm_system_desc &d, const SysPrep &p, SysHost &h, bool fill) {
+  SysDev &S = h.dev;
+  const int n = p.n;
+  Carver cv;
+  cv.base = static_cast<char *>(h.block);
+  carve(c, p, d, cv, S, &h.ctl_snap);
+  // (the fill covers block_bytes rounded up to 8; the static layout keeps the
+  // block's end padded to 256 bytes)
+  if (fill) pm::launch_fill_zero(h.block, (h.block_bytes + 7) / 8 * 8, c->stream);
+  else HIPCHECK(hipMemset(h.block, 0, h.block_bytes));
+  upload(const_cast<int *>(S.excl_start), p.excl_start.data(), p.excl_start.size());
+  upload(const_cast<int *>(S.excl_partner), p.excl_partner.data(), p.excl_partner.size());
+  upload(const_cast<int *>(S.excl_class), p.excl_class.data(), p.excl_class.size());
+  upload(const_cast<unsigned long long *>(S.excl_win), p.excl_win.data(), p.excl_win.size());
+  upload(const_cast<int *>(S.corr), p.corr.data(), p.corr.size());
+  upload(const_cast<int *>(S.loc_start), p.loc_start.data(), p.loc_start.size());
+  upload(const_cast<int *>(S.loc_item), p.loc_item.data(), p.loc_item.size());
+  upload(const_cast<double *>(S.charge), d.scaled_charge, n);
+  upload(const_cast<double *>(S.inv_mass), p.invm.data(), n);
+  upload(const_cast<int *>(S.molecule), d.molecule, n);
+  if (c->molvir) {
+    upload(const_cast<int *>(S.mol_start), p.mol_start.data(), p.mol_start.size());
+    upload(const_cast<int *>(S.mol_atoms), p.mol_atoms.data(), p.mol_atoms.size());
+    upload(const_cast<double *>(S.mass), d.mass, n);
+  }
+  if (d.lj_mode == PM_LJ_PERATOM) {
+    upload(const_cast<float *>(S.sig_half), p.sig_half.data(), n);
+    upload(const_cast<float *>(S.eps_sqrt), p.eps_sqrt.data(), n);
+  } else {
+    upload(const_cast<int *>(S.type), d.type, n);
+    upload(const_cast<float4 *>(S.lj_table), p.lj.data(), p.lj.size());
+  }
+  upload(const_cast<int *>(S.bonds), d.bonds, 2 * (size_t)p.n_bonds);
+  upload(const_cast<double *>(S.bond_k), d.bond_k, p.n_bonds);
+  upload(const_cast<double *>(S.bond_r0), d.bond_r0, p.n_bonds);
+  upload(const_cast<int *>(S.angles), d.angles, 3 * (size_t)p.n_angles);
+  upload(const_cast<double *>(S.angle_k), d.angle_k, p.n_angles);
+  upload(const_cast<double *>(S.angle_t0), d.angle_theta0, p.n_angles);
+  if (c->gb) {
+    upload(const_cast<double *>(S.gb_rho), d.gb_radius, n);
+    upload(const_cast<double *>(S.gb_S), d.gb_scaling, n);
+    std::vector<int> ident(n);
+    for (int i = 0; i < n; ++i) ident[i] = i;
+    upload(S.slot, ident.data(), n);
+    S.ncx = S.ncy = S.ncol = 1;
+  }
+  if (c->recip) {
+    upload(const_cast<int4 *>(S.ew_rows), p.ew_rows.data(), p.ew_rows.size());
+    upload(const_cast<double4 *>(S.ew_kd), p.ew_kd.data(), p.ew_kd.size());
+    upload(const_cast<int4 *>(S.ew_tiles), p.ew_tiles.data(), p.ew_tiles.size());
+    upload(const_cast<short2 *>(S.ew_pts), p.ew_pts.data(), p.ew_pts.size());
+    upload(const_cast<int *>(S.ew_kmap), p.ew_kmap.data(), p.ew_kmap.size());
+    upload(const_cast<int *>(S.ew_fb_rows), p.ew_fb_rows.data(), p.ew_fb_rows.size());
+  }
+}
+
+// add a system; returns its id
+int add_system(pm_ctx *c, const pm_system_desc &d) {
+  SysPrep p = prepare(c, d);
+  auto hp = std::make_unique<SysHost>();
+  SysHost &h = *hp;
+  init_system(c, d, p, h);
   h.block = dev_alloc(c, h.block_bytes);
   try {
-    Carver cv;
-    cv.base = static_cast<char *>(h.block);
-    carve(c, p, d, cv, S, &h.ctl_snap);
-    HIPCHECK(hipMemset(h.block, 0, h.block_bytes));
-    upload(const_cast<int *>(S.excl_start), p.excl_start.data(), p.excl_start.size());
-    upload(const_cast<int *>(S.excl_partner), p.excl_partner.data(), p.excl_partner.size());
-    upload(const_cast<int *>(S.excl_class), p.excl_class.data(), p.excl_class.size());
-    upload(const_cast<unsigned long long *>(S.excl_win), p.excl_win.data(), p.excl_win.size());
-    upload(const_cast<int *>(S.corr), p.corr.data(), p.corr.size());
-    upload(const_cast<int *>(S.loc_start), p.loc_start.data(), p.loc_start.size());
-    upload(const_cast<int *>(S.loc_item), p.loc_item.data(), p.loc_item.size());
-    upload(const_cast<double *>(S.charge), d.scaled_charge, n);
-    upload(const_cast<double *>(S.inv_mass), p.invm.data(), n);
-    upload(const_cast<int *>(S.molecule), d.molecule, n);
-    if (c->molvir) {
-      upload(const_cast<int *>(S.mol_start), p.mol_start.data(), p.mol_start.size());
-      upload(const_cast<int *>(S.mol_atoms), p.mol_atoms.data(), p.mol_atoms.size());
-      upload(const_cast<double *>(S.mass), d.mass, n);
-    }
-    if (d.lj_mode == PM_LJ_PERATOM) {
-      upload(const_cast<float *>(S.sig_half), p.sig_half.data(), n);
-      upload(const_cast<float *>(S.eps_sqrt), p.eps_sqrt.data(), n);
-    } else {
-      upload(const_cast<int *>(S.type), d.type, n);
-      upload(const_cast<float4 *>(S.lj_table), p.lj.data(), p.lj.size());
-    }
-    upload(const_cast<int *>(S.bonds), d.bonds, 2 * (size_t)p.n_bonds);
-    upload(const_cast<double *>(S.bond_k), d.bond_k, p.n_bonds);
-    upload(const_cast<double *>(S.bond_r0), d.bond_r0, p.n_bonds);
-    upload(const_cast<int *>(S.angles), d.angles, 3 * (size_t)p.n_angles);
-    upload(const_cast<double *>(S.angle_k), d.angle_k, p.n_angles);
-    upload(const_cast<double *>(S.angle_t0), d.angle_theta0, p.n_angles);
-    if (c->gb) {
-      upload(const_cast<double *>(S.gb_rho), d.gb_radius, n);
-      upload(const_cast<double *>(S.gb_S), d.gb_scaling, n);
-      std::vector<int> ident(n);
-      for (int i = 0; i < n; ++i) ident[i] = i;
-      upload(S.slot, ident.data(), n);
-      S.ncx = S.ncy = S.ncol = 1;
-    } else {
-      alloc_lists(c, h);
-    }
-    if (c->recip) {
-      upload(const_cast<int4 *>(S.ew_rows), p.ew_rows.data(), p.ew_rows.size());
-      upload(const_cast<double4 *>(S.ew_kd), p.ew_kd.data(), p.ew_kd.size());
-      upload(const_cast<int4 *>(S.ew_tiles), p.ew_tiles.data(), p.ew_tiles.size());
-      upload(const_cast<short2 *>(S.ew_pts), p.ew_pts.data(), p.ew_pts.size());
-      upload(const_cast<int *>(S.ew_kmap), p.ew_kmap.data(), p.ew_kmap.size());
-      upload(const_cast<int *>(S.ew_fb_rows), p.ew_fb_rows.data(), p.ew_fb_rows.size());
-    }
+    fill_block(c, d, p, h, false);
+    if (!c->gb) alloc_lists(c, h);
   } catch (...) {
     dev_free(c, h.lists, h.lists_bytes);
     dev_free(c, h.block, h.block_bytes);
@@ -1494,6 +1515,224 @@ void get_state_ids(pm_ctx *c, const std::vector<int> &ids, double *x, double *v,
   }
 }
 
+// The force description's device record and the context's cutoff and list
+// radii, validated as ProtoMol's make() of each force does; n_systems /
+// systems (may be 0) bound the pair-list buffer by the smallest box.
+void setup_force(pm_ctx *c, const pm_force_desc *force, int32_t n_systems, const pm_system_desc *systems) {
+  ForceDev &P = c->P;
+  std::memset(&P, 0, sizeof P);
+  P.flags = force->flags;
+  P.pair_waves = force->pair_waves ? force->pair_waves : 4;
+  P.pair_split = force->pair_split ? force->pair_split : pm::kPairSplitDefault;
+  if (P.pair_split < 1 || P.pair_split > 64) throw Fail{PM_EINVAL, "pair_split must be 1 .. 64"};
+  if (P.pair_waves != 1 && P.pair_waves != 2 && P.pair_waves != 4 && P.pair_waves != 8)
+    throw Fail{PM_EINVAL, "pair_waves must be 1, 2, 4 or 8"};
+  if (force->cell_scale != 0.0 && !(force->cell_scale >= 0.6 && force->cell_scale <= 2.0))
+    throw Fail{PM_EINVAL, "cell_scale must be 0 (default) or within 0.6 .. 2"};
+  c->gb = (force->flags & PM_FLAG_GB) != 0;
+  c->recip = (force->flags & PM_FLAG_EWALD_RECIPROCAL) != 0;
+  c->molvir = (force->flags & PM_FLAG_MOLVIRIAL) != 0;
+  P.virial = (force->flags & PM_FLAG_VIRIAL) ? 1 : 0;
+  P.molvirial = c->molvir ? 1 : 0;
+  if ((force->flags & PM_FLAG_GB_ACE) && !c->gb) throw Fail{PM_EINVAL, "PM_FLAG_GB_ACE needs PM_FLAG_GB"};
+  if (c->recip && (c->gb || force->term_b != PM_TERM_EWALD_REAL))
+    throw Fail{PM_EINVAL, "PM_FLAG_EWALD_RECIPROCAL needs term_b = PM_TERM_EWALD_REAL (and no GB)"};
+  if (c->gb) {
+    pm::gb_setup_force(P, *force);
+  } else {
+    const int ta = force->term_a, tb = force->term_b;
+    const bool ok = (ta == PM_TERM_LJ && (tb == PM_TERM_NONE || tb == PM_TERM_COULOMB || tb == PM_TERM_EWALD_REAL)) ||
+                    (ta == PM_TERM_COULOMB && tb == PM_TERM_NONE) || (ta == PM_TERM_NONE && tb == PM_TERM_EWALD_REAL);
+    if (!ok) throw Fail{PM_EINVAL, "unsupported term combination"};
+    P.term_a = ta;
+    P.term_b = tb;
+    double cA2 = 0, cB2 = 0;
+    if (ta != PM_TERM_NONE) cA2 = build_switch(force->switch_a, P.sw_a, false, 0);
+    if (tb == PM_TERM_EWALD_REAL) {
+      // NonbondedFullEwaldSystemForce::initialize (NonbondedFullEwaldSystemForce.h:610-619)
+      if (!(force->ewald_alpha > 0 && force->ewald_accuracy > 0 && force->ewald_accuracy < 1))
+        throw Fail{PM_EINVAL, "Ewald needs alpha > 0 and 0 < accuracy < 1"};
+      double p = -std::log(force->ewald_accuracy);
+      double rc = std::sqrt(p) / force->ewald_alpha;
+      cB2 = build_switch(force->switch_b, P.sw_b, true, rc);
+      cB2 = rc * rc;
+      P.alpha = (float)force->ewald_alpha;
+      P.alpha_p = (float)(0.3275911 * force->ewald_alpha);
+      P.alpha2_log2e = (float)(-force->ewald_alpha * force->ewald_alpha / std::log(2.0));
+      P.two_alpha_pi = (float)(2.0 * force->ewald_alpha / std::sqrt(M_PI));
+      P.dalpha = force->ewald_alpha;
+      P.dtwo_alpha_pi = 2.0 * force->ewald_alpha / std::sqrt(M_PI);
+    } else if (tb == PM_TERM_COULOMB) {
+      cB2 = build_switch(force->switch_b, P.sw_b, false, 0);
+      // OneAtomPairTwo includes a pair within the larger of the two switch
+      // cutoffs (OneAtomPairTwo.h:25-27, 41-44) and applies both switches when
+      // either modifies the values (:66-76).  A CutoffSwitchingFunction
+      // (MODIFY = 0) then zeroes its own term beyond its cutoff, a step that
+      // must fall where the FP64 test puts it: that term keeps its own
+      // cutoff as its inclusion cutoff (FP64-checked like the other).
+      // Otherwise a term is included out to the larger cutoff and its
+      // switch takes it to zero.
+      const double cmax = std::max(cA2, cB2);
+      const bool modA = P.sw_a.kind != PM_SWITCH_CUTOFF, modB = P.sw_b.kind != PM_SWITCH_CUTOFF;
+      if (modA || modB) {
+        if (modA) cA2 = cmax;
+        if (modB) cB2 = cmax;
+      } else {
+        cA2 = cB2 = cmax;
+      }
+    }
+    P.cutA2d = cA2;
+    P.cutB2d = cB2;
+    P.cutA2 = (float)cA2;
+    P.cutB2 = (float)cB2;
+    const double cmax = std::max(cA2, cB2);
+    P.guard = (float)(2e-3 + 1e-5 * cmax);
+    c->rc = std::sqrt(cmax);
+    c->skin_md = force->skin > 0 ? force->skin : 2.0;
+    c->skin_eval = 0.1;
+    if (force->rebuild_interval < 0) throw Fail{PM_EINVAL, "rebuild_interval must be >= 0"};
+    // the pair list radius rc + skin must stay below half the smallest box
+    // edge (one image per pair); a larger requested skin is reduced to fit
+    // the systems given here
+    if (n_systems > 0) {
+      double minL = 1e300;
+      for (int s = 0; s < n_systems; ++s)
+        for (int k = 0; k < 3; ++k) minL = std::min(minL, systems[s].box[k]);
+      const double room = 0.5 * minL - c->rc - 0.01;
+      if (room < 0.05)
+        throw Fail{PM_EINVAL, "box edge " + std::to_string(minL) + " leaves no room for a pair list beyond rc " +
+                                  std::to_string(c->rc) + " (needs box > 2 rc)"};
+      c->skin_md = std::min(c->skin_md, room);
+      c->skin_eval = std::min(c->skin_eval, room);
+    }
+  }
+}
+
+// ------------------------------------------------ allocation-free evaluation
+
+// The device memory of pm_evaluate_static: one file-scope array.  Each GPU
+// the library runs on holds its own copy until the process ends.  Host code
+// reaches a device's copy only through the address hipGetSymbolAddress gives
+// on that device (scratch_ptr); the array's name on the host is a stand-in,
+// not a device address.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[kMaxDevices] = {};
+
+void *scratch_ptr() {
+  int dev = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw Fail{PM_EDEVICE, "device id out of range"};
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw Fail{PM_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e)};
+    g_scratch[dev] = p;
+  }
+  return g_scratch[dev];
+}
+
+// SystemForce::evaluate of one system with every device array -- the
+// system's block, its pair lists, the launch table and the packed scalars --
+// carved out of the device's scratch array: no context, no device or pinned
+// host allocation, no events; the null stream; synchronous on return.
+void evaluate_static(const pm_force_desc &force, const pm_system_desc &d, const double *x, double *f, double *e) {
+  pm_ctx c;
+  c.desc = force;
+  c.desc.flags &= ~PM_FLAG_TIMING;
+  setup_force(&c, &c.desc, 1, &d);
+  if (c.recip) throw Fail{PM_EINVAL, "pm_evaluate_static: no reciprocal-space term (use a context)"};
+  char *const base = static_cast<char *>(scratch_ptr());
+  const SysPrep p = prepare(&c, d);
+  SysHost h;
+  init_system(&c, d, p, h);
+  for (int attempt = 0;; ++attempt) {
+    // [system block | pair lists | SysDev | 10 prefix arrays of 2 ints | scalars]
+    constexpr int kPre = 10;
+    const size_t o_lists = align_up(h.block_bytes);
+    const size_t lb = c.gb ? 0 : lists_bytes(h.sc_cap, h.list_cap);
+    const size_t o_tab = align_up(o_lists + lb);
+    const size_t o_pre = align_up(o_tab + sizeof(SysDev));
+    const size_t o_scal = align_up(o_pre + kPre * 2 * sizeof(int));
+    const size_t total = o_scal + pm::kScalN * sizeof(long long);
+    if (total > kScratchBytes)
+      throw Fail{PM_ECAPACITY, "pm_evaluate_static: the system needs " + std::to_string(total) +
+                                   " bytes of device memory, more than the " + std::to_string(kScratchBytes) +
+                                   " of the static scratch array (use a context)"};
+    h.block = base;
+    fill_block(&c, d, p, h, true);
+    if (!c.gb) place_lists(h, base + o_lists);
+    const int cnt[kPre] = {h.n, h.cl_cap, h.ncol, h.slot_cap, h.n_corr, h.n_bonded, h.n_gb_tiles, h.n_local, h.sc_cap,
+                           h.n_gb_pairs};
+    int pre[2 * kPre];
+    for (int q = 0; q < kPre; ++q) {
+      pre[2 * q] = 0;
+      pre[2 * q + 1] = cnt[q];
+    }
+    SysDev *d_tab = reinterpret_cast<SysDev *>(base + o_tab);
+    int *d_pre = reinterpret_cast<int *>(base + o_pre);
+    long long *d_scal = reinterpret_cast<long long *>(base + o_scal);
+    HIPCHECK(hipMemcpy(d_tab, &h.dev, sizeof(SysDev), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(d_pre, pre, sizeof pre, hipMemcpyHostToDevice));
+    Launch L;
+    L.stream = nullptr;
+    L.sys = d_tab;
+    L.nsys = 1;
+    L.atom_prefix = d_pre;
+    L.cl_prefix = d_pre + 2;
+    L.col_prefix = d_pre + 4;
+    L.slot_prefix = d_pre + 6;
+    L.corr_prefix = d_pre + 8;
+    L.bonded_prefix = d_pre + 10;
+    L.gb_prefix = d_pre + 12;
+    L.sc_prefix = d_pre + 16;
+    L.gbp_prefix = d_pre + 18;
+    L.n_atoms_total = h.n;
+    L.n_cl_total = h.cl_cap;
+    L.n_cols_total = h.ncol;
+    L.n_slots_total = h.slot_cap;
+    L.n_corr_total = h.n_corr;
+    L.n_bonded_total = h.n_bonded;
+    L.n_gb_tiles_total = h.n_gb_tiles;
+    L.n_gb_pairs_total = h.n_gb_pairs;
+    L.n_local_total = h.n_local;
+    L.n_sc_total = h.sc_cap;
+    L.max_mol = h.dev.n_mol;
+    const size_t xb = 3 * (size_t)h.n * sizeof(double);
+    HIPCHECK(hipMemcpy(h.dev.xaos, x, xb, hipMemcpyHostToDevice));
+    pm::launch_import(L, pm::kXferX);
+    force_eval(&c, L, pm::kStepForce, eval_policy(&c), pm::kPairForcesEnergies, false, false);
+    pm::launch_gather_scalars(L, d_scal);
+    long long scal[pm::kScalN];
+    HIPCHECK(hipMemcpy(scal, d_scal, sizeof scal, hipMemcpyDeviceToHost));
+    const unsigned err = (unsigned)scal[pm::kScalErr];
+    if (err & pm::kErrListCap) {
+      // a longer list, if it still fits: the whole evaluation again
+      if (attempt > 4) throw Fail{PM_ECAPACITY, "pair list capacity kept overflowing"};
+      h.list_cap *= 2;
+      continue;
+    }
+    if (err & pm::kErrNonFinite) throw Fail{PM_EINVAL, "non-finite position"};
+    if (err & pm::kErrColumnCap)
+      throw Fail{PM_ECAPACITY, "more than " + std::to_string(pm::kSortCap) + " atoms in one cell column"};
+    if (err & pm::kErrClusterCap) throw Fail{PM_ECAPACITY, "cluster capacity"};
+    if (f) {
+      pm::launch_export(L, pm::kXferF);
+      std::vector<double> fo(3 * (size_t)h.n);
+      HIPCHECK(hipMemcpy(fo.data(), h.dev.faos, xb, hipMemcpyDeviceToHost));
+      for (size_t k = 0; k < fo.size(); ++k) f[k] += fo[k];
+    }
+    if (e) add_energies(&c, h, scal, e);
+    HIPCHECK(hipStreamSynchronize(nullptr));
+    return;
+  }
+}
+
 // randomNumber (MathUtilities.cpp:153-187) with both of its generators.
 // randomtype 0: the POSIX drand48 sequence -- X <- (0x5DEECE66D X + 11) mod 2^48
 // from X = seed << 16 | 0x330E (srand48), each draw X / 2^48 -- which the
@@ -1611,93 +1850,7 @@ int pm_ctx_create(const pm_force_desc *force, int32_t n_systems, const pm_system
     HIPCHECK(hipEventCreate(&c->evp1));
     HIPCHECK(hipEventCreate(&c->evr0));
     HIPCHECK(hipEventCreate(&c->evr1));
-    ForceDev &P = c->P;
-    std::memset(&P, 0, sizeof P);
-    P.flags = force->flags;
-    P.pair_waves = force->pair_waves ? force->pair_waves : 4;
-    P.pair_split = force->pair_split ? force->pair_split : pm::kPairSplitDefault;
-    if (P.pair_split < 1 || P.pair_split > 64) throw Fail{PM_EINVAL, "pair_split must be 1 .. 64"};
-    if (P.pair_waves != 1 && P.pair_waves != 2 && P.pair_waves != 4 && P.pair_waves != 8)
-      throw Fail{PM_EINVAL, "pair_waves must be 1, 2, 4 or 8"};
-    if (force->cell_scale != 0.0 && !(force->cell_scale >= 0.6 && force->cell_scale <= 2.0))
-      throw Fail{PM_EINVAL, "cell_scale must be 0 (default) or within 0.6 .. 2"};
-    c->gb = (force->flags & PM_FLAG_GB) != 0;
-    c->recip = (force->flags & PM_FLAG_EWALD_RECIPROCAL) != 0;
-    c->molvir = (force->flags & PM_FLAG_MOLVIRIAL) != 0;
-    P.virial = (force->flags & PM_FLAG_VIRIAL) ? 1 : 0;
-    P.molvirial = c->molvir ? 1 : 0;
-    if ((force->flags & PM_FLAG_GB_ACE) && !c->gb) throw Fail{PM_EINVAL, "PM_FLAG_GB_ACE needs PM_FLAG_GB"};
-    if (c->recip && (c->gb || force->term_b != PM_TERM_EWALD_REAL))
-      throw Fail{PM_EINVAL, "PM_FLAG_EWALD_RECIPROCAL needs term_b = PM_TERM_EWALD_REAL (and no GB)"};
-    if (c->gb) {
-      pm::gb_setup_force(P, *force);
-    } else {
-      const int ta = force->term_a, tb = force->term_b;
-      const bool ok = (ta == PM_TERM_LJ && (tb == PM_TERM_NONE || tb == PM_TERM_COULOMB || tb == PM_TERM_EWALD_REAL)) ||
-                      (ta == PM_TERM_COULOMB && tb == PM_TERM_NONE) || (ta == PM_TERM_NONE && tb == PM_TERM_EWALD_REAL);
-      if (!ok) throw Fail{PM_EINVAL, "unsupported term combination"};
-      P.term_a = ta;
-      P.term_b = tb;
-      double cA2 = 0, cB2 = 0;
-      if (ta != PM_TERM_NONE) cA2 = build_switch(force->switch_a, P.sw_a, false, 0);
-      if (tb == PM_TERM_EWALD_REAL) {
-        // NonbondedFullEwaldSystemForce::initialize (NonbondedFullEwaldSystemForce.h:610-619)
-        if (!(force->ewald_alpha > 0 && force->ewald_accuracy > 0 && force->ewald_accuracy < 1))
-          throw Fail{PM_EINVAL, "Ewald needs alpha > 0 and 0 < accuracy < 1"};
-        double p = -std::log(force->ewald_accuracy);
-        double rc = std::sqrt(p) / force->ewald_alpha;
-        cB2 = build_switch(force->switch_b, P.sw_b, true, rc);
-        cB2 = rc * rc;
-        P.alpha = (float)force->ewald_alpha;
-        P.alpha_p = (float)(0.3275911 * force->ewald_alpha);
-        P.alpha2_log2e = (float)(-force->ewald_alpha * force->ewald_alpha / std::log(2.0));
-        P.two_alpha_pi = (float)(2.0 * force->ewald_alpha / std::sqrt(M_PI));
-        P.dalpha = force->ewald_alpha;
-        P.dtwo_alpha_pi = 2.0 * force->ewald_alpha / std::sqrt(M_PI);
-      } else if (tb == PM_TERM_COULOMB) {
-        cB2 = build_switch(force->switch_b, P.sw_b, false, 0);
-        // OneAtomPairTwo includes a pair within the larger of the two switch
-        // cutoffs (OneAtomPairTwo.h:25-27, 41-44) and applies both switches when
-        // either modifies the values (:66-76).  A CutoffSwitchingFunction
-        // (MODIFY = 0) then zeroes its own term beyond its cutoff, a step that
-        // must fall where the FP64 test puts it: that term keeps its own
-        // cutoff as its inclusion cutoff (FP64-checked like the other).
-        // Otherwise a term is included out to the larger cutoff and its
-        // switch takes it to zero.
-        const double cmax = std::max(cA2, cB2);
-        const bool modA = P.sw_a.kind != PM_SWITCH_CUTOFF, modB = P.sw_b.kind != PM_SWITCH_CUTOFF;
-        if (modA || modB) {
-          if (modA) cA2 = cmax;
-          if (modB) cB2 = cmax;
-        } else {
-          cA2 = cB2 = cmax;
-        }
-      }
-      P.cutA2d = cA2;
-      P.cutB2d = cB2;
-      P.cutA2 = (float)cA2;
-      P.cutB2 = (float)cB2;
-      const double cmax = std::max(cA2, cB2);
-      P.guard = (float)(2e-3 + 1e-5 * cmax);
-      c->rc = std::sqrt(cmax);
-      c->skin_md = force->skin > 0 ? force->skin : 2.0;
-      c->skin_eval = 0.1;
-      if (force->rebuild_interval < 0) throw Fail{PM_EINVAL, "rebuild_interval must be >= 0"};
-      // the pair list radius rc + skin must stay below half the smallest box
-      // edge (one image per pair); a larger requested skin is reduced to fit
-      // the systems given here
-      if (n_systems > 0) {
-        double minL = 1e300;
-        for (int s = 0; s < n_systems; ++s)
-          for (int k = 0; k < 3; ++k) minL = std::min(minL, systems[s].box[k]);
-        const double room = 0.5 * minL - c->rc - 0.01;
-        if (room < 0.05)
-          throw Fail{PM_EINVAL, "box edge " + std::to_string(minL) + " leaves no room for a pair list beyond rc " +
-                                    std::to_string(c->rc) + " (needs box > 2 rc)"};
-        c->skin_md = std::min(c->skin_md, room);
-        c->skin_eval = std::min(c->skin_eval, room);
-      }
-    }
+    setup_force(c, force, n_systems, systems);
     for (int s = 0; s < n_systems; ++s) add_system(c, systems[s]);
     HIPCHECK(hipDeviceSynchronize());
   });
@@ -1917,6 +2070,14 @@ int pm_random_velocity(int32_t n, const double *mass, double temperature, uint32
       const double s = std::sqrt(kbt / mass[i]);
       for (int d = 0; d < 3; ++d) v[3 * (size_t)i + d] = s * rng.gaussian();
     }
+  });
+}
+
+int pm_evaluate_static(const pm_force_desc *force, const pm_system_desc *system, const double *x, double *f,
+                       double *e) {
+  return run_guarded(nullptr, [&] {
+    if (!force || !system || !x) throw Fail{PM_EINVAL, "force, system and positions required"};
+    evaluate_static(*force, *system, x, f, e);
   });
 }
 

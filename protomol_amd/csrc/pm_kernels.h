@@ -50,6 +50,8 @@ void launch_kick_drift(const Launch &L, double h, const LangevinDev &lv);
 void launch_md_arm(const Launch &L);
 void launch_half_kick(const Launch &L, double hh, const LangevinDev &lv);
 void launch_zero_forces(const Launch &L);
+// zero `bytes` (a multiple of 8) of device memory with a fill kernel
+void launch_fill_zero(void *p, size_t bytes, hipStream_t stream);
 // the molecules' centres of mass at the current positions (molecular virial)
 void launch_mol_com(const Launch &L);
 void launch_step_begin(const Launch &L, int mode, float lim2, int kmax);

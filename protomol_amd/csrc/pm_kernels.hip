@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <algorithm>
+
 #include "../../include/protomol_amd.h"
 
 namespace pm {
@@ -1772,6 +1774,18 @@ __global__ __launch_bounds__(256) void k_mol_com(SysDev *sys) {
 void launch_mol_com(const Launch &L) {
   if (L.max_mol > 0)
     hipLaunchKernelGGL(k_mol_com, dim3(nblk(L.max_mol, 256), L.nsys), dim3(256), 0, L.stream, L.sys);
+}
+
+// zero `bytes` (a multiple of 8) at p: a fill kernel, not a memset
+__global__ void k_fill_zero(unsigned long long *p, size_t n) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) p[k] = 0ull;
+}
+
+void launch_fill_zero(void *p, size_t bytes, hipStream_t stream) {
+  const size_t n = bytes / 8;
+  if (!n) return;
+  const int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_zero, dim3(blocks), dim3(256), 0, stream, static_cast<unsigned long long *>(p), n);
 }
 
 void launch_zero_forces(const Launch &L) {
