@@ -39,7 +39,7 @@
 extern "C" {
 #endif
 
-#define PM_ABI_VERSION 4
+#define PM_ABI_VERSION 5
 
 /* status codes */
 #define PM_OK 0
@@ -88,6 +88,12 @@ extern "C" {
                                  OneAtomPairNoExclusion.h:73-81, NonbondedFullEwaldSystemForce.h:
                                  839-848, 1039-1078) and f (x) d to the VIRIAL slots even with
                                  PM_FLAG_VIRIAL off, as the reference's addVirial does */
+
+#define PM_FLAG_ALL_PAIRS 128u /* all-pairs vacuum context, NonbondedSimpleFullSystemForce
+                                  (force/nonbonded/NonbondedSimpleFullSystemForce.h:125-150), with
+                                  term_a = PM_TERM_LJ and/or term_b = PM_TERM_COULOMB (either may be
+                                  PM_TERM_NONE) and, with PM_FLAG_GB as well, the GB passes;
+                                  PM_FLAG_GB alone is GB + LJ + Coulomb */
 
 /* ScalarStructure slots (type/ScalarStructure.h:26-67); energies[] has 36 */
 #define PM_E_COULOMB 0

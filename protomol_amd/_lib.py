@@ -18,6 +18,7 @@ SWITCH_NONE, SWITCH_CUTOFF, SWITCH_C1, SWITCH_C2, SWITCH_CN, SWITCH_CMPCNCN = ra
 TERM_NONE, TERM_LJ, TERM_COULOMB, TERM_EWALD_REAL = range(4)
 LJ_TABLE, LJ_PERATOM = 0, 1
 FLAG_VIRIAL, FLAG_GB, FLAG_BONDED, FLAG_TIMING, FLAG_EWALD_RECIPROCAL, FLAG_GB_ACE, FLAG_MOLVIRIAL = 1, 2, 4, 8, 16, 32, 64
+FLAG_ALL_PAIRS = 128
 NSLOTS = 36
 
 EXPORTED = ("pm_abi_version", "pm_device_info", "pm_ctx_create", "pm_ctx_destroy", "pm_ctx_reserve",

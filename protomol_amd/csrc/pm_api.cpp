@@ -126,7 +126,7 @@ struct SysPrep {
   std::vector<unsigned long long> excl_win;  // per atom: FULL, MODIFIED partner bits (b - a + 31), far flag
   std::vector<double> invm;
   std::vector<float> sig_half, eps_sqrt;
-  std::vector<float4> lj;
+  std::vector<float4> lj, lj12;
   int n_corr = 0, n_bonds = 0, n_angles = 0, ncol = 1, ncx = 1, ncy = 1, rx = 0, ry = 0;
   int cl_cap = 0, slot_cap = 0, sc_cap = 0, list_cap = 0, n_gb_tiles = 0;
   double ax = 0, ay = 0;
@@ -597,17 +597,20 @@ SysPrep prepare(const pm_ctx *c, const pm_system_desc &d) {
     }
   } else {
     if (d.n_types < 1 || !d.lj_table || !d.type) throw Fail{PM_EINVAL, "LJ table mode needs n_types, type and lj_table"};
-    if (d.n_types > 16 && !c->gb)
-      throw Fail{PM_EINVAL, "the cell-list pair kernel keeps the LJ table in LDS: at most 16 types (use PM_LJ_PERATOM)"};
+    if (d.n_types > 4096) throw Fail{PM_EINVAL, "at most 4096 LJ types"};
     for (int i = 0; i < n; ++i)
       if (d.type[i] < 0 || d.type[i] >= d.n_types) throw Fail{PM_EINVAL, "atom type out of range"};
     p.lj.resize((size_t)d.n_types * d.n_types);
-    for (size_t k = 0; k < p.lj.size(); ++k)
+    p.lj12.resize(p.lj.size());
+    for (size_t k = 0; k < p.lj.size(); ++k) {
       p.lj[k] = make_float4((float)d.lj_table[4 * k], (float)d.lj_table[4 * k + 1], (float)d.lj_table[4 * k + 2],
                             (float)d.lj_table[4 * k + 3]);
+      // the pair kernel's row format: A, B, 12 A, 6 B
+      p.lj12[k] = make_float4(p.lj[k].x, p.lj[k].y, 12.0f * p.lj[k].x, 6.0f * p.lj[k].y);
+    }
   }
   if (c->gb) {
-    if (!d.gb_radius || !d.gb_scaling) throw Fail{PM_EINVAL, "GB needs gb_radius and gb_scaling"};
+    if (c->P.gb_on && (!d.gb_radius || !d.gb_scaling)) throw Fail{PM_EINVAL, "GB needs gb_radius and gb_scaling"};
     p.n_gb_tiles = pm::gb_tile_count(n);
     p.cl_cap = 0;
     p.slot_cap = n;  // forces are kept per atom: slot = atom
@@ -675,6 +678,7 @@ size_t carve(const pm_ctx *c, const SysPrep &p, const pm_system_desc &d, Carver 
     S.eps_sqrt = cv.take<float>(n);
   } else {
     S.lj_table = cv.take<float4>(p.lj.size());
+    S.lj_table12 = cv.take<float4>(p.lj12.size());
   }
   S.x = cv.take<double>(3 * (size_t)n);
   S.v = cv.take<double>(3 * (size_t)n);
@@ -919,6 +923,7 @@ void fill_block(pm_ctx *c, const pm_system_desc &d, const SysPrep &p, SysHost &h
   } else {
     upload(const_cast<int *>(S.type), d.type, n);
     upload(const_cast<float4 *>(S.lj_table), p.lj.data(), p.lj.size());
+    upload(const_cast<float4 *>(S.lj_table12), p.lj12.data(), p.lj12.size());
   }
   upload(const_cast<int *>(S.bonds), d.bonds, 2 * (size_t)p.n_bonds);
   upload(const_cast<double *>(S.bond_k), d.bond_k, p.n_bonds);
@@ -927,8 +932,10 @@ void fill_block(pm_ctx *c, const pm_system_desc &d, const SysPrep &p, SysHost &h
   upload(const_cast<double *>(S.angle_k), d.angle_k, p.n_angles);
   upload(const_cast<double *>(S.angle_t0), d.angle_theta0, p.n_angles);
   if (c->gb) {
-    upload(const_cast<double *>(S.gb_rho), d.gb_radius, n);
-    upload(const_cast<double *>(S.gb_S), d.gb_scaling, n);
+    if (c->P.gb_on) {
+      upload(const_cast<double *>(S.gb_rho), d.gb_radius, n);
+      upload(const_cast<double *>(S.gb_S), d.gb_scaling, n);
+    }
     std::vector<int> ident(n);
     for (int i = 0; i < n; ++i) ident[i] = i;
     upload(S.slot, ident.data(), n);
@@ -1529,12 +1536,13 @@ void setup_force(pm_ctx *c, const pm_force_desc *force, int32_t n_systems, const
     throw Fail{PM_EINVAL, "pair_waves must be 1, 2, 4 or 8"};
   if (force->cell_scale != 0.0 && !(force->cell_scale >= 0.6 && force->cell_scale <= 2.0))
     throw Fail{PM_EINVAL, "cell_scale must be 0 (default) or within 0.6 .. 2"};
-  c->gb = (force->flags & PM_FLAG_GB) != 0;
+  c->gb = (force->flags & (PM_FLAG_GB | PM_FLAG_ALL_PAIRS)) != 0;  // the all-pairs (vacuum) path
   c->recip = (force->flags & PM_FLAG_EWALD_RECIPROCAL) != 0;
   c->molvir = (force->flags & PM_FLAG_MOLVIRIAL) != 0;
   P.virial = (force->flags & PM_FLAG_VIRIAL) ? 1 : 0;
   P.molvirial = c->molvir ? 1 : 0;
-  if ((force->flags & PM_FLAG_GB_ACE) && !c->gb) throw Fail{PM_EINVAL, "PM_FLAG_GB_ACE needs PM_FLAG_GB"};
+  if ((force->flags & PM_FLAG_GB_ACE) && !(force->flags & PM_FLAG_GB))
+    throw Fail{PM_EINVAL, "PM_FLAG_GB_ACE needs PM_FLAG_GB"};
   if (c->recip && (c->gb || force->term_b != PM_TERM_EWALD_REAL))
     throw Fail{PM_EINVAL, "PM_FLAG_EWALD_RECIPROCAL needs term_b = PM_TERM_EWALD_REAL (and no GB)"};
   if (c->gb) {

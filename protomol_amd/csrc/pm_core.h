@@ -111,6 +111,8 @@ struct ForceDev {
   // reference's PI = 3.14169
   int gb_ace;
   double gb_ace_sigma, gb_ace_rho_s, gb_ace_c1;
+  // all-pairs (vacuum) context: the GB passes, the LJ term, the Coulomb term
+  int gb_on, lj_on, coul_on;
   int virial, molvirial;   // ScalarStructure::virial() / molecularVirial()
 };
 
@@ -168,8 +170,10 @@ struct SysDev {
   uint4 *list;
   unsigned long long *list_mask;
   int mask_cap;            // mask slots per super-cluster
-  // LJ table (table mode): n_types^2 float4 (A, B, A14, B14)
+  // LJ table (table mode): n_types^2 float4 (A, B, A14, B14), and the pair
+  // kernel's rows (A, B, 12 A, 6 B)
   const float4 *lj_table;
+  const float4 *lj_table12;
   // bonded
   int n_bonds, n_angles;
   const int *bonds, *angles;

@@ -31,8 +31,15 @@ void gb_setup_force(ForceDev &P, const pm_force_desc &d) {
   P.gb_ace_sigma = d.gb_ace_sigma > 0 ? d.gb_ace_sigma : 2.26 / 418.4;  // GBACEForce.cpp:218-222 defaults
   P.gb_ace_rho_s = d.gb_ace_rho_s > 0 ? d.gb_ace_rho_s : 1.4;
   P.gb_ace_c1 = 24 * kAcePi * P.gb_ace_sigma;
-  P.term_a = PM_TERM_LJ;
-  P.term_b = PM_TERM_COULOMB;
+  // PM_FLAG_GB alone: GB + LJ + Coulomb; with PM_FLAG_ALL_PAIRS the pair
+  // terms are term_a (LJ or none) and term_b (Coulomb or none), and the GB
+  // passes run only with PM_FLAG_GB
+  const bool sel = (d.flags & PM_FLAG_ALL_PAIRS) != 0;
+  P.gb_on = (d.flags & PM_FLAG_GB) ? 1 : 0;
+  P.lj_on = !sel || d.term_a == PM_TERM_LJ ? 1 : 0;
+  P.coul_on = !sel || d.term_b == PM_TERM_COULOMB ? 1 : 0;
+  P.term_a = P.lj_on ? PM_TERM_LJ : PM_TERM_NONE;
+  P.term_b = P.coul_on ? PM_TERM_COULOMB : PM_TERM_NONE;
 }
 
 void gb_setup_system(SysDev &S, const double *rho, const double *scaling, double *born, double *drfac,
@@ -359,6 +366,7 @@ __global__ __launch_bounds__(kGBPWaves * 64) void k_gb_pairs(SysDev *sys, const 
   const float qiB2 = 0.5f * qi * qi * iBi * iBi;  // q_i^2 / (2 B_i^2)
   const float csf = S.csf;
   const float ace_c1 = (float)P.gb_ace_c1;
+  const float lj_on = P.lj_on ? 1.0f : 0.0f, coul_on = P.coul_on ? 1.0f : 0.0f;
   double fx = 0.0, fy = 0.0, fz = 0.0, gx = 0.0, gy = 0.0, gz = 0.0, egb = 0.0, elj = 0.0, eco = 0.0;
   double vir[6] = {0, 0, 0, 0, 0, 0}, mvir[6] = {0, 0, 0, 0, 0, 0};
   const int r0 = diag ? 1 : 0, r1 = diag ? 32 : 63;
@@ -379,28 +387,31 @@ __global__ __launch_bounds__(kGBPWaves * 64) void k_gb_pairs(SysDev *sys, const 
       const float dx = (float)(pj.x - xi), dy = (float)(pj.y - yi), dz = (float)(pj.z - zi);
       const float r2 = ok ? dx * dx + dy * dy + dz * dz : 1.0e8f;
       const float rinv = rsqrtf(r2);
-      const float dist = r2 * rinv;
-      const float Bj = aj.z, iBj = aj.w;
-      const float BB = Bi * Bj;
-      const float ex = __expf(-r2 * (i4Bi * iBj));  // exp(-r^2 / (4 Bi Bj))
-      const float inv = rsqrtf(r2 + BB * ex);       // 1 / f_GB
       const float qq = qi * bj4.x;
-      const float id4 = 0.25f * rinv, q4 = id4 * rinv;
-      const float dRi = dri * dburial(dist, r2, id4, q4, ori, aj.x, aj.y);
-      const float dRj = bj4.z * dburial(dist, r2, id4, q4, aj.x, ori, Si);
-      const float c0 = qq * inv * inv * 0.5f * inv;
-      const float ai = Bj + r2 * i4Bi, bj = Bi + r2 * (0.25f * iBj);
-      float force = -c0 * ((2.0f * dist - 0.5f * ex * dist) + ex * dRi * ai + ex * dRj * bj) * rinv;
-      force -= qiB2 * dRi * rinv;
-      force -= bj4.y * dRj * rinv;
-      const float part = c0 * ex;
-      force -= (Pi - part * ai) * (dRi * rinv);
-      force -= (bj4.w - part * bj) * (dRj * rinv);
-      force *= fac;
-      // GBACEForce.cpp:155: c1 (c_i R_i^-7 dR_i/dr + c_j R_j^-7 dR_j/dr) / r
-      if (P.gb_ace) force += ace_c1 * (aki * dRi + cj4.z * dRj) * rinv;
-      float fs = force;
-      if (energies) egb += ok ? (double)(-qq * inv * fac) : 0.0;
+      float fs = 0.0f;
+      if (P.gb_on) {  // wave-uniform
+        const float dist = r2 * rinv;
+        const float Bj = aj.z, iBj = aj.w;
+        const float BB = Bi * Bj;
+        const float ex = __expf(-r2 * (i4Bi * iBj));  // exp(-r^2 / (4 Bi Bj))
+        const float inv = rsqrtf(r2 + BB * ex);       // 1 / f_GB
+        const float id4 = 0.25f * rinv, q4 = id4 * rinv;
+        const float dRi = dri * dburial(dist, r2, id4, q4, ori, aj.x, aj.y);
+        const float dRj = bj4.z * dburial(dist, r2, id4, q4, aj.x, ori, Si);
+        const float c0 = qq * inv * inv * 0.5f * inv;
+        const float ai = Bj + r2 * i4Bi, bj = Bi + r2 * (0.25f * iBj);
+        float force = -c0 * ((2.0f * dist - 0.5f * ex * dist) + ex * dRi * ai + ex * dRj * bj) * rinv;
+        force -= qiB2 * dRi * rinv;
+        force -= bj4.y * dRj * rinv;
+        const float part = c0 * ex;
+        force -= (Pi - part * ai) * (dRi * rinv);
+        force -= (bj4.w - part * bj) * (dRj * rinv);
+        force *= fac;
+        // GBACEForce.cpp:155: c1 (c_i R_i^-7 dR_i/dr + c_j R_j^-7 dR_j/dr) / r
+        if (P.gb_ace) force += ace_c1 * (aki * dRi + cj4.z * dRj) * rinv;
+        fs = force;
+        if (energies) egb += ok ? (double)(-qq * inv * fac) : 0.0;
+      }
       if (ok && cls != PM_EXCL_FULL) {
         float A, Bc;
         if (peratom) {
@@ -419,8 +430,8 @@ __global__ __launch_bounds__(kGBPWaves * 64) void k_gb_pairs(SysDev *sys, const 
         }
         const float r2inv = rinv * rinv;
         const float r6 = r2inv * r2inv * r2inv;
-        const float r6B = Bc * r6, r12A = A * r6 * r6;
-        float ec = qq * rinv;
+        const float r6B = lj_on * Bc * r6, r12A = lj_on * A * r6 * r6;
+        float ec = coul_on * qq * rinv;
         if (cls == PM_EXCL_MODIFIED) ec *= csf;
         fs += (12.0f * r12A - 6.0f * r6B) * r2inv + ec * r2inv;
         if (energies) {
@@ -519,7 +530,7 @@ __global__ __launch_bounds__(kGBI) void k_gb_fsum(SysDev *sys, const int *gp, in
     S.fx[i] += __double2ll_rn(f[0] * kFScale);
     S.fy[i] += __double2ll_rn(f[1] * kFScale);
     S.fz[i] += __double2ll_rn(f[2] * kFScale);
-    if (energies) {
+    if (energies && P.gb_on) {
       e = -0.5 * (double)S.charge[i] * (double)S.charge[i] * (1.0 / S.gb_born[i]) * P.gb_fac;
       if (P.gb_ace) {
         // 4 PI sigma (rho_i + rho_s)^2 (rho_i / R_i)^6
@@ -539,10 +550,12 @@ void launch_gb(const Launch &L, const ForceDev &P, bool energies) {
   if (!L.n_gb_tiles_total) return;
   dim3 gt(L.n_gb_tiles_total), bt(kGBI);
   dim3 gp2((L.n_gb_pairs_total + kGBPWaves - 1) / kGBPWaves), bp(kGBPWaves * 64);
-  hipLaunchKernelGGL(k_gb_sym<1>, gp2, bp, 0, L.stream, L.sys, L.gbp_prefix, L.nsys, P);
-  hipLaunchKernelGGL(k_gb_sums<1>, gt, bt, 0, L.stream, L.sys, L.gb_prefix, L.nsys, P);
-  hipLaunchKernelGGL(k_gb_sym<2>, gp2, bp, 0, L.stream, L.sys, L.gbp_prefix, L.nsys, P);
-  hipLaunchKernelGGL(k_gb_sums<2>, gt, bt, 0, L.stream, L.sys, L.gb_prefix, L.nsys, P);
+  if (P.gb_on) {  // Born radii, then partial sums
+    hipLaunchKernelGGL(k_gb_sym<1>, gp2, bp, 0, L.stream, L.sys, L.gbp_prefix, L.nsys, P);
+    hipLaunchKernelGGL(k_gb_sums<1>, gt, bt, 0, L.stream, L.sys, L.gb_prefix, L.nsys, P);
+    hipLaunchKernelGGL(k_gb_sym<2>, gp2, bp, 0, L.stream, L.sys, L.gbp_prefix, L.nsys, P);
+    hipLaunchKernelGGL(k_gb_sums<2>, gt, bt, 0, L.stream, L.sys, L.gb_prefix, L.nsys, P);
+  }
   if (energies && (P.virial || P.molvirial))
     hipLaunchKernelGGL((k_gb_pairs<true, true>), gp2, bp, 0, L.stream, L.sys, L.gbp_prefix, L.nsys, P);
   else if (energies)

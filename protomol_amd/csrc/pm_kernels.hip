@@ -857,7 +857,7 @@ __device__ __forceinline__ float dpp_sum8_scatter(float x, float y, float z, int
   return c + r4;
 }
 
-constexpr int kLJLds = 256;  // LJ table entries kept in LDS per wave (T <= 16)
+constexpr int kLJLds = 256;  // LJ table entries kept in LDS per wave (T <= 16; larger tables are read from L1/L2)
 // LDS per wave of the pair kernel: the LJ table as (A, B, 12 A, 6 B), then the
 // super-cluster's 64 i atoms (position + charge, byte offset of the atom's
 // LJ table row, atom index, per-atom LJ)
@@ -980,6 +980,10 @@ __global__ __launch_bounds__(512, (KIND == kPairForces ? PM_PAIR_MIN_WAVES : PM_
   PM_GLOBAL long long *gfc = gfx + (size_t)(ii == 1 ? 2 : (ii == 2 ? 1 : 0)) * S->f_stride;
   const bool peratom = S->lj_mode == PM_LJ_PERATOM;
   const int T = S->n_types;
+  // more types than the LDS copy holds: rows from the global (A, B, 12 A, 6 B)
+  // table through the caches (wave-uniform)
+  const bool big_table = T * T > kLJLds;
+  const char *gtab12 = reinterpret_cast<const char *>(S->lj_table12);
   // stage the super-cluster's i atoms: lane = c * 8 + atom
   unsigned long long valid_ca;  // bit c * 8 + atom: a real atom
   {
@@ -1006,10 +1010,10 @@ __global__ __launch_bounds__(512, (KIND == kPairForces ? PM_PAIR_MIN_WAVES : PM_
     lpp[lane] = pp;
     valid_ca = lane_mask(valid);
   }
-  // LJ table (T <= 16, checked at context creation) as (A, B, 12 A, 6 B); the
+  // LJ table (T <= 16; larger ones stay in global memory) as (A, B, 12 A, 6 B); the
   // modified-pair coefficients stay in the global table
   const PM_GLOBAL float4 *gtab = gp(S->lj_table);
-  if (TA == PM_TERM_LJ && !peratom) {
+  if (TA == PM_TERM_LJ && !peratom && !big_table) {
     for (int k = lane; k < T * T; k += kWave) {
       const float4 g = gtab[k];
       ljs[k] = make_float4(g.x, g.y, 12.0f * g.x, 6.0f * g.y);
@@ -1119,7 +1123,11 @@ __global__ __launch_bounds__(512, (KIND == kPairForces ? PM_PAIR_MIN_WAVES : PM_
           A12 = 12.0f * A;
           B6 = 6.0f * B;
         } else {
-          const float4 p = *reinterpret_cast<const float4 *>(ljrow + info);
+          float4 p;
+          if (__builtin_expect(big_table, 0))
+            p = *gp(reinterpret_cast<const float4 *>(gtab12 + info + tj * 16));
+          else
+            p = *reinterpret_cast<const float4 *>(ljrow + info);
           A = p.x;
           B = p.y;
           A12 = p.z;

@@ -224,7 +224,7 @@ CHAIN_EPS = (0.195, 0.0914)
 
 
 def chain_box(n_mol: int, box: float, seed: int, length: int = 4, per_atom: bool = False,
-              atom_noise: float = 0.05) -> System:
+              atom_noise: float = 0.05, n_types: int = 2) -> System:
     """Periodic box of ``n_mol`` united-atom chains of ``length`` sites on a
     cubic lattice, random orientations, planar zig-zag geometry plus Gaussian
     ``atom_noise`` (A).  Exclusions: 1-2 and 1-3 pairs FULL, 1-4 pairs
@@ -232,7 +232,9 @@ def chain_box(n_mol: int, box: float, seed: int, length: int = 4, per_atom: bool
     +-0.25 e alternate along the chain and ``coulomb_scaling`` is 1/1.2.
     ``per_atom`` gives per-atom sigma / epsilon with GROMACS mixing and
     ``lj_scaling`` 0.5 instead of the type table (whose 1-4 entries use
-    sigma 0.9x and epsilon 0.5x)."""
+    sigma 0.9x and epsilon 0.5x).  ``n_types`` > 2 gives every site one of
+    that many types (seeded), each with its own sigma U[2.8, 4.0] A and
+    epsilon U[0.05, 0.2] kcal/mol, mixed and scaled the same way."""
     rng = np.random.default_rng(seed)
     m = 1
     while m ** 3 < n_mol:
@@ -271,11 +273,18 @@ def chain_box(n_mol: int, box: float, seed: int, length: int = 4, per_atom: bool
         sig = np.array(CHAIN_SIGMA)[typ]
         eps = np.array(CHAIN_EPS)[typ]
         return System(lj_sigma=sig, lj_epsilon=eps, lj_scaling=0.5, **kw)
-    table = np.zeros((2, 2, 4))
-    for ti in range(2):
-        for tj in range(2):
-            sg = 0.5 * (CHAIN_SIGMA[ti] + CHAIN_SIGMA[tj])
-            ep = math.sqrt(CHAIN_EPS[ti] * CHAIN_EPS[tj])
+    sigmas, epss = list(CHAIN_SIGMA), list(CHAIN_EPS)
+    if n_types > 2:
+        trng = np.random.default_rng(seed + 7919)
+        sigmas = trng.uniform(2.8, 4.0, size=n_types).tolist()
+        epss = trng.uniform(0.05, 0.2, size=n_types).tolist()
+        typ = trng.integers(0, n_types, size=n).astype(np.int32)
+    nt = len(sigmas)
+    table = np.zeros((nt, nt, 4))
+    for ti in range(nt):
+        for tj in range(nt):
+            sg = 0.5 * (sigmas[ti] + sigmas[tj])
+            ep = math.sqrt(epss[ti] * epss[tj])
             A, B = 4 * ep * sg ** 12, 4 * ep * sg ** 6
             A14, B14 = 4 * 0.5 * ep * (0.9 * sg) ** 12, 4 * 0.5 * ep * (0.9 * sg) ** 6
             table[ti, tj] = (A, B, A14, B14)

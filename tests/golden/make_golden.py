@@ -364,6 +364,20 @@ def copy_water_expected():
 VELOCITY_CASES = (("t0", 0, 1234, 300.0), ("t1", 1, 4242, 310.0), ("t0_big_seed", 0, 3000000000, 250.0))
 
 
+def make_many_types():
+    """A 40-type LJ table (more types than the pair kernel's LDS copy holds):
+    the united-atom chains with every site given one of 40 seeded types, LJ +
+    Coulomb C2 cutoff pairs with MODIFIED 1-4 pairs."""
+    c = S.chain_box(512, 52.0, seed=3, n_types=40)
+    ta, tb, sa, sb = CUTOFF_CASES["c2c2"]
+    r = refio.system_records(c)
+    r.update(mode=np.int32(1), term_a=np.int32(ta), term_b=np.int32(tb), sw_a=np.array(sa, float),
+             sw_b=np.array(sb, float), alg_cutoff=10.0, cellsize=5.0, virial=np.int32(1))
+    o = refio.run(r)
+    np.savez_compressed(os.path.join(HERE, "ref_lj40.npz"), x=c.x, f=o["f"].reshape(-1, 3), e=o["e"],
+                        census=np.array(o["census"], dtype=np.uint64))
+
+
 def make_velocity():
     """The reference's velocity start (MainModule's randomNumber(seed,
     randomtype) seeding, randomVelocity, kineticEnergy, temperature) and
@@ -389,6 +403,9 @@ def main():
     w = water_fixture_system()
     if sys.argv[1:] == ["velocity"]:
         make_velocity()
+        return
+    if sys.argv[1:] == ["lj40"]:
+        make_many_types()
         return
     if sys.argv[1:] == ["cutoff_switch"]:
         for name in ("coul_cut", "cut_cut_two", "cut_c2_two", "c2_cut_two"):
@@ -427,6 +444,7 @@ def main():
     make_water423()
     copy_water_expected()
     make_velocity()
+    make_many_types()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

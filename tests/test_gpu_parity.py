@@ -868,3 +868,50 @@ def test_md_list_regrow_rerun_is_bitwise_unconstrained():
     print(f"  overflow during the advance at list capacity {hit}", flush=True)
     assert np.array_equal(x1, x0) and np.array_equal(v1, vv0) and np.array_equal(f1, f0)
     assert np.array_equal(e1, e0)
+
+
+def test_lj_table_40_types_vs_reference_golden():
+    """More LJ types than the pair kernel's LDS copy holds (40 > 16): the
+    rows come from the global table; evaluation and the MD path's forces
+    against the reference's classes (ref_lj40.npz, MODIFIED 1-4 pairs too)."""
+    print("case lj table 40 types", flush=True)
+    g = T.golden("ref_lj40.npz")
+    c = S.chain_box(512, 52.0, seed=3, n_types=40)
+    ta, sa, tb, sb = MODELS["c2c2"]
+    with L.Context(force_desc(ta, sa, tb, sb), [c]) as ctx:
+        f, e = evaluate(ctx, 0, c.x)
+        cen = ctx.census(0, c.x)
+        ctx.md_set_state(c.x, np.zeros_like(c.x))
+        _, _, fmd, _ = ctx.md_get_state()
+    assert cen == (int(g["census"][0]), int(g["census"][1]))
+    assert T.rms_rel(f, g["f"]) <= RTOL_F
+    assert T.rms_rel(fmd, g["f"]) <= RTOL_F
+    check_energies(e, g["e"], npairs=cen[0])
+    check_virial(e, g["e"], npairs=cen[0])
+
+
+@pytest.mark.parametrize("terms", ["lj", "coulomb", "both", "gb_lj"])
+def test_all_pairs_term_selection_vs_oracle(terms):
+    """PM_FLAG_ALL_PAIRS (NonbondedSimpleFullSystemForce in vacuum): LJ and
+    Coulomb chosen by term_a / term_b, the GB passes only with PM_FLAG_GB."""
+    print(f"case all pairs {terms}", flush=True)
+    c = S.gb_chain(700, seed=21)
+    d = gb_desc()
+    d.flags = L.FLAG_ALL_PAIRS | (L.FLAG_GB if terms == "gb_lj" else 0)
+    d.term_a = L.TERM_LJ if terms in ("lj", "both", "gb_lj") else L.TERM_NONE
+    d.term_b = L.TERM_COULOMB if terms in ("coulomb", "both") else L.TERM_NONE
+    with L.Context(d, [c]) as ctx:
+        f, e = evaluate(ctx, 0, c.x)
+    full = O.vacuum_lj_coulomb(c)
+    # Coulomb alone: LJ off in a copy of the system; LJ alone: charges zeroed
+    nolj = S.System(**{**c.__dict__, "lj_epsilon": np.zeros(c.n)})
+    noq = S.System(**{**c.__dict__, "charge": np.zeros(c.n)})
+    fo, eo = {"lj": O.vacuum_lj_coulomb(noq), "coulomb": O.vacuum_lj_coulomb(nolj), "both": full,
+              "gb_lj": O.vacuum_lj_coulomb(noq)}[terms]
+    if terms == "gb_lj":
+        fg, eg, _ = O.gb_evaluate(c)
+        fo, eo = fo + fg, eo + eg
+    assert T.rms_rel(f, fo) <= RTOL_F
+    npairs = c.n * (c.n - 1) // 2
+    for s in (0, 1):
+        assert abs(e[s] - eo[s]) <= RTOL_E * max(1.0, abs(eo[s])) + 1e-7 * npairs, (s, e[s], eo[s])

@@ -436,3 +436,15 @@ def test_cutoff_switch_does_not_modify():
     fc2, ec2, _ = O.nonbonded_cutoff(w, O.TERM_COULOMB, c2, cellsize=5.0)
     assert np.allclose(f2, fl8 + fc2, rtol=0, atol=1e-10) and abs(e2[1] - el8[1]) < 1e-9 * abs(el8[1])
     assert abs(el[1] - el8[1]) > 1.0  # the two readings differ
+
+
+def test_many_types_matches_reference():
+    # a 40-type LJ table with MODIFIED 1-4 pairs (ref_lj40.npz)
+    g = T.golden("ref_lj40.npz")
+    c = S.chain_box(512, 52.0, seed=3, n_types=40)
+    assert np.array_equal(c.x, g["x"]) and c.lj_table.shape[0] == 40
+    sw = O.make_switch(O.SW_C2, 8.0, 10.0)
+    f, e, cen = O.nonbonded_cutoff(c, O.TERM_LJ, sw, O.TERM_COULOMB, sw, algorithm_cutoff=10.0, cellsize=5.0,
+                                   virial=True)
+    assert np.array_equal(f, g["f"]) and np.array_equal(e, g["e"])
+    assert (int(cen[0]), int(cen[1])) == (int(g["census"][0]), int(g["census"][1]))
