@@ -11,7 +11,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libprotomol_amd.so")
+# PROTOMOL_AMD_LIB names another build of the library (e.g. one of two builds timed side by side)
+LIB_PATH = os.environ.get("PROTOMOL_AMD_LIB") or os.path.join(HERE, "libprotomol_amd.so")
 
 PM_OK, PM_EINVAL, PM_ENOMEM, PM_EDEVICE, PM_ECAPACITY, PM_ENOTREADY, PM_EFULL = 0, -1, -2, -3, -4, -5, -6
 SWITCH_NONE, SWITCH_CUTOFF, SWITCH_C1, SWITCH_C2, SWITCH_CN, SWITCH_CMPCNCN = range(6)

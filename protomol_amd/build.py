@@ -14,7 +14,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libprotomol_amd.so")
-SOURCES = ["pm_kernels.hip", "pm_gb.hip", "pm_ewald.hip", "pm_api.cpp"]
+# the pair-kernel sources first: they take longest
+SOURCES = ["pm_pair_lj_ewald.hip", "pm_pair_lj_coulomb.hip", "pm_pair_lj.hip", "pm_pair_coulomb.hip",
+           "pm_pair_ewald.hip", "pm_kernels.hip", "pm_gb.hip", "pm_ewald.hip", "pm_api.cpp"]
 ARCH = os.environ.get("PM_OFFLOAD_ARCH", "gfx950")
 
 # packed-FP32 instructions are kept out of the generated code on purpose
@@ -30,7 +32,8 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose: bool = False, jobs: int = 3) -> str:
+def build(verbose: bool = False, jobs: int = 0) -> str:
+    jobs = jobs or max(1, min(len(SOURCES), os.cpu_count() or 1))
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     cc = hipcc()

@@ -38,6 +38,12 @@ void launch_gather_scalars(const Launch &L, long long *out);
 void launch_clear_err(const Launch &L);
 void launch_rebuild(const Launch &L, float rlist2);
 bool launch_pair(const Launch &L, const ForceDev &P, int kind);
+// the pair kernels of one term combination each (pm_pair_*.hip)
+void launch_pair_lj(const Launch &L, const ForceDev &P, int kind);
+void launch_pair_coulomb(const Launch &L, const ForceDev &P, int kind);
+void launch_pair_lj_coulomb(const Launch &L, const ForceDev &P, int kind);
+void launch_pair_lj_ewald(const Launch &L, const ForceDev &P, int kind);
+void launch_pair_ewald(const Launch &L, const ForceDev &P, int kind);
 void launch_local(const Launch &L, const ForceDev &P, int mode);
 // LangevinLeapfrogIntegrator::doHalfKick constants for one time step
 // (integrator/base/LangevinLeapfrogIntegrator.cpp:46-70); on = 0: the plain
