@@ -712,6 +712,7 @@ size_t carve(const pm_ctx *c, const SysPrep &p, const pm_system_desc &d, Carver 
   S.fx = cv.take<long long>(3 * (size_t)S.f_stride);
   S.fy = S.fx ? S.fx + S.f_stride : nullptr;
   S.fz = S.fx ? S.fx + 2 * (size_t)S.f_stride : nullptr;
+  S.fl = cv.take<long long>(3 * (size_t)n);
   if (c->molvir) {
     S.n_mol = p.n_mol;
     S.mol_start = cv.take<int>(p.mol_start.size());
@@ -1117,25 +1118,25 @@ void force_eval(pm_ctx *c, const Launch &L, int mode, const StepPolicy &pol, int
   // the molecular virial reads the molecules' centres at these positions
   if (c->molvir && local_en) pm::launch_mol_com(L);
   if (c->gb) {
-    if (timed) HIPCHECK(hipEventRecord(c->evp0, c->stream));
-    if (pool) HIPCHECK(hipEventRecord(next_event(c), c->stream));
+    if (timed) HIPCHECK(hipEventRecord(c->evp0, L.stream));
+    if (pool) HIPCHECK(hipEventRecord(next_event(c), L.stream));
     pm::launch_gb(L, c->P, en || gb_energies);
-    if (pool) HIPCHECK(hipEventRecord(next_event(c), c->stream));
-    if (timed) HIPCHECK(hipEventRecord(c->evp1, c->stream));
+    if (pool) HIPCHECK(hipEventRecord(next_event(c), L.stream));
+    if (timed) HIPCHECK(hipEventRecord(c->evp1, L.stream));
     pm::launch_local(L, c->P, local);
     return;
   }
   pm::launch_rebuild(L, pol.rlist * pol.rlist);
-  if (timed) HIPCHECK(hipEventRecord(c->evp0, c->stream));
-  if (pool) HIPCHECK(hipEventRecord(next_event(c), c->stream));
+  if (timed) HIPCHECK(hipEventRecord(c->evp0, L.stream));
+  if (pool) HIPCHECK(hipEventRecord(next_event(c), L.stream));
   if (!pm::launch_pair(L, c->P, en && c->molvir ? pm::kPairForcesEnergiesMV : kind))
     throw Fail{PM_EINVAL, "unsupported term combination"};
-  if (pool) HIPCHECK(hipEventRecord(next_event(c), c->stream));
-  if (timed) HIPCHECK(hipEventRecord(c->evp1, c->stream));
+  if (pool) HIPCHECK(hipEventRecord(next_event(c), L.stream));
+  if (timed) HIPCHECK(hipEventRecord(c->evp1, L.stream));
   if (c->recip && kind != pm::kPairCensus) {
-    if (timed) HIPCHECK(hipEventRecord(c->evr0, c->stream));
+    if (timed) HIPCHECK(hipEventRecord(c->evr0, L.stream));
     pm::launch_ewald(L, pm::kEwForces | (en ? ewald_energy_mode(c) : 0));
-    if (timed) HIPCHECK(hipEventRecord(c->evr1, c->stream));
+    if (timed) HIPCHECK(hipEventRecord(c->evr1, L.stream));
   }
   pm::launch_local(L, c->P, local);
 }

@@ -155,6 +155,9 @@ struct SysDev {
   int *orig;               // slot -> atom (-1 padding)
   int *smol;               // slot -> molecule (-1 padding)
   long long *fx, *fy, *fz; // fixed-point forces; fy = fx + f_stride, fz = fx + 2 f_stride
+  long long *fl;           // fixed-point forces of the local terms (bonded, Ewald correction),
+                           // original atom order, 3 * n SoA; every force pass of k_local
+                           // writes all of them, and the readers of fx add them in
   int f_stride;
   // clusters
   double *cl_origin;       // 3 * cl_cap SoA

@@ -682,11 +682,10 @@ __global__ __launch_bounds__(256) void k_local(SysDev *sys, const int *ap, int n
         v[5] += d[2] * fs * d[2];
       }
     }
-    if ((mode & kLocalForces) && ls[3] > ls[0]) {
-      const int sl = S.slot[a];
-      S.fx[sl] += to_fixed(f[0], kFScale);
-      S.fy[sl] += to_fixed(f[1], kFScale);
-      S.fz[sl] += to_fixed(f[2], kFScale);
+    if (mode & kLocalForces) {  // every atom, zero for one without local terms
+      S.fl[a] = to_fixed(f[0], kFScale);
+      S.fl[n + a] = to_fixed(f[1], kFScale);
+      S.fl[2 * n + a] = to_fixed(f[2], kFScale);
     }
   }
   if (!(mode & kLocalEnergies)) return;
@@ -762,6 +761,29 @@ __device__ __forceinline__ void lv_half(double v[3], const double F[3], double m
   for (int d = 0; d < 3; ++d) v[d] = v[d] * lv.vdt + F[d] * lv.fdt / m + g[d] * sf * lv.ndt;
 }
 
+// one system's step bookkeeping (one thread): counters, the census, and the
+// rebuild decision from the largest displacement since the last rebuild
+__device__ __forceinline__ void step_begin_ctl(SysDev &S, int mode, float lim2, int kmax) {
+  S.census[0] = 0;
+  S.census[1] = 0;
+  int *ctl = S.ctl;
+  ctl[kCtlFirst] = 0;  // the step's kick has run
+  ctl[kCtlNoise] += 1;
+  const unsigned d2 = *S.maxdisp2;
+  bool rb = (mode & kStepForce) != 0;
+  if (!rb && (mode & kStepAuto)) rb = __uint_as_float(d2) > lim2 || (kmax > 0 && ctl[kCtlSince] + 1 >= kmax);
+  *S.maxdisp_run = max(*S.maxdisp_run, d2);
+  *S.maxdisp2 = 0u;
+  ctl[kCtlRebuild] = rb ? 1 : 0;
+  if (rb) {
+    ctl[kCtlSince] = 0;
+    ctl[kCtlCount] += 1;
+    ctl[kCtlEntries] = 0;
+  } else {
+    ctl[kCtlSince] += 1;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_kick_drift(SysDev *sys, const int *ap, int nsys, double h, LangevinDev lv) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   float d2 = 0.f;
@@ -774,8 +796,8 @@ __global__ __launch_bounds__(256) void k_kick_drift(SysDev *sys, const int *ap, 
     const int sl = S.slot[a];
     const int mode = S.ctl[kCtlFirst] ? 0 : 1;
     const double m = 1.0 / S.inv_mass[a];
-    const double F[3] = {(double)S.fx[sl] * kFScaleInv, (double)S.fy[sl] * kFScaleInv,
-                         (double)S.fz[sl] * kFScaleInv};
+    const double F[3] = {(double)(S.fx[sl] + S.fl[a]) * kFScaleInv, (double)(S.fy[sl] + S.fl[n + a]) * kFScaleInv,
+                         (double)(S.fz[sl] + S.fl[2 * n + a]) * kFScaleInv};
     S.fx[sl] = 0;
     S.fy[sl] = 0;
     S.fz[sl] = 0;
@@ -838,8 +860,8 @@ __global__ void k_half_kick(SysDev *sys, const int *ap, int nsys, double hh, Lan
   const int a = g - ap[s], n = S.n_atoms;
   const int sl = S.slot[a];
   const double m = 1.0 / S.inv_mass[a];
-  const double F[3] = {(double)S.fx[sl] * kFScaleInv, (double)S.fy[sl] * kFScaleInv,
-                       (double)S.fz[sl] * kFScaleInv};
+  const double F[3] = {(double)(S.fx[sl] + S.fl[a]) * kFScaleInv, (double)(S.fy[sl] + S.fl[n + a]) * kFScaleInv,
+                       (double)(S.fz[sl] + S.fl[2 * n + a]) * kFScaleInv};
   if (lv.on) {
     // the closing half kick of the last step: noise counter 2c
     double v[3] = {S.v[a], S.v[n + a], S.v[2 * n + a]};
@@ -872,27 +894,7 @@ __global__ void k_step_begin(SysDev *sys, int nsys, int mode, float lim2, int km
   if (s >= nsys) return;
   SysDev &S = sys[s];
   for (int k = threadIdx.x; k < kAccN * kAccCopies; k += blockDim.x) S.acc[k] = 0;
-  if (threadIdx.x == 0) {
-    S.census[0] = 0;
-    S.census[1] = 0;
-    int *ctl = S.ctl;
-    ctl[kCtlFirst] = 0;  // the step's kick has run
-    ctl[kCtlNoise] += 1;
-    const unsigned d2 = *S.maxdisp2;
-    bool rb = (mode & kStepForce) != 0;
-    if (!rb && (mode & kStepAuto))
-      rb = __uint_as_float(d2) > lim2 || (kmax > 0 && ctl[kCtlSince] + 1 >= kmax);
-    *S.maxdisp_run = max(*S.maxdisp_run, d2);
-    *S.maxdisp2 = 0u;
-    ctl[kCtlRebuild] = rb ? 1 : 0;
-    if (rb) {
-      ctl[kCtlSince] = 0;
-      ctl[kCtlCount] += 1;
-      ctl[kCtlEntries] = 0;
-    } else {
-      ctl[kCtlSince] += 1;
-    }
-  }
+  if (threadIdx.x == 0) step_begin_ctl(S, mode, lim2, kmax);
 }
 
 // device state -> host staging (AoS FP64, per system); what: kXfer* bits
@@ -908,9 +910,9 @@ __global__ void k_export(SysDev *sys, const int *ap, int nsys, int what) {
     if (what & kXferV) S.vaos[3 * (size_t)a + d] = S.v[d * n + a];
   }
   if (what & kXferF) {
-    S.faos[3 * (size_t)a] = (double)S.fx[sl] * kFScaleInv;
-    S.faos[3 * (size_t)a + 1] = (double)S.fy[sl] * kFScaleInv;
-    S.faos[3 * (size_t)a + 2] = (double)S.fz[sl] * kFScaleInv;
+    S.faos[3 * (size_t)a] = (double)(S.fx[sl] + S.fl[a]) * kFScaleInv;
+    S.faos[3 * (size_t)a + 1] = (double)(S.fy[sl] + S.fl[n + a]) * kFScaleInv;
+    S.faos[3 * (size_t)a + 2] = (double)(S.fz[sl] + S.fl[2 * n + a]) * kFScaleInv;
   }
 }
 
