@@ -279,296 +279,267 @@ __device__ __forceinline__ int floordiv(int a, int b) {
   return (a % b != 0 && ((a < 0) != (b < 0))) ? q - 1 : q;
 }
 
-__device__ __forceinline__ int excl_lookup(const SysDev &S, int a, int b) {
-  int e0 = S.excl_start[a], e1 = S.excl_start[a + 1];
-  for (int k = e0; k < e1; ++k)
-    if (S.excl_partner[k] == b) return S.excl_class[k];
-  return 0;
-}
-
 // Super-cluster pair list within rlist, half shell over columns.  A
 // super-cluster is kScCl z-consecutive clusters of one column; its list holds
 // the j clusters within rlist of any of them, with an 8-bit mask of which.
-// One workgroup per column: the boxes of the j clusters in the column's window
-// of neighbour columns (periodic x/y image applied) are staged in LDS once;
-// each wave then tests its super-clusters' clusters against them, one z image
-// at a time.  Windows larger than the LDS stage take several passes.  A second
-// pass per super-cluster fills in the entries' FP32 image offsets and the
-// FULL/MODIFIED lane masks of cluster pairs that share a molecule.
-constexpr int kListThreads = 1024;
-constexpr int kListWaves = kListThreads / kWave;
-constexpr int kCandCap = 1280;   // staged j clusters per pass
-constexpr int kWinCap = 128;     // neighbour columns per pass
-constexpr int kColCap = kSortCap / kCluster;  // clusters of one column
-constexpr int kColSc = kColCap / kScCl;       // super-clusters of one column
-static_assert(kColCap < kCandCap, "a column's clusters must fit one LDS stage");
+// One wave per super-cluster.  The candidates are the clusters of the
+// neighbour columns (periodic x/y image applied) whose z extent, in one of the
+// three z images, can come within rlist of the super-cluster's z range: the
+// clusters of a column are z-ordered, so two binary searches bound each
+// (column, z image) segment.  The wave walks the segments in a fixed order, so
+// the list order is the same on every rebuild of the same positions.  A
+// second pass fills in the entries' FP32 image offsets and the FULL/MODIFIED
+// lane masks of cluster pairs that share a molecule.
+constexpr int kListWaves = 4;                     // waves (super-clusters) per workgroup
+constexpr int kListThreads = kListWaves * kWave;
+constexpr int kWinChunk = kWave;                  // neighbour columns per segment pass
+constexpr int kSegCap = 3 * kWinChunk;            // (column, z image) segments per pass
 
-struct ListScratch {  // per wave, in the candidate stage's LDS once pass A is over
-  int mol[kWave], at[kWave];
-  unsigned char rowF[kScCl][kScCl], rowM[kScCl][kScCl];
+struct ListScratch {  // per wave, pass B: each lane's entry's FULL and MODIFIED words per i cluster
+  unsigned long long mask[kWave][2 * kScCl];
 };
 
-__global__ __launch_bounds__(kListThreads) void k_pairlist(SysDev *sys, const int *colp, int nsys, float rlist2) {
-  __shared__ float4 c_lo[kCandCap], c_hi[kCandCap];
-  __shared__ int c_id[kCandCap];             // cj | (nvalid - 1) << 28
-  __shared__ int w_col[kWinCap], w_start[kWinCap + 1], w_sxy[kWinCap];
-  __shared__ int n_win, next_q, sc_count[kColSc];
+// first k in [0, m) with key(k) >= v (m if none), keys ascending
+__device__ __forceinline__ int lower_bound_hi_z(const float4 *hi, int c0, int m, float v) {
+  int lo = 0, hi_i = m;
+  while (lo < hi_i) {
+    const int mid = (lo + hi_i) >> 1;
+    if (hi[c0 + mid].z > v) hi_i = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+// first k in [0, m) with lo(k).z >= v (m if none)
+__device__ __forceinline__ int lower_bound_lo_z(const float4 *lo_b, int c0, int m, float v) {
+  int lo = 0, hi_i = m;
+  while (lo < hi_i) {
+    const int mid = (lo + hi_i) >> 1;
+    if (lo_b[c0 + mid].z >= v) hi_i = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(kListThreads) void k_pairlist(SysDev *sys, const int *scp, int nsys, float rlist2) {
   __shared__ float4 ib_lo[kListWaves][kScCl], ib_hi[kListWaves][kScCl];
-  static_assert(sizeof(ListScratch) * kListWaves <= sizeof(float4) * kCandCap, "scratch fits the stage");
-  const int gcol = blockIdx.x;
-  if (gcol >= colp[nsys]) return;
-  const int s = find_sys(colp, nsys, gcol);
+  __shared__ int seg_start[kListWaves][kSegCap + 1], seg_cl[kListWaves][kSegCap], seg_sh[kListWaves][kSegCap];
+  __shared__ ListScratch scratch[kListWaves];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int scg = blockIdx.x * kListWaves + wv;
+  if (scg >= scp[nsys]) return;
+  const int s = find_sys(scp, nsys, scg);
   SysDev &S = sys[s];
   if (!S.ctl[kCtlRebuild]) return;
-  const int coli = gcol - colp[s];
-  const int ci0 = S.col_cl_start[coli], nci = S.col_ncl[coli];
-  if (nci == 0) return;
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  if (nci > kColCap) {  // the column sort has already flagged it
-    if (t == 0) atomicOr(S.err, (unsigned)kErrColumnCap);
-    return;
-  }
-  const int nsc = S.col_nsc[coli], sc0 = S.col_sc_start[coli];
+  const int sc = scg - scp[s];
+  if (sc >= *S.n_sc) return;
+  const int coli = S.sc_col[sc];
+  const int kz = sc - S.col_sc_start[coli];
+  const int cl0 = S.col_cl_start[coli] + kz * kScCl;
+  const int nic = min(kScCl, S.col_ncl[coli] - kz * kScCl);
   const int cxi = coli % S.ncx, cyi = coli / S.ncx;
   const float Lx = (float)S.L[0], Ly = (float)S.L[1], Lz = (float)S.L[2];
   const float ax = (float)S.ax, ay = (float)S.ay;
   const float ix0 = cxi * ax, ix1 = (cxi + 1) * ax, iy0 = cyi * ay, iy1 = (cyi + 1) * ay;
+  const float rl = sqrtf(rlist2);
   const int cap = S.list_cap;
   const int wx = 2 * S.rx + 1, nwin_all = wx * (2 * S.ry + 1);
-  for (int k = t; k < nsc; k += kListThreads) sc_count[k] = 0;
-  if (t == 0) next_q = 0;
-  __syncthreads();
-  // ---- pass A: entries (j cluster, image, i-cluster mask)
-  while (true) {
-    if (t == 0) {  // this pass's neighbour columns, in window order
-      int q = next_q, nw = 0, tot = 0;
-      w_start[0] = 0;
-      for (; q < nwin_all && nw < kWinCap; ++q) {
-        const int dy = q / wx - S.ry, dx = q % wx - S.rx;
-        if (dy < 0 || (dy == 0 && dx < 0)) continue;  // half shell over columns
+  if (lane < kScCl) {
+    const bool ok = lane < nic && S.cl_nvalid[cl0 + lane] > 0;
+    ib_lo[wv][lane] = ok ? S.cl_lo[cl0 + lane] : make_float4(1e30f, 1e30f, 1e30f, 0.f);
+    ib_hi[wv][lane] = ok ? S.cl_hi[cl0 + lane] : make_float4(1e30f, 1e30f, 1e30f, 0.f);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  float zlo = 1e30f, zhi = -1e30f;
+  for (int c = 0; c < nic; ++c) {
+    zlo = fminf(zlo, ib_lo[wv][c].z);
+    zhi = fmaxf(zhi, ib_hi[wv][c].z);
+  }
+  uint4 *out = S.list + 2 * (size_t)sc * cap;
+  int count = 0;
+  // ---- pass A: entries (j cluster, image, i-cluster mask), segment by segment
+  for (int q0 = 0; q0 < nwin_all; q0 += kWinChunk) {
+    // lane q: one neighbour column of the window and its three z-image segments
+    const int q = q0 + lane;
+    int colj = 0, sxy = 0, cs = 0, ncj = 0;
+    bool col_ok = false;
+    if (q < nwin_all) {
+      const int dy = q / wx - S.ry, dx = q % wx - S.rx;
+      if (!(dy < 0 || (dy == 0 && dx < 0))) {  // half shell over columns
         const int cyu = cyi + dy, cxu = cxi + dx;
         const float gy = fmaxf(0.f, fmaxf(cyu * ay - iy1, iy0 - (cyu + 1) * ay) - 0.01f);
         const float gx = fmaxf(0.f, fmaxf(cxu * ax - ix1, ix0 - (cxu + 1) * ax) - 0.01f);
-        if (gx * gx + gy * gy >= rlist2) continue;
-        const int sy = floordiv(cyu, S.ncy), sx = floordiv(cxu, S.ncx);
-        const int colj = (cyu - sy * S.ncy) * S.ncx + (cxu - sx * S.ncx);
-        const int nc = S.col_ncl[colj];
-        if (tot + nc > kCandCap) break;  // next pass (nc <= kColCap < kCandCap: a pass never ends empty)
-        w_col[nw] = colj;
-        w_sxy[nw] = (sx + 1) | ((sy + 1) << 2) | ((dx == 0 && dy == 0) ? 16 : 0);
-        tot += nc;
-        w_start[++nw] = tot;
-      }
-      n_win = nw;
-      next_q = q;
-    }
-    __syncthreads();
-    const int nw = n_win, ncand = w_start[nw];
-    for (int k = t; k < ncand; k += kListThreads) {
-      int w = 0;
-      while (w_start[w + 1] <= k) ++w;
-      const int cj = S.col_cl_start[w_col[w]] + (k - w_start[w]);
-      const int sxy = w_sxy[w];
-      const float sxL = ((sxy & 3) - 1) * Lx, syL = (((sxy >> 2) & 3) - 1) * Ly;
-      const int nvj = S.cl_nvalid[cj];
-      float4 lo = S.cl_lo[cj], hi = S.cl_hi[cj];
-      lo.x += sxL; hi.x += sxL;
-      lo.y += syL; hi.y += syL;
-      lo.w = __int_as_float(sxy);
-      if (nvj == 0) lo.x = 1e30f;  // never within range
-      c_lo[k] = lo;
-      c_hi[k] = hi;
-      c_id[k] = cj | (max(nvj - 1, 0) << 28);
-    }
-    __syncthreads();
-    for (int scl = wv; scl < nsc; scl += kListWaves) {
-      const int cl0 = ci0 + scl * kScCl, nic = min(kScCl, nci - scl * kScCl);
-      if (lane < kScCl) {
-        const bool ok = lane < nic && S.cl_nvalid[cl0 + lane] > 0;
-        ib_lo[wv][lane] = ok ? S.cl_lo[cl0 + lane] : make_float4(1e30f, 1e30f, 1e30f, 0.f);
-        ib_hi[wv][lane] = ok ? S.cl_hi[cl0 + lane] : make_float4(1e30f, 1e30f, 1e30f, 0.f);
-      }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      float zlo = 1e30f, zhi = -1e30f;
-      for (int c = 0; c < nic; ++c) {
-        zlo = fminf(zlo, ib_lo[wv][c].z);
-        zhi = fmaxf(zhi, ib_hi[wv][c].z);
-      }
-      uint4 *out = S.list + 2 * (size_t)(sc0 + scl) * cap;
-      int count = sc_count[scl];
-      for (int sz = -1; sz <= 1; ++sz) {
-        // a z image is a candidate only if the super-cluster's z range comes
-        // within rlist of that image of the wrapped frame [0, Lz)
-        if (sz == 1 && fmaxf(0.f, Lz - zhi - 0.01f) * fmaxf(0.f, Lz - zhi - 0.01f) >= rlist2) continue;
-        if (sz == -1 && fmaxf(0.f, zlo - 0.01f) * fmaxf(0.f, zlo - 0.01f) >= rlist2) continue;
-        const float szL = sz * Lz;
-        for (int base = 0; base < ncand; base += kWave) {
-          const int k = base + lane;
-          unsigned im = 0;
-          int cj = 0, sxy = 0, idw = 0;
-          if (k < ncand) {
-            const float4 loj = c_lo[k], hij = c_hi[k];
-            idw = c_id[k];
-            cj = idw & 0x7FFFFF;
-            sxy = __float_as_int(loj.w);
-            const bool samecol = (sxy & 16) != 0;
-            for (int c = 0; c < nic; ++c) {
-              // each cluster pair once: within a column, j at or above i (z image 0)
-              if (samecol && !(sz > 0 || (sz == 0 && cj >= cl0 + c))) continue;
-              const float4 loi = ib_lo[wv][c], hii = ib_hi[wv][c];
-              const float ex = fmaxf(0.f, fmaxf(loj.x - hii.x, loi.x - hij.x));
-              const float ey = fmaxf(0.f, fmaxf(loj.y - hii.y, loi.y - hij.y));
-              const float ez = fmaxf(0.f, fmaxf(loj.z + szL - hii.z, loi.z - (hij.z + szL)));
-              if (ex * ex + ey * ey + ez * ez < rlist2) im |= 1u << c;
-            }
-          }
-          const bool hit = im != 0;
-          const unsigned long long b = wave_ballot(hit);
-          if (b == 0) continue;
-          const int pos = count + __popcll(b & ((1ull << lane) - 1ull));
-          if (hit && pos < cap) {
-            // the image offset and masks are filled in by pass B
-            const int sh = (sxy & 3) * 9 + ((sxy >> 2) & 3) * 3 + (sz + 1);
-            out[2 * pos].x = (unsigned)cj | ((unsigned)sh << 23) | ((unsigned)idw & 0x70000000u);
-            out[2 * pos + 1] = make_uint4(im, 0u, 0u, 0u);
-          }
-          count += __popcll(b);
+        if (gx * gx + gy * gy < rlist2) {
+          const int sy = floordiv(cyu, S.ncy), sx = floordiv(cxu, S.ncx);
+          colj = (cyu - sy * S.ncy) * S.ncx + (cxu - sx * S.ncx);
+          sxy = (sx + 1) | ((sy + 1) << 2) | ((dx == 0 && dy == 0) ? 16 : 0);
+          cs = S.col_cl_start[colj];
+          ncj = S.col_ncl[colj];
+          col_ok = true;
         }
       }
-      if (lane == 0) sc_count[scl] = count;
     }
-    __syncthreads();
-    if (next_q >= nwin_all) break;
-    __syncthreads();
-  }
-  // ---- pass B, per super-cluster: offsets, exclusion masks, lengths
-  ListScratch &W = reinterpret_cast<ListScratch *>(c_lo)[wv];
-  long long pairs8 = 0;  // 8x8 cluster pairs, for the statistics
-  for (int scl = wv; scl < nsc; scl += kListWaves) {
-    const int sc = sc0 + scl;
-    const int cl0 = ci0 + scl * kScCl, nic = min(kScCl, nci - scl * kScCl);
-    int count = sc_count[scl];
-    if (count > cap) {
-      if (lane == 0) atomicOr(S.err, (unsigned)kErrListCap);
-      count = cap;
-    }
-    uint4 *out = S.list + 2 * (size_t)sc * cap;
-    const double ox = S.cl_origin[cl0], oy = S.cl_origin[S.cl_cap + cl0], oz = S.cl_origin[2 * S.cl_cap + cl0];
-    {
-      const int c = lane >> 3, ii = lane & 7;
-      int mol = -1, at = -1;
-      if (c < nic) {
-        const int si = (cl0 + c) * kCluster + ii;
-        mol = S.smol[si];
-        at = S.orig[si];
+    // segments in (z image, window column) order
+    int nseg = 0, ntot = 0;
+    for (int sz = -1; sz <= 1; ++sz) {
+      const float szL = sz * Lz;
+      int k0 = 0, k1 = 0;
+      if (col_ok) {
+        // a candidate's box in this image must reach above zlo - rl and below zhi + rl
+        // (0.01 A of slack: the boxes are z-ordered up to FP32 rounding of the sort keys)
+        k0 = lower_bound_hi_z(S.cl_hi, cs, ncj, zlo - rl - szL - 0.01f);
+        k1 = lower_bound_lo_z(S.cl_lo, cs, ncj, zhi + rl - szL + 0.01f);
+        if (k1 < k0) k1 = k0;
       }
-      W.mol[lane] = mol;
-      W.at[lane] = at;
+      const int cnt = k1 - k0;
+      const unsigned long long has = wave_ballot(cnt > 0);
+      // inclusive prefix of cnt over the lanes (wave scan)
+      int pre = cnt;
+      for (int o = 1; o < kWave; o <<= 1) {
+        const int v = __shfl_up(pre, o);
+        if (lane >= o) pre += v;
+      }
+      const int tot = __shfl(pre, kWave - 1);
+      if (cnt > 0) {
+        const int slot = nseg + __popcll(has & ((1ull << lane) - 1ull));
+        seg_start[wv][slot] = ntot + pre - cnt;
+        seg_cl[wv][slot] = cs + k0;
+        seg_sh[wv][slot] = sxy | ((sz + 1) << 5);
+      }
+      nseg += __popcll(has);
+      ntot += tot;
     }
+    if (lane == 0) seg_start[wv][nseg] = ntot;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    int mslot = 0;
-    unsigned long long *mbase = S.list_mask + (size_t)sc * S.mask_cap * (2 * kScCl);
-    for (int e0 = 0; e0 < count; e0 += kWave) {
-      const int e = e0 + lane;
-      unsigned share = 0;
-      int cj = 0;
-      if (e < count) {
-        const unsigned x = out[2 * e].x;
-        const unsigned im = out[2 * e + 1].x & 0xFFu;
-        cj = (int)(x & 0x7FFFFFu);
-        const int sh = (int)((x >> 23) & 31u);
-        const int sx = sh / 9 - 1, sy = (sh / 3) % 3 - 1, sz = sh % 3 - 1;
-        const double ojx = S.cl_origin[cj] + (double)sx * S.L[0];
-        const double ojy = S.cl_origin[S.cl_cap + cj] + (double)sy * S.L[1];
-        const double ojz = S.cl_origin[2 * S.cl_cap + cj] + (double)sz * S.L[2];
-        out[2 * e].y = __float_as_uint((float)(ojx - ox));
-        out[2 * e].z = __float_as_uint((float)(ojy - oy));
-        out[2 * e].w = __float_as_uint((float)(ojz - oz));
-        pairs8 += __popc(im);
-        const int4 m0 = *reinterpret_cast<const int4 *>(S.smol + cj * kCluster);
-        const int4 m1 = *reinterpret_cast<const int4 *>(S.smol + cj * kCluster + 4);
-        const int mj[kCluster] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
-        for (int c = 0; c < kScCl; ++c) {
-          bool sh_c = false;
-          for (int b = 0; b < kCluster; ++b) {
-            const int mi = W.mol[c * kCluster + b];
-            for (int a = 0; a < kCluster; ++a) sh_c = sh_c || (mi >= 0 && mj[a] == mi);
-          }
-          if (sh_c && ((im >> c) & 1u)) share |= 1u << c;
+    for (int base = 0; base < ntot; base += kWave) {
+      const int k = base + lane;
+      unsigned im = 0;
+      int cj = 0, shw = 0;
+      if (k < ntot) {
+        int lo = 0, hi = nseg;  // seg_start[lo] <= k < seg_start[hi]
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (seg_start[wv][mid] <= k) lo = mid;
+          else hi = mid;
         }
+        cj = seg_cl[wv][lo] + (k - seg_start[wv][lo]);
+        shw = seg_sh[wv][lo];
+        const int sz = ((shw >> 5) & 3) - 1;
+        const bool samecol = (shw & 16) != 0;
+        const float sxL = ((shw & 3) - 1) * Lx, syL = (((shw >> 2) & 3) - 1) * Ly, szL = sz * Lz;
+        float4 loj = S.cl_lo[cj], hij = S.cl_hi[cj];
+        loj.x += sxL;
+        hij.x += sxL;
+        loj.y += syL;
+        hij.y += syL;
+        loj.z += szL;
+        hij.z += szL;
+        if (S.cl_nvalid[cj] > 0)
+          for (int c = 0; c < nic; ++c) {
+            // each cluster pair once: within a column, j at or above i (z image 0)
+            if (samecol && !(sz > 0 || (sz == 0 && cj >= cl0 + c))) continue;
+            const float4 loi = ib_lo[wv][c], hii = ib_hi[wv][c];
+            const float ex = fmaxf(0.f, fmaxf(loj.x - hii.x, loi.x - hij.x));
+            const float ey = fmaxf(0.f, fmaxf(loj.y - hii.y, loi.y - hij.y));
+            const float ez = fmaxf(0.f, fmaxf(loj.z - hii.z, loi.z - hij.z));
+            if (ex * ex + ey * ey + ez * ez < rlist2) im |= 1u << c;
+          }
       }
-      unsigned long long flagged = wave_ballot(share != 0);
-      while (flagged) {
-        const int g = __ffsll((long long)flagged) - 1;
-        flagged &= flagged - 1;
-        const unsigned sh_e = __shfl(share, g);
-        const int cj_e = __shfl(cj, g);
-        const int ee = e0 + g;
-        if (mslot >= S.mask_cap) {
-          if (lane == 0) atomicOr(S.err, (unsigned)kErrListCap);
-          continue;
-        }
-        // lanes (k, jj): the k-th i cluster sharing a molecule with j cluster cj_e
-        const int k = lane >> 3, jj = lane & 7;
-        const int ncomb = __popc(sh_e);
-        int ck = -1;
-        {
-          unsigned m = sh_e;
-          for (int r = 0; r < k && m; ++r) m &= m - 1;
-          if (m && k < ncomb) ck = __ffs(m) - 1;
-        }
-        unsigned rf = 0, rm = 0;
-        if (ck >= 0) {
-          const int sj = cj_e * kCluster + jj;
-          const int mj = S.smol[sj], bj = S.orig[sj];
-          // j atom's exclusions as index-window bits (one load pair per lane);
-          // an atom with a partner outside the window looks up the CSR
-          const unsigned long long wf = bj >= 0 ? S.excl_win[2 * (size_t)bj] : 0ull;
-          const unsigned long long wm = bj >= 0 ? S.excl_win[2 * (size_t)bj + 1] : 0ull;
-          const bool far = (wf >> 63) != 0;
-          for (int ii = 0; ii < kCluster; ++ii) {
-            const int mi = W.mol[ck * kCluster + ii], ai = W.at[ck * kCluster + ii];
-            if (mi >= 0 && mi == mj && ai != bj) {
-              const int dd = ai - bj + 31;
-              const bool inw = dd >= 0 && dd < 63;
-              const int cls = far ? excl_lookup(S, ai, bj)
-                                  : (inw && ((wf >> dd) & 1ull) ? PM_EXCL_FULL
-                                     : (inw && ((wm >> dd) & 1ull) ? PM_EXCL_MODIFIED : 0));
-              if (cls == PM_EXCL_FULL) rf |= 1u << ii;
-              if (cls == PM_EXCL_MODIFIED) rm |= 1u << ii;
-            }
+      const bool hit = im != 0;
+      const unsigned long long b = wave_ballot(hit);
+      if (b == 0) continue;
+      const int pos = count + __popcll(b & ((1ull << lane) - 1ull));
+      if (hit && pos < cap) {
+        // the image offset and masks are filled in by pass B
+        const int sz = ((shw >> 5) & 3) - 1;
+        const int sh = (shw & 3) * 9 + ((shw >> 2) & 3) * 3 + (sz + 1);
+        const int nvj = S.cl_nvalid[cj];
+        out[2 * pos].x = (unsigned)cj | ((unsigned)sh << 23) | ((unsigned)max(nvj - 1, 0) << 28);
+        out[2 * pos + 1] = make_uint4(im, 0u, 0u, 0u);
+      }
+      count += __popcll(b);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  // ---- pass B: offsets, exclusion masks, length
+  __threadfence_block();  // pass A's entries (global stores of other lanes) are read back below
+  long long pairs8 = 0;  // 8x8 cluster pairs, for the statistics
+  if (count > cap) {
+    if (lane == 0) atomicOr(S.err, (unsigned)kErrListCap);
+    count = cap;
+  }
+  const double ox = S.cl_origin[cl0], oy = S.cl_origin[S.cl_cap + cl0], oz = S.cl_origin[2 * S.cl_cap + cl0];
+  int mslot = 0;
+  unsigned long long *mbase = S.list_mask + (size_t)sc * S.mask_cap * (2 * kScCl);
+  unsigned long long *M = scratch[wv].mask[lane];  // this lane's entry: FULL, MODIFIED word per i cluster
+  for (int e0 = 0; e0 < count; e0 += kWave) {
+    const int e = e0 + lane;
+    unsigned share = 0;
+    for (int w = 0; w < 2 * kScCl; ++w) M[w] = 0ull;
+    if (e < count) {
+      const unsigned x = out[2 * e].x;
+      const unsigned im = out[2 * e + 1].x & 0xFFu;
+      const int cj = (int)(x & 0x7FFFFFu);
+      const int sh = (int)((x >> 23) & 31u);
+      const int sx = sh / 9 - 1, sy = (sh / 3) % 3 - 1, sz = sh % 3 - 1;
+      const double ojx = S.cl_origin[cj] + (double)sx * S.L[0];
+      const double ojy = S.cl_origin[S.cl_cap + cj] + (double)sy * S.L[1];
+      const double ojz = S.cl_origin[2 * S.cl_cap + cj] + (double)sz * S.L[2];
+      out[2 * e].y = __float_as_uint((float)(ojx - ox));
+      out[2 * e].z = __float_as_uint((float)(ojy - oy));
+      out[2 * e].w = __float_as_uint((float)(ojz - oz));
+      pairs8 += __popc(im);
+      // the excluded pairs of the entry's tiles: each j atom's exclusion
+      // partners (its index window, or the symmetric CSR for an atom with a
+      // partner outside it), located by slot -- the super-cluster's i clusters
+      // are the contiguous cl0 .. cl0 + nic - 1 -- and kept when the partner
+      // is in a tile of the entry and in the same molecule; lane bit jj * 8 + ii
+      // of the tile's FULL or MODIFIED word
+      for (int jj = 0; jj < kCluster; ++jj) {
+        const int sj = cj * kCluster + jj;
+        const int bj = S.orig[sj];
+        if (bj < 0) continue;
+        const int mj = S.smol[sj];
+        const unsigned long long wf = S.excl_win[2 * (size_t)bj], wm = S.excl_win[2 * (size_t)bj + 1];
+        auto partner = [&](int b, int cls) {
+          const int sb = S.slot[b];
+          const int c = (sb >> 3) - cl0;
+          if (c >= 0 && c < nic && ((im >> c) & 1u) && S.smol[sb] == mj &&
+              (cls == PM_EXCL_FULL || cls == PM_EXCL_MODIFIED)) {
+            share |= 1u << c;
+            M[2 * c + (cls == PM_EXCL_MODIFIED ? 1 : 0)] |= 1ull << (jj * kCluster + (sb & (kCluster - 1)));
           }
-          W.rowF[k][jj] = (unsigned char)rf;
-          W.rowM[k][jj] = (unsigned char)rm;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // 16 words of the slot: FULL, MODIFIED lane masks (bit jj * 8 + ii) per i cluster
-        if (lane < 2 * kScCl) {
-          const int c = lane >> 1;
-          unsigned long long v = 0;
-          if ((sh_e >> c) & 1u) {
-            const int kc = __popc(sh_e & ((1u << c) - 1u));
-            v = (lane & 1) ? *reinterpret_cast<const unsigned long long *>(W.rowM[kc])
-                           : *reinterpret_cast<const unsigned long long *>(W.rowF[kc]);
+        };
+        if ((wf >> 63) != 0) {
+          for (int k = S.excl_start[bj]; k < S.excl_start[bj + 1]; ++k) partner(S.excl_partner[k], S.excl_class[k]);
+        } else {
+          unsigned long long bits = (wf | wm) & ~(1ull << 63);
+          while (bits) {
+            const int d = __ffsll((long long)bits) - 1;
+            bits &= bits - 1;
+            partner(bj + d - 31, ((wf >> d) & 1ull) ? PM_EXCL_FULL : PM_EXCL_MODIFIED);
           }
-          mbase[(size_t)mslot * (2 * kScCl) + lane] = v;
         }
-        if (lane == 0) {
-          out[2 * ee].x |= 0x80000000u;
-          out[2 * ee + 1].x |= (unsigned)mslot << 8;
-        }
-        ++mslot;
-        __builtin_amdgcn_wave_barrier();
       }
     }
-    if (lane == 0) S.list_len[sc] = count;
+    // mask slots in entry order
+    const unsigned long long fl = wave_ballot(share != 0);
+    const int my = mslot + __popcll(fl & ((1ull << lane) - 1ull));
+    if (share != 0) {
+      if (my < S.mask_cap) {
+        for (int w = 0; w < 2 * kScCl; ++w) mbase[(size_t)my * (2 * kScCl) + w] = M[w];
+        out[2 * e].x |= 0x80000000u;
+        out[2 * e + 1].x |= (unsigned)my << 8;
+      } else {
+        atomicOr(S.err, (unsigned)kErrListCap);
+      }
+    }
+    mslot += __popcll(fl);
   }
+  if (lane == 0) S.list_len[sc] = count;
   pairs8 = wave_sum_ll(pairs8);
   if (lane == 0 && pairs8) atomicAdd(&S.ctl[kCtlEntries], (int)pairs8);
 }
@@ -964,7 +935,8 @@ void launch_rebuild(const Launch &L, float rlist2) {
   hipLaunchKernelGGL(k_col_fill, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys);
   hipLaunchKernelGGL(k_col_sort, dim3(L.n_cols_total), dim3(1024), 0, L.stream, L.sys, L.col_prefix, L.nsys);
   hipLaunchKernelGGL(k_cluster_geom, dim3(nblk(L.n_cl_total, 128)), dim3(128), 0, L.stream, L.sys, L.cl_prefix, L.nsys);
-  hipLaunchKernelGGL(k_pairlist, dim3(L.n_cols_total), dim3(kListThreads), 0, L.stream, L.sys, L.col_prefix, L.nsys, rlist2);
+  hipLaunchKernelGGL(k_pairlist, dim3(nblk(L.n_sc_total, kListWaves)), dim3(kListThreads), 0, L.stream, L.sys,
+                     L.sc_prefix, L.nsys, rlist2);
 }
 
 bool launch_pair(const Launch &L, const ForceDev &P, int kind) {
