@@ -688,9 +688,6 @@ size_t carve(const pm_ctx *c, const SysPrep &p, const pm_system_desc &d, Carver 
   S.faos = cv.take<double>(3 * (size_t)n);
   S.slot = cv.take<int>(n);
   S.lbuild = cv.take<float4>(n);
-  S.bin_col = cv.take<int>(n);
-  S.bin_z = cv.take<float>(n);
-  S.colbuf = cv.take<unsigned long long>(n);
   S.bonds = cv.take<int>(2 * (size_t)p.n_bonds);
   S.bond_k = cv.take<double>(p.n_bonds);
   S.bond_r0 = cv.take<double>(p.n_bonds);
@@ -734,8 +731,7 @@ size_t carve(const pm_ctx *c, const SysPrep &p, const pm_system_desc &d, Carver 
     return cv.off;
   }
   S.col_count = cv.take<int>(p.ncol);
-  S.col_cursor = cv.take<int>(p.ncol);
-  S.col_atom_start = cv.take<int>(p.ncol);
+  S.bucket = cv.take<unsigned long long>((size_t)p.ncol * pm::kSortCap);
   S.col_cl_start = cv.take<int>(p.ncol);
   S.col_ncl = cv.take<int>(p.ncol);
   S.ljt = cv.take<int>(p.slot_cap);
@@ -1109,7 +1105,8 @@ int ewald_energy_mode(const pm_ctx *c) {
 }
 
 void force_eval(pm_ctx *c, const Launch &L, int mode, const StepPolicy &pol, int kind, bool gb_energies, bool timed) {
-  pm::launch_step_begin(L, mode, pol.lim2, pol.kmax);
+  // the cutoff path's list check and bookkeeping ride on the rebuild kernels
+  if (c->gb) pm::launch_step_begin(L, mode, pol.lim2, pol.kmax);
   const bool pool = !timed && (c->desc.flags & PM_FLAG_TIMING);
   if (mode & pm::kStepForce) pm::launch_zero_forces(L);
   const bool en = kind == pm::kPairForcesEnergies;
@@ -1126,7 +1123,7 @@ void force_eval(pm_ctx *c, const Launch &L, int mode, const StepPolicy &pol, int
     pm::launch_local(L, c->P, local);
     return;
   }
-  pm::launch_rebuild(L, pol.rlist * pol.rlist);
+  pm::launch_rebuild(L, pol.rlist * pol.rlist, mode, pol.lim2, pol.kmax);
   if (timed) HIPCHECK(hipEventRecord(c->evp0, L.stream));
   if (pool) HIPCHECK(hipEventRecord(next_event(c), L.stream));
   if (!pm::launch_pair(L, c->P, en && c->molvir ? pm::kPairForcesEnergiesMV : kind))

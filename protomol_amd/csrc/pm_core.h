@@ -140,14 +140,12 @@ struct SysDev {
   const unsigned long long *excl_win;  // 2 per atom: FULL / MODIFIED bits over partners a-31..a+31; FULL bit 63: far partner
   int *slot;               // atom -> slot
   float4 *lbuild;          // local coordinates at the last rebuild (displacement check)
-  // binning scratch
-  int *col_count, *col_cursor, *col_atom_start, *col_cl_start, *col_ncl;
+  // binning: per column a bucket of kSortCap (zkey << 32 | atom) keys and its count
+  int *col_count, *col_cl_start, *col_ncl;
   int *col_sc_start, *col_nsc;  // per column: first super-cluster, count
   int *sc_col;             // super-cluster -> column
   int sc_cap;
-  int *bin_col;            // atom -> column
-  float *bin_z;            // atom -> wrapped z
-  unsigned long long *colbuf; // column-ordered (zkey << 32 | atom)
+  unsigned long long *bucket;
   // slot order
   float4 *xq;              // local coordinates + scaled charge
   int *ljt;                // LJ type (table mode) or -1 padding

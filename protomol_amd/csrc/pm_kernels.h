@@ -36,7 +36,9 @@ void launch_import(const Launch &L, int what);
 void launch_export(const Launch &L, int what);
 void launch_gather_scalars(const Launch &L, long long *out);
 void launch_clear_err(const Launch &L);
-void launch_rebuild(const Launch &L, float rlist2);
+// the list check of a force pass (the bookkeeping of launch_step_begin with
+// the same mode, lim2, kmax) and, where it decides so, the pair-list rebuild
+void launch_rebuild(const Launch &L, float rlist2, int mode, float lim2, int kmax);
 bool launch_pair(const Launch &L, const ForceDev &P, int kind);
 // the pair kernels of one term combination each (pm_pair_*.hip)
 void launch_pair_lj(const Launch &L, const ForceDev &P, int kind);

@@ -34,36 +34,97 @@ __global__ void k_import(SysDev *sys, const int *ap, int nsys, int what) {
   }
 }
 
-// Lanes of a wave that share a (system, column) key are served by one atomic:
-// the wave peels off the group of the first remaining lane until every lane is
-// done.  Atoms in original order are spatially coherent, so a wave usually
-// holds only a few keys.
-__global__ void k_bin(SysDev *sys, const int *ap, int nsys) {
+// The list check of a force pass (the Verlet criterion: rebuild when an atom
+// moved more than lim from where the list was built, or every kmax steps, or
+// always with kStepForce), from values no kernel of the rebuild changes
+__device__ __forceinline__ bool rebuild_due(const SysDev &S, int mode, float lim2, int kmax) {
+  if (mode & kStepForce) return true;
+  if (!(mode & kStepAuto)) return false;
+  return __uint_as_float(*S.maxdisp2) > lim2 || (kmax > 0 && S.ctl[kCtlSince] + 1 >= kmax);
+}
+
+// one system's step bookkeeping (one thread): counters, the census, and the
+// rebuild decision from the largest displacement since the last rebuild
+__device__ __forceinline__ void step_begin_ctl(SysDev &S, int mode, float lim2, int kmax) {
+  S.census[0] = 0;
+  S.census[1] = 0;
+  int *ctl = S.ctl;
+  ctl[kCtlFirst] = 0;  // the step's kick has run
+  ctl[kCtlNoise] += 1;
+  const unsigned d2 = *S.maxdisp2;
+  const bool rb = rebuild_due(S, mode, lim2, kmax);
+  *S.maxdisp_run = max(*S.maxdisp_run, d2);
+  *S.maxdisp2 = 0u;
+  ctl[kCtlRebuild] = rb ? 1 : 0;
+  if (rb) {
+    ctl[kCtlSince] = 0;
+    ctl[kCtlCount] += 1;
+    ctl[kCtlEntries] = 0;
+  } else {
+    ctl[kCtlSince] += 1;
+  }
+}
+
+// the rest of a step's bookkeeping once k_bin has published the decision in
+// ctl[kCtlRebuild] (one thread per system)
+__device__ __forceinline__ void step_finish_ctl(SysDev &S) {
+  int *ctl = S.ctl;
+  ctl[kCtlFirst] = 0;  // the step's kick has run
+  ctl[kCtlNoise] += 1;
+  const unsigned d2 = *S.maxdisp2;
+  *S.maxdisp_run = max(*S.maxdisp_run, d2);
+  *S.maxdisp2 = 0u;
+  if (ctl[kCtlRebuild]) {
+    ctl[kCtlSince] = 0;
+    ctl[kCtlCount] += 1;
+    ctl[kCtlEntries] = 0;
+  } else {
+    ctl[kCtlSince] += 1;
+  }
+}
+
+// Rebuild, step 1 of 3, one thread per atom.  It also starts the force pass:
+// every system's thread of atom 0 publishes the list check's decision and
+// clears the census, and the first threads clear the energy accumulators.
+// When the list is rebuilt, the atom's (z, index) key goes to its x-y
+// column's bucket.  Lanes of a wave that share a (system, column) key are
+// served by one atomic: the wave peels off the group of the first remaining
+// lane until every lane is done.  Atoms in original order are spatially
+// coherent, so a wave usually holds only a few keys.
+__global__ void k_bin(SysDev *sys, const int *ap, int nsys, int mode, float lim2, int kmax) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   bool todo = g < ap[nsys];
   int s = 0, col = 0;
+  unsigned long long key = 0;
   if (todo) {
     s = find_sys(ap, nsys, g);
-    todo = sys[s].ctl[kCtlRebuild] != 0;
-  }
-  if (todo) {
     SysDev &S = sys[s];
     const int a = g - ap[s], n = S.n_atoms;
-    double x = S.x[a], y = S.x[n + a], z = S.x[2 * n + a];
-    if (!(isfinite(x) && isfinite(y) && isfinite(z))) {
-      atomicOr(S.err, (unsigned)kErrNonFinite);
-      x = y = z = 0.0;
+    const bool rb = rebuild_due(S, mode, lim2, kmax);
+    for (int k = a; k < kAccN * kAccCopies; k += n) S.acc[k] = 0;
+    if (a == 0) {
+      S.ctl[kCtlRebuild] = rb ? 1 : 0;
+      S.census[0] = 0;
+      S.census[1] = 0;
     }
-    const double rx = wrap_rel(x, S.omin[0], S.L[0]);
-    const double ry = wrap_rel(y, S.omin[1], S.L[1]);
-    const double rz = wrap_rel(z, S.omin[2], S.L[2]);
-    int cx = (int)(rx / S.ax), cy = (int)(ry / S.ay);
-    cx = cx < 0 ? 0 : (cx >= S.ncx ? S.ncx - 1 : cx);
-    cy = cy < 0 ? 0 : (cy >= S.ncy ? S.ncy - 1 : cy);
-    col = cy * S.ncx + cx;
-    S.bin_col[a] = col;
-    S.bin_z[a] = (float)rz;
+    todo = rb;
+    if (rb) {
+      double x = S.x[a], y = S.x[n + a], z = S.x[2 * n + a];
+      if (!(isfinite(x) && isfinite(y) && isfinite(z))) {
+        atomicOr(S.err, (unsigned)kErrNonFinite);
+        x = y = z = 0.0;
+      }
+      const double rx = wrap_rel(x, S.omin[0], S.L[0]);
+      const double ry = wrap_rel(y, S.omin[1], S.L[1]);
+      const double rz = wrap_rel(z, S.omin[2], S.L[2]);
+      int cx = (int)(rx / S.ax), cy = (int)(ry / S.ay);
+      cx = cx < 0 ? 0 : (cx >= S.ncx ? S.ncx - 1 : cx);
+      cy = cy < 0 ? 0 : (cy >= S.ncy ? S.ncy - 1 : cy);
+      col = cy * S.ncx + cx;
+      const unsigned zb = __float_as_uint((float)rz);  // z >= 0: bit order == value order
+      key = ((unsigned long long)zb << 32) | (unsigned)a;
+    }
   }
   // one atomic per (system, column) group of the wave
   while (true) {
@@ -73,121 +134,79 @@ __global__ void k_bin(SysDev *sys, const int *ap, int nsys) {
     const int ls = __shfl(s, leader), lc = __shfl(col, leader);
     const bool mine = todo && s == ls && col == lc;
     const unsigned long long grp = wave_ballot(mine);
-    if (lane == leader) atomicAdd(&sys[ls].col_count[lc], __popcll(grp));
-    todo = todo && !mine;
-  }
-}
-
-// one block per system: exclusive scans of atoms and clusters over columns
-__global__ __launch_bounds__(1024) void k_col_scan(SysDev *sys) {
-  SysDev &S = sys[blockIdx.x];
-  if (!S.ctl[kCtlRebuild]) return;
-  __shared__ int sa[1024], sc[1024], ss[1024];
-  const int t = threadIdx.x, nt = blockDim.x;
-  const int ncol = S.ncol;
-  const int chunk = (ncol + nt - 1) / nt;
-  const int c0 = t * chunk, c1 = min(ncol, c0 + chunk);
-  int la = 0, lc = 0, ls = 0;
-  for (int c = c0; c < c1; ++c) {
-    const int n = S.col_count[c];
-    const int k = (n + kCluster - 1) / kCluster;
-    la += n;
-    lc += k;
-    ls += (k + kScCl - 1) / kScCl;
-  }
-  sa[t] = la;
-  sc[t] = lc;
-  ss[t] = ls;
-  __syncthreads();
-  for (int o = 1; o < nt; o <<= 1) {
-    const int va = t >= o ? sa[t - o] : 0, vc = t >= o ? sc[t - o] : 0, vs = t >= o ? ss[t - o] : 0;
-    __syncthreads();
-    sa[t] += va;
-    sc[t] += vc;
-    ss[t] += vs;
-    __syncthreads();
-  }
-  int ba = sa[t] - la, bc = sc[t] - lc, bs = ss[t] - ls;
-  for (int c = c0; c < c1; ++c) {
-    const int n = S.col_count[c];
-    const int k = (n + kCluster - 1) / kCluster;
-    const int q = (k + kScCl - 1) / kScCl;
-    S.col_atom_start[c] = ba;
-    S.col_cl_start[c] = bc;
-    S.col_ncl[c] = k;
-    S.col_sc_start[c] = bs;
-    S.col_nsc[c] = q;
-    S.col_cursor[c] = 0;
-    for (int j = 0; j < q; ++j)
-      if (bs + j < S.sc_cap) S.sc_col[bs + j] = c;
-    ba += n;
-    bc += k;
-    bs += q;
-  }
-  if (t == nt - 1) {
-    int tot = sc[t], tsc = ss[t];
-    if (tot > S.cl_cap || tsc > S.sc_cap) {
-      atomicOr(S.err, (unsigned)kErrClusterCap);
-      tot = min(tot, S.cl_cap);
-      tsc = min(tsc, S.sc_cap);
-    }
-    *S.n_clusters = tot;
-    *S.n_sc = tsc;
-  }
-}
-
-__global__ void k_col_fill(SysDev *sys, const int *ap, int nsys) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  bool todo = g < ap[nsys];
-  int s = 0, col = 0, a = 0;
-  if (todo) {
-    s = find_sys(ap, nsys, g);
-    todo = sys[s].ctl[kCtlRebuild] != 0;
-  }
-  if (todo) {
-    a = g - ap[s];
-    col = sys[s].bin_col[a];
-  }
-  while (true) {
-    const unsigned long long rem = wave_ballot(todo);
-    if (rem == 0) break;
-    const int leader = __ffsll((long long)rem) - 1;
-    const int ls = __shfl(s, leader), lc = __shfl(col, leader);
-    const bool mine = todo && s == ls && col == lc;
-    const unsigned long long grp = wave_ballot(mine);
     int base = 0;
-    if (lane == leader) base = atomicAdd(&sys[ls].col_cursor[lc], __popcll(grp));
+    if (lane == leader) base = atomicAdd(&sys[ls].col_count[lc], __popcll(grp));
     base = __shfl(base, leader);
     if (mine) {
-      SysDev &S = sys[s];
       const int pos = base + __popcll(grp & ((1ull << lane) - 1ull));
-      const unsigned zb = __float_as_uint(S.bin_z[a]);  // z >= 0: bit order == value order
-      S.colbuf[S.col_atom_start[col] + pos] = ((unsigned long long)zb << 32) | (unsigned)a;
+      if (pos < kSortCap) sys[s].bucket[(size_t)col * kSortCap + pos] = key;
+      else atomicOr(sys[s].err, (unsigned)kErrColumnCap);
     }
     todo = todo && !mine;
   }
 }
 
-// one block per column: bitonic sort by (z, atom) and slot assignment
+// Rebuild, step 2 of 3, one block per column: the column's first atom,
+// cluster and super-cluster from the counts of the columns before it, a
+// bitonic sort of its bucket by (z, atom), slot assignment, and each
+// cluster's geometry (8 lanes per cluster): bounding box, origin, the atoms'
+// FP32 local coordinates.  The block of each system's column 0 finishes the
+// step's bookkeeping.
 __global__ __launch_bounds__(1024) void k_col_sort(SysDev *sys, const int *cp, int nsys) {
   __shared__ unsigned long long keys[kSortCap];
-  int gcol = blockIdx.x;
+  __shared__ int red[3][1024 / kWave];
+  const int gcol = blockIdx.x;
   if (gcol >= cp[nsys]) return;
-  int s = find_sys(cp, nsys, gcol);
+  const int s = find_sys(cp, nsys, gcol);
   SysDev &S = sys[s];
+  const int col = gcol - cp[s];
+  const int t = threadIdx.x, nt = blockDim.x, lane = t & 63, wv = t >> 6;
+  if (col == 0 && t == 0) step_finish_ctl(S);
   if (!S.ctl[kCtlRebuild]) return;
-  int col = gcol - cp[s];
+  // clusters and super-clusters of the columns before this one
+  int lc = 0, lsc = 0;
+  for (int c = t; c < col; c += nt) {
+    const int k = (S.col_count[c] + kCluster - 1) / kCluster;
+    lc += k;
+    lsc += (k + kScCl - 1) / kScCl;
+  }
+  lc = (int)wave_sum_ll(lc);
+  lsc = (int)wave_sum_ll(lsc);
+  if (lane == 0) {
+    red[0][wv] = lc;
+    red[1][wv] = lsc;
+  }
+  __syncthreads();
+  int cl0 = 0, sc0 = 0;
+  for (int w = 0; w < nt / kWave; ++w) {
+    cl0 += red[0][w];
+    sc0 += red[1][w];
+  }
   const int n = S.col_count[col];
-  const int t = threadIdx.x, nt = blockDim.x;
   if (n > kSortCap) {
     if (t == 0) atomicOr(S.err, (unsigned)kErrColumnCap);
     return;
   }
-  const int start = S.col_atom_start[col];
+  const int ncl = (n + kCluster - 1) / kCluster, nsc = (ncl + kScCl - 1) / kScCl;
+  if (cl0 + ncl > S.cl_cap || sc0 + nsc > S.sc_cap) {
+    if (t == 0) atomicOr(S.err, (unsigned)kErrClusterCap);
+    return;
+  }
+  if (t == 0) {
+    S.col_cl_start[col] = cl0;
+    S.col_ncl[col] = ncl;
+    S.col_sc_start[col] = sc0;
+    S.col_nsc[col] = nsc;
+    if (col == S.ncol - 1) {  // the system's totals
+      *S.n_clusters = cl0 + ncl;
+      *S.n_sc = sc0 + nsc;
+    }
+  }
+  for (int j = t; j < nsc; j += nt) S.sc_col[sc0 + j] = col;
   int P = 1;
   while (P < n) P <<= 1;
-  for (int i = t; i < P; i += nt) keys[i] = i < n ? S.colbuf[start + i] : ~0ull;
+  const unsigned long long *bk = S.bucket + (size_t)col * kSortCap;
+  for (int i = t; i < P; i += nt) keys[i] = i < n ? bk[i] : ~0ull;
   __syncthreads();
   for (int k = 2; k <= P; k <<= 1) {
     for (int j = k >> 1; j > 0; j >>= 1) {
@@ -205,7 +224,6 @@ __global__ __launch_bounds__(1024) void k_col_sort(SysDev *sys, const int *cp, i
       __syncthreads();
     }
   }
-  const int cl0 = S.col_cl_start[col], ncl = S.col_ncl[col];
   for (int k = t; k < ncl * kCluster; k += nt) {
     int slot = cl0 * kCluster + k;
     if (k < n) {
@@ -227,50 +245,44 @@ __global__ __launch_bounds__(1024) void k_col_sort(SysDev *sys, const int *cp, i
     S.cl_col[cl0 + c] = col;
     S.cl_nvalid[cl0 + c] = min(kCluster, n - c * kCluster);
   }
-  __syncthreads();
-  if (t == 0) S.col_count[col] = 0;  // ready for the next rebuild
-}
-
-// one thread per cluster: bounding box, origin, local coordinates
-__global__ void k_cluster_geom(SysDev *sys, const int *clp, int nsys) {
-  int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= clp[nsys]) return;
-  int s = find_sys(clp, nsys, g);
-  SysDev &S = sys[s];
-  int c = g - clp[s];
-  if (!S.ctl[kCtlRebuild] || c >= *S.n_clusters) return;
-  const int n = S.n_atoms, nv = S.cl_nvalid[c];
-  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-  double rel[kCluster][3];
-  int at[kCluster];
-  for (int k = 0; k < nv; ++k) {
-    int a = S.orig[c * kCluster + k];
-    at[k] = a;
+  // cluster geometry: lane k of each aligned group of 8 holds atom k of a cluster
+  const int na = S.n_atoms;
+  for (int i0 = 0; i0 < ncl * kCluster; i0 += nt) {  // block-uniform trip count
+    const int i = i0 + t, c = i >> 3;
+    const bool has = i < n;
+    int a = 0;
+    double rel[3] = {0.0, 0.0, 0.0}, lo[3], hi[3];
+    if (has) a = (int)(keys[i] & 0xffffffffull);
     for (int d = 0; d < 3; ++d) {
-      double r = wrap_rel(S.x[d * n + a], S.omin[d], S.L[d]);
-      rel[k][d] = r;
-      lo[d] = fmin(lo[d], r);
-      hi[d] = fmax(hi[d], r);
+      if (has) rel[d] = wrap_rel(S.x[d * na + a], S.omin[d], S.L[d]);
+      lo[d] = has ? rel[d] : 1e300;
+      hi[d] = has ? rel[d] : -1e300;
+      for (int o = 1; o < kCluster; o <<= 1) {
+        lo[d] = fmin(lo[d], __shfl_xor(lo[d], o));
+        hi[d] = fmax(hi[d], __shfl_xor(hi[d], o));
+      }
     }
-  }
-  double org[3];
-  for (int d = 0; d < 3; ++d) {
-    org[d] = 0.5 * (lo[d] + hi[d]);
-    S.cl_origin[d * S.cl_cap + c] = org[d];
-  }
-  S.cl_lo[c] = make_float4((float)lo[0] - 1e-3f, (float)lo[1] - 1e-3f, (float)lo[2] - 1e-3f, 0.f);
-  S.cl_hi[c] = make_float4((float)hi[0] + 1e-3f, (float)hi[1] + 1e-3f, (float)hi[2] + 1e-3f, 0.f);
-  for (int k = 0; k < nv; ++k) {
-    int a = at[k];
-    float l[3];
-    for (int d = 0; d < 3; ++d) {
-      double xa = S.x[d * n + a];
-      double ref = S.omin[d] + org[d] + (xa - S.omin[d] - rel[k][d]);
-      S.xref[d * n + a] = ref;
-      l[d] = (float)(xa - ref);
+    if (i < ncl * kCluster) {
+      const int cg = cl0 + c;
+      double org[3];
+      for (int d = 0; d < 3; ++d) org[d] = 0.5 * (lo[d] + hi[d]);
+      if ((i & (kCluster - 1)) == 0) {
+        for (int d = 0; d < 3; ++d) S.cl_origin[d * S.cl_cap + cg] = org[d];
+        S.cl_lo[cg] = make_float4((float)lo[0] - 1e-3f, (float)lo[1] - 1e-3f, (float)lo[2] - 1e-3f, 0.f);
+        S.cl_hi[cg] = make_float4((float)hi[0] + 1e-3f, (float)hi[1] + 1e-3f, (float)hi[2] + 1e-3f, 0.f);
+      }
+      if (has) {
+        float l[3];
+        for (int d = 0; d < 3; ++d) {
+          const double xa = S.x[d * na + a];
+          const double ref = S.omin[d] + org[d] + (xa - S.omin[d] - rel[d]);
+          S.xref[d * na + a] = ref;
+          l[d] = (float)(xa - ref);
+        }
+        S.xq[cg * kCluster + (i & (kCluster - 1))] = make_float4(l[0], l[1], l[2], (float)S.charge[a]);
+        S.lbuild[a] = make_float4(l[0], l[1], l[2], 0.f);
+      }
     }
-    S.xq[c * kCluster + k] = make_float4(l[0], l[1], l[2], (float)S.charge[a]);
-    S.lbuild[a] = make_float4(l[0], l[1], l[2], 0.f);
   }
 }
 
@@ -337,6 +349,9 @@ __global__ __launch_bounds__(kListThreads) void k_pairlist(SysDev *sys, const in
   const int cl0 = S.col_cl_start[coli] + kz * kScCl;
   const int nic = min(kScCl, S.col_ncl[coli] - kz * kScCl);
   const int cxi = coli % S.ncx, cyi = coli / S.ncx;
+  // the column's bucket count has been read by every block of k_col_sort:
+  // clear it for the next rebuild (a column without atoms kept its 0)
+  if (kz == 0 && lane == 0) S.col_count[coli] = 0;
   const float Lx = (float)S.L[0], Ly = (float)S.L[1], Lz = (float)S.L[2];
   const float ax = (float)S.ax, ay = (float)S.ay;
   const float ix0 = cxi * ax, ix1 = (cxi + 1) * ax, iy0 = cyi * ay, iy1 = (cyi + 1) * ay;
@@ -732,29 +747,6 @@ __device__ __forceinline__ void lv_half(double v[3], const double F[3], double m
   for (int d = 0; d < 3; ++d) v[d] = v[d] * lv.vdt + F[d] * lv.fdt / m + g[d] * sf * lv.ndt;
 }
 
-// one system's step bookkeeping (one thread): counters, the census, and the
-// rebuild decision from the largest displacement since the last rebuild
-__device__ __forceinline__ void step_begin_ctl(SysDev &S, int mode, float lim2, int kmax) {
-  S.census[0] = 0;
-  S.census[1] = 0;
-  int *ctl = S.ctl;
-  ctl[kCtlFirst] = 0;  // the step's kick has run
-  ctl[kCtlNoise] += 1;
-  const unsigned d2 = *S.maxdisp2;
-  bool rb = (mode & kStepForce) != 0;
-  if (!rb && (mode & kStepAuto)) rb = __uint_as_float(d2) > lim2 || (kmax > 0 && ctl[kCtlSince] + 1 >= kmax);
-  *S.maxdisp_run = max(*S.maxdisp_run, d2);
-  *S.maxdisp2 = 0u;
-  ctl[kCtlRebuild] = rb ? 1 : 0;
-  if (rb) {
-    ctl[kCtlSince] = 0;
-    ctl[kCtlCount] += 1;
-    ctl[kCtlEntries] = 0;
-  } else {
-    ctl[kCtlSince] += 1;
-  }
-}
-
 __global__ __launch_bounds__(256) void k_kick_drift(SysDev *sys, const int *ap, int nsys, double h, LangevinDev lv) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   float d2 = 0.f;
@@ -928,13 +920,14 @@ void launch_clear_err(const Launch &L) {
   hipLaunchKernelGGL(k_clear_err, dim3(nblk(L.nsys, 64)), dim3(64), 0, L.stream, L.sys, L.nsys);
 }
 
-void launch_rebuild(const Launch &L, float rlist2) {
-  if (!L.n_atoms_total) return;
-  hipLaunchKernelGGL(k_bin, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys);
-  hipLaunchKernelGGL(k_col_scan, dim3(L.nsys), dim3(1024), 0, L.stream, L.sys);
-  hipLaunchKernelGGL(k_col_fill, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys);
+void launch_rebuild(const Launch &L, float rlist2, int mode, float lim2, int kmax) {
+  if (!L.n_atoms_total) {
+    launch_step_begin(L, mode, lim2, kmax);
+    return;
+  }
+  hipLaunchKernelGGL(k_bin, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys,
+                     mode, lim2, kmax);
   hipLaunchKernelGGL(k_col_sort, dim3(L.n_cols_total), dim3(1024), 0, L.stream, L.sys, L.col_prefix, L.nsys);
-  hipLaunchKernelGGL(k_cluster_geom, dim3(nblk(L.n_cl_total, 128)), dim3(128), 0, L.stream, L.sys, L.cl_prefix, L.nsys);
   hipLaunchKernelGGL(k_pairlist, dim3(nblk(L.n_sc_total, kListWaves)), dim3(kListThreads), 0, L.stream, L.sys,
                      L.sc_prefix, L.nsys, rlist2);
 }
