@@ -202,7 +202,7 @@ def cpu_baseline(system, pairs, seconds):
     info = cpu_info()
     if O is None:
         return {"value": None, "unit": "pair interactions/s", "cores": 0, "kind": "port", "sample": "unavailable"}
-    nt = info["threads_all"]
+    nt = info["host_share_threads"]
     t1 = timed(lambda: oracle_water32k(O, system, nthreads=1), seconds * 0.3)
     log(f"cpu baseline: oracle 1 thread {t1:.3f} s per evaluation")
     ta = timed(lambda: oracle_water32k(O, system, nthreads=nt), seconds * 0.3, at_least=2)
