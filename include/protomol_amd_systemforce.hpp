@@ -10,8 +10,13 @@
 //       (force/nonbonded/NonbondedCutoffSystemForce.h:45-55,
 //       NonbondedCutoffForce.h:70-89), TOneAtomPair = OneAtomPair<BC, S,
 //       LennardJonesForce | CoulombForce> or OneAtomPairTwo<BC, SA,
-//       LennardJonesForce, SB, CoulombForce> (module/NonbondedCutoffForceModule.cpp:56-88),
-//       periodic boundary conditions; keyword "NonbondedCutoffHIP"
+//       LennardJonesForce, SB, CoulombForce>, BC periodic or vacuum
+//       (module/NonbondedCutoffForceModule.cpp:56-88, 110-140); keyword
+//       "NonbondedCutoffHIP".  A vacuum system keeps its atoms in a region
+//       (origin +- box / 2) made from the positions of the first evaluation,
+//       grown by kPmVacuumMargin on every side; an atom that leaves it makes
+//       the evaluation build the device context again around the positions
+//       of that time
 //   NonbondedFullEwaldSystemForceHIP<real, reciprocal, correction>
 //       for NonbondedFullEwaldSystemForce<PBC, Cubic, real, reciprocal,
 //       correction, Cutoff> (force/nonbonded/NonbondedFullEwaldSystemForce.h:
@@ -144,6 +149,24 @@ namespace ProtoMol {
     }
   };
 
+  // The region a vacuum system's atoms stay in on the cell-list path: the
+  // bounding box of the positions grown by this margin (A) on every side.
+  static const double kPmVacuumMargin = 10.0;
+
+  inline void pm_vacuum_region(const Vector3DBlock *positions, pm_system_desc &d) {
+    double lo[3] = {0.0, 0.0, 0.0}, hi[3] = {0.0, 0.0, 0.0};
+    for (size_t i = 0; i < positions->size(); i++)
+      for (int k = 0; k < 3; k++) {
+        const double v = (*positions)[i].c[k];
+        if (i == 0 || v < lo[k]) lo[k] = v;
+        if (i == 0 || v > hi[k]) hi[k] = v;
+      }
+    for (int k = 0; k < 3; k++) {
+      d.box[k] = hi[k] - lo[k] + 2.0 * kPmVacuumMargin;
+      d.origin[k] = 0.5 * (lo[k] + hi[k]);
+    }
+  }
+
   // The C ABI's switch record of a ProtoMol switching function, from its
   // getParameters() values (keywords -switchon, -cutoff, -switchoff, -n).
   template <class TSwitch>
@@ -213,19 +236,27 @@ namespace ProtoMol {
   class PmSystemForceBase : public SystemForce {
   public:
     explicit PmSystemForceBase(const pm_force_desc &d)
-      : myDesc(d), myCtx(0), myTopo(0), myVirial(false), myMolVirial(false) {}
+      : myDesc(d), myCtx(0), myTopo(0), myVirial(false), myMolVirial(false), myRegion(false) {}
     PmSystemForceBase(const PmSystemForceBase &o)
-      : SystemForce(o), myDesc(o.myDesc), myCtx(0), myTopo(0), myVirial(false), myMolVirial(false) {}
+      : SystemForce(o), myDesc(o.myDesc), myCtx(0), myTopo(0), myVirial(false), myMolVirial(false),
+        myRegion(false) {}
     virtual ~PmSystemForceBase() { if (myCtx) pm_ctx_destroy(myCtx); }
 
     virtual void evaluate(const GenericTopology *topo, const Vector3DBlock *positions,
                           Vector3DBlock *forces, ScalarStructure *energies) {
       const bool vir = energies->virial(), mol = energies->molecularVirial();
-      ensure(topo, vir, mol);
+      ensure(topo, positions, vir, mol);
       std::vector<double> e(PM_E_NSLOTS, 0.0);
       const double *x = positions->size() ? (*positions)[0].c : 0;
       double *f = forces->size() ? (*forces)[0].c : 0;
       int rc = pm_evaluate(myCtx, 0, x, f, &e[0]);
+      if (rc == PM_ECAPACITY && myRegion) {
+        // an atom left the vacuum region the context was made for (a failed
+        // evaluation adds nothing): once more, in a region around these positions
+        uncache();
+        ensure(topo, positions, vir, mol);
+        rc = pm_evaluate(myCtx, 0, x, f, &e[0]);
+      }
       if (rc != PM_OK) THROW(std::string("protomol_amd: ") + pm_last_error(myCtx));
       (*energies)[ScalarStructure::COULOMB] += e[PM_E_COULOMB];
       (*energies)[ScalarStructure::LENNARDJONES] += e[PM_E_LENNARDJONES];
@@ -254,12 +285,16 @@ namespace ProtoMol {
     // GB constants come from it)
     virtual pm_force_desc describeFor(const GenericTopology *, const PmTopologyImage &) const { return myDesc; }
     virtual bool needsGB() const { return false; }
-    void ensure(const GenericTopology *topo, bool vir, bool mol) {
+    // on the cell-list path a vacuum system needs a region (all-pairs does not)
+    virtual bool cellPath() const { return false; }
+    void ensure(const GenericTopology *topo, const Vector3DBlock *positions, bool vir, bool mol) {
       if (myCtx && myTopo == topo && myVirial == vir && myMolVirial == mol &&
           std::memcmp(&myBuilt, &myDesc, sizeof myDesc) == 0)
         return;
       uncache();
       PmTopologyImage img(topo, needsGB());
+      myRegion = !img.desc.periodic && cellPath();
+      if (myRegion) pm_vacuum_region(positions, img.desc);
       pm_force_desc d = describeFor(topo, img);
       if (vir) d.flags |= PM_FLAG_VIRIAL;
       if (mol) d.flags |= PM_FLAG_MOLVIRIAL;
@@ -273,7 +308,7 @@ namespace ProtoMol {
     pm_force_desc myDesc, myBuilt;
     pm_ctx *myCtx;
     const GenericTopology *myTopo;
-    bool myVirial, myMolVirial;
+    bool myVirial, myMolVirial, myRegion;
   };
 
   // ------------------------------------------------ NonbondedCutoffHIP
@@ -298,6 +333,9 @@ namespace ProtoMol {
       parameters.push_back(Parameter("-cutoff", Value(myCutoff, ConstraintValueType::Positive()),
                                      Text("algorithm cutoff")));
     }
+
+  protected:
+    virtual bool cellPath() const { return true; }
 
   private:
     // NonbondedCutoffForce::doMake (NonbondedCutoffForce.h:70-74): the pair

@@ -11,7 +11,9 @@
 // usage: adapter_check in.rec out.rec
 //   in.rec   the system records of ref_driver; mode 1: a periodic system and
 //            the cutoff / Ewald cases, mode 3: a vacuum GB chain and the
-//            all-pairs force group
+//            all-pairs force group, mode 6: a vacuum system and the cutoff
+//            cases, each at the given positions and again with the molecule
+//            of the atom of largest x moved `exit` A further out in x
 //   out.rec  per case k: f_ref<k>, e_ref<k> (ProtoMol's force) and
 //            f_hip<k>, e_hip<k> (the adapter), and "cases" (their count)
 #include "ref_common.hpp"
@@ -45,6 +47,12 @@ static const char *const kPeriodic[][3] = {
      "FullEwaldHIP"},
 };
 
+// the vacuum cutoff cases (NonbondedCutoffForceModule.cpp:110-140)
+static const char *const kVacuum[] = {
+    "LennardJones Coulomb -algorithm %s -switchingFunction C2 -switchingFunction C2 -switchon 6 -cutoff 8 "
+    "-switchon 8 -cutoff 10 -cutoff 10",
+    "Coulomb -algorithm %s -switchingFunction C1 -cutoff 10 -cutoff 10"};
+
 // the all-pairs lines of the reference's GB regression tests (e.g.
 // test/tests/www-fip35_FFAMBER_GB_LL.conf), one force group
 static const char *const kFullGroup[] = {
@@ -73,6 +81,13 @@ static void register_periodic(ForceFactory &ff) {
   ff.reg(new NonbondedFullEwaldSystemForceHIP<true, false, true>());
   ff.reg(new NonbondedFullEwaldSystemForce<PBC, CCM, true, true, true, Cut>());
   ff.reg(new NonbondedFullEwaldSystemForceHIP<true, true, true>());
+}
+
+static void register_vacuum(ForceFactory &ff) {
+  ff.reg(new NonbondedCutoffSystemForce<CCM, OneAtomPairTwo<VBC, C2, LJ, C2, CF> >());
+  ff.reg(new NonbondedCutoffSystemForceHIP<OneAtomPairTwo<VBC, C2, LJ, C2, CF> >());
+  ff.reg(new NonbondedCutoffSystemForce<CCM, OneAtomPair<VBC, C1, CF> >());
+  ff.reg(new NonbondedCutoffSystemForceHIP<OneAtomPair<VBC, C1, CF> >());
 }
 
 static void register_full(ForceFactory &ff) {
@@ -118,10 +133,10 @@ static void put_result(FILE *fp, const char *tag, int k, const Vector3DBlock &f,
 }
 
 // one side (reference algorithm or adapter) of a set of lines evaluated as
-// one force group
+// one force group, at each of the positions in turn (the same force objects)
 template <class T>
-static void run_side(ForceFactory &ff, T &topo, const std::vector<std::string> &lines, Vector3DBlock &pos,
-                     FILE *out, const char *tag, int k) {
+static void run_side(ForceFactory &ff, T &topo, const std::vector<std::string> &lines,
+                     const std::vector<Vector3DBlock *> &positions, FILE *out, const char *tag, int k) {
   ForceGroup group;  // owns the forces
   std::vector<SystemForce *> fs;
   for (size_t i = 0; i < lines.size(); i++) {
@@ -131,14 +146,16 @@ static void run_side(ForceFactory &ff, T &topo, const std::vector<std::string> &
     group.addForce(f);
     fs.push_back(sf);
   }
-  Vector3DBlock f(pos.size());
-  f.zero();
-  ScalarStructure e;
-  e.clear();
-  e.virial(true);
-  e.molecularVirial(false);
-  evaluate_group(topo, fs, pos, f, e);
-  put_result(out, tag, k, f, e);
+  for (size_t p = 0; p < positions.size(); p++) {
+    Vector3DBlock f(positions[p]->size());
+    f.zero();
+    ScalarStructure e;
+    e.clear();
+    e.virial(true);
+    e.molecularVirial(false);
+    evaluate_group(topo, fs, *positions[p], f, e);
+    put_result(out, tag, k + (int)p, f, e);
+  }
 }
 
 int main(int argc, char **argv) {
@@ -166,10 +183,11 @@ int main(int argc, char **argv) {
       PBC bc(Vector3D(box[0], 0, 0), Vector3D(0, box[1], 0), Vector3D(0, 0, box[2]), Vector3D(org[0], org[1], org[2]));
       PTopo topo(getd(r, "csf", 1.0), ExclusionType(ExclusionType::ONE3), bc, CCM(getd(r, "cellsize", 5.0)), 16.0f);
       fill_common(topo, r, n);
+      const std::vector<Vector3DBlock *> at(1, &pos);
       for (size_t k = 0; k < sizeof kPeriodic / sizeof kPeriodic[0]; k++, cases++) {
-        run_side(ff, topo, std::vector<std::string>(1, line(kPeriodic[k][0], kPeriodic[k][1])), pos, out, "ref",
+        run_side(ff, topo, std::vector<std::string>(1, line(kPeriodic[k][0], kPeriodic[k][1])), at, out, "ref",
                  (int)k);
-        run_side(ff, topo, std::vector<std::string>(1, line(kPeriodic[k][0], kPeriodic[k][2])), pos, out, "hip",
+        run_side(ff, topo, std::vector<std::string>(1, line(kPeriodic[k][0], kPeriodic[k][2])), at, out, "hip",
                  (int)k);
       }
     } else if (mode == 3) {
@@ -182,9 +200,28 @@ int main(int argc, char **argv) {
         ref.push_back(line(kFullGroup[k], "NonbondedSimpleFull"));
         hip.push_back(line(kFullGroup[k], "NonbondedSimpleFullHIP"));
       }
-      run_side(ff, topo, ref, pos, out, "ref", 0);
-      run_side(ff, topo, hip, pos, out, "hip", 0);
+      const std::vector<Vector3DBlock *> at(1, &pos);
+      run_side(ff, topo, ref, at, out, "ref", 0);
+      run_side(ff, topo, hip, at, out, "hip", 0);
       cases = 1;
+    } else if (mode == 6) {
+      register_vacuum(ff);
+      VTopo topo(getd(r, "csf", 1.0), ExclusionType(ExclusionType::ONE3), VBC(), CCM(getd(r, "cellsize", 5.0)), 16.0f);
+      fill_common(topo, r, n);
+      int far = 0;
+      for (int i = 1; i < n; i++)
+        if (pos[i].c[0] > pos[far].c[0]) far = i;
+      Vector3DBlock moved(pos);
+      for (int i = 0; i < n; i++)
+        if (topo.atoms[i].molecule == topo.atoms[far].molecule) moved[i].c[0] += getd(r, "exit", 15.0);
+      std::vector<Vector3DBlock *> at;
+      at.push_back(&pos);
+      at.push_back(&moved);
+      for (size_t k = 0; k < sizeof kVacuum / sizeof kVacuum[0]; k++, cases += 2) {
+        run_side(ff, topo, std::vector<std::string>(1, line(kVacuum[k], "NonbondedCutoff")), at, out, "ref", cases);
+        run_side(ff, topo, std::vector<std::string>(1, line(kVacuum[k], "NonbondedCutoffHIP")), at, out, "hip",
+                 cases);
+      }
     }
   } catch (const Exception &e) {
     fprintf(stderr, "adapter_check: %s\n", e.getMessage().c_str());

@@ -6,7 +6,9 @@
 // TEST INFRASTRUCTURE ONLY.  Each pm_ctx_create appends the force record and
 // a summary of the system record to the file $PM_STUB_LOG as key=value
 // lines; pm_evaluate adds 1 to every force component and 0.5 to every energy
-// slot (add-into semantics are visible to the caller).
+// slot (add-into semantics are visible to the caller), or, for a vacuum
+// system with a region (periodic 0, box > 0) and an atom outside it, adds
+// nothing and returns PM_ECAPACITY as the library does.
 #include <cstdio>
 #include <cstdlib>
 
@@ -14,6 +16,8 @@
 
 struct pm_ctx {
   int n;
+  int region;
+  double lo[3], hi[3];
 };
 
 static void put_switch(FILE *fp, const char *tag, const pm_switch_desc &s) {
@@ -58,6 +62,11 @@ int pm_ctx_create(const pm_force_desc *f, int32_t n_systems, const pm_system_des
   }
   pm_ctx *c = new pm_ctx();
   c->n = n_systems > 0 ? s[0].n_atoms : 0;
+  c->region = n_systems > 0 && !s[0].periodic && s[0].box[0] > 0;
+  for (int k = 0; k < 3 && c->region; k++) {
+    c->lo[k] = s[0].origin[k] - 0.5 * s[0].box[k];
+    c->hi[k] = s[0].origin[k] + 0.5 * s[0].box[k];
+  }
   *out = c;
   return PM_OK;
 }
@@ -67,7 +76,10 @@ int pm_ctx_destroy(pm_ctx *c) {
   return PM_OK;
 }
 
-int pm_evaluate(pm_ctx *c, int32_t, const double *, double *f, double *e) {
+int pm_evaluate(pm_ctx *c, int32_t, const double *x, double *f, double *e) {
+  for (int i = 0; i < c->n && c->region; i++)
+    for (int k = 0; k < 3; k++)
+      if (x[3 * i + k] < c->lo[k] || x[3 * i + k] >= c->hi[k]) return PM_ECAPACITY;
   for (int k = 0; k < 3 * c->n; k++) f[k] += 1.0;
   for (int k = 0; k < PM_E_NSLOTS; k++) e[k] += 0.5;
   return PM_OK;

@@ -687,12 +687,41 @@ static int run_walk(const orc_cells *cl, const cell_pair *pairs, int np, int nno
   return 0;
 }
 
+/* The vacuum case (VacuumBoundaryConditions: minimalDifference is x_j - x_i,
+ * VacuumBoundaryConditions.h): the cell list with CellListEnumeratorStandard
+ * yields every pair of atoms in neighbouring cells, and the pair function keeps
+ * those within the cutoff, so the pair set is every pair within it.  The cell
+ * walk only orders the FP64 sums; here the pairs go in index order. */
+static int nonbonded_cutoff_vacuum(const orc_system *sys, const orc_nonbonded *nb, const double *x, double *f,
+                                   double *e, orc_census *census, double cut2) {
+  orc_excl ex;
+  if (excl_build(&ex, sys)) return -1;
+  double *com = NULL;
+  if (nb->molvirial) {
+    com = (double *)calloc((size_t)orc_n_molecules(sys) * 3 + 3, sizeof(double));
+    if (!com) { free(ex.tab); return -1; }
+    orc_molecule_com(sys, x, com);
+  }
+  pair_ctx c;
+  memset(&c, 0, sizeof c);
+  c.sys = sys; c.nb = nb; c.pbc = NULL; c.excl = &ex; c.x = x; c.f = f; c.e = e; c.census = census;
+  c.cut2 = cut2; c.com = com;
+  for (int i = 0; i < sys->n; i++)
+    for (int j = i + 1; j < sys->n; j++) one_atom_pair(&c, i, j);
+  free(com); free(ex.tab);
+  return 0;
+}
+
 /* NonbondedCutoffSystemForce::evaluate (NonbondedCutoffSystemForce.h:45-55) */
 int orc_nonbonded_cutoff(const orc_system *sys, const orc_nonbonded *nb, const double *x,
                          double *f, double *e, orc_census *census, int nthreads) {
-  if (!sys->periodic) return -2;
   if (nb->term_a == ORC_TERM_NONE || nb->term_a == ORC_TERM_EWALD_REAL || nb->term_b == ORC_TERM_EWALD_REAL)
     return -3;
+  if (!sys->periodic) {
+    double c2 = nb->sw_a.cutoff2;
+    if (nb->term_b != ORC_TERM_NONE && nb->sw_b.cutoff2 > c2) c2 = nb->sw_b.cutoff2;
+    return nonbonded_cutoff_vacuum(sys, nb, x, f, e, census, c2);
+  }
   if (nthreads < 1) nthreads = 1;
   orc_pbc pbc;
   pbc_set(&pbc, sys->box, sys->origin);

@@ -43,8 +43,8 @@ static CnSwitchingFunction mkCn(const double *s) { return CnSwitchingFunction(s[
 static CmpCnCnSwitchingFunction mkCmp(const double *s) { return CmpCnCnSwitchingFunction(s[1], s[2], s[3], s[4]); }
 static CutoffSwitchingFunction mkCut(const double *s) { return CutoffSwitchingFunction(s[2]); }
 
-template <class Pair>
-static void run_cutoff(PTopo &topo, Vector3DBlock &pos, Vector3DBlock &f, ScalarStructure &e, double cutoff,
+template <class Topo, class Pair>
+static void run_cutoff(Topo &topo, Vector3DBlock &pos, Vector3DBlock &f, ScalarStructure &e, double cutoff,
                        const Pair &pair) {
   NonbondedCutoffSystemForce<CCM, Pair> force(cutoff, pair);
   force.preProcess(&topo, &pos);
@@ -53,10 +53,10 @@ static void run_cutoff(PTopo &topo, Vector3DBlock &pos, Vector3DBlock &f, Scalar
   topo.uncacheCellList();
 }
 
-template <class S>
-static void census_run(PTopo &topo, Vector3DBlock &pos, double cutoff, const S &sw) {
+template <class BC, class S>
+static void census_run(Topology<BC, CCM> &topo, Vector3DBlock &pos, double cutoff, const S &sw) {
   if (!g_census) return;
-  typedef OneAtomPair<PBC, S, CensusForce> CP;
+  typedef OneAtomPair<BC, S, CensusForce> CP;
   Vector3DBlock f(pos.size());
   f.zero();
   ScalarStructure e;
@@ -66,8 +66,11 @@ static void census_run(PTopo &topo, Vector3DBlock &pos, double cutoff, const S &
   run_cutoff(topo, pos, f, e, cutoff, CP(CensusForce(), sw));
 }
 
-// kinds: 1 C1, 2 C2, 3 Cn, 4 CmpCnCn, 5 Cutoff (switch vector: kind, switchon, cutoff, switchoff, order)
-static int do_cutoff(Recs &r, PTopo &topo, Vector3DBlock &pos, Vector3DBlock &f, ScalarStructure &e) {
+// kinds: 1 C1, 2 C2, 3 Cn, 4 CmpCnCn, 5 Cutoff (switch vector: kind, switchon, cutoff, switchoff, order);
+// BC: PeriodicBoundaryConditions (mode 1) or VacuumBoundaryConditions (mode 6,
+// NonbondedCutoffForceModule.cpp:110-140)
+template <class BC>
+static int do_cutoff(Recs &r, Topology<BC, CCM> &topo, Vector3DBlock &pos, Vector3DBlock &f, ScalarStructure &e) {
   int ta = geti(r, "term_a", 1), tb = geti(r, "term_b", 0);
   const double *sa = &r["sw_a"].d[0];
   const double *sb = tb ? &r["sw_b"].d[0] : sa;
@@ -78,8 +81,8 @@ static int do_cutoff(Recs &r, PTopo &topo, Vector3DBlock &pos, Vector3DBlock &f,
   if (tb == 0) {
 #define ONE(K, S, MK)                                                                               \
   if (ka == K) {                                                                                   \
-    if (ta == 1) run_cutoff(topo, pos, f, e, cut, OneAtomPair<PBC, S, LJ>(LJ(), MK(sa)));          \
-    else run_cutoff(topo, pos, f, e, cut, OneAtomPair<PBC, S, CF>(CF(), MK(sa)));                  \
+    if (ta == 1) run_cutoff(topo, pos, f, e, cut, OneAtomPair<BC, S, LJ>(LJ(), MK(sa)));          \
+    else run_cutoff(topo, pos, f, e, cut, OneAtomPair<BC, S, CF>(CF(), MK(sa)));                  \
     census_run(topo, pos, cut, MK(sa));                                                            \
     return 0;                                                                                      \
   }
@@ -95,7 +98,7 @@ static int do_cutoff(Recs &r, PTopo &topo, Vector3DBlock &pos, Vector3DBlock &f,
   // OneAtomPairTwo combinations registered in NonbondedCutoffForceModule.cpp:77-88
 #define TWO(KA, KB, SA, SB, MKA, MKB)                                                              \
   if (ka == KA && kb == KB) {                                                                      \
-    run_cutoff(topo, pos, f, e, cut, OneAtomPairTwo<PBC, SA, LJ, SB, CF>(LJ(), MKA(sa), CF(), MKB(sb))); \
+    run_cutoff(topo, pos, f, e, cut, OneAtomPairTwo<BC, SA, LJ, SB, CF>(LJ(), MKA(sa), CF(), MKB(sb))); \
     double ca = MKA(sa).cutoff(), cb = MKB(sb).cutoff();                                           \
     if (ca >= cb) census_run(topo, pos, cut, MKA(sa));                                             \
     else census_run(topo, pos, cut, MKB(sb));                                                      \
@@ -257,7 +260,12 @@ int main(int argc, char **argv) {
     clock_gettime(CLOCK_MONOTONIC, &t1);
     seconds = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
   } else if (mode == 5) rc = do_velocity(r, n, f, u0, ke, uu, gg);
-  else {
+  else if (mode == 6) {
+    VTopo topo(getd(r, "csf", 1.0), ExclusionType(ExclusionType::ONE3), VBC(), CCM(getd(r, "cellsize", 5.0)), 16.0f);
+    fill_common(topo, r, n);
+    molecule_positions(topo, r, pos);
+    rc = do_cutoff(r, topo, pos, f, e);
+  } else {
     const std::vector<double> &box = r["box"].d, &org = r["origin"].d;
     PBC bc(Vector3D(box[0], 0, 0), Vector3D(0, box[1], 0), Vector3D(0, 0, box[2]), Vector3D(org[0], org[1], org[2]));
     PTopo topo(getd(r, "csf", 1.0), ExclusionType(ExclusionType::ONE3), bc, CCM(getd(r, "cellsize", 5.0)), 16.0f);

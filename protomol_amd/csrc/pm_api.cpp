@@ -141,6 +141,11 @@ struct SysPrep {
   // molecules (molecular virial): CSR of their atoms in index order
   int n_mol = 0;
   std::vector<int> mol_start, mol_atoms;
+  // the periodic cell of the cutoff path: the system's own box, or for a
+  // vacuum system one that encloses its region (vac: the region's bounds)
+  double box[3] = {0, 0, 0}, origin[3] = {0, 0, 0};
+  bool vac = false;
+  double vlo[3] = {0, 0, 0}, vhi[3] = {0, 0, 0};
 };
 
 struct SysHost {
@@ -467,6 +472,10 @@ SysPrep prepare(const pm_ctx *c, const pm_system_desc &d) {
     for (int k = 0; k < 3; ++k)
       if (!(d.box[k] > 0)) throw Fail{PM_EINVAL, "box edges must be positive"};
   }
+  for (int k = 0; k < 3; ++k) {
+    p.box[k] = d.box[k];
+    p.origin[k] = d.origin[k];
+  }
   // exclusion table: last add() of a pair wins (ExclusionTable.cpp:97-120);
   // the Ewald correction walks the table's pairs with exact duplicates dropped
   struct Ex {
@@ -616,7 +625,28 @@ SysPrep prepare(const pm_ctx *c, const pm_system_desc &d) {
     p.slot_cap = n;  // forces are kept per atom: slot = atom
     return p;
   }
-  if (!d.periodic) throw Fail{PM_EINVAL, "the cell-list pair path needs periodic boundary conditions"};
+  const double rlist = c->rc + std::max(c->skin_md, c->skin_eval);
+  if (!d.periodic) {
+    // VacuumBoundaryConditions (minimalDifference = x_j - x_i) on the cell-list
+    // path: box and origin give the region (origin +- box / 2) the atoms stay
+    // in; the device works in a periodic cell larger than the region by more
+    // than rlist, so that no image of an atom comes within rlist of another
+    // atom, and every pair within the cutoff is found at its true separation
+    // (minimal image of a vector shorter than half the cell).  An atom that
+    // leaves the region stops the run with PM_ECAPACITY.
+    if (c->desc.term_b == PM_TERM_EWALD_REAL || c->recip)
+      throw Fail{PM_EINVAL, "Ewald summation needs periodic boundary conditions"};
+    for (int k = 0; k < 3; ++k) {
+      if (!(d.box[k] > 0))
+        throw Fail{PM_EINVAL, "a vacuum system on the cutoff path needs box (the extent of the region its atoms "
+                              "stay in) and origin (the region's centre)"};
+      p.vlo[k] = d.origin[k] - 0.5 * d.box[k];
+      p.vhi[k] = d.origin[k] + 0.5 * d.box[k];
+      p.box[k] = std::max(d.box[k] + rlist + 1.0, 2.0 * rlist + 1.0);
+      p.origin[k] = p.vlo[k] + 0.5 * p.box[k];
+    }
+    p.vac = true;
+  }
   // Ewald point-self and charged-system constants (NonbondedFullEwaldSystemForce.h:694-714)
   if (c->desc.term_b == PM_TERM_EWALD_REAL && c->desc.ewald_correction) {
     double q2 = 0.0, q = 0.0;
@@ -631,19 +661,20 @@ SysPrep prepare(const pm_ctx *c, const pm_system_desc &d) {
     p.vir_const = charged;
   }
   // x-y column grid: columns about (8/rho)^(1/3) wide so 8-atom clusters are compact
+  // density over the region the atoms occupy (the system's box, or a vacuum
+  // system's region)
   const double V = d.box[0] * d.box[1] * d.box[2];
   const double rho = n / V;
   const double scale = c->desc.cell_scale > 0 ? c->desc.cell_scale : 1.0;
   const double a = scale * std::cbrt(pm::kCluster / rho);
-  const double rlist = c->rc + std::max(c->skin_md, c->skin_eval);
   for (int k = 0; k < 3; ++k)
-    if (d.box[k] < 2.0 * rlist)
-      throw Fail{PM_EINVAL, "box edge " + std::to_string(d.box[k]) + " is shorter than twice the pair-list radius " +
+    if (p.box[k] < 2.0 * rlist)
+      throw Fail{PM_EINVAL, "box edge " + std::to_string(p.box[k]) + " is shorter than twice the pair-list radius " +
                                 std::to_string(rlist)};
-  p.ncx = std::max(1, (int)std::floor(d.box[0] / a));
-  p.ncy = std::max(1, (int)std::floor(d.box[1] / a));
-  p.ax = d.box[0] / p.ncx;
-  p.ay = d.box[1] / p.ncy;
+  p.ncx = std::max(1, (int)std::floor(p.box[0] / a));
+  p.ncy = std::max(1, (int)std::floor(p.box[1] / a));
+  p.ax = p.box[0] / p.ncx;
+  p.ay = p.box[1] / p.ncy;
   p.ncol = p.ncx * p.ncy;
   p.rx = (int)std::ceil(rlist / p.ax);
   p.ry = (int)std::ceil(rlist / p.ay);
@@ -816,21 +847,27 @@ void init_system(const pm_ctx *c, const pm_system_desc &d, const SysPrep &p, Sys
   h.n = n;
   S.n_atoms = n;
   S.lv_seed = c->lv_seed;
-  S.periodic = d.periodic ? 1 : 0;
+  // the cutoff path works in a periodic cell (a vacuum system's encloses its region)
+  S.periodic = (d.periodic || p.vac) ? 1 : 0;
+  S.vac = p.vac ? 1 : 0;
+  for (int k = 0; k < 3; ++k) {
+    S.vac_lo[k] = p.vlo[k];
+    S.vac_hi[k] = p.vhi[k];
+  }
   S.lj_mode = d.lj_mode;
   S.n_types = d.n_types;
   S.csf = (float)d.coulomb_scaling;
   S.lj_scaling = (float)d.lj_scaling;
   if (S.periodic) {
     // PeriodicBoundaryConditions::set (PeriodicBoundaryConditions.cpp:32-77), orthogonal
-    const double Lx = d.box[0], Ly = d.box[1], Lz = d.box[2];
+    const double Lx = p.box[0], Ly = p.box[1], Lz = p.box[2];
     double a1 = Ly * Lz - 0.0 * 0.0, d1 = Lx * a1 + 0.0 * 0.0 + 0.0 * 0.0;
     double a2 = Lz * Lx - 0.0 * 0.0, d2 = 0.0 * 0.0 + Ly * a2 + 0.0 * 0.0;
     double a3 = Lx * Ly - 0.0 * 0.0, d3 = 0.0 * 0.0 + 0.0 * 0.0 + Lz * a3;
     S.L[0] = Lx; S.L[1] = Ly; S.L[2] = Lz;
     S.Lr[0] = a1 / d1; S.Lr[1] = a2 / d2; S.Lr[2] = a3 / d3;
     for (int k = 0; k < 3; ++k) {
-      double lo = d.origin[k] - d.box[k] * 0.5, hi = d.origin[k] + d.box[k] * 0.5;
+      double lo = p.origin[k] - p.box[k] * 0.5, hi = p.origin[k] + p.box[k] * 0.5;
       double mn = std::min(lo, hi), mx = std::max(lo, hi);
       S.H[k] = mx - mn;
       S.H2[k] = S.H[k] * 0.5;
@@ -1214,6 +1251,11 @@ std::vector<int> check_errors(pm_ctx *c, LaunchSet &ls) {
                                    " atoms in one cell column"};
     }
     if (e & pm::kErrClusterCap) { pm::launch_clear_err(ls.L); throw Fail{PM_ECAPACITY, "system " + std::to_string(id) + ": cluster capacity"}; }
+    if (e & pm::kErrVacuumRegion) {
+      pm::launch_clear_err(ls.L);
+      throw Fail{PM_ECAPACITY, "system " + std::to_string(id) +
+                                   ": an atom left the region (origin +- box / 2) of the vacuum system's cell grid"};
+    }
   }
   if (!any) return grown;
   pm::launch_clear_err(ls.L);
@@ -1727,6 +1769,8 @@ void evaluate_static(const pm_force_desc &force, const pm_system_desc &d, const 
     if (err & pm::kErrColumnCap)
       throw Fail{PM_ECAPACITY, "more than " + std::to_string(pm::kSortCap) + " atoms in one cell column"};
     if (err & pm::kErrClusterCap) throw Fail{PM_ECAPACITY, "cluster capacity"};
+    if (err & pm::kErrVacuumRegion)
+      throw Fail{PM_ECAPACITY, "an atom left the region (origin +- box / 2) of the vacuum system's cell grid"};
     if (f) {
       pm::launch_export(L, pm::kXferF);
       std::vector<double> fo(3 * (size_t)h.n);

@@ -54,7 +54,7 @@ __host__ __device__ inline long long *acc_copy(long long *acc, int w) {
 }
 
 // device-side status bits (per system)
-enum { kErrColumnCap = 1, kErrListCap = 2, kErrNonFinite = 4, kErrClusterCap = 8 };
+enum { kErrColumnCap = 1, kErrListCap = 2, kErrNonFinite = 4, kErrClusterCap = 8, kErrVacuumRegion = 16 };
 
 // items of the per-atom lists of bonded/correction terms, (index << 2) | role,
 // in a CSR keyed by (atom, kind)
@@ -123,6 +123,8 @@ struct SysDev {
   double L[3], Lr[3], H[3], H2[3], D;
   double omin[3];          // origin - L/2: lower corner of the wrapped frame
   int periodic;
+  int vac;                 // a vacuum system in an enclosing periodic cell: its atoms must stay
+  double vac_lo[3], vac_hi[3];  // within [vac_lo, vac_hi) (checked at each rebuild)
   int ncx, ncy, ncol;      // x-y column grid
   int rx, ry;              // column search radius for the pair list
   double ax, ay;           // column widths

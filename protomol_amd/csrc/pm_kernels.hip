@@ -115,6 +115,9 @@ __global__ void k_bin(SysDev *sys, const int *ap, int nsys, int mode, float lim2
         atomicOr(S.err, (unsigned)kErrNonFinite);
         x = y = z = 0.0;
       }
+      if (S.vac && (x < S.vac_lo[0] || x >= S.vac_hi[0] || y < S.vac_lo[1] || y >= S.vac_hi[1] ||
+                    z < S.vac_lo[2] || z >= S.vac_hi[2]))
+        atomicOr(S.err, (unsigned)kErrVacuumRegion);
       const double rx = wrap_rel(x, S.omin[0], S.L[0]);
       const double ry = wrap_rel(y, S.omin[1], S.L[1]);
       const double rz = wrap_rel(z, S.omin[2], S.L[2]);

@@ -207,6 +207,40 @@ def water_box(n_mol: int, box: float, seed: int, origin=(0.0, 0.0, 0.0), jitter:
                   name=f"water{n_mol}")
 
 
+def vacuum_region(x: np.ndarray, margin: float = 10.0) -> tuple[np.ndarray, np.ndarray]:
+    """(box, origin) of the region a vacuum system's atoms occupy: their
+    bounding box grown by ``margin`` on every side.  The cutoff path keeps the
+    atoms of a vacuum system in this region (origin +- box / 2)."""
+    lo, hi = x.min(axis=0), x.max(axis=0)
+    return (hi - lo) + 2.0 * margin, 0.5 * (lo + hi)
+
+
+def water_droplet(radius: float, seed: int, margin: float = 10.0) -> System:
+    """The whole molecules of a water box whose oxygen lies within ``radius``
+    A of the box centre (its origin), in vacuum (VacuumBoundaryConditions), with the
+    region of ``vacuum_region``."""
+    edge = 2.0 * radius + 6.0
+    n_mol = int(math.ceil(0.0334 * edge ** 3))
+    w = water_box(n_mol, edge, seed=seed, jitter=0.3, atom_noise=0.03)
+    o = w.x[0::3]
+    keep = np.flatnonzero(np.linalg.norm(o - np.asarray(w.origin), axis=1) <= radius)
+    atoms = (3 * keep[:, None] + np.arange(3)[None, :]).reshape(-1)
+    remap = -np.ones(w.n, np.int64)
+    remap[atoms] = np.arange(atoms.size)
+    ex = w.exclusions[np.isin(w.exclusions[:, 0], atoms) & np.isin(w.exclusions[:, 1], atoms)]
+    ex = np.column_stack([remap[ex[:, 0]], remap[ex[:, 1]], ex[:, 2]]).astype(np.int32)
+    bsel = np.isin(w.bonds[:, 0], atoms)
+    asel = np.isin(w.angles[:, 0], atoms)
+    x = w.x[atoms].copy()
+    box, org = vacuum_region(x, margin)
+    mol = (np.arange(atoms.size) // 3).astype(np.int32)
+    return dataclasses.replace(
+        w, x=x, charge=w.charge[atoms], mass=w.mass[atoms], molecule=mol, box=box, origin=org, periodic=False,
+        type=w.type[atoms], exclusions=ex, bonds=remap[w.bonds[bsel]].astype(np.int32), bond_k=w.bond_k[bsel],
+        bond_r0=w.bond_r0[bsel], angles=remap[w.angles[asel]].astype(np.int32), angle_k=w.angle_k[asel],
+        angle_t0=w.angle_t0[asel], name=f"water droplet r={radius:g} A ({keep.size} molecules)")
+
+
 def water_box_density(n_mol: int, seed: int, density: float = 0.0334) -> System:
     """Water box of ``n_mol`` molecules at ``density`` molecules/A^3."""
     box = (n_mol / density) ** (1.0 / 3.0)

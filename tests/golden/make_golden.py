@@ -77,6 +77,36 @@ def make_cutoff(name, sysobj, case):
                         sw_b=np.array(sb if sb is not None else sa, float))
 
 
+# vacuum cutoff cases (NonbondedCutoffForceModule.cpp:110-140): name -> (term_a, term_b, sw_a, sw_b)
+VACUUM_CASES = {
+    # OneAtomPairTwo: LJ switched off at 8 A, Coulomb at 10 A
+    "c2c2_two": (1, 2, (2, 6.0, 8.0, 0.0, 0.0), (2, 8.0, 10.0, 0.0, 0.0)),
+    "lj_c2": (1, 0, (2, 6.0, 8.0, 0.0, 0.0), None),
+    "coul_c1": (2, 0, (1, 0.0, 10.0, 0.0, 0.0), None),
+}
+
+
+def droplet_fixture_system():
+    return S.water_droplet(14.0, seed=3)
+
+
+def make_vacuum(name, sysobj, case):
+    """NonbondedCutoffSystemForce<CubicCellManager, OneAtomPair[Two]<VacuumBoundaryConditions, ...>>,
+    with the atomic and the molecular virial."""
+    ta, tb, sa, sb = case
+    r = refio.system_records(sysobj)
+    r.update(mode=np.int32(6), term_a=np.int32(ta), term_b=np.int32(tb), sw_a=np.array(sa, float),
+             alg_cutoff=max(sa[2], sb[2] if sb is not None else 0.0), cellsize=5.0, virial=np.int32(1),
+             molvirial=np.int32(1))
+    if sb is not None:
+        r["sw_b"] = np.array(sb, float)
+    out = refio.run(r)
+    np.savez_compressed(os.path.join(HERE, f"ref_vacuum_{name}.npz"), x=sysobj.x, f=out["f"].reshape(-1, 3),
+                        e=out["e"], census=np.array(out["census"], dtype=np.uint64),
+                        term_a=ta, term_b=tb, sw_a=np.array(sa, float),
+                        sw_b=np.array(sb if sb is not None else sa, float))
+
+
 def make_ewald(sysobj):
     acc = math.exp(-10.24)
     r = refio.system_records(sysobj)
@@ -404,6 +434,11 @@ def main():
     if sys.argv[1:] == ["velocity"]:
         make_velocity()
         return
+    if sys.argv[1:] == ["vacuum"]:
+        d = droplet_fixture_system()
+        for name, case in VACUUM_CASES.items():
+            make_vacuum(name, d, case)
+        return
     if sys.argv[1:] == ["lj40"]:
         make_many_types()
         return
@@ -445,6 +480,9 @@ def main():
     copy_water_expected()
     make_velocity()
     make_many_types()
+    d = droplet_fixture_system()
+    for name, case in VACUUM_CASES.items():
+        make_vacuum(name, d, case)
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -139,3 +139,32 @@ def test_full_group_is_one_device_evaluation():
     assert float(kv["gb_r"]) == pytest.approx(float(np.sum(c.gb_radius)), rel=1e-12)
     assert float(kv["gb_s"]) == pytest.approx(float(np.sum(c.gb_scaling)), rel=1e-12)
     assert np.all(out["f_hip0"] == 1.0)
+
+
+@pytest.mark.skipif(not (HAVE_REF and os.path.exists(STUB)), reason="needs oracle/_ref/adapter_check_stub")
+def test_vacuum_cutoff_region_and_rebuild():
+    """A vacuum topology on NonbondedCutoffHIP: the context's region is the
+    positions' bounding box grown by 10 A; when an atom leaves it (the C ABI
+    answers PM_ECAPACITY and adds nothing) the adapter makes the context
+    again around the positions of that evaluation and adds their forces once."""
+    import refio
+    d = S.water_droplet(14.0, seed=3)
+    r = refio.system_records(d)
+    exit_ = 15.0
+    r.update(mode=np.int32(6), cellsize=5.0, exit=exit_)
+    creates, out = _run_stub(r)
+    assert int(out["cases"][0]) == 4 and len(creates) == 4  # 2 lines x (first region, rebuilt region)
+    far = int(np.argmax(d.x[:, 0]))
+    moved = d.x.copy()
+    moved[d.molecule == d.molecule[far], 0] += exit_
+    for kv, x in zip(creates, (d.x, moved, d.x, moved)):
+        lo, hi = x.min(axis=0), x.max(axis=0)
+        box = np.array([float(v) for v in kv["box"].split(",")])
+        org = np.array([float(v) for v in kv["origin"].split(",")])
+        assert int(kv["periodic"]) == 0
+        assert np.allclose(box, hi - lo + 20.0, rtol=0, atol=1e-12)
+        assert np.allclose(org, 0.5 * (lo + hi), rtol=0, atol=1e-12)
+    assert (int(creates[0]["term_a"]), int(creates[0]["term_b"]), float(creates[0]["sw_a.cutoff"])) == (1, 2, 8.0)
+    assert (int(creates[2]["term_a"]), int(creates[2]["term_b"]), float(creates[2]["cutoff"])) == (2, 0, 10.0)
+    for k in range(4):
+        assert np.all(out[f"f_hip{k}"] == 1.0) and out[f"e_hip{k}"][0] == 0.5

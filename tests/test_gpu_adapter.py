@@ -68,3 +68,20 @@ def test_gb_force_group_adapters_vs_reference_forces():
              gb_params=np.array([0.09, 1.0, 0.8, 4.85, 1.0, 78.3]), cellsize=np.int32(5))
     out = _run(r)
     _check(out, 0, "GB + ACE + LJ + Coulomb, all pairs", 3 * 300 * 299 // 2)
+
+
+@pytest.mark.skipif(not os.path.exists(CHECK), reason="oracle/_ref/adapter_check is built where the reference is")
+def test_vacuum_cutoff_adapters_vs_reference_forces():
+    """OneAtomPair[Two]<VacuumBoundaryConditions, ...> on NonbondedCutoffHIP:
+    a water droplet, then the same droplet with one molecule moved out of the
+    context's region (the adapter makes its context again)."""
+    import refio
+    print("case adapters vacuum cutoff (water droplet 1,164 atoms)", flush=True)
+    d = S.water_droplet(14.0, seed=3)
+    r = refio.system_records(d)
+    r.update(mode=np.int32(6), cellsize=5.0, exit=15.0)
+    out = _run(r)
+    assert int(out["cases"][0]) == 4
+    for k, name in enumerate(["LJ C2 6-8 A + Coulomb C2 8-10 A", "the same, one molecule out of the region",
+                              "Coulomb C1", "Coulomb C1, one molecule out of the region"]):
+        _check(out, k, name, 150000)

@@ -102,6 +102,90 @@ def test_cutoff_vs_reference_golden(model):
     check_virial(e, g["e"], npairs=cen[0])
 
 
+VACUUM_MODELS = {
+    # the reference's NonbondedCutoffSystemForce over VacuumBoundaryConditions
+    "c2c2_two": (L.TERM_LJ, (3, 6.0, 8.0, 0.0, 0.0), L.TERM_COULOMB, (3, 8.0, 10.0, 0.0, 0.0)),
+    "lj_c2": (L.TERM_LJ, (3, 6.0, 8.0, 0.0, 0.0), L.TERM_NONE, (0, 0, 0, 0, 0)),
+    "coul_c1": (L.TERM_COULOMB, (2, 0.0, 10.0, 0.0, 0.0), L.TERM_NONE, (0, 0, 0, 0, 0)),
+}
+
+
+@pytest.mark.parametrize("model", list(VACUUM_MODELS))
+def test_vacuum_cutoff_vs_reference_golden(model):
+    """A water droplet in vacuum (no periodic images) on the cell-list path:
+    pair set, forces, energies, virial and molecular virial."""
+    print(f"case vacuum cutoff/{model}", flush=True)
+    g = T.golden(f"ref_vacuum_{model}.npz")
+    d = S.water_droplet(14.0, seed=3)
+    ta, sa, tb, sb = VACUUM_MODELS[model]
+    cut = max(sa[2], sb[2])
+    desc = force_desc(ta, sa, tb, sb, flags=L.FLAG_VIRIAL | L.FLAG_MOLVIRIAL, cutoff=cut)
+    with L.Context(desc, [d]) as ctx:
+        f, e = evaluate(ctx, 0, d.x)
+        cen = ctx.census(0, d.x)
+    assert cen == (int(g["census"][0]), int(g["census"][1]))
+    assert T.rms_rel(f, g["f"]) <= RTOL_F
+    check_energies(e, g["e"], npairs=cen[0])
+    check_virial(e, g["e"], npairs=cen[0])
+    check_tensor(e[16:25], g["e"][16:25], cen[0])
+
+
+def _droplet_forces(d, sw):
+    def forces(x):
+        f, e, _ = O.nonbonded_cutoff(d, O.TERM_LJ, sw, O.TERM_COULOMB, sw, x=x, cellsize=5.0)
+        fb, eb = O.bonded(d, x=x)
+        return f + fb, e + eb
+    return forces
+
+
+def test_md_vacuum_droplet_vs_oracle_leapfrog():
+    """Leapfrog of a water droplet in vacuum against the oracle's FP64
+    leapfrog, with list rebuilds inside the run."""
+    print("case md vacuum droplet", flush=True)
+    d = S.water_droplet(14.0, seed=3)
+    v0 = S.maxwell_velocities(d, 300.0, seed=2)
+    steps, dt = 20, 0.5
+    xo, vo, Fo, Eo = T.leapfrog_oracle(_droplet_forces(d, O.make_switch(O.SW_C2, 8.0, 10.0)), d.x.copy(),
+                                       v0.copy(), d.mass, dt, steps)
+    ta, sa, tb, sb = MODELS["c2c2"]
+    x, v, F, E, st = _md(force_desc(ta, sa, tb, sb, flags=L.FLAG_BONDED, skin=1.5, interval=5), [d], [v0],
+                         steps, dt)
+    assert np.max(np.abs(x - xo)) < 1e-4
+    assert T.rms_rel(v, vo) < 1e-4
+    assert T.rms_rel(F, Fo) < 1e-4
+    for s in (0, 1, 2, 3):
+        assert abs(E[0][s] - Eo[s]) <= 1e-4 * max(1.0, abs(Eo[s]))
+    assert st["rebuilds"] >= steps // 5
+
+
+def test_vacuum_atom_outside_region_is_an_error():
+    """An atom outside the region (origin +- box / 2) of a vacuum system stops
+    an evaluation and an MD run with PM_ECAPACITY; the context stays usable."""
+    print("case vacuum region exit", flush=True)
+    d = S.water_droplet(14.0, seed=3)
+    ta, sa, tb, sb = MODELS["c2c2"]
+    with L.Context(force_desc(ta, sa, tb, sb), [d]) as ctx:
+        f_in, e_in = evaluate(ctx, 0, d.x)
+        x = d.x.copy()
+        x[1, 0] = d.origin[0] + 0.5 * d.box[0] + 0.25  # one hydrogen just past the +x face
+        with pytest.raises(L.PMError) as ei:
+            evaluate(ctx, 0, x)
+        assert ei.value.code == L.PM_ECAPACITY and "region" in str(ei.value)
+        f2, e2 = evaluate(ctx, 0, d.x)
+    assert np.array_equal(f2, f_in) and np.array_equal(e2, e_in)
+    # the molecule with the outermost oxygen (+x) leaves through the +x face at
+    # 0.25 A/fs (velocities in A per 1/INV_TIMEFACTOR fs): past the 10 A
+    # margin well within 100 steps of 1 fs
+    o = np.argmax(d.x[0::3, 0])
+    v0 = np.zeros_like(d.x)
+    v0[3 * o:3 * o + 3, 0] = 0.25 / S.INV_TIMEFACTOR
+    with L.Context(force_desc(ta, sa, tb, sb, flags=L.FLAG_BONDED, skin=1.0, interval=5), [d]) as ctx:
+        ctx.md_set_state(d.x, v0)
+        with pytest.raises(L.PMError) as ei:
+            ctx.md_run(100, 1.0, energies=False)
+        assert ei.value.code == L.PM_ECAPACITY and "region" in str(ei.value)
+
+
 @pytest.mark.parametrize("model", ["c2c2", "c2c2_two", "cncn", "cut_c2_two"])
 def test_md_forces_vs_reference_golden(model):
     # the MD path's forces-only pair kernels (F(x0) of pm_md_set_state)

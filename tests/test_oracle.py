@@ -39,6 +39,30 @@ def test_cutoff_matches_reference(case):
     assert (int(cen[0]), int(cen[1])) == (int(g["census"][0]), int(g["census"][1]))
 
 
+VACUUM_CASES = ["c2c2_two", "lj_c2", "coul_c1"]
+
+
+@pytest.mark.parametrize("case", VACUUM_CASES)
+def test_vacuum_cutoff_matches_reference(case):
+    """NonbondedCutoffSystemForce over VacuumBoundaryConditions (a water
+    droplet, no images): forces, energies, virial and molecular virial from
+    the reference's own classes, plus the pair set against brute force."""
+    g = T.golden(f"ref_vacuum_{case}.npz")
+    d = S.water_droplet(14.0, seed=3)
+    assert not d.periodic and np.array_equal(d.x, g["x"])
+    ta, tb = int(g["term_a"]), int(g["term_b"])
+    swa = _switch_from_driver(g["sw_a"])
+    swb = _switch_from_driver(g["sw_b"]) if tb else None
+    cut = max(float(g["sw_a"][2]), float(g["sw_b"][2]) if tb else 0.0)
+    f, e, cen = O.nonbonded_cutoff(d, ta, swa, tb, swb, algorithm_cutoff=cut, cellsize=5.0, virial=True,
+                                   molvirial=True)
+    gf = g["f"]
+    assert np.max(np.abs(f - gf)) <= 1e-11 * np.max(np.abs(gf))
+    assert np.allclose(e, g["e"], rtol=1e-11, atol=1e-11 * np.max(np.abs(g["e"])))
+    assert (int(cen[0]), int(cen[1])) == (int(g["census"][0]), int(g["census"][1]))
+    assert cen == O.census_bruteforce(d, cut * cut)
+
+
 def test_cutoff_multithreaded_matches_serial():
     w = T.fixture_water()
     sw = O.make_switch(O.SW_C2, 8.0, 10.0)
