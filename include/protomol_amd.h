@@ -128,8 +128,11 @@ typedef struct pm_system_desc {
   const int32_t *exclusions;   /* n_exclusions * 3: atom a, atom b, PM_EXCL_* (ExclusionTable::add order) */
   double coulomb_scaling;      /* GenericTopology::coulombScalingFactor */
   int32_t periodic;            /* 1: orthogonal PeriodicBoundaryConditions, 0: vacuum */
-  double box[3];               /* cellBasisVector1.x, 2.y, 3.z */
-  double origin[3];            /* cellorigin */
+  double box[3];               /* periodic: cellBasisVector1.x, 2.y, 3.z; vacuum on the cell-list
+                                  path: the extent of the region the atoms stay in (an atom
+                                  outside origin +- box / 2 is PM_ECAPACITY); vacuum all-pairs
+                                  (PM_FLAG_GB, PM_FLAG_ALL_PAIRS): unused */
+  double origin[3];            /* cellorigin; vacuum: the region's centre */
   int32_t n_bonds;
   const int32_t *bonds;        /* 2 per bond */
   const double *bond_k, *bond_r0;
@@ -248,7 +251,7 @@ int pm_system_set_seed(pm_ctx *ctx, int32_t id, uint32_t seed);
  * energies into e[36] as pm_evaluate does.  Every device array -- the
  * system's tables, cell and pair lists, the launch table -- is carved out of
  * a static device array of 64 MiB that each GPU holds until the process ends
- * (about 0.8 KB per atom of water at a 10 A cutoff: some 75,000 atoms fit); a
+ * (about 1.1 KB per atom of water at a 10 A cutoff: 60,000 atoms fit); a
  * system that needs more returns PM_ECAPACITY.  No reciprocal-space term
  * (PM_FLAG_EWALD_RECIPROCAL returns PM_EINVAL).  Runs on the null stream of
  * the current device and returns when done.  All calls on one GPU share its
