@@ -69,6 +69,49 @@ __device__ __forceinline__ float dpp_sum8_scatter(float x, float y, float z, int
   return c + r4;
 }
 
+// gfx950 cross-row swaps: swap32 exchanges the upper 32 lanes of a with the
+// lower 32 lanes of b; swap16 exchanges the odd 16-lane rows of a with the
+// even rows of b.  Afterwards a + b holds, in each half (row), the sum of one
+// register's two halves (rows): the first register's in the lower half (even
+// rows), the second's in the upper half (odd rows).
+__device__ __forceinline__ void swap32(float &a, float &b) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+  a = __uint_as_float(r[0]);
+  b = __uint_as_float(r[1]);
+}
+__device__ __forceinline__ void swap16(float &a, float &b) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+  a = __uint_as_float(r[0]);
+  b = __uint_as_float(r[1]);
+}
+
+// The i forces of a super-cluster: v[3 c + k] holds component k of i cluster
+// c for the lane's (jj, ii) = (lane >> 3, lane & 7).  Sums each of the 24
+// over the 8 lanes of equal ii (the jj lanes) and scatters the results so
+// lane (jj, ii) ends with cluster jj's three components for atom ii in v[0..2]:
+// halving steps over lane bit 5 (swap32), bit 4 (swap16) and bit 3 (DPP row
+// rotate by 8), each keeping the half of the values that the lane's bit
+// selects.  Fixed order: the result bits depend on the inputs only.
+__device__ __forceinline__ void reduce_scatter_i(float (&v)[3 * kScCl], int lane) {
+#pragma unroll
+  for (int m = 0; m < 12; ++m) {
+    swap32(v[m], v[m + 12]);
+    v[m] += v[m + 12];
+  }
+#pragma unroll
+  for (int m = 0; m < 6; ++m) {
+    swap16(v[m], v[m + 6]);
+    v[m] += v[m + 6];
+  }
+  const bool b0 = (lane >> 3) & 1;
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const float send = b0 ? v[m] : v[m + 3];
+    const float keep = b0 ? v[m + 3] : v[m];
+    v[m] = keep + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(send), 0x128, 0xF, 0xF, false));
+  }
+}
+
 constexpr int kLJLds = 256;  // LJ table entries kept in LDS per wave (T <= 16; larger tables are read from L1/L2)
 // LDS per wave of the pair kernel: the LJ table as (A, B, 12 A, 6 B), then the
 // super-cluster's 64 i atoms (position + charge, byte offset of the atom's
@@ -420,27 +463,22 @@ __device__ __forceinline__ void pair_wave(const SysDev *__restrict__ S, const in
       if (ii < 3 && jj < nvj) atomic_add_ll(&gfc[sj], to_fixed((double)g, kFScale));
     }
   }
-  // i force = sum over the 8 j lanes that share ii
+  // i force = sum over the 8 j lanes that share ii; lane (jj, ii) then holds
+  // atom ii of i cluster jj, which is real where valid_ca has its bit
   if (FORCES) {
+    float v[3 * kScCl];
 #pragma unroll
     for (int c = 0; c < kScCl; ++c) {
-      if (c >= nic) break;
-      float ax = fi[c][0], ay = fi[c][1], az = fi[c][2];
-      ax += __shfl_xor(ax, 8);
-      ay += __shfl_xor(ay, 8);
-      az += __shfl_xor(az, 8);
-      ax += __shfl_xor(ax, 16);
-      ay += __shfl_xor(ay, 16);
-      az += __shfl_xor(az, 16);
-      ax += __shfl_xor(ax, 32);
-      ay += __shfl_xor(ay, 32);
-      az += __shfl_xor(az, 32);
-      if (jj == 0 && lane_bit(vim[c])) {
-        const int si = (cl0 + c) * kCluster + ii;
-        atomic_add_ll(&gfx[si], to_fixed((double)ax, kFScale));
-        atomic_add_ll(&gfy[si], to_fixed((double)ay, kFScale));
-        atomic_add_ll(&gfz[si], to_fixed((double)az, kFScale));
-      }
+      v[3 * c] = fi[c][0];
+      v[3 * c + 1] = fi[c][1];
+      v[3 * c + 2] = fi[c][2];
+    }
+    reduce_scatter_i(v, lane);
+    if (lane_bit(valid_ca)) {
+      const int si = (cl0 + jj) * kCluster + ii;
+      atomic_add_ll(&gfx[si], to_fixed((double)v[0], kFScale));
+      atomic_add_ll(&gfy[si], to_fixed((double)v[1], kFScale));
+      atomic_add_ll(&gfz[si], to_fixed((double)v[2], kFScale));
     }
   }
   if (EN) {
