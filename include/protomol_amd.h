@@ -191,6 +191,15 @@ int pm_ctx_memory(pm_ctx *ctx, uint64_t *reserved, uint64_t *used);
 int pm_system_footprint(pm_ctx *ctx, const pm_system_desc *system, uint64_t *bytes);
 /* Add a system (copied; the description may be freed afterwards). */
 int pm_system_add(pm_ctx *ctx, const pm_system_desc *system, int32_t *id);
+/* Add n systems in order, as n pm_system_add calls would, for a serving loop
+ * that admits many requests at once: the host tables are prepared on worker
+ * threads and each system's tables reach the device in one staged copy.
+ * ids[k] receives system k's id, or -1 when the reservation has no room for
+ * it; after max_full (>= 1) such systems the rest are not tried (-1 as well).
+ * PM_OK unless another failure occurs, which removes the systems this call
+ * added.  (No counterpart in the reference, whose systems are fixed at
+ * set-up; it batches what pm_system_add does one system at a time.) */
+int pm_system_add_many(pm_ctx *ctx, int32_t n, const pm_system_desc *systems, int32_t max_full, int32_t *ids);
 /* Remove a system and give its device memory back. */
 int pm_system_remove(pm_ctx *ctx, int32_t id);
 /* Last error message for ctx (or the calling thread when ctx is NULL). */

@@ -23,7 +23,8 @@ FLAG_ALL_PAIRS = 128
 NSLOTS = 36
 
 EXPORTED = ("pm_abi_version", "pm_device_info", "pm_ctx_create", "pm_ctx_destroy", "pm_ctx_reserve",
-            "pm_ctx_memory", "pm_system_footprint", "pm_system_add", "pm_system_remove", "pm_last_error",
+            "pm_ctx_memory", "pm_system_footprint", "pm_system_add", "pm_system_add_many", "pm_system_remove",
+            "pm_last_error",
             "pm_evaluate", "pm_evaluate_batch", "pm_evaluate_systems", "pm_pair_census", "pm_md_set_state",
             "pm_md_run", "pm_md_get_state", "pm_md_set_state_systems", "pm_md_begin", "pm_md_advance",
             "pm_md_end", "pm_md_get_state_systems", "pm_md_set_langevin", "pm_system_set_seed", "pm_stats",
@@ -92,6 +93,7 @@ def lib():
         L.pm_ctx_memory.argtypes = [C.c_void_p, u64p, u64p]
         L.pm_system_footprint.argtypes = [C.c_void_p, C.POINTER(SystemDesc), u64p]
         L.pm_system_add.argtypes = [C.c_void_p, C.POINTER(SystemDesc), _ip]
+        L.pm_system_add_many.argtypes = [C.c_void_p, C.c_int32, C.POINTER(SystemDesc), C.c_int32, _ip]
         L.pm_system_remove.argtypes = [C.c_void_p, C.c_int32]
         L.pm_evaluate_systems.argtypes = [C.c_void_p, C.c_int32, _ip, _dp, _dp, _dp]
         L.pm_md_set_state_systems.argtypes = [C.c_void_p, C.c_int32, _ip, _dp, _dp]
@@ -258,6 +260,24 @@ class Context:
         check(lib().pm_system_add(self.h, C.byref(_Keep(system, self._bonded).desc), C.byref(i)), self.h)
         self.members[i.value] = system
         return i.value
+
+    def add_many(self, systems, max_full: int = 1 << 30) -> list:
+        """Add systems in order (pm_system_add_many): a list of ids, None for a
+        system the reservation had no room for (and for every system after
+        ``max_full`` of those, which are not tried)."""
+        systems = list(systems)
+        if not systems:
+            return []
+        keep = [_Keep(s, self._bonded) for s in systems]
+        arr = (SystemDesc * len(systems))(*[k.desc for k in keep])
+        ids = np.full(len(systems), -1, np.int32)
+        check(lib().pm_system_add_many(self.h, len(systems), arr, max(1, int(max_full)), ptr(ids, _ip)), self.h)
+        out = []
+        for s, i in zip(systems, ids.tolist()):
+            if i >= 0:
+                self.members[i] = s
+            out.append(i if i >= 0 else None)
+        return out
 
     def remove(self, sid: int):
         check(lib().pm_system_remove(self.h, sid), self.h)

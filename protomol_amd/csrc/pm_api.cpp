@@ -13,6 +13,9 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <atomic>
+#include <exception>
+#include <thread>
 #include <string>
 #include <utility>
 #include <vector>
@@ -141,6 +144,7 @@ struct SysPrep {
   // molecules (molecular virial): CSR of their atoms in index order
   int n_mol = 0;
   std::vector<int> mol_start, mol_atoms;
+  std::vector<int> gb_ident;  // GB: slot = atom
   // the periodic cell of the cutoff path: the system's own box, or for a
   // vacuum system one that encloses its region (vac: the region's bounds)
   double box[3] = {0, 0, 0}, origin[3] = {0, 0, 0};
@@ -302,6 +306,12 @@ struct pm_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evp0 = nullptr, evp1 = nullptr, evr0 = nullptr, evr1 = nullptr;
   std::vector<hipEvent_t> evs;  // PM_FLAG_TIMING: start/stop per pair launch of an MD call
   int evs_used = 0;
+  // pinned host staging of large transfers (the tables of systems being added,
+  // positions and velocities of many systems); pin_done is recorded after the
+  // copies that read it
+  char *pin = nullptr;
+  size_t pin_cap = 0;
+  hipEvent_t pin_done = nullptr;
 };
 
 namespace {
@@ -623,6 +633,8 @@ SysPrep prepare(const pm_ctx *c, const pm_system_desc &d) {
     p.n_gb_tiles = pm::gb_tile_count(n);
     p.cl_cap = 0;
     p.slot_cap = n;  // forces are kept per atom: slot = atom
+    p.gb_ident.resize(n);
+    for (int i = 0; i < n; ++i) p.gb_ident[i] = i;
     return p;
   }
   const double rlist = c->rc + std::max(c->skin_md, c->skin_eval);
@@ -711,6 +723,30 @@ size_t carve(const pm_ctx *c, const SysPrep &p, const pm_system_desc &d, Carver 
     S.lj_table = cv.take<float4>(p.lj.size());
     S.lj_table12 = cv.take<float4>(p.lj12.size());
   }
+  S.bonds = cv.take<int>(2 * (size_t)p.n_bonds);
+  S.bond_k = cv.take<double>(p.n_bonds);
+  S.bond_r0 = cv.take<double>(p.n_bonds);
+  S.angles = cv.take<int>(3 * (size_t)p.n_angles);
+  S.angle_k = cv.take<double>(p.n_angles);
+  S.angle_t0 = cv.take<double>(p.n_angles);
+  if (c->molvir) {
+    S.mol_start = cv.take<int>(p.mol_start.size());
+    S.mol_atoms = cv.take<int>(p.mol_atoms.size());
+    S.mass = cv.take<double>(n);
+  }
+  if (c->gb) {
+    S.gb_rho = cv.take<double>(n);
+    S.gb_S = cv.take<double>(n);
+  }
+  if (c->recip) {
+    S.ew_rows = cv.take<int4>(p.ew_rows.size());
+    S.ew_kd = cv.take<double4>(p.ew_kd.size());
+    S.ew_tiles = cv.take<int4>(p.ew_tiles.size());
+    S.ew_pts = cv.take<short2>(p.ew_pts.size());
+    S.ew_kmap = cv.take<int>(p.ew_kmap.size());
+    S.ew_fb_rows = cv.take<int>(p.ew_fb_rows.size());
+  }
+  // (the tables above are uploaded; everything below starts at zero)
   S.x = cv.take<double>(3 * (size_t)n);
   S.v = cv.take<double>(3 * (size_t)n);
   S.xref = cv.take<double>(3 * (size_t)n);
@@ -719,12 +755,6 @@ size_t carve(const pm_ctx *c, const SysPrep &p, const pm_system_desc &d, Carver 
   S.faos = cv.take<double>(3 * (size_t)n);
   S.slot = cv.take<int>(n);
   S.lbuild = cv.take<float4>(n);
-  S.bonds = cv.take<int>(2 * (size_t)p.n_bonds);
-  S.bond_k = cv.take<double>(p.n_bonds);
-  S.bond_r0 = cv.take<double>(p.n_bonds);
-  S.angles = cv.take<int>(3 * (size_t)p.n_angles);
-  S.angle_k = cv.take<double>(p.n_angles);
-  S.angle_t0 = cv.take<double>(p.n_angles);
   S.n_clusters = cv.take<int>(1);
   S.acc = cv.take<long long>((size_t)pm::kAccN * pm::kAccCopies);
   S.census = cv.take<unsigned long long>(2);
@@ -743,14 +773,9 @@ size_t carve(const pm_ctx *c, const SysPrep &p, const pm_system_desc &d, Carver 
   S.fl = cv.take<long long>(3 * (size_t)n);
   if (c->molvir) {
     S.n_mol = p.n_mol;
-    S.mol_start = cv.take<int>(p.mol_start.size());
-    S.mol_atoms = cv.take<int>(p.mol_atoms.size());
-    S.mass = cv.take<double>(n);
     S.com = cv.take<double>(3 * (size_t)p.n_mol);
   }
   if (c->gb) {
-    S.gb_rho = cv.take<double>(n);
-    S.gb_S = cv.take<double>(n);
     S.gb_born = cv.take<double>(n);
     S.gb_drfac = cv.take<double>(n);
     S.gb_partial = cv.take<double>(n);
@@ -781,12 +806,6 @@ size_t carve(const pm_ctx *c, const SysPrep &p, const pm_system_desc &d, Carver 
   S.list_len = cv.take<int>(p.sc_cap);
   if (c->recip) {
     S.ew_tab = cv.take<double2>((size_t)(p.ew_xrows + p.ew_yrows + p.ew_zrows) * n);
-    S.ew_rows = cv.take<int4>(p.ew_rows.size());
-    S.ew_kd = cv.take<double4>(p.ew_kd.size());
-    S.ew_tiles = cv.take<int4>(p.ew_tiles.size());
-    S.ew_pts = cv.take<short2>(p.ew_pts.size());
-    S.ew_kmap = cv.take<int>(p.ew_kmap.size());
-    S.ew_fb_rows = cv.take<int>(p.ew_fb_rows.size());
     S.ew_psum = cv.take<double2>((size_t)p.ew_nchunks * p.ew_nk);
     S.ew_sk = cv.take<double4>(p.ew_kd.size());
     S.ew_pf = cv.take<double>((size_t)(p.ew_fb_rows.size() - 1) * 3 * n);
@@ -834,9 +853,103 @@ uint64_t footprint(const pm_ctx *c, const pm_system_desc &d) {
   return b;
 }
 
+// Table uploads of systems being added: gathered, then copied from one pinned
+// staging buffer, one copy per run of adjacent tables (a system's uploaded
+// tables lie first in its block, so one run each).
+struct Uploads {
+  struct Seg {
+    char *dst;
+    const void *src;
+    size_t bytes;
+  };
+  std::vector<Seg> segs;
+};
+
 template <class T>
-void upload(T *dst, const T *src, size_t n) {
-  if (n && src) HIPCHECK(hipMemcpy(dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+void upload(Uploads *up, T *dst, const T *src, size_t n) {
+  if (!n || !src) return;
+  if (up) up->segs.push_back({reinterpret_cast<char *>(dst), src, n * sizeof(T)});
+  else HIPCHECK(hipMemcpy(dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+}
+
+// f(0) .. f(count - 1) on up to 16 host threads
+void parallel_for(int count, const std::function<void(int)> &f) {
+  const int nt = std::max(1, std::min({count, 16, (int)std::max(1u, std::thread::hardware_concurrency())}));
+  if (nt == 1) {
+    for (int i = 0; i < count; ++i) f(i);
+    return;
+  }
+  std::atomic<int> next{0};
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; ++t)
+    th.emplace_back([&] {
+      for (int i = next++; i < count; i = next++) f(i);
+    });
+  for (auto &t : th) t.join();
+}
+
+// transfers at least this large go through the pinned staging buffer
+constexpr size_t kPinnedTransfer = 16u << 20;
+
+// The pinned staging buffer with room for `bytes`, free to be written (the
+// copies of its previous use have finished).
+char *pinned(pm_ctx *c, size_t bytes) {
+  if (!c->pin_done) HIPCHECK(hipEventCreateWithFlags(&c->pin_done, hipEventDisableTiming));
+  HIPCHECK(hipEventSynchronize(c->pin_done));
+  if (bytes > c->pin_cap) {
+    if (c->pin) (void)hipHostFree(c->pin);
+    c->pin = nullptr;
+    c->pin_cap = 0;
+    const size_t cap = std::max(bytes + bytes / 2, (size_t)(4u << 20));
+    if (hipHostMalloc(reinterpret_cast<void **>(&c->pin), cap, hipHostMallocDefault) != hipSuccess)
+      throw Fail{PM_ENOMEM, "hipHostMalloc of the transfer staging buffer failed"};
+    c->pin_cap = cap;
+  }
+  return c->pin;
+}
+
+// Issue the gathered uploads on the context's stream (asynchronous: the
+// caller synchronizes the stream before the systems are used; the next use of
+// the staging buffer waits for these copies).
+void flush_uploads(pm_ctx *c, Uploads &up) {
+  if (up.segs.empty()) return;
+  std::sort(up.segs.begin(), up.segs.end(), [](const Uploads::Seg &a, const Uploads::Seg &b) { return a.dst < b.dst; });
+  // runs: segments whose gaps are alignment padding (< 4 KiB) travel together
+  struct Run {
+    char *dst;
+    size_t bytes, stage;
+  };
+  std::vector<Run> runs;
+  size_t total = 0;
+  for (const auto &g : up.segs) {
+    if (!runs.empty() && g.dst >= runs.back().dst + runs.back().bytes &&
+        g.dst - (runs.back().dst + runs.back().bytes) < 4096) {
+      runs.back().bytes = (size_t)(g.dst + g.bytes - runs.back().dst);
+    } else {
+      total = align_up(total);
+      runs.push_back({g.dst, g.bytes, total});
+    }
+    total = runs.back().stage + runs.back().bytes;
+  }
+  char *pin = pinned(c, total);
+  // segment k's run; each segment also clears the padding before it in its run
+  std::vector<int> run_of(up.segs.size());
+  for (size_t k = 0, r = 0; k < up.segs.size(); ++k) {
+    const auto &g = up.segs[k];
+    while (!(g.dst >= runs[r].dst && g.dst + g.bytes <= runs[r].dst + runs[r].bytes)) ++r;
+    run_of[k] = (int)r;
+  }
+  parallel_for((int)up.segs.size(), [&](int k) {
+    const auto &g = up.segs[k];
+    const Run &R = runs[run_of[k]];
+    char *gap0 = (k > 0 && run_of[k - 1] == run_of[k]) ? up.segs[k - 1].dst + up.segs[k - 1].bytes : R.dst;
+    if (g.dst > gap0) std::memset(pin + R.stage + (gap0 - R.dst), 0, (size_t)(g.dst - gap0));
+    std::memcpy(pin + R.stage + (g.dst - R.dst), g.src, g.bytes);
+  });
+  for (const Run &R : runs)
+    HIPCHECK(hipMemcpyAsync(R.dst, pin + R.stage, R.bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipEventRecord(c->pin_done, c->stream));
+  up.segs.clear();
 }
 
 // The host record of a system and the scalars of its device record, from
@@ -924,9 +1037,11 @@ void init_system(const pm_ctx *c, const pm_system_desc &d, const SysPrep &p, Sys
   h.block_bytes = carve(c, p, d, dry, tmp, &snap);
 }
 
-// Lay the system's arrays out in h.block, clear them (a fill kernel when
-// `fill`, else a memset) and upload its tables.
-void fill_block(pm_ctx *c, const pm_system_desc &d, const SysPrep &p, SysHost &h, bool fill) {
+// Lay the system's arrays out in h.block, clear them (a fill kernel on the
+// context's stream when `fill`, else a memset) and upload its tables: into
+// `up` when given (copied by flush_uploads, on the same stream after the
+// fill), else at once.  `p` must outlive the flush.
+void fill_block(pm_ctx *c, const pm_system_desc &d, const SysPrep &p, SysHost &h, bool fill, Uploads *up = nullptr) {
   SysDev &S = h.dev;
   const int n = p.n;
   Carver cv;
@@ -936,68 +1051,75 @@ void fill_block(pm_ctx *c, const pm_system_desc &d, const SysPrep &p, SysHost &h
   // block's end padded to 256 bytes)
   if (fill) pm::launch_fill_zero(h.block, (h.block_bytes + 7) / 8 * 8, c->stream);
   else HIPCHECK(hipMemset(h.block, 0, h.block_bytes));
-  upload(const_cast<int *>(S.excl_start), p.excl_start.data(), p.excl_start.size());
-  upload(const_cast<int *>(S.excl_partner), p.excl_partner.data(), p.excl_partner.size());
-  upload(const_cast<int *>(S.excl_class), p.excl_class.data(), p.excl_class.size());
-  upload(const_cast<unsigned long long *>(S.excl_win), p.excl_win.data(), p.excl_win.size());
-  upload(const_cast<int *>(S.corr), p.corr.data(), p.corr.size());
-  upload(const_cast<int *>(S.loc_start), p.loc_start.data(), p.loc_start.size());
-  upload(const_cast<int *>(S.loc_item), p.loc_item.data(), p.loc_item.size());
-  upload(const_cast<double *>(S.charge), d.scaled_charge, n);
-  upload(const_cast<double *>(S.inv_mass), p.invm.data(), n);
-  upload(const_cast<int *>(S.molecule), d.molecule, n);
+  upload(up, const_cast<int *>(S.excl_start), p.excl_start.data(), p.excl_start.size());
+  upload(up, const_cast<int *>(S.excl_partner), p.excl_partner.data(), p.excl_partner.size());
+  upload(up, const_cast<int *>(S.excl_class), p.excl_class.data(), p.excl_class.size());
+  upload(up, const_cast<unsigned long long *>(S.excl_win), p.excl_win.data(), p.excl_win.size());
+  upload(up, const_cast<int *>(S.corr), p.corr.data(), p.corr.size());
+  upload(up, const_cast<int *>(S.loc_start), p.loc_start.data(), p.loc_start.size());
+  upload(up, const_cast<int *>(S.loc_item), p.loc_item.data(), p.loc_item.size());
+  upload(up, const_cast<double *>(S.charge), d.scaled_charge, n);
+  upload(up, const_cast<double *>(S.inv_mass), p.invm.data(), n);
+  upload(up, const_cast<int *>(S.molecule), d.molecule, n);
   if (c->molvir) {
-    upload(const_cast<int *>(S.mol_start), p.mol_start.data(), p.mol_start.size());
-    upload(const_cast<int *>(S.mol_atoms), p.mol_atoms.data(), p.mol_atoms.size());
-    upload(const_cast<double *>(S.mass), d.mass, n);
+    upload(up, const_cast<int *>(S.mol_start), p.mol_start.data(), p.mol_start.size());
+    upload(up, const_cast<int *>(S.mol_atoms), p.mol_atoms.data(), p.mol_atoms.size());
+    upload(up, const_cast<double *>(S.mass), d.mass, n);
   }
   if (d.lj_mode == PM_LJ_PERATOM) {
-    upload(const_cast<float *>(S.sig_half), p.sig_half.data(), n);
-    upload(const_cast<float *>(S.eps_sqrt), p.eps_sqrt.data(), n);
+    upload(up, const_cast<float *>(S.sig_half), p.sig_half.data(), n);
+    upload(up, const_cast<float *>(S.eps_sqrt), p.eps_sqrt.data(), n);
   } else {
-    upload(const_cast<int *>(S.type), d.type, n);
-    upload(const_cast<float4 *>(S.lj_table), p.lj.data(), p.lj.size());
-    upload(const_cast<float4 *>(S.lj_table12), p.lj12.data(), p.lj12.size());
+    upload(up, const_cast<int *>(S.type), d.type, n);
+    upload(up, const_cast<float4 *>(S.lj_table), p.lj.data(), p.lj.size());
+    upload(up, const_cast<float4 *>(S.lj_table12), p.lj12.data(), p.lj12.size());
   }
-  upload(const_cast<int *>(S.bonds), d.bonds, 2 * (size_t)p.n_bonds);
-  upload(const_cast<double *>(S.bond_k), d.bond_k, p.n_bonds);
-  upload(const_cast<double *>(S.bond_r0), d.bond_r0, p.n_bonds);
-  upload(const_cast<int *>(S.angles), d.angles, 3 * (size_t)p.n_angles);
-  upload(const_cast<double *>(S.angle_k), d.angle_k, p.n_angles);
-  upload(const_cast<double *>(S.angle_t0), d.angle_theta0, p.n_angles);
+  upload(up, const_cast<int *>(S.bonds), d.bonds, 2 * (size_t)p.n_bonds);
+  upload(up, const_cast<double *>(S.bond_k), d.bond_k, p.n_bonds);
+  upload(up, const_cast<double *>(S.bond_r0), d.bond_r0, p.n_bonds);
+  upload(up, const_cast<int *>(S.angles), d.angles, 3 * (size_t)p.n_angles);
+  upload(up, const_cast<double *>(S.angle_k), d.angle_k, p.n_angles);
+  upload(up, const_cast<double *>(S.angle_t0), d.angle_theta0, p.n_angles);
   if (c->gb) {
     if (c->P.gb_on) {
-      upload(const_cast<double *>(S.gb_rho), d.gb_radius, n);
-      upload(const_cast<double *>(S.gb_S), d.gb_scaling, n);
+      upload(up, const_cast<double *>(S.gb_rho), d.gb_radius, n);
+      upload(up, const_cast<double *>(S.gb_S), d.gb_scaling, n);
     }
-    std::vector<int> ident(n);
-    for (int i = 0; i < n; ++i) ident[i] = i;
-    upload(S.slot, ident.data(), n);
+    upload(up, S.slot, p.gb_ident.data(), n);
     S.ncx = S.ncy = S.ncol = 1;
   }
   if (c->recip) {
-    upload(const_cast<int4 *>(S.ew_rows), p.ew_rows.data(), p.ew_rows.size());
-    upload(const_cast<double4 *>(S.ew_kd), p.ew_kd.data(), p.ew_kd.size());
-    upload(const_cast<int4 *>(S.ew_tiles), p.ew_tiles.data(), p.ew_tiles.size());
-    upload(const_cast<short2 *>(S.ew_pts), p.ew_pts.data(), p.ew_pts.size());
-    upload(const_cast<int *>(S.ew_kmap), p.ew_kmap.data(), p.ew_kmap.size());
-    upload(const_cast<int *>(S.ew_fb_rows), p.ew_fb_rows.data(), p.ew_fb_rows.size());
+    upload(up, const_cast<int4 *>(S.ew_rows), p.ew_rows.data(), p.ew_rows.size());
+    upload(up, const_cast<double4 *>(S.ew_kd), p.ew_kd.data(), p.ew_kd.size());
+    upload(up, const_cast<int4 *>(S.ew_tiles), p.ew_tiles.data(), p.ew_tiles.size());
+    upload(up, const_cast<short2 *>(S.ew_pts), p.ew_pts.data(), p.ew_pts.size());
+    upload(up, const_cast<int *>(S.ew_kmap), p.ew_kmap.data(), p.ew_kmap.size());
+    upload(up, const_cast<int *>(S.ew_fb_rows), p.ew_fb_rows.data(), p.ew_fb_rows.size());
   }
 }
 
-// add a system; returns its id
-int add_system(pm_ctx *c, const pm_system_desc &d) {
-  SysPrep p = prepare(c, d);
+// Place a prepared system in the context: its device block (and pair
+// lists), cleared, with its tables queued in `up`.  Returns its id, or -1 when
+// the reservation has no room for it (nothing kept).
+int commit_system(pm_ctx *c, const pm_system_desc &d, const SysPrep &p, Uploads &up) {
   auto hp = std::make_unique<SysHost>();
   SysHost &h = *hp;
   init_system(c, d, p, h);
-  h.block = dev_alloc(c, h.block_bytes);
   try {
-    fill_block(c, d, p, h, false);
+    h.block = dev_alloc(c, h.block_bytes);
+    fill_block(c, d, p, h, true, &up);
     if (!c->gb) alloc_lists(c, h);
-  } catch (...) {
+  } catch (const Fail &f) {
+    if (h.block) {
+      // drop this system's queued uploads (they point into its block)
+      char *b0 = static_cast<char *>(h.block), *b1 = b0 + h.block_bytes;
+      up.segs.erase(std::remove_if(up.segs.begin(), up.segs.end(),
+                                   [&](const Uploads::Seg &g) { return g.dst >= b0 && g.dst < b1; }),
+                    up.segs.end());
+    }
     dev_free(c, h.lists, h.lists_bytes);
     dev_free(c, h.block, h.block_bytes);
+    if (f.code == PM_EFULL) return -1;
     throw;
   }
   int id = -1;
@@ -1010,6 +1132,67 @@ int add_system(pm_ctx *c, const pm_system_desc &d) {
   h.id = id;
   c->sys[id] = std::move(hp);
   drop_sets_with(c, id);  // a reused id must not find a stale table
+  return id;
+}
+
+// Add systems in order: ids[k] = the new id, or -1 when the reservation has no
+// room for system k; after max_full such systems the rest are not tried (-1).
+// The host tables are prepared on worker threads, a chunk at a time, and each
+// chunk's uploads travel in one staged copy per system.  Any other failure
+// removes the systems this call added and is raised.
+void add_many(pm_ctx *c, int n, const pm_system_desc *ds, int max_full, int32_t *ids) {
+  for (int k = 0; k < n; ++k) ids[k] = -1;
+  std::vector<int> added;
+  int full = 0;
+  try {
+    constexpr int kChunk = 32;
+    for (int k0 = 0; k0 < n && full < max_full; k0 += kChunk) {
+      const int m = std::min(kChunk, n - k0);
+      std::vector<SysPrep> preps(m);
+      std::vector<std::exception_ptr> errs(m);
+      parallel_for(m, [&](int i) {
+        try {
+          preps[i] = prepare(c, ds[k0 + i]);
+        } catch (...) {
+          errs[i] = std::current_exception();
+        }
+      });
+      Uploads up;
+      for (int i = 0; i < m && full < max_full; ++i) {
+        if (errs[i]) std::rethrow_exception(errs[i]);
+        const int id = commit_system(c, ds[k0 + i], preps[i], up);
+        if (id < 0) {
+          ++full;
+          continue;
+        }
+        ids[k0 + i] = id;
+        added.push_back(id);
+      }
+      flush_uploads(c, up);  // overlaps the next chunk's preparation
+    }
+    HIPCHECK(hipStreamSynchronize(c->stream));  // fills and uploads done
+  } catch (...) {
+    (void)hipStreamSynchronize(c->stream);
+    for (int id : added) {
+      SysHost &h = *c->sys[id];
+      dev_free(c, h.lists, h.lists_bytes);
+      dev_free(c, h.block, h.block_bytes);
+      c->sys[id].reset();
+      drop_sets_with(c, id);
+    }
+    for (int k = 0; k < n; ++k) ids[k] = -1;
+    throw;
+  }
+}
+
+// add a system; returns its id
+int add_system(pm_ctx *c, const pm_system_desc &d) {
+  int32_t id = -1;
+  add_many(c, 1, &d, 1, &id);
+  if (id < 0) {
+    const size_t need = footprint(c, d);
+    throw Fail{PM_EFULL, "the device reservation has no room for " + std::to_string(need) + " bytes"};
+  }
   return id;
 }
 
@@ -1271,6 +1454,31 @@ std::vector<int> check_errors(pm_ctx *c, LaunchSet &ls) {
 
 // H2D of concatenated AoS host arrays into the systems' staging buffers
 void put_staging(pm_ctx *c, const std::vector<int> &ids, const double *x, const double *v) {
+  size_t atoms = 0;
+  for (int id : ids) atoms += c->sys[id]->n;
+  const int na = (x ? 1 : 0) + (v ? 1 : 0);
+  const size_t arr = 3 * atoms * sizeof(double);
+  if (na * arr >= kPinnedTransfer) {
+    // large: through the pinned buffer (host copies on worker threads, DMA per array)
+    char *pin = pinned(c, na * arr);
+    const double *src[2] = {x ? x : v, x ? v : nullptr};
+    parallel_for(16 * na, [&](int k) {
+      const int a = k / 16, part = k % 16;
+      const size_t b0 = arr * part / 16, b1 = arr * (part + 1) / 16;
+      std::memcpy(pin + a * arr + b0, reinterpret_cast<const char *>(src[a]) + b0, b1 - b0);
+    });
+    size_t off = 0;
+    for (int id : ids) {
+      const SysHost &h = *c->sys[id];
+      const size_t b = 3 * (size_t)h.n * sizeof(double);
+      const char *px = pin + 3 * off * sizeof(double);
+      if (x) HIPCHECK(hipMemcpyAsync(h.dev.xaos, px, b, hipMemcpyHostToDevice, c->stream));
+      if (v) HIPCHECK(hipMemcpyAsync(h.dev.vaos, px + (x ? arr : 0), b, hipMemcpyHostToDevice, c->stream));
+      off += h.n;
+    }
+    HIPCHECK(hipEventRecord(c->pin_done, c->stream));
+    return;
+  }
   size_t off = 0;
   for (int id : ids) {
     const SysHost &h = *c->sys[id];
@@ -1281,7 +1489,43 @@ void put_staging(pm_ctx *c, const std::vector<int> &ids, const double *x, const 
   }
 }
 
+// D2H of the systems' staging buffers into concatenated AoS host arrays; the
+// copies have finished when it returns
 void get_staging(pm_ctx *c, const std::vector<int> &ids, double *x, double *v, double *f) {
+  size_t atoms = 0;
+  for (int id : ids) atoms += c->sys[id]->n;
+  double *dst[3] = {x, v, f};
+  int na = 0;
+  for (double *d : dst) na += d ? 1 : 0;
+  const size_t arr = 3 * atoms * sizeof(double);
+  if (na * arr >= kPinnedTransfer) {
+    char *pin = pinned(c, na * arr);
+    size_t off = 0;
+    for (int id : ids) {
+      const SysHost &h = *c->sys[id];
+      const size_t b = 3 * (size_t)h.n * sizeof(double);
+      int a = 0;
+      const double *srcs[3] = {h.dev.xaos, h.dev.vaos, h.dev.faos};
+      for (int k = 0; k < 3; ++k)
+        if (dst[k]) {
+          HIPCHECK(hipMemcpyAsync(pin + a * arr + 3 * off * sizeof(double), srcs[k], b, hipMemcpyDeviceToHost,
+                                  c->stream));
+          ++a;
+        }
+      off += h.n;
+    }
+    HIPCHECK(hipEventRecord(c->pin_done, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    std::vector<double *> outs;
+    for (double *d : dst)
+      if (d) outs.push_back(d);
+    parallel_for(16 * na, [&](int k) {
+      const int a = k / 16, part = k % 16;
+      const size_t b0 = arr * part / 16, b1 = arr * (part + 1) / 16;
+      std::memcpy(reinterpret_cast<char *>(outs[a]) + b0, pin + a * arr + b0, b1 - b0);
+    });
+    return;
+  }
   size_t off = 0;
   for (int id : ids) {
     const SysHost &h = *c->sys[id];
@@ -1291,6 +1535,7 @@ void get_staging(pm_ctx *c, const std::vector<int> &ids, double *x, double *v, d
     if (f) HIPCHECK(hipMemcpyAsync(f + 3 * off, h.dev.faos, b, hipMemcpyDeviceToHost, c->stream));
     off += h.n;
   }
+  HIPCHECK(hipStreamSynchronize(c->stream));
 }
 
 size_t atoms_of(const pm_ctx *c, const std::vector<int> &ids) {
@@ -1901,7 +2146,10 @@ int pm_ctx_create(const pm_force_desc *force, int32_t n_systems, const pm_system
     HIPCHECK(hipEventCreate(&c->evr0));
     HIPCHECK(hipEventCreate(&c->evr1));
     setup_force(c, force, n_systems, systems);
-    for (int s = 0; s < n_systems; ++s) add_system(c, systems[s]);
+    std::vector<int32_t> ids(std::max(n_systems, 1));
+    add_many(c, n_systems, systems, n_systems, ids.data());
+    for (int s = 0; s < n_systems; ++s)
+      if (ids[s] != s) throw Fail{PM_EINVAL, "system " + std::to_string(s) + " was not placed"};
     HIPCHECK(hipDeviceSynchronize());
   });
   if (rc != PM_OK) {
@@ -1928,6 +2176,8 @@ int pm_ctx_destroy(pm_ctx *c) {
     if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : c->evs) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->pin) (void)hipHostFree(c->pin);
+  if (c->pin_done) (void)hipEventDestroy(c->pin_done);
   delete c;
   return PM_OK;
 }
@@ -1961,6 +2211,14 @@ int pm_system_add(pm_ctx *c, const pm_system_desc *system, int32_t *id) {
   return run_guarded(c, [&] {
     if (!system || !id) throw Fail{PM_EINVAL, "system and id required"};
     *id = add_system(c, *system);
+  });
+}
+
+int pm_system_add_many(pm_ctx *c, int32_t n, const pm_system_desc *systems, int32_t max_full, int32_t *ids) {
+  if (!c) return PM_EINVAL;
+  return run_guarded(c, [&] {
+    if (n < 0 || (n > 0 && (!systems || !ids))) throw Fail{PM_EINVAL, "systems and ids required"};
+    add_many(c, n, systems, std::max(max_full, 1), ids);
   });
 }
 

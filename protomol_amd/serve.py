@@ -88,23 +88,36 @@ class Server:
 
     # -- one scheduling round
     def _admit(self):
+        """Admit waiting requests in arrival order in one pm_system_add_many
+        call: a request that does not fit waits, later ones are still tried
+        until ``lookahead`` requests have not fitted."""
         new_eval, new_md = [], []
-        skipped = 0
+        if not self.queue:
+            return new_eval, new_md
+        cand = list(self.queue)
+        if hasattr(self.ctx, "add_many"):
+            ids = self.ctx.add_many([r.system for r in cand], max_full=self.lookahead)
+        else:
+            ids, full = [], 0
+            for r in cand:
+                if full >= self.lookahead:
+                    ids.append(None)
+                    continue
+                try:
+                    ids.append(self.ctx.add(r.system))
+                except L.PMError as err:
+                    if err.code != L.PM_EFULL:
+                        raise
+                    ids.append(None)
+                    full += 1
         pending = deque()
-        while self.queue:
-            req = self.queue.popleft()
-            if skipped >= self.lookahead:
+        now = self.now()
+        for req, sid in zip(cand, ids):
+            if sid is None:
                 pending.append(req)
                 continue
-            try:
-                req.sid = self.ctx.add(req.system)
-            except L.PMError as err:
-                if err.code != L.PM_EFULL:
-                    raise
-                pending.append(req)
-                skipped += 1
-                continue
-            req.admitted = self.now()
+            req.sid = sid
+            req.admitted = now
             (new_eval if req.kind == "eval" else new_md).append(req)
         self.queue = pending
         return new_eval, new_md

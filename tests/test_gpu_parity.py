@@ -626,6 +626,44 @@ def test_pool_admission_by_footprint():
         assert np.array_equal(f1[0], f2) and np.array_equal(e1[0], e2)
 
 
+def test_pool_add_many_equals_single_adds():
+    """pm_system_add_many places systems as pm_system_add one at a time does:
+    the same ids, PM_EFULL as None (none tried after max_full of those), the
+    same forces and energies bit for bit; an invalid system removes the ones
+    the call had added."""
+    print("case pool add_many", flush=True)
+    a, b, c, d = _water_set((700, 1400, 500, 450), 140)
+    ta, sa, tb, sb = MODELS["c2c2"]
+    desc = force_desc(ta, sa, tb, sb, flags=L.FLAG_BONDED | L.FLAG_VIRIAL, skin=0.8, interval=0, ewald=True)
+    with L.Context(desc) as probe:
+        fa, fb, fc, fd = (probe.footprint(s) for s in (a, b, c, d))
+    assert fc + fd < fb
+    res = fa + fc + fd + (fb - fc - fd) // 2   # after a: room for c and d, not for b
+    with L.Context(desc, [], reserve_bytes=res) as one:
+        ref = {}
+        for s in (a, c, d):
+            sid = one.add(s)
+            ref[s.name] = one.evaluate_systems([sid], [s.x])
+    with L.Context(desc, [], reserve_bytes=res) as ctx:
+        assert ctx.add_many([a, b, c, d], max_full=1) == [0, None, None, None]
+        assert ctx.add_many([b, c, d], max_full=2) == [None, 1, 2]
+        for sid, s in ((0, a), (1, c), (2, d)):
+            f, e = ctx.evaluate_systems([sid], [s.x])
+            assert np.array_equal(f[0], ref[s.name][0][0]) and np.array_equal(e[0], ref[s.name][1][0]), s.name
+        r0, u0 = ctx.memory()
+        bad = S.water_box(300, 22.0, seed=5)
+        bad.mass = bad.mass.copy()
+        bad.mass[7] = 0.0                   # invalid: the call fails and keeps nothing
+        ctx.remove(2)
+        with pytest.raises(L.PMError) as err:
+            ctx.add_many([d, bad])
+        assert err.value.code == L.PM_EINVAL
+        assert ctx.memory()[1] < u0
+        assert ctx.add_many([d]) == [2]
+        f, e = ctx.evaluate_systems([2], [d.x])
+        assert np.array_equal(f[0], ref[d.name][0][0]) and np.array_equal(e[0], ref[d.name][1][0])
+
+
 # ------------------------------------------------- Ewald reciprocal space
 
 RECIP = L.FLAG_EWALD_RECIPROCAL

@@ -62,6 +62,29 @@ class FakeCtx:
         return xs, xs, xs, np.zeros((len(ids), L.NSLOTS))
 
 
+class FakeCtxMany(FakeCtx):
+    """The same with pm_system_add_many's admission: in order, None for a
+    system that does not fit, none tried after max_full of those."""
+
+    def __init__(self, capacity):
+        super().__init__(capacity)
+        self.calls = 0
+
+    def add_many(self, systems, max_full=1 << 30):
+        self.calls += 1
+        out, full = [], 0
+        for s in systems:
+            if full >= max_full:
+                out.append(None)
+                continue
+            try:
+                out.append(self.add(s))
+            except L.PMError:
+                out.append(None)
+                full += 1
+        return out
+
+
 def _requests(n, seed):
     rng = np.random.default_rng(seed)
     out = []
@@ -76,8 +99,9 @@ def _requests(n, seed):
     return out
 
 
-def test_every_request_completes_within_capacity():
-    ctx = FakeCtx(capacity=200 * 100 * 4)
+@pytest.mark.parametrize("make", [FakeCtx, FakeCtxMany])
+def test_every_request_completes_within_capacity(make):
+    ctx = make(capacity=200 * 100 * 4)
     srv = serve.Server(ctx, chunk=10)
     reqs = _requests(300, 1)
     done = srv.run(reqs, max_seconds=60)
@@ -106,3 +130,18 @@ def test_md_steps_must_be_whole_chunks():
     srv = serve.Server(FakeCtx(10 ** 6), chunk=10)
     with pytest.raises(ValueError):
         srv.submit(serve.Request(rid=0, kind="md", system=FakeSystem(3), x=np.zeros((3, 3)), steps=15))
+
+
+def test_admission_lookahead_with_add_many():
+    """A request that does not fit waits; later ones are tried until
+    ``lookahead`` have not fitted, and admission keeps arrival order."""
+    ctx = FakeCtxMany(capacity=100 * 100)
+    srv = serve.Server(ctx, chunk=10, lookahead=2)
+    sizes = [60, 50, 45, 30, 20, 10]   # 60 fits; 50, 45 do not; 30 is not tried
+    for k, n in enumerate(sizes):
+        srv.submit(serve.Request(rid=k, kind="md", system=FakeSystem(n), x=np.zeros((n, 3)),
+                                 v=np.zeros((n, 3)), steps=10))
+    new_eval, new_md = srv._admit()
+    assert [r.rid for r in new_md] == [0] and not new_eval
+    assert [r.rid for r in srv.queue] == [1, 2, 3, 4, 5]
+    assert ctx.calls == 1
