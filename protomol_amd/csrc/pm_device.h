@@ -49,6 +49,17 @@ __device__ __forceinline__ int find_sys(const int *__restrict__ prefix, int nsys
   return lo;
 }
 
+// The system of a per-atom launch's wave (prefix ap over nsys systems, total
+// atoms): wave-uniform, in scalar registers, when every live lane of the wave
+// belongs to that system; -1 when the wave straddles two systems.  Every lane
+// of the wave must be active.
+__device__ __forceinline__ int wave_sys(const int *__restrict__ ap, int nsys, int g, int total) {
+  const int g0 = __builtin_amdgcn_readfirstlane(g);  // lane 0 holds the wave's lowest index
+  const int s = __builtin_amdgcn_readfirstlane(find_sys(ap, nsys, min(g0, max(total - 1, 0))));
+  const int end = ap[s + 1];
+  return __ballot(g < total && g >= end) == 0 ? s : -1;
+}
+
 __device__ __forceinline__ long long to_fixed(double v, double scale) {
   return __double2ll_rn(v * scale);
 }

@@ -45,7 +45,7 @@ __device__ __forceinline__ bool rebuild_due(const SysDev &S, int mode, float lim
 
 // one system's step bookkeeping (one thread): counters, the census, and the
 // rebuild decision from the largest displacement since the last rebuild
-__device__ __forceinline__ void step_begin_ctl(SysDev &S, int mode, float lim2, int kmax) {
+__device__ __forceinline__ void step_begin_ctl(const SysDev &S, int mode, float lim2, int kmax) {
   S.census[0] = 0;
   S.census[1] = 0;
   int *ctl = S.ctl;
@@ -67,7 +67,7 @@ __device__ __forceinline__ void step_begin_ctl(SysDev &S, int mode, float lim2, 
 
 // the rest of a step's bookkeeping once k_bin has published the decision in
 // ctl[kCtlRebuild] (one thread per system)
-__device__ __forceinline__ void step_finish_ctl(SysDev &S) {
+__device__ __forceinline__ void step_finish_ctl(const SysDev &S) {
   int *ctl = S.ctl;
   ctl[kCtlFirst] = 0;  // the step's kick has run
   ctl[kCtlNoise] += 1;
@@ -91,43 +91,53 @@ __device__ __forceinline__ void step_finish_ctl(SysDev &S) {
 // served by one atomic: the wave peels off the group of the first remaining
 // lane until every lane is done.  Atoms in original order are spatially
 // coherent, so a wave usually holds only a few keys.
-__global__ void k_bin(SysDev *sys, const int *ap, int nsys, int mode, float lim2, int kmax) {
+// one atom of k_bin: the force pass's start and, when the list is rebuilt,
+// the atom's column and sort key; returns whether it has a key to place
+__device__ __forceinline__ bool bin_atom(const SysDev &S, int a, int mode, float lim2, int kmax, int &col,
+                                         unsigned long long &key) {
+  const int n = S.n_atoms;
+  const bool rb = rebuild_due(S, mode, lim2, kmax);
+  for (int k = a; k < kAccN * kAccCopies; k += n) S.acc[k] = 0;
+  if (a == 0) {
+    S.ctl[kCtlRebuild] = rb ? 1 : 0;
+    S.census[0] = 0;
+    S.census[1] = 0;
+  }
+  if (!rb) return false;
+  double x = S.x[a], y = S.x[n + a], z = S.x[2 * n + a];
+  if (!(isfinite(x) && isfinite(y) && isfinite(z))) {
+    atomicOr(S.err, (unsigned)kErrNonFinite);
+    x = y = z = 0.0;
+  }
+  if (S.vac && (x < S.vac_lo[0] || x >= S.vac_hi[0] || y < S.vac_lo[1] || y >= S.vac_hi[1] ||
+                z < S.vac_lo[2] || z >= S.vac_hi[2]))
+    atomicOr(S.err, (unsigned)kErrVacuumRegion);
+  const double rx = wrap_rel(x, S.omin[0], S.L[0]);
+  const double ry = wrap_rel(y, S.omin[1], S.L[1]);
+  const double rz = wrap_rel(z, S.omin[2], S.L[2]);
+  int cx = (int)(rx / S.ax), cy = (int)(ry / S.ay);
+  cx = cx < 0 ? 0 : (cx >= S.ncx ? S.ncx - 1 : cx);
+  cy = cy < 0 ? 0 : (cy >= S.ncy ? S.ncy - 1 : cy);
+  col = cy * S.ncx + cx;
+  const unsigned zb = __float_as_uint((float)rz);  // z >= 0: bit order == value order
+  key = ((unsigned long long)zb << 32) | (unsigned)a;
+  return true;
+}
+
+__global__ void k_bin(const SysDev *__restrict__ sys, const int *__restrict__ ap, int nsys, int mode, float lim2,
+                      int kmax) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
-  bool todo = g < ap[nsys];
-  int s = 0, col = 0;
+  const int total = ap[nsys];
+  bool todo = false;
+  int s = wave_sys(ap, nsys, g, total), col = 0;
   unsigned long long key = 0;
-  if (todo) {
-    s = find_sys(ap, nsys, g);
-    SysDev &S = sys[s];
-    const int a = g - ap[s], n = S.n_atoms;
-    const bool rb = rebuild_due(S, mode, lim2, kmax);
-    for (int k = a; k < kAccN * kAccCopies; k += n) S.acc[k] = 0;
-    if (a == 0) {
-      S.ctl[kCtlRebuild] = rb ? 1 : 0;
-      S.census[0] = 0;
-      S.census[1] = 0;
-    }
-    todo = rb;
-    if (rb) {
-      double x = S.x[a], y = S.x[n + a], z = S.x[2 * n + a];
-      if (!(isfinite(x) && isfinite(y) && isfinite(z))) {
-        atomicOr(S.err, (unsigned)kErrNonFinite);
-        x = y = z = 0.0;
-      }
-      if (S.vac && (x < S.vac_lo[0] || x >= S.vac_hi[0] || y < S.vac_lo[1] || y >= S.vac_hi[1] ||
-                    z < S.vac_lo[2] || z >= S.vac_hi[2]))
-        atomicOr(S.err, (unsigned)kErrVacuumRegion);
-      const double rx = wrap_rel(x, S.omin[0], S.L[0]);
-      const double ry = wrap_rel(y, S.omin[1], S.L[1]);
-      const double rz = wrap_rel(z, S.omin[2], S.L[2]);
-      int cx = (int)(rx / S.ax), cy = (int)(ry / S.ay);
-      cx = cx < 0 ? 0 : (cx >= S.ncx ? S.ncx - 1 : cx);
-      cy = cy < 0 ? 0 : (cy >= S.ncy ? S.ncy - 1 : cy);
-      col = cy * S.ncx + cx;
-      const unsigned zb = __float_as_uint((float)rz);  // z >= 0: bit order == value order
-      key = ((unsigned long long)zb << 32) | (unsigned)a;
-    }
+  if (s >= 0) {
+    const SysDev *__restrict__ S = sys + s;  // scalar loads of the record
+    if (g < total) todo = bin_atom(*S, g - ap[s], mode, lim2, kmax, col, key);
+  } else {
+    s = g < total ? find_sys(ap, nsys, g) : 0;
+    if (g < total) todo = bin_atom(sys[s], g - ap[s], mode, lim2, kmax, col, key);
   }
   // one atomic per (system, column) group of the wave
   while (true) {
@@ -155,13 +165,13 @@ __global__ void k_bin(SysDev *sys, const int *ap, int nsys, int mode, float lim2
 // cluster's geometry (8 lanes per cluster): bounding box, origin, the atoms'
 // FP32 local coordinates.  The block of each system's column 0 finishes the
 // step's bookkeeping.
-__global__ __launch_bounds__(1024) void k_col_sort(SysDev *sys, const int *cp, int nsys) {
+__global__ __launch_bounds__(1024) void k_col_sort(const SysDev *__restrict__ sys, const int *__restrict__ cp, int nsys) {
   __shared__ unsigned long long keys[kSortCap];
   __shared__ int red[3][1024 / kWave];
   const int gcol = blockIdx.x;
   if (gcol >= cp[nsys]) return;
-  const int s = find_sys(cp, nsys, gcol);
-  SysDev &S = sys[s];
+  const int s = __builtin_amdgcn_readfirstlane(find_sys(cp, nsys, gcol));
+  const SysDev &S = sys[s];
   const int col = gcol - cp[s];
   const int t = threadIdx.x, nt = blockDim.x, lane = t & 63, wv = t >> 6;
   if (col == 0 && t == 0) step_finish_ctl(S);
@@ -335,15 +345,16 @@ __device__ __forceinline__ int lower_bound_lo_z(const float4 *lo_b, int c0, int 
   return lo;
 }
 
-__global__ __launch_bounds__(kListThreads) void k_pairlist(SysDev *sys, const int *scp, int nsys, float rlist2) {
+__global__ __launch_bounds__(kListThreads) void k_pairlist(const SysDev *__restrict__ sys, const int *__restrict__ scp,
+                                                           int nsys, float rlist2) {
   __shared__ float4 ib_lo[kListWaves][kScCl], ib_hi[kListWaves][kScCl];
   __shared__ int seg_start[kListWaves][kSegCap + 1], seg_cl[kListWaves][kSegCap], seg_sh[kListWaves][kSegCap];
   __shared__ ListScratch scratch[kListWaves];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int scg = blockIdx.x * kListWaves + wv;
+  const int scg = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * kListWaves + wv));
   if (scg >= scp[nsys]) return;
-  const int s = find_sys(scp, nsys, scg);
-  SysDev &S = sys[s];
+  const int s = __builtin_amdgcn_readfirstlane(find_sys(scp, nsys, scg));
+  const SysDev &S = sys[s];
   if (!S.ctl[kCtlRebuild]) return;
   const int sc = scg - scp[s];
   if (sc >= *S.n_sc) return;
@@ -577,107 +588,227 @@ __global__ __launch_bounds__(kListThreads) void k_pairlist(SysDev *sys, const in
 //   BondSystemForce::calcBond (force/bonded/BondSystemForce.h:77-108)
 //   AngleSystemForce::calcAngle (force/bonded/AngleSystemForce.h:66-140)
 //   NonbondedFullEwaldSystemForce::correctionTerm (NonbondedFullEwaldSystemForce.h:1097-1131)
-__global__ __launch_bounds__(256) void k_local(SysDev *sys, const int *ap, int nsys, ForceDev P, int mode) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool ok = g < ap[nsys];
-  int s = 0;
+// terms of each kind one batch of an atom's local terms takes
+constexpr int kLocBatchB = 2, kLocBatchA = 1, kLocBatchC = 2;
+
+// One atom's local terms.  The terms are taken in batches; within a batch the
+// loads are issued level by level for every term at once (items, then the
+// terms' atoms and constants, then the atoms' positions), so an atom waits
+// for three round trips per batch instead of three per term.  The arithmetic
+// then runs in item order, bonds, angles, corrections.
+template <bool EN>
+__device__ __forceinline__ void local_atom(const SysDev &S, int a, const ForceDev &P, int mode, double &eb, double &ea,
+                                           double &ec, double v[6]) {
+  const int n = S.n_atoms;
+  const PM_GLOBAL double *X = gp(S.x);
+  const PM_GLOBAL int *item = gp(S.loc_item);
+  const PM_GLOBAL int *ls = gp(S.loc_start) + 3 * (size_t)a;
+  const int b_end = ls[kLocAngle], a_end = ls[kLocCorr], c_end = ls[kLocCorr + 1];
+  int ib = ls[kLocBond], ia = b_end, ic = a_end;
   double f[3] = {0.0, 0.0, 0.0};
-  double eb = 0.0, ea = 0.0, ec = 0.0, v[6] = {0, 0, 0, 0, 0, 0};
-  if (ok) {
-    s = find_sys(ap, nsys, g);
-    const SysDev &S = sys[s];
-    const int a = g - ap[s], n = S.n_atoms;
-    const double *X = S.x;
-    const int *ls = S.loc_start + 3 * (size_t)a;
-    // one kind at a time, so lanes of a wave run the same term code together
-    for (int it = ls[kLocBond]; it < ls[kLocBond + 1]; ++it) {
-      const int item = S.loc_item[it];
-      const int role = item & 3, k = item >> 2;
-      const int a1 = S.bonds[2 * k], a2 = S.bonds[2 * k + 1];
-      double r12[3] = {X[a1] - X[a2], X[n + a1] - X[n + a2], X[2 * n + a1] - X[2 * n + a2]};
-      min_image_vec(S, r12);
-      const double r = sqrt(r12[0] * r12[0] + r12[1] * r12[1] + r12[2] * r12[2]);
-      const double dp = 2.0 * S.bond_k[k] * (r - S.bond_r0[k]);
-      const double c = -dp / r;
-      const double sg = role == 0 ? 1.0 : -1.0;
-      for (int d = 0; d < 3; ++d) f[d] += sg * (r12[d] * c);
-      if (role == 0) {
-        eb += S.bond_k[k] * (r - S.bond_r0[k]) * (r - S.bond_r0[k]);
-        v[0] += r12[0] * c * r12[0];
-        v[1] += r12[0] * c * r12[1];
-        v[2] += r12[0] * c * r12[2];
-        v[3] += r12[1] * c * r12[1];
-        v[4] += r12[1] * c * r12[2];
-        v[5] += r12[2] * c * r12[2];
+  auto pos = [&](int k, double p[3]) {
+    p[0] = X[k];
+    p[1] = X[n + k];
+    p[2] = X[2 * n + k];
+  };
+  // one batch; the first is straight-line code (a water atom's terms fit in
+  // it), the loop takes the rest: a loop around every batch would hoist the
+  // FP64 library constants of erf / atan2 into registers for its whole length
+  auto batch = [&]() {
+    // items
+    int tb[kLocBatchB], ta[kLocBatchA], tc[kLocBatchC];
+#pragma unroll
+    for (int j = 0; j < kLocBatchB; ++j) tb[j] = ib + j < b_end ? item[ib + j] : -1;
+#pragma unroll
+    for (int j = 0; j < kLocBatchA; ++j) ta[j] = ia + j < a_end ? item[ia + j] : -1;
+#pragma unroll
+    for (int j = 0; j < kLocBatchC; ++j) tc[j] = ic + j < c_end ? item[ic + j] : -1;
+    // the terms' atoms and constants
+    int b1[kLocBatchB], b2[kLocBatchB], a1[kLocBatchA], a2[kLocBatchA], a3[kLocBatchA], c1[kLocBatchC],
+        c2[kLocBatchC], cc[kLocBatchC];
+    double bk[kLocBatchB], br[kLocBatchB], ak[kLocBatchA], at[kLocBatchA];
+#pragma unroll
+    for (int j = 0; j < kLocBatchB; ++j)
+      if (tb[j] >= 0) {
+        const int k = tb[j] >> 2;
+        b1[j] = gp(S.bonds)[2 * k];
+        b2[j] = gp(S.bonds)[2 * k + 1];
+        bk[j] = gp(S.bond_k)[k];
+        br[j] = gp(S.bond_r0)[k];
+      }
+#pragma unroll
+    for (int j = 0; j < kLocBatchA; ++j)
+      if (ta[j] >= 0) {
+        const int k = ta[j] >> 2;
+        a1[j] = gp(S.angles)[3 * k];
+        a2[j] = gp(S.angles)[3 * k + 1];
+        a3[j] = gp(S.angles)[3 * k + 2];
+        ak[j] = gp(S.angle_k)[k];
+        at[j] = gp(S.angle_t0)[k];
+      }
+#pragma unroll
+    for (int j = 0; j < kLocBatchC; ++j)
+      if (tc[j] >= 0) {
+        const int k = tc[j] >> 2;
+        c1[j] = gp(S.corr)[3 * k];
+        c2[j] = gp(S.corr)[3 * k + 1];
+        cc[j] = gp(S.corr)[3 * k + 2];
+      }
+    // positions (and the corrections' charges), kept as the terms' vectors
+    double rb[kLocBatchB][3], ra12[kLocBatchA][3], ra32[kLocBatchA][3], rc[kLocBatchC][3], qq0[kLocBatchC];
+    {
+      double pb1[kLocBatchB][3], pb2[kLocBatchB][3], pa1[kLocBatchA][3], pa2[kLocBatchA][3], pa3[kLocBatchA][3],
+          pc1[kLocBatchC][3], pc2[kLocBatchC][3], q1[kLocBatchC], q2[kLocBatchC];
+#pragma unroll
+      for (int j = 0; j < kLocBatchB; ++j)
+        if (tb[j] >= 0) {
+          pos(b1[j], pb1[j]);
+          pos(b2[j], pb2[j]);
+        }
+#pragma unroll
+      for (int j = 0; j < kLocBatchA; ++j)
+        if (ta[j] >= 0) {
+          pos(a1[j], pa1[j]);
+          pos(a2[j], pa2[j]);
+          pos(a3[j], pa3[j]);
+        }
+#pragma unroll
+      for (int j = 0; j < kLocBatchC; ++j)
+        if (tc[j] >= 0) {
+          pos(c1[j], pc1[j]);
+          pos(c2[j], pc2[j]);
+          q1[j] = gp(S.charge)[c1[j]];
+          q2[j] = gp(S.charge)[c2[j]];
+        }
+#pragma unroll
+      for (int j = 0; j < kLocBatchB; ++j)
+        for (int d = 0; d < 3; ++d) rb[j][d] = tb[j] >= 0 ? pb1[j][d] - pb2[j][d] : 0.0;
+#pragma unroll
+      for (int j = 0; j < kLocBatchA; ++j)
+        for (int d = 0; d < 3; ++d) {
+          ra12[j][d] = ta[j] >= 0 ? pa1[j][d] - pa2[j][d] : 0.0;
+          ra32[j][d] = ta[j] >= 0 ? pa3[j][d] - pa2[j][d] : 0.0;
+        }
+#pragma unroll
+      for (int j = 0; j < kLocBatchC; ++j) {
+        for (int d = 0; d < 3; ++d) rc[j][d] = tc[j] >= 0 ? pc2[j][d] - pc1[j][d] : 0.0;
+        qq0[j] = tc[j] >= 0 ? q1[j] * q2[j] : 0.0;
       }
     }
-    for (int it = ls[kLocAngle]; it < ls[kLocAngle + 1]; ++it) {
-      const int item = S.loc_item[it];
-      const int role = item & 3, k = item >> 2;
-      const int a1 = S.angles[3 * k], a2 = S.angles[3 * k + 1], a3 = S.angles[3 * k + 2];
-      double r12[3] = {X[a1] - X[a2], X[n + a1] - X[n + a2], X[2 * n + a1] - X[2 * n + a2]};
-      double r32[3] = {X[a3] - X[a2], X[n + a3] - X[n + a2], X[2 * n + a3] - X[2 * n + a2]};
-      min_image_vec(S, r12);
-      min_image_vec(S, r32);
-      const double d12 = sqrt(r12[0] * r12[0] + r12[1] * r12[1] + r12[2] * r12[2]);
-      const double d32 = sqrt(r32[0] * r32[0] + r32[1] * r32[1] + r32[2] * r32[2]);
-      const double cr0 = r12[1] * r32[2] - r12[2] * r32[1], cr1 = r12[2] * r32[0] - r12[0] * r32[2],
-                   cr2 = r12[0] * r32[1] - r12[1] * r32[0];
-      const double cn = sqrt(cr0 * cr0 + cr1 * cr1 + cr2 * cr2);
-      const double dt = r12[0] * r32[0] + r12[1] * r32[1] + r12[2] * r32[2];
-      const double theta = atan2(cn, dt);
-      // sin and cos of theta = atan2(|r12 x r32|, r12 . r32), from the same vectors
-      const double inv = 1.0 / (d12 * d32);
-      const double st = cn * inv, ct = dt * inv;
-      const double dp = 2.0 * S.angle_k[k] * (theta - S.angle_t0[k]);
-      double f1[3], f3[3];
-      for (int d = 0; d < 3; ++d) {
-        const double u12 = r12[d] / d12, u32 = r32[d] / d32;
-        f1[d] = -((u12 * ct - u32) / (st * d12)) * dp;
-        f3[d] = -((u32 * ct - u12) / (st * d32)) * dp;
+    // BondSystemForce::calcBond (force/bonded/BondSystemForce.h:77-108)
+#pragma unroll
+    for (int j = 0; j < kLocBatchB; ++j)
+      if (tb[j] >= 0) {
+        const int role = tb[j] & 3;
+        double r12[3] = {rb[j][0], rb[j][1], rb[j][2]};
+        min_image_vec(S, r12);
+        const double r = sqrt(r12[0] * r12[0] + r12[1] * r12[1] + r12[2] * r12[2]);
+        const double dp = 2.0 * bk[j] * (r - br[j]);
+        const double c = -dp / r;
+        const double sg = role == 0 ? 1.0 : -1.0;
+        for (int d = 0; d < 3; ++d) f[d] += sg * (r12[d] * c);
+        if (EN && role == 0) {
+          eb += bk[j] * (r - br[j]) * (r - br[j]);
+          v[0] += r12[0] * c * r12[0];
+          v[1] += r12[0] * c * r12[1];
+          v[2] += r12[0] * c * r12[2];
+          v[3] += r12[1] * c * r12[1];
+          v[4] += r12[1] * c * r12[2];
+          v[5] += r12[2] * c * r12[2];
+        }
       }
-      for (int d = 0; d < 3; ++d) f[d] += role == 0 ? f1[d] : (role == 2 ? f3[d] : -f1[d] - f3[d]);
-      if (role == 0) {
-        ea += S.angle_k[k] * (theta - S.angle_t0[k]) * (theta - S.angle_t0[k]);
-        v[0] += f1[0] * r12[0] + f3[0] * r32[0];
-        v[1] += f1[0] * r12[1] + f3[0] * r32[1];
-        v[2] += f1[0] * r12[2] + f3[0] * r32[2];
-        v[3] += f1[1] * r12[1] + f3[1] * r32[1];
-        v[4] += f1[1] * r12[2] + f3[1] * r32[2];
-        v[5] += f1[2] * r12[2] + f3[2] * r32[2];
+    // AngleSystemForce::calcAngle (force/bonded/AngleSystemForce.h:66-140)
+#pragma unroll
+    for (int j = 0; j < kLocBatchA; ++j)
+      if (ta[j] >= 0) {
+        const int role = ta[j] & 3;
+        double r12[3] = {ra12[j][0], ra12[j][1], ra12[j][2]};
+        double r32[3] = {ra32[j][0], ra32[j][1], ra32[j][2]};
+        min_image_vec(S, r12);
+        min_image_vec(S, r32);
+        const double d12 = sqrt(r12[0] * r12[0] + r12[1] * r12[1] + r12[2] * r12[2]);
+        const double d32 = sqrt(r32[0] * r32[0] + r32[1] * r32[1] + r32[2] * r32[2]);
+        const double cr0 = r12[1] * r32[2] - r12[2] * r32[1], cr1 = r12[2] * r32[0] - r12[0] * r32[2],
+                     cr2 = r12[0] * r32[1] - r12[1] * r32[0];
+        const double cn = sqrt(cr0 * cr0 + cr1 * cr1 + cr2 * cr2);
+        const double dt = r12[0] * r32[0] + r12[1] * r32[1] + r12[2] * r32[2];
+        const double theta = atan2(cn, dt);
+        // sin and cos of theta = atan2(|r12 x r32|, r12 . r32), from the same vectors
+        const double inv = 1.0 / (d12 * d32);
+        const double st = cn * inv, ct = dt * inv;
+        const double dp = 2.0 * ak[j] * (theta - at[j]);
+        double f1[3], f3[3];
+        for (int d = 0; d < 3; ++d) {
+          const double u12 = r12[d] / d12, u32 = r32[d] / d32;
+          f1[d] = -((u12 * ct - u32) / (st * d12)) * dp;
+          f3[d] = -((u32 * ct - u12) / (st * d32)) * dp;
+        }
+        for (int d = 0; d < 3; ++d) f[d] += role == 0 ? f1[d] : (role == 2 ? f3[d] : -f1[d] - f3[d]);
+        if (EN && role == 0) {
+          ea += ak[j] * (theta - at[j]) * (theta - at[j]);
+          v[0] += f1[0] * r12[0] + f3[0] * r32[0];
+          v[1] += f1[0] * r12[1] + f3[0] * r32[1];
+          v[2] += f1[0] * r12[2] + f3[0] * r32[2];
+          v[3] += f1[1] * r12[1] + f3[1] * r32[1];
+          v[4] += f1[1] * r12[2] + f3[1] * r32[2];
+          v[5] += f1[2] * r12[2] + f3[2] * r32[2];
+        }
       }
-    }
-    for (int it = ls[kLocCorr]; it < ls[kLocCorr + 1]; ++it) {
-      const int item = S.loc_item[it];
-      const int role = item & 3, k = item >> 2;
-      const int a1 = S.corr[3 * k], a2 = S.corr[3 * k + 1], cls = S.corr[3 * k + 2];
-      double d[3] = {X[a2] - X[a1], X[n + a2] - X[n + a1], X[2 * n + a2] - X[2 * n + a1]};
-      min_image_vec(S, d);
-      const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-      double qq = S.charge[a1] * S.charge[a2];
-      if (cls == PM_EXCL_MODIFIED) qq *= 1.0 - (double)S.csf;
-      const double r = sqrt(r2), rr = 1.0 / r;
-      const double e = erf(P.dalpha * r) * rr;  // C-library erf, as the reference on Linux
-      const double fs = qq * (P.dtwo_alpha_pi * exp(-(P.dalpha * P.dalpha) * r2) - e) * rr * rr;
-      const double sg = role == 0 ? -1.0 : 1.0;
-      for (int q = 0; q < 3; ++q) f[q] += sg * (d[q] * fs);
-      if (role == 0) {
-        ec += -qq * e;
-        v[0] += d[0] * fs * d[0];
-        v[1] += d[0] * fs * d[1];
-        v[2] += d[0] * fs * d[2];
-        v[3] += d[1] * fs * d[1];
-        v[4] += d[1] * fs * d[2];
-        v[5] += d[2] * fs * d[2];
+    // NonbondedFullEwaldSystemForce::correctionTerm (NonbondedFullEwaldSystemForce.h:1097-1131)
+#pragma unroll
+    for (int j = 0; j < kLocBatchC; ++j)
+      if (tc[j] >= 0) {
+        const int role = tc[j] & 3;
+        double d[3] = {rc[j][0], rc[j][1], rc[j][2]};
+        min_image_vec(S, d);
+        const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        double qq = qq0[j];
+        if (cc[j] == PM_EXCL_MODIFIED) qq *= 1.0 - (double)S.csf;
+        const double r = sqrt(r2), rr = 1.0 / r;
+        const double e = erf(P.dalpha * r) * rr;  // C-library erf, as the reference on Linux
+        const double fs = qq * (P.dtwo_alpha_pi * exp(-(P.dalpha * P.dalpha) * r2) - e) * rr * rr;
+        const double sg = role == 0 ? -1.0 : 1.0;
+        for (int q = 0; q < 3; ++q) f[q] += sg * (d[q] * fs);
+        if (EN && role == 0) {
+          ec += -qq * e;
+          v[0] += d[0] * fs * d[0];
+          v[1] += d[0] * fs * d[1];
+          v[2] += d[0] * fs * d[2];
+          v[3] += d[1] * fs * d[1];
+          v[4] += d[1] * fs * d[2];
+          v[5] += d[2] * fs * d[2];
+        }
       }
-    }
-    if (mode & kLocalForces) {  // every atom, zero for one without local terms
-      S.fl[a] = to_fixed(f[0], kFScale);
-      S.fl[n + a] = to_fixed(f[1], kFScale);
-      S.fl[2 * n + a] = to_fixed(f[2], kFScale);
-    }
+    ib += kLocBatchB;
+    ia += kLocBatchA;
+    ic += kLocBatchC;
+  };
+  batch();
+  while (ib < b_end || ia < a_end || ic < c_end) batch();
+  if (mode & kLocalForces) {  // every atom, zero for one without local terms
+    PM_GLOBAL long long *fl = gp(S.fl);
+    fl[a] = to_fixed(f[0], kFScale);
+    fl[n + a] = to_fixed(f[1], kFScale);
+    fl[2 * n + a] = to_fixed(f[2], kFScale);
   }
-  if (!(mode & kLocalEnergies)) return;
+}
+
+template <bool EN>
+__global__ __launch_bounds__(256, 3) void k_local(const SysDev *__restrict__ sys, const int *__restrict__ ap, int nsys,
+                                               ForceDev P, int mode) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int total = ap[nsys];
+  const bool ok = g < total;
+  double eb = 0.0, ea = 0.0, ec = 0.0, v[6] = {0, 0, 0, 0, 0, 0};
+  int s = wave_sys(ap, nsys, g, total);
+  if (s >= 0) {
+    const SysDev *__restrict__ S = sys + s;  // scalar loads of the record
+    if (ok) local_atom<EN>(*S, g - ap[s], P, mode, eb, ea, ec, v);
+  } else {
+    s = ok ? find_sys(ap, nsys, g) : 0;
+    if (ok) local_atom<EN>(sys[s], g - ap[s], P, mode, eb, ea, ec, v);
+  }
+  if (!EN) return;
   // bonded and correction terms add to the atomic virial only when it is on
   // (BondSystemForce.h:121, AngleSystemForce.h:155, NonbondedFullEwaldSystemForce.h:1125)
   if (!P.virial)
@@ -750,56 +881,81 @@ __device__ __forceinline__ void lv_half(double v[3], const double F[3], double m
   for (int d = 0; d < 3; ++d) v[d] = v[d] * lv.vdt + F[d] * lv.fdt / m + g[d] * sf * lv.ndt;
 }
 
-__global__ __launch_bounds__(256) void k_kick_drift(SysDev *sys, const int *ap, int nsys, double h, LangevinDev lv) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  float d2 = 0.f;
-  int s = 0;
-  bool ok = g < ap[nsys];
-  if (ok) {
-    s = find_sys(ap, nsys, g);
-    SysDev &S = sys[s];
-    const int a = g - ap[s], n = S.n_atoms;
-    const int sl = S.slot[a];
-    const int mode = S.ctl[kCtlFirst] ? 0 : 1;
-    const double m = 1.0 / S.inv_mass[a];
-    const double F[3] = {(double)(S.fx[sl] + S.fl[a]) * kFScaleInv, (double)(S.fy[sl] + S.fl[n + a]) * kFScaleInv,
-                         (double)(S.fz[sl] + S.fl[2 * n + a]) * kFScaleInv};
-    S.fx[sl] = 0;
-    S.fy[sl] = 0;
-    S.fz[sl] = 0;
-    float l[3];
-    double vl[3];
-    if (lv.on) {
-      // the closing half kick of the previous step (noise 2c) unless this is
-      // the first step of a run, then the opening half kick of this one (2c + 1)
-      const unsigned c = (unsigned)S.ctl[kCtlNoise];
-      for (int d = 0; d < 3; ++d) vl[d] = S.v[d * n + a];
-      if (mode == 1) lv_half(vl, F, m, lv, S.lv_seed, a, 2u * c);
-      lv_half(vl, F, m, lv, S.lv_seed, a, 2u * c + 1u);
-    }
+// One atom's kick + drift (see k_kick_drift); returns the squared distance
+// of its cluster-local position from where the pair list was built.
+__device__ __forceinline__ float kick_drift_atom(const SysDev &S, int a, double h, const LangevinDev &lv) {
+  const int n = S.n_atoms;
+  // every load first (global address space, so none waits on a store), then
+  // the arithmetic, then the stores
+  PM_GLOBAL long long *fx = gp(S.fx), *fy = gp(S.fy), *fz = gp(S.fz);
+  const PM_GLOBAL long long *fl = gp(S.fl);
+  PM_GLOBAL double *V = gp(S.v), *X = gp(S.x);
+  const PM_GLOBAL double *XR = gp(S.xref);
+  PM_GLOBAL float4 *xq = gp(S.xq);
+  const int sl = gp(S.slot)[a];
+  const int mode = gp(S.ctl)[kCtlFirst] ? 0 : 1;
+  const unsigned c = (unsigned)gp(S.ctl)[kCtlNoise];
+  const double m = 1.0 / gp(S.inv_mass)[a];
+  const long long l0 = fl[a], l1 = fl[n + a], l2 = fl[2 * n + a];
+  double v[3] = {V[a], V[n + a], V[2 * n + a]};
+  const double x[3] = {X[a], X[n + a], X[2 * n + a]};
+  const double xr[3] = {XR[a], XR[n + a], XR[2 * n + a]};
+  const float4 b = gp(S.lbuild)[a];
+  const long long p0 = fx[sl], p1 = fy[sl], p2 = fz[sl];
+  const float qw = xq[sl].w;
+  const double F[3] = {(double)(p0 + l0) * kFScaleInv, (double)(p1 + l1) * kFScaleInv, (double)(p2 + l2) * kFScaleInv};
+  if (lv.on) {
+    // the closing half kick of the previous step (noise 2c) unless this is
+    // the first step of a run, then the opening half kick of this one (2c + 1)
+    if (mode == 1) lv_half(v, F, m, lv, S.lv_seed, a, 2u * c);
+    lv_half(v, F, m, lv, S.lv_seed, a, 2u * c + 1u);
+  } else {
     for (int d = 0; d < 3; ++d) {
-      double vv = S.v[d * n + a];
-      if (lv.on) vv = vl[d];
-      else if (mode == 0) vv += F[d] * h * 0.5 / m;
-      else vv += F[d] * h / m;
-      S.v[d * n + a] = vv;
-      const double xx = S.x[d * n + a] + vv * h;
-      S.x[d * n + a] = xx;
-      l[d] = (float)(xx - S.xref[d * n + a]);
+      if (mode == 0) v[d] += F[d] * h * 0.5 / m;
+      else v[d] += F[d] * h / m;
     }
-    const float4 q = S.xq[sl];
-    S.xq[sl] = make_float4(l[0], l[1], l[2], q.w);
-    const float4 b = S.lbuild[a];
-    const float ex = l[0] - b.x, ey = l[1] - b.y, ez = l[2] - b.z;
-    d2 = ex * ex + ey * ey + ez * ez;
-    if (!isfinite(d2)) d2 = 3.0e38f;
+  }
+  double xx[3];
+  float l[3];
+  for (int d = 0; d < 3; ++d) {
+    xx[d] = x[d] + v[d] * h;
+    l[d] = (float)(xx[d] - xr[d]);
+  }
+  fx[sl] = 0;
+  fy[sl] = 0;
+  fz[sl] = 0;
+  for (int d = 0; d < 3; ++d) {
+    V[d * n + a] = v[d];
+    X[d * n + a] = xx[d];
+  }
+  xq[sl] = make_float4(l[0], l[1], l[2], qw);
+  const float ex = l[0] - b.x, ey = l[1] - b.y, ez = l[2] - b.z;
+  float d2 = ex * ex + ey * ey + ez * ez;
+  if (!isfinite(d2)) d2 = 3.0e38f;
+  return d2;
+}
+
+__global__ __launch_bounds__(256) void k_kick_drift(const SysDev *__restrict__ sys, const int *__restrict__ ap,
+                                                    int nsys, double h, LangevinDev lv) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int total = ap[nsys];
+  const bool ok = g < total;
+  float d2 = 0.f;
+  int s = wave_sys(ap, nsys, g, total);
+  if (s >= 0) {
+    // the wave's atoms are one system's: its record is read with scalar loads
+    const SysDev *__restrict__ S = sys + s;
+    if (ok) d2 = kick_drift_atom(*S, g - ap[s], h, lv);
+  } else {
+    s = ok ? find_sys(ap, nsys, g) : 0;
+    if (ok) d2 = kick_drift_atom(sys[s], g - ap[s], h, lv);
   }
   // max per system: a block whose atoms all belong to one system reduces
   // through LDS to one atomic; a block straddling systems uses one per lane
   __shared__ float red[4];
   __shared__ int bs[2];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int glast = min((int)(blockIdx.x * blockDim.x + blockDim.x), ap[nsys]) - 1;
+  const int glast = min((int)(blockIdx.x * blockDim.x + blockDim.x), total) - 1;
   if (threadIdx.x == 0) bs[0] = s;
   if (g == glast) bs[1] = s;
   const float mx = wave_max_f(d2);
@@ -818,24 +974,39 @@ __global__ __launch_bounds__(256) void k_kick_drift(SysDev *sys, const int *ap, 
 }
 
 // StandardIntegrator::doHalfKick (StandardIntegrator.cpp:80-92): h = 0.5*dt*ITF, v += F*h/m
-__global__ void k_half_kick(SysDev *sys, const int *ap, int nsys, double hh, LangevinDev lv) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= ap[nsys]) return;
-  const int s = find_sys(ap, nsys, g);
-  SysDev &S = sys[s];
-  const int a = g - ap[s], n = S.n_atoms;
-  const int sl = S.slot[a];
-  const double m = 1.0 / S.inv_mass[a];
-  const double F[3] = {(double)(S.fx[sl] + S.fl[a]) * kFScaleInv, (double)(S.fy[sl] + S.fl[n + a]) * kFScaleInv,
-                       (double)(S.fz[sl] + S.fl[2 * n + a]) * kFScaleInv};
+__device__ __forceinline__ void half_kick_atom(const SysDev &S, int a, double hh, const LangevinDev &lv) {
+  const int n = S.n_atoms;
+  const PM_GLOBAL long long *fl = gp(S.fl);
+  PM_GLOBAL double *V = gp(S.v);
+  const int sl = gp(S.slot)[a];
+  const unsigned c = (unsigned)gp(S.ctl)[kCtlNoise];
+  const double m = 1.0 / gp(S.inv_mass)[a];
+  const long long l0 = fl[a], l1 = fl[n + a], l2 = fl[2 * n + a];
+  double v[3] = {V[a], V[n + a], V[2 * n + a]};
+  const long long p0 = gp(S.fx)[sl], p1 = gp(S.fy)[sl], p2 = gp(S.fz)[sl];
+  const double F[3] = {(double)(p0 + l0) * kFScaleInv, (double)(p1 + l1) * kFScaleInv, (double)(p2 + l2) * kFScaleInv};
   if (lv.on) {
     // the closing half kick of the last step: noise counter 2c
-    double v[3] = {S.v[a], S.v[n + a], S.v[2 * n + a]};
-    lv_half(v, F, m, lv, S.lv_seed, a, 2u * (unsigned)S.ctl[kCtlNoise]);
-    for (int d = 0; d < 3; ++d) S.v[d * n + a] = v[d];
-    return;
+    lv_half(v, F, m, lv, S.lv_seed, a, 2u * c);
+  } else {
+    for (int d = 0; d < 3; ++d) v[d] += F[d] * hh / m;
   }
-  for (int d = 0; d < 3; ++d) S.v[d * n + a] += F[d] * hh / m;
+  for (int d = 0; d < 3; ++d) V[d * n + a] = v[d];
+}
+
+__global__ void k_half_kick(const SysDev *__restrict__ sys, const int *__restrict__ ap, int nsys, double hh,
+                            LangevinDev lv) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int total = ap[nsys];
+  const int s = wave_sys(ap, nsys, g, total);
+  if (g >= total) return;
+  if (s >= 0) {
+    const SysDev *__restrict__ S = sys + s;
+    half_kick_atom(*S, g - ap[s], hh, lv);
+  } else {
+    const int t = find_sys(ap, nsys, g);
+    half_kick_atom(sys[t], g - ap[t], hh, lv);
+  }
 }
 
 // zero the fixed-point forces of every slot (a fill kernel, not a memset)
@@ -948,8 +1119,12 @@ bool launch_pair(const Launch &L, const ForceDev &P, int kind) {
 }
 
 void launch_local(const Launch &L, const ForceDev &P, int mode) {
-  if (L.n_atoms_total && L.n_local_total)
-    hipLaunchKernelGGL(k_local, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys, P, mode);
+  if (!L.n_atoms_total || !L.n_local_total) return;
+  const dim3 g(nblk(L.n_atoms_total, 256)), b(256);
+  if (mode & kLocalEnergies)
+    hipLaunchKernelGGL(k_local<true>, g, b, 0, L.stream, L.sys, L.atom_prefix, L.nsys, P, mode);
+  else
+    hipLaunchKernelGGL(k_local<false>, g, b, 0, L.stream, L.sys, L.atom_prefix, L.nsys, P, mode);
 }
 void launch_kick_drift(const Launch &L, double h, const LangevinDev &lv) {
   if (L.n_atoms_total) hipLaunchKernelGGL(k_kick_drift, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys, h, lv);
