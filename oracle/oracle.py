@@ -319,37 +319,62 @@ def kinetic_energy(sys, v):
 def vacuum_lj_coulomb(c, x=None, block=512):
     """All-pairs LennardJonesForce + CoulombForce in vacuum as
     OneAtomPair<VacuumBoundaryConditions, Universal, ...> sums them
-    (OneAtomPair.h:47-107, NonbondedSimpleFullSystemForce.h:125-150): per-atom
-    GROMACS mixing (BuildTopology.cpp:650-659), exclusions only within a
-    molecule, MODIFIED pairs scaled by lj_scaling / coulomb_scaling.  Every
-    pair in row blocks, then the excluded pairs taken back out (FULL) or
-    rescaled (MODIFIED); FP64 numpy.  Returns forces (n, 3) and 36 slots
-    (Coulomb, LJ; no virial)."""
+    (OneAtomPair.h:47-107, NonbondedSimpleFullSystemForce.h:125-150): the
+    type table's A, B (A14, B14 for a MODIFIED pair, LennardJonesForce.h:44-50)
+    or per-atom GROMACS mixing (BuildTopology.cpp:650-659, MODIFIED pairs
+    scaled by lj_scaling), exclusions only within a molecule, MODIFIED Coulomb
+    scaled by coulomb_scaling (CoulombForce.h:29-40).  Every pair in row
+    blocks with the excluded ones skipped, then the MODIFIED pairs in their
+    own form; FP64 numpy.  Returns forces (n, 3) and 36 slots (Coulomb,
+    LJ; no virial)."""
     x = np.asarray(c.x if x is None else x, np.float64)
     n = c.n
     q = np.asarray(c.charge, np.float64)
-    sh, se = 0.5 * np.asarray(c.lj_sigma), np.sqrt(np.asarray(c.lj_epsilon))
+    table = c.lj_table is not None
+    if table:
+        tab = np.asarray(c.lj_table, np.float64)
+        ty = np.asarray(c.type)
+    else:
+        sh, se = 0.5 * np.asarray(c.lj_sigma), np.sqrt(np.asarray(c.lj_epsilon))
     f = np.zeros((n, 3))
     elj = ecou = 0.0
 
-    def terms(i, j, d):
-        rr = 1.0 / np.sum(d * d, axis=-1)
+    def coef(i, j, mod):
+        if table:
+            t = tab[ty[i], ty[j]]
+            return (t[..., 2], t[..., 3]) if mod else (t[..., 0], t[..., 1])
         sg = sh[i] + sh[j]
         ep = se[i] * se[j]
         s6 = sg ** 6
         B = 4.0 * ep * s6
         A = B * s6
+        return (A * c.lj_scaling, B * c.lj_scaling) if mod else (A, B)
+
+    def terms(i, j, d, mod=False):
+        rr = 1.0 / np.sum(d * d, axis=-1)
+        A, B = coef(i, j, mod)
         r6 = rr ** 3
         el = A * r6 * r6 - B * r6
         fl = (12 * A * r6 * r6 - 6 * B * r6) * rr
         ec = q[i] * q[j] * np.sqrt(rr)
         return el, fl, ec, ec * rr
 
+    # the excluded pairs (last add wins, one molecule), skipped inside the
+    # blocks as OneAtomPair skips them; MODIFIED pairs evaluated on their own
+    mol = np.asarray(c.molecule)
+    last = {}
+    for a, b, cls in c.exclusions:
+        if mol[a] == mol[b] and a != b:
+            last[(min(a, b), max(a, b))] = cls
+    pairs = np.array([(a, b, k) for (a, b), k in last.items()], dtype=np.int64).reshape(-1, 3)
     for b0 in range(0, n, block):
-        i = np.arange(b0, min(n, b0 + block))[:, None]
+        b1 = min(n, b0 + block)
+        i = np.arange(b0, b1)[:, None]
         j = np.arange(n)[None, :]
         d = x[None, :, :] - x[i[:, 0], None, :]
         mask = j > i
+        sel = (pairs[:, 0] >= b0) & (pairs[:, 0] < b1)
+        mask[pairs[sel, 0] - b0, pairs[sel, 1]] = False
         el, fl, ec, fc = terms(i, j, np.where(mask[..., None], d, 1.0))
         el, ec = np.where(mask, el, 0.0), np.where(mask, ec, 0.0)
         fs = np.where(mask, fl + fc, 0.0)
@@ -358,24 +383,18 @@ def vacuum_lj_coulomb(c, x=None, block=512):
         f += fij.sum(axis=0)
         elj += el.sum()
         ecou += ec.sum()
-    mol = np.asarray(c.molecule)
-    last = {}
-    for a, b, cls in c.exclusions:
-        if mol[a] == mol[b] and a != b:
-            last[(min(a, b), max(a, b))] = cls
-    if last:
-        pairs = np.array([(a, b, k) for (a, b), k in last.items()])
-        i, j, cls = pairs[:, 0], pairs[:, 1], pairs[:, 2]
+    mod = pairs[pairs[:, 2] == 2]
+    if len(mod):
+        # MODIFIED: LJ with its 1-4 coefficients, Coulomb scaled by coulomb_scaling
+        i, j = mod[:, 0], mod[:, 1]
         d = x[j] - x[i]
-        el, fl, ec, fc = terms(i, j, d)
-        # FULL: remove; MODIFIED: scale LJ by lj_scaling and Coulomb by coulomb_scaling
-        wl = np.where(cls == 1, -1.0, c.lj_scaling - 1.0)
-        wc = np.where(cls == 1, -1.0, c.coulomb_scaling - 1.0)
-        fij = d * (wl * fl + wc * fc)[:, None]
+        el, fl, ec, fc = terms(i, j, d, mod=True)
+        ec, fc = ec * c.coulomb_scaling, fc * c.coulomb_scaling
+        fij = d * (fl + fc)[:, None]
         np.add.at(f, i, -fij)
         np.add.at(f, j, fij)
-        elj += float(np.sum(wl * el))
-        ecou += float(np.sum(wc * ec))
+        elj += float(np.sum(el))
+        ecou += float(np.sum(ec))
     e = np.zeros(36)
     e[0], e[1] = ecou, elj
     return f, e

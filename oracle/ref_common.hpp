@@ -39,6 +39,14 @@
 #include <protomol/type/ScalarStructure.h>
 #include <protomol/type/Vector3DBlock.h>
 #include <protomol/base/Report.h>
+#include <protomol/topology/BuildTopology.h>
+#include <protomol/type/PSF.h>
+#include <protomol/type/PAR.h>
+#include <protomol/type/PDB.h>
+#include <protomol/type/GromacsTopology.h>
+#include <protomol/type/GromacsParameters.h>
+#include <protomol/io/PDBReader.h>
+#include <protomol/io/gromacs/PortGromacsParameters.h>
 
 #include <cstdio>
 #include <cstdlib>

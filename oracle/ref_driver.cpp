@@ -205,6 +205,173 @@ static int do_bonded(Recs &r, PTopo &topo, Vector3DBlock &pos, Vector3DBlock &f,
   return 0;
 }
 
+// mode 7: a GROMACS/AMBER implicit-solvent regression input of the reference
+// (test/tests/*_FFAMBER_GB_LL.conf), read by the reference's own parsers: the
+// PDB positions, the .top with its force-field directory and optional GB
+// parameter file (IOModule.cpp:148-170), the topology as ProtoMolApp::build
+// makes it for doGBSAObc 2 (ProtoMolApp.cpp:283-313: OBC type 2, dielectric
+// offset 0.09) with the run's exclusion rule, then each force of the GB group
+// at those positions through NonbondedSimpleFull, one term at a time.  Writes
+// the built topology (per-atom charge, mass, type, molecule, GB radius and
+// scale, the LJ table, every excluded pair with its class, the harmonic bonds
+// and angles) and the forces and energies of each term, the bonds and angles
+// included.
+static std::string gets(Recs &r, const char *k) {
+  std::string s;
+  Recs::iterator it = r.find(k);
+  if (it == r.end()) return s;
+  for (size_t i = 0; i < it->second.i.size(); i++) s += (char)it->second.i[i];
+  return s;
+}
+
+template <class P>
+static void term_full(VTopo &topo, Vector3DBlock &pos, const P &pair, std::vector<double> &fo, std::vector<double> &eo) {
+  const int n = (int)pos.size();
+  Vector3DBlock f(n);
+  f.zero();
+  ScalarStructure e;
+  e.clear();
+  e.virial(true);
+  e.molecularVirial(false);
+  NonbondedSimpleFullSystemForce<P> force(pair);
+  force.preProcess(&topo, &pos);
+  force.evaluate(&topo, &pos, &f, &e);
+  force.postProcess(&topo, &e, &f);
+  fo.resize(3 * (size_t)n);
+  for (int i = 0; i < n; i++) { fo[3 * i] = f[i].c[0]; fo[3 * i + 1] = f[i].c[1]; fo[3 * i + 2] = f[i].c[2]; }
+  eo.resize(ScalarStructure::LAST);
+  for (int s = 0; s < ScalarStructure::LAST; s++) eo[s] = e[(ScalarStructure::Index)s];
+}
+
+static int do_gromacs_gb(Recs &r, const char *out) {
+  PSF psf;
+  PAR par;
+  GromacsTopology gTopo;
+  GromacsParameters gParams;
+  PortGromacsParameters port;
+  if (!port.Read_Basic_Gromacs_Parameters(psf, par, gTopo, gParams, gets(r, "path_top"), gets(r, "path_ff"))) return -1;
+  const std::string gbfile = gets(r, "path_gb");
+  if (!gbfile.empty() && !port.Read_Gromacs_GB_Parameters(gbfile)) return -1;
+  PDBReader reader(gets(r, "path_pdb"));
+  PDB pdb;
+  if (!reader.read(pdb)) return -1;
+  Vector3DBlock pos = pdb.coords;
+  const int n = (int)pos.size();
+  // "exclude scaled1-4" (the regressions' setting) is ONE4MODIFIED
+  VTopo topo(1.0, ExclusionType(ExclusionType::ONE4MODIFIED), VBC(), CCM(getd(r, "cellsize", 5.0)), 16.0f);
+  topo.doGBSAOpenMM = 1;
+  topo.obcType = 2;
+  topo.alphaObc = 1.0;
+  topo.betaObc = 0.8;
+  topo.gammaObc = 4.85;
+  topo.dielecOffset = 0.09;
+  topo.forceFieldFlag = GROMACS;
+  buildTopology(&topo, psf, par, false, NULL);
+  if ((int)topo.atoms.size() != n) return -1;
+  typedef UniversalSwitchingFunction U;
+  std::vector<double> f_gb, e_gb, f_lj, e_lj, f_cl, e_cl, tmpf, tmpe;
+  // the GB chain in its force-group order; the first two passes only fill the
+  // Born radii and partial sums (no force)
+  term_full(topo, pos, OneAtomPairNoExclusion<VBC, U, GBBornRadii>(GBBornRadii(), U()), tmpf, tmpe);
+  term_full(topo, pos, OneAtomPairNoExclusion<VBC, U, GBPartialSum>(GBPartialSum(), U()), tmpf, tmpe);
+  term_full(topo, pos, OneAtomPairNoExclusion<VBC, U, GBForce>(GBForce(getd(r, "eps_in", 1.0), getd(r, "eps_out", 78.3)), U()),
+            f_gb, e_gb);
+  std::vector<double> f_ace, e_ace;
+  term_full(topo, pos, OneAtomPairNoExclusion<VBC, U, GBACEForce>(GBACEForce(2.26 / 418.4, 1.4), U()), f_ace, e_ace);
+  term_full(topo, pos, OneAtomPair<VBC, U, LennardJonesForce>(LennardJonesForce(), U()), f_lj, e_lj);
+  term_full(topo, pos, OneAtomPair<VBC, U, CoulombForce>(CoulombForce(), U()), f_cl, e_cl);
+  // the built topology
+  std::vector<double> x(3 * (size_t)n), q(n), m(n), type(n), mol(n), rad(n), scl(n), off(n), born(n);
+  for (int i = 0; i < n; i++) {
+    for (int d = 0; d < 3; d++) x[3 * i + d] = pos[i].c[d];
+    const Atom &a = topo.atoms[i];
+    q[i] = a.scaledCharge;
+    m[i] = a.scaledMass;
+    type[i] = a.type;
+    mol[i] = a.molecule;
+    rad[i] = a.myGBSA_T->vanDerWaalRadius;
+    scl[i] = a.myGBSA_T->scalingFactor;
+    off[i] = a.myGBSA_T->offsetRadius;
+    born[i] = a.myGBSA_T->bornRad;
+  }
+  const int T = (int)topo.atomTypes.size();
+  std::vector<double> lj(4 * (size_t)T * T);
+  for (int a = 0; a < T; a++)
+    for (int b = 0; b < T; b++) {
+      const LennardJonesParameters &p = topo.lennardJonesParameters(a, b);
+      double *o = &lj[4 * ((size_t)a * T + b)];
+      o[0] = p.A; o[1] = p.B; o[2] = p.A14; o[3] = p.B14;
+    }
+  std::vector<double> excl;
+  for (int a = 0; a < n; a++)
+    for (int b = a + 1; b < n; b++) {
+      const ExclusionClass c = topo.exclusions.check(a, b);
+      if (c == EXCLUSION_NONE) continue;
+      excl.push_back(a);
+      excl.push_back(b);
+      excl.push_back(c == EXCLUSION_MODIFIED ? 2 : 1);
+    }
+  // the harmonic bonds and angles of the topology, and their forces
+  // (BondSystemForce, AngleSystemForce; the dihedral terms stay out)
+  std::vector<double> bonds, angles, f_bd, e_bd;
+  for (size_t k = 0; k < topo.bonds.size(); k++) {
+    const Bond &b = topo.bonds[k];
+    bonds.push_back(b.atom1); bonds.push_back(b.atom2);
+    bonds.push_back(b.springConstant); bonds.push_back(b.restLength);
+  }
+  for (size_t k = 0; k < topo.angles.size(); k++) {
+    const Angle &a = topo.angles[k];
+    angles.push_back(a.atom1); angles.push_back(a.atom2); angles.push_back(a.atom3);
+    angles.push_back(a.forceConstant); angles.push_back(a.restAngle);
+    angles.push_back(a.ureyBradleyConstant); angles.push_back(a.ureyBradleyRestLength);
+  }
+  {
+    Vector3DBlock fb(n);
+    fb.zero();
+    ScalarStructure eb;
+    eb.clear();
+    eb.virial(true);
+    eb.molecularVirial(false);
+    BondSystemForce<VBC> bf;
+    bf.evaluate(&topo, &pos, &fb, &eb);
+    AngleSystemForce<VBC> af;
+    af.evaluate(&topo, &pos, &fb, &eb);
+    f_bd.resize(3 * (size_t)n);
+    for (int i = 0; i < n; i++) { f_bd[3 * i] = fb[i].c[0]; f_bd[3 * i + 1] = fb[i].c[1]; f_bd[3 * i + 2] = fb[i].c[2]; }
+    e_bd.resize(ScalarStructure::LAST);
+    for (int q2 = 0; q2 < ScalarStructure::LAST; q2++) e_bd[q2] = eb[(ScalarStructure::Index)q2];
+  }
+  double scal[4] = {topo.coulombScalingFactor, topo.LJScalingFactor, (double)T, (double)n};
+  FILE *fp = fopen(out, "wb");
+  if (!fp) return -2;
+  write_rec_d(fp, "x", &x[0], 3LL * n);
+  write_rec_d(fp, "charge", &q[0], n);
+  write_rec_d(fp, "mass", &m[0], n);
+  write_rec_d(fp, "type", &type[0], n);
+  write_rec_d(fp, "molecule", &mol[0], n);
+  write_rec_d(fp, "gb_radius", &rad[0], n);
+  write_rec_d(fp, "gb_scaling", &scl[0], n);
+  write_rec_d(fp, "gb_offset", &off[0], n);
+  write_rec_d(fp, "born", &born[0], n);
+  write_rec_d(fp, "lj", &lj[0], (long long)lj.size());
+  write_rec_d(fp, "excl", excl.empty() ? NULL : &excl[0], (long long)excl.size());
+  write_rec_d(fp, "scalars", scal, 4);
+  write_rec_d(fp, "f_gb", &f_gb[0], 3LL * n);
+  write_rec_d(fp, "e_gb", &e_gb[0], (long long)e_gb.size());
+  write_rec_d(fp, "f_ace", &f_ace[0], 3LL * n);
+  write_rec_d(fp, "e_ace", &e_ace[0], (long long)e_ace.size());
+  write_rec_d(fp, "f_lj", &f_lj[0], 3LL * n);
+  write_rec_d(fp, "e_lj", &e_lj[0], (long long)e_lj.size());
+  write_rec_d(fp, "f_coul", &f_cl[0], 3LL * n);
+  write_rec_d(fp, "e_coul", &e_cl[0], (long long)e_cl.size());
+  write_rec_d(fp, "bonds", bonds.empty() ? NULL : &bonds[0], (long long)bonds.size());
+  write_rec_d(fp, "angles", angles.empty() ? NULL : &angles[0], (long long)angles.size());
+  write_rec_d(fp, "f_bonded", &f_bd[0], 3LL * n);
+  write_rec_d(fp, "e_bonded", &e_bd[0], (long long)e_bd.size());
+  fclose(fp);
+  return 0;
+}
+
 // mode 5: the velocity start of a run as MainModule draws it.  configure()
 // seeds the generator with randomNumber(seed, randomtype), which also returns
 // one number (MainModule.cpp:88-102); postBuild() then calls
@@ -237,6 +404,11 @@ int main(int argc, char **argv) {
   if (!read_recs(argv[1], r)) { fprintf(stderr, "cannot read %s\n", argv[1]); return 2; }
   int n = geti(r, "n", 0);
   int mode = geti(r, "mode", 1);
+  if (mode == 7) {
+    const int rc7 = do_gromacs_gb(r, argv[2]);
+    if (rc7) fprintf(stderr, "cannot build or evaluate the GROMACS GB input (%d)\n", rc7);
+    return rc7 ? 3 : 0;
+  }
   Vector3DBlock pos(n), f(n);
   for (int i = 0; i < n; i++) pos[i] = Vector3D(r["x"].d[3 * i], r["x"].d[3 * i + 1], r["x"].d[3 * i + 2]);
   f.zero();

@@ -429,7 +429,61 @@ def make_velocity():
     np.savez_compressed(os.path.join(HERE, "ref_velocity.npz"), **out)
 
 
+# the reference's GROMACS/AMBER implicit-solvent regressions: conf -> (pdb,
+# top, force-field directory, GB parameter file or None), all under test/tests/data
+GB_LL_CASES = {
+    "alanylalanine": ("alanylalanine_capped_processed.withH.pdb", "alanylalanine_capped_processed.top", "ffamber96",
+                      None),
+    "wwwfip35": ("www-fip35.pdb", "www-fip35.top", "ffamber03", "params.agb"),
+}
+
+
+def _path_rec(p):
+    return np.frombuffer(p.encode(), dtype=np.uint8).astype(np.int32)
+
+
+def make_gb_ll(name):
+    """test/tests/{alanylalanine,www-fip35}_FFAMBER_GB_LL.conf at step 0: the
+    reference reads the inputs with its own parsers and builds the topology
+    (ref_driver mode 7); the fixture keeps that topology as arrays and each GB
+    group force of it (GBForce after the Born-radii and partial-sum passes,
+    GBACEForce, LennardJones, Coulomb; NonbondedSimpleFull, vacuum), and its
+    harmonic bonds and angles with their forces."""
+    pdb, top, ff, gb = GB_LL_CASES[name]
+    data = os.path.join(REFTESTS, "data")
+    r = {"mode": np.int32(7), "n": np.int32(0), "path_pdb": _path_rec(os.path.join(data, pdb)),
+         "path_top": _path_rec(os.path.join(data, top)), "path_ff": _path_rec(os.path.join(data, ff)),
+         "eps_in": 1.0, "eps_out": 78.3}
+    if gb:
+        r["path_gb"] = _path_rec(os.path.join(data, gb))
+    # the parsers resolve the force-field directory relative to the run's directory
+    cwd = os.getcwd()
+    os.chdir(REFTESTS)
+    try:
+        o = refio.run(r)
+    finally:
+        os.chdir(cwd)
+    n = o["charge"].size
+    T = int(o["scalars"][2])
+    ex = o["excl"].reshape(-1, 3).astype(np.int32)
+    np.savez_compressed(os.path.join(HERE, f"ref_gb_ll_{name}.npz"), x=o["x"].reshape(-1, 3), charge=o["charge"],
+                        mass=o["mass"], type=o["type"].astype(np.int32), molecule=o["molecule"].astype(np.int32),
+                        gb_radius=o["gb_radius"], gb_scaling=o["gb_scaling"], gb_offset=o["gb_offset"],
+                        born=o["born"], lj_table=o["lj"].reshape(T, T, 4), exclusions=ex,
+                        coulomb_scaling=o["scalars"][0], lj_scaling=o["scalars"][1],
+                        f_gb=o["f_gb"].reshape(-1, 3), e_gb=o["e_gb"], f_ace=o["f_ace"].reshape(-1, 3),
+                        e_ace=o["e_ace"], f_lj=o["f_lj"].reshape(-1, 3), e_lj=o["e_lj"],
+                        f_coul=o["f_coul"].reshape(-1, 3), e_coul=o["e_coul"],
+                        bonds=o["bonds"].reshape(-1, 4), angles=o["angles"].reshape(-1, 7),
+                        f_bonded=o["f_bonded"].reshape(-1, 3), e_bonded=o["e_bonded"])
+    print(name, n, "atoms", T, "types", len(ex), "exclusions")
+
+
 def main():
+    if sys.argv[1:] == ["gb_ll"]:
+        for name in GB_LL_CASES:
+            make_gb_ll(name)
+        return
     w = water_fixture_system()
     if sys.argv[1:] == ["velocity"]:
         make_velocity()
@@ -480,6 +534,8 @@ def main():
     copy_water_expected()
     make_velocity()
     make_many_types()
+    for name in GB_LL_CASES:
+        make_gb_ll(name)
     d = droplet_fixture_system()
     for name, case in VACUUM_CASES.items():
         make_vacuum(name, d, case)
