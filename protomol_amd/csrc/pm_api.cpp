@@ -200,6 +200,10 @@ double build_switch(const pm_switch_desc &d, SwitchDev &s, bool ewald_hard, doub
     s.s1 = (float)(1.0 / (dd * dd * dd));
     s.s2 = (float)(d.cutoff * d.cutoff - 3.0 * d.switchon * d.switchon);
     s.s3 = (float)(4.0 / (dd * dd * dd));
+    s.c2d = (float)dd;
+    s.c2hd = (float)(0.5 * dd);
+    s.c2a1 = (float)(2.0 / (dd * dd * dd));
+    s.c2sc = (float)(12.0 / (dd * dd * dd));
     break;
   }
   case PM_SWITCH_CN: {  // CnSwitchingFunction.cpp:41-60, make() checks at :83-110

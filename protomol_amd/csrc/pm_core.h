@@ -83,6 +83,9 @@ struct SwitchDev {
   int kind;                // PM_SWITCH_*
   float cut2, on2;         // C2 / cutoff: cutoff^2, switchon^2 (FP32 path)
   float s1, s2, s3;        // C2 constants (C2SwitchingFunction.cpp:12-18)
+  // C2 in factored form (u = max(r^2, on^2), w = u - on^2, c2 = cut^2 - u):
+  // S = a1 c2^2 (w + D/2), dS = -6 a1 c2 w, D = cut^2 - on^2, a1 = 2 s1
+  float c2d, c2hd, c2a1, c2sc;  // D, D / 2, a1, 12 s1 (scale of pre-scaled LJ rows)
   float c15rc1, c05rc3, c15rc3;  // C1 constants (C1SwitchingFunction.cpp:12-18)
   // Cn / CmpCnCn are evaluated in FP64: their alternating-sign polynomials lose
   // too many digits in FP32

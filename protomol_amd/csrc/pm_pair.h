@@ -12,25 +12,37 @@ namespace pm {
 constexpr int kSwGeneric = -1;
 
 // OUTER: the switch's cutoff may be shorter than the pair's inclusion cutoff
-// (OneAtomPairTwo with two different cutoffs), so S = 0 beyond it
-template <int SW, bool OUTER = true>
+// (OneAtomPairTwo with two different cutoffs), so S = 0 beyond it.
+// PRE: E and F come pre-scaled, E' = 6 a1 E and F' = a1 F (the LJ rows of the
+// table in LDS are staged that way); E returns unscaled.
+template <int SW, bool OUTER = true, bool PRE = false>
 __device__ __forceinline__ void apply_switch_t(const SwitchDev &sw, float r2, float &E, float &F) {
   if (SW == PM_SWITCH_C2) {  // C2SwitchingFunction.h:42-53
-    // inside r_on the switch is 1 with zero derivative: the unswitched values
-    // are kept as they are (one select on each result)
-    const float c2 = sw.cut2 - r2;
-    const float c4 = c2 * (sw.s2 + 2.0f * r2);
-    const float S = sw.s1 * (c2 * c4);
-    const float dS = sw.s3 * (c2 * c2 - c4);
-    const bool inner = r2 < sw.on2;
-    float Fs = F * S - E * dS, Es = E * S;
+    // S = s1 c2 c4 and dS = s3 (c2^2 - c4), c2 = cut^2 - r^2, c4 = c2 (s2 + 2 r^2),
+    // s2 = cut^2 - 3 on^2, s3 = 4 s1 = 4 / D^3, D = cut^2 - on^2.  Factored:
+    // with w = r^2 - on^2, S = a1 c2^2 (w + D/2) and dS = -6 a1 c2 w (a1 = 2 s1),
+    // so F S - E dS = a1 c2 (c2 (w + D/2) F + 6 w E).  Inside r_on, w is
+    // clamped to 0 (an integer max of the bits: no canonicalizing float max),
+    // where S = 1 and dS = 0 to FP32 rounding: the unswitched values.
+    const float w = __int_as_float(max(__float_as_int(r2 - sw.on2), 0));
+    const float c2 = sw.c2d - w;
+    const float cp = c2 * (w + sw.c2hd);
+    float Fs, Es;
+    if (PRE) {
+      Fs = c2 * fmaf(cp, F, w * E);
+      Es = (E * c2) * cp * (1.0f / 6.0f);
+    } else {
+      const float ac = sw.c2a1 * c2;
+      Fs = ac * fmaf(w * E, 6.0f, cp * F);
+      Es = ac * (cp * E);
+    }
     if (OUTER) {
       const bool outer = r2 > sw.cut2;
       Fs = outer ? 0.0f : Fs;
       Es = outer ? 0.0f : Es;
     }
-    F = inner ? F : Fs;
-    E = inner ? E : Es;
+    F = Fs;
+    E = Es;
   } else if (SW == PM_SWITCH_CUTOFF || SW == PM_SWITCH_NONE) {
     // inside the inclusion cutoff the value is 1 (CutoffSwitchingFunction.h:41-44)
   } else {
@@ -122,7 +134,9 @@ constexpr int kPairWaveLds = kLJLds * 16 + kWave * (16 + 4 + 4 + 8);
 // the scalar force factor F/r for the pair, energies into ea / eb.  qq is the
 // charge product with the modified-pair scale already applied; A, B are the
 // pair's LJ coefficients and A12 = 12 A, B6 = 6 B.
-template <int TA, int TB, int SWA, int SWB, bool SAME>
+// PRE: the LJ coefficients come pre-scaled for the C2 switch's factored form
+// (A, B, A12, B6 times 12 s1, 12 s1, 2 s1, 2 s1)
+template <int TA, int TB, int SWA, int SWB, bool SAME, bool PRE = false>
 __device__ __forceinline__ float pair_terms(const ForceDev &P, float r2, float qq, float A, float B, float A12,
                                             float B6, bool inA, bool inB, float &ea, float &eb) {
   const float rinv = __builtin_amdgcn_rsqf(r2);
@@ -136,7 +150,7 @@ __device__ __forceinline__ float pair_terms(const ForceDev &P, float r2, float q
     const float r6 = r2inv * r2inv * r2inv;
     float E = r6 * fmaf(A, r6, -B);
     float F = r6 * fmaf(A12, r6, -B6) * r2inv;
-    apply_switch_t<SWA, !SAME>(P.sw_a, r2, E, F);
+    apply_switch_t<SWA, !SAME, PRE>(P.sw_a, r2, E, F);
     fs = inA ? F : 0.f;
     ea = inA ? E : 0.f;
   } else if (TA == PM_TERM_COULOMB) {
@@ -209,6 +223,8 @@ __device__ __forceinline__ void pair_wave(const SysDev *__restrict__ S, const in
   constexpr bool CEN = KIND == kPairCensus;
   constexpr bool peratom = LJM == kLjPerAtom;
   constexpr bool big_table = LJM == kLjGlobal;
+  // the LDS table's rows pre-scaled for the C2 switch's factored form
+  constexpr bool PRE = TA == PM_TERM_LJ && SWA == PM_SWITCH_C2 && LJM == kLjLds;
   extern __shared__ float4 pair_lds[];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
@@ -260,13 +276,17 @@ __device__ __forceinline__ void pair_wave(const SysDev *__restrict__ S, const in
     lpp[lane] = pp;
     valid_ca = lane_mask(valid);
   }
-  // LJ table (T <= 16; larger ones stay in global memory) as (A, B, 12 A, 6 B); the
-  // modified-pair coefficients stay in the global table
+  // LJ table (T <= 16; larger ones stay in global memory) as (A, B, 12 A, 6 B),
+  // or with a C2 switch as (12 s1) (A, B, 2 A, B): the energy 6 a1 E and force
+  // a1 F the factored switch takes (a1 = 2 s1); the modified-pair
+  // coefficients stay in the global table
   const PM_GLOBAL float4 *gtab = gp(S->lj_table);
   if (TA == PM_TERM_LJ && !peratom && !big_table) {
+    const float sc = P.sw_a.c2sc;
     for (int k = lane; k < T * T; k += kWave) {
       const float4 g = gtab[k];
-      ljs[k] = make_float4(g.x, g.y, 12.0f * g.x, 6.0f * g.y);
+      ljs[k] = PRE ? make_float4(sc * g.x, sc * g.y, 2.0f * (sc * g.x), sc * g.y)
+                   : make_float4(g.x, g.y, 12.0f * g.x, 6.0f * g.y);
     }
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -398,13 +418,20 @@ __device__ __forceinline__ void pair_wave(const SysDev *__restrict__ S, const in
               A = g.z;
               B = g.w;
             }
-            A12 = 12.0f * A;
-            B6 = 6.0f * B;
+            if (PRE) {
+              A *= P.sw_a.c2sc;
+              B *= P.sw_a.c2sc;
+              A12 = 2.0f * A;
+              B6 = B;
+            } else {
+              A12 = 12.0f * A;
+              B6 = 6.0f * B;
+            }
           }
         }
       }
       float ea, eb;
-      const float fs = pair_terms<TA, TB, SWA, SWB, SAME>(P, r2, qq, A, B, A12, B6, inA, inB, ea, eb);
+      const float fs = pair_terms<TA, TB, SWA, SWB, SAME, PRE>(P, r2, qq, A, B, A12, B6, inA, inB, ea, eb);
       const float fx = fs * dx, fy = fs * dy, fz = fs * dz;
       fi[c][0] -= fx;
       fi[c][1] -= fy;
