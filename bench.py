@@ -265,6 +265,21 @@ def headline(args):
         t3 = time.perf_counter()
         log(f"energies every 10th step: {1e3 * (t3 - t2) / n_en:.3f} ms/step; counting pairs")
         cnt, _ = ctx.census(0, x)
+    # the drop-in call (SystemForce::evaluate through pm_evaluate): host
+    # positions in, host forces and energies out, so PCIe is included; the
+    # list is built and energies and virial are summed on every call
+    with L.Context(desc(0), [system]) as ectx:
+        fe = np.zeros_like(x)
+        ee = np.zeros(L.NSLOTS)
+        for _ in range(3):
+            ectx.evaluate(0, x, fe, ee)
+        n_ev = 20
+        t6 = time.perf_counter()
+        for _ in range(n_ev):
+            ectx.evaluate(0, x, fe, ee)
+        t7 = time.perf_counter()
+    ev_ms = 1e3 * (t7 - t6) / n_ev
+    log(f"drop-in pm_evaluate (host buffers, PCIe included): {ev_ms:.3f} ms per evaluation")
     # the pair kernel's launch duration: HIP events on the launch stream around
     # every pair launch, over the same number of steps from the timed region's
     # start state (run eagerly: a replayed graph holds no per-launch events)
@@ -313,6 +328,10 @@ def headline(args):
         "ns_per_day": steps_s * DT_FS * 0.0864,
         "energies_every_10": {"ms_per_step": ms_en, "ns_per_day": 1e3 / ms_en * DT_FS * 0.0864,
                               "pair_interactions_per_s": pairs * 1e3 / ms_en, "steps": n_en},
+        "drop_in_evaluate": {"ms_per_evaluation": ev_ms, "pair_interactions_per_s": pairs * 1e3 / ev_ms,
+                             "what": "pm_evaluate with host positions in and host forces and energies out "
+                                     "(PCIe included), list built and energies summed on every call; not "
+                                     "the headline"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": pair_kernel_traffic(),
                      "kernel": "k_pair<LJ, EWALD_REAL> (forces)", "kernel_ms": pair_ms,
