@@ -16,7 +16,10 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libprotomol_amd.so")
 # the pair-kernel sources first: they take longest
 SOURCES = ["pm_pair_lj_ewald.hip", "pm_pair_lj_coulomb.hip", "pm_pair_lj.hip", "pm_pair_coulomb.hip",
-           "pm_pair_ewald.hip", "pm_kernels.hip", "pm_gb.hip", "pm_ewald.hip", "pm_api.cpp"]
+           "pm_pair_ewald.hip", "pm_kernels.hip", "pm_local.hip", "pm_gb.hip", "pm_ewald.hip", "pm_api.cpp"]
+# per-source flags: the local terms are built without machine-level LICM
+# (their FP64 library constants would otherwise stay in registers across loops)
+EXTRA = {"pm_local.hip": ["-mllvm", "-disable-machine-licm"]}
 ARCH = os.environ.get("PM_OFFLOAD_ARCH", "gfx950")
 
 # packed-FP32 instructions are kept out of the generated code on purpose
@@ -42,7 +45,7 @@ def build(verbose: bool = False, jobs: int = 0) -> str:
         obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
         objs.append(obj)
         lang = ["-x", "hip"]
-        cmd = [cc, *lang, *HIPFLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [cc, *lang, *HIPFLAGS, *EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))

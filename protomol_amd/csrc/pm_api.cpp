@@ -1362,16 +1362,12 @@ void force_eval(pm_ctx *c, const Launch &L, int mode, const StepPolicy &pol, int
   pm::launch_local(L, c->P, local);
 }
 
-// energies and virial at the current positions with the current pair list
-// (no forces): the pair kernel's energy-only form and the local terms' sums
-void energy_pass(pm_ctx *c, const Launch &L) {
-  if (c->gb) return;  // GB sums energies inside its force passes
-  if (c->molvir) pm::launch_mol_com(L);
-  if (!pm::launch_pair(L, c->P, c->molvir ? pm::kPairEnergiesMV : pm::kPairEnergies))
-    throw Fail{PM_EINVAL, "unsupported term combination"};
-  // from the structure factors and partial forces of the force pass
-  if (c->recip) pm::launch_ewald(L, ewald_energy_mode(c));
-  pm::launch_local(L, c->P, pm::kLocalEnergies);
+// The pair-kernel kind of an MD step's force pass: a step that sums energies
+// does it in the same pass as the forces (GB sums its energies inside its own
+// passes).  Every kind rounds the forces alike, so the trajectory does not
+// depend on which steps sum energies.
+int step_kind(const pm_ctx *c, bool energies) {
+  return energies && !c->gb ? pm::kPairForcesEnergies : pm::kPairForces;
 }
 
 // copy the packed scalars of the set's systems to the host (synchronizes)
@@ -1664,8 +1660,7 @@ hipGraphExec_t md_graph(pm_ctx *c, LaunchSet &ls, double h, const StepPolicy &po
     for (int k = 0; k < kGraphSteps; ++k) {
       const bool en = energies && k == kGraphSteps - 1;
       pm::launch_kick_drift(ls.L, h, lv);
-      force_eval(c, ls.L, pm::kStepAuto, pol, pm::kPairForces, en, false);
-      if (en) energy_pass(c, ls.L);
+      force_eval(c, ls.L, pm::kStepAuto, pol, step_kind(c, en), en, false);
     }
   } catch (...) {
     (void)hipStreamEndCapture(c->stream, &g);
@@ -1730,8 +1725,7 @@ void md_advance(pm_ctx *c, const std::vector<int> &ids, int n_steps, double dt_f
     for (; i < n_steps; ++i) {
       const bool en = energy_step(i);
       pm::launch_kick_drift(ls.L, h, lv_params(c, dt_fs));
-      force_eval(c, ls.L, pm::kStepAuto, pol, pm::kPairForces, en, false);
-      if (en) energy_pass(c, ls.L);
+      force_eval(c, ls.L, pm::kStepAuto, pol, step_kind(c, en), en, false);
     }
     if (depth == 0) HIPCHECK(hipEventRecord(c->ev1, c->stream));
     read_scalars(c, ls);

@@ -1,9 +1,9 @@
 // pm_kernels.hip -- gfx950 kernels of the nonbonded force path.
 //
-//   rebuild:  k_bin -> k_col_scan -> k_col_fill -> k_col_sort -> k_cluster_geom -> k_pairlist
-//   force:    k_local (bonded + Ewald correction); k_pair lives in pm_pair.h,
+//   rebuild:  k_bin -> k_col_sort -> k_pairlist
+//   force:    k_local (bonded + Ewald correction) lives in pm_local.hip, k_pair in pm_pair.h,
 //             instantiated by pm_pair_*.hip
-//   step:     k_kick_drift, k_half_kick, k_export_f, k_kinetic
+//   step:     k_kick_drift, k_half_kick, k_md_arm; state in and out: k_import, k_export
 //
 // All kernels take the device table of SysDev records plus a prefix array
 // over the systems, so one launch covers every system of a batch.
@@ -588,251 +588,6 @@ __global__ __launch_bounds__(kListThreads) void k_pairlist(const SysDev *__restr
 //   BondSystemForce::calcBond (force/bonded/BondSystemForce.h:77-108)
 //   AngleSystemForce::calcAngle (force/bonded/AngleSystemForce.h:66-140)
 //   NonbondedFullEwaldSystemForce::correctionTerm (NonbondedFullEwaldSystemForce.h:1097-1131)
-// terms of each kind one batch of an atom's local terms takes
-constexpr int kLocBatchB = 2, kLocBatchA = 1, kLocBatchC = 2;
-
-// One atom's local terms.  The terms are taken in batches; within a batch the
-// loads are issued level by level for every term at once (items, then the
-// terms' atoms and constants, then the atoms' positions), so an atom waits
-// for three round trips per batch instead of three per term.  The arithmetic
-// then runs in item order, bonds, angles, corrections.
-template <bool EN>
-__device__ __forceinline__ void local_atom(const SysDev &S, int a, const ForceDev &P, int mode, double &eb, double &ea,
-                                           double &ec, double v[6]) {
-  const int n = S.n_atoms;
-  const PM_GLOBAL double *X = gp(S.x);
-  const PM_GLOBAL int *item = gp(S.loc_item);
-  const PM_GLOBAL int *ls = gp(S.loc_start) + 3 * (size_t)a;
-  const int b_end = ls[kLocAngle], a_end = ls[kLocCorr], c_end = ls[kLocCorr + 1];
-  int ib = ls[kLocBond], ia = b_end, ic = a_end;
-  double f[3] = {0.0, 0.0, 0.0};
-  auto pos = [&](int k, double p[3]) {
-    p[0] = X[k];
-    p[1] = X[n + k];
-    p[2] = X[2 * n + k];
-  };
-  // one batch; the first is straight-line code (a water atom's terms fit in
-  // it), the loop takes the rest: a loop around every batch would hoist the
-  // FP64 library constants of erf / atan2 into registers for its whole length
-  auto batch = [&]() {
-    // items
-    int tb[kLocBatchB], ta[kLocBatchA], tc[kLocBatchC];
-#pragma unroll
-    for (int j = 0; j < kLocBatchB; ++j) tb[j] = ib + j < b_end ? item[ib + j] : -1;
-#pragma unroll
-    for (int j = 0; j < kLocBatchA; ++j) ta[j] = ia + j < a_end ? item[ia + j] : -1;
-#pragma unroll
-    for (int j = 0; j < kLocBatchC; ++j) tc[j] = ic + j < c_end ? item[ic + j] : -1;
-    // the terms' atoms and constants
-    int b1[kLocBatchB], b2[kLocBatchB], a1[kLocBatchA], a2[kLocBatchA], a3[kLocBatchA], c1[kLocBatchC],
-        c2[kLocBatchC], cc[kLocBatchC];
-    double bk[kLocBatchB], br[kLocBatchB], ak[kLocBatchA], at[kLocBatchA];
-#pragma unroll
-    for (int j = 0; j < kLocBatchB; ++j)
-      if (tb[j] >= 0) {
-        const int k = tb[j] >> 2;
-        b1[j] = gp(S.bonds)[2 * k];
-        b2[j] = gp(S.bonds)[2 * k + 1];
-        bk[j] = gp(S.bond_k)[k];
-        br[j] = gp(S.bond_r0)[k];
-      }
-#pragma unroll
-    for (int j = 0; j < kLocBatchA; ++j)
-      if (ta[j] >= 0) {
-        const int k = ta[j] >> 2;
-        a1[j] = gp(S.angles)[3 * k];
-        a2[j] = gp(S.angles)[3 * k + 1];
-        a3[j] = gp(S.angles)[3 * k + 2];
-        ak[j] = gp(S.angle_k)[k];
-        at[j] = gp(S.angle_t0)[k];
-      }
-#pragma unroll
-    for (int j = 0; j < kLocBatchC; ++j)
-      if (tc[j] >= 0) {
-        const int k = tc[j] >> 2;
-        c1[j] = gp(S.corr)[3 * k];
-        c2[j] = gp(S.corr)[3 * k + 1];
-        cc[j] = gp(S.corr)[3 * k + 2];
-      }
-    // positions (and the corrections' charges), kept as the terms' vectors
-    double rb[kLocBatchB][3], ra12[kLocBatchA][3], ra32[kLocBatchA][3], rc[kLocBatchC][3], qq0[kLocBatchC];
-    {
-      double pb1[kLocBatchB][3], pb2[kLocBatchB][3], pa1[kLocBatchA][3], pa2[kLocBatchA][3], pa3[kLocBatchA][3],
-          pc1[kLocBatchC][3], pc2[kLocBatchC][3], q1[kLocBatchC], q2[kLocBatchC];
-#pragma unroll
-      for (int j = 0; j < kLocBatchB; ++j)
-        if (tb[j] >= 0) {
-          pos(b1[j], pb1[j]);
-          pos(b2[j], pb2[j]);
-        }
-#pragma unroll
-      for (int j = 0; j < kLocBatchA; ++j)
-        if (ta[j] >= 0) {
-          pos(a1[j], pa1[j]);
-          pos(a2[j], pa2[j]);
-          pos(a3[j], pa3[j]);
-        }
-#pragma unroll
-      for (int j = 0; j < kLocBatchC; ++j)
-        if (tc[j] >= 0) {
-          pos(c1[j], pc1[j]);
-          pos(c2[j], pc2[j]);
-          q1[j] = gp(S.charge)[c1[j]];
-          q2[j] = gp(S.charge)[c2[j]];
-        }
-#pragma unroll
-      for (int j = 0; j < kLocBatchB; ++j)
-        for (int d = 0; d < 3; ++d) rb[j][d] = tb[j] >= 0 ? pb1[j][d] - pb2[j][d] : 0.0;
-#pragma unroll
-      for (int j = 0; j < kLocBatchA; ++j)
-        for (int d = 0; d < 3; ++d) {
-          ra12[j][d] = ta[j] >= 0 ? pa1[j][d] - pa2[j][d] : 0.0;
-          ra32[j][d] = ta[j] >= 0 ? pa3[j][d] - pa2[j][d] : 0.0;
-        }
-#pragma unroll
-      for (int j = 0; j < kLocBatchC; ++j) {
-        for (int d = 0; d < 3; ++d) rc[j][d] = tc[j] >= 0 ? pc2[j][d] - pc1[j][d] : 0.0;
-        qq0[j] = tc[j] >= 0 ? q1[j] * q2[j] : 0.0;
-      }
-    }
-    // BondSystemForce::calcBond (force/bonded/BondSystemForce.h:77-108)
-#pragma unroll
-    for (int j = 0; j < kLocBatchB; ++j)
-      if (tb[j] >= 0) {
-        const int role = tb[j] & 3;
-        double r12[3] = {rb[j][0], rb[j][1], rb[j][2]};
-        min_image_vec(S, r12);
-        const double r = sqrt(r12[0] * r12[0] + r12[1] * r12[1] + r12[2] * r12[2]);
-        const double dp = 2.0 * bk[j] * (r - br[j]);
-        const double c = -dp / r;
-        const double sg = role == 0 ? 1.0 : -1.0;
-        for (int d = 0; d < 3; ++d) f[d] += sg * (r12[d] * c);
-        if (EN && role == 0) {
-          eb += bk[j] * (r - br[j]) * (r - br[j]);
-          v[0] += r12[0] * c * r12[0];
-          v[1] += r12[0] * c * r12[1];
-          v[2] += r12[0] * c * r12[2];
-          v[3] += r12[1] * c * r12[1];
-          v[4] += r12[1] * c * r12[2];
-          v[5] += r12[2] * c * r12[2];
-        }
-      }
-    // AngleSystemForce::calcAngle (force/bonded/AngleSystemForce.h:66-140)
-#pragma unroll
-    for (int j = 0; j < kLocBatchA; ++j)
-      if (ta[j] >= 0) {
-        const int role = ta[j] & 3;
-        double r12[3] = {ra12[j][0], ra12[j][1], ra12[j][2]};
-        double r32[3] = {ra32[j][0], ra32[j][1], ra32[j][2]};
-        min_image_vec(S, r12);
-        min_image_vec(S, r32);
-        const double d12 = sqrt(r12[0] * r12[0] + r12[1] * r12[1] + r12[2] * r12[2]);
-        const double d32 = sqrt(r32[0] * r32[0] + r32[1] * r32[1] + r32[2] * r32[2]);
-        const double cr0 = r12[1] * r32[2] - r12[2] * r32[1], cr1 = r12[2] * r32[0] - r12[0] * r32[2],
-                     cr2 = r12[0] * r32[1] - r12[1] * r32[0];
-        const double cn = sqrt(cr0 * cr0 + cr1 * cr1 + cr2 * cr2);
-        const double dt = r12[0] * r32[0] + r12[1] * r32[1] + r12[2] * r32[2];
-        const double theta = atan2(cn, dt);
-        // sin and cos of theta = atan2(|r12 x r32|, r12 . r32), from the same vectors
-        const double inv = 1.0 / (d12 * d32);
-        const double st = cn * inv, ct = dt * inv;
-        const double dp = 2.0 * ak[j] * (theta - at[j]);
-        double f1[3], f3[3];
-        for (int d = 0; d < 3; ++d) {
-          const double u12 = r12[d] / d12, u32 = r32[d] / d32;
-          f1[d] = -((u12 * ct - u32) / (st * d12)) * dp;
-          f3[d] = -((u32 * ct - u12) / (st * d32)) * dp;
-        }
-        for (int d = 0; d < 3; ++d) f[d] += role == 0 ? f1[d] : (role == 2 ? f3[d] : -f1[d] - f3[d]);
-        if (EN && role == 0) {
-          ea += ak[j] * (theta - at[j]) * (theta - at[j]);
-          v[0] += f1[0] * r12[0] + f3[0] * r32[0];
-          v[1] += f1[0] * r12[1] + f3[0] * r32[1];
-          v[2] += f1[0] * r12[2] + f3[0] * r32[2];
-          v[3] += f1[1] * r12[1] + f3[1] * r32[1];
-          v[4] += f1[1] * r12[2] + f3[1] * r32[2];
-          v[5] += f1[2] * r12[2] + f3[2] * r32[2];
-        }
-      }
-    // NonbondedFullEwaldSystemForce::correctionTerm (NonbondedFullEwaldSystemForce.h:1097-1131)
-#pragma unroll
-    for (int j = 0; j < kLocBatchC; ++j)
-      if (tc[j] >= 0) {
-        const int role = tc[j] & 3;
-        double d[3] = {rc[j][0], rc[j][1], rc[j][2]};
-        min_image_vec(S, d);
-        const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-        double qq = qq0[j];
-        if (cc[j] == PM_EXCL_MODIFIED) qq *= 1.0 - (double)S.csf;
-        const double r = sqrt(r2), rr = 1.0 / r;
-        const double e = erf(P.dalpha * r) * rr;  // C-library erf, as the reference on Linux
-        const double fs = qq * (P.dtwo_alpha_pi * exp(-(P.dalpha * P.dalpha) * r2) - e) * rr * rr;
-        const double sg = role == 0 ? -1.0 : 1.0;
-        for (int q = 0; q < 3; ++q) f[q] += sg * (d[q] * fs);
-        if (EN && role == 0) {
-          ec += -qq * e;
-          v[0] += d[0] * fs * d[0];
-          v[1] += d[0] * fs * d[1];
-          v[2] += d[0] * fs * d[2];
-          v[3] += d[1] * fs * d[1];
-          v[4] += d[1] * fs * d[2];
-          v[5] += d[2] * fs * d[2];
-        }
-      }
-    ib += kLocBatchB;
-    ia += kLocBatchA;
-    ic += kLocBatchC;
-  };
-  batch();
-  while (ib < b_end || ia < a_end || ic < c_end) batch();
-  if (mode & kLocalForces) {  // every atom, zero for one without local terms
-    PM_GLOBAL long long *fl = gp(S.fl);
-    fl[a] = to_fixed(f[0], kFScale);
-    fl[n + a] = to_fixed(f[1], kFScale);
-    fl[2 * n + a] = to_fixed(f[2], kFScale);
-  }
-}
-
-template <bool EN>
-__global__ __launch_bounds__(256, 3) void k_local(const SysDev *__restrict__ sys, const int *__restrict__ ap, int nsys,
-                                               ForceDev P, int mode) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  const int total = ap[nsys];
-  const bool ok = g < total;
-  double eb = 0.0, ea = 0.0, ec = 0.0, v[6] = {0, 0, 0, 0, 0, 0};
-  int s = wave_sys(ap, nsys, g, total);
-  if (s >= 0) {
-    const SysDev *__restrict__ S = sys + s;  // scalar loads of the record
-    if (ok) local_atom<EN>(*S, g - ap[s], P, mode, eb, ea, ec, v);
-  } else {
-    s = ok ? find_sys(ap, nsys, g) : 0;
-    if (ok) local_atom<EN>(sys[s], g - ap[s], P, mode, eb, ea, ec, v);
-  }
-  if (!EN) return;
-  // bonded and correction terms add to the atomic virial only when it is on
-  // (BondSystemForce.h:121, AngleSystemForce.h:155, NonbondedFullEwaldSystemForce.h:1125)
-  if (!P.virial)
-    for (int k = 0; k < 6; ++k) v[k] = 0.0;
-  // each lane's terms go to fixed point first; integer sums are exact, so the
-  // totals do not depend on how lanes are grouped
-  long long q[9] = {to_fixed(eb, kEScale), to_fixed(ea, kEScale), to_fixed(ec, kEScale),
-                    to_fixed(v[0], kEScale), to_fixed(v[1], kEScale), to_fixed(v[2], kEScale),
-                    to_fixed(v[3], kEScale), to_fixed(v[4], kEScale), to_fixed(v[5], kEScale)};
-  const int slots[9] = {kAccBond, kAccAngle, kAccCoul, kAccVxx, kAccVxy, kAccVxz, kAccVyy, kAccVyz, kAccVzz};
-  const int s0 = __shfl(s, 0);
-  const bool mixed = wave_ballot(ok && s != s0) != 0;
-  const int wid = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  if (!mixed) {
-    for (int k = 0; k < 9; ++k) q[k] = wave_sum_ll(q[k]);
-    if ((threadIdx.x & 63) == 0)
-      for (int k = 0; k < 9; ++k)
-        if (q[k]) atomic_add_ll(&acc_copy(sys[s0].acc, wid)[slots[k]], q[k]);
-  } else if (ok) {
-    for (int k = 0; k < 9; ++k)
-      if (q[k]) atomic_add_ll(&acc_copy(sys[s].acc, wid)[slots[k]], q[k]);
-  }
-}
-
 // ------------------------------------------------------------ integrator
 
 // First step of a run (the system's kCtlFirst word is set):
@@ -1118,14 +873,6 @@ bool launch_pair(const Launch &L, const ForceDev &P, int kind) {
   return true;
 }
 
-void launch_local(const Launch &L, const ForceDev &P, int mode) {
-  if (!L.n_atoms_total || !L.n_local_total) return;
-  const dim3 g(nblk(L.n_atoms_total, 256)), b(256);
-  if (mode & kLocalEnergies)
-    hipLaunchKernelGGL(k_local<true>, g, b, 0, L.stream, L.sys, L.atom_prefix, L.nsys, P, mode);
-  else
-    hipLaunchKernelGGL(k_local<false>, g, b, 0, L.stream, L.sys, L.atom_prefix, L.nsys, P, mode);
-}
 void launch_kick_drift(const Launch &L, double h, const LangevinDev &lv) {
   if (L.n_atoms_total) hipLaunchKernelGGL(k_kick_drift, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys, h, lv);
 }

@@ -1,0 +1,193 @@
+// pm_local.hip -- the local terms of the force pass (bonds, angles, Ewald
+// excluded-pair corrections), FP64, owner computes, no atomics on forces.
+// Built with machine-level LICM off: a loop around these terms would
+// otherwise keep the FP64 library constants of erf / exp / atan2 in
+// registers for its whole length (178 against 92 VGPRs per lane).
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "pm_device.h"
+
+namespace pm {
+
+// Local terms of an atom split over the waves of a block by (kind, slot):
+// a block covers 64 atoms with kLocWaves waves, wave w takes kind kLocKind[w]
+// and the items slot, slot + nslots, ... of that kind of each of its atoms,
+// so a wave runs one kind's code and an atom waits for the longest of its
+// terms instead of their sum.  Slots per kind: bonds 2, angles 2, Ewald
+// corrections 4.
+constexpr int kLocWaves = 8;
+__device__ __forceinline__ void loc_slot(int w, int &kind, int &slot, int &nslots) {
+  if (w < 2) {
+    kind = kLocBond, slot = w, nslots = 2;
+  } else if (w < 4) {
+    kind = kLocAngle, slot = w - 2, nslots = 2;
+  } else {
+    kind = kLocCorr, slot = w - 4, nslots = 4;
+  }
+}
+
+// An atom's items of one kind in slot `slot` of `nslots`, FP64, evaluated
+// from that atom's side ("owner computes": each participant evaluates the
+// term with the same expressions, so the forces on its atoms are exactly
+// opposite; energies and virial are counted by the term's first atom).
+//   BondSystemForce::calcBond (force/bonded/BondSystemForce.h:77-108)
+//   AngleSystemForce::calcAngle (force/bonded/AngleSystemForce.h:66-140)
+//   NonbondedFullEwaldSystemForce::correctionTerm (NonbondedFullEwaldSystemForce.h:1097-1131)
+template <bool EN>
+__device__ __forceinline__ void local_terms(const SysDev &S, int a, const ForceDev &P, int kind, int slot, int nslots,
+                                            double f[3], double &eb, double &ea, double &ec, double v[6]) {
+  const int n = S.n_atoms;
+  const PM_GLOBAL double *X = gp(S.x);
+  const PM_GLOBAL int *ls = gp(S.loc_start) + 3 * (size_t)a;
+  const int beg = ls[kind], end = ls[kind + 1];
+  for (int it = beg + slot; it < end; it += nslots) {
+    const int item = gp(S.loc_item)[it];
+    const int role = item & 3, k = item >> 2;
+    if (kind == kLocBond) {
+      const int a1 = gp(S.bonds)[2 * k], a2 = gp(S.bonds)[2 * k + 1];
+      const double kb = gp(S.bond_k)[k], r0 = gp(S.bond_r0)[k];
+      double r12[3] = {X[a1] - X[a2], X[n + a1] - X[n + a2], X[2 * n + a1] - X[2 * n + a2]};
+      min_image_vec(S, r12);
+      const double r = sqrt(r12[0] * r12[0] + r12[1] * r12[1] + r12[2] * r12[2]);
+      const double dp = 2.0 * kb * (r - r0);
+      const double c = -dp / r;
+      const double sg = role == 0 ? 1.0 : -1.0;
+      for (int d = 0; d < 3; ++d) f[d] += sg * (r12[d] * c);
+      if (EN && role == 0) {
+        eb += kb * (r - r0) * (r - r0);
+        v[0] += r12[0] * c * r12[0];
+        v[1] += r12[0] * c * r12[1];
+        v[2] += r12[0] * c * r12[2];
+        v[3] += r12[1] * c * r12[1];
+        v[4] += r12[1] * c * r12[2];
+        v[5] += r12[2] * c * r12[2];
+      }
+    } else if (kind == kLocAngle) {
+      const int a1 = gp(S.angles)[3 * k], a2 = gp(S.angles)[3 * k + 1], a3 = gp(S.angles)[3 * k + 2];
+      const double ka = gp(S.angle_k)[k], t0 = gp(S.angle_t0)[k];
+      double r12[3] = {X[a1] - X[a2], X[n + a1] - X[n + a2], X[2 * n + a1] - X[2 * n + a2]};
+      double r32[3] = {X[a3] - X[a2], X[n + a3] - X[n + a2], X[2 * n + a3] - X[2 * n + a2]};
+      min_image_vec(S, r12);
+      min_image_vec(S, r32);
+      const double d12 = sqrt(r12[0] * r12[0] + r12[1] * r12[1] + r12[2] * r12[2]);
+      const double d32 = sqrt(r32[0] * r32[0] + r32[1] * r32[1] + r32[2] * r32[2]);
+      const double cr0 = r12[1] * r32[2] - r12[2] * r32[1], cr1 = r12[2] * r32[0] - r12[0] * r32[2],
+                   cr2 = r12[0] * r32[1] - r12[1] * r32[0];
+      const double cn = sqrt(cr0 * cr0 + cr1 * cr1 + cr2 * cr2);
+      const double dt = r12[0] * r32[0] + r12[1] * r32[1] + r12[2] * r32[2];
+      const double theta = atan2(cn, dt);
+      // sin and cos of theta = atan2(|r12 x r32|, r12 . r32), from the same vectors
+      const double inv = 1.0 / (d12 * d32);
+      const double st = cn * inv, ct = dt * inv;
+      const double dp = 2.0 * ka * (theta - t0);
+      double f1[3], f3[3];
+      for (int d = 0; d < 3; ++d) {
+        const double u12 = r12[d] / d12, u32 = r32[d] / d32;
+        f1[d] = -((u12 * ct - u32) / (st * d12)) * dp;
+        f3[d] = -((u32 * ct - u12) / (st * d32)) * dp;
+      }
+      for (int d = 0; d < 3; ++d) f[d] += role == 0 ? f1[d] : (role == 2 ? f3[d] : -f1[d] - f3[d]);
+      if (EN && role == 0) {
+        ea += ka * (theta - t0) * (theta - t0);
+        v[0] += f1[0] * r12[0] + f3[0] * r32[0];
+        v[1] += f1[0] * r12[1] + f3[0] * r32[1];
+        v[2] += f1[0] * r12[2] + f3[0] * r32[2];
+        v[3] += f1[1] * r12[1] + f3[1] * r32[1];
+        v[4] += f1[1] * r12[2] + f3[1] * r32[2];
+        v[5] += f1[2] * r12[2] + f3[2] * r32[2];
+      }
+    } else {
+      const int a1 = gp(S.corr)[3 * k], a2 = gp(S.corr)[3 * k + 1], cls = gp(S.corr)[3 * k + 2];
+      double d[3] = {X[a2] - X[a1], X[n + a2] - X[n + a1], X[2 * n + a2] - X[2 * n + a1]};
+      min_image_vec(S, d);
+      const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      double qq = gp(S.charge)[a1] * gp(S.charge)[a2];
+      if (cls == PM_EXCL_MODIFIED) qq *= 1.0 - (double)S.csf;
+      const double r = sqrt(r2), rr = 1.0 / r;
+      const double e = erf(P.dalpha * r) * rr;  // C-library erf, as the reference on Linux
+      const double fs = qq * (P.dtwo_alpha_pi * exp(-(P.dalpha * P.dalpha) * r2) - e) * rr * rr;
+      const double sg = role == 0 ? -1.0 : 1.0;
+      for (int q = 0; q < 3; ++q) f[q] += sg * (d[q] * fs);
+      if (EN && role == 0) {
+        ec += -qq * e;
+        v[0] += d[0] * fs * d[0];
+        v[1] += d[0] * fs * d[1];
+        v[2] += d[0] * fs * d[2];
+        v[3] += d[1] * fs * d[1];
+        v[4] += d[1] * fs * d[2];
+        v[5] += d[2] * fs * d[2];
+      }
+    }
+  }
+}
+
+template <bool EN>
+__global__ __launch_bounds__(64 * kLocWaves) void k_local(const SysDev *__restrict__ sys, const int *__restrict__ ap,
+                                                          int nsys, ForceDev P, int mode) {
+  __shared__ double part[kLocWaves][3][64];
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int g = blockIdx.x * 64 + lane;  // the atom, over the launch's systems
+  const int total = ap[nsys];
+  const bool ok = g < total;
+  int kind, slot, nslots;
+  loc_slot(wv, kind, slot, nslots);
+  double f[3] = {0.0, 0.0, 0.0};
+  double eb = 0.0, ea = 0.0, ec = 0.0, v[6] = {0, 0, 0, 0, 0, 0};
+  int s = wave_sys(ap, nsys, g, total);
+  if (s >= 0) {
+    const SysDev *__restrict__ S = sys + s;  // scalar loads of the record
+    if (ok) local_terms<EN>(*S, g - ap[s], P, kind, slot, nslots, f, eb, ea, ec, v);
+  } else {
+    s = ok ? find_sys(ap, nsys, g) : 0;
+    if (ok) local_terms<EN>(sys[s], g - ap[s], P, kind, slot, nslots, f, eb, ea, ec, v);
+  }
+  if (mode & kLocalForces) {
+    part[wv][0][lane] = f[0];
+    part[wv][1][lane] = f[1];
+    part[wv][2][lane] = f[2];
+    __syncthreads();
+    // every atom, zero for one without local terms; the slots in a fixed order
+    if (wv < 3 && ok) {
+      double t = 0.0;
+      for (int w = 0; w < kLocWaves; ++w) t += part[w][wv][lane];
+      const SysDev &S = sys[s];
+      gp(S.fl)[(size_t)wv * S.n_atoms + (g - ap[s])] = to_fixed(t, kFScale);
+    }
+  }
+  if (!EN) return;
+  // bonded and correction terms add to the atomic virial only when it is on
+  // (BondSystemForce.h:121, AngleSystemForce.h:155, NonbondedFullEwaldSystemForce.h:1125)
+  if (!P.virial)
+    for (int k = 0; k < 6; ++k) v[k] = 0.0;
+  // each lane's terms go to fixed point first; integer sums are exact, so the
+  // totals do not depend on how lanes are grouped
+  long long q[9] = {to_fixed(eb, kEScale), to_fixed(ea, kEScale), to_fixed(ec, kEScale),
+                    to_fixed(v[0], kEScale), to_fixed(v[1], kEScale), to_fixed(v[2], kEScale),
+                    to_fixed(v[3], kEScale), to_fixed(v[4], kEScale), to_fixed(v[5], kEScale)};
+  const int slots[9] = {kAccBond, kAccAngle, kAccCoul, kAccVxx, kAccVxy, kAccVxz, kAccVyy, kAccVyz, kAccVzz};
+  const int s0 = __shfl(s, 0);
+  const bool mixed = wave_ballot(ok && s != s0) != 0;
+  const int wid = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (!mixed) {
+    for (int k = 0; k < 9; ++k) q[k] = wave_sum_ll(q[k]);
+    if (lane == 0)
+      for (int k = 0; k < 9; ++k)
+        if (q[k]) atomic_add_ll(&acc_copy(sys[s0].acc, wid)[slots[k]], q[k]);
+  } else if (ok) {
+    for (int k = 0; k < 9; ++k)
+      if (q[k]) atomic_add_ll(&acc_copy(sys[s].acc, wid)[slots[k]], q[k]);
+  }
+}
+
+void launch_local(const Launch &L, const ForceDev &P, int mode) {
+  if (!L.n_atoms_total || !L.n_local_total) return;
+  const dim3 g(nblk(L.n_atoms_total, 64)), b(64 * kLocWaves);
+  if (mode & kLocalEnergies)
+    hipLaunchKernelGGL(k_local<true>, g, b, 0, L.stream, L.sys, L.atom_prefix, L.nsys, P, mode);
+  else
+    hipLaunchKernelGGL(k_local<false>, g, b, 0, L.stream, L.sys, L.atom_prefix, L.nsys, P, mode);
+}
+
+}  // namespace pm

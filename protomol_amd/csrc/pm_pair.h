@@ -432,13 +432,16 @@ __device__ __forceinline__ void pair_wave(const SysDev *__restrict__ S, const in
       }
       float ea, eb;
       const float fs = pair_terms<TA, TB, SWA, SWB, SAME, PRE>(P, r2, qq, A, B, A12, B6, inA, inB, ea, eb);
+      // fused multiply-adds, written out: every kind of the kernel rounds the
+      // forces alike (the energy kinds' virial products do not change them),
+      // so a step that also sums energies moves the atoms as one that does not
+      fi[c][0] = fmaf(-fs, dx, fi[c][0]);
+      fi[c][1] = fmaf(-fs, dy, fi[c][1]);
+      fi[c][2] = fmaf(-fs, dz, fi[c][2]);
+      fjx = fmaf(fs, dx, fjx);
+      fjy = fmaf(fs, dy, fjy);
+      fjz = fmaf(fs, dz, fjz);
       const float fx = fs * dx, fy = fs * dy, fz = fs * dz;
-      fi[c][0] -= fx;
-      fi[c][1] -= fy;
-      fi[c][2] -= fz;
-      fjx += fx;
-      fjy += fy;
-      fjz += fz;
       if (EN) {
         eA += (double)ea;
         eB += (double)eb;

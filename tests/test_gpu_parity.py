@@ -626,6 +626,34 @@ def test_pool_admission_by_footprint():
         assert np.array_equal(f1[0], f2) and np.array_equal(e1[0], e2)
 
 
+@pytest.mark.parametrize("case", ["c2c2_ewald", "c2c2_two", "cncn", "c2c2_molvir", "c2c2_recip"])
+def test_energy_steps_move_atoms_alike(case):
+    """The pair, local and reciprocal kernels give the same force bits whether
+    or not they also sum energies: 30 MD steps that sum energies and virial on
+    every step end at the same positions, velocities and forces, bit for bit,
+    as 30 steps that never sum them."""
+    print(f"case energy steps {case}", flush=True)
+    w = _water_set((700,), 210)[0]
+    model = "c2c2" if case.startswith("c2c2_") else case
+    ta, sa, tb, sb = MODELS[model]
+    flags = L.FLAG_BONDED | L.FLAG_VIRIAL
+    if case == "c2c2_molvir":
+        flags |= L.FLAG_MOLVIRIAL
+    if case == "c2c2_recip":
+        flags |= L.FLAG_EWALD_RECIPROCAL
+    ewald = case in ("c2c2_ewald", "c2c2_molvir", "c2c2_recip")
+    v0 = np.random.default_rng(5).normal(scale=0.005, size=w.x.shape)
+    out = []
+    for every in (1, 0):
+        with L.Context(force_desc(ta, sa, tb, sb, flags=flags, skin=1.0, interval=10, ewald=ewald), [w]) as ctx:
+            ctx.md_set_state(w.x, v0)
+            ctx.md_run(30, 1.0, energies=every > 0, every=every)
+            out.append(ctx.md_get_state())
+    (x1, v1, f1, e1), (x0, v0_, f0, _) = out
+    assert np.array_equal(x1, x0) and np.array_equal(v1, v0_) and np.array_equal(f1, f0)
+    assert np.isfinite(e1).all() and e1[0, 1] != 0.0
+
+
 def test_pool_add_many_equals_single_adds():
     """pm_system_add_many places systems as pm_system_add one at a time does:
     the same ids, PM_EFULL as None (none tried after max_full of those), the
