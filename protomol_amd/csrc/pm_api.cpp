@@ -1231,15 +1231,17 @@ LaunchSet &get_set(pm_ctx *c, const std::vector<int> &ids) {
                            h.n_gb_pairs};
     for (int q = 0; q < kPre; ++q) pre[q].push_back(pre[q].back() + cnt[q]);
   }
-  // one allocation: [SysDev x ns][kPre prefix arrays of ns + 1][kScalN int64 x ns]
+  // one allocation: [SysDev x ns][kPre prefix arrays of ns + 1][kScalN int64 x ns][rebuild flag]
   const size_t o_pre = align_up((size_t)ns * sizeof(SysDev));
   const size_t o_scal = align_up(o_pre + kPre * (size_t)(ns + 1) * sizeof(int));
-  const size_t bytes = o_scal + (size_t)ns * pm::kScalN * sizeof(long long);
+  const size_t o_flag = align_up(o_scal + (size_t)ns * pm::kScalN * sizeof(long long));
+  const size_t bytes = o_flag + sizeof(int);
   HIPCHECK(hipMalloc(&ls->mem, bytes));
   char *base = static_cast<char *>(ls->mem);
   SysDev *d_tab = reinterpret_cast<SysDev *>(base);
   int *d_pre = reinterpret_cast<int *>(base + o_pre);
   ls->d_scal = reinterpret_cast<long long *>(base + o_scal);
+  HIPCHECK(hipMemset(base + o_flag, 0, sizeof(int)));
   HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&ls->h_scal), (size_t)ns * pm::kScalN * sizeof(long long), 0));
   std::vector<int> flat;
   for (auto &v : pre) flat.insert(flat.end(), v.begin(), v.end());
@@ -1259,6 +1261,7 @@ LaunchSet &get_set(pm_ctx *c, const std::vector<int> &ids) {
   L.gb_prefix = d_pre + 6 * st;
   L.gbp_prefix = d_pre + 9 * st;
   L.sc_prefix = d_pre + 8 * st;
+  L.rb_any = reinterpret_cast<int *>(base + o_flag);
   L.n_atoms_total = pre[0].back();
   L.n_cl_total = pre[1].back();
   L.n_cols_total = pre[2].back();
@@ -1347,7 +1350,10 @@ void force_eval(pm_ctx *c, const Launch &L, int mode, const StepPolicy &pol, int
     pm::launch_local(L, c->P, local);
     return;
   }
-  pm::launch_rebuild(L, pol.rlist * pol.rlist, mode, pol.lim2, pol.kmax);
+  // a forces-only pass computes the local terms' forces in the list check's
+  // launch (they read the same positions)
+  const bool local_done =
+      pm::launch_rebuild(L, pol.rlist * pol.rlist, mode, pol.lim2, pol.kmax, kind == pm::kPairForces ? &c->P : nullptr);
   if (timed) HIPCHECK(hipEventRecord(c->evp0, L.stream));
   if (pool) HIPCHECK(hipEventRecord(next_event(c), L.stream));
   if (!pm::launch_pair(L, c->P, en && c->molvir ? pm::kPairForcesEnergiesMV : kind))
@@ -1359,7 +1365,7 @@ void force_eval(pm_ctx *c, const Launch &L, int mode, const StepPolicy &pol, int
     pm::launch_ewald(L, pm::kEwForces | (en ? ewald_energy_mode(c) : 0));
     if (timed) HIPCHECK(hipEventRecord(c->evr1, L.stream));
   }
-  pm::launch_local(L, c->P, local);
+  if (!local_done) pm::launch_local(L, c->P, local);
 }
 
 // The pair-kernel kind of an MD step's force pass: a step that sums energies
@@ -1812,6 +1818,7 @@ void setup_force(pm_ctx *c, const pm_force_desc *force, int32_t n_systems, const
   ForceDev &P = c->P;
   std::memset(&P, 0, sizeof P);
   P.flags = force->flags;
+  P.finish_step = 1;
   P.pair_waves = force->pair_waves ? force->pair_waves : 4;
   P.pair_split = force->pair_split ? force->pair_split : pm::kPairSplitDefault;
   if (P.pair_split < 1 || P.pair_split > 64) throw Fail{PM_EINVAL, "pair_split must be 1 .. 64"};
@@ -1878,6 +1885,10 @@ void setup_force(pm_ctx *c, const pm_force_desc *force, int32_t n_systems, const
     P.cutB2 = (float)cB2;
     const double cmax = std::max(cA2, cB2);
     P.guard = (float)(2e-3 + 1e-5 * cmax);
+    P.cutA2lo = (float)(cA2 - P.guard);
+    P.cutA2hi = (float)(cA2 + P.guard);
+    P.cutB2lo = (float)(cB2 - P.guard);
+    P.cutB2hi = (float)(cB2 + P.guard);
     c->rc = std::sqrt(cmax);
     c->skin_md = force->skin > 0 ? force->skin : 2.0;
     c->skin_eval = 0.1;
@@ -2266,7 +2277,9 @@ int pm_pair_census(pm_ctx *c, int32_t system, const double *x, uint64_t *count, 
     require_md(c, {system});
     LaunchSet &ls = get_set(c, {system});
     HIPCHECK(hipMemsetAsync(c->sys[system]->dev.census, 0, 2 * sizeof(unsigned long long), c->stream));
-    if (!pm::launch_pair(ls.L, c->P, pm::kPairCensus)) throw Fail{PM_EINVAL, "unsupported term combination"};
+    pm::ForceDev P = c->P;
+    P.finish_step = 0;  // not a force pass: the step counters stay as they are
+    if (!pm::launch_pair(ls.L, P, pm::kPairCensus)) throw Fail{PM_EINVAL, "unsupported term combination"};
     read_scalars(c, ls);
     *count = (uint64_t)ls.h_scal[pm::kScalCensus];
     *hash = (uint64_t)ls.h_scal[pm::kScalCensus + 1];

@@ -67,8 +67,7 @@ enum { kXferX = 1, kXferV = 2, kXferF = 4 };
 enum { kScalCensus = 16, kScalErr = 18, kScalCtl = 19, kScalMaxdispRun = 25, kScalN = 26 };
 // what a pair-kernel launch accumulates; the MV kinds also sum the molecular
 // virial (and the atomic one as the reference does with it on)
-enum { kPairForces = 0, kPairForcesEnergies = 1, kPairEnergies = 2, kPairCensus = 3, kPairForcesEnergiesMV = 4,
-       kPairEnergiesMV = 5 };
+enum { kPairForces = 0, kPairForcesEnergies = 1, kPairCensus = 3, kPairForcesEnergiesMV = 4 };
 
 // per-system control words: [0] rebuild the pair list in this step (0/1),
 // [1] steps since the last rebuild, [2] rebuilds since the start of the run,
@@ -100,6 +99,12 @@ struct ForceDev {
   float cutA2, cutB2;      // inclusion cutoff^2 per term (FP32 decision)
   double cutA2d, cutB2d;   // the same in FP64 (guard-band recheck)
   float guard;             // |r2 - cut2| <= guard  ->  FP64 recheck
+  // the band's edges, cut2 -/+ guard per term: r2 <= lo is inside, r2 > hi
+  // outside, anything between goes to the FP64 recheck
+  float cutA2lo, cutA2hi, cutB2lo, cutB2hi;
+  // the pair kernel finishes the force pass's step bookkeeping (step_finish_ctl);
+  // 0 for a pair launch outside a force pass (the census of the MD state's list)
+  int finish_step;
   // Ewald real space: alpha, 0.3275911 alpha (the A&S 7.1.26 t argument),
   // -alpha^2 log2(e) (exp2 argument per r^2), 2 alpha / sqrt(pi)
   float alpha, alpha_p, alpha2_log2e, two_alpha_pi;

@@ -19,6 +19,10 @@ struct Launch {
   const struct EwLaunch *ew = nullptr;  // per system with a reciprocal-space term (host array)
   int n_ew = 0;
   int max_mol = 0;                      // most molecules of a system (molecular virial; 0 = off)
+  // one device int: some system of the launch rebuilds its list this force
+  // pass (raised by the list check, cleared by the kick); null: the rebuild
+  // kernels check each system's own decision only
+  int *rb_any = nullptr;
 };
 
 // Host-side launch shape of one system's reciprocal-space passes
@@ -37,8 +41,13 @@ void launch_export(const Launch &L, int what);
 void launch_gather_scalars(const Launch &L, long long *out);
 void launch_clear_err(const Launch &L);
 // the list check of a force pass (the bookkeeping of launch_step_begin with
-// the same mode, lim2, kmax) and, where it decides so, the pair-list rebuild
-void launch_rebuild(const Launch &L, float rlist2, int mode, float lim2, int kmax);
+// the same mode, lim2, kmax) and, where it decides so, the pair-list rebuild.
+// local (forces-only passes): its first kernel also runs the local terms'
+// forces, as launch_local(L, *local, kLocalForces) would; returns whether it
+// did
+bool launch_rebuild(const Launch &L, float rlist2, int mode, float lim2, int kmax, const ForceDev *local = nullptr);
+// k_bin with the local terms' forces in the same launch (pm_local.hip)
+void launch_bin_local(const Launch &L, const ForceDev &P, int mode, float lim2, int kmax);
 bool launch_pair(const Launch &L, const ForceDev &P, int kind);
 // the pair kernels of one term combination each (pm_pair_*.hip)
 void launch_pair_lj(const Launch &L, const ForceDev &P, int kind);

@@ -16,6 +16,7 @@
 
 #include "../../include/protomol_amd.h"
 #include "pm_device.h"
+#include "pm_rebuild.h"
 
 namespace pm {
 
@@ -32,15 +33,6 @@ __global__ void k_import(SysDev *sys, const int *ap, int nsys, int what) {
     if (what & kXferX) S.x[d * n + a] = S.xaos[3 * (size_t)a + d];
     if (what & kXferV) S.v[d * n + a] = S.vaos[3 * (size_t)a + d];
   }
-}
-
-// The list check of a force pass (the Verlet criterion: rebuild when an atom
-// moved more than lim from where the list was built, or every kmax steps, or
-// always with kStepForce), from values no kernel of the rebuild changes
-__device__ __forceinline__ bool rebuild_due(const SysDev &S, int mode, float lim2, int kmax) {
-  if (mode & kStepForce) return true;
-  if (!(mode & kStepAuto)) return false;
-  return __uint_as_float(*S.maxdisp2) > lim2 || (kmax > 0 && S.ctl[kCtlSince] + 1 >= kmax);
 }
 
 // one system's step bookkeeping (one thread): counters, the census, and the
@@ -65,107 +57,20 @@ __device__ __forceinline__ void step_begin_ctl(const SysDev &S, int mode, float 
   }
 }
 
-// the rest of a step's bookkeeping once k_bin has published the decision in
-// ctl[kCtlRebuild] (one thread per system)
-__device__ __forceinline__ void step_finish_ctl(const SysDev &S) {
-  int *ctl = S.ctl;
-  ctl[kCtlFirst] = 0;  // the step's kick has run
-  ctl[kCtlNoise] += 1;
-  const unsigned d2 = *S.maxdisp2;
-  *S.maxdisp_run = max(*S.maxdisp_run, d2);
-  *S.maxdisp2 = 0u;
-  if (ctl[kCtlRebuild]) {
-    ctl[kCtlSince] = 0;
-    ctl[kCtlCount] += 1;
-    ctl[kCtlEntries] = 0;
-  } else {
-    ctl[kCtlSince] += 1;
-  }
-}
-
-// Rebuild, step 1 of 3, one thread per atom.  It also starts the force pass:
-// every system's thread of atom 0 publishes the list check's decision and
-// clears the census, and the first threads clear the energy accumulators.
-// When the list is rebuilt, the atom's (z, index) key goes to its x-y
-// column's bucket.  Lanes of a wave that share a (system, column) key are
-// served by one atomic: the wave peels off the group of the first remaining
-// lane until every lane is done.  Atoms in original order are spatially
-// coherent, so a wave usually holds only a few keys.
-// one atom of k_bin: the force pass's start and, when the list is rebuilt,
-// the atom's column and sort key; returns whether it has a key to place
-__device__ __forceinline__ bool bin_atom(const SysDev &S, int a, int mode, float lim2, int kmax, int &col,
-                                         unsigned long long &key) {
-  const int n = S.n_atoms;
-  const bool rb = rebuild_due(S, mode, lim2, kmax);
-  for (int k = a; k < kAccN * kAccCopies; k += n) S.acc[k] = 0;
-  if (a == 0) {
-    S.ctl[kCtlRebuild] = rb ? 1 : 0;
-    S.census[0] = 0;
-    S.census[1] = 0;
-  }
-  if (!rb) return false;
-  double x = S.x[a], y = S.x[n + a], z = S.x[2 * n + a];
-  if (!(isfinite(x) && isfinite(y) && isfinite(z))) {
-    atomicOr(S.err, (unsigned)kErrNonFinite);
-    x = y = z = 0.0;
-  }
-  if (S.vac && (x < S.vac_lo[0] || x >= S.vac_hi[0] || y < S.vac_lo[1] || y >= S.vac_hi[1] ||
-                z < S.vac_lo[2] || z >= S.vac_hi[2]))
-    atomicOr(S.err, (unsigned)kErrVacuumRegion);
-  const double rx = wrap_rel(x, S.omin[0], S.L[0]);
-  const double ry = wrap_rel(y, S.omin[1], S.L[1]);
-  const double rz = wrap_rel(z, S.omin[2], S.L[2]);
-  int cx = (int)(rx / S.ax), cy = (int)(ry / S.ay);
-  cx = cx < 0 ? 0 : (cx >= S.ncx ? S.ncx - 1 : cx);
-  cy = cy < 0 ? 0 : (cy >= S.ncy ? S.ncy - 1 : cy);
-  col = cy * S.ncx + cx;
-  const unsigned zb = __float_as_uint((float)rz);  // z >= 0: bit order == value order
-  key = ((unsigned long long)zb << 32) | (unsigned)a;
-  return true;
-}
-
 __global__ void k_bin(const SysDev *__restrict__ sys, const int *__restrict__ ap, int nsys, int mode, float lim2,
-                      int kmax) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  const int total = ap[nsys];
-  bool todo = false;
-  int s = wave_sys(ap, nsys, g, total), col = 0;
-  unsigned long long key = 0;
-  if (s >= 0) {
-    const SysDev *__restrict__ S = sys + s;  // scalar loads of the record
-    if (g < total) todo = bin_atom(*S, g - ap[s], mode, lim2, kmax, col, key);
-  } else {
-    s = g < total ? find_sys(ap, nsys, g) : 0;
-    if (g < total) todo = bin_atom(sys[s], g - ap[s], mode, lim2, kmax, col, key);
-  }
-  // one atomic per (system, column) group of the wave
-  while (true) {
-    const unsigned long long rem = wave_ballot(todo);
-    if (rem == 0) break;
-    const int leader = __ffsll((long long)rem) - 1;
-    const int ls = __shfl(s, leader), lc = __shfl(col, leader);
-    const bool mine = todo && s == ls && col == lc;
-    const unsigned long long grp = wave_ballot(mine);
-    int base = 0;
-    if (lane == leader) base = atomicAdd(&sys[ls].col_count[lc], __popcll(grp));
-    base = __shfl(base, leader);
-    if (mine) {
-      const int pos = base + __popcll(grp & ((1ull << lane) - 1ull));
-      if (pos < kSortCap) sys[s].bucket[(size_t)col * kSortCap + pos] = key;
-      else atomicOr(sys[s].err, (unsigned)kErrColumnCap);
-    }
-    todo = todo && !mine;
-  }
+                      int kmax, int *rb_any) {
+  bin_thread(sys, ap, nsys, mode, lim2, kmax, blockIdx.x * blockDim.x + threadIdx.x, rb_any);
 }
 
 // Rebuild, step 2 of 3, one block per column: the column's first atom,
 // cluster and super-cluster from the counts of the columns before it, a
 // bitonic sort of its bucket by (z, atom), slot assignment, and each
 // cluster's geometry (8 lanes per cluster): bounding box, origin, the atoms'
-// FP32 local coordinates.  The block of each system's column 0 finishes the
-// step's bookkeeping.
-__global__ __launch_bounds__(1024) void k_col_sort(const SysDev *__restrict__ sys, const int *__restrict__ cp, int nsys) {
+// FP32 local coordinates.  rb_any (when given): no system of the launch
+// rebuilds, every wave returns after one load.
+__global__ __launch_bounds__(1024) void k_col_sort(const SysDev *__restrict__ sys, const int *__restrict__ cp, int nsys,
+                                                   const int *__restrict__ rb_any) {
+  if (rb_any && !*rb_any) return;
   __shared__ unsigned long long keys[kSortCap];
   __shared__ int red[3][1024 / kWave];
   const int gcol = blockIdx.x;
@@ -174,7 +79,6 @@ __global__ __launch_bounds__(1024) void k_col_sort(const SysDev *__restrict__ sy
   const SysDev &S = sys[s];
   const int col = gcol - cp[s];
   const int t = threadIdx.x, nt = blockDim.x, lane = t & 63, wv = t >> 6;
-  if (col == 0 && t == 0) step_finish_ctl(S);
   if (!S.ctl[kCtlRebuild]) return;
   // clusters and super-clusters of the columns before this one
   int lc = 0, lsc = 0;
@@ -346,7 +250,8 @@ __device__ __forceinline__ int lower_bound_lo_z(const float4 *lo_b, int c0, int 
 }
 
 __global__ __launch_bounds__(kListThreads) void k_pairlist(const SysDev *__restrict__ sys, const int *__restrict__ scp,
-                                                           int nsys, float rlist2) {
+                                                           int nsys, float rlist2, const int *__restrict__ rb_any) {
+  if (rb_any && !*rb_any) return;
   __shared__ float4 ib_lo[kListWaves][kScCl], ib_hi[kListWaves][kScCl];
   __shared__ int seg_start[kListWaves][kSegCap + 1], seg_cl[kListWaves][kSegCap], seg_sh[kListWaves][kSegCap];
   __shared__ ListScratch scratch[kListWaves];
@@ -573,21 +478,6 @@ __global__ __launch_bounds__(kListThreads) void k_pairlist(const SysDev *__restr
   if (lane == 0 && pairs8) atomicAdd(&S.ctl[kCtlEntries], (int)pairs8);
 }
 
-// ------------------------------------------------------- FP64 side terms
-
-
-
-// Bonded terms and Ewald excluded-pair corrections, one thread per atom
-// ("owner computes"): the thread walks the terms its atom takes part in and
-// evaluates each in FP64 from that atom's side.  A term is evaluated once per
-// participant with the same expressions, so the forces on its atoms are
-// exactly opposite, and each atom's slot is updated by one thread only -- no
-// atomics, and the result does not depend on which other systems share the
-// launch.  Energies and virial (mode & kLocalEnergies) are counted by the
-// term's first atom.
-//   BondSystemForce::calcBond (force/bonded/BondSystemForce.h:77-108)
-//   AngleSystemForce::calcAngle (force/bonded/AngleSystemForce.h:66-140)
-//   NonbondedFullEwaldSystemForce::correctionTerm (NonbondedFullEwaldSystemForce.h:1097-1131)
 // ------------------------------------------------------------ integrator
 
 // First step of a run (the system's kCtlFirst word is set):
@@ -691,8 +581,9 @@ __device__ __forceinline__ float kick_drift_atom(const SysDev &S, int a, double 
 }
 
 __global__ __launch_bounds__(256) void k_kick_drift(const SysDev *__restrict__ sys, const int *__restrict__ ap,
-                                                    int nsys, double h, LangevinDev lv) {
+                                                    int nsys, double h, LangevinDev lv, int *rb_any) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g == 0 && rb_any) *rb_any = 0;  // the next list check raises it again
   const int total = ap[nsys];
   const bool ok = g < total;
   float d2 = 0.f;
@@ -849,16 +740,20 @@ void launch_clear_err(const Launch &L) {
   hipLaunchKernelGGL(k_clear_err, dim3(nblk(L.nsys, 64)), dim3(64), 0, L.stream, L.sys, L.nsys);
 }
 
-void launch_rebuild(const Launch &L, float rlist2, int mode, float lim2, int kmax) {
+bool launch_rebuild(const Launch &L, float rlist2, int mode, float lim2, int kmax, const ForceDev *local) {
   if (!L.n_atoms_total) {
     launch_step_begin(L, mode, lim2, kmax);
-    return;
+    return false;
   }
-  hipLaunchKernelGGL(k_bin, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys,
-                     mode, lim2, kmax);
-  hipLaunchKernelGGL(k_col_sort, dim3(L.n_cols_total), dim3(1024), 0, L.stream, L.sys, L.col_prefix, L.nsys);
+  if (local && L.n_local_total)
+    launch_bin_local(L, *local, mode, lim2, kmax);
+  else
+    hipLaunchKernelGGL(k_bin, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys,
+                       mode, lim2, kmax, L.rb_any);
+  hipLaunchKernelGGL(k_col_sort, dim3(L.n_cols_total), dim3(1024), 0, L.stream, L.sys, L.col_prefix, L.nsys, L.rb_any);
   hipLaunchKernelGGL(k_pairlist, dim3(nblk(L.n_sc_total, kListWaves)), dim3(kListThreads), 0, L.stream, L.sys,
-                     L.sc_prefix, L.nsys, rlist2);
+                     L.sc_prefix, L.nsys, rlist2, L.rb_any);
+  return local && L.n_local_total;
 }
 
 bool launch_pair(const Launch &L, const ForceDev &P, int kind) {
@@ -874,7 +769,9 @@ bool launch_pair(const Launch &L, const ForceDev &P, int kind) {
 }
 
 void launch_kick_drift(const Launch &L, double h, const LangevinDev &lv) {
-  if (L.n_atoms_total) hipLaunchKernelGGL(k_kick_drift, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys, h, lv);
+  if (L.n_atoms_total)
+    hipLaunchKernelGGL(k_kick_drift, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix,
+                       L.nsys, h, lv, L.rb_any);
 }
 void launch_half_kick(const Launch &L, double hh, const LangevinDev &lv) {
   if (L.n_atoms_total) hipLaunchKernelGGL(k_half_kick, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys, hh, lv);

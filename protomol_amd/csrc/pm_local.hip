@@ -7,6 +7,7 @@
 #include <math.h>
 
 #include "pm_device.h"
+#include "pm_rebuild.h"
 
 namespace pm {
 
@@ -122,10 +123,11 @@ __device__ __forceinline__ void local_terms(const SysDev &S, int a, const ForceD
   }
 }
 
+// One block of k_local: the 64 atoms from blockIdx.x * 64 over the launch's
+// systems, wave w taking share w (every thread of the block calls it)
 template <bool EN>
-__global__ __launch_bounds__(64 * kLocWaves) void k_local(const SysDev *__restrict__ sys, const int *__restrict__ ap,
-                                                          int nsys, ForceDev P, int mode) {
-  __shared__ double part[kLocWaves][3][64];
+__device__ __forceinline__ void local_block(const SysDev *__restrict__ sys, const int *__restrict__ ap, int nsys,
+                                            const ForceDev &P, int mode, double (*part)[3][64]) {
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int g = blockIdx.x * 64 + lane;  // the atom, over the launch's systems
@@ -179,6 +181,31 @@ __global__ __launch_bounds__(64 * kLocWaves) void k_local(const SysDev *__restri
     for (int k = 0; k < 9; ++k)
       if (q[k]) atomic_add_ll(&acc_copy(sys[s].acc, wid)[slots[k]], q[k]);
   }
+}
+
+template <bool EN>
+__global__ __launch_bounds__(64 * kLocWaves) void k_local(const SysDev *__restrict__ sys, const int *__restrict__ ap,
+                                                          int nsys, ForceDev P, int mode) {
+  __shared__ double part[kLocWaves][3][64];
+  local_block<EN>(sys, ap, nsys, P, mode, part);
+}
+
+// k_bin and k_local (forces) in one launch, for a forces-only pass: the list
+// check and binning of a block's 64 atoms ride on its last wave, before that
+// wave's share of the local terms (both read the positions of the pass, and
+// neither writes what the other reads)
+__global__ __launch_bounds__(64 * kLocWaves) void k_bin_local(const SysDev *__restrict__ sys,
+                                                              const int *__restrict__ ap, int nsys, ForceDev P,
+                                                              int mode, float lim2, int kmax, int *rb_any) {
+  __shared__ double part[kLocWaves][3][64];
+  if ((threadIdx.x >> 6) == kLocWaves - 1)
+    bin_thread(sys, ap, nsys, mode, lim2, kmax, blockIdx.x * 64 + (threadIdx.x & 63), rb_any);
+  local_block<false>(sys, ap, nsys, P, kLocalForces, part);
+}
+
+void launch_bin_local(const Launch &L, const ForceDev &P, int mode, float lim2, int kmax) {
+  hipLaunchKernelGGL(k_bin_local, dim3(nblk(L.n_atoms_total, 64)), dim3(64 * kLocWaves), 0, L.stream, L.sys,
+                     L.atom_prefix, L.nsys, P, mode, lim2, kmax, L.rb_any);
 }
 
 void launch_local(const Launch &L, const ForceDev &P, int mode) {

@@ -3,6 +3,7 @@
 // combination, so the instantiations compile in parallel.
 #pragma once
 #include "pm_device.h"
+#include "pm_rebuild.h"
 
 namespace pm {
 
@@ -13,29 +14,22 @@ constexpr int kSwGeneric = -1;
 
 // OUTER: the switch's cutoff may be shorter than the pair's inclusion cutoff
 // (OneAtomPairTwo with two different cutoffs), so S = 0 beyond it.
-// PRE: E and F come pre-scaled, E' = 6 a1 E and F' = a1 F (the LJ rows of the
-// table in LDS are staged that way); E returns unscaled.
-template <int SW, bool OUTER = true, bool PRE = false>
+// C2SwitchingFunction.h:42-53: S = s1 c2 c4 and dS = s3 (c2^2 - c4),
+// c2 = cut^2 - r^2, c4 = c2 (s2 + 2 r^2), s2 = cut^2 - 3 on^2, s3 = 4 s1 =
+// 4 / D^3, D = cut^2 - on^2.  Factored: with w = r^2 - on^2,
+// S = a1 c2^2 (w + D/2) and dS = -6 a1 c2 w (a1 = 2 s1), so
+// F S - E dS = a1 c2 (c2 (w + D/2) F + 6 w E).  Inside r_on, w is clamped to
+// 0 (an integer max of the bits: no canonicalizing float max), where S = 1
+// and dS = 0 to FP32 rounding: the unswitched values.
+template <int SW, bool OUTER = true>
 __device__ __forceinline__ void apply_switch_t(const SwitchDev &sw, float r2, float &E, float &F) {
-  if (SW == PM_SWITCH_C2) {  // C2SwitchingFunction.h:42-53
-    // S = s1 c2 c4 and dS = s3 (c2^2 - c4), c2 = cut^2 - r^2, c4 = c2 (s2 + 2 r^2),
-    // s2 = cut^2 - 3 on^2, s3 = 4 s1 = 4 / D^3, D = cut^2 - on^2.  Factored:
-    // with w = r^2 - on^2, S = a1 c2^2 (w + D/2) and dS = -6 a1 c2 w (a1 = 2 s1),
-    // so F S - E dS = a1 c2 (c2 (w + D/2) F + 6 w E).  Inside r_on, w is
-    // clamped to 0 (an integer max of the bits: no canonicalizing float max),
-    // where S = 1 and dS = 0 to FP32 rounding: the unswitched values.
+  if (SW == PM_SWITCH_C2) {
     const float w = __int_as_float(max(__float_as_int(r2 - sw.on2), 0));
     const float c2 = sw.c2d - w;
     const float cp = c2 * (w + sw.c2hd);
-    float Fs, Es;
-    if (PRE) {
-      Fs = c2 * fmaf(cp, F, w * E);
-      Es = (E * c2) * cp * (1.0f / 6.0f);
-    } else {
-      const float ac = sw.c2a1 * c2;
-      Fs = ac * fmaf(w * E, 6.0f, cp * F);
-      Es = ac * (cp * E);
-    }
+    const float ac = sw.c2a1 * c2;
+    float Fs = ac * fmaf(w * E, 6.0f, cp * F);
+    float Es = ac * (cp * E);
     if (OUTER) {
       const bool outer = r2 > sw.cut2;
       Fs = outer ? 0.0f : Fs;
@@ -48,6 +42,30 @@ __device__ __forceinline__ void apply_switch_t(const SwitchDev &sw, float r2, fl
   } else {
     apply_switch(sw, r2, E, F);
   }
+}
+
+// The LJ term through the factored C2 switch, with coefficients that come
+// pre-scaled (A', B', A12', B6') = (12 s1 A, 12 s1 B, 24 s1 A, 12 s1 B) (the
+// LJ rows of the table in LDS are staged that way), so the LJ energy enters the
+// switch as 6 a1 E and its force as a1 F.  With r^-6 taken out of both,
+// F S - E dS = c2 r^-6 (c2 (w + D/2) r^-2 (A12' r^-6 - B6') + w (A' r^-6 - B')).
+// E returns the switched energy E S.
+template <bool OUTER>
+__device__ __forceinline__ void lj_c2_pre(const SwitchDev &sw, float r2, float r2inv, float r6, float A, float B,
+                                          float A12, float B6, float &E, float &F) {
+  const float pe = fmaf(A, r6, -B), pg = fmaf(A12, r6, -B6);
+  const float w = __int_as_float(max(__float_as_int(r2 - sw.on2), 0));
+  const float c2 = sw.c2d - w;
+  const float cp = c2 * (w + sw.c2hd);
+  float Fs = (c2 * r6) * fmaf(cp * r2inv, pg, w * pe);
+  float Es = ((r6 * pe) * c2) * cp * (1.0f / 6.0f);
+  if (OUTER) {
+    const bool outer = r2 > sw.cut2;
+    Fs = outer ? 0.0f : Fs;
+    Es = outer ? 0.0f : Es;
+  }
+  F = Fs;
+  E = Es;
 }
 
 // sum over 8 consecutive lanes with DPP (quad swaps, then half-row mirror);
@@ -148,9 +166,14 @@ __device__ __forceinline__ float pair_terms(const ForceDev &P, float r2, float q
     // LennardJonesForce::operator() (LennardJonesForce.h:45-58):
     // E = A r^-12 - B r^-6 = r^-6 (A r^-6 - B), F r = r^-6 (12 A r^-6 - 6 B)
     const float r6 = r2inv * r2inv * r2inv;
-    float E = r6 * fmaf(A, r6, -B);
-    float F = r6 * fmaf(A12, r6, -B6) * r2inv;
-    apply_switch_t<SWA, !SAME, PRE>(P.sw_a, r2, E, F);
+    float E, F;
+    if (PRE) {
+      lj_c2_pre<!SAME>(P.sw_a, r2, r2inv, r6, A, B, A12, B6, E, F);
+    } else {
+      E = r6 * fmaf(A, r6, -B);
+      F = r6 * fmaf(A12, r6, -B6) * r2inv;
+      apply_switch_t<SWA, !SAME>(P.sw_a, r2, E, F);
+    }
     fs = inA ? F : 0.f;
     ea = inA ? E : 0.f;
   } else if (TA == PM_TERM_COULOMB) {
@@ -217,9 +240,8 @@ constexpr int kLjLds = 0, kLjGlobal = 1, kLjPerAtom = 2;
 template <int TA, int TB, int SWA, int SWB, int KIND, bool SAME, int LJM>
 __device__ __forceinline__ void pair_wave(const SysDev *__restrict__ S, const int sc, const int part, const int split,
                                           const int w, const ForceDev &P) {
-  constexpr bool MV = KIND == kPairForcesEnergiesMV || KIND == kPairEnergiesMV;
-  constexpr bool FORCES = KIND != kPairEnergies && KIND != kPairEnergiesMV;
-  constexpr bool EN = KIND == kPairForcesEnergies || KIND == kPairEnergies || MV;
+  constexpr bool MV = KIND == kPairForcesEnergiesMV;
+  constexpr bool EN = KIND == kPairForcesEnergies || MV;
   constexpr bool CEN = KIND == kPairCensus;
   constexpr bool peratom = LJM == kLjPerAtom;
   constexpr bool big_table = LJM == kLjGlobal;
@@ -350,10 +372,12 @@ __device__ __forceinline__ void pair_wave(const SysDev *__restrict__ S, const in
         am &= ~M[2 * c];
         mm = M[2 * c + 1];
       }
+      // two compares per cutoff: inside the band's lower edge is in, above
+      // its upper edge out, between them the FP64 recheck decides
       unsigned long long mA, mB;
       if (SAME) {
-        mA = lane_mask(r2 <= P.cutA2) & am;
-        const unsigned long long near = lane_mask(fabsf(r2 - P.cutA2) <= P.guard) & am;
+        mA = lane_mask(r2 <= P.cutA2lo) & am;
+        const unsigned long long near = lane_mask(r2 <= P.cutA2hi) & am & ~mA;
         if (__builtin_expect(near != 0, 0)) {
           bool in = lane_bit(mA);
           if (lane_bit(near)) in = min_image_r2(*S, lorig[c * kCluster + ii], orig[sj]) <= P.cutA2d;
@@ -361,11 +385,11 @@ __device__ __forceinline__ void pair_wave(const SysDev *__restrict__ S, const in
         }
         mB = mA;
       } else {
-        mA = TA != PM_TERM_NONE ? lane_mask(r2 <= P.cutA2) & am : 0ull;
-        mB = TB != PM_TERM_NONE ? lane_mask(r2 <= P.cutB2) & am : 0ull;
-        const bool nearA = (TA != PM_TERM_NONE) && fabsf(r2 - P.cutA2) <= P.guard;
-        const bool nearB = (TB != PM_TERM_NONE) && fabsf(r2 - P.cutB2) <= P.guard;
-        const unsigned long long near = lane_mask(nearA || nearB) & am;
+        mA = TA != PM_TERM_NONE ? lane_mask(r2 <= P.cutA2lo) & am : 0ull;
+        mB = TB != PM_TERM_NONE ? lane_mask(r2 <= P.cutB2lo) & am : 0ull;
+        const unsigned long long nearA = TA != PM_TERM_NONE ? lane_mask(r2 <= P.cutA2hi) & am & ~mA : 0ull;
+        const unsigned long long nearB = TB != PM_TERM_NONE ? lane_mask(r2 <= P.cutB2hi) & am & ~mB : 0ull;
+        const unsigned long long near = nearA | nearB;
         if (__builtin_expect(near != 0, 0)) {
           bool a = lane_bit(mA), b = lane_bit(mB);
           if (lane_bit(near)) {
@@ -488,14 +512,14 @@ __device__ __forceinline__ void pair_wave(const SysDev *__restrict__ S, const in
     }
     // Newton's third law: j force = sum over the i clusters and the 8 i lanes
     // (nothing to add when no pair of the entry was inside the cutoff)
-    if (FORCES && any) {
+    if (any) {
       const float g = dpp_sum8_scatter(fjx, fjy, fjz, ii);
       if (ii < 3 && jj < nvj) atomic_add_ll(&gfc[sj], to_fixed((double)g, kFScale));
     }
   }
   // i force = sum over the 8 j lanes that share ii; lane (jj, ii) then holds
   // atom ii of i cluster jj, which is real where valid_ca has its bit
-  if (FORCES) {
+  {
     float v[3 * kScCl];
 #pragma unroll
     for (int c = 0; c < kScCl; ++c) {
@@ -570,6 +594,8 @@ __global__ __launch_bounds__(512, (KIND == kPairForces ? PM_PAIR_MIN_WAVES : PM_
   const int s = __builtin_amdgcn_readfirstlane(find_sys(scp, nsys, scg));
   const SysDev *__restrict__ S = sys + s;
   const int sc = scg - scp[s];
+  // the step's bookkeeping, after the rebuild kernels have read it
+  if (P.finish_step && sc == 0 && part == 0 && (threadIdx.x & 63) == 0) step_finish_ctl(*S);
   if (sc >= *cp(S->n_sc)) return;
   if (TA != PM_TERM_LJ) {
     pair_wave<TA, TB, SWA, SWB, KIND, SAME, kLjLds>(S, sc, part, split, w, P);
@@ -597,15 +623,8 @@ static void launch_pair_t(const Launch &L, const ForceDev &P, int kind) {
   case kPairForcesEnergies:
     hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairForcesEnergies, SAME>), g, b, lds, L.stream, L.sys, L.sc_prefix, L.nsys, P);
     break;
-  case kPairEnergies:
-    hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairEnergies, SAME>), g, b, lds, L.stream, L.sys, L.sc_prefix, L.nsys, P);
-    break;
   case kPairForcesEnergiesMV:
     hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairForcesEnergiesMV, SAME>), g, b, lds, L.stream, L.sys, L.sc_prefix,
-                       L.nsys, P);
-    break;
-  case kPairEnergiesMV:
-    hipLaunchKernelGGL((k_pair<TA, TB, SWA, SWB, kPairEnergiesMV, SAME>), g, b, lds, L.stream, L.sys, L.sc_prefix,
                        L.nsys, P);
     break;
   default:
