@@ -126,6 +126,8 @@ struct Carver {
 struct SysPrep {
   int n = 0;
   std::vector<int> excl_start, excl_partner, excl_class, corr, loc_start, loc_item;
+  std::vector<pm::LocRec> loc_rec;  // rows of local-term records (SysDev::loc_rec)
+  int loc_rows[pm::kLocShares] = {}, loc_base[pm::kLocShares] = {};
   std::vector<unsigned long long> excl_win;  // per atom: FULL, MODIFIED partner bits (b - a + 31), far flag
   std::vector<double> invm;
   std::vector<float> sig_half, eps_sqrt;
@@ -589,6 +591,54 @@ SysPrep prepare(const pm_ctx *c, const pm_system_desc &d) {
       for (int r = 0; r < 3; ++r) p.loc_item[fill[3 * d.angles[3 * k + r] + pm::kLocAngle]++] = (k << 2) | r;
     for (int k = 0; k < p.n_corr; ++k)
       for (int r = 0; r < 2; ++r) p.loc_item[fill[3 * p.corr[3 * k + r] + pm::kLocCorr]++] = (k << 2) | r;
+    // the records: for share w, row loc_base[w] + k holds item slot + k nslots
+    // of the atom's list of that kind (role -1 where the atom has fewer)
+    int rows = 0;
+    for (int w = 0; w < pm::kLocShares; ++w) {
+      int kind, slot, ns;
+      pm::loc_share(w, kind, slot, ns);
+      int m = 0;
+      for (int a = 0; a < n; ++a) {
+        const int cnt = p.loc_start[3 * a + kind + 1] - p.loc_start[3 * a + kind];
+        m = std::max(m, cnt > slot ? (cnt - slot + ns - 1) / ns : 0);
+      }
+      p.loc_rows[w] = m;
+      p.loc_base[w] = rows;
+      rows += m;
+    }
+    const double mscale = 1.0 - (double)(float)d.coulomb_scaling;  // as the device's float csf
+    p.loc_rec.assign((size_t)rows * n, pm::LocRec{-1, -1, -1, -1, 0.0, 0.0});
+    for (int w = 0; w < pm::kLocShares; ++w) {
+      int kind, slot, ns;
+      pm::loc_share(w, kind, slot, ns);
+      for (int a = 0; a < n; ++a) {
+        const int beg = p.loc_start[3 * a + kind], end = p.loc_start[3 * a + kind + 1];
+        int k = 0;
+        for (int it = beg + slot; it < end; it += ns, ++k) {
+          const int item = p.loc_item[it], role = item & 3, t = item >> 2;
+          pm::LocRec &r = p.loc_rec[(size_t)(p.loc_base[w] + k) * n + a];
+          r.role = role;
+          if (kind == pm::kLocBond) {
+            r.a0 = d.bonds[2 * t];
+            r.a1 = d.bonds[2 * t + 1];
+            r.p0 = d.bond_k[t];
+            r.p1 = d.bond_r0[t];
+          } else if (kind == pm::kLocAngle) {
+            r.a0 = d.angles[3 * t];
+            r.a1 = d.angles[3 * t + 1];
+            r.a2 = d.angles[3 * t + 2];
+            r.p0 = d.angle_k[t];
+            r.p1 = d.angle_theta0[t];
+          } else {
+            r.a0 = p.corr[3 * t];
+            r.a1 = p.corr[3 * t + 1];
+            double qq = d.scaled_charge[r.a0] * d.scaled_charge[r.a1];
+            if (p.corr[3 * t + 2] == PM_EXCL_MODIFIED) qq *= mscale;
+            r.p0 = qq;
+          }
+        }
+      }
+    }
   }
   // per-atom data
   p.invm.resize(n);
@@ -713,9 +763,11 @@ size_t carve(const pm_ctx *c, const SysPrep &p, const pm_system_desc &d, Carver 
   S.excl_partner = cv.take<int>(p.excl_partner.size());
   S.excl_class = cv.take<int>(p.excl_class.size());
   S.excl_win = cv.take<unsigned long long>(p.excl_win.size());
-  S.corr = cv.take<int>(p.corr.size());
-  S.loc_start = cv.take<int>(p.loc_start.size());
-  S.loc_item = cv.take<int>(p.loc_item.size());
+  S.loc_rec = cv.take<pm::LocRec>(p.loc_rec.size());
+  for (int w = 0; w < pm::kLocShares; ++w) {
+    S.loc_rows[w] = p.loc_rows[w];
+    S.loc_base[w] = p.loc_base[w];
+  }
   S.charge = cv.take<double>(n);
   S.inv_mass = cv.take<double>(n);
   S.molecule = cv.take<int>(n);
@@ -727,12 +779,6 @@ size_t carve(const pm_ctx *c, const SysPrep &p, const pm_system_desc &d, Carver 
     S.lj_table = cv.take<float4>(p.lj.size());
     S.lj_table12 = cv.take<float4>(p.lj12.size());
   }
-  S.bonds = cv.take<int>(2 * (size_t)p.n_bonds);
-  S.bond_k = cv.take<double>(p.n_bonds);
-  S.bond_r0 = cv.take<double>(p.n_bonds);
-  S.angles = cv.take<int>(3 * (size_t)p.n_angles);
-  S.angle_k = cv.take<double>(p.n_angles);
-  S.angle_t0 = cv.take<double>(p.n_angles);
   if (c->molvir) {
     S.mol_start = cv.take<int>(p.mol_start.size());
     S.mol_atoms = cv.take<int>(p.mol_atoms.size());
@@ -1059,9 +1105,7 @@ void fill_block(pm_ctx *c, const pm_system_desc &d, const SysPrep &p, SysHost &h
   upload(up, const_cast<int *>(S.excl_partner), p.excl_partner.data(), p.excl_partner.size());
   upload(up, const_cast<int *>(S.excl_class), p.excl_class.data(), p.excl_class.size());
   upload(up, const_cast<unsigned long long *>(S.excl_win), p.excl_win.data(), p.excl_win.size());
-  upload(up, const_cast<int *>(S.corr), p.corr.data(), p.corr.size());
-  upload(up, const_cast<int *>(S.loc_start), p.loc_start.data(), p.loc_start.size());
-  upload(up, const_cast<int *>(S.loc_item), p.loc_item.data(), p.loc_item.size());
+  upload(up, const_cast<pm::LocRec *>(S.loc_rec), p.loc_rec.data(), p.loc_rec.size());
   upload(up, const_cast<double *>(S.charge), d.scaled_charge, n);
   upload(up, const_cast<double *>(S.inv_mass), p.invm.data(), n);
   upload(up, const_cast<int *>(S.molecule), d.molecule, n);
@@ -1078,12 +1122,6 @@ void fill_block(pm_ctx *c, const pm_system_desc &d, const SysPrep &p, SysHost &h
     upload(up, const_cast<float4 *>(S.lj_table), p.lj.data(), p.lj.size());
     upload(up, const_cast<float4 *>(S.lj_table12), p.lj12.data(), p.lj12.size());
   }
-  upload(up, const_cast<int *>(S.bonds), d.bonds, 2 * (size_t)p.n_bonds);
-  upload(up, const_cast<double *>(S.bond_k), d.bond_k, p.n_bonds);
-  upload(up, const_cast<double *>(S.bond_r0), d.bond_r0, p.n_bonds);
-  upload(up, const_cast<int *>(S.angles), d.angles, 3 * (size_t)p.n_angles);
-  upload(up, const_cast<double *>(S.angle_k), d.angle_k, p.n_angles);
-  upload(up, const_cast<double *>(S.angle_t0), d.angle_theta0, p.n_angles);
   if (c->gb) {
     if (c->P.gb_on) {
       upload(up, const_cast<double *>(S.gb_rho), d.gb_radius, n);

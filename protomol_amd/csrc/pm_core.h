@@ -56,10 +56,31 @@ __host__ __device__ inline long long *acc_copy(long long *acc, int w) {
 // device-side status bits (per system)
 enum { kErrColumnCap = 1, kErrListCap = 2, kErrNonFinite = 4, kErrClusterCap = 8, kErrVacuumRegion = 16 };
 
-// items of the per-atom lists of bonded/correction terms, (index << 2) | role,
-// in a CSR keyed by (atom, kind)
+// kinds of the local terms of an atom (bonded terms, Ewald correction pairs)
 enum { kLocBond = 0, kLocAngle = 1, kLocCorr = 2 };
 enum { kLocalForces = 1, kLocalEnergies = 2 };
+// The local terms of an atom in kLocShares shares by (kind, slot): share w
+// takes the items slot, slot + nslots, ... of one kind of each atom (in the
+// order of the atom's list of that kind), so a wave per share runs one kind's
+// code.  Slots per kind: bonds 2, angles 2, Ewald corrections 4.
+constexpr int kLocShares = 8;
+__host__ __device__ inline void loc_share(int w, int &kind, int &slot, int &nslots) {
+  if (w < 2) {
+    kind = kLocBond, slot = w, nslots = 2;
+  } else if (w < 4) {
+    kind = kLocAngle, slot = w - 2, nslots = 2;
+  } else {
+    kind = kLocCorr, slot = w - 4, nslots = 4;
+  }
+}
+// One local term of an atom, seen from that atom: the term's atoms, the
+// atom's role in it (-1: no term) and its parameters -- bond (k, r0), angle
+// (k, theta0), Ewald correction pair (q_a q_b, times 1 - csf for a modified
+// pair).
+struct LocRec {
+  int a0, a1, a2, role;
+  double p0, p1;
+};
 // host staging transfers (k_import / k_export)
 enum { kXferX = 1, kXferV = 2, kXferF = 4 };
 // packed per-system scalars (k_gather_scalars): acc[0..16), census[16..18), err,
@@ -185,15 +206,13 @@ struct SysDev {
   // kernel's rows (A, B, 12 A, 6 B)
   const float4 *lj_table;
   const float4 *lj_table12;
-  // bonded
-  int n_bonds, n_angles;
-  const int *bonds, *angles;
-  const double *bond_k, *bond_r0, *angle_k, *angle_t0;
-  // Ewald correction pairs (unique exclusion table pairs)
-  int n_corr;
-  const int *corr;         // 3 per pair: a, b, class
-  const int *loc_start;    // 3 n + 1: CSR of bonded/correction items by (atom, kind)
-  const int *loc_item;
+  // local terms: bonds, angles, Ewald correction pairs (unique exclusion
+  // table pairs), as rows of records -- row loc_base[w] + k holds item k of
+  // share w of every atom (loc_rows[w] rows), so a share's loads are
+  // coalesced and need no index lookups
+  int n_bonds, n_angles, n_corr;
+  const LocRec *loc_rec;
+  int loc_rows[kLocShares], loc_base[kLocShares];
   // GB
   const double *gb_rho, *gb_S;
   double *gb_born, *gb_drfac, *gb_partial, *gb_spare;

@@ -11,43 +11,32 @@
 
 namespace pm {
 
-// Local terms of an atom split over the waves of a block by (kind, slot):
-// a block covers 64 atoms with kLocWaves waves, wave w takes kind kLocKind[w]
-// and the items slot, slot + nslots, ... of that kind of each of its atoms,
-// so a wave runs one kind's code and an atom waits for the longest of its
-// terms instead of their sum.  Slots per kind: bonds 2, angles 2, Ewald
-// corrections 4.
-constexpr int kLocWaves = 8;
-__device__ __forceinline__ void loc_slot(int w, int &kind, int &slot, int &nslots) {
-  if (w < 2) {
-    kind = kLocBond, slot = w, nslots = 2;
-  } else if (w < 4) {
-    kind = kLocAngle, slot = w - 2, nslots = 2;
-  } else {
-    kind = kLocCorr, slot = w - 4, nslots = 4;
-  }
-}
+// one wave per share (loc_share): an atom waits for the longest of its terms
+// instead of their sum
+constexpr int kLocWaves = kLocShares;
 
-// An atom's items of one kind in slot `slot` of `nslots`, FP64, evaluated
-// from that atom's side ("owner computes": each participant evaluates the
-// term with the same expressions, so the forces on its atoms are exactly
-// opposite; energies and virial are counted by the term's first atom).
+// An atom's items of share w (kind `kind`), FP64, evaluated from that atom's
+// side ("owner computes": each participant evaluates the term with the same
+// expressions, so the forces on its atoms are exactly opposite; energies and
+// virial are counted by the term's first atom).  The items come as records
+// (LocRec): one coalesced load, then the positions of the term's atoms.
 //   BondSystemForce::calcBond (force/bonded/BondSystemForce.h:77-108)
 //   AngleSystemForce::calcAngle (force/bonded/AngleSystemForce.h:66-140)
 //   NonbondedFullEwaldSystemForce::correctionTerm (NonbondedFullEwaldSystemForce.h:1097-1131)
 template <bool EN>
-__device__ __forceinline__ void local_terms(const SysDev &S, int a, const ForceDev &P, int kind, int slot, int nslots,
-                                            double f[3], double &eb, double &ea, double &ec, double v[6]) {
+__device__ __forceinline__ void local_terms(const SysDev &S, int a, const ForceDev &P, int w, int kind, double f[3],
+                                            double &eb, double &ea, double &ec, double v[6]) {
   const int n = S.n_atoms;
   const PM_GLOBAL double *X = gp(S.x);
-  const PM_GLOBAL int *ls = gp(S.loc_start) + 3 * (size_t)a;
-  const int beg = ls[kind], end = ls[kind + 1];
-  for (int it = beg + slot; it < end; it += nslots) {
-    const int item = gp(S.loc_item)[it];
-    const int role = item & 3, k = item >> 2;
+  const PM_GLOBAL LocRec *R = gp(S.loc_rec) + (size_t)S.loc_base[w] * n + a;
+  const int rows = S.loc_rows[w];
+  for (int k = 0; k < rows; ++k) {
+    const LocRec rec = R[(size_t)k * n];
+    const int role = rec.role;
+    if (role < 0) break;  // the atom's items of the share come first
     if (kind == kLocBond) {
-      const int a1 = gp(S.bonds)[2 * k], a2 = gp(S.bonds)[2 * k + 1];
-      const double kb = gp(S.bond_k)[k], r0 = gp(S.bond_r0)[k];
+      const int a1 = rec.a0, a2 = rec.a1;
+      const double kb = rec.p0, r0 = rec.p1;
       double r12[3] = {X[a1] - X[a2], X[n + a1] - X[n + a2], X[2 * n + a1] - X[2 * n + a2]};
       min_image_vec(S, r12);
       const double r = sqrt(r12[0] * r12[0] + r12[1] * r12[1] + r12[2] * r12[2]);
@@ -65,8 +54,8 @@ __device__ __forceinline__ void local_terms(const SysDev &S, int a, const ForceD
         v[5] += r12[2] * c * r12[2];
       }
     } else if (kind == kLocAngle) {
-      const int a1 = gp(S.angles)[3 * k], a2 = gp(S.angles)[3 * k + 1], a3 = gp(S.angles)[3 * k + 2];
-      const double ka = gp(S.angle_k)[k], t0 = gp(S.angle_t0)[k];
+      const int a1 = rec.a0, a2 = rec.a1, a3 = rec.a2;
+      const double ka = rec.p0, t0 = rec.p1;
       double r12[3] = {X[a1] - X[a2], X[n + a1] - X[n + a2], X[2 * n + a1] - X[2 * n + a2]};
       double r32[3] = {X[a3] - X[a2], X[n + a3] - X[n + a2], X[2 * n + a3] - X[2 * n + a2]};
       min_image_vec(S, r12);
@@ -99,12 +88,11 @@ __device__ __forceinline__ void local_terms(const SysDev &S, int a, const ForceD
         v[5] += f1[2] * r12[2] + f3[2] * r32[2];
       }
     } else {
-      const int a1 = gp(S.corr)[3 * k], a2 = gp(S.corr)[3 * k + 1], cls = gp(S.corr)[3 * k + 2];
+      const int a1 = rec.a0, a2 = rec.a1;
       double d[3] = {X[a2] - X[a1], X[n + a2] - X[n + a1], X[2 * n + a2] - X[2 * n + a1]};
       min_image_vec(S, d);
       const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-      double qq = gp(S.charge)[a1] * gp(S.charge)[a2];
-      if (cls == PM_EXCL_MODIFIED) qq *= 1.0 - (double)S.csf;
+      const double qq = rec.p0;  // q_a q_b, times 1 - csf for a modified pair
       const double r = sqrt(r2), rr = 1.0 / r;
       const double e = erf(P.dalpha * r) * rr;  // C-library erf, as the reference on Linux
       const double fs = qq * (P.dtwo_alpha_pi * exp(-(P.dalpha * P.dalpha) * r2) - e) * rr * rr;
@@ -134,16 +122,16 @@ __device__ __forceinline__ void local_block(const SysDev *__restrict__ sys, cons
   const int total = ap[nsys];
   const bool ok = g < total;
   int kind, slot, nslots;
-  loc_slot(wv, kind, slot, nslots);
+  loc_share(wv, kind, slot, nslots);
   double f[3] = {0.0, 0.0, 0.0};
   double eb = 0.0, ea = 0.0, ec = 0.0, v[6] = {0, 0, 0, 0, 0, 0};
   int s = wave_sys(ap, nsys, g, total);
   if (s >= 0) {
     const SysDev *__restrict__ S = sys + s;  // scalar loads of the record
-    if (ok) local_terms<EN>(*S, g - ap[s], P, kind, slot, nslots, f, eb, ea, ec, v);
+    if (ok) local_terms<EN>(*S, g - ap[s], P, wv, kind, f, eb, ea, ec, v);
   } else {
     s = ok ? find_sys(ap, nsys, g) : 0;
-    if (ok) local_terms<EN>(sys[s], g - ap[s], P, kind, slot, nslots, f, eb, ea, ec, v);
+    if (ok) local_terms<EN>(sys[s], g - ap[s], P, wv, kind, f, eb, ea, ec, v);
   }
   if (mode & kLocalForces) {
     part[wv][0][lane] = f[0];
@@ -169,17 +157,29 @@ __device__ __forceinline__ void local_block(const SysDev *__restrict__ sys, cons
                     to_fixed(v[0], kEScale), to_fixed(v[1], kEScale), to_fixed(v[2], kEScale),
                     to_fixed(v[3], kEScale), to_fixed(v[4], kEScale), to_fixed(v[5], kEScale)};
   const int slots[9] = {kAccBond, kAccAngle, kAccCoul, kAccVxx, kAccVxy, kAccVxz, kAccVyy, kAccVyz, kAccVzz};
+  // the block's 64 atoms are all of one system or not, alike in every wave
   const int s0 = __shfl(s, 0);
   const bool mixed = wave_ballot(ok && s != s0) != 0;
-  const int wid = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  if (!mixed) {
-    for (int k = 0; k < 9; ++k) q[k] = wave_sum_ll(q[k]);
-    if (lane == 0)
+  if (mixed) {
+    if (ok)
       for (int k = 0; k < 9; ++k)
-        if (q[k]) atomic_add_ll(&acc_copy(sys[s0].acc, wid)[slots[k]], q[k]);
-  } else if (ok) {
-    for (int k = 0; k < 9; ++k)
-      if (q[k]) atomic_add_ll(&acc_copy(sys[s].acc, wid)[slots[k]], q[k]);
+        if (q[k]) atomic_add_ll(&acc_copy(sys[s].acc, blockIdx.x)[slots[k]], q[k]);
+    return;
+  }
+  // wave sums, then the block's through LDS: nine atomics per block
+  for (int k = 0; k < 9; ++k) q[k] = wave_sum_ll(q[k]);
+  __syncthreads();  // every read of part above is done
+  long long *red = reinterpret_cast<long long *>(&part[0][0][0]);
+  if (lane == 0)
+    for (int k = 0; k < 9; ++k) red[wv * 9 + k] = q[k];
+  __syncthreads();
+  if (wv == 0 && lane == 0) {
+    long long *acc = acc_copy(sys[s0].acc, blockIdx.x);
+    for (int k = 0; k < 9; ++k) {
+      long long t = 0;
+      for (int w = 0; w < kLocWaves; ++w) t += red[w * 9 + k];
+      if (t) atomic_add_ll(&acc[slots[k]], t);
+    }
   }
 }
 
