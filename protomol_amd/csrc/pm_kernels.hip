@@ -224,8 +224,22 @@ constexpr int kListThreads = kListWaves * kWave;
 constexpr int kWinChunk = kWave;                  // neighbour columns per segment pass
 constexpr int kSegCap = 3 * kWinChunk;            // (column, z image) segments per pass
 
-struct ListScratch {  // per wave, pass B: each lane's entry's FULL and MODIFIED words per i cluster
-  unsigned long long mask[kWave][2 * kScCl];
+// per wave, pass B: each lane's entry's FULL and MODIFIED words per i cluster,
+// word-major (word w of lane l at mask[w][l]), so a wave's accesses spread
+// over the LDS banks
+struct ListScratch {
+  unsigned long long mask[2 * kScCl][kWave];
+};
+// per wave, pass B: the excluded pairs of the super-cluster's atoms as (the
+// partner's cluster, bits (partner lane << 8) | (i cluster << 4) |
+// (i lane << 1) | MODIFIED), chained per bucket of the partner's cluster
+// index (head, next; -1 ends a chain); more than kHitCap send the
+// super-cluster to the per-entry walk of the j atoms' partners
+constexpr int kHitCap = 256, kHitBuckets = 256;
+struct ListHits {
+  int2 h[kHitCap];
+  int next[kHitCap];
+  int head[kHitBuckets];
 };
 
 // first k in [0, m) with key(k) >= v (m if none), keys ascending
@@ -255,6 +269,7 @@ __global__ __launch_bounds__(kListThreads) void k_pairlist(const SysDev *__restr
   __shared__ float4 ib_lo[kListWaves][kScCl], ib_hi[kListWaves][kScCl];
   __shared__ int seg_start[kListWaves][kSegCap + 1], seg_cl[kListWaves][kSegCap], seg_sh[kListWaves][kSegCap];
   __shared__ ListScratch scratch[kListWaves];
+  __shared__ ListHits hits[kListWaves];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int scg = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * kListWaves + wv));
   if (scg >= scp[nsys]) return;
@@ -406,13 +421,73 @@ __global__ __launch_bounds__(kListThreads) void k_pairlist(const SysDev *__restr
     count = cap;
   }
   const double ox = S.cl_origin[cl0], oy = S.cl_origin[S.cl_cap + cl0], oz = S.cl_origin[2 * S.cl_cap + cl0];
+  // the super-cluster's excluded pairs, from its own atoms' side: lane = i
+  // atom (i cluster lane >> 3, i lane lane & 7), one partner per round, each
+  // kept when it is in the same molecule -- every excluded pair of an entry's
+  // tiles is one of these, found in LDS by the partner's cluster
+  int nhit = 0;
+  {
+    for (int b = lane; b < kHitBuckets; b += kWave) hits[wv].head[b] = -1;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int c = lane >> 3, ii = lane & 7;
+    int bi = -1, mi = -1, k = 0, kend = 0;
+    unsigned long long wf = 0, bits = 0;
+    if (c < nic) {
+      bi = S.orig[(cl0 + c) * kCluster + ii];
+      if (bi >= 0) {
+        mi = S.molecule[bi];
+        wf = S.excl_win[2 * (size_t)bi];
+        if ((wf >> 63) != 0) {
+          k = S.excl_start[bi];
+          kend = S.excl_start[bi + 1];
+        } else {
+          bits = (wf | S.excl_win[2 * (size_t)bi + 1]) & ~(1ull << 63);
+        }
+      }
+    }
+    while (true) {
+      int b = -1, cls = 0;
+      if (bits) {
+        const int d = __ffsll((long long)bits) - 1;
+        bits &= bits - 1;
+        b = bi + d - 31;
+        cls = ((wf >> d) & 1ull) ? PM_EXCL_FULL : PM_EXCL_MODIFIED;
+      } else if (k < kend) {
+        b = S.excl_partner[k];
+        cls = S.excl_class[k];
+        ++k;
+      }
+      if (wave_ballot(b >= 0) == 0) break;
+      bool keep = false;
+      int2 hv = make_int2(0, 0);
+      if (b >= 0 && (cls == PM_EXCL_FULL || cls == PM_EXCL_MODIFIED) && S.molecule[b] == mi) {
+        const int sb = S.slot[b];
+        keep = true;
+        hv = make_int2(sb >> 3, ((sb & (kCluster - 1)) << 8) | (c << 4) | (ii << 1) |
+                                     (cls == PM_EXCL_MODIFIED ? 1 : 0));
+      }
+      const unsigned long long kb = wave_ballot(keep);
+      const int pos = nhit + __popcll(kb & ((1ull << lane) - 1ull));
+      if (keep && pos < kHitCap) {
+        hits[wv].h[pos] = hv;
+        hits[wv].next[pos] = atomicExch(&hits[wv].head[hv.x & (kHitBuckets - 1)], pos);
+      }
+      nhit += __popcll(kb);
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   int mslot = 0;
   unsigned long long *mbase = S.list_mask + (size_t)sc * S.mask_cap * (2 * kScCl);
-  unsigned long long *M = scratch[wv].mask[lane];  // this lane's entry: FULL, MODIFIED word per i cluster
+  // this lane's entry: FULL, MODIFIED word per i cluster, word w at M[w * kWave]
+  unsigned long long *M = &scratch[wv].mask[0][lane];
   for (int e0 = 0; e0 < count; e0 += kWave) {
     const int e = e0 + lane;
     unsigned share = 0;
-    for (int w = 0; w < 2 * kScCl; ++w) M[w] = 0ull;
+    for (int w = 0; w < 2 * kScCl; ++w) M[w * kWave] = 0ull;
     if (e < count) {
       const unsigned x = out[2 * e].x;
       const unsigned im = out[2 * e + 1].x & 0xFFu;
@@ -426,12 +501,23 @@ __global__ __launch_bounds__(kListThreads) void k_pairlist(const SysDev *__restr
       out[2 * e].z = __float_as_uint((float)(ojy - oy));
       out[2 * e].w = __float_as_uint((float)(ojz - oz));
       pairs8 += __popc(im);
-      // the excluded pairs of the entry's tiles: each j atom's exclusion
+      // the excluded pairs of the entry's tiles, lane bit jj * 8 + ii of the
+      // tile's FULL or MODIFIED word: the super-cluster's pairs whose partner
+      // is in cluster cj, or, past kHitCap of those, each j atom's exclusion
       // partners (its index window, or the symmetric CSR for an atom with a
-      // partner outside it), located by slot -- the super-cluster's i clusters
-      // are the contiguous cl0 .. cl0 + nic - 1 -- and kept when the partner
-      // is in a tile of the entry and in the same molecule; lane bit jj * 8 + ii
-      // of the tile's FULL or MODIFIED word
+      // partner outside it), located by slot -- the super-cluster's i
+      // clusters are the contiguous cl0 .. cl0 + nic - 1 -- and kept when
+      // the partner is in a tile of the entry and in the same molecule
+      if (nhit <= kHitCap) {
+        for (int q = hits[wv].head[cj & (kHitBuckets - 1)]; q >= 0; q = hits[wv].next[q]) {
+          const int2 hv = hits[wv].h[q];
+          const int c = (hv.y >> 4) & 7;
+          if (hv.x == cj && ((im >> c) & 1u)) {
+            share |= 1u << c;
+            M[(2 * c + (hv.y & 1)) * kWave] |= 1ull << (((hv.y >> 8) & 7) * kCluster + ((hv.y >> 1) & 7));
+          }
+        }
+      } else
       for (int jj = 0; jj < kCluster; ++jj) {
         const int sj = cj * kCluster + jj;
         const int bj = S.orig[sj];
@@ -444,7 +530,8 @@ __global__ __launch_bounds__(kListThreads) void k_pairlist(const SysDev *__restr
           if (c >= 0 && c < nic && ((im >> c) & 1u) && S.smol[sb] == mj &&
               (cls == PM_EXCL_FULL || cls == PM_EXCL_MODIFIED)) {
             share |= 1u << c;
-            M[2 * c + (cls == PM_EXCL_MODIFIED ? 1 : 0)] |= 1ull << (jj * kCluster + (sb & (kCluster - 1)));
+            M[(2 * c + (cls == PM_EXCL_MODIFIED ? 1 : 0)) * kWave] |=
+                1ull << (jj * kCluster + (sb & (kCluster - 1)));
           }
         };
         if ((wf >> 63) != 0) {
@@ -464,7 +551,7 @@ __global__ __launch_bounds__(kListThreads) void k_pairlist(const SysDev *__restr
     const int my = mslot + __popcll(fl & ((1ull << lane) - 1ull));
     if (share != 0) {
       if (my < S.mask_cap) {
-        for (int w = 0; w < 2 * kScCl; ++w) mbase[(size_t)my * (2 * kScCl) + w] = M[w];
+        for (int w = 0; w < 2 * kScCl; ++w) mbase[(size_t)my * (2 * kScCl) + w] = M[w * kWave];
         out[2 * e].x |= 0x80000000u;
         out[2 * e + 1].x |= (unsigned)my << 8;
       } else {
