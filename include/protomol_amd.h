@@ -183,7 +183,10 @@ int pm_ctx_create(const pm_force_desc *force, int32_t n_systems, const pm_system
 int pm_ctx_destroy(pm_ctx *ctx);
 /* Reserve `bytes` of device memory for the context's systems (once, before
  * any system is added).  Afterwards every system is placed inside it and
- * pm_system_add returns PM_EFULL when it does not fit. */
+ * pm_system_add returns PM_EFULL when it does not fit.  Admission leaves 1/16
+ * of the reservation free for the pair lists of admitted systems, which grow
+ * (double) when an MD step overflows them; a growth that still finds no room
+ * makes that call return PM_EFULL. */
 int pm_ctx_reserve(pm_ctx *ctx, uint64_t bytes);
 /* Reserved and used bytes of the reservation (0, 0 without one). */
 int pm_ctx_memory(pm_ctx *ctx, uint64_t *reserved, uint64_t *used);

@@ -56,6 +56,9 @@ inline size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 struct Arena {
   char *base = nullptr;
   size_t cap = 0, used = 0;
+  // admission leaves this much of the reservation free: the pair lists of
+  // admitted systems grow into it when an MD step overflows them
+  size_t keep = 0;
   std::map<size_t, size_t> free_;  // offset -> bytes
 
   void init(size_t bytes) {
@@ -65,11 +68,14 @@ struct Arena {
     base = static_cast<char *>(p);
     cap = bytes;
     used = 0;
+    keep = bytes / 16;
     free_.clear();
     free_[0] = bytes;
   }
-  void *alloc(size_t bytes) {
+  // admit: an admission, which must leave `keep` bytes free
+  void *alloc(size_t bytes, bool admit = false) {
     bytes = align_up(std::max<size_t>(bytes, 1));
+    if (admit && used + bytes + keep > cap) return nullptr;
     for (auto it = free_.begin(); it != free_.end(); ++it) {
       if (it->second < bytes) continue;
       const size_t off = it->first, rest = it->second - bytes;
@@ -327,9 +333,11 @@ void set_err(pm_ctx *c, const std::string &m) {
   g_thread_err = m;
 }
 
-void *dev_alloc(pm_ctx *c, size_t bytes) {
+// admit: the allocation admits a system (it leaves the reservation's growth
+// share free); otherwise it grows an admitted one
+void *dev_alloc(pm_ctx *c, size_t bytes, bool admit = false) {
   if (c->arena.base) {
-    void *p = c->arena.alloc(bytes);
+    void *p = c->arena.alloc(bytes, admit);
     if (!p) throw Fail{PM_EFULL, "the device reservation has no room for " + std::to_string(bytes) + " bytes"};
     return p;
   }
@@ -592,52 +600,66 @@ SysPrep prepare(const pm_ctx *c, const pm_system_desc &d) {
     for (int k = 0; k < p.n_corr; ++k)
       for (int r = 0; r < 2; ++r) p.loc_item[fill[3 * p.corr[3 * k + r] + pm::kLocCorr]++] = (k << 2) | r;
     // the records: for share w, row loc_base[w] + k holds item slot + k nslots
-    // of the atom's list of that kind (role -1 where the atom has fewer)
+    // of the atom's list of that kind (role -1 where the atom has fewer);
+    // rows per share from the longest list of each kind
+    int maxcnt[3] = {0, 0, 0};
+    for (int a = 0; a < n; ++a)
+      for (int kind = 0; kind < 3; ++kind)
+        maxcnt[kind] = std::max(maxcnt[kind], p.loc_start[3 * a + kind + 1] - p.loc_start[3 * a + kind]);
     int rows = 0;
     for (int w = 0; w < pm::kLocShares; ++w) {
       int kind, slot, ns;
       pm::loc_share(w, kind, slot, ns);
-      int m = 0;
-      for (int a = 0; a < n; ++a) {
-        const int cnt = p.loc_start[3 * a + kind + 1] - p.loc_start[3 * a + kind];
-        m = std::max(m, cnt > slot ? (cnt - slot + ns - 1) / ns : 0);
-      }
-      p.loc_rows[w] = m;
+      p.loc_rows[w] = maxcnt[kind] > slot ? (maxcnt[kind] - slot + ns - 1) / ns : 0;
       p.loc_base[w] = rows;
-      rows += m;
+      rows += p.loc_rows[w];
     }
     const double mscale = 1.0 - (double)(float)d.coulomb_scaling;  // as the device's float csf
-    p.loc_rec.assign((size_t)rows * n, pm::LocRec{-1, -1, -1, -1, 0.0, 0.0});
-    for (int w = 0; w < pm::kLocShares; ++w) {
-      int kind, slot, ns;
-      pm::loc_share(w, kind, slot, ns);
-      for (int a = 0; a < n; ++a) {
-        const int beg = p.loc_start[3 * a + kind], end = p.loc_start[3 * a + kind + 1];
-        int k = 0;
-        for (int it = beg + slot; it < end; it += ns, ++k) {
-          const int item = p.loc_item[it], role = item & 3, t = item >> 2;
-          pm::LocRec &r = p.loc_rec[(size_t)(p.loc_base[w] + k) * n + a];
-          r.role = role;
-          if (kind == pm::kLocBond) {
-            r.a0 = d.bonds[2 * t];
-            r.a1 = d.bonds[2 * t + 1];
-            r.p0 = d.bond_k[t];
-            r.p1 = d.bond_r0[t];
-          } else if (kind == pm::kLocAngle) {
-            r.a0 = d.angles[3 * t];
-            r.a1 = d.angles[3 * t + 1];
-            r.a2 = d.angles[3 * t + 2];
-            r.p0 = d.angle_k[t];
-            r.p1 = d.angle_theta0[t];
-          } else {
-            r.a0 = p.corr[3 * t];
-            r.a1 = p.corr[3 * t + 1];
-            double qq = d.scaled_charge[r.a0] * d.scaled_charge[r.a1];
-            if (p.corr[3 * t + 2] == PM_EXCL_MODIFIED) qq *= mscale;
-            r.p0 = qq;
+    p.loc_rec.resize((size_t)rows * n);
+    // atoms [a0, a1) of every row; large systems fill their rows on a few threads
+    auto fill = [&](int a0, int a1) {
+      for (int w = 0; w < pm::kLocShares; ++w) {
+        int kind, slot, ns;
+        pm::loc_share(w, kind, slot, ns);
+        for (int k = 0; k < p.loc_rows[w]; ++k) {
+          pm::LocRec *row = p.loc_rec.data() + (size_t)(p.loc_base[w] + k) * n;
+          for (int a = a0; a < a1; ++a) {
+            const int it = p.loc_start[3 * a + kind] + slot + k * ns;
+            pm::LocRec r{-1, -1, -1, -1, 0.0, 0.0};
+            if (it < p.loc_start[3 * a + kind + 1]) {
+              const int item = p.loc_item[it], t = item >> 2;
+              r.role = item & 3;
+              if (kind == pm::kLocBond) {
+                r.a0 = d.bonds[2 * t];
+                r.a1 = d.bonds[2 * t + 1];
+                r.p0 = d.bond_k[t];
+                r.p1 = d.bond_r0[t];
+              } else if (kind == pm::kLocAngle) {
+                r.a0 = d.angles[3 * t];
+                r.a1 = d.angles[3 * t + 1];
+                r.a2 = d.angles[3 * t + 2];
+                r.p0 = d.angle_k[t];
+                r.p1 = d.angle_theta0[t];
+              } else {
+                r.a0 = p.corr[3 * t];
+                r.a1 = p.corr[3 * t + 1];
+                double qq = d.scaled_charge[r.a0] * d.scaled_charge[r.a1];
+                if (p.corr[3 * t + 2] == PM_EXCL_MODIFIED) qq *= mscale;
+                r.p0 = qq;
+              }
+            }
+            row[a] = r;
           }
         }
       }
+    };
+    const int nt = std::min(8, 1 + n / 16384);
+    if (nt == 1) {
+      fill(0, n);
+    } else {
+      std::vector<std::thread> th;
+      for (int t = 0; t < nt; ++t) th.emplace_back(fill, (int)((long long)n * t / nt), (int)((long long)n * (t + 1) / nt));
+      for (auto &t : th) t.join();
     }
   }
   // per-atom data
@@ -883,10 +905,10 @@ void place_lists(SysHost &h, void *mem) {
   h.dev.list_mask = cv.take<unsigned long long>((size_t)h.sc_cap * h.dev.mask_cap * 2 * pm::kScCl);
 }
 
-void alloc_lists(pm_ctx *c, SysHost &h) {
+void alloc_lists(pm_ctx *c, SysHost &h, bool admit = false) {
   if (h.cl_cap == 0) return;
   const size_t bytes = lists_bytes(h.sc_cap, h.list_cap);
-  void *mem = dev_alloc(c, bytes);
+  void *mem = dev_alloc(c, bytes, admit);
   dev_free(c, h.lists, h.lists_bytes);
   h.lists = mem;
   h.lists_bytes = bytes;
@@ -1148,9 +1170,9 @@ int commit_system(pm_ctx *c, const pm_system_desc &d, const SysPrep &p, Uploads 
   SysHost &h = *hp;
   init_system(c, d, p, h);
   try {
-    h.block = dev_alloc(c, h.block_bytes);
+    h.block = dev_alloc(c, h.block_bytes, true);
     fill_block(c, d, p, h, true, &up);
-    if (!c->gb) alloc_lists(c, h);
+    if (!c->gb) alloc_lists(c, h, true);
   } catch (const Fail &f) {
     if (h.block) {
       // drop this system's queued uploads (they point into its block)
