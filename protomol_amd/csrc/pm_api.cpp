@@ -617,7 +617,7 @@ SysPrep prepare(const pm_ctx *c, const pm_system_desc &d) {
     const double mscale = 1.0 - (double)(float)d.coulomb_scaling;  // as the device's float csf
     p.loc_rec.resize((size_t)rows * n);
     // atoms [a0, a1) of every row; large systems fill their rows on a few threads
-    auto fill = [&](int a0, int a1) {
+    auto fill_rows = [&](int a0, int a1) {
       for (int w = 0; w < pm::kLocShares; ++w) {
         int kind, slot, ns;
         pm::loc_share(w, kind, slot, ns);
@@ -655,10 +655,10 @@ SysPrep prepare(const pm_ctx *c, const pm_system_desc &d) {
     };
     const int nt = std::min(8, 1 + n / 16384);
     if (nt == 1) {
-      fill(0, n);
+      fill_rows(0, n);
     } else {
       std::vector<std::thread> th;
-      for (int t = 0; t < nt; ++t) th.emplace_back(fill, (int)((long long)n * t / nt), (int)((long long)n * (t + 1) / nt));
+      for (int t = 0; t < nt; ++t) th.emplace_back(fill_rows, (int)((long long)n * t / nt), (int)((long long)n * (t + 1) / nt));
       for (auto &t : th) t.join();
     }
   }
