@@ -216,9 +216,10 @@ __device__ __forceinline__ int floordiv(int a, int b) {
 // three z images, can come within rlist of the super-cluster's z range: the
 // clusters of a column are z-ordered, so two binary searches bound each
 // (column, z image) segment.  The wave walks the segments in a fixed order, so
-// the list order is the same on every rebuild of the same positions.  A
-// second pass fills in the entries' FP32 image offsets and the FULL/MODIFIED
-// lane masks of cluster pairs that share a molecule.
+// the list order is the same on every rebuild of the same positions.  Each
+// entry is written whole as it is emitted: its FP32 image offset and the
+// FULL/MODIFIED lane masks of cluster pairs that share a molecule, from the
+// super-cluster's own excluded pairs, gathered in LDS before the walk.
 constexpr int kListWaves = 4;                     // waves (super-clusters) per workgroup
 constexpr int kListThreads = kListWaves * kWave;
 constexpr int kWinChunk = kWave;                  // neighbour columns per segment pass
@@ -305,6 +306,71 @@ __global__ __launch_bounds__(kListThreads) void k_pairlist(const SysDev *__restr
     zlo = fminf(zlo, ib_lo[wv][c].z);
     zhi = fmaxf(zhi, ib_hi[wv][c].z);
   }
+  const double ox = S.cl_origin[cl0], oy = S.cl_origin[S.cl_cap + cl0], oz = S.cl_origin[2 * S.cl_cap + cl0];
+  // the super-cluster's excluded pairs, from its own atoms' side: lane = i
+  // atom (i cluster lane >> 3, i lane lane & 7), one partner per round, each
+  // kept when it is in the same molecule -- every excluded pair of an entry's
+  // tiles is one of these, found in LDS by the partner's cluster
+  int nhit = 0;
+  {
+    for (int b = lane; b < kHitBuckets; b += kWave) hits[wv].head[b] = -1;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int c = lane >> 3, ii = lane & 7;
+    int bi = -1, mi = -1, k = 0, kend = 0;
+    unsigned long long wf = 0, bits = 0;
+    if (c < nic) {
+      bi = S.orig[(cl0 + c) * kCluster + ii];
+      if (bi >= 0) {
+        mi = S.molecule[bi];
+        wf = S.excl_win[2 * (size_t)bi];
+        if ((wf >> 63) != 0) {
+          k = S.excl_start[bi];
+          kend = S.excl_start[bi + 1];
+        } else {
+          bits = (wf | S.excl_win[2 * (size_t)bi + 1]) & ~(1ull << 63);
+        }
+      }
+    }
+    while (true) {
+      int b = -1, cls = 0;
+      if (bits) {
+        const int d = __ffsll((long long)bits) - 1;
+        bits &= bits - 1;
+        b = bi + d - 31;
+        cls = ((wf >> d) & 1ull) ? PM_EXCL_FULL : PM_EXCL_MODIFIED;
+      } else if (k < kend) {
+        b = S.excl_partner[k];
+        cls = S.excl_class[k];
+        ++k;
+      }
+      if (wave_ballot(b >= 0) == 0) break;
+      bool keep = false;
+      int2 hv = make_int2(0, 0);
+      if (b >= 0 && (cls == PM_EXCL_FULL || cls == PM_EXCL_MODIFIED) && S.molecule[b] == mi) {
+        const int sb = S.slot[b];
+        keep = true;
+        hv = make_int2(sb >> 3, ((sb & (kCluster - 1)) << 8) | (c << 4) | (ii << 1) |
+                                     (cls == PM_EXCL_MODIFIED ? 1 : 0));
+      }
+      const unsigned long long kb = wave_ballot(keep);
+      const int pos = nhit + __popcll(kb & ((1ull << lane) - 1ull));
+      if (keep && pos < kHitCap) {
+        hits[wv].h[pos] = hv;
+        hits[wv].next[pos] = atomicExch(&hits[wv].head[hv.x & (kHitBuckets - 1)], pos);
+      }
+      nhit += __popcll(kb);
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  long long pairs8 = 0;  // 8x8 cluster pairs, for the statistics
+  int mslot = 0;
+  unsigned long long *mbase = S.list_mask + (size_t)sc * S.mask_cap * (2 * kScCl);
+  // this lane's entry: FULL, MODIFIED word per i cluster, word w at M[w * kWave]
+  unsigned long long *M = &scratch[wv].mask[0][lane];
   uint4 *out = S.list + 2 * (size_t)sc * cap;
   int count = 0;
   // ---- pass A: entries (j cluster, image, i-cluster mask), segment by segment
@@ -401,164 +467,91 @@ __global__ __launch_bounds__(kListThreads) void k_pairlist(const SysDev *__restr
       const unsigned long long b = wave_ballot(hit);
       if (b == 0) continue;
       const int pos = count + __popcll(b & ((1ull << lane) - 1ull));
-      if (hit && pos < cap) {
-        // the image offset and masks are filled in by pass B
-        const int sz = ((shw >> 5) & 3) - 1;
-        const int sh = (shw & 3) * 9 + ((shw >> 2) & 3) * 3 + (sz + 1);
+      const bool put = hit && pos < cap;
+      // the entry whole: j cluster, image, valid j atoms, the j cluster's
+      // origin relative to the super-cluster's (FP32), and its exclusion masks
+      unsigned share = 0, w0 = 0;
+      float offx = 0.f, offy = 0.f, offz = 0.f;
+      for (int w = 0; w < 2 * kScCl; ++w) M[w * kWave] = 0ull;
+      if (put) {
+        const int sx = (shw & 3) - 1, sy = ((shw >> 2) & 3) - 1, sz = ((shw >> 5) & 3) - 1;
+        const int sh = (sx + 1) * 9 + (sy + 1) * 3 + (sz + 1);
         const int nvj = S.cl_nvalid[cj];
-        out[2 * pos].x = (unsigned)cj | ((unsigned)sh << 23) | ((unsigned)max(nvj - 1, 0) << 28);
-        out[2 * pos + 1] = make_uint4(im, 0u, 0u, 0u);
+        w0 = (unsigned)cj | ((unsigned)sh << 23) | ((unsigned)max(nvj - 1, 0) << 28);
+        offx = (float)(S.cl_origin[cj] + (double)sx * S.L[0] - ox);
+        offy = (float)(S.cl_origin[S.cl_cap + cj] + (double)sy * S.L[1] - oy);
+        offz = (float)(S.cl_origin[2 * S.cl_cap + cj] + (double)sz * S.L[2] - oz);
+        pairs8 += __popc(im);
+        // the excluded pairs of the entry's tiles, lane bit jj * 8 + ii of the
+        // tile's FULL or MODIFIED word: the super-cluster's pairs whose partner
+        // is in cluster cj, or, past kHitCap of those, each j atom's exclusion
+        // partners (its index window, or the symmetric CSR for an atom with a
+        // partner outside it), located by slot -- the super-cluster's i
+        // clusters are the contiguous cl0 .. cl0 + nic - 1 -- and kept when
+        // the partner is in a tile of the entry and in the same molecule
+        if (nhit <= kHitCap) {
+          for (int q = hits[wv].head[cj & (kHitBuckets - 1)]; q >= 0; q = hits[wv].next[q]) {
+            const int2 hv = hits[wv].h[q];
+            const int c = (hv.y >> 4) & 7;
+            if (hv.x == cj && ((im >> c) & 1u)) {
+              share |= 1u << c;
+              M[(2 * c + (hv.y & 1)) * kWave] |= 1ull << (((hv.y >> 8) & 7) * kCluster + ((hv.y >> 1) & 7));
+            }
+          }
+        } else
+        for (int jj = 0; jj < kCluster; ++jj) {
+          const int sj = cj * kCluster + jj;
+          const int bj = S.orig[sj];
+          if (bj < 0) continue;
+          const int mj = S.smol[sj];
+          const unsigned long long wf = S.excl_win[2 * (size_t)bj], wm = S.excl_win[2 * (size_t)bj + 1];
+          auto partner = [&](int b, int cls) {
+            const int sb = S.slot[b];
+            const int c = (sb >> 3) - cl0;
+            if (c >= 0 && c < nic && ((im >> c) & 1u) && S.smol[sb] == mj &&
+                (cls == PM_EXCL_FULL || cls == PM_EXCL_MODIFIED)) {
+              share |= 1u << c;
+              M[(2 * c + (cls == PM_EXCL_MODIFIED ? 1 : 0)) * kWave] |=
+                  1ull << (jj * kCluster + (sb & (kCluster - 1)));
+            }
+          };
+          if ((wf >> 63) != 0) {
+            for (int k = S.excl_start[bj]; k < S.excl_start[bj + 1]; ++k) partner(S.excl_partner[k], S.excl_class[k]);
+          } else {
+            unsigned long long bits = (wf | wm) & ~(1ull << 63);
+            while (bits) {
+              const int d = __ffsll((long long)bits) - 1;
+              bits &= bits - 1;
+              partner(bj + d - 31, ((wf >> d) & 1ull) ? PM_EXCL_FULL : PM_EXCL_MODIFIED);
+            }
+          }
+        }
       }
+      // mask slots in entry order
+      const unsigned long long fl = wave_ballot(share != 0);
+      const int my = mslot + __popcll(fl & ((1ull << lane) - 1ull));
+      if (put) {
+        unsigned e1 = im;
+        if (share != 0) {
+          if (my < S.mask_cap) {
+            for (int w = 0; w < 2 * kScCl; ++w) mbase[(size_t)my * (2 * kScCl) + w] = M[w * kWave];
+            w0 |= 0x80000000u;
+            e1 |= (unsigned)my << 8;
+          } else {
+            atomicOr(S.err, (unsigned)kErrListCap);
+          }
+        }
+        out[2 * pos] = make_uint4(w0, __float_as_uint(offx), __float_as_uint(offy), __float_as_uint(offz));
+        out[2 * pos + 1] = make_uint4(e1, 0u, 0u, 0u);
+      }
+      mslot += __popcll(fl);
       count += __popcll(b);
     }
     __builtin_amdgcn_wave_barrier();
   }
-  // ---- pass B: offsets, exclusion masks, length
-  __threadfence_block();  // pass A's entries (global stores of other lanes) are read back below
-  long long pairs8 = 0;  // 8x8 cluster pairs, for the statistics
   if (count > cap) {
     if (lane == 0) atomicOr(S.err, (unsigned)kErrListCap);
     count = cap;
-  }
-  const double ox = S.cl_origin[cl0], oy = S.cl_origin[S.cl_cap + cl0], oz = S.cl_origin[2 * S.cl_cap + cl0];
-  // the super-cluster's excluded pairs, from its own atoms' side: lane = i
-  // atom (i cluster lane >> 3, i lane lane & 7), one partner per round, each
-  // kept when it is in the same molecule -- every excluded pair of an entry's
-  // tiles is one of these, found in LDS by the partner's cluster
-  int nhit = 0;
-  {
-    for (int b = lane; b < kHitBuckets; b += kWave) hits[wv].head[b] = -1;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const int c = lane >> 3, ii = lane & 7;
-    int bi = -1, mi = -1, k = 0, kend = 0;
-    unsigned long long wf = 0, bits = 0;
-    if (c < nic) {
-      bi = S.orig[(cl0 + c) * kCluster + ii];
-      if (bi >= 0) {
-        mi = S.molecule[bi];
-        wf = S.excl_win[2 * (size_t)bi];
-        if ((wf >> 63) != 0) {
-          k = S.excl_start[bi];
-          kend = S.excl_start[bi + 1];
-        } else {
-          bits = (wf | S.excl_win[2 * (size_t)bi + 1]) & ~(1ull << 63);
-        }
-      }
-    }
-    while (true) {
-      int b = -1, cls = 0;
-      if (bits) {
-        const int d = __ffsll((long long)bits) - 1;
-        bits &= bits - 1;
-        b = bi + d - 31;
-        cls = ((wf >> d) & 1ull) ? PM_EXCL_FULL : PM_EXCL_MODIFIED;
-      } else if (k < kend) {
-        b = S.excl_partner[k];
-        cls = S.excl_class[k];
-        ++k;
-      }
-      if (wave_ballot(b >= 0) == 0) break;
-      bool keep = false;
-      int2 hv = make_int2(0, 0);
-      if (b >= 0 && (cls == PM_EXCL_FULL || cls == PM_EXCL_MODIFIED) && S.molecule[b] == mi) {
-        const int sb = S.slot[b];
-        keep = true;
-        hv = make_int2(sb >> 3, ((sb & (kCluster - 1)) << 8) | (c << 4) | (ii << 1) |
-                                     (cls == PM_EXCL_MODIFIED ? 1 : 0));
-      }
-      const unsigned long long kb = wave_ballot(keep);
-      const int pos = nhit + __popcll(kb & ((1ull << lane) - 1ull));
-      if (keep && pos < kHitCap) {
-        hits[wv].h[pos] = hv;
-        hits[wv].next[pos] = atomicExch(&hits[wv].head[hv.x & (kHitBuckets - 1)], pos);
-      }
-      nhit += __popcll(kb);
-    }
-  }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  int mslot = 0;
-  unsigned long long *mbase = S.list_mask + (size_t)sc * S.mask_cap * (2 * kScCl);
-  // this lane's entry: FULL, MODIFIED word per i cluster, word w at M[w * kWave]
-  unsigned long long *M = &scratch[wv].mask[0][lane];
-  for (int e0 = 0; e0 < count; e0 += kWave) {
-    const int e = e0 + lane;
-    unsigned share = 0;
-    for (int w = 0; w < 2 * kScCl; ++w) M[w * kWave] = 0ull;
-    if (e < count) {
-      const unsigned x = out[2 * e].x;
-      const unsigned im = out[2 * e + 1].x & 0xFFu;
-      const int cj = (int)(x & 0x7FFFFFu);
-      const int sh = (int)((x >> 23) & 31u);
-      const int sx = sh / 9 - 1, sy = (sh / 3) % 3 - 1, sz = sh % 3 - 1;
-      const double ojx = S.cl_origin[cj] + (double)sx * S.L[0];
-      const double ojy = S.cl_origin[S.cl_cap + cj] + (double)sy * S.L[1];
-      const double ojz = S.cl_origin[2 * S.cl_cap + cj] + (double)sz * S.L[2];
-      out[2 * e].y = __float_as_uint((float)(ojx - ox));
-      out[2 * e].z = __float_as_uint((float)(ojy - oy));
-      out[2 * e].w = __float_as_uint((float)(ojz - oz));
-      pairs8 += __popc(im);
-      // the excluded pairs of the entry's tiles, lane bit jj * 8 + ii of the
-      // tile's FULL or MODIFIED word: the super-cluster's pairs whose partner
-      // is in cluster cj, or, past kHitCap of those, each j atom's exclusion
-      // partners (its index window, or the symmetric CSR for an atom with a
-      // partner outside it), located by slot -- the super-cluster's i
-      // clusters are the contiguous cl0 .. cl0 + nic - 1 -- and kept when
-      // the partner is in a tile of the entry and in the same molecule
-      if (nhit <= kHitCap) {
-        for (int q = hits[wv].head[cj & (kHitBuckets - 1)]; q >= 0; q = hits[wv].next[q]) {
-          const int2 hv = hits[wv].h[q];
-          const int c = (hv.y >> 4) & 7;
-          if (hv.x == cj && ((im >> c) & 1u)) {
-            share |= 1u << c;
-            M[(2 * c + (hv.y & 1)) * kWave] |= 1ull << (((hv.y >> 8) & 7) * kCluster + ((hv.y >> 1) & 7));
-          }
-        }
-      } else
-      for (int jj = 0; jj < kCluster; ++jj) {
-        const int sj = cj * kCluster + jj;
-        const int bj = S.orig[sj];
-        if (bj < 0) continue;
-        const int mj = S.smol[sj];
-        const unsigned long long wf = S.excl_win[2 * (size_t)bj], wm = S.excl_win[2 * (size_t)bj + 1];
-        auto partner = [&](int b, int cls) {
-          const int sb = S.slot[b];
-          const int c = (sb >> 3) - cl0;
-          if (c >= 0 && c < nic && ((im >> c) & 1u) && S.smol[sb] == mj &&
-              (cls == PM_EXCL_FULL || cls == PM_EXCL_MODIFIED)) {
-            share |= 1u << c;
-            M[(2 * c + (cls == PM_EXCL_MODIFIED ? 1 : 0)) * kWave] |=
-                1ull << (jj * kCluster + (sb & (kCluster - 1)));
-          }
-        };
-        if ((wf >> 63) != 0) {
-          for (int k = S.excl_start[bj]; k < S.excl_start[bj + 1]; ++k) partner(S.excl_partner[k], S.excl_class[k]);
-        } else {
-          unsigned long long bits = (wf | wm) & ~(1ull << 63);
-          while (bits) {
-            const int d = __ffsll((long long)bits) - 1;
-            bits &= bits - 1;
-            partner(bj + d - 31, ((wf >> d) & 1ull) ? PM_EXCL_FULL : PM_EXCL_MODIFIED);
-          }
-        }
-      }
-    }
-    // mask slots in entry order
-    const unsigned long long fl = wave_ballot(share != 0);
-    const int my = mslot + __popcll(fl & ((1ull << lane) - 1ull));
-    if (share != 0) {
-      if (my < S.mask_cap) {
-        for (int w = 0; w < 2 * kScCl; ++w) mbase[(size_t)my * (2 * kScCl) + w] = M[w * kWave];
-        out[2 * e].x |= 0x80000000u;
-        out[2 * e + 1].x |= (unsigned)my << 8;
-      } else {
-        atomicOr(S.err, (unsigned)kErrListCap);
-      }
-    }
-    mslot += __popcll(fl);
   }
   if (lane == 0) S.list_len[sc] = count;
   pairs8 = wave_sum_ll(pairs8);
