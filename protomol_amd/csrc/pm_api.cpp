@@ -1395,7 +1395,8 @@ void force_eval(pm_ctx *c, const Launch &L, int mode, const StepPolicy &pol, int
   // the cutoff path's list check and bookkeeping ride on the rebuild kernels
   if (c->gb) pm::launch_step_begin(L, mode, pol.lim2, pol.kmax);
   const bool pool = !timed && (c->desc.flags & PM_FLAG_TIMING);
-  if (mode & pm::kStepForce) pm::launch_zero_forces(L);
+  // (the cutoff path's list check clears the pair forces itself)
+  if (c->gb && (mode & pm::kStepForce)) pm::launch_zero_forces(L);
   const bool en = kind == pm::kPairForcesEnergies;
   const bool local_en = en || (c->gb && gb_energies);
   const int local = pm::kLocalForces | (local_en ? pm::kLocalEnergies : 0);
@@ -1725,7 +1726,7 @@ hipGraphExec_t md_graph(pm_ctx *c, LaunchSet &ls, double h, const StepPolicy &po
   try {
     for (int k = 0; k < kGraphSteps; ++k) {
       const bool en = energies && k == kGraphSteps - 1;
-      pm::launch_kick_drift(ls.L, h, lv);
+      pm::launch_kick_drift(ls.L, h, lv, c->gb);
       force_eval(c, ls.L, pm::kStepAuto, pol, step_kind(c, en), en, false);
     }
   } catch (...) {
@@ -1790,7 +1791,7 @@ void md_advance(pm_ctx *c, const std::vector<int> &ids, int n_steps, double dt_f
       }
     for (; i < n_steps; ++i) {
       const bool en = energy_step(i);
-      pm::launch_kick_drift(ls.L, h, lv_params(c, dt_fs));
+      pm::launch_kick_drift(ls.L, h, lv_params(c, dt_fs), c->gb);
       force_eval(c, ls.L, pm::kStepAuto, pol, step_kind(c, en), en, false);
     }
     if (depth == 0) HIPCHECK(hipEventRecord(c->ev1, c->stream));

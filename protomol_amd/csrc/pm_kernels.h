@@ -63,7 +63,9 @@ struct LangevinDev {
   int on = 0;
   double vdt = 1.0, fdt = 0.0, ndt = 0.0, fc = 0.0;  // e^{-g dt/2}, (1 - e^{-g dt/2})/g, noise scale, 2 kB T g
 };
-void launch_kick_drift(const Launch &L, double h, const LangevinDev &lv);
+// zero_forces: the kick clears the pair forces it consumed (the GB path;
+// the cutoff path's list check clears them)
+void launch_kick_drift(const Launch &L, double h, const LangevinDev &lv, bool zero_forces);
 void launch_md_arm(const Launch &L);
 void launch_half_kick(const Launch &L, double hh, const LangevinDev &lv);
 void launch_zero_forces(const Launch &L);

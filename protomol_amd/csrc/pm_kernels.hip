@@ -564,8 +564,9 @@ __global__ __launch_bounds__(kListThreads) void k_pairlist(const SysDev *__restr
 //   doHalfKickdoDrift (LeapfrogIntegrator.cpp:33-56)  v += F*h*0.5/m; x += v*h
 // later steps:
 //   doKickdoDrift     (LeapfrogIntegrator.cpp:58-87)  v += F*h/m;     x += v*h
-// The force buffer is consumed (zeroed) and the FP32 cluster-local
-// coordinates for the next force evaluation are written.
+// The FP32 cluster-local coordinates for the next force evaluation are
+// written; the consumed pair forces are cleared here on the GB path and by
+// the next list check (in slot order) on the cutoff path.
 // Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
 // easy as 1, 2, 3", SC 2011): a counter-based generator, so atom a's noise
 // at kick k of its system depends on (seed, a, k) only
@@ -608,7 +609,7 @@ __device__ __forceinline__ void lv_half(double v[3], const double F[3], double m
 
 // One atom's kick + drift (see k_kick_drift); returns the squared distance
 // of its cluster-local position from where the pair list was built.
-__device__ __forceinline__ float kick_drift_atom(const SysDev &S, int a, double h, const LangevinDev &lv) {
+__device__ __forceinline__ float kick_drift_atom(const SysDev &S, int a, double h, const LangevinDev &lv, bool zero_f) {
   const int n = S.n_atoms;
   // every load first (global address space, so none waits on a store), then
   // the arithmetic, then the stores
@@ -646,9 +647,11 @@ __device__ __forceinline__ float kick_drift_atom(const SysDev &S, int a, double 
     xx[d] = x[d] + v[d] * h;
     l[d] = (float)(xx[d] - xr[d]);
   }
-  fx[sl] = 0;
-  fy[sl] = 0;
-  fz[sl] = 0;
+  if (zero_f) {  // (the cutoff path's list check clears them in slot order instead)
+    fx[sl] = 0;
+    fy[sl] = 0;
+    fz[sl] = 0;
+  }
   for (int d = 0; d < 3; ++d) {
     V[d * n + a] = v[d];
     X[d * n + a] = xx[d];
@@ -661,7 +664,7 @@ __device__ __forceinline__ float kick_drift_atom(const SysDev &S, int a, double 
 }
 
 __global__ __launch_bounds__(256) void k_kick_drift(const SysDev *__restrict__ sys, const int *__restrict__ ap,
-                                                    int nsys, double h, LangevinDev lv, int *rb_any) {
+                                                    int nsys, double h, LangevinDev lv, int *rb_any, int zero_f) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g == 0 && rb_any) *rb_any = 0;  // the next list check raises it again
   const int total = ap[nsys];
@@ -671,10 +674,10 @@ __global__ __launch_bounds__(256) void k_kick_drift(const SysDev *__restrict__ s
   if (s >= 0) {
     // the wave's atoms are one system's: its record is read with scalar loads
     const SysDev *__restrict__ S = sys + s;
-    if (ok) d2 = kick_drift_atom(*S, g - ap[s], h, lv);
+    if (ok) d2 = kick_drift_atom(*S, g - ap[s], h, lv, zero_f != 0);
   } else {
     s = ok ? find_sys(ap, nsys, g) : 0;
-    if (ok) d2 = kick_drift_atom(sys[s], g - ap[s], h, lv);
+    if (ok) d2 = kick_drift_atom(sys[s], g - ap[s], h, lv, zero_f != 0);
   }
   // max per system: a block whose atoms all belong to one system reduces
   // through LDS to one atomic; a block straddling systems uses one per lane
@@ -848,10 +851,10 @@ bool launch_pair(const Launch &L, const ForceDev &P, int kind) {
   return true;
 }
 
-void launch_kick_drift(const Launch &L, double h, const LangevinDev &lv) {
+void launch_kick_drift(const Launch &L, double h, const LangevinDev &lv, bool zero_forces) {
   if (L.n_atoms_total)
     hipLaunchKernelGGL(k_kick_drift, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix,
-                       L.nsys, h, lv, L.rb_any);
+                       L.nsys, h, lv, L.rb_any, zero_forces ? 1 : 0);
 }
 void launch_half_kick(const Launch &L, double hh, const LangevinDev &lv) {
   if (L.n_atoms_total) hipLaunchKernelGGL(k_half_kick, dim3(nblk(L.n_atoms_total, 256)), dim3(256), 0, L.stream, L.sys, L.atom_prefix, L.nsys, hh, lv);

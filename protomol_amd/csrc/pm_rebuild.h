@@ -34,7 +34,8 @@ __device__ __forceinline__ void step_finish_ctl(const SysDev &S) {
 }
 
 // Rebuild, step 1 of 3, one thread per atom.  It also starts the force pass:
-// every system's thread of atom 0 publishes the list check's decision (and
+// it clears the pair forces; every system's thread of atom 0 publishes the
+// list check's decision (and
 // raises the launch's rb_any flag, cleared by the step's kick, when the
 // system rebuilds) and clears the census, and the first threads clear the
 // energy accumulators.
@@ -50,6 +51,13 @@ __device__ __forceinline__ bool bin_atom(const SysDev &S, int a, int mode, float
   const int n = S.n_atoms;
   const bool rb = rebuild_due(S, mode, lim2, kmax);
   for (int k = a; k < kAccN * kAccCopies; k += n) S.acc[k] = 0;
+  // the pass's pair forces start from zero (every slot, before any rebuild
+  // reassigns them; slot order, so a wave's stores are contiguous)
+  for (int k = a; k < S.slot_cap; k += n) {
+    S.fx[k] = 0;
+    S.fy[k] = 0;
+    S.fz[k] = 0;
+  }
   if (a == 0) {
     S.ctl[kCtlRebuild] = rb ? 1 : 0;
     S.census[0] = 0;
