@@ -190,6 +190,69 @@ __global__ __launch_bounds__(64 * kLocWaves) void k_local(const SysDev *__restri
   local_block<EN>(sys, ap, nsys, P, mode, part);
 }
 
+// The local terms with their energies and virial: one wave per 64 atoms
+// walks the shares in order.  Each share's force is added to the atom's sum
+// in the order k_local's LDS sum uses, and each share's energies go to fixed
+// point on their own, so forces and totals are the same bits as k_local's.
+// (With a wave per share, the energy form took 25 us at the 96k box; here
+// fewer, longer waves.)
+__global__ __launch_bounds__(64) void k_local_seq(const SysDev *__restrict__ sys, const int *__restrict__ ap,
+                                                  int nsys, ForceDev P) {
+  const int lane = threadIdx.x & 63;
+  const int g = blockIdx.x * 64 + lane;
+  const int total = ap[nsys];
+  const bool ok = g < total;
+  int s = wave_sys(ap, nsys, g, total);
+  const bool uniform = s >= 0;
+  if (!uniform) s = ok ? find_sys(ap, nsys, g) : 0;
+  double t[3] = {0.0, 0.0, 0.0};
+  long long q[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int sh = 0; sh < kLocWaves; ++sh) {
+    int kind, slot, nslots;
+    loc_share(sh, kind, slot, nslots);
+    double f[3] = {0.0, 0.0, 0.0};
+    double eb = 0.0, ea = 0.0, ec = 0.0, v[6] = {0, 0, 0, 0, 0, 0};
+    if (ok) {
+      if (uniform) {
+        const SysDev *__restrict__ S = sys + s;
+        local_terms<true>(*S, g - ap[s], P, sh, kind, f, eb, ea, ec, v);
+      } else {
+        local_terms<true>(sys[s], g - ap[s], P, sh, kind, f, eb, ea, ec, v);
+      }
+    }
+    t[0] += f[0];
+    t[1] += f[1];
+    t[2] += f[2];
+    // bonded and correction terms add to the atomic virial only when it is on
+    // (BondSystemForce.h:121, AngleSystemForce.h:155, NonbondedFullEwaldSystemForce.h:1125)
+    if (!P.virial)
+      for (int k = 0; k < 6; ++k) v[k] = 0.0;
+    q[0] += to_fixed(eb, kEScale);
+    q[1] += to_fixed(ea, kEScale);
+    q[2] += to_fixed(ec, kEScale);
+    for (int k = 0; k < 6; ++k) q[3 + k] += to_fixed(v[k], kEScale);
+  }
+  if (ok) {
+    const SysDev &S = sys[s];
+    for (int c = 0; c < 3; ++c) gp(S.fl)[(size_t)c * S.n_atoms + (g - ap[s])] = to_fixed(t[c], kFScale);
+  }
+  const int slots[9] = {kAccBond, kAccAngle, kAccCoul, kAccVxx, kAccVxy, kAccVxz, kAccVyy, kAccVyz, kAccVzz};
+  const int s0 = __shfl(s, 0);
+  const bool mixed = wave_ballot(ok && s != s0) != 0;
+  if (mixed) {
+    if (ok)
+      for (int k = 0; k < 9; ++k)
+        if (q[k]) atomic_add_ll(&acc_copy(sys[s].acc, blockIdx.x)[slots[k]], q[k]);
+    return;
+  }
+  for (int k = 0; k < 9; ++k) q[k] = wave_sum_ll(q[k]);
+  if (lane == 0) {
+    long long *acc = acc_copy(sys[s0].acc, blockIdx.x);
+    for (int k = 0; k < 9; ++k)
+      if (q[k]) atomic_add_ll(&acc[slots[k]], q[k]);
+  }
+}
+
 // k_bin and k_local (forces) in one launch, for a forces-only pass: the list
 // check and binning of a block's 64 atoms ride on its last wave, before that
 // wave's share of the local terms (both read the positions of the pass, and
@@ -212,7 +275,7 @@ void launch_local(const Launch &L, const ForceDev &P, int mode) {
   if (!L.n_atoms_total || !L.n_local_total) return;
   const dim3 g(nblk(L.n_atoms_total, 64)), b(64 * kLocWaves);
   if (mode & kLocalEnergies)
-    hipLaunchKernelGGL(k_local<true>, g, b, 0, L.stream, L.sys, L.atom_prefix, L.nsys, P, mode);
+    hipLaunchKernelGGL(k_local_seq, g, dim3(64), 0, L.stream, L.sys, L.atom_prefix, L.nsys, P);
   else
     hipLaunchKernelGGL(k_local<false>, g, b, 0, L.stream, L.sys, L.atom_prefix, L.nsys, P, mode);
 }
