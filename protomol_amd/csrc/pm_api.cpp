@@ -831,7 +831,7 @@ size_t carve(const pm_ctx *c, const SysPrep &p, const pm_system_desc &d, Carver 
   S.acc = cv.take<long long>((size_t)pm::kAccN * pm::kAccCopies);
   S.census = cv.take<unsigned long long>(2);
   S.err = cv.take<unsigned>(1);
-  S.maxdisp2 = cv.take<unsigned>(1);
+  S.maxdisp2 = cv.take<unsigned>(pm::kDispSlots);
   S.maxdisp_run = cv.take<unsigned>(1);
   S.ctl = cv.take<int>(pm::kCtlN);
   *ctl_snap = cv.take<int>(pm::kCtlN);

@@ -49,6 +49,7 @@ enum { kAccLJ = 0, kAccCoul = 1, kAccVxx, kAccVxy, kAccVxz, kAccVyy, kAccVyz, kA
 // copies.  Integer sums are exact, so the totals do not depend on the split,
 // and thousands of waves no longer queue for the atomics of one cache line.
 constexpr int kAccCopies = 64;
+constexpr int kDispSlots = 32;   // per-system slots of the step's largest displacement
 __host__ __device__ inline long long *acc_copy(long long *acc, int w) {
   return acc + (size_t)(w & (kAccCopies - 1)) * kAccN;
 }
@@ -223,7 +224,9 @@ struct SysDev {
   long long *acc;          // kAccCopies x kAccN fixed-point energy/virial accumulators
   unsigned long long *census; // count, hash
   unsigned *err;
-  unsigned *maxdisp2;      // float bits of the largest squared displacement from the last rebuild (this step)
+  unsigned *maxdisp2;      // kDispSlots: float bits of the largest squared displacement from the last
+                           // rebuild (this step), spread over slots so the kick's atomics do not queue on one
+                           // address; the value is the maximum over them
   unsigned *maxdisp_run;   // the same, folded over every step of the current run
   int *ctl;                // kCtlN control words
   unsigned lv_seed;        // Langevin noise stream key

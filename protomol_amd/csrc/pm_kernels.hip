@@ -43,10 +43,10 @@ __device__ __forceinline__ void step_begin_ctl(const SysDev &S, int mode, float 
   int *ctl = S.ctl;
   ctl[kCtlFirst] = 0;  // the step's kick has run
   ctl[kCtlNoise] += 1;
-  const unsigned d2 = *S.maxdisp2;
+  const unsigned d2 = max_disp2(S);
   const bool rb = rebuild_due(S, mode, lim2, kmax);
   *S.maxdisp_run = max(*S.maxdisp_run, d2);
-  *S.maxdisp2 = 0u;
+  clear_disp2(S);
   ctl[kCtlRebuild] = rb ? 1 : 0;
   if (rb) {
     ctl[kCtlSince] = 0;
@@ -695,10 +695,10 @@ __global__ __launch_bounds__(256) void k_kick_drift(const SysDev *__restrict__ s
     if (threadIdx.x == 0) {
       float m = red[0];
       for (int k = 1; k < nw; ++k) m = fmaxf(m, red[k]);
-      atomicMax(sys[s].maxdisp2, __float_as_uint(m));
+      atomicMax(sys[s].maxdisp2 + (blockIdx.x & (kDispSlots - 1)), __float_as_uint(m));
     }
   } else if (ok) {
-    atomicMax(sys[s].maxdisp2, __float_as_uint(d2));
+    atomicMax(sys[s].maxdisp2 + (blockIdx.x & (kDispSlots - 1)), __float_as_uint(d2));
   }
 }
 

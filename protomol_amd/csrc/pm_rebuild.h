@@ -9,10 +9,19 @@ namespace pm {
 // The list check of a force pass (the Verlet criterion: rebuild when an atom
 // moved more than lim from where the list was built, or every kmax steps, or
 // always with kStepForce), from values no kernel of the rebuild changes
+// the step's largest squared displacement (float bits), over the slots
+__device__ __forceinline__ unsigned max_disp2(const SysDev &S) {
+  unsigned d = 0u;
+  for (int k = 0; k < kDispSlots; ++k) d = max(d, S.maxdisp2[k]);
+  return d;
+}
+__device__ __forceinline__ void clear_disp2(const SysDev &S) {
+  for (int k = 0; k < kDispSlots; ++k) S.maxdisp2[k] = 0u;
+}
 __device__ __forceinline__ bool rebuild_due(const SysDev &S, int mode, float lim2, int kmax) {
   if (mode & kStepForce) return true;
   if (!(mode & kStepAuto)) return false;
-  return __uint_as_float(*S.maxdisp2) > lim2 || (kmax > 0 && S.ctl[kCtlSince] + 1 >= kmax);
+  return __uint_as_float(max_disp2(S)) > lim2 || (kmax > 0 && S.ctl[kCtlSince] + 1 >= kmax);
 }
 
 // the rest of a step's bookkeeping once k_bin has published the decision in
@@ -22,9 +31,9 @@ __device__ __forceinline__ void step_finish_ctl(const SysDev &S) {
   int *ctl = S.ctl;
   ctl[kCtlFirst] = 0;  // the step's kick has run
   ctl[kCtlNoise] += 1;
-  const unsigned d2 = *S.maxdisp2;
+  const unsigned d2 = max_disp2(S);
   *S.maxdisp_run = max(*S.maxdisp_run, d2);
-  *S.maxdisp2 = 0u;
+  clear_disp2(S);
   if (ctl[kCtlRebuild]) {
     ctl[kCtlSince] = 0;
     ctl[kCtlCount] += 1;
