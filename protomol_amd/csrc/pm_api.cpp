@@ -1322,6 +1322,9 @@ LaunchSet &get_set(pm_ctx *c, const std::vector<int> &ids) {
   L.gbp_prefix = d_pre + 9 * st;
   L.sc_prefix = d_pre + 8 * st;
   L.rb_any = reinterpret_cast<int *>(base + o_flag);
+  // test knob: a lower limit on the list build's LDS excluded pairs, so
+  // tests can drive the per-entry partner walk
+  if (const char *hc = std::getenv("PM_DEBUG_HIT_CAP")) L.hit_cap = std::max(0, std::atoi(hc));
   L.n_atoms_total = pre[0].back();
   L.n_cl_total = pre[1].back();
   L.n_cols_total = pre[2].back();

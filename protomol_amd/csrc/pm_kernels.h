@@ -23,6 +23,10 @@ struct Launch {
   // pass (raised by the list check, cleared by the kick); null: the rebuild
   // kernels check each system's own decision only
   int *rb_any = nullptr;
+  // the list build takes a super-cluster's excluded pairs from LDS up to this
+  // many (kHitCap); past it, each entry walks its j atoms' partners.  A test
+  // knob (PM_DEBUG_HIT_CAP) lowers it to drive that path
+  int hit_cap = 256;
 };
 
 // Host-side launch shape of one system's reciprocal-space passes

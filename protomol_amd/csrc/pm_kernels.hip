@@ -265,7 +265,8 @@ __device__ __forceinline__ int lower_bound_lo_z(const float4 *lo_b, int c0, int 
 }
 
 __global__ __launch_bounds__(kListThreads) void k_pairlist(const SysDev *__restrict__ sys, const int *__restrict__ scp,
-                                                           int nsys, float rlist2, const int *__restrict__ rb_any) {
+                                                           int nsys, float rlist2, const int *__restrict__ rb_any,
+                                                           int hit_cap) {
   if (rb_any && !*rb_any) return;
   __shared__ float4 ib_lo[kListWaves][kScCl], ib_hi[kListWaves][kScCl];
   __shared__ int seg_start[kListWaves][kSegCap + 1], seg_cl[kListWaves][kSegCap], seg_sh[kListWaves][kSegCap];
@@ -489,7 +490,7 @@ __global__ __launch_bounds__(kListThreads) void k_pairlist(const SysDev *__restr
         // partner outside it), located by slot -- the super-cluster's i
         // clusters are the contiguous cl0 .. cl0 + nic - 1 -- and kept when
         // the partner is in a tile of the entry and in the same molecule
-        if (nhit <= kHitCap) {
+        if (nhit <= min(hit_cap, kHitCap)) {
           for (int q = hits[wv].head[cj & (kHitBuckets - 1)]; q >= 0; q = hits[wv].next[q]) {
             const int2 hv = hits[wv].h[q];
             const int c = (hv.y >> 4) & 7;
@@ -835,7 +836,7 @@ bool launch_rebuild(const Launch &L, float rlist2, int mode, float lim2, int kma
                        mode, lim2, kmax, L.rb_any);
   hipLaunchKernelGGL(k_col_sort, dim3(L.n_cols_total), dim3(1024), 0, L.stream, L.sys, L.col_prefix, L.nsys, L.rb_any);
   hipLaunchKernelGGL(k_pairlist, dim3(nblk(L.n_sc_total, kListWaves)), dim3(kListThreads), 0, L.stream, L.sys,
-                     L.sc_prefix, L.nsys, rlist2, L.rb_any);
+                     L.sc_prefix, L.nsys, rlist2, L.rb_any, L.hit_cap);
   return local && L.n_local_total;
 }
 

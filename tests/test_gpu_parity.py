@@ -1020,6 +1020,37 @@ def test_md_list_regrow_rerun_is_bitwise_unconstrained():
     assert np.array_equal(e1, e0)
 
 
+@pytest.mark.parametrize("system", ["water", "chains"])
+def test_list_exclusion_masks_both_ways_alike(system):
+    """The list build's exclusion masks come from the super-cluster's own
+    excluded pairs gathered in LDS, or, past their limit, from each entry's
+    walk of its j atoms' partners: PM_DEBUG_HIT_CAP=0 drives every
+    super-cluster down the walk, and forces, energies and the pair census
+    are the same bits either way (water: FULL pairs; 4-atom chains: FULL and
+    MODIFIED 1-4 pairs)."""
+    import os
+    print(f"case list masks both ways {system}", flush=True)
+    if system == "water":
+        w = _water_set((700,), 310)[0]
+    else:
+        w = S.chain_box(512, 52.0, seed=3, n_types=40)
+    ta, sa, tb, sb = MODELS["c2c2"]
+    desc = force_desc(ta, sa, tb, sb, flags=L.FLAG_VIRIAL, ewald=True)
+    out = []
+    for cap in (None, "0"):
+        try:
+            if cap is not None:
+                os.environ["PM_DEBUG_HIT_CAP"] = cap
+            with L.Context(desc, [w]) as ctx:
+                f, e = evaluate(ctx, 0, w.x)
+                out.append((f, e, ctx.census(0, w.x)))
+        finally:
+            os.environ.pop("PM_DEBUG_HIT_CAP", None)
+    (f0, e0, c0), (f1, e1, c1) = out
+    assert c0 == c1
+    assert np.array_equal(f0, f1) and np.array_equal(e0, e1)
+
+
 def test_lj_table_40_types_vs_reference_golden():
     """More LJ types than the pair kernel's LDS copy holds (40 > 16): the
     rows come from the global table; evaluation and the MD path's forces
